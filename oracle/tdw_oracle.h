@@ -1,0 +1,196 @@
+/*
+ * tdw_oracle.h -- CPU FP64 restatement of the W-SLDA time-dependent step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (wslda_amd/, include/)
+ * may include, link or call this code; only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it, as the checker and as the timed
+ * CPU baseline.
+ *
+ * Every routine names the reference file:line it restates.  Reference paths
+ * are relative to the W-SLDA Toolkit tree (hpc-engine/...).
+ *
+ * Pinning (what checks each routine, see tests/test_oracle_*.py):
+ *   - td-gauss-slda goldens (testsuite/td-gauss-slda/ref/test.ss0-1d.wlog.ref,
+ *     all 101 rows at 10 digits; ss0-3d first rows): pins the uniform solver,
+ *     plane-wave init, ABM, normalisation, Laplacian, densities, tau, SLDA
+ *     potentials, H application, qpe subtraction and the energy sums together.
+ *   - closed forms: plane-wave eigenstates (spectral derivatives of e^{ikx}),
+ *     stationarity of the uniform solution, ABM order on y'=-i*lam*y,
+ *     Parseval for the FFT, brute-force DFT on tiny lattices.
+ *   - parity unpinned: none of the BdG-functional paths are covered by a
+ *     golden in this build (the td-gauss-bdg golden needs the BdG uniform
+ *     solver, which is not restated); BdG potentials are pinned only by the
+ *     stationarity check of a self-consistent plane-wave state.
+ */
+#ifndef TDW_ORACLE_H
+#define TDW_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Functional ids (pca_settings.h:63-67). */
+#define TDWO_SLDA 111
+#define TDWO_BDG  113
+/* Regularisation ids (pca_settings.h:223-224). */
+#define TDWO_SPHERICAL 88
+#define TDWO_CUBIC     89
+/* ABM schemes (pca_settings.h:201-203). */
+#define TDWO_AB3AM4 34
+#define TDWO_AB4AM5 45
+#define TDWO_AB5AM5 55
+
+/* Energy / observable slots (pca_settings.h:146-161). */
+#define TDWO_EKIN 0
+#define TDWO_EPOT 1
+#define TDWO_EPAIR 2
+#define TDWO_ECURRENT 3
+#define TDWO_EPOTEXT 4
+#define TDWO_EPAIREXT 5
+#define TDWO_EVELEXT 6
+#define TDWO_NPARTA 7
+#define TDWO_NPARTB 8
+#define TDWO_LZA 9
+#define TDWO_LZB 10
+#define TDWO_NITEMS 11
+
+/* External potential family: sum of Gaussians, harmonic trap, smooth on/off
+ * switch.  The td-gauss testsuite problem (testsuite/td-gauss-slda/
+ * problem-definition.h v_ext + process_params) is the member with one
+ * Gaussian centred at the origin. */
+#define TDWO_MAX_GAUSS 4
+typedef struct {
+    int ngauss;
+    double amp[TDWO_MAX_GAUSS];       /* amplitude (code units)            */
+    double cx[TDWO_MAX_GAUSS], cy[TDWO_MAX_GAUSS], cz[TDWO_MAX_GAUSS];
+    double wx[TDWO_MAX_GAUSS], wy[TDWO_MAX_GAUSS], wz[TDWO_MAX_GAUSS];
+                                      /* exponent coefficients: exp(w*(x-c)^2), w<=0 */
+    double omega;                     /* harmonic trap 0.5*omega^2*r^2      */
+    int switch_on;                    /* 1: multiply Gaussians by smooth_step */
+    double t_start, t_stop, t_switch; /* smooth_step window (code time)    */
+    int flip_b;                       /* 1: V_b = -V_a for the Gaussian part */
+} tdwo_vext;
+
+typedef struct {
+    int dim;              /* 1, 2 or 3 (CODEDIM)                          */
+    int nx, ny, nz;       /* NX, NY, NZ; for dim<3 the extra ones are transverse */
+    double dx, dy, dz;
+    int functional;       /* TDWO_SLDA / TDWO_BDG                          */
+    int regularization;   /* TDWO_SPHERICAL / TDWO_CUBIC                   */
+    int mf_correction;    /* INCLUDE_MF_CORRECTION_FROM_REG_SCHEME         */
+    int scheme;           /* TDWO_AB3AM4 / AB4AM5 / AB5AM5                 */
+    double mu_a, mu_b, ec, dt, t0;
+    double sclgth;        /* BdG scattering length (memcopy_const_sclgth)  */
+    double p_nmin, p_nmax, denseps;
+    tdwo_vext vext;
+} tdwo_cfg;
+
+/* Number of evolved lattice points (NUMBER_ELEMENT). */
+size_t tdwo_npts(const tdwo_cfg *c);
+
+/* --- spectral derivatives (own mixed-radix FFT) ------------------------- */
+/* forward(sign=-1)/inverse(sign=+1) unnormalised multi-dimensional FFT of one
+ * array of tdwo_npts() complex values, in place. */
+void tdwo_fft(const tdwo_cfg *c, double *z, int sign);
+/* compute_laplace: lap[k] for m arrays (pca_derivative.cu:282-380). */
+void tdwo_laplace(const tdwo_cfg *c, int m, const double *in, double *out);
+/* compute_derivatives: d/dx,d/dy,d/dz (Nyquist zeroed) and laplacian
+ * (pca_derivative.cu:139-277).  dy/dz may be NULL for dim<2 / dim<3. */
+void tdwo_derivatives(const tdwo_cfg *c, int m, const double *in,
+                      double *ddx, double *ddy, double *ddz, double *lap);
+/* compute_laplace_real_f (pca_derivative.cu:601-668). */
+void tdwo_laplace_real(const tdwo_cfg *c, const double *f, double *lap);
+
+/* --- ODE integrator (tdwslda_ode_integrator.h) ---------------------------- */
+void tdwo_abm_step1(const tdwo_cfg *c, int n, double *y, double *f1, double *f2,
+                    double *f3, double *f4, double *f5);
+void tdwo_abm_step4(const tdwo_cfg *c, int n, const double *hy, double *y,
+                    const double *flast);
+
+/* --- normalisation (pca_kernels.cu:913-976) ------------------------------- */
+void tdwo_norms(const tdwo_cfg *c, int n, const double *wf, double *norm);
+void tdwo_normalize(const tdwo_cfg *c, int n, double *wf);
+
+/* --- densities (pca_dens.cu:13-267 and 1D/2D twins) ------------------------
+ * dens: 12*N doubles, layout of wslda_potdens.c:14-36.
+ * kt: 2*n transverse momenta [ky(n) | kz(n)] for dim 1, [kz(n)] for dim 2,
+ *     NULL for dim 3.  cnt: degeneracy (dim 1), NULL otherwise.
+ * gradients: 1 => dx/dy/dz/lap used (kernel_calculate_densities),
+ *            0 => rho, nu only (kernel_calculate_densities_limited). */
+void tdwo_densities(const tdwo_cfg *c, int n, const double *wf,
+                    const double *ddx, const double *ddy, const double *ddz,
+                    const double *lap, const double *fbeta, const double *kt,
+                    const int *cnt, double *dens, int gradients);
+/* density_caculate_tau (pca_dens.cu:275-338). */
+void tdwo_density_tau(const tdwo_cfg *c, double *dens);
+
+/* --- functional (tdwslda_functionals.h) ------------------------------------
+ * pots: 12*N doubles (wslda_potdens.c:41-70), input = previous potentials. */
+void tdwo_vext_eval(const tdwo_cfg *c, int it, double *va, double *vb);
+void tdwo_potentials(const tdwo_cfg *c, int it, const double *dens, double *pots);
+
+/* --- Hamiltonian (pca_kernels.cu:233-495, 552-594, 674-848) ---------------
+ * out = H*in - qpe*in, qpe[n] returned (may be NULL).  lap = laplacian of in.
+ * useqpe: if non-NULL use these energies instead of computing them. */
+void tdwo_apply_h(const tdwo_cfg *c, int n, const double *in, double *out,
+                  const double *lap, const double *pots, const double *kt,
+                  const double *useqpe, double *qpe);
+
+/* --- energy (tdwslda_energy_generic.h:21-110) ----------------------------- */
+void tdwo_energy(const tdwo_cfg *c, int it, const double *dens, const double *pots,
+                 double *e);
+
+/* --- full PECE step driver (pca.c:1401-1570) --------------------------------
+ * st holds all state-sized arrays; f[0..4] are fkm1..fkm5 (rotated in place
+ * by pointer swaps, as pca.c:1543-1562 does).  grad_last: compute full
+ * derivatives on this step's corrector (last step of a measurement). */
+typedef struct {
+    int n;
+    double *wf, *lap, *ddx, *ddy, *ddz;
+    double *f[5];
+    double *fbeta, *kt; int *cnt;
+    double *dens, *pots;
+    double qpe_last[1];   /* unused placeholder for ABI stability */
+    double *qpe;          /* n doubles: qpe of the last H application */
+} tdwo_state;
+int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last);
+/* Initial measurement of pca.c:939-975: normalise, derivatives, densities,
+ * tau; potentials are left untouched. */
+void tdwo_initial_measure(const tdwo_cfg *c, tdwo_state *s);
+
+/* --- uniform solution (pca_uniform.h) ----------------------------------- */
+typedef struct {
+    double n0_a, n0_b, mu_a, mu_b, V_a, V_b, alph_a, alph_b, kc, ec;
+    double nu, delta, tau_a, tau_b, beta, S, ekin, epot, epair;
+    int nwf, iters;
+} tdwo_uniform;
+/* solve_uniform_problem (pca_uniform.h:86-502) for SLDA.
+ * mu_rule 0: chemical-potential update of pca_uniform.h:427-428 (relative
+ * density error times eF); mu_rule 1: the update kept commented out at
+ * pca_uniform.h:425-426 (absolute density error times Tstart/T), which is the
+ * rule of the code version that wrote the td-gauss-* goldens (2021.11.06):
+ * with it the golden rows are reproduced to all printed digits. */
+int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc,
+                       double Tstart, double Tstop, double dT, double eps,
+                       double scmix, double muchange, int maxiter, int mu_rule,
+                       tdwo_uniform *u);
+/* Count states (create_uniform_wf* in counting mode). */
+int tdwo_uniform_count(const tdwo_cfg *c, const tdwo_uniform *u);
+/* create_uniform_wf (pca_uniform.h:636-850), create_uniform_wf_2d_generic
+ * (:2044-2287), create_uniform_wf_1d_generic (:2321-2560).
+ * wf: 2*n*N complex; En: n energies; kt/cnt as in tdwo_densities. */
+int tdwo_uniform_wf(const tdwo_cfg *c, const tdwo_uniform *u, int idxfrom, int idxto,
+                    double *wf, double *En, double *kt, int *cnt);
+/* fbeta (pca_uniform.h:70-78). */
+double tdwo_fbeta(double E, double beta);
+
+/* OpenMP thread count control for the CPU baseline (0 = library default). */
+void tdwo_set_threads(int nthreads);
+int tdwo_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,72 @@
+"""Set-up of the td-gauss-slda testsuite problem (testsuite/td-gauss-slda):
+12^3 lattice, DX=1.15, SLDA, spherical cutoff, AB4AM5, UNIFORM_TEST_MODE
+(Na = ceil(L^3/(6 pi^2)), Nb = Na+1), uniform start at T=0.05 eF, a Gaussian
+V_ext switched on between t*eF = 5 and 55 (problem-definition.h v_ext and
+process_params), params7=1 flips its sign for spin b."""
+import json
+import math
+import os
+
+import numpy as np
+
+from oracle import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "td_gauss_slda.json")
+
+
+def golden():
+    return json.load(open(GOLDEN))
+
+
+def setup(dim, scheme=po.AB4AM5):
+    g = golden()["inputs"]
+    NX, DXv = g["NX"], g["DX"]
+    LXYZ = (NX * DXv) ** 3
+    Na = math.ceil(1.0 / (6.0 * math.pi ** 2) * LXYZ)
+    Nb = Na + 1
+    cfg = po.make_cfg(dim, NX, NX, NX, dx=DXv, scheme=scheme)
+    kc = 0.999999 * math.pi / DXv
+    u, rc = po.uniform_solve(cfg, Na / LXYZ, Nb / LXYZ, kc, g["temperature"], eps=g["init0eps"],
+                             scmix=g["init0scmix"], muchange=g["init0muchange"],
+                             maxiter=g["init0maxiter"], mu_rule=1)
+    assert rc == 0
+    n = u.n0_a + u.n0_b
+    eF = (3 * math.pi ** 2 * n) ** (2 / 3) / 2
+    kF = (3 * math.pi ** 2 * n) ** (1 / 3)
+    Effg = 0.6 * n * eF * LXYZ
+    wf, En, kt, cnt = po.uniform_states(cfg, u)
+    fb = np.array([po.fbeta(E, u.beta) for E in En])
+    p = golden()["runs"]["ss0-%dd" % dim]["params"]
+    A = p[0] * eF
+
+    def wcoef(s):
+        s = s / kF
+        return -0.5 / s ** 2 if s > 0.1 else 0.0
+
+    cfg.vext = po.gauss_vext([(-A, (0, 0, 0), (wcoef(p[1]), wcoef(p[2]), wcoef(p[3])))],
+                             switch=(p[4] / eF, p[5] / eF, p[6] / eF), flip_b=int(p[7] > 0.5))
+    cfg.mu_a, cfg.mu_b, cfg.ec = u.mu_a, u.mu_b, u.ec
+    cfg.dt = g["dt_eF"] / eF
+    N = po.npts(cfg)
+    pots = np.zeros(12 * N)
+    pots[:N] = u.V_a
+    pots[N:2 * N] = u.V_b
+    pots[2 * N:4 * N:2] = u.delta
+    pots[4 * N:6 * N] = 1.0
+    return dict(cfg=cfg, u=u, wf=wf, fbeta=fb, kt=kt, cnt=cnt, pots=pots, eF=eF, kF=kF,
+                Effg=Effg, timesteps=g["timesteps"])
+
+
+def wlog_row(e, Effg):
+    """Columns 3..13 of the td-gauss logger (Na, Nb, N, Etot, Ekin, ...)."""
+    et = e[:7].sum()
+    return [e[7], e[8], e[7] + e[8], et / Effg] + [e[k] / Effg for k in range(7)]
+
+
+def digits_match(mine, ref, sig=10):
+    """True if mine equals ref when both are printed with %.{sig}g (1 ulp slack
+    in the last printed digit)."""
+    if ref == 0.0:
+        return abs(mine) < 1e-20
+    ulp = 10 ** (math.floor(math.log10(abs(ref))) - (sig - 1))
+    return abs(mine - ref) <= 1.01 * ulp
