@@ -1,0 +1,236 @@
+/*
+ * wslda_td.h -- C ABI of the MI355X td-W-SLDA step library (libwslda_td.so).
+ *
+ * Two layers, both plain C: raw device pointers, sizes, int return codes
+ * (0 = success, otherwise a hipError_t value or one of the TDW_E* codes), no
+ * C++ or torch types.
+ *
+ *  1. The generic layer, tdw_*: a context (lattice, functional, integration
+ *     scheme, constants) created at run time, so one library serves 1D, 2D
+ *     and 3D problems and any lattice size.
+ *
+ *  2. The drop-in layer: the extern "C" entry points of the reference's
+ *     kernel interface with the same names, argument order and meaning
+ *     (hpc-engine/pca_kernels.h:44-95, pca_derivative.h:12-22,
+ *     pca_dens.h:5-19, wslda_cuda_utils.h:47-157).  They act on the context
+ *     bound with tdw_bind(); a project build binds it from its predefines.h
+ *     (see INTEGRATION.md).  The 1D and 2D codes of the reference declare
+ *     apply_hamiltonian / calculate_densities with extra arguments under the
+ *     same names (ccpca_kernels.h:74-77, cpca_kernels.h:74-77,
+ *     ccpca_dens.h:5-11); those are exported here as *_1d / *_2d and mapped
+ *     to the reference names when TDW_CODEDIM is 1 or 2.
+ *
+ * State layout (identical to the reference, pca.c:307): n quasiparticle
+ * states are stored as [u_0 .. u_{n-1}, v_0 .. v_{n-1}], each a contiguous
+ * array of N complex values (N = NX, NX*NY or NX*NY*NZ).
+ * Density block: 12*N doubles [nu(2N) rho_a tau_a jax jay jaz rho_b tau_b
+ * jbx jby jbz] (wslda_potdens.c:14-36).  Potential block: 12*N doubles
+ * [V_a V_b delta(2N) alpha_a alpha_b A_a(3) A_b(3)] (wslda_potdens.c:41-70).
+ */
+#ifndef WSLDA_TD_H
+#define WSLDA_TD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI of cufftDoubleComplex / C99 double complex (pca_kernels.h:6-8). */
+typedef struct { double re, im; } tdw_complex;
+typedef struct { float re, im; } tdw_complex32;
+
+/* ids shared with the reference's predefines (pca_settings.h:63-67, 201-203, 223-224) */
+#define TDW_SLDA 111
+#define TDW_BDG 113
+#define TDW_SPHERICAL_CUTOFF 88
+#define TDW_CUBIC_CUTOFF 89
+#define TDW_AB3AM4 34
+#define TDW_AB4AM5 45
+#define TDW_AB5AM5 55
+/* precision of the operator application */
+#define TDW_FP64 0
+#define TDW_SPLIT 1   /* FP32 FFT and operator, FP64 state, history update and sums */
+
+/* observable slots of compute_energy (pca_settings.h:146-161) */
+#define TDW_EKIN 0
+#define TDW_EPOT 1
+#define TDW_EPAIR 2
+#define TDW_ECURRENT 3
+#define TDW_EPOTEXT 4
+#define TDW_EPAIREXT 5
+#define TDW_EVELEXT 6
+#define TDW_NPARTA 7
+#define TDW_NPARTB 8
+#define TDW_LZA 9
+#define TDW_LZB 10
+#define TDW_NITEMS 11
+
+/* error codes (besides hipError_t values) */
+#define TDW_EINVAL (-1001)  /* bad argument / unsupported combination   */
+#define TDW_ENOCTX (-1002)  /* drop-in call before tdw_bind()           */
+#define TDW_ESIZE (-1003)   /* lattice axis not a product of 2,3,4,8    */
+#define TDW_ENOMEM (-1004)
+
+/* External potential of the built-in problem: a sum of Gaussians
+ * amp*s(t)*exp(wx*(x-cx)^2 + wy*(y-cy)^2 + wz*(z-cz)^2), a harmonic trap
+ * 0.5*omega^2*r^2 and the smooth on/off switch s(t) of the testsuite's
+ * problem-definition.h; x = DX*(ix-NX/2) etc.  A project build replaces it
+ * with the user's v_ext() callback. */
+#define TDW_MAX_GAUSS 4
+typedef struct {
+    int ngauss;
+    double amp[TDW_MAX_GAUSS];
+    double cx[TDW_MAX_GAUSS], cy[TDW_MAX_GAUSS], cz[TDW_MAX_GAUSS];
+    double wx[TDW_MAX_GAUSS], wy[TDW_MAX_GAUSS], wz[TDW_MAX_GAUSS];
+    double omega;
+    int switch_on;
+    double t_start, t_stop, t_switch;
+    int flip_b;
+} tdw_vext;
+
+typedef struct {
+    int dim;             /* 1, 2, 3 (CODEDIM)                                 */
+    int nx, ny, nz;      /* NX, NY, NZ; for dim<3 the extra axes are transverse */
+    double dx, dy, dz;   /* DX, DY, DZ                                        */
+    int functional;      /* TDW_SLDA or TDW_BDG                               */
+    int regularization;  /* TDW_SPHERICAL_CUTOFF or TDW_CUBIC_CUTOFF          */
+    int mf_correction;   /* INCLUDE_MF_CORRECTION_FROM_REG_SCHEME             */
+    int scheme;          /* TDW_AB3AM4 / TDW_AB4AM5 / TDW_AB5AM5              */
+    double mu_a, mu_b, ec, dt, t0; /* memcopy_const()                          */
+    double sclgth;       /* memcopy_const_sclgth(), BdG scattering length      */
+    double p_nmin, p_nmax, denseps; /* stabilisation densities, DENSEPSILON   */
+    tdw_vext vext;
+    int precision;       /* TDW_FP64 or TDW_SPLIT                              */
+} tdw_params;
+
+typedef struct tdw_ctx tdw_ctx;
+
+/* ---------------------------------------------------------------------------
+ * Generic layer.  All pointers are device pointers unless named h_*.
+ * Work is queued on the context's stream; results are ordered with every
+ * later call on the same context (as the reference's default stream is).
+ * ------------------------------------------------------------------------- */
+int tdw_create(const tdw_params *p, int nwf, int device, tdw_ctx **out);
+int tdw_destroy(tdw_ctx *c);
+int tdw_sync(tdw_ctx *c);
+size_t tdw_npts(const tdw_ctx *c);
+int tdw_set_params(tdw_ctx *c, const tdw_params *p);   /* constants, V_ext; lattice fixed */
+/* per-state data: occupation weights f(E_n) (n doubles), transverse momenta
+ * ([ky|kz], 2n doubles for dim 1; kz, n doubles for dim 2; NULL for dim 3) and
+ * degeneracies (n ints, dim 1).  Host pointers; copied. */
+int tdw_set_states(tdw_ctx *c, const double *h_fbeta, const double *h_kt, const int *h_cnt);
+
+int tdw_abm_step1(tdw_ctx *c, int n, tdw_complex *y, tdw_complex *f1, tdw_complex *f2,
+                  tdw_complex *f3, tdw_complex *f4, tdw_complex *f5);
+int tdw_abm_step4(tdw_ctx *c, int n, const tdw_complex *hy, tdw_complex *y, const tdw_complex *flast);
+int tdw_normalize(tdw_ctx *c, int n, tdw_complex *wf, double *norms /* n, may be NULL */);
+int tdw_laplace(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *out);
+int tdw_derivatives(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *ddx,
+                    tdw_complex *ddy, tdw_complex *ddz, tdw_complex *lap);
+int tdw_laplace_real(tdw_ctx *c, const double *f, double *lap);
+int tdw_densities(tdw_ctx *c, int n, const tdw_complex *wf, const tdw_complex *ddx,
+                  const tdw_complex *ddy, const tdw_complex *ddz, const tdw_complex *lap,
+                  const double *weights /* n or NULL */, double *dens, int gradients);
+int tdw_density_tau(tdw_ctx *c, double *dens);
+int tdw_potentials(tdw_ctx *c, int it, const double *dens, double *pots);
+int tdw_apply_h(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
+                const tdw_complex *lap, const double *pots, const double *useqpe, double *qpe);
+int tdw_energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e /* 11 */);
+int tdw_vext_eval(tdw_ctx *c, int it, double *va, double *vb);
+
+/* One PECE step of pca.c:1401-1570 on the arrays of a run.  f[0..4] are
+ * fkm1..fkm5 and are rotated in place (pointer swaps) as pca.c:1543-1562
+ * does.  grad_last: full derivatives (tau, j) on this step's corrector. */
+typedef struct {
+    int n;
+    tdw_complex *wf, *lap, *ddx, *ddy, *ddz;
+    tdw_complex *f[5];
+    double *dens, *pots;
+    double *qpe;          /* n: <H> of the last H application */
+} tdw_state;
+int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
+/* Same step with the fused kernels (fewer HBM passes); same results to
+ * rounding.  Histories keep the reference meaning. */
+int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last);
+/* pca.c:939-975: normalise, derivatives, densities, tau (potentials kept). */
+int tdw_initial_measure(tdw_ctx *c, tdw_state *s);
+
+/* ---------------------------------------------------------------------------
+ * Drop-in layer: the reference's entry points, acting on the bound context.
+ * ------------------------------------------------------------------------- */
+int tdw_bind(tdw_ctx *c);
+tdw_ctx *tdw_bound(void);
+
+int set_gpu(int deviceId);                                         /* wslda_cuda_utils.h:109-113 */
+int gpu_malloc(size_t size, void **ptr);                           /* wslda_cuda_utils.h:47-57   */
+int gpu_free(void *ptr);
+int host_malloc_pl(size_t size, void **ptr);                       /* pinned host memory         */
+int host_free_pl(void *ptr);
+int memcopy_host2gpu(void *h_src, void *d_dst, size_t size);       /* wslda_cuda_utils.h:63-76   */
+int memcopy_gpu2host(void *d_src, void *h_dst, size_t size);
+int memcopy_gpu2gpu(void *d_src, void *d_dst, size_t size);
+int memcopy_const(double mu_a, double mu_b, double ec, double t0, double dt, double kF); /* :21-36 */
+int memcopy_const_params(double *params);                          /* :80-90 (no-op: params live in tdw_vext) */
+int memcopy_const_sclgth(double sclgth);                           /* :92-97 */
+
+int create_cufftPlans(int batch_size, int nwfip, size_t *workSize); /* pca_derivative.h:12 */
+int set_workspace_for_cufftPlan(void *workArea);                    /* pca_derivative.h:13 */
+int compute_derivatives(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_complex *wf_d_dy,
+                        tdw_complex *wf_d_dz, tdw_complex *wf_laplace, int nthreads); /* pca_derivative.cu:217 */
+int compute_laplace(int n, tdw_complex *wf, tdw_complex *wf_laplace, int nthreads); /* pca_derivative.cu:332 */
+int compute_laplace_real_f(double *f, double *laplace_f, int nthreads);            /* pca_derivative.cu:640 */
+
+int calculate_densities(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_complex *wf_d_dy,
+                        tdw_complex *wf_d_dz, tdw_complex *d_wf_laplace, double *d_fbetaEn,
+                        double *weights, double *d_densities, int gradients_computed,
+                        int nthreads);                              /* pca_dens.cu:220 */
+int calculate_densities_1d(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_complex *d_wf_laplace,
+                           double *d_kkyz, double *d_fbetaEn, double *weights, int *d_cnt,
+                           double *d_densities, int gradients_computed, int nthreads); /* ccpca_dens.h:5-11 */
+int calculate_densities_2d(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_complex *wf_d_dy,
+                           tdw_complex *d_wf_laplace, double *d_kkz, double *d_fbetaEn,
+                           double *weights, double *d_densities, int gradients_computed,
+                           int nthreads);                            /* cpca_dens.h */
+int density_caculate_tau(double *d_densities, int nthreads);         /* pca_dens.cu:300 */
+
+int compute_potentials(int it, double *d_densities, double *d_potentials, double cccoeff,
+                       int nthreads);                                /* pca_kernels.cu:81 */
+int compute_energy(int it, double *d_densities, double *d_potentials, double *d_workarea,
+                   int nthreads);                                    /* pca_kernels.cu:210 */
+int apply_hamiltonian(int it, int n, tdw_complex *wf_in, tdw_complex *wf_out,
+                      tdw_complex *wf_d_dx, tdw_complex *wf_d_dy, tdw_complex *wf_d_dz,
+                      tdw_complex *wf_laplace, tdw_complex *alphawf_laplace,
+                      double *d_densities, double *d_potentials, double qfswitch,
+                      double *useqpe, double pccoeff, int nthreads); /* pca_kernels.cu:674 */
+int apply_hamiltonian_1d(int it, int n, tdw_complex *wf_in, tdw_complex *wf_out,
+                         tdw_complex *wf_d_dx, double *d_kkyz, tdw_complex *wf_laplace,
+                         tdw_complex *alphawf_laplace, double *d_densities, double *d_potentials,
+                         double qfswitch, double *useqpe, double pccoeff, int nthreads);
+int apply_hamiltonian_2d(int it, int n, tdw_complex *wf_in, tdw_complex *wf_out,
+                         tdw_complex *wf_d_dx, tdw_complex *wf_d_dy, double *d_kkz,
+                         tdw_complex *wf_laplace, tdw_complex *alphawf_laplace,
+                         double *d_densities, double *d_potentials, double qfswitch,
+                         double *useqpe, double pccoeff, int nthreads);
+int normalize_wf(int n, tdw_complex *wf, int nthreads);              /* pca_kernels.cu:955 */
+int abm_step1(int n, tdw_complex *ykm1, tdw_complex *fkm1, tdw_complex *fkm2, tdw_complex *fkm3,
+              tdw_complex *fkm4, tdw_complex *fkm5, int abm_scheme, int nthreads); /* tdwslda_ode_integrator.h:106 */
+int abm_step4(int n, tdw_complex *ykm1_in, tdw_complex *ykm1_out, tdw_complex *fkm_last,
+              int abm_scheme, int nthreads);                         /* tdwslda_ode_integrator.h:195 */
+int taylor_expansion_contribution(int it, double dt, int n, tdw_complex *wf_hpsi,
+                                  tdw_complex *wf_update, tdw_complex *wf_contr,
+                                  int nthreads);                     /* pca_kernels.cu:1076 */
+int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kernels.cu:1106 (host data) */
+
+#if defined(TDW_CODEDIM) && TDW_CODEDIM == 1
+#define apply_hamiltonian apply_hamiltonian_1d
+#define calculate_densities calculate_densities_1d
+#elif defined(TDW_CODEDIM) && TDW_CODEDIM == 2
+#define apply_hamiltonian apply_hamiltonian_2d
+#define calculate_densities calculate_densities_2d
+#endif
+
+#ifdef __cplusplus
+}
+#endif
+#endif

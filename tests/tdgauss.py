@@ -70,3 +70,13 @@ def digits_match(mine, ref, sig=10):
         return abs(mine) < 1e-20
     ulp = 10 ** (math.floor(math.log10(abs(ref))) - (sig - 1))
     return abs(mine - ref) <= 1.01 * ulp
+
+
+def to_params(cfg, precision=0):
+    """Library parameters (wslda_amd.td.Params) equal to an oracle Cfg."""
+    import ctypes
+    from wslda_amd import td
+    p = td.Params()
+    ctypes.memmove(ctypes.addressof(p), ctypes.addressof(cfg), ctypes.sizeof(cfg))
+    p.precision = precision
+    return p

@@ -1,0 +1,560 @@
+// tdw_api.hip -- host side: context, FFT plans, the generic tdw_* layer, the
+// modular PECE step and the reference-named drop-in entry points.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "tdw_internal.hpp"
+#include "tdw_physics.hpp"
+#include "tdw_fused.hpp"
+
+using namespace tdw;
+
+namespace tdw {
+
+int ensure_part(tdw_ctx *c, size_t nd)
+{
+    if (nd <= c->part_cap) return 0;
+    TDW_HIP(hipStreamSynchronize(c->stream));   // in-flight kernels may still read it
+    if (c->d_part) TDW_HIP(hipFree(c->d_part));
+    c->d_part = nullptr;
+    c->part_cap = 0;
+    TDW_HIP(hipMalloc(&c->d_part, nd * sizeof(double)));
+    c->part_cap = nd;
+    return 0;
+}
+
+int ensure_T(tdw_ctx *c, size_t ne)
+{
+    if (ne <= c->T_cap) return 0;
+    TDW_HIP(hipStreamSynchronize(c->stream));
+    if (c->d_T) TDW_HIP(hipFree(c->d_T));
+    c->d_T = nullptr;
+    c->T_cap = 0;
+    TDW_HIP(hipMalloc(&c->d_T, ne * sizeof(double2)));
+    c->T_cap = ne;
+    return 0;
+}
+
+// Laplacian of m arrays.  1D: one outer pass.  2D/3D: outer pass into T
+// (the output array when it differs from the input), inner pass into out.
+int laplace(tdw_ctx *c, int m, const double2 *in, double2 *out)
+{
+    if (m <= 0) return 0;
+    if (c->dim == 1) return launch_outer(c, m, in, out, nullptr, 0);
+    if (out != in) {
+        TDW_TRY(launch_outer(c, m, in, out, nullptr, 0));
+        return launch_inner(c, m, in, out, out, nullptr, nullptr, 0);
+    }
+    // in place: batch through the scratch array
+    const size_t N = c->N;
+    int batch = m;
+    const size_t cap = (size_t)1 << 26;   // 64 Mi complex = 1 GiB
+    if ((size_t)batch * N > cap) batch = (int)(cap / N) > 0 ? (int)(cap / N) : 1;
+    TDW_TRY(ensure_T(c, (size_t)batch * N));
+    for (int a0 = 0; a0 < m; a0 += batch) {
+        int mm = (m - a0 < batch) ? m - a0 : batch;
+        TDW_TRY(launch_outer(c, mm, in + (size_t)a0 * N, c->d_T, nullptr, 0));
+        TDW_TRY(launch_inner(c, mm, in + (size_t)a0 * N, c->d_T, out + (size_t)a0 * N, nullptr, nullptr, 0));
+    }
+    return 0;
+}
+
+int derivatives(tdw_ctx *c, int m, const double2 *in, double2 *ddx, double2 *ddy, double2 *ddz, double2 *lap)
+{
+    if (m <= 0) return 0;
+    if (lap == in || ddx == in) return TDW_EINVAL;
+    TDW_TRY(launch_outer(c, m, in, lap, ddx, 1));
+    if (c->dim == 1) return 0;
+    return launch_inner(c, m, in, lap, lap, ddy, c->dim == 3 ? ddz : nullptr, 1);
+}
+
+}  // namespace tdw
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static int plan_axis(int L, tdw_axis *a)
+{
+    a->L = L;
+    a->nst = 0;
+    int rem = L, Ns = 1;
+    while (rem > 1) {
+        int R;
+        if (rem % 8 == 0) R = 8;
+        else if (rem % 4 == 0) R = 4;
+        else if (rem % 3 == 0) R = 3;
+        else if (rem % 2 == 0) R = 2;
+        else return TDW_ESIZE;
+        if (a->nst >= TDW_MAXST) return TDW_ESIZE;
+        a->R[a->nst] = R;
+        a->Ns[a->nst] = Ns;
+        a->nst++;
+        Ns *= R;
+        rem /= R;
+    }
+    return 0;
+}
+
+static int upload_twiddles(tdw_ctx *c, int k, int L)
+{
+    double2 *h = (double2 *)malloc(sizeof(double2) * L);
+    float2 *hf = (float2 *)malloc(sizeof(float2) * L);
+    for (int t = 0; t < L; t++) {
+        double a = 2.0 * M_PI * (double)t / (double)L;
+        h[t] = make_double2(cos(a), -sin(a));
+        hf[t] = make_float2((float)cos(a), (float)-sin(a));
+    }
+    int e = 0;
+    if (hipMalloc(&c->d_tw[k], sizeof(double2) * L) != hipSuccess) e = TDW_ENOMEM;
+    if (!e && hipMalloc(&c->d_tw32[k], sizeof(float2) * L) != hipSuccess) e = TDW_ENOMEM;
+    if (!e) e = (int)hipMemcpy(c->d_tw[k], h, sizeof(double2) * L, hipMemcpyHostToDevice);
+    if (!e) e = (int)hipMemcpy(c->d_tw32[k], hf, sizeof(float2) * L, hipMemcpyHostToDevice);
+    free(h);
+    free(hf);
+    return e;
+}
+
+extern "C" int tdw_create(const tdw_params *p, int nwf, int device, tdw_ctx **out)
+{
+    if (!p || !out || nwf <= 0) return TDW_EINVAL;
+    if (p->dim < 1 || p->dim > 3 || p->nx < 2) return TDW_EINVAL;
+    if (p->functional != TDW_SLDA && p->functional != TDW_BDG) return TDW_EINVAL;
+    if (p->scheme != TDW_AB3AM4 && p->scheme != TDW_AB4AM5 && p->scheme != TDW_AB5AM5) return TDW_EINVAL;
+    TDW_HIP(hipSetDevice(device));
+    tdw_ctx *c = (tdw_ctx *)calloc(1, sizeof(tdw_ctx));
+    if (!c) return TDW_ENOMEM;
+    c->p = *p;
+    c->dim = p->dim;
+    c->nx = p->nx;
+    c->ny = p->dim >= 2 ? p->ny : 1;
+    c->nz = p->dim >= 3 ? p->nz : 1;
+    c->N = (size_t)c->nx * c->ny * c->nz;
+    c->S = (int)(c->N / c->nx);
+    c->nwf = nwf;
+    c->device = device;
+    c->Lphys[0] = p->nx * p->dx;
+    c->Lphys[1] = p->ny * p->dy;
+    c->Lphys[2] = p->nz * p->dz;
+    int e = plan_axis(c->nx, &c->ax[0]);
+    if (!e && c->dim >= 2) e = plan_axis(c->ny, &c->ax[1]);
+    if (!e && c->dim >= 3) e = plan_axis(c->nz, &c->ax[2]);
+    if (!e) e = (int)hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (!e && c->max_lds > 160 * 1024) c->max_lds = 160 * 1024;
+    if (!e) e = (int)hipStreamCreate(&c->stream);
+    if (!e) e = upload_twiddles(c, 0, c->nx);
+    if (!e && c->dim >= 2) e = upload_twiddles(c, 1, c->ny);
+    if (!e && c->dim >= 3) e = upload_twiddles(c, 2, c->nz);
+    if (!e) e = (int)hipMalloc(&c->d_fbeta, sizeof(double) * nwf);
+    if (!e) e = (int)hipMalloc(&c->d_kt, sizeof(double) * 2 * nwf);
+    if (!e) e = (int)hipMalloc(&c->d_cnt, sizeof(int) * nwf);
+    if (!e) e = (int)hipMalloc(&c->d_nrm, sizeof(double) * nwf);
+    if (!e) e = (int)hipMalloc(&c->d_qpe, sizeof(double) * nwf);
+    if (!e) e = (int)hipMalloc(&c->d_scale, sizeof(double) * nwf);
+    if (!e) e = (int)hipMalloc(&c->d_esum, sizeof(double) * 64);
+    if (!e) e = (int)hipMalloc(&c->d_rtmp, sizeof(double2) * c->N);
+    if (!e) e = (int)hipMalloc(&c->d_vext, sizeof(double) * 2 * c->N);
+    if (!e) e = (int)hipMemset(c->d_kt, 0, sizeof(double) * 2 * nwf);
+    if (!e) {
+        int *ones = (int *)malloc(sizeof(int) * nwf);
+        for (int i = 0; i < nwf; i++) ones[i] = 1;
+        e = (int)hipMemcpy(c->d_cnt, ones, sizeof(int) * nwf, hipMemcpyHostToDevice);
+        free(ones);
+    }
+    if (e) {
+        tdw_destroy(c);
+        return e;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int tdw_destroy(tdw_ctx *c)
+{
+    if (!c) return 0;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 3; k++) {
+        if (c->d_tw[k]) hipFree(c->d_tw[k]);
+        if (c->d_tw32[k]) hipFree(c->d_tw32[k]);
+    }
+    void *bufs[] = {c->d_fbeta, c->d_kt, c->d_cnt, c->d_part, c->d_nrm, c->d_qpe, c->d_esum,
+                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale};
+    for (void *b : bufs)
+        if (b) hipFree(b);
+    fused_release(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    if (tdw_bound() == c) tdw_bind(nullptr);
+    free(c);
+    return 0;
+}
+
+extern "C" int tdw_sync(tdw_ctx *c)
+{
+    if (!c) return TDW_EINVAL;
+    TDW_HIP(hipStreamSynchronize(c->stream));
+    return (int)hipGetLastError();
+}
+
+extern "C" size_t tdw_npts(const tdw_ctx *c) { return c ? c->N : 0; }
+
+extern "C" int tdw_set_params(tdw_ctx *c, const tdw_params *p)
+{
+    if (!c || !p) return TDW_EINVAL;
+    if (p->dim != c->p.dim || p->nx != c->p.nx || (c->dim >= 2 && p->ny != c->p.ny) ||
+        (c->dim >= 3 && p->nz != c->p.nz))
+        return TDW_EINVAL;
+    c->p = *p;
+    return 0;
+}
+
+extern "C" int tdw_set_states(tdw_ctx *c, const double *h_fbeta, const double *h_kt, const int *h_cnt)
+{
+    if (!c) return TDW_EINVAL;
+    const int n = c->nwf;
+    if (h_fbeta) TDW_HIP(hipMemcpy(c->d_fbeta, h_fbeta, sizeof(double) * n, hipMemcpyHostToDevice));
+    if (h_kt) TDW_HIP(hipMemcpy(c->d_kt, h_kt, sizeof(double) * (c->dim == 1 ? 2 * n : n), hipMemcpyHostToDevice));
+    if (h_cnt) TDW_HIP(hipMemcpy(c->d_cnt, h_cnt, sizeof(int) * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+#define D2(x) ((double2 *)(x))
+#define CD2(x) ((const double2 *)(x))
+
+extern "C" int tdw_abm_step1(tdw_ctx *c, int n, tdw_complex *y, tdw_complex *f1, tdw_complex *f2,
+                             tdw_complex *f3, tdw_complex *f4, tdw_complex *f5)
+{
+    if (!c) return TDW_EINVAL;
+    return abm1(c, n, D2(y), D2(f1), D2(f2), D2(f3), D2(f4), D2(f5));
+}
+extern "C" int tdw_abm_step4(tdw_ctx *c, int n, const tdw_complex *hy, tdw_complex *y, const tdw_complex *fl)
+{
+    if (!c) return TDW_EINVAL;
+    return abm4(c, n, CD2(hy), D2(y), CD2(fl));
+}
+extern "C" int tdw_normalize(tdw_ctx *c, int n, tdw_complex *wf, double *nrm)
+{
+    if (!c || n > c->nwf) return TDW_EINVAL;
+    return normalize(c, n, D2(wf), nrm ? nrm : c->d_nrm);
+}
+extern "C" int tdw_laplace(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *out)
+{
+    if (!c) return TDW_EINVAL;
+    return laplace(c, m, CD2(in), D2(out));
+}
+extern "C" int tdw_derivatives(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *ddx, tdw_complex *ddy,
+                               tdw_complex *ddz, tdw_complex *lap)
+{
+    if (!c) return TDW_EINVAL;
+    return derivatives(c, m, CD2(in), D2(ddx), D2(ddy), D2(ddz), D2(lap));
+}
+extern "C" int tdw_laplace_real(tdw_ctx *c, const double *f, double *lap)
+{
+    if (!c) return TDW_EINVAL;
+    return laplace_real(c, f, lap);
+}
+extern "C" int tdw_densities(tdw_ctx *c, int n, const tdw_complex *wf, const tdw_complex *ddx,
+                             const tdw_complex *ddy, const tdw_complex *ddz, const tdw_complex *lap,
+                             const double *weights, double *dens, int gradients)
+{
+    if (!c || n > c->nwf) return TDW_EINVAL;
+    return densities(c, n, CD2(wf), CD2(ddx), CD2(ddy), CD2(ddz), CD2(lap), c->d_fbeta, weights, c->d_kt,
+                     c->d_cnt, dens, gradients);
+}
+extern "C" int tdw_density_tau(tdw_ctx *c, double *dens)
+{
+    if (!c) return TDW_EINVAL;
+    return density_tau(c, dens);
+}
+extern "C" int tdw_potentials(tdw_ctx *c, int it, const double *dens, double *pots)
+{
+    if (!c) return TDW_EINVAL;
+    return potentials(c, it, dens, pots);
+}
+extern "C" int tdw_apply_h(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
+                           const tdw_complex *lap, const double *pots, const double *useqpe, double *qpe)
+{
+    if (!c || n > c->nwf || in == out) return TDW_EINVAL;
+    return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
+}
+extern "C" int tdw_energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e)
+{
+    if (!c) return TDW_EINVAL;
+    return energy(c, it, dens, pots, h_e);
+}
+extern "C" int tdw_vext_eval(tdw_ctx *c, int it, double *va, double *vb)
+{
+    if (!c) return TDW_EINVAL;
+    return vext_eval(c, it, va, vb);
+}
+
+// ---------------------------------------------------------------------------
+// modular PECE step: the call sequence of pca.c:1401-1570
+// ---------------------------------------------------------------------------
+static int half_densities(tdw_ctx *c, tdw_state *s, int grad)
+{
+    const int n = s->n;
+    if (grad) TDW_TRY(derivatives(c, 2 * n, D2(s->wf), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap)));
+    else TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    TDW_TRY(densities(c, n, D2(s->wf), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), c->d_fbeta, nullptr,
+                      c->d_kt, c->d_cnt, s->dens, grad));
+    if (grad) TDW_TRY(density_tau(c, s->dens));
+    return 0;
+}
+
+static int nhist(const tdw_ctx *c)
+{
+    return c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5;
+}
+
+extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    if (!c || !s || s->n > c->nwf) return TDW_EINVAL;
+    const int n = s->n, k = nhist(c);
+    double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    // predictor
+    TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
+    TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
+    TDW_TRY(half_densities(c, s, 0));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    // corrector
+    TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
+    TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
+    TDW_TRY(half_densities(c, s, grad_last));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    tdw_complex *tmp = s->f[k - 1];
+    for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+    s->f[0] = tmp;
+    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    return 0;
+}
+
+extern "C" int tdw_initial_measure(tdw_ctx *c, tdw_state *s)
+{
+    if (!c || !s) return TDW_EINVAL;
+    TDW_TRY(normalize(c, s->n, D2(s->wf), c->d_nrm));
+    return half_densities(c, s, 1);
+}
+
+extern "C" int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    if (!c || !s || s->n > c->nwf) return TDW_EINVAL;
+    return fused_step(c, s, it, grad_last);
+}
+
+// ---------------------------------------------------------------------------
+// drop-in layer
+// ---------------------------------------------------------------------------
+static tdw_ctx *g_bound = nullptr;
+
+extern "C" int tdw_bind(tdw_ctx *c)
+{
+    g_bound = c;
+    return 0;
+}
+extern "C" tdw_ctx *tdw_bound(void) { return g_bound; }
+
+#define BOUND()                        \
+    tdw_ctx *c = g_bound;              \
+    if (!c) return TDW_ENOCTX;
+
+extern "C" int set_gpu(int deviceId) { return (int)hipSetDevice(deviceId); }
+extern "C" int gpu_malloc(size_t size, void **ptr) { return (int)hipMalloc(ptr, size); }
+extern "C" int gpu_free(void *ptr) { return (int)hipFree(ptr); }
+extern "C" int host_malloc_pl(size_t size, void **ptr) { return (int)hipHostMalloc(ptr, size, hipHostMallocDefault); }
+extern "C" int host_free_pl(void *ptr) { return (int)hipHostFree(ptr); }
+extern "C" int memcopy_host2gpu(void *h, void *d, size_t size)
+{
+    if (g_bound) TDW_HIP(hipStreamSynchronize(g_bound->stream));
+    return (int)hipMemcpy(d, h, size, hipMemcpyHostToDevice);
+}
+extern "C" int memcopy_gpu2host(void *d, void *h, size_t size)
+{
+    if (g_bound) TDW_HIP(hipStreamSynchronize(g_bound->stream));
+    return (int)hipMemcpy(h, d, size, hipMemcpyDeviceToHost);
+}
+extern "C" int memcopy_gpu2gpu(void *src, void *dst, size_t size)
+{
+    if (g_bound) TDW_HIP(hipStreamSynchronize(g_bound->stream));
+    return (int)hipMemcpy(dst, src, size, hipMemcpyDeviceToDevice);
+}
+extern "C" int memcopy_const(double mu_a, double mu_b, double ec, double t0, double dt, double kF)
+{
+    BOUND();
+    (void)kF;
+    c->p.mu_a = mu_a;
+    c->p.mu_b = mu_b;
+    c->p.ec = ec;
+    c->p.t0 = t0;
+    c->p.dt = dt;
+    return 0;
+}
+extern "C" int memcopy_const_params(double *params)
+{
+    (void)params;
+    return 0;
+}
+extern "C" int memcopy_const_sclgth(double a)
+{
+    BOUND();
+    c->p.sclgth = a;
+    return 0;
+}
+
+extern "C" int create_cufftPlans(int batch_size, int nwfip, size_t *workSize)
+{
+    BOUND();
+    (void)batch_size;
+    if (nwfip > c->nwf) return TDW_EINVAL;
+    // the caller's work area carries the <H> values and the energies back
+    // (pca.c:1575, :1590), so it must hold at least that much
+    if (workSize) *workSize = sizeof(double) * (size_t)(nwfip > TDW_NITEMS ? nwfip : TDW_NITEMS);
+    return 0;
+}
+extern "C" int set_workspace_for_cufftPlan(void *workArea)
+{
+    BOUND();
+    c->ref_workarea = workArea;
+    return 0;
+}
+extern "C" int compute_derivatives(int n, tdw_complex *wf, tdw_complex *dx, tdw_complex *dy, tdw_complex *dz,
+                                   tdw_complex *lap, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return derivatives(c, n, CD2(wf), D2(dx), D2(dy), D2(dz), D2(lap));
+}
+extern "C" int compute_laplace(int n, tdw_complex *wf, tdw_complex *lap, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return laplace(c, n, CD2(wf), D2(lap));
+}
+extern "C" int compute_laplace_real_f(double *f, double *lap, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return laplace_real(c, f, lap);
+}
+extern "C" int calculate_densities(int n, tdw_complex *wf, tdw_complex *dx, tdw_complex *dy, tdw_complex *dz,
+                                   tdw_complex *lap, double *fbeta, double *weights, double *dens, int grad,
+                                   int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return densities(c, n, CD2(wf), CD2(dx), CD2(dy), CD2(dz), CD2(lap), fbeta, weights, c->d_kt, c->d_cnt, dens,
+                     grad);
+}
+extern "C" int calculate_densities_1d(int n, tdw_complex *wf, tdw_complex *dx, tdw_complex *lap, double *kkyz,
+                                      double *fbeta, double *weights, int *cnt, double *dens, int grad,
+                                      int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return densities(c, n, CD2(wf), CD2(dx), nullptr, nullptr, CD2(lap), fbeta, weights, kkyz, cnt, dens, grad);
+}
+extern "C" int calculate_densities_2d(int n, tdw_complex *wf, tdw_complex *dx, tdw_complex *dy, tdw_complex *lap,
+                                      double *kkz, double *fbeta, double *weights, double *dens, int grad,
+                                      int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return densities(c, n, CD2(wf), CD2(dx), CD2(dy), nullptr, CD2(lap), fbeta, weights, kkz, c->d_cnt, dens, grad);
+}
+extern "C" int density_caculate_tau(double *dens, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return density_tau(c, dens);
+}
+extern "C" int compute_potentials(int it, double *dens, double *pots, double cccoeff, int nthreads)
+{
+    BOUND();
+    (void)cccoeff;
+    (void)nthreads;
+    return potentials(c, it, dens, pots);
+}
+extern "C" int compute_energy(int it, double *dens, double *pots, double *work, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    double e[TDW_NITEMS];
+    TDW_TRY(energy(c, it, dens, pots, e));
+    if (work) TDW_HIP(hipMemcpy(work, e, sizeof(e), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *lap,
+                       const double *kt, double *pots, double qfswitch, double *useqpe)
+{
+    if (qfswitch > 0.0) return TDW_EINVAL;   // quantum friction is not part of this path
+    TDW_TRY(apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, kt, useqpe, c->d_qpe));
+    // <H> is left at the head of the work area (pca_kernels.cu:812-822)
+    if (c->ref_workarea)
+        TDW_HIP(hipMemcpyAsync(c->ref_workarea, useqpe ? useqpe : c->d_qpe, sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+extern "C" int apply_hamiltonian(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx, tdw_complex *dy,
+                                 tdw_complex *dz, tdw_complex *lap, tdw_complex *alap, double *dens, double *pots,
+                                 double qfswitch, double *useqpe, double pccoeff, int nthreads)
+{
+    BOUND();
+    (void)dx; (void)dy; (void)dz; (void)alap; (void)dens; (void)pccoeff; (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, lap, c->d_kt, pots, qfswitch, useqpe);
+}
+extern "C" int apply_hamiltonian_1d(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx, double *kkyz,
+                                    tdw_complex *lap, tdw_complex *alap, double *dens, double *pots, double qfswitch,
+                                    double *useqpe, double pccoeff, int nthreads)
+{
+    BOUND();
+    (void)dx; (void)alap; (void)dens; (void)pccoeff; (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, lap, kkyz, pots, qfswitch, useqpe);
+}
+extern "C" int apply_hamiltonian_2d(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx,
+                                    tdw_complex *dy, double *kkz, tdw_complex *lap, tdw_complex *alap, double *dens,
+                                    double *pots, double qfswitch, double *useqpe, double pccoeff, int nthreads)
+{
+    BOUND();
+    (void)dx; (void)dy; (void)alap; (void)dens; (void)pccoeff; (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, lap, kkz, pots, qfswitch, useqpe);
+}
+extern "C" int normalize_wf(int n, tdw_complex *wf, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return normalize(c, n, D2(wf), c->d_nrm);
+}
+extern "C" int abm_step1(int n, tdw_complex *y, tdw_complex *f1, tdw_complex *f2, tdw_complex *f3, tdw_complex *f4,
+                         tdw_complex *f5, int scheme, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (scheme != TDW_AB3AM4 && scheme != TDW_AB4AM5 && scheme != TDW_AB5AM5) return -999999;
+    c->p.scheme = scheme;
+    return abm1(c, n, D2(y), D2(f1), D2(f2), D2(f3), D2(f4), D2(f5));
+}
+extern "C" int abm_step4(int n, tdw_complex *in, tdw_complex *out, tdw_complex *flast, int scheme, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (scheme != TDW_AB3AM4 && scheme != TDW_AB4AM5 && scheme != TDW_AB5AM5) return -999999;
+    c->p.scheme = scheme;
+    return abm4(c, n, CD2(in), D2(out), CD2(flast));
+}
+extern "C" int taylor_expansion_contribution(int order, double dt, int n, tdw_complex *hpsi, tdw_complex *upd,
+                                             tdw_complex *contr, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return taylor(c, order, dt, n, CD2(hpsi), D2(upd), D2(contr));
+}
+extern "C" int get_v_ext(int datadim, int spin, int it, double *data)
+{
+    BOUND();
+    (void)datadim;
+    TDW_TRY(vext_eval(c, it, c->d_vext, c->d_vext + c->N));
+    TDW_HIP(hipStreamSynchronize(c->stream));
+    return (int)hipMemcpy(data, c->d_vext + (spin ? c->N : 0), sizeof(double) * c->N, hipMemcpyDeviceToHost);
+}

@@ -1,0 +1,237 @@
+// tdw_fft.hip -- spectral Laplacian / gradient passes of the modular path.
+//
+// Outer pass: pencils along x (stride S = N/nx).  A workgroup takes TP
+// consecutive pencils (consecutive inner offsets j, so the loads of one x-row
+// are contiguous runs of TP complex values), transforms them in LDS and
+// writes T = D_x(in) (and ddx = G_x(in)).
+// Inner pass: contiguous blocks of S = ny*nz (3D) or ny (2D) points; per block
+// the z- and y-transforms run in LDS and out = T + D_y(in) + D_z(in).
+// 1D problems have no inner axis: the outer pass writes the Laplacian.
+#include "tdw_fft.hpp"
+
+namespace tdw {
+
+static constexpr int NTHR = 256;
+
+// --------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(NTHR)
+k_outer(const double2 *in, double2 *T, double2 *__restrict__ ddx,
+        int M, long N, int S, int TP, tdw_axis ax, const double2 *__restrict__ twg, double Lp)
+{
+    extern __shared__ double2 sm[];
+    const int L = ax.L;
+    double2 *tw = sm;
+    double2 *A = sm + L;
+    double2 *B = A + TP * L;
+    double2 *D = B + TP * L;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < L; k += NTHR) tw[k] = twg[k];
+    const long P0 = (long)blockIdx.x * TP;
+    const long Ptot = (long)M * S;
+    const bool contig = (S == 1);
+    for (int e = tid; e < TP * L; e += NTHR) {
+        int p, i;
+        if (contig) { i = e % L; p = e / L; }
+        else { p = e % TP; i = e / TP; }
+        long P = P0 + p;
+        double2 v = make_double2(0.0, 0.0);
+        if (P < Ptot) {
+            long a = P / S, j = P - a * S;
+            v = in[a * N + j + (long)i * S];
+        }
+        A[p + i * TP] = v;
+    }
+    __syncthreads();
+    double2 *X = lds_fft(A, B, TP, 1, TP, ax, tw, -1, tid, NTHR);
+    double2 *Y = (X == A) ? B : A;
+    for (int e = tid; e < TP * L; e += NTHR) {
+        int i = e / TP;
+        double2 v = X[e];
+        if (MODE == 1) D[e] = imul(v, kfac1(i, L, Lp));
+        X[e] = v * kfac2(i, L, Lp);
+    }
+    __syncthreads();
+    double2 *R = lds_fft(X, Y, TP, 1, TP, ax, tw, +1, tid, NTHR);
+    for (int e = tid; e < TP * L; e += NTHR) {
+        int p, i;
+        if (contig) { i = e % L; p = e / L; }
+        else { p = e % TP; i = e / TP; }
+        long P = P0 + p;
+        if (P < Ptot) {
+            long a = P / S, j = P - a * S;
+            T[a * N + j + (long)i * S] = R[p + i * TP];
+        }
+    }
+    if (MODE == 1) {
+        __syncthreads();
+        // A and B are free again: ping-pong D <-> A
+        double2 *R2 = lds_fft(D, A, TP, 1, TP, ax, tw, +1, tid, NTHR);
+        for (int e = tid; e < TP * L; e += NTHR) {
+            int p, i;
+            if (contig) { i = e % L; p = e / L; }
+            else { p = e % TP; i = e / TP; }
+            long P = P0 + p;
+            if (P < Ptot) {
+                long a = P / S, j = P - a * S;
+                ddx[a * N + j + (long)i * S] = R2[p + i * TP];
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// Inner pass.  NAX = 1 (2D: y) or 2 (3D: z then y).  KM = max elements per
+// thread.  For 3D one block = one yz-plane (TB = 1); for 2D TB y-lines.
+// in, T and out may alias: each element is read and written by one thread.
+template <int MODE, int NAX, int KM>
+__global__ void __launch_bounds__(NTHR)
+k_inner(const double2 *in, const double2 *T, double2 *out,
+        double2 *__restrict__ ddy, double2 *__restrict__ ddz, long nblk, int S, int TB, int ny,
+        int nz, tdw_axis axy, tdw_axis axz, const double2 *__restrict__ twy,
+        const double2 *__restrict__ twz, double Ly, double Lz)
+{
+    extern __shared__ double2 sm[];
+    const int E = TB * S;
+    double2 *ty = sm;
+    double2 *tz = ty + ny;
+    double2 *A = tz + (NAX == 2 ? nz : 0);
+    double2 *B = A + E;
+    double2 *D = B + E;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < ny; k += NTHR) ty[k] = twy[k];
+    if (NAX == 2)
+        for (int k = tid; k < nz; k += NTHR) tz[k] = twz[k];
+    const long off = (long)blockIdx.x * E;
+    const long lim = nblk * (long)S;   // total elements
+    double2 X[KM], O[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        int e = tid + k * NTHR;
+        X[k] = make_double2(0.0, 0.0);
+        O[k] = make_double2(0.0, 0.0);
+        if (e < E && off + e < lim) {
+            X[k] = in[off + e];
+            if (T) O[k] = T[off + e];
+        }
+    }
+#pragma unroll
+    for (int axi = 0; axi < NAX; axi++) {
+        // axis geometry: 3D: axi 0 = z (contiguous), axi 1 = y (stride nz)
+        //                2D: axi 0 = y (contiguous)
+        const bool isz = (NAX == 2 && axi == 0);
+        const tdw_axis &ax = isz ? axz : axy;
+        const double2 *tw = isz ? tz : ty;
+        const int L = ax.L;
+        const double Lp = isz ? Lz : Ly;
+        int P, ps, is;
+        if (NAX == 2) {
+            if (isz) { P = ny; ps = nz; is = 1; }
+            else { P = nz; ps = 1; is = nz; }
+        } else { P = TB; ps = ny; is = 1; }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            int e = tid + k * NTHR;
+            if (e < E) A[e] = X[k];
+        }
+        __syncthreads();
+        double2 *F = lds_fft(A, B, P, ps, is, ax, tw, -1, tid, NTHR);
+        double2 *G = (F == A) ? B : A;
+        for (int e = tid; e < E; e += NTHR) {
+            int i = isz ? (e % nz) : (NAX == 2 ? (e / nz) % ny : e % ny);
+            double2 v = F[e];
+            if (MODE == 1) D[e] = imul(v, kfac1(i, L, Lp));
+            F[e] = v * kfac2(i, L, Lp);
+        }
+        __syncthreads();
+        double2 *R = lds_fft(F, G, P, ps, is, ax, tw, +1, tid, NTHR);
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            int e = tid + k * NTHR;
+            if (e < E) O[k] = O[k] + R[e];
+        }
+        if (MODE == 1) {
+            __syncthreads();
+            double2 *R2 = lds_fft(D, A, P, ps, is, ax, tw, +1, tid, NTHR);
+            double2 *dd = (NAX == 2 && isz) ? ddz : ddy;
+            for (int e = tid; e < E; e += NTHR)
+                if (off + e < lim) dd[off + e] = R2[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        int e = tid + k * NTHR;
+        if (e < E && off + e < lim) out[off + e] = O[k];
+    }
+}
+
+// --------------------------------------------------------------------------
+static int pick_tp(const tdw_ctx *c, int mode)
+{
+    const size_t per = sizeof(double2) * (size_t)c->nx * (mode ? 3 : 2);
+    int tp = c->S == 1 ? 8 : 32;
+    while (tp > 1 && ((c->S > 1 && (c->S % tp) != 0) ||
+                      sizeof(double2) * (size_t)c->nx + per * tp > (size_t)c->max_lds))
+        tp >>= 1;
+    return tp;
+}
+
+int launch_outer(tdw_ctx *c, int m, const double2 *in, double2 *T, double2 *ddx, int mode)
+{
+    const int L = c->nx, S = c->S, TP = pick_tp(c, mode);
+    long Ptot = (long)m * S;
+    long nblk = (Ptot + TP - 1) / TP;
+    size_t lds = sizeof(double2) * ((size_t)L + (size_t)(mode ? 3 : 2) * TP * L);
+    if (lds > (size_t)c->max_lds) return TDW_ESIZE;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_outer<0>, dim3(nblk), dim3(NTHR), lds, c->stream, in, T, ddx, m,
+                           (long)c->N, S, TP, c->ax[0], c->d_tw[0], c->Lphys[0]);
+    else
+        hipLaunchKernelGGL(k_outer<1>, dim3(nblk), dim3(NTHR), lds, c->stream, in, T, ddx, m,
+                           (long)c->N, S, TP, c->ax[0], c->d_tw[0], c->Lphys[0]);
+    return (int)hipGetLastError();
+}
+
+template <int MODE, int NAX>
+static int launch_inner_k(tdw_ctx *c, long nblk_in, int TB, const double2 *in, const double2 *T,
+                          double2 *out, double2 *ddy, double2 *ddz)
+{
+    const int S = c->S, E = TB * S;
+    const int km = (E + NTHR - 1) / NTHR;
+    size_t lds = sizeof(double2) * ((size_t)c->ny + (NAX == 2 ? c->nz : 0) + (size_t)(MODE ? 3 : 2) * E);
+    if (lds > (size_t)c->max_lds) return TDW_ESIZE;
+    long grid = (nblk_in + TB - 1) / TB;
+    int nz = NAX == 2 ? c->nz : 1;
+    const double2 *twz = NAX == 2 ? c->d_tw[2] : c->d_tw[1];
+#define TDW_LI(KMV)                                                                                    \
+    hipLaunchKernelGGL((k_inner<MODE, NAX, KMV>), dim3(grid), dim3(NTHR), lds, c->stream, in, T, out,  \
+                       ddy, ddz, nblk_in, S, TB, c->ny, nz, c->ax[1], c->ax[NAX == 2 ? 2 : 1],       \
+                       c->d_tw[1], twz, c->Lphys[1], c->Lphys[NAX == 2 ? 2 : 1])
+    if (km <= 1) TDW_LI(1);
+    else if (km <= 2) TDW_LI(2);
+    else if (km <= 4) TDW_LI(4);
+    else if (km <= 8) TDW_LI(8);
+    else if (km <= 16) TDW_LI(16);
+    else if (km <= 32) TDW_LI(32);
+    else return TDW_ESIZE;
+#undef TDW_LI
+    return (int)hipGetLastError();
+}
+
+int launch_inner(tdw_ctx *c, int m, const double2 *in, const double2 *T, double2 *out,
+                 double2 *ddy, double2 *ddz, int mode)
+{
+    if (c->dim == 1) return TDW_EINVAL;
+    long nblk = (long)m * c->nx;   // inner blocks of S points
+    if (c->dim == 3) {
+        if (mode == 0) return launch_inner_k<0, 2>(c, nblk, 1, in, T, out, ddy, ddz);
+        return launch_inner_k<1, 2>(c, nblk, 1, in, T, out, ddy, ddz);
+    }
+    int TB = 1;
+    while (TB * c->S < 256 && TB < 64) TB <<= 1;
+    if (mode == 0) return launch_inner_k<0, 1>(c, nblk, TB, in, T, out, ddy, ddz);
+    return launch_inner_k<1, 1>(c, nblk, TB, in, T, out, ddy, ddz);
+}
+
+}  // namespace tdw

@@ -1,0 +1,73 @@
+// tdw_internal.hpp -- library-internal context and launcher declarations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "wslda_td.h"
+
+#define TDW_MAXST 8
+
+// Stockham plan of one lattice axis: radices R[s] applied with span Ns[s]
+// (product of the earlier radices).
+struct tdw_axis {
+    int L;
+    int nst;
+    int R[TDW_MAXST];
+    int Ns[TDW_MAXST];
+};
+
+struct tdw_ctx {
+    tdw_params p;
+    int dim;
+    int nx, ny, nz;   // evolved axes: nx always, ny if dim>=2, nz if dim==3
+    size_t N;         // evolved points
+    int S;            // inner block = N / nx (ny*nz, ny, or 1)
+    int nwf;
+    int device;
+    int max_lds;      // bytes of LDS one workgroup may allocate
+    hipStream_t stream;
+    tdw_axis ax[3];            // 0:x 1:y 2:z
+    double2 *d_tw[3];          // exp(-2*pi*i*k/L), k < L, per axis
+    float2 *d_tw32[3];
+    double Lphys[3];           // nx*dx, ny*dy, nz*dz
+    // per-state data
+    double *d_fbeta;
+    double *d_kt;
+    int *d_cnt;
+    // scratch
+    double *d_part;  size_t part_cap;   // partial sums (doubles)
+    double *d_nrm;                       // n norms
+    double *d_qpe;                       // n <H>
+    double *d_esum;                      // energies (TDW_NITEMS)
+    double2 *d_T;    size_t T_cap;       // laplacian scratch (complex elements)
+    double2 *d_rtmp;                     // N complex (real-field transforms)
+    double *d_vext;                      // 2N
+    void *ref_workarea;                  // set_workspace_for_cufftPlan()
+    // fused-path state
+    double *d_scale;                     // n: 1/sqrt(norm) of the unnormalised state
+    int fused_ready;
+};
+
+namespace tdw {
+int ensure_part(tdw_ctx *c, size_t ndoubles);
+int ensure_T(tdw_ctx *c, size_t nelem);
+
+// FFT launchers (tdw_fft.hip).  mode 0: laplacian, 1: derivatives.
+// Outer (x) pass: T = D_x(in) [and ddx = G_x(in)] for m arrays.
+int launch_outer(tdw_ctx *c, int m, const double2 *in, double2 *T, double2 *ddx, int mode);
+// Inner (y[,z]) pass: out = T + D_y(in) + D_z(in) [ddy, ddz].  T may be NULL.
+int launch_inner(tdw_ctx *c, int m, const double2 *in, const double2 *T, double2 *out,
+                 double2 *ddy, double2 *ddz, int mode);
+}  // namespace tdw
+
+#define TDW_TRY(x)                                  \
+    do {                                            \
+        int _e = (int)(x);                          \
+        if (_e != 0) return _e;                     \
+    } while (0)
+#define TDW_HIP(x)                                  \
+    do {                                            \
+        hipError_t _e = (x);                        \
+        if (_e != hipSuccess) return (int)_e;       \
+    } while (0)

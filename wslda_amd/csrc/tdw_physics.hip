@@ -1,0 +1,606 @@
+// tdw_physics.hip -- pointwise and reduction kernels of the modular path:
+// ABM predictor/corrector (tdwslda_ode_integrator.h:26-220), normalisation
+// (pca_kernels.cu:913-976), densities (pca_dens.cu:13-198 and the 1D/2D
+// twins), tau (pca_dens.cu:275-338), potentials (tdwslda_functionals.h),
+// H application with the <H> subtraction (pca_kernels.cu:233-495, 552-594,
+// 806-832) and the energy sums (tdwslda_energy_generic.h:21-110).
+#include "tdw_functional.hpp"
+#include "tdw_internal.hpp"
+#include "tdw_physics.hpp"
+
+namespace tdw {
+
+static constexpr int NT = 256;
+
+// ---------------------------------------------------------------------------
+// ABM
+// ---------------------------------------------------------------------------
+template <int SCH>
+__global__ void __launch_bounds__(NT)
+k_abm1(long tot, double dt, double2 *__restrict__ y, double2 *__restrict__ f1, double2 *__restrict__ f2,
+       double2 *__restrict__ f3, double2 *__restrict__ f4, double2 *__restrict__ f5)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= tot) return;
+    const double2 yk = y[i];
+    const double2 a = f1[i], b = f2[i], d = f3[i];
+    double2 sp, sc;
+    if (SCH == 34) {
+        sp = a * (23. / 12.) - b * (16. / 12.) + d * (5. / 12.);
+        sc = a * (19. / 24.) - b * (5. / 24.) + d * (1. / 24.);
+    } else if (SCH == 45) {
+        const double2 e = f4[i];
+        sp = a * (55. / 24.) - b * (59. / 24.) + d * (37. / 24.) - e * (9. / 24.);
+        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
+    } else {
+        const double2 e = f4[i], g = f5[i];
+        sp = a * (1901. / 720.) - b * (2774. / 720.) + d * (2616. / 720.) - e * (1274. / 720.) + g * (251. / 720.);
+        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
+    }
+    // y + (-i dt) * s
+    y[i] = yk + imul(sp, -dt);
+    double2 c = yk + imul(sc, -dt);
+    if (SCH == 34) f3[i] = c;
+    else if (SCH == 45) f4[i] = c;
+    else f5[i] = c;
+}
+
+__global__ void __launch_bounds__(NT)
+k_abm4(long tot, double dta0, const double2 *__restrict__ hy, double2 *__restrict__ y,
+       const double2 *__restrict__ fl)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= tot) return;
+    // (-i dt) * hy * a0 + flast
+    y[i] = imul(hy[i], -dta0) + fl[i];
+}
+
+int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, double2 *f4, double2 *f5)
+{
+    long tot = 2L * n * (long)c->N;
+    dim3 g((tot + NT - 1) / NT);
+    double dt = c->p.dt;
+    switch (c->p.scheme) {
+    case TDW_AB3AM4: hipLaunchKernelGGL(k_abm1<34>, g, dim3(NT), 0, c->stream, tot, dt, y, f1, f2, f3, f4, f5); break;
+    case TDW_AB4AM5: hipLaunchKernelGGL(k_abm1<45>, g, dim3(NT), 0, c->stream, tot, dt, y, f1, f2, f3, f4, f5); break;
+    case TDW_AB5AM5: hipLaunchKernelGGL(k_abm1<55>, g, dim3(NT), 0, c->stream, tot, dt, y, f1, f2, f3, f4, f5); break;
+    default: return TDW_EINVAL;
+    }
+    return (int)hipGetLastError();
+}
+
+int abm4(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl)
+{
+    long tot = 2L * n * (long)c->N;
+    double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
+    hipLaunchKernelGGL(k_abm4, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, tot, c->p.dt * a0, hy, y, fl);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// per-state reductions: norms and <psi|H psi>
+// grid (nchunk, n); block b of state a sums points [b*chunk, (b+1)*chunk).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT)
+k_norm_part(const double2 *__restrict__ wf, int n, long N, long chunk, double *__restrict__ part)
+{
+    __shared__ double red[NT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    const double2 *u = wf + (long)a * N, *v = wf + ((long)n + a) * N;
+    double s = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += NT) s += cnorm(u[i]) + cnorm(v[i]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_rowsum(const double *__restrict__ part, int n, int nchunk, double scale,
+                         double *__restrict__ out)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; k++) s += part[(long)a * nchunk + k];
+    out[a] = s * scale;
+}
+
+__global__ void __launch_bounds__(NT)
+k_scale_states(double2 *__restrict__ wf, int n, long N, const double *__restrict__ nrm)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    long tot = 2L * n * N;
+    if (i >= tot) return;
+    long a = (i / N) % n;
+    double s = 1.0 / sqrt(nrm[a]);
+    wf[i] = wf[i] * s;
+}
+
+static long pick_chunk(const tdw_ctx *c, int *nchunk)
+{
+    long N = (long)c->N;
+    long chunk = 4096;
+    if (chunk > N) chunk = N;
+    *nchunk = (int)((N + chunk - 1) / chunk);
+    return chunk;
+}
+
+int norms(tdw_ctx *c, int n, const double2 *wf, double *d_nrm)
+{
+    int nchunk;
+    long chunk = pick_chunk(c, &nchunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    hipLaunchKernelGGL(k_norm_part, dim3(nchunk, n), dim3(NT), 0, c->stream, wf, n, (long)c->N, chunk, c->d_part);
+    hipLaunchKernelGGL(k_rowsum, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk,
+                       dvol_norm(c), d_nrm);
+    return (int)hipGetLastError();
+}
+
+int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm)
+{
+    TDW_TRY(norms(c, n, wf, d_nrm));
+    long tot = 2L * n * (long)c->N;
+    hipLaunchKernelGGL(k_scale_states, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, wf, n, (long)c->N, d_nrm);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// densities: partial sums over state chunks, then a fixed-order reduction.
+// NV = 4 (rho_a, rho_b, nu) or 12 (+ tau_a, tau_b, j_a, j_b).
+// ---------------------------------------------------------------------------
+template <int DIM, bool GRAD>
+__global__ void __launch_bounds__(NT)
+k_dens_part(const double2 *__restrict__ wf, const double2 *__restrict__ ddx, const double2 *__restrict__ ddy,
+            const double2 *__restrict__ ddz, const double2 *__restrict__ lap, const double *__restrict__ fbeta,
+            const double *__restrict__ wgt, const double *__restrict__ kt, const int *__restrict__ cnt,
+            int n, long N, int chunk, int cubic, double piz, double *__restrict__ part)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
+    double na = 0, nb = 0, ta = 0, tb = 0, jax = 0, jay = 0, jaz = 0, jbx = 0, jby = 0, jbz = 0;
+    double2 nuu = make_double2(0.0, 0.0);
+    for (int a = a0; a < a1; a++) {
+        double fb = fbeta[a] * 10000.;
+        double fbm = 10000. - fb;
+        if (wgt) { fb *= wgt[a]; fbm *= wgt[a]; }
+        double w = 1.0, k2t = 0.0;
+        if (DIM == 1) {
+            double ky = kt[a], kz = kt[n + a];
+            k2t = kz * kz + ky * ky;
+            w = (double)cnt[a];
+        } else if (DIM == 2) {
+            double kz = kt[a];
+            k2t = kz * kz;
+            w = (fabs(kz) < 1.0e-12) ? 1.0 : 2.0;
+            if (cubic && fabs(kz + piz) < 1.0e-12) w = 1.0;
+        }
+        const double2 u = wf[(long)a * N + i], v = wf[((long)n + a) * N + i];
+        const double fw = fb * w, fmw = fbm * w;
+        na += cnorm(u) * fw;
+        nb += cnorm(v) * fmw;
+        double2 uv = cmul(u, make_double2(v.x, -v.y));
+        nuu = nuu + uv * ((fbm - fb) / 2.0 * w);
+        if (GRAD) {
+            const double2 ux = ddx[(long)a * N + i], vx = ddx[((long)n + a) * N + i];
+            double2 lu = lap[(long)a * N + i], lv = lap[((long)n + a) * N + i];
+            if (DIM < 3) { lu = lu - u * k2t; lv = lv - v * k2t; }
+            ta += cmulc(u, lu).x * fw;
+            tb += cmulc(v, lv).x * fmw;
+            jax += cmulc(u, ux).y * fw;
+            jbx -= cmulc(v, vx).y * fmw;
+            if (DIM >= 2) {
+                const double2 uy = ddy[(long)a * N + i], vy = ddy[((long)n + a) * N + i];
+                jay += cmulc(u, uy).y * fw;
+                jby -= cmulc(v, vy).y * fmw;
+            }
+            if (DIM == 3) {
+                const double2 uz = ddz[(long)a * N + i], vz = ddz[((long)n + a) * N + i];
+                jaz += cmulc(u, uz).y * fw;
+                jbz -= cmulc(v, vz).y * fmw;
+            }
+        }
+    }
+    const int NV = GRAD ? 12 : 4;
+    double *p = part + (long)blockIdx.y * NV * N + i;
+    p[0 * N] = na;
+    p[1 * N] = nb;
+    p[2 * N] = nuu.x;
+    p[3 * N] = nuu.y;
+    if (GRAD) {
+        p[4 * N] = ta;  p[5 * N] = tb;
+        p[6 * N] = jax; p[7 * N] = jay; p[8 * N] = jaz;
+        p[9 * N] = jbx; p[10 * N] = jby; p[11 * N] = jbz;
+    }
+}
+
+template <bool GRAD>
+__global__ void __launch_bounds__(NT)
+k_dens_final(const double *__restrict__ part, int nchunk, long N, double tr, double *__restrict__ dens)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    const int NV = GRAD ? 12 : 4;
+    double s[12];
+#pragma unroll
+    for (int v = 0; v < 12; v++) s[v] = 0.0;
+    for (int k = 0; k < nchunk; k++) {
+        const double *p = part + (long)k * NV * N + i;
+#pragma unroll
+        for (int v = 0; v < NV; v++) s[v] += p[v * N];
+    }
+    const double sc = 10000.;
+    dens[2 * i] = s[2] / sc / tr;          // nu (complex, first 2N doubles)
+    dens[2 * i + 1] = s[3] / sc / tr;
+    dens[2 * N + i] = s[0] / sc / tr;      // rho_a
+    dens[7 * N + i] = s[1] / sc / tr;      // rho_b
+    dens[3 * N + i] = GRAD ? s[4] / sc / tr : 0.0;
+    dens[8 * N + i] = GRAD ? s[5] / sc / tr : 0.0;
+    dens[4 * N + i] = GRAD ? s[6] / sc / tr : 0.0;
+    dens[5 * N + i] = GRAD ? s[7] / sc / tr : 0.0;
+    dens[6 * N + i] = GRAD ? s[8] / sc / tr : 0.0;
+    dens[9 * N + i] = GRAD ? s[9] / sc / tr : 0.0;
+    dens[10 * N + i] = GRAD ? s[10] / sc / tr : 0.0;
+    dens[11 * N + i] = GRAD ? s[11] / sc / tr : 0.0;
+}
+
+int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const double2 *ddy,
+              const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
+              const double *kt, const int *cnt, double *dens, int grad)
+{
+    const long N = (long)c->N;
+    int chunk = 128;
+    long nblk = (N + NT - 1) / NT;
+    // keep >= ~2048 blocks in flight
+    while (chunk > 8 && nblk * ((n + chunk - 1) / chunk) < 2048) chunk >>= 1;
+    int nchunk = (n + chunk - 1) / chunk;
+    const int NV = grad ? 12 : 4;
+    TDW_TRY(ensure_part(c, (size_t)nchunk * NV * N));
+    double piz = M_PI / c->p.dz;
+    int cubic = c->p.regularization == TDW_CUBIC_CUTOFF;
+    dim3 g(nblk, nchunk);
+#define TDW_DP(D, G)                                                                                   \
+    hipLaunchKernelGGL((k_dens_part<D, G>), g, dim3(NT), 0, c->stream, wf, ddx, ddy, ddz, lap, fbeta, wgt, \
+                       kt, cnt, n, N, chunk, cubic, piz, c->d_part)
+    if (c->dim == 1) { if (grad) TDW_DP(1, true); else TDW_DP(1, false); }
+    else if (c->dim == 2) { if (grad) TDW_DP(2, true); else TDW_DP(2, false); }
+    else { if (grad) TDW_DP(3, true); else TDW_DP(3, false); }
+#undef TDW_DP
+    double tr = dens_transverse(c);
+    if (grad) hipLaunchKernelGGL(k_dens_final<true>, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, tr, dens);
+    else hipLaunchKernelGGL(k_dens_final<false>, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, tr, dens);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// tau = max(0, lap(rho)/2 - tau_acc)
+// ---------------------------------------------------------------------------
+__global__ void k_real2cplx(const double *__restrict__ f, double2 *__restrict__ z, long N)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i < N) z[i] = make_double2(f[i], 0.0);
+}
+__global__ void k_cplx_re(const double2 *__restrict__ z, double *__restrict__ f, long N)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i < N) f[i] = z[i].x;
+}
+__global__ void k_tau(const double2 *__restrict__ lr, double *__restrict__ tau, long N)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    double t = 0.5 * lr[i].x - tau[i];
+    if (t < 0.0) t = 0.0;
+    tau[i] = t;
+}
+
+int laplace_real_c(tdw_ctx *c, const double *f, double2 *out_c)
+{
+    long N = (long)c->N;
+    hipLaunchKernelGGL(k_real2cplx, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, f, c->d_rtmp, N);
+    return laplace(c, 1, c->d_rtmp, out_c);
+}
+
+int laplace_real(tdw_ctx *c, const double *f, double *lap)
+{
+    long N = (long)c->N;
+    TDW_TRY(ensure_T(c, (size_t)N));
+    TDW_TRY(laplace_real_c(c, f, c->d_T));
+    hipLaunchKernelGGL(k_cplx_re, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, c->d_T, lap, N);
+    return (int)hipGetLastError();
+}
+
+int density_tau(tdw_ctx *c, double *dens)
+{
+    long N = (long)c->N;
+    TDW_TRY(ensure_T(c, (size_t)N));
+    // b component first, then a (pca_dens.cu:317-333)
+    TDW_TRY(laplace_real_c(c, dens + 7 * N, c->d_T));
+    hipLaunchKernelGGL(k_tau, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, c->d_T, dens + 8 * N, N);
+    TDW_TRY(laplace_real_c(c, dens + 2 * N, c->d_T));
+    hipLaunchKernelGGL(k_tau, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, c->d_T, dens + 3 * N, N);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// external potential of the built-in problem
+// ---------------------------------------------------------------------------
+struct VextDev {
+    tdw_vext v;
+    double s;          // switch value at this time
+    int dim, nx, ny, nz;
+    double dx, dy, dz;
+};
+
+__device__ __forceinline__ void decode(int dim, int ny, int nz, long i, int &ix, int &iy, int &iz)
+{
+    if (dim == 1) { ix = (int)i; iy = 0; iz = 0; }
+    else if (dim == 2) { ix = (int)(i / ny); iy = (int)(i - (long)ix * ny); iz = 0; }
+    else {
+        ix = (int)(i / ((long)ny * nz));
+        long r = i - (long)ix * ny * nz;
+        iy = (int)(r / nz);
+        iz = (int)(r - (long)iy * nz);
+    }
+}
+
+__device__ __forceinline__ void vext_point(const VextDev &V, long i, double &va, double &vb)
+{
+    int ix, iy, iz;
+    decode(V.dim, V.ny, V.nz, i, ix, iy, iz);
+    const double x = V.dx * (ix - V.nx / 2);
+    const double y = V.dim >= 2 ? V.dy * (iy - V.ny / 2) : 0.0;
+    const double z = V.dim >= 3 ? V.dz * (iz - V.nz / 2) : 0.0;
+    double g = 0.0;
+    for (int k = 0; k < V.v.ngauss; k++) {
+        const double X = x - V.v.cx[k], Y = y - V.v.cy[k], Z = z - V.v.cz[k];
+        g += V.v.amp[k] * V.s * exp(V.v.wx[k] * (X * X) + V.v.wy[k] * (Y * Y) + V.v.wz[k] * (Z * Z));
+    }
+    const double trap = 0.5 * V.v.omega * V.v.omega * (x * x + y * y + z * z);
+    va = g + trap;
+    vb = (V.v.flip_b ? -g : g) + trap;
+}
+
+static double switch_function(double t, double T, double alpha)
+{
+    return 0.5 * (1.0 + tanh(alpha * tan(M_PI_2 * (2.0 * t / T - 1.0))));
+}
+static double smooth_step(double t, double t0, double t1, double T, double alpha)
+{
+    if (t <= t0 || t >= t1) return 0.0;
+    if (t >= t0 + T && t <= t1 - T) return 1.0;
+    if (t > t0 && t < t0 + T) return switch_function(t - t0, T, alpha);
+    return 1.0 - switch_function(t - t1 + T, T, alpha);
+}
+
+static VextDev make_vext(const tdw_ctx *c, int it)
+{
+    VextDev V;
+    V.v = c->p.vext;
+    double t = c->p.t0 + c->p.dt * it;
+    V.s = c->p.vext.switch_on ? smooth_step(t, c->p.vext.t_start, c->p.vext.t_stop, c->p.vext.t_switch, 1.0) : 1.0;
+    V.dim = c->dim; V.nx = c->nx; V.ny = c->dim >= 2 ? c->ny : 1; V.nz = c->dim >= 3 ? c->nz : 1;
+    V.dx = c->p.dx; V.dy = c->p.dy; V.dz = c->p.dz;
+    return V;
+}
+
+__global__ void k_vext(VextDev V, long N, double *__restrict__ va, double *__restrict__ vb)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    double a, b;
+    vext_point(V, i, a, b);
+    va[i] = a;
+    vb[i] = b;
+}
+
+int vext_eval(tdw_ctx *c, int it, double *va, double *vb)
+{
+    long N = (long)c->N;
+    hipLaunchKernelGGL(k_vext, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, make_vext(c, it), N, va, vb);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// potentials
+// ---------------------------------------------------------------------------
+FParams fparams(const tdw_ctx *c)
+{
+    FParams f;
+    f.p_nmin = c->p.p_nmin; f.p_nmax = c->p.p_nmax; f.denseps = c->p.denseps;
+    f.mu_a = c->p.mu_a; f.mu_b = c->p.mu_b; f.ec = c->p.ec; f.dx = c->p.dx;
+    f.cubic = c->p.regularization == TDW_CUBIC_CUTOFF;
+    if (f.cubic) { f.R0 = 2.442749607806335 / (4.0 * M_PI * M_PI); f.R1 = 0.31786000; }
+    else { f.R0 = 1. / (2.0 * M_PI * M_PI); f.R1 = 0.5; }
+    f.mf = c->p.mf_correction;
+    f.bdg = c->p.functional == TDW_BDG;
+    f.sclgth = c->p.sclgth;
+    return f;
+}
+
+__global__ void __launch_bounds__(NT)
+k_potentials(FParams f, VextDev V, long N, const double *__restrict__ dens, double *__restrict__ pots)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    const double na = dens[2 * N + i], nb = dens[7 * N + i];
+    const double2 nu = make_double2(dens[2 * i], dens[2 * i + 1]);
+    double ua, ub;
+    vext_point(V, i, ua, ub);
+    double Va = pots[i], Vb = pots[N + i];
+    double2 d;
+    if (f.bdg) bdg_point(f, na, nb, nu, ua, ub, pots[4 * N + i], pots[5 * N + i], Va, Vb, d);
+    else slda_point(f, na, nb, nu, ua, ub, Va, Vb, d);
+    pots[i] = Va;
+    pots[N + i] = Vb;
+    pots[2 * N + 2 * i] = d.x;
+    pots[2 * N + 2 * i + 1] = d.y;
+}
+
+int potentials(tdw_ctx *c, int it, const double *dens, double *pots)
+{
+    long N = (long)c->N;
+    hipLaunchKernelGGL(k_potentials, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, fparams(c),
+                       make_vext(c, it), N, dens, pots);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// H application: out = H in ; part = per-block Re<in|H in> ; then <H> and
+// the subtraction out -= <H> in.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT)
+k_apply_h(const double2 *__restrict__ in, double2 *out, const double2 *lap, const double *__restrict__ pots,
+          const double *__restrict__ kt, int n, long N, long chunk, int dim, int bdg, double mu_a,
+          double mu_b, double *__restrict__ part)
+{
+    __shared__ double red[NT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    double k2t = 0.0;
+    if (dim == 1) k2t = kt[n + a] * kt[n + a] + kt[a] * kt[a];
+    else if (dim == 2) k2t = kt[a] * kt[a];
+    const long ou = (long)a * N, ov = ((long)n + a) * N;
+    double s = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += NT) {
+        const double aa = bdg ? -0.5 : pots[4 * N + i] * -0.5;
+        const double ab = bdg ? -0.5 : pots[5 * N + i] * -0.5;
+        const double ja = -1.0 * aa * k2t, jb = -1.0 * ab * k2t;
+        const double Va = pots[i] - mu_a + ja, Vb = pots[N + i] - mu_b + jb;
+        const double2 D = make_double2(pots[2 * N + 2 * i], pots[2 * N + 2 * i + 1]);
+        const double2 tu = in[ou + i], tv = in[ov + i];
+        const double2 lu = lap[ou + i], lv = lap[ov + i];
+        double2 u = tu * Va + cmul(tv, D) + lu * aa;
+        double2 v = tv * (-Vb) + cmulc(D, tu) - lv * ab;
+        out[ou + i] = u;
+        out[ov + i] = v;
+        s += cmulc(tu, u).x + cmulc(tv, v).x;
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(NT)
+k_sub_qpe(const double2 *__restrict__ in, double2 *__restrict__ out, int n, long N, const double *__restrict__ qpe)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    long tot = 2L * n * N;
+    if (i >= tot) return;
+    long a = (i / N) % n;
+    out[i] = out[i] - in[i] * qpe[a];
+}
+
+int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const double2 *lap,
+            const double *pots, const double *kt, const double *useqpe, double *d_qpe)
+{
+    (void)it;
+    const long N = (long)c->N;
+    int nchunk;
+    long chunk = pick_chunk(c, &nchunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    hipLaunchKernelGGL(k_apply_h, dim3(nchunk, n), dim3(NT), 0, c->stream, in, out, lap, pots, kt, n, N,
+                       chunk, c->dim, c->p.functional == TDW_BDG, c->p.mu_a, c->p.mu_b, c->d_part);
+    const double *q = useqpe;
+    if (!q) {
+        hipLaunchKernelGGL(k_rowsum, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk,
+                           dvol_norm(c), d_qpe);
+        q = d_qpe;
+    } else if (d_qpe && d_qpe != useqpe) {
+        TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    long tot = 2L * n * N;
+    hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// energy
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT)
+k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restrict__ dens,
+         const double *__restrict__ pots, double *__restrict__ part)
+{
+    __shared__ double red[NT / 64];
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    double e[TDW_NITEMS];
+#pragma unroll
+    for (int k = 0; k < TDW_NITEMS; k++) e[k] = 0.0;
+    if (i < N) {
+        const double na = dens[2 * N + i], nb = dens[7 * N + i];
+        const double tx1 = dens[4 * N + i], tx2 = dens[9 * N + i];
+        const double ty1 = dim >= 2 ? dens[5 * N + i] : 0.0, ty2 = dim >= 2 ? dens[10 * N + i] : 0.0;
+        const double tz1 = dim >= 3 ? dens[6 * N + i] : 0.0, tz2 = dim >= 3 ? dens[11 * N + i] : 0.0;
+        const double ja2 = tx1 * tx1 + ty1 * ty1 + tz1 * tz1, jb2 = tx2 * tx2 + ty2 * ty2 + tz2 * tz2;
+        const double ta = dens[3 * N + i] - p_reg(f, na) * ja2 / na;
+        const double tb = dens[8 * N + i] - p_reg(f, nb) * jb2 / nb;
+        if (!f.bdg) {
+            const double p = polar(f, na, nb);
+            e[TDW_EKIN] = 0.5 * (alpha_f(p) * ta + alpha_f(-1.0 * p) * tb) * VE;
+            e[TDW_EPOT] = funD(f, na, nb) * VE;
+        } else {
+            e[TDW_EKIN] = 0.5 * (pots[4 * N + i] * ta + pots[5 * N + i] * tb) * VE;
+        }
+        const double2 d = make_double2(pots[2 * N + 2 * i], pots[2 * N + 2 * i + 1]);
+        const double2 nu = make_double2(dens[2 * i], dens[2 * i + 1]);
+        e[TDW_EPAIR] = (d.x * nu.x + d.y * nu.y) * (-1.0) * VE;
+        e[TDW_ECURRENT] = p_reg(f, na) * ja2 / (2. * na) * VE + p_reg(f, nb) * jb2 / (2. * nb) * VE;
+        double ua, ub;
+        vext_point(V, i, ua, ub);
+        e[TDW_EPOTEXT] = (na * ua + nb * ub) * VE;
+        e[TDW_NPARTA] = na;
+        e[TDW_NPARTB] = nb;
+        if (dim >= 2) {
+            int ix, iy, iz;
+            decode(dim, V.ny, V.nz, i, ix, iy, iz);
+            const double x = (double)(ix - V.nx / 2) * V.dx, y = (double)(iy - V.ny / 2) * V.dy;
+            e[TDW_LZA] = (x * dens[5 * N + i] - y * dens[4 * N + i]) * VE;
+            e[TDW_LZB] = (x * dens[10 * N + i] - y * dens[9 * N + i]) * VE;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TDW_NITEMS; k++) {
+        double s = block_sum(e[k], red);
+        if (threadIdx.x == 0) part[(long)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+int energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e)
+{
+    const long N = (long)c->N;
+    const int nb = (int)((N + NT - 1) / NT);
+    TDW_TRY(ensure_part(c, (size_t)TDW_NITEMS * nb));
+    double VE = dvol_energy(c);
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(NT), 0, c->stream, fparams(c), make_vext(c, it), N, c->dim, VE,
+                       dens, pots, c->d_part);
+    hipLaunchKernelGGL(k_rowsum, dim3(1), dim3(64), 0, c->stream, c->d_part, TDW_NITEMS, nb, 1.0, c->d_esum);
+    TDW_HIP(hipGetLastError());
+    double e[TDW_NITEMS];
+    TDW_HIP(hipMemcpyAsync(e, c->d_esum, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    TDW_HIP(hipStreamSynchronize(c->stream));
+    e[TDW_NPARTA] *= VE;
+    e[TDW_NPARTB] *= VE;
+    for (int k = 0; k < TDW_NITEMS; k++) h_e[k] = e[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Taylor self-start contribution (pca_kernels.cu:1032-1085)
+// ---------------------------------------------------------------------------
+__global__ void k_taylor(long tot, double tc, const double2 *__restrict__ hpsi, double2 *__restrict__ upd,
+                         double2 *__restrict__ contr)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= tot) return;
+    double2 u = imul(hpsi[i], tc);
+    contr[i] = u;
+    upd[i] = upd[i] + u;
+}
+
+int taylor(tdw_ctx *c, int order, double dt, int n, const double2 *hpsi, double2 *upd, double2 *contr)
+{
+    long tot = 2L * n * (long)c->N;
+    hipLaunchKernelGGL(k_taylor, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, tot, -1.0 * dt / (double)order,
+                       hpsi, upd, contr);
+    return (int)hipGetLastError();
+}
+
+}  // namespace tdw

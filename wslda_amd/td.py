@@ -1,0 +1,352 @@
+"""Python view of libwslda_td.so (the MI355X td-W-SLDA step).
+
+The library is plain C ABI (include/wslda_td.h); this module binds it with
+ctypes and uses torch tensors only as device memory.  Names and argument
+meaning follow the reference's kernel interface (hpc-engine/pca_kernels.h,
+pca_derivative.h, pca_dens.h), so a run reads like the loop of pca.c.
+
+Nothing here falls back to the CPU: if the library or a HIP device is
+missing, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libwslda_td.so")
+
+SLDA, BDG = 111, 113
+SPHERICAL_CUTOFF, CUBIC_CUTOFF = 88, 89
+AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
+FP64, SPLIT = 0, 1
+EKIN, EPOT, EPAIR, ECURRENT, EPOTEXT, EPAIREXT, EVELEXT, NPARTA, NPARTB, LZA, LZB = range(11)
+NITEMS = 11
+MAX_GAUSS = 4
+
+
+class TDWError(RuntimeError):
+    pass
+
+
+class Vext(C.Structure):
+    _fields_ = [("ngauss", C.c_int),
+                ("amp", C.c_double * MAX_GAUSS), ("cx", C.c_double * MAX_GAUSS),
+                ("cy", C.c_double * MAX_GAUSS), ("cz", C.c_double * MAX_GAUSS),
+                ("wx", C.c_double * MAX_GAUSS), ("wy", C.c_double * MAX_GAUSS),
+                ("wz", C.c_double * MAX_GAUSS),
+                ("omega", C.c_double), ("switch_on", C.c_int),
+                ("t_start", C.c_double), ("t_stop", C.c_double), ("t_switch", C.c_double),
+                ("flip_b", C.c_int)]
+
+
+class Params(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+                ("functional", C.c_int), ("regularization", C.c_int),
+                ("mf_correction", C.c_int), ("scheme", C.c_int),
+                ("mu_a", C.c_double), ("mu_b", C.c_double), ("ec", C.c_double),
+                ("dt", C.c_double), ("t0", C.c_double), ("sclgth", C.c_double),
+                ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
+                ("vext", Vext), ("precision", C.c_int)]
+
+
+class State(C.Structure):
+    _fields_ = [("n", C.c_int),
+                ("wf", C.c_void_p), ("lap", C.c_void_p), ("ddx", C.c_void_p),
+                ("ddy", C.c_void_p), ("ddz", C.c_void_p),
+                ("f", C.c_void_p * 5),
+                ("dens", C.c_void_p), ("pots", C.c_void_p), ("qpe", C.c_void_p)]
+
+
+def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
+                regularization=SPHERICAL_CUTOFF, mf_correction=0, scheme=AB4AM5,
+                mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
+                p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, precision=FP64):
+    p = Params()
+    p.dim, p.nx, p.ny, p.nz = dim, nx, ny, nz
+    p.dx = dx
+    p.dy = dx if dy is None else dy
+    p.dz = dx if dz is None else dz
+    p.functional, p.regularization, p.mf_correction, p.scheme = functional, regularization, mf_correction, scheme
+    p.mu_a, p.mu_b, p.ec, p.dt, p.t0, p.sclgth = mu_a, mu_b, ec, dt, t0, sclgth
+    p.p_nmin, p.p_nmax, p.denseps = p_nmin, p_nmax, denseps
+    if vext is not None:
+        p.vext = vext
+    p.precision = precision
+    return p
+
+
+def gauss_vext(gaussians, omega=0.0, switch=None, flip_b=0):
+    """gaussians: list of (amp, (cx, cy, cz), (wx, wy, wz)): amp*exp(w*(x-c)^2)."""
+    v = Vext()
+    v.ngauss = len(gaussians)
+    for k, (a, cc, ww) in enumerate(gaussians):
+        v.amp[k] = a
+        v.cx[k], v.cy[k], v.cz[k] = cc
+        v.wx[k], v.wy[k], v.wz[k] = ww
+    v.omega = omega
+    if switch is not None:
+        v.switch_on = 1
+        v.t_start, v.t_stop, v.t_switch = switch
+    v.flip_b = flip_b
+    return v
+
+
+_lib = None
+
+_SIGS = None
+
+
+def _sigs():
+    P, I, D, Z = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+    return {
+        "tdw_create": (I, [C.POINTER(Params), I, I, C.POINTER(C.c_void_p)]),
+        "tdw_destroy": (I, [P]),
+        "tdw_sync": (I, [P]),
+        "tdw_npts": (Z, [P]),
+        "tdw_set_params": (I, [P, C.POINTER(Params)]),
+        "tdw_set_states": (I, [P, P, P, P]),
+        "tdw_abm_step1": (I, [P, I, P, P, P, P, P, P]),
+        "tdw_abm_step4": (I, [P, I, P, P, P]),
+        "tdw_normalize": (I, [P, I, P, P]),
+        "tdw_laplace": (I, [P, I, P, P]),
+        "tdw_derivatives": (I, [P, I, P, P, P, P, P]),
+        "tdw_laplace_real": (I, [P, P, P]),
+        "tdw_densities": (I, [P, I, P, P, P, P, P, P, P, I]),
+        "tdw_density_tau": (I, [P, P]),
+        "tdw_potentials": (I, [P, I, P, P]),
+        "tdw_apply_h": (I, [P, I, I, P, P, P, P, P, P]),
+        "tdw_energy": (I, [P, I, P, P, P]),
+        "tdw_vext_eval": (I, [P, I, P, P]),
+        "tdw_step": (I, [P, C.POINTER(State), I, I]),
+        "tdw_step_fused": (I, [P, C.POINTER(State), I, I]),
+        "tdw_initial_measure": (I, [P, C.POINTER(State)]),
+        "tdw_bind": (I, [P]),
+        "tdw_bound": (P, []),
+        # drop-in layer (reference names)
+        "set_gpu": (I, [I]),
+        "gpu_malloc": (I, [Z, C.POINTER(C.c_void_p)]),
+        "gpu_free": (I, [P]),
+        "host_malloc_pl": (I, [Z, C.POINTER(C.c_void_p)]),
+        "host_free_pl": (I, [P]),
+        "memcopy_host2gpu": (I, [P, P, Z]),
+        "memcopy_gpu2host": (I, [P, P, Z]),
+        "memcopy_gpu2gpu": (I, [P, P, Z]),
+        "memcopy_const": (I, [D, D, D, D, D, D]),
+        "memcopy_const_params": (I, [P]),
+        "memcopy_const_sclgth": (I, [D]),
+        "create_cufftPlans": (I, [I, I, C.POINTER(C.c_size_t)]),
+        "set_workspace_for_cufftPlan": (I, [P]),
+        "compute_derivatives": (I, [I, P, P, P, P, P, I]),
+        "compute_laplace": (I, [I, P, P, I]),
+        "compute_laplace_real_f": (I, [P, P, I]),
+        "calculate_densities": (I, [I, P, P, P, P, P, P, P, P, I, I]),
+        "calculate_densities_1d": (I, [I, P, P, P, P, P, P, P, P, I, I]),
+        "calculate_densities_2d": (I, [I, P, P, P, P, P, P, P, P, I, I]),
+        "density_caculate_tau": (I, [P, I]),
+        "compute_potentials": (I, [I, P, P, D, I]),
+        "compute_energy": (I, [I, P, P, P, I]),
+        "apply_hamiltonian": (I, [I, I, P, P, P, P, P, P, P, P, P, D, P, D, I]),
+        "apply_hamiltonian_1d": (I, [I, I, P, P, P, P, P, P, P, P, D, P, D, I]),
+        "apply_hamiltonian_2d": (I, [I, I, P, P, P, P, P, P, P, P, P, D, P, D, I]),
+        "normalize_wf": (I, [I, P, I]),
+        "abm_step1": (I, [I, P, P, P, P, P, P, I, I]),
+        "abm_step4": (I, [I, P, P, P, I, I]),
+        "taylor_expansion_contribution": (I, [I, D, I, P, P, P, I]),
+        "get_v_ext": (I, [I, I, I, P]),
+    }
+
+
+EXPORTS = sorted(_sigs().keys())
+
+
+def lib():
+    """Load the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise TDWError("libwslda_td.so not built: run `python -m wslda_amd.build` "
+                           "(or __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in _sigs().items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise TDWError("%s failed with code %d" % (what, rc))
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+class TDContext:
+    """One td run's library context (lattice, functional, constants)."""
+
+    def __init__(self, params, nwf, device=0):
+        import torch
+        if not torch.cuda.is_available():
+            raise TDWError("no HIP device: the td step runs only on the GPU")
+        self.torch = torch
+        self.params = params
+        self.nwf = nwf
+        self.device = device
+        h = C.c_void_p()
+        check(lib().tdw_create(C.byref(params), nwf, device, C.byref(h)), "tdw_create")
+        self.h = h
+        self.N = int(lib().tdw_npts(h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tdw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().tdw_sync(self.h), "tdw_sync")
+
+    def set_params(self, params):
+        self.params = params
+        check(lib().tdw_set_params(self.h, C.byref(params)), "tdw_set_params")
+
+    def set_states(self, fbeta, kt=None, cnt=None):
+        fb = np.ascontiguousarray(fbeta, dtype=np.float64)
+        kt_ = None if kt is None else np.ascontiguousarray(kt, dtype=np.float64)
+        cnt_ = None if cnt is None else np.ascontiguousarray(cnt, dtype=np.int32)
+        check(lib().tdw_set_states(self.h, fb.ctypes.data_as(C.c_void_p),
+                                   None if kt_ is None else kt_.ctypes.data_as(C.c_void_p),
+                                   None if cnt_ is None else cnt_.ctypes.data_as(C.c_void_p)),
+              "tdw_set_states")
+
+    # -- building blocks (device tensors) ---------------------------------
+    def _pre(self):
+        # torch queues on its own stream: order it before the library's
+        self.torch.cuda.synchronize(self.device)
+
+    def abm_step1(self, y, f):
+        self._pre()
+        fs = [_ptr(x) for x in f] + [None] * (5 - len(f))
+        check(lib().tdw_abm_step1(self.h, self.nwf_of(y), _ptr(y), *fs), "abm_step1")
+
+    def abm_step4(self, hy, y, flast):
+        self._pre()
+        check(lib().tdw_abm_step4(self.h, self.nwf_of(y), _ptr(hy), _ptr(y), _ptr(flast)), "abm_step4")
+
+    def normalize_wf(self, wf, norms=None):
+        self._pre()
+        check(lib().tdw_normalize(self.h, self.nwf_of(wf), _ptr(wf), _ptr(norms)), "normalize_wf")
+
+    def compute_laplace(self, inp, out):
+        self._pre()
+        m = inp.numel() // self.N
+        check(lib().tdw_laplace(self.h, m, _ptr(inp), _ptr(out)), "compute_laplace")
+
+    def compute_derivatives(self, inp, ddx, ddy, ddz, lap):
+        self._pre()
+        m = inp.numel() // self.N
+        check(lib().tdw_derivatives(self.h, m, _ptr(inp), _ptr(ddx), _ptr(ddy), _ptr(ddz), _ptr(lap)),
+              "compute_derivatives")
+
+    def compute_laplace_real(self, f, out):
+        self._pre()
+        check(lib().tdw_laplace_real(self.h, _ptr(f), _ptr(out)), "compute_laplace_real_f")
+
+    def calculate_densities(self, wf, ddx, ddy, ddz, lap, dens, gradients, weights=None):
+        self._pre()
+        check(lib().tdw_densities(self.h, self.nwf_of(wf), _ptr(wf), _ptr(ddx), _ptr(ddy), _ptr(ddz),
+                                  _ptr(lap), _ptr(weights), _ptr(dens), int(gradients)),
+              "calculate_densities")
+
+    def density_caculate_tau(self, dens):
+        self._pre()
+        check(lib().tdw_density_tau(self.h, _ptr(dens)), "density_caculate_tau")
+
+    def compute_potentials(self, it, dens, pots):
+        self._pre()
+        check(lib().tdw_potentials(self.h, it, _ptr(dens), _ptr(pots)), "compute_potentials")
+
+    def apply_hamiltonian(self, it, wf_in, wf_out, lap, pots, useqpe=None, qpe=None):
+        self._pre()
+        check(lib().tdw_apply_h(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(lap),
+                                _ptr(pots), _ptr(useqpe), _ptr(qpe)), "apply_hamiltonian")
+
+    def compute_energy(self, it, dens, pots):
+        self._pre()
+        e = np.zeros(NITEMS)
+        check(lib().tdw_energy(self.h, it, _ptr(dens), _ptr(pots), e.ctypes.data_as(C.c_void_p)),
+              "compute_energy")
+        return e
+
+    def vext(self, it):
+        self._pre()
+        va = self.torch.empty(self.N, dtype=self.torch.float64, device="cuda:%d" % self.device)
+        vb = self.torch.empty_like(va)
+        check(lib().tdw_vext_eval(self.h, it, _ptr(va), _ptr(vb)), "tdw_vext_eval")
+        self.sync()
+        return va, vb
+
+    def nwf_of(self, wf):
+        return wf.numel() // (2 * self.N)
+
+
+class TDRun:
+    """All device arrays of one td run, in the reference's layout, and the
+    time loop of pca.c (initial measurement, PECE steps, energies)."""
+
+    def __init__(self, ctx, wf, pots=None, fused=False):
+        torch = ctx.torch
+        self.ctx = ctx
+        dev = "cuda:%d" % ctx.device
+        N, n = ctx.N, ctx.nwf
+        self.n = n
+        self.wf = torch.as_tensor(np.ascontiguousarray(wf).reshape(2 * n, N), dtype=torch.complex128).to(dev)
+        z = lambda: torch.zeros((2 * n, N), dtype=torch.complex128, device=dev)
+        self.lap = z()
+        self.ddx = z()
+        self.ddy = z() if ctx.params.dim >= 2 else None
+        self.ddz = z() if ctx.params.dim >= 3 else None
+        self.f = [z() for _ in range(5)]
+        self.dens = torch.zeros(12 * N, dtype=torch.float64, device=dev)
+        if pots is None:
+            pots = np.zeros(12 * N)
+            pots[4 * N:6 * N] = 1.0
+        self.pots = torch.as_tensor(np.ascontiguousarray(pots, dtype=np.float64)).to(dev)
+        self.qpe = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.fused = fused
+        self.st = State()
+        self.st.n = n
+        self.st.wf, self.st.lap = self.wf.data_ptr(), self.lap.data_ptr()
+        self.st.ddx = self.ddx.data_ptr()
+        self.st.ddy = self.ddy.data_ptr() if self.ddy is not None else None
+        self.st.ddz = self.ddz.data_ptr() if self.ddz is not None else None
+        for k in range(5):
+            self.st.f[k] = self.f[k].data_ptr()
+        self.st.dens, self.st.pots, self.st.qpe = self.dens.data_ptr(), self.pots.data_ptr(), self.qpe.data_ptr()
+        torch.cuda.synchronize(ctx.device)
+
+    def history(self):
+        by = {t.data_ptr(): t for t in self.f}
+        return [by[self.st.f[k]] for k in range(5)]
+
+    def initial_measure(self):
+        check(lib().tdw_initial_measure(self.ctx.h, C.byref(self.st)), "tdw_initial_measure")
+
+    def step(self, it, grad_last=False):
+        fn = lib().tdw_step_fused if self.fused else lib().tdw_step
+        check(fn(self.ctx.h, C.byref(self.st), it, int(grad_last)), "tdw_step")
+
+    def energy(self, it):
+        return self.ctx.compute_energy(it, self.dens, self.pots)
