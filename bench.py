@@ -1,0 +1,209 @@
+"""Benchmark of the td-W-SLDA step on one MI355X.
+
+Metric: quasiparticle-wavefunction steps per second (states x ABM steps /
+wall time of the step loop, init and measurements excluded), FP64 and
+split precision.  Default workload: the plain 3D uniform SLDA problem
+(24^3 lattice, 16384 states, AB5AM5, FP64).
+
+Prints ONE JSON line (rank 0).  Extra objects:
+  roofline     -- dominant kernel: algorithmic HBM bytes per launch over its
+                  average launch time (HIP events on the library stream over
+                  the timed region), against 8 TB/s;
+  cpu_baseline -- the CPU FP64 oracle (oracle/tdw_oracle.c, OpenMP) timed on
+                  the host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(name, n, N, scheme, precision):
+    """Compulsory HBM bytes of one launch of the named kernel family, for n
+    states of N points (S = 2 components * N * 16 B in FP64)."""
+    S = 2 * N * 16
+    h = {34: 3, 45: 4, 55: 5}[scheme]
+    table = {
+        "abm_step1": (h + 1) + 2,     # read y, f_k..; write y_p, c
+        "abm_step4": 3,               # read H y_p, c; write y
+        "scale": 2,
+        "norms": 1,
+        "fft_outer": 2,               # read psi, write T
+        "fft_inner": 3,               # read psi, T; write lap
+        "fft_outer_deriv": 3,
+        "fft_inner_deriv": 5,
+        "densities": 1,
+        "apply_h": 6,                 # H psi (read psi, lap; write) + <H> subtraction (read psi, Hpsi; write)
+        # fused kernels
+        "f_pred": (h + 1) + 3,        # read y, F_k, older f; write y_p, c, f_k (+T counted below)
+        "f_pred_T": 1,
+        "f_corr": 3 + 2,              # read Hpsi, y_p, c; write y, T
+        "f_hx": 3,                    # read psi, T; write H psi
+        "f_dens": 1,
+    }
+    return table.get(name, 0) * S * n
+
+
+def run_gpu(args):
+    import numpy as np
+    import torch
+    from wslda_amd import problems, td
+
+    dev = "cuda:0"
+    prec = td.SPLIT if args.mode == "split" else td.FP64
+    t0 = time.time()
+    w = problems.config2(torch, precision=prec, n_states=args.states, device=dev)
+    p = w["params"]
+    n = w["wf"].shape[0] // 2
+    ctx = td.TDContext(p, n, 0)
+    ctx.set_states(w["fbeta"], w["kt"], w["cnt"])
+    R = td.TDRun(ctx, w["wf"], w["pots"], fused=args.fused)
+    del w["wf"]
+    R.initial_measure()
+    ctx.sync()
+    e0 = R.energy(0)
+    setup_s = time.time() - t0
+    it = 0
+    meas = args.meas_every
+
+    def run_steps(k):
+        # the corrector of every meas-th step and of the loop's last step
+        # computes gradients (tau, j), as the last step of a measurement does
+        nonlocal it
+        for q in range(k):
+            last = ((it + 1) % meas == 0) or q == k - 1
+            R.step(it, grad_last=last)
+            it += 1
+
+    run_steps(args.warmup)
+    ctx.sync()
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream_ptr())
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ctx.timing(True)
+    ctx.timing_report()   # reset
+    ev0.record(stream)
+    wall0 = time.time()
+    run_steps(args.steps)
+    R.finalize()
+    ev1.record(stream)
+    ev1.synchronize()
+    ctx.sync()
+    wall = time.time() - wall0
+    ms = ev0.elapsed_time(ev1)
+    rep = ctx.timing_report()
+    ctx.timing(False)
+    e1 = R.energy(it)
+    N = ctx.N
+    qps = n * args.steps / (ms / 1000.0)
+    # dominant kernel
+    dom = max(rep.items(), key=lambda kv: kv[1][0])
+    dname, (dtot, dcnt) = dom
+    davg = dtot / dcnt / 1000.0
+    dbytes = algorithmic_bytes(dname, n, N, p.scheme, prec)
+    achieved = dbytes / davg / 1e9 if dbytes else None
+    step_bytes = {34: 11, 45: 12, 55: 13}[p.scheme] * 2 * N * 16 * n
+    out = dict(qps=qps, ms=ms, wall=wall, n=n, N=N, rep=rep, dname=dname, davg=davg, dbytes=dbytes,
+               achieved=achieved, step_bytes=step_bytes, setup_s=setup_s, scheme=p.scheme,
+               e0=e0.tolist(), e1=e1.tolist(), name=w.get("name"), params=p, uniform=w["uniform"],
+               pots=w["pots"], fbeta=w["fbeta"], R=R, ctx=ctx)
+    return out
+
+
+def cpu_baseline(g, args):
+    """Oracle on the host: a sample of the same workload (the first states of
+    the GPU problem), a few PECE steps."""
+    import ctypes as C
+    import numpy as np
+    from oracle import pyoracle as po
+    threads = min(os.cpu_count() or 1, 16)
+    po.lib().tdwo_set_threads(threads)
+    p = g["params"]
+    cfg = po.Cfg()
+    C.memmove(C.addressof(cfg), C.addressof(p), C.sizeof(cfg))
+    n, N = g["n"], g["N"]
+    ns = args.cpu_states
+    # the initial (noisy, normalised) state of the GPU run is not kept; use the
+    # current GPU state: same lattice, same functional, same work per state
+    wf = g["R"].wf
+    sample = np.concatenate([wf[:ns].cpu().numpy(), wf[n:n + ns].cpu().numpy()])
+    P = po.Problem(cfg, sample, g["fbeta"][:ns], pots=g["R"].pots.cpu().numpy())
+    P.initial_measure()
+    P.step(0, grad_last=0)   # warm
+    t = time.time()
+    k = 0
+    while True:
+        P.step(1 + k, grad_last=0)
+        k += 1
+        if time.time() - t > args.cpu_seconds or k >= args.cpu_steps:
+            break
+    dt = time.time() - t
+    return dict(value=ns * k / dt, unit="qp-steps/s", cores=threads, kind="port",
+                sample="%s lattice, %d of %d states, %d PECE steps after 1 warm-up, %d OpenMP threads, %.1f s"
+                       % ("x".join(str(x) for x in (cfg.nx, cfg.ny, cfg.nz)), ns, n, k, threads, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--states", type=int, default=16384)
+    ap.add_argument("--mode", choices=["fp64", "split"], default="fp64")
+    ap.add_argument("--fused", type=int, default=1)
+    ap.add_argument("--meas-every", type=int, default=100,
+                    help="steps between measurements (the corrector of the last step of each computes gradients)")
+    ap.add_argument("--cpu-states", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
+                         "GPUs is a design, not code); run with --gpus 1\n")
+        sys.exit(2)
+    g = run_gpu(args)
+    n, N = g["n"], g["N"]
+    res = {
+        "metric": "qp-wavefunction steps/s, FP64 vs split-prec at rel density err <=1e-6",
+        "value": g["qps"],
+        "unit": "qp-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.mode == "fp64" else "f32-operator/f64-state",
+        "data": "synthetic",
+        "config": {"workload": "td-SLDA 3D uniform, %dx%dx%d lattice, %d qp states, AB5AM5, plane-wave BdG "
+                               "+ seeded 1e-6 noise, gradients every %d steps" % (24, 24, 24, n, args.meas_every),
+                   "lattice": [24, 24, 24], "states": n, "scheme": "AB5AM5", "precision": args.mode,
+                   "fused": bool(args.fused)},
+        "roofline": {"bound": "hbm", "kernel": g["dname"],
+                     "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (g["achieved"] / HBM_PEAK_GBS) if g["achieved"] else None,
+                     "traffic": None,
+                     "avg_launch_ms": g["davg"] * 1e3, "algorithmic_bytes_per_launch": g["dbytes"],
+                     "step_level_frac": g["step_bytes"] / (g["ms"] / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(g["rep"].items(), key=lambda kv: -kv[1][0])},
+        "energy_drift": abs(sum(g["e1"][:7]) - sum(g["e0"][:7])) / abs(sum(g["e0"][:7])),
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(g, args)
+    g["ctx"].close()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -150,11 +150,50 @@ typedef struct {
     double *qpe;          /* n: <H> of the last H application */
 } tdw_state;
 int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
-/* Same step with the fused kernels (fewer HBM passes); same results to
- * rounding.  Histories keep the reference meaning. */
+/* Same step with the fused kernels (six passes over the state arrays instead
+ * of the reference's ~57 touches); same results to rounding.  Between fused
+ * steps the stored wf is unnormalised and f[0] still holds H psi: the
+ * normalisation and <H> subtraction are applied by the next fused step, or by
+ * tdw_finalize(), which every other entry point calls first.  Densities,
+ * potentials and qpe are exact after every step.  Lattices the fused kernels
+ * are not built for run the modular step (tdw_fused_supported() says which). */
 int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last);
+int tdw_finalize(tdw_ctx *c, tdw_state *s);
+int tdw_fused_supported(tdw_ctx *c);
 /* pca.c:939-975: normalise, derivatives, densities, tau (potentials kept). */
 int tdw_initial_measure(tdw_ctx *c, tdw_state *s);
+
+/* Uniform SLDA solution and plane-wave BdG states (host arrays).
+ * tdw_fbeta: fbeta (pca_uniform.h:70-78).
+ * tdw_uniform_solve: solve_uniform_problem (pca_uniform.h:86-502); mu_rule 0
+ *   is the chemical-potential update of pca_uniform.h:427-428, mu_rule 1 the
+ *   one kept commented at :425-426 (the rule of the code that produced the
+ *   testsuite's td-gauss goldens).  Returns 1 if maxiter was reached.
+ * tdw_uniform_states: create_uniform_wf (pca_uniform.h:636-850) and its 2D /
+ *   1D twins (:2044-2287, :2321-2560): states [idxfrom, idxto) in the
+ *   reference's order, energies E_n, transverse momenta and degeneracies. */
+typedef struct {
+    double n0_a, n0_b, mu_a, mu_b, V_a, V_b, alph_a, alph_b, kc, ec;
+    double nu, delta, tau_a, tau_b, beta, S, ekin, epot, epair;
+    int nwf, iters;
+} tdw_uniform;
+double tdw_fbeta(double E, double beta);
+int tdw_uniform_solve(const tdw_params *p, double n0_a, double n0_b, double kc, double Tstart,
+                      double Tstop, double dT, double eps, double scmix, double muchange,
+                      int maxiter, int mu_rule, tdw_uniform *u);
+int tdw_uniform_count(const tdw_params *p, const tdw_uniform *u);
+int tdw_uniform_states(const tdw_params *p, const tdw_uniform *u, int idxfrom, int idxto,
+                       tdw_complex *h_wf, double *h_En, double *h_kt, int *h_cnt);
+
+/* The context's HIP stream (hipStream_t), for callers that order their own
+ * work or record events around the library's. */
+void *tdw_stream(tdw_ctx *c);
+/* Per-kernel timing: while enabled, every kernel launch is bracketed by HIP
+ * events on the context stream.  tdw_timing_report() synchronises, writes at
+ * most max entries (name: 32 chars each, total ms, launch count) and resets
+ * the counters; returns the number of entries. */
+int tdw_timing(tdw_ctx *c, int enable);
+int tdw_timing_report(tdw_ctx *c, int max, char *names, double *total_ms, int *count);
 
 /* ---------------------------------------------------------------------------
  * Drop-in layer: the reference's entry points, acting on the bound context.

@@ -15,7 +15,7 @@ def declared_functions():
     for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-        for m in re.finditer(r"^\s*(?:int|size_t|void|tdw_ctx)\s*\**\s*([a-zA-Z_]\w*)\s*\(", src, re.M):
+        for m in re.finditer(r"^\s*(?:int|size_t|void|double|tdw_ctx)\s*\**\s*([a-zA-Z_]\w*)\s*\(", src, re.M):
             names.add(m.group(1))
     return names
 

@@ -12,6 +12,37 @@ using namespace tdw;
 
 namespace tdw {
 
+static hipEvent_t ev_get(tdw_ctx *c)
+{
+    if (!c->evpool) c->evpool = new std::vector<hipEvent_t>();
+    if (!c->evpool->empty()) {
+        hipEvent_t e = c->evpool->back();
+        c->evpool->pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    hipEventCreate(&e);
+    return e;
+}
+
+int timer_begin(tdw_ctx *c, const char *name)
+{
+    if (!c->timers) c->timers = new std::vector<tdw_timer_rec>();
+    tdw_timer_rec r;
+    r.name = name;
+    r.a = ev_get(c);
+    r.b = ev_get(c);
+    hipEventRecord(r.a, c->stream);
+    c->timers->push_back(r);
+    return (int)c->timers->size() - 1;
+}
+
+void timer_end(tdw_ctx *c, int idx)
+{
+    if (!c->timers || idx < 0 || idx >= (int)c->timers->size()) return;
+    hipEventRecord((*c->timers)[idx].b, c->stream);
+}
+
 int ensure_part(tdw_ctx *c, size_t nd)
 {
     if (nd <= c->part_cap) return 0;
@@ -183,10 +214,60 @@ extern "C" int tdw_destroy(tdw_ctx *c)
     for (void *b : bufs)
         if (b) hipFree(b);
     fused_release(c);
+    if (c->timers) {
+        for (auto &r : *c->timers) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
+        delete c->timers;
+    }
+    if (c->evpool) {
+        for (auto e : *c->evpool) hipEventDestroy(e);
+        delete c->evpool;
+    }
     if (c->stream) hipStreamDestroy(c->stream);
     if (tdw_bound() == c) tdw_bind(nullptr);
     free(c);
     return 0;
+}
+
+extern "C" void *tdw_stream(tdw_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int tdw_timing(tdw_ctx *c, int enable)
+{
+    if (!c) return TDW_EINVAL;
+    c->timing = enable ? 1 : 0;
+    return 0;
+}
+
+extern "C" int tdw_timing_report(tdw_ctx *c, int max, char *names, double *total_ms, int *count)
+{
+    if (!c) return TDW_EINVAL;
+    TDW_HIP(hipStreamSynchronize(c->stream));
+    int nout = 0;
+    if (c->timers) {
+        for (auto &r : *c->timers) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) {
+                (void)hipGetLastError();   // do not leave the query's error behind
+                ms = 0.f;
+            }
+            int k;
+            for (k = 0; k < nout; k++)
+                if (strncmp(names + 32 * k, r.name, 31) == 0) break;
+            if (k == nout) {
+                if (nout >= max) continue;
+                strncpy(names + 32 * k, r.name, 31);
+                names[32 * k + 31] = 0;
+                total_ms[k] = 0.0;
+                count[k] = 0;
+                nout++;
+            }
+            total_ms[k] += ms;
+            count[k] += 1;
+            c->evpool->push_back(r.a);
+            c->evpool->push_back(r.b);
+        }
+        c->timers->clear();
+    }
+    return nout;
 }
 
 extern "C" int tdw_sync(tdw_ctx *c)
@@ -310,6 +391,7 @@ static int nhist(const tdw_ctx *c)
 extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 {
     if (!c || !s || s->n > c->nwf) return TDW_EINVAL;
+    TDW_TRY(fused_finalize(c, s));
     const int n = s->n, k = nhist(c);
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
     // predictor
@@ -333,6 +415,7 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 extern "C" int tdw_initial_measure(tdw_ctx *c, tdw_state *s)
 {
     if (!c || !s) return TDW_EINVAL;
+    c->pending = 0;
     TDW_TRY(normalize(c, s->n, D2(s->wf), c->d_nrm));
     return half_densities(c, s, 1);
 }
@@ -342,6 +425,14 @@ extern "C" int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     if (!c || !s || s->n > c->nwf) return TDW_EINVAL;
     return fused_step(c, s, it, grad_last);
 }
+
+extern "C" int tdw_finalize(tdw_ctx *c, tdw_state *s)
+{
+    if (!c || !s) return TDW_EINVAL;
+    return fused_finalize(c, s);
+}
+
+extern "C" int tdw_fused_supported(tdw_ctx *c) { return c ? (int)fused_supported(c) : 0; }
 
 // ---------------------------------------------------------------------------
 // drop-in layer

@@ -184,6 +184,7 @@ int launch_outer(tdw_ctx *c, int m, const double2 *in, double2 *T, double2 *ddx,
     long nblk = (Ptot + TP - 1) / TP;
     size_t lds = sizeof(double2) * ((size_t)L + (size_t)(mode ? 3 : 2) * TP * L);
     if (lds > (size_t)c->max_lds) return TDW_ESIZE;
+    TDW_TIMED(c, mode ? "fft_outer_deriv" : "fft_outer");
     if (mode == 0)
         hipLaunchKernelGGL(k_outer<0>, dim3(nblk), dim3(NTHR), lds, c->stream, in, T, ddx, m,
                            (long)c->N, S, TP, c->ax[0], c->d_tw[0], c->Lphys[0]);
@@ -202,6 +203,7 @@ static int launch_inner_k(tdw_ctx *c, long nblk_in, int TB, const double2 *in, c
     size_t lds = sizeof(double2) * ((size_t)c->ny + (NAX == 2 ? c->nz : 0) + (size_t)(MODE ? 3 : 2) * E);
     if (lds > (size_t)c->max_lds) return TDW_ESIZE;
     long grid = (nblk_in + TB - 1) / TB;
+    TDW_TIMED(c, MODE ? "fft_inner_deriv" : "fft_inner");
     int nz = NAX == 2 ? c->nz : 1;
     const double2 *twz = NAX == 2 ? c->d_tw[2] : c->d_tw[1];
 #define TDW_LI(KMV)                                                                                    \

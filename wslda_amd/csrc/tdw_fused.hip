@@ -1,14 +1,546 @@
-// tdw_fused.hip -- fused PECE step.
+// tdw_fused.hip -- fused PECE step for 3D lattices (kernel families a + b).
+//
+// The reference step (pca.c:1401-1570) touches the state arrays ~57 times per
+// step.  Here one step is six passes:
+//   f_pred  (x-pencils, one lane per pencil, FFT in registers): ABM predictor
+//           (tdwslda_ode_integrator.h:26-135) + D_x of the prediction + norm
+//           partials; also applies the pending normalisation of y and the
+//           pending <H> subtraction of the newest history entry.
+//   dens    densities of the prediction, normalisation folded into the
+//           occupation weights (f*s^2).
+//   f_hyz   (yz-planes, Stockham in LDS): D_y + D_z + T, then H psi
+//           (pca_kernels.cu:233-391) and the <psi|H psi> partials.
+//   f_corr  (x-pencils): AM corrector y = c - i dt a0 (H psi - <H> psi)
+//           (:138-220) + D_x + norm partials.
+//   dens, f_hyz again: H of the corrected state into the newest history slot.
+// Normalisation (pca_kernels.cu:955-976) and the <H> subtraction
+// (pca_kernels.cu:806-832) are carried as per-state scalars (s = 1/sqrt(norm),
+// e = <H>) into the next pass instead of extra passes over the state: the
+// stored y stays unnormalised and the newest history entry keeps H psi until
+// the next predictor (or tdw_finalize) applies them.
+#include "tdw_fft.hpp"
 #include "tdw_fused.hpp"
 #include "tdw_physics.hpp"
+#include "tdw_regfft.hpp"
 
 namespace tdw {
 
-int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+static constexpr int FT = 256;
+
+// ---------------------------------------------------------------------------
+// ABM coefficient sums (tdwslda_ode_integrator.h:26-97)
+// ---------------------------------------------------------------------------
+template <int SCH>
+__device__ __forceinline__ void abm_sums(double2 a, double2 b, double2 d, double2 e, double2 g, double2 &sp,
+                                         double2 &sc)
 {
-    return tdw_step(c, s, it, grad_last);
+    if constexpr (SCH == 34) {
+        sp = a * (23. / 12.) - b * (16. / 12.) + d * (5. / 12.);
+        sc = a * (19. / 24.) - b * (5. / 24.) + d * (1. / 24.);
+    } else if constexpr (SCH == 45) {
+        sp = a * (55. / 24.) - b * (59. / 24.) + d * (37. / 24.) - e * (9. / 24.);
+        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
+    } else {
+        sp = a * (1901. / 720.) - b * (2774. / 720.) + d * (2616. / 720.) - e * (1274. / 720.) + g * (251. / 720.);
+        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
+    }
 }
 
-void fused_release(tdw_ctx *c) { (void)c; }
+template <int LX>
+__device__ __forceinline__ void dxx_inplace(double2 (&r)[LX], double kscale)
+{
+    // D_x = F^-1 (-k^2/LX) F ; kscale = (2 pi / Lx)^2 / LX
+    rf::fft<LX, -1, double2>(r);
+#pragma unroll
+    for (int i = 0; i < LX; i++) {
+        const int m = (i < LX / 2) ? i : i - LX;
+        r[i] = r[i] * (-kscale * (double)(m * m));
+    }
+    rf::fft<LX, +1, double2>(r);
+}
+
+// wave: S % 64 == 0, so every wave lies inside one component and Ptot % 64 == 0
+__device__ __forceinline__ void put_norm_part(double nrm, bool wave, long P, long Ptot, double *part)
+{
+    if (wave) {
+        nrm = wave_sum(nrm);
+        if ((threadIdx.x & 63) == 0 && P < Ptot) part[P >> 6] = nrm;
+    } else if (P < Ptot) {
+        part[P] = nrm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// predictor x-pass
+// ---------------------------------------------------------------------------
+template <int LX, int SCH>
+__global__ void __launch_bounds__(FT)
+f_pred(double2 *__restrict__ y, double2 *f1, const double2 *f2, const double2 *f3, const double2 *f4,
+       const double2 *f5, double2 *fold, double2 *__restrict__ T, const double *__restrict__ sk,
+       const double *__restrict__ ek, int pending, int n, long N, int S, double dt, double kscale, int wave,
+       double *__restrict__ part)
+{
+    const long P = (long)blockIdx.x * FT + threadIdx.x;
+    const long Ptot = 2L * n * S;
+    const bool act = P < Ptot;
+    double nrm = 0.0;
+    if (act) {
+        const long a2 = P / S, j = P - a2 * S;
+        const int a = (int)(a2 < n ? a2 : a2 - n);
+        const double s = pending ? sk[a] : 1.0;
+        const double e = pending ? ek[a] : 0.0;
+        const long base = a2 * N + j;
+        double2 r[LX];
+#pragma unroll
+        for (int i = 0; i < LX; i++) {
+            const long idx = base + (long)i * S;
+            const double2 yk = y[idx] * s;
+            double2 F1 = f1[idx];
+            if (pending) {
+                F1 = F1 - yk * e;
+                f1[idx] = F1;
+            }
+            const double2 F2 = f2[idx], F3 = f3[idx];
+            const double2 F4 = (SCH >= 45) ? f4[idx] : make_double2(0.0, 0.0);
+            const double2 F5 = (SCH >= 55) ? f5[idx] : make_double2(0.0, 0.0);
+            double2 sp, sc;
+            abm_sums<SCH>(F1, F2, F3, F4, F5, sp, sc);
+            const double2 yp = yk + imul(sp, -dt);
+            y[idx] = yp;
+            fold[idx] = yk + imul(sc, -dt);
+            r[i] = yp;
+            nrm += cnorm(yp);
+        }
+        dxx_inplace<LX>(r, kscale);
+#pragma unroll
+        for (int i = 0; i < LX; i++) T[base + (long)i * S] = r[i];
+    }
+    put_norm_part(nrm, wave != 0, P, Ptot, part);
+}
+
+// ---------------------------------------------------------------------------
+// corrector x-pass
+// ---------------------------------------------------------------------------
+template <int LX>
+__global__ void __launch_bounds__(FT)
+f_corr(double2 *__restrict__ y, const double2 *__restrict__ hin, const double2 *__restrict__ cin,
+       double2 *__restrict__ T, const double *__restrict__ sp_, const double *__restrict__ ep_, int n, long N,
+       int S, double dta0, double kscale, int wave, double *__restrict__ part)
+{
+    const long P = (long)blockIdx.x * FT + threadIdx.x;
+    const long Ptot = 2L * n * S;
+    const bool act = P < Ptot;
+    double nrm = 0.0;
+    if (act) {
+        const long a2 = P / S, j = P - a2 * S;
+        const int a = (int)(a2 < n ? a2 : a2 - n);
+        const double s = sp_[a], e = ep_[a];
+        const long base = a2 * N + j;
+        double2 r[LX];
+#pragma unroll
+        for (int i = 0; i < LX; i++) {
+            const long idx = base + (long)i * S;
+            const double2 psi = y[idx] * s;
+            const double2 fp = hin[idx] - psi * e;          // (H - <H>) psi_p
+            const double2 yn = imul(fp, -dta0) + cin[idx];  // (-i dt a0) fp + c
+            y[idx] = yn;
+            r[i] = yn;
+            nrm += cnorm(yn);
+        }
+        dxx_inplace<LX>(r, kscale);
+#pragma unroll
+        for (int i = 0; i < LX; i++) T[base + (long)i * S] = r[i];
+    }
+    put_norm_part(nrm, wave != 0, P, Ptot, part);
+}
+
+// s = 1/sqrt(norm * dV), s2 = s^2 per state; part holds `per` entries per component
+__global__ void f_norm_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ s,
+                             double *__restrict__ s2)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double su = 0.0, sv = 0.0;
+    const double *pu = part + (long)a * per, *pv = part + ((long)n + a) * per;
+    for (int k = 0; k < per; k++) su += pu[k];
+    for (int k = 0; k < per; k++) sv += pv[k];
+    const double nrm = (su + sv) * dv;
+    const double sc = 1.0 / sqrt(nrm);
+    s[a] = sc;
+    s2[a] = sc * sc;
+}
+
+// ---------------------------------------------------------------------------
+// yz-plane pass: lap = s (T + D_y y + D_z y); H psi; <psi|H psi> partials
+// ---------------------------------------------------------------------------
+template <int L, int R, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN>
+__device__ __forceinline__ void cstage(const double2 *src, double2 *dst, const double2 *tw, int tid)
+{
+    constexpr int nb = L / R;
+    constexpr int tstep = L / (Ns * R);
+    for (int b = tid; b < P * nb; b += FT) {
+        const int q = b / nb, j = b - q * nb;
+        const int k = j % Ns;
+        const int off = (q / QD) * QH + (q % QD) * QL;
+        double2 v[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) v[t] = src[off + (j + t * nb) * IS];
+#pragma unroll
+        for (int t = 1; t < R; t++) {
+            const double2 w = tw[t * k * tstep];
+            v[t] = cmul(v[t], SIGN > 0 ? make_double2(w.x, -w.y) : w);
+        }
+        rf::small_dft<R, SIGN, double2>::run(v);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int t = 0; t < R; t++) dst[off + (base + t * Ns) * IS] = v[t];
+    }
+}
+
+// Compile-time Stockham plan: radices 8, 4, 3, 2 (largest first).
+template <int L> struct splan {
+    static constexpr int R0 = (L % 8 == 0 && L >= 8) ? 8 : (L % 4 == 0) ? 4 : (L % 3 == 0) ? 3 : 2;
+};
+
+template <int L, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN>
+__device__ __forceinline__ double2 *cfft(double2 *a, double2 *b, const double2 *tw, int tid)
+{
+    if constexpr (Ns >= L) {
+        return a;
+    } else {
+        constexpr int rem = L / Ns;
+        constexpr int R = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 3 == 0) ? 3 : 2;
+        cstage<L, R, Ns, P, QD, QH, QL, IS, SIGN>(a, b, tw, tid);
+        __syncthreads();
+        return cfft<L, Ns * R, P, QD, QH, QL, IS, SIGN>(b, a, tw, tid);
+    }
+}
+
+struct HParams {
+    double mu_a, mu_b;
+    int bdg;
+};
+
+template <int LY, int LZ>
+__global__ void __launch_bounds__(FT)
+f_hyz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__restrict__ hout,
+      const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
+      const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double kys, double kzs, HParams hp,
+      double *__restrict__ part)
+{
+    constexpr int E = LY * LZ;              // points of one plane
+    constexpr int KM = (E + FT - 1) / FT;   // owned points per thread
+    __shared__ double2 twy[LY], twz[LZ];
+    __shared__ double red[FT / 64];
+    extern __shared__ double2 lds[];
+    double2 *A = lds, *B = lds + 2 * E;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < LY; k += FT) twy[k] = twy_g[k];
+    for (int k = tid; k < LZ; k += FT) twz[k] = twz_g[k];
+    const int a = blockIdx.x / NX;
+    const int x = blockIdx.x - a * NX;
+    const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
+    double2 Xu[KM], Xv[KM], Ou[KM], Ov[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * FT;
+        if (e < E) {
+            Xu[k] = y[gu + e];
+            Xv[k] = y[gv + e];
+            Ou[k] = T[gu + e];
+            Ov[k] = T[gv + e];
+        }
+    }
+    // z axis: pencils (c, iy), offset q*LZ, stride 1; y axis: pencils (c, iz), offset (q/LZ)*E + q%LZ, stride LZ
+#pragma unroll
+    for (int ax = 0; ax < 2; ax++) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * FT;
+            if (e < E) {
+                A[e] = Xu[k];
+                A[E + e] = Xv[k];
+            }
+        }
+        __syncthreads();
+        double2 *F;
+        if (ax == 0) F = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, -1>(A, B, twz, tid);
+        else F = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, -1>(A, B, twy, tid);
+        double2 *G = (F == A) ? B : A;
+        for (int e2 = tid; e2 < 2 * E; e2 += FT) {
+            const int e = e2 < E ? e2 : e2 - E;
+            int m;
+            double ks;
+            if (ax == 0) { const int i = e % LZ; m = (i < LZ / 2) ? i : i - LZ; ks = kzs; }
+            else { const int i = e / LZ; m = (i < LY / 2) ? i : i - LY; ks = kys; }
+            F[e2] = F[e2] * (-ks * (double)(m * m));
+        }
+        __syncthreads();
+        double2 *Rr;
+        if (ax == 0) Rr = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, +1>(F, G, twz, tid);
+        else Rr = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, +1>(F, G, twy, tid);
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * FT;
+            if (e < E) {
+                Ou[k] = Ou[k] + Rr[e];
+                Ov[k] = Ov[k] + Rr[E + e];
+            }
+        }
+    }
+    // H psi on owned points
+    const double s = sc[a];
+    double ssum = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * FT;
+        if (e < E) {
+            const long gi = (long)x * E + e;
+            const double2 pu = Xu[k] * s, pv = Xv[k] * s;
+            const double2 lu = Ou[k] * s, lv = Ov[k] * s;
+            const double aa = hp.bdg ? -0.5 : pots[4 * N + gi] * -0.5;
+            const double ab = hp.bdg ? -0.5 : pots[5 * N + gi] * -0.5;
+            const double Va = pots[gi] - hp.mu_a, Vb = pots[N + gi] - hp.mu_b;
+            const double2 D = make_double2(pots[2 * N + 2 * gi], pots[2 * N + 2 * gi + 1]);
+            const double2 hu = pu * Va + cmul(pv, D) + lu * aa;
+            const double2 hv = pv * (-Vb) + cmulc(D, pu) - lv * ab;
+            hout[gu + e] = hu;
+            hout[gv + e] = hv;
+            ssum += cmulc(pu, hu).x + cmulc(pv, hv).x;
+        }
+    }
+    ssum = block_sum(ssum, red);
+    if (tid == 0) part[blockIdx.x] = ssum;
+}
+
+__global__ void f_e_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ e,
+                          double *__restrict__ e2)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < per; k++) s += part[(long)a * per + k];
+    e[a] = s * dv;
+    if (e2) e2[a] = s * dv;
+}
+
+// y *= s ; f0 -= e * y   (applies the pending normalisation and <H> subtraction)
+__global__ void __launch_bounds__(FT)
+f_fixup(double2 *__restrict__ y, double2 *__restrict__ f0, const double *__restrict__ s,
+        const double *__restrict__ e, int n, long N, int do_f)
+{
+    const long i = (long)blockIdx.x * FT + threadIdx.x;
+    if (i >= 2L * n * N) return;
+    const long a = (i / N) % n;
+    const double2 v = y[i] * s[a];
+    y[i] = v;
+    if (do_f) f0[i] = f0[i] - v * e[a];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int ensure_f(tdw_ctx *c, size_t nd)
+{
+    if (nd <= c->fpart_cap) return 0;
+    TDW_HIP(hipStreamSynchronize(c->stream));
+    if (c->d_fpart) TDW_HIP(hipFree(c->d_fpart));
+    c->d_fpart = nullptr;
+    c->fpart_cap = 0;
+    TDW_HIP(hipMalloc(&c->d_fpart, nd * sizeof(double)));
+    c->fpart_cap = nd;
+    return 0;
+}
+
+static int ensure_state_vecs(tdw_ctx *c)
+{
+    if (c->d_sp) return 0;
+    TDW_HIP(hipMalloc(&c->d_sp, sizeof(double) * c->nwf));
+    TDW_HIP(hipMalloc(&c->d_s2, sizeof(double) * c->nwf));
+    TDW_HIP(hipMalloc(&c->d_ep, sizeof(double) * c->nwf));
+    return 0;
+}
+
+void fused_release(tdw_ctx *c)
+{
+    if (c->d_fpart) hipFree(c->d_fpart);
+    if (c->d_sp) hipFree(c->d_sp);
+    if (c->d_s2) hipFree(c->d_s2);
+    if (c->d_ep) hipFree(c->d_ep);
+    c->d_fpart = nullptr;
+    c->d_sp = c->d_s2 = c->d_ep = nullptr;
+}
+
+typedef void (*pred_fn)(double2 *, double2 *, const double2 *, const double2 *, const double2 *, const double2 *,
+                        double2 *, double2 *, const double *, const double *, int, int, long, int, double, double,
+                        int, double *);
+typedef void (*corr_fn)(double2 *, const double2 *, const double2 *, double2 *, const double *, const double *, int,
+                        long, int, double, double, int, double *);
+typedef void (*hyz_fn)(const double2 *, const double2 *, double2 *, const double *, const double *, int, long, int,
+                       const double2 *, const double2 *, double, double, HParams, double *);
+
+template <int LX> static pred_fn pick_pred(int sch)
+{
+    if (sch == TDW_AB3AM4) return f_pred<LX, 34>;
+    if (sch == TDW_AB4AM5) return f_pred<LX, 45>;
+    return f_pred<LX, 55>;
+}
+
+static bool xpass_fns(int nx, int sch, pred_fn *p, corr_fn *cf)
+{
+    switch (nx) {
+    case 8: *p = pick_pred<8>(sch); *cf = f_corr<8>; return true;
+    case 12: *p = pick_pred<12>(sch); *cf = f_corr<12>; return true;
+    case 16: *p = pick_pred<16>(sch); *cf = f_corr<16>; return true;
+    case 24: *p = pick_pred<24>(sch); *cf = f_corr<24>; return true;
+    case 32: *p = pick_pred<32>(sch); *cf = f_corr<32>; return true;
+    default: return false;
+    }
+}
+
+static bool hyz_fn_for(int ny, int nz, hyz_fn *h)
+{
+    if (ny == 12 && nz == 12) { *h = f_hyz<12, 12>; return true; }
+    if (ny == 24 && nz == 24) { *h = f_hyz<24, 24>; return true; }
+    if (ny == 32 && nz == 32) { *h = f_hyz<32, 32>; return true; }
+    if (ny == 16 && nz == 16) { *h = f_hyz<16, 16>; return true; }
+    if (ny == 8 && nz == 8) { *h = f_hyz<8, 8>; return true; }
+    if (ny == 8 && nz == 12) { *h = f_hyz<8, 12>; return true; }
+    if (ny == 12 && nz == 6) { *h = f_hyz<12, 6>; return true; }
+    return false;
+}
+
+bool fused_supported(const tdw_ctx *c)
+{
+    if (c->dim != 3 || c->p.precision != TDW_FP64) return false;
+    pred_fn p;
+    corr_fn cf;
+    hyz_fn h;
+    if (!xpass_fns(c->nx, c->p.scheme, &p, &cf)) return false;
+    if (!hyz_fn_for(c->ny, c->nz, &h)) return false;
+    size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+    return lds <= (size_t)c->max_lds;
+}
+
+static int nhist(const tdw_ctx *c) { return c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5; }
+
+int fused_finalize(tdw_ctx *c, tdw_state *s)
+{
+    if (!c->pending) return 0;
+    const long tot = 2L * s->n * (long)c->N;
+    TDW_TIMED(c, "f_fixup");
+    hipLaunchKernelGGL(f_fixup, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, (double2 *)s->wf,
+                       (double2 *)s->f[0], c->d_scale, c->d_qpe, s->n, (long)c->N, 1);
+    c->pending = 0;
+    return (int)hipGetLastError();
+}
+
+static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave)
+{
+    const int per = wave ? c->S / 64 : c->S;
+    TDW_TIMED(c, "f_norm");
+    hipLaunchKernelGGL(f_norm_final, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_fpart, n, per,
+                       dvol_norm(c), s_out, s2_out);
+    return (int)hipGetLastError();
+}
+
+static int hyz(tdw_ctx *c, int n, const double2 *y, const double2 *T, double2 *out, const double *pots,
+               const double *scale, double *e_out, double *e_out2)
+{
+    hyz_fn h;
+    hyz_fn_for(c->ny, c->nz, &h);
+    const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+    HParams hp{c->p.mu_a, c->p.mu_b, c->p.functional == TDW_BDG};
+    const double kys = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
+    const double kzs = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
+    // the norm partials are consumed already: reuse the partial buffer
+    {
+        TDW_TIMED(c, "f_hyz");
+        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(FT), lds, c->stream, y, T, out, pots, scale, n,
+                           (long)c->N, c->nx, c->d_tw[1], c->d_tw[2], kys, kzs, hp, c->d_fpart);
+        TDW_HIP(hipGetLastError());
+    }
+    TDW_TIMED(c, "f_e");
+    hipLaunchKernelGGL(f_e_final, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_fpart, n, c->nx,
+                       dvol_norm(c), e_out, e_out2);
+    return (int)hipGetLastError();
+}
+
+int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    if (!fused_supported(c)) {
+        TDW_TRY(fused_finalize(c, s));
+        return tdw_step(c, s, it, grad_last);
+    }
+    const int n = s->n, h = nhist(c);
+    const long N = (long)c->N;
+    const int S = c->S;
+    const int wave = (S % 64) == 0;
+    TDW_TRY(ensure_state_vecs(c));
+    TDW_TRY(ensure_f(c, wave ? (size_t)2 * n * (S / 64) : (size_t)2 * n * S));
+    TDW_TRY(ensure_f(c, (size_t)n * c->nx));
+    pred_fn fp;
+    corr_fn fc;
+    xpass_fns(c->nx, c->p.scheme, &fp, &fc);
+    const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
+    const long nblk = (2L * n * S + FT - 1) / FT;
+    double2 *y = (double2 *)s->wf, *T = (double2 *)s->ddx, *H = (double2 *)s->lap;
+    double2 **f = (double2 **)s->f;
+    double *qpe = s->qpe ? s->qpe : c->d_qpe;
+
+    // ---- predictor ----
+    {
+        TDW_TIMED(c, "f_pred");
+        hipLaunchKernelGGL(fp, dim3(nblk), dim3(FT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], f[h - 1], T,
+                           c->d_scale, c->d_qpe, c->pending, n, N, S, c->p.dt, kxs, wave, c->d_fpart);
+        TDW_HIP(hipGetLastError());
+    }
+    c->pending = 0;
+    TDW_TRY(xpass_norm(c, n, c->d_sp, c->d_s2, wave));
+    TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt, s->dens, 0));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hyz(c, n, y, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
+
+    // ---- corrector ----
+    const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
+    {
+        TDW_TIMED(c, "f_corr");
+        hipLaunchKernelGGL(fc, dim3(nblk), dim3(FT), 0, c->stream, y, H, f[h - 1], T, c->d_sp, c->d_ep, n, N, S,
+                           c->p.dt * a0, kxs, wave, c->d_fpart);
+        TDW_HIP(hipGetLastError());
+    }
+    TDW_TRY(xpass_norm(c, n, c->d_scale, c->d_s2, wave));
+    if (!grad_last) {
+        TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
+                          s->dens, 0));
+        TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+        tdw_complex *tmp = s->f[h - 1];
+        for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+        s->f[0] = tmp;
+        TDW_TRY(hyz(c, n, y, T, (double2 *)s->f[0], s->pots, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
+        c->pending = 1;
+        return 0;
+    }
+    // measurement step: normalise in place, then the reference's derivative /
+    // density / tau / potential / H sequence
+    {
+        const long tot = 2L * n * N;
+        TDW_TIMED(c, "f_fixup");
+        hipLaunchKernelGGL(f_fixup, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, y, (double2 *)nullptr,
+                           c->d_scale, c->d_qpe, n, N, 0);
+        TDW_HIP(hipGetLastError());
+    }
+    TDW_TRY(derivatives(c, 2 * n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap));
+    TDW_TRY(densities(c, n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap,
+                      c->d_fbeta, nullptr, c->d_kt, c->d_cnt, s->dens, 1));
+    TDW_TRY(density_tau(c, s->dens));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    tdw_complex *tmp = s->f[h - 1];
+    for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+    s->f[0] = tmp;
+    TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, s->pots, c->d_kt, nullptr, qpe));
+    return 0;
+}
 
 }  // namespace tdw

@@ -6,7 +6,14 @@
 
 #include "wslda_td.h"
 
+#include <vector>
+
 #define TDW_MAXST 8
+
+struct tdw_timer_rec {
+    const char *name;
+    hipEvent_t a, b;
+};
 
 // Stockham plan of one lattice axis: radices R[s] applied with span Ns[s]
 // (product of the earlier radices).
@@ -45,11 +52,31 @@ struct tdw_ctx {
     double *d_vext;                      // 2N
     void *ref_workarea;                  // set_workspace_for_cufftPlan()
     // fused-path state
-    double *d_scale;                     // n: 1/sqrt(norm) of the unnormalised state
-    int fused_ready;
+    double *d_scale;                     // n: 1/sqrt(norm) of the stored (unnormalised) y
+    double *d_sp, *d_s2, *d_ep;          // n: predictor scale, scale^2, predictor <H>
+    double *d_fpart;  size_t fpart_cap;  // partial sums of the fused kernels
+    int pending;                         // y and f[0] carry a pending scale / <H> fix-up
+    // per-kernel timing
+    int timing;
+    std::vector<tdw_timer_rec> *timers;  // pending records
+    std::vector<hipEvent_t> *evpool;
 };
 
 namespace tdw {
+// Brackets one kernel launch with events when timing is enabled.
+// Records nest (a timed launcher may call other timed launchers): each scope
+// closes its own record.
+int timer_begin(tdw_ctx *c, const char *name);
+void timer_end(tdw_ctx *c, int idx);
+struct TimerScope {
+    tdw_ctx *c;
+    int idx;
+    TimerScope(tdw_ctx *cc, const char *name) : c(cc), idx(-1) { if (c->timing) idx = timer_begin(c, name); }
+    ~TimerScope() { if (idx >= 0) timer_end(c, idx); }
+};
+#define TDW_CAT2(a, b) a##b
+#define TDW_CAT(a, b) TDW_CAT2(a, b)
+#define TDW_TIMED(ctx, name) tdw::TimerScope TDW_CAT(_tdw_ts_, __LINE__)(ctx, name)
 int ensure_part(tdw_ctx *c, size_t ndoubles);
 int ensure_T(tdw_ctx *c, size_t nelem);
 

@@ -57,6 +57,7 @@ k_abm4(long tot, double dta0, const double2 *__restrict__ hy, double2 *__restric
 
 int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, double2 *f4, double2 *f5)
 {
+    TDW_TIMED(c, "abm_step1");
     long tot = 2L * n * (long)c->N;
     dim3 g((tot + NT - 1) / NT);
     double dt = c->p.dt;
@@ -71,6 +72,7 @@ int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, d
 
 int abm4(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl)
 {
+    TDW_TIMED(c, "abm_step4");
     long tot = 2L * n * (long)c->N;
     double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
     hipLaunchKernelGGL(k_abm4, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, tot, c->p.dt * a0, hy, y, fl);
@@ -126,6 +128,7 @@ static long pick_chunk(const tdw_ctx *c, int *nchunk)
 
 int norms(tdw_ctx *c, int n, const double2 *wf, double *d_nrm)
 {
+    TDW_TIMED(c, "norms");
     int nchunk;
     long chunk = pick_chunk(c, &nchunk);
     TDW_TRY(ensure_part(c, (size_t)n * nchunk));
@@ -139,6 +142,7 @@ int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm)
 {
     TDW_TRY(norms(c, n, wf, d_nrm));
     long tot = 2L * n * (long)c->N;
+    TDW_TIMED(c, "scale");
     hipLaunchKernelGGL(k_scale_states, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, wf, n, (long)c->N, d_nrm);
     return (int)hipGetLastError();
 }
@@ -247,6 +251,7 @@ int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const do
               const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
               const double *kt, const int *cnt, double *dens, int grad)
 {
+    TDW_TIMED(c, "densities");
     const long N = (long)c->N;
     int chunk = 128;
     long nblk = (N + NT - 1) / NT;
@@ -311,6 +316,7 @@ int laplace_real(tdw_ctx *c, const double *f, double *lap)
 
 int density_tau(tdw_ctx *c, double *dens)
 {
+    TDW_TIMED(c, "tau");
     long N = (long)c->N;
     TDW_TRY(ensure_T(c, (size_t)N));
     // b component first, then a (pca_dens.cu:317-333)
@@ -438,6 +444,7 @@ k_potentials(FParams f, VextDev V, long N, const double *__restrict__ dens, doub
 
 int potentials(tdw_ctx *c, int it, const double *dens, double *pots)
 {
+    TDW_TIMED(c, "potentials");
     long N = (long)c->N;
     hipLaunchKernelGGL(k_potentials, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, fparams(c),
                        make_vext(c, it), N, dens, pots);
@@ -492,6 +499,7 @@ k_sub_qpe(const double2 *__restrict__ in, double2 *__restrict__ out, int n, long
 int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const double2 *lap,
             const double *pots, const double *kt, const double *useqpe, double *d_qpe)
 {
+    TDW_TIMED(c, "apply_h");
     (void)it;
     const long N = (long)c->N;
     int nchunk;

@@ -1,0 +1,142 @@
+"""Synthetic td workloads of the benchmark configurations.
+
+Every workload is a uniform SLDA (or prescribed BdG) state expanded in plane
+waves exactly as pca_uniform.h's create_uniform_wf does it (order of states,
+u_k, v_k, branch signs, 1/sqrt(V) normalisation), then optionally perturbed
+(noise, envelopes, imprinted phases) and fed to the td step.  Large state
+sets are generated directly on the GPU.
+
+Units: kF = 1, so eF = 1/2; energies and times below are quoted in eF.
+"""
+import math
+
+import numpy as np
+
+from . import td
+
+PI = math.pi
+
+
+def kaxis(n, L):
+    i = np.arange(n)
+    return np.where(i < n // 2, 2 * PI / L * i, 2 * PI / L * (i - n))
+
+
+def bdg_branch(eta_a, eta_b, delta, branch):
+    """Solution `branch` (0: upper, 1: lower) of the uniform 2x2 BdG problem
+    (pca_uniform.h:689-696): energy E, u_k, v_k (v_k signed)."""
+    sq = np.sqrt((eta_b - eta_a) ** 2 + 4.0 * (eta_a * eta_b + delta * delta))
+    if branch == 0:
+        E = -0.5 * (eta_b - eta_a) + 0.5 * sq
+        v2 = delta * delta / ((0.5 * (eta_a + eta_b) + 0.5 * sq) ** 2 + delta * delta)
+    else:
+        E = -0.5 * (eta_b - eta_a) - 0.5 * sq
+        v2 = delta * delta / ((0.5 * (eta_a + eta_b) - 0.5 * sq) ** 2 + delta * delta)
+    u = np.sqrt(1.0 - v2)
+    v = np.sqrt(v2)
+    v = np.where(eta_b + E < 0.0, -v, v)
+    return E, u, v
+
+
+def plane_wave_list(params, mu_a, mu_b, V_a, V_b, delta, ec, alph=1.0):
+    """(kx, ky, kz, E, u, v) of every state with -ec < E < ec in the order of
+    create_uniform_wf (3D, all k) -- loops ix, iy, iz, then the two branches."""
+    assert params.dim == 3
+    kx = kaxis(params.nx, params.nx * params.dx)
+    ky = kaxis(params.ny, params.ny * params.dy)
+    kz = kaxis(params.nz, params.nz * params.dz)
+    KX, KY, KZ = np.meshgrid(kx, ky, kz, indexing="ij")
+    k2 = (KX ** 2 + KY ** 2 + KZ ** 2).ravel()
+    ea = alph * k2 / 2.0 + V_a - mu_a
+    eb = alph * k2 / 2.0 + V_b - mu_b
+    rows = []
+    for br in (0, 1):
+        E, u, v = bdg_branch(ea, eb, delta, br)
+        rows.append((E, u, v))
+    # interleave the branches per k point, keep |E| < ec
+    E = np.stack([rows[0][0], rows[1][0]], 1).ravel()
+    U = np.stack([rows[0][1], rows[1][1]], 1).ravel()
+    Vv = np.stack([rows[0][2], rows[1][2]], 1).ravel()
+    K = np.repeat(np.stack([KX.ravel(), KY.ravel(), KZ.ravel()], 1), 2, axis=0)
+    sel = (E > -ec) & (E < ec)
+    return K[sel], E[sel], U[sel], Vv[sel]
+
+
+def states_on_gpu(torch, params, K, U, V, device="cuda:0", chunk=2048, envelope=None, phase=None):
+    """psi_n(r) = exp(i k_n.r) (u_n, v_n) / sqrt(V_box) on the lattice, 3D,
+    optionally times an envelope(r) and exp(i*phase(r)) on the u component."""
+    n = K.shape[0]
+    nx, ny, nz = params.nx, params.ny, params.nz
+    N = nx * ny * nz
+    vol = (nx * params.dx) * (ny * params.dy) * (nz * params.dz)
+    X = torch.arange(nx, dtype=torch.float64, device=device) * params.dx
+    Y = torch.arange(ny, dtype=torch.float64, device=device) * params.dy
+    Z = torch.arange(nz, dtype=torch.float64, device=device) * params.dz
+    gx, gy, gz = torch.meshgrid(X, Y, Z, indexing="ij")
+    gx, gy, gz = gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)
+    wf = torch.empty((2 * n, N), dtype=torch.complex128, device=device)
+    Kt = torch.as_tensor(K, device=device)
+    Ut = torch.as_tensor(U, device=device)
+    Vt = torch.as_tensor(V, device=device)
+    s = 1.0 / math.sqrt(vol)
+    for a0 in range(0, n, chunk):
+        a1 = min(n, a0 + chunk)
+        ph = Kt[a0:a1, 0:1] * gx + Kt[a0:a1, 1:2] * gy + Kt[a0:a1, 2:3] * gz
+        e = torch.polar(torch.ones_like(ph), ph) * s
+        if envelope is not None:
+            e = e * envelope
+        eu = e if phase is None else e * phase
+        wf[a0:a1] = eu * Ut[a0:a1, None]
+        wf[n + a0:n + a1] = e * Vt[a0:a1, None]
+    return wf
+
+
+def _base_params(dim, nx, ny, nz, dx, scheme=td.AB5AM5, precision=td.FP64, vext=None, dt_eF=0.005):
+    eF = 0.5
+    return td.make_params(dim, nx, ny, nz, dx=dx, functional=td.SLDA, regularization=td.SPHERICAL_CUTOFF,
+                          mf_correction=1, scheme=scheme, dt=dt_eF / eF, vext=vext, precision=precision)
+
+
+def uniform_slda(params, T=1.0e-9):
+    """Self-consistent uniform SLDA state at kF = 1 (n_a = n_b = n/2)."""
+    n0 = 1.0 / (3.0 * PI ** 2)
+    kc = 0.999999 * PI / max(params.dx, params.dy, params.dz)
+    u, rc = td.uniform_solve(params, n0 / 2, n0 / 2, kc, T, eps=1e-9, scmix=0.5, muchange=0.5, maxiter=10000)
+    if rc != 0:
+        raise td.TDWError("uniform SLDA solution did not converge")
+    return u
+
+
+def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, device="cuda:0", scheme=td.AB5AM5,
+            shape=(24, 24, 24), dx=1.0):
+    """Plain 3D uniform SLDA: 24^3 periodic lattice, the n_states lowest-|E|
+    plane-wave BdG states of the self-consistent uniform solution (Delta of
+    the solution ~ 0.5 eF), seeded random-phase noise of amplitude 1e-6."""
+    nx, ny, nz = shape
+    p = _base_params(3, nx, ny, nz, dx, scheme=scheme, precision=precision)
+    u = uniform_slda(p)
+    # all plane-wave states of the lattice, the n_states of lowest |E| (a few
+    # may lie above the solution's cutoff when n_states exceeds its count)
+    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf)
+    order = np.argsort(np.abs(E), kind="stable")[:n_states]
+    order.sort()   # keep create_uniform_wf's order among the selected states
+    K, E, U, V = K[order], E[order], U[order], V[order]
+    wf = states_on_gpu(torch, p, K, U, V, device=device)
+    if noise > 0:
+        g = torch.Generator(device=device)
+        g.manual_seed(seed)
+        vol = (nx * dx) * (ny * dx) * (nz * dx)
+        th = torch.rand(wf.shape, generator=g, dtype=torch.float64, device=device) * (2 * PI)
+        wf += torch.polar(torch.full_like(th, noise / math.sqrt(vol)), th)
+        del th
+    p.mu_a, p.mu_b, p.ec = u.mu_a, u.mu_b, u.ec
+    N = nx * ny * nz
+    pots = np.zeros(12 * N)
+    pots[:N] = u.V_a
+    pots[N:2 * N] = u.V_b
+    pots[2 * N:4 * N:2] = u.delta
+    pots[4 * N:6 * N] = 1.0
+    fb = np.array([td.fbeta(e, u.beta) for e in E])
+    eF = 0.5
+    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
+                name="3D uniform SLDA %dx%dx%d, %d states" % (nx, ny, nz, len(E)))

@@ -51,6 +51,13 @@ class Params(C.Structure):
                 ("vext", Vext), ("precision", C.c_int)]
 
 
+class Uniform(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("n0_a", "n0_b", "mu_a", "mu_b", "V_a", "V_b", "alph_a", "alph_b", "kc", "ec",
+                 "nu", "delta", "tau_a", "tau_b", "beta", "S", "ekin", "epot", "epair")] + \
+               [("nwf", C.c_int), ("iters", C.c_int)]
+
+
 class State(C.Structure):
     _fields_ = [("n", C.c_int),
                 ("wf", C.c_void_p), ("lap", C.c_void_p), ("ddx", C.c_void_p),
@@ -121,7 +128,16 @@ def _sigs():
         "tdw_vext_eval": (I, [P, I, P, P]),
         "tdw_step": (I, [P, C.POINTER(State), I, I]),
         "tdw_step_fused": (I, [P, C.POINTER(State), I, I]),
+        "tdw_finalize": (I, [P, C.POINTER(State)]),
+        "tdw_fused_supported": (I, [P]),
         "tdw_initial_measure": (I, [P, C.POINTER(State)]),
+        "tdw_uniform_solve": (I, [C.POINTER(Params), D, D, D, D, D, D, D, D, D, I, I, C.POINTER(Uniform)]),
+        "tdw_uniform_count": (I, [C.POINTER(Params), C.POINTER(Uniform)]),
+        "tdw_uniform_states": (I, [C.POINTER(Params), C.POINTER(Uniform), I, I, P, P, P, P]),
+        "tdw_fbeta": (D, [D, D]),
+        "tdw_stream": (P, [P]),
+        "tdw_timing": (I, [P, I]),
+        "tdw_timing_report": (I, [P, I, P, P, P]),
         "tdw_bind": (I, [P]),
         "tdw_bound": (P, []),
         # drop-in layer (reference names)
@@ -290,6 +306,29 @@ class TDContext:
               "compute_energy")
         return e
 
+    def stream_ptr(self):
+        return int(lib().tdw_stream(self.h))
+
+    def timing(self, enable):
+        check(lib().tdw_timing(self.h, int(enable)), "tdw_timing")
+
+    def timing_report(self):
+        """{kernel name: (total ms, launches)} since the last report."""
+        mx = 64
+        names = C.create_string_buffer(32 * mx)
+        ms = np.zeros(mx)
+        cnt = np.zeros(mx, dtype=np.int32)
+        k = lib().tdw_timing_report(self.h, mx, names, ms.ctypes.data_as(C.c_void_p),
+                                    cnt.ctypes.data_as(C.c_void_p))
+        if k < 0:
+            raise TDWError("tdw_timing_report failed with code %d" % k)
+        raw = names.raw
+        out = {}
+        for i in range(k):
+            nm = raw[32 * i:32 * i + 32].split(b"\0")[0].decode()
+            out[nm] = (float(ms[i]), int(cnt[i]))
+        return out
+
     def vext(self, it):
         self._pre()
         va = self.torch.empty(self.N, dtype=self.torch.float64, device="cuda:%d" % self.device)
@@ -302,6 +341,40 @@ class TDContext:
         return wf.numel() // (2 * self.N)
 
 
+def uniform_solve(params, n0_a, n0_b, kc, T, eps=1e-9, scmix=0.5, muchange=0.5, maxiter=10000,
+                  Tstart=None, dT=0.01, mu_rule=0):
+    """solve_uniform_problem of pca_uniform.h (SLDA); returns (Uniform, rc)."""
+    u = Uniform()
+    rc = lib().tdw_uniform_solve(C.byref(params), n0_a, n0_b, kc, T if Tstart is None else Tstart, T, dT,
+                                 eps, scmix, muchange, maxiter, mu_rule, C.byref(u))
+    if rc < 0:
+        raise TDWError("tdw_uniform_solve failed with code %d" % rc)
+    return u, rc
+
+
+def uniform_states(params, u):
+    """All plane-wave BdG states of a uniform solution, as pca_uniform.h's
+    create_uniform_wf orders them: (wf [2n, N] complex128, E_n, kt, cnt)."""
+    n = lib().tdw_uniform_count(C.byref(params), C.byref(u))
+    N = params.nx if params.dim == 1 else params.nx * params.ny if params.dim == 2 else params.nx * params.ny * params.nz
+    wf = np.zeros((2 * n, N), dtype=np.complex128)
+    En = np.zeros(n)
+    kt = np.zeros(2 * n)
+    cnt = np.zeros(n, dtype=np.int32)
+    check(lib().tdw_uniform_states(C.byref(params), C.byref(u), 0, n, wf.ctypes.data_as(C.c_void_p),
+                                   En.ctypes.data_as(C.c_void_p), kt.ctypes.data_as(C.c_void_p),
+                                   cnt.ctypes.data_as(C.c_void_p)), "tdw_uniform_states")
+    if params.dim == 3:
+        return wf, En, None, None
+    if params.dim == 2:
+        return wf, En, kt[:n].copy(), None
+    return wf, En, kt, cnt
+
+
+def fbeta(E, beta):
+    return lib().tdw_fbeta(float(E), float(beta))
+
+
 class TDRun:
     """All device arrays of one td run, in the reference's layout, and the
     time loop of pca.c (initial measurement, PECE steps, energies)."""
@@ -312,7 +385,10 @@ class TDRun:
         dev = "cuda:%d" % ctx.device
         N, n = ctx.N, ctx.nwf
         self.n = n
-        self.wf = torch.as_tensor(np.ascontiguousarray(wf).reshape(2 * n, N), dtype=torch.complex128).to(dev)
+        if isinstance(wf, np.ndarray):
+            self.wf = torch.as_tensor(np.ascontiguousarray(wf).reshape(2 * n, N), dtype=torch.complex128).to(dev)
+        else:
+            self.wf = wf.reshape(2 * n, N).to(dev).contiguous()
         z = lambda: torch.zeros((2 * n, N), dtype=torch.complex128, device=dev)
         self.lap = z()
         self.ddx = z()
@@ -347,6 +423,10 @@ class TDRun:
     def step(self, it, grad_last=False):
         fn = lib().tdw_step_fused if self.fused else lib().tdw_step
         check(fn(self.ctx.h, C.byref(self.st), it, int(grad_last)), "tdw_step")
+
+    def finalize(self):
+        """Apply a fused step's pending normalisation / <H> subtraction."""
+        check(lib().tdw_finalize(self.ctx.h, C.byref(self.st)), "tdw_finalize")
 
     def energy(self, it):
         return self.ctx.compute_energy(it, self.dens, self.pots)
