@@ -49,12 +49,20 @@ def test_fused_vs_oracle_td_gauss_3d(torch, scheme):
             dens_check(R.dens.cpu().numpy(), P.dens, N)
     R.finalize()
     ctx.sync()
-    assert rel(R.wf.cpu().numpy(), P.wf) < 1e-11
+    w = R.wf.cpu().numpy()
+    d = np.abs(w - P.wf)
+    i, j = np.unravel_index(d.argmax(), d.shape)
+    assert rel(w, P.wf) < 1e-11, ("max diff at state-row %d point %d: %r vs %r" % (i, j, w[i, j], P.wf[i, j]))
     hist_ref = P.history()
     hist = R.history()
     k = {po.AB3AM4: 3, po.AB4AM5: 4, po.AB5AM5: 5}[scheme]
+    # f = (H - <H>) psi of near-eigenstates is a small difference of O(E psi)
+    # terms: compare on the scale of H psi (max|psi| times the largest
+    # kinetic energy on the lattice), not of f itself
+    emax = 3 * 0.5 * (np.pi / s["cfg"].dx) ** 2
+    scale = np.abs(P.wf).max() * emax
     for j in range(k - 1):
-        assert rel(hist[j].cpu().numpy(), hist_ref[j]) < 1e-11, j
+        assert np.abs(hist[j].cpu().numpy() - hist_ref[j]).max() < 1e-12 * scale, j
     e_ref = P.energy(it0 + nsteps)
     e = R.energy(it0 + nsteps)
     assert abs(e[7] + e[8] - e_ref[7] - e_ref[8]) <= TOL * abs(e_ref[7] + e_ref[8])
