@@ -26,30 +26,45 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(name, n, N, scheme, precision):
-    """Compulsory HBM bytes of one launch of the named kernel family, for n
-    states of N points (S = 2 components * N * 16 B in FP64)."""
-    S = 2 * N * 16
-    h = {34: 3, 45: 4, 55: 5}[scheme]
-    table = {
-        "abm_step1": (h + 1) + 2,     # read y, f_k..; write y_p, c
-        "abm_step4": 3,               # read H y_p, c; write y
-        "scale": 2,
-        "norms": 1,
-        "fft_outer": 2,               # read psi, write T
-        "fft_inner": 3,               # read psi, T; write lap
-        "fft_outer_deriv": 3,
-        "fft_inner_deriv": 5,
-        "densities": 1,
-        "apply_h": 6,                 # H psi (read psi, lap; write) + <H> subtraction (read psi, Hpsi; write)
-        # fused kernels
-        "f_pred": (h + 1) + 3,        # read y, F_k, older f; write y_p, c, f_k (+T counted below)
-        "f_pred_T": 1,
-        "f_corr": 3 + 2,              # read Hpsi, y_p, c; write y, T
-        "f_hx": 3,                    # read psi, T; write H psi
-        "f_dens": 1,
-    }
-    return table.get(name, 0) * S * n
+def nhist(scheme):
+    return {34: 3, 45: 4, 55: 5}[scheme]
+
+
+def algorithmic_units(name, scheme):
+    """A kernel's share of the compulsory traffic of one PECE step, in units
+    of S (one state's u and v in FP64) per state and launch.
+
+    The compulsory traffic of a step (AB5AM5) is B_qp = 13 S: the predictor
+    reads y, f_1..f_5 and writes y_p and the partial corrector c (8 S); the
+    corrector reads y_p and c and writes y (3 S); the history refresh reads
+    y and writes the newest f (2 S).  FFTs, densities, norms and <H> are
+    taken as fused into those passes, so whatever a kernel moves beyond its
+    share counts against its achieved bandwidth."""
+    h = nhist(scheme)
+    return {
+        "f_pred": (h + 1) + 2,
+        "f_corr": 3,
+        "f_hyz": 1,          # two launches per step: H of y_p (share 0), history refresh (share 2)
+        "abm_step1": (h + 1) + 2,
+        "abm_step4": 3,
+    }.get(name, 0)
+
+
+def design_units(name, scheme):
+    """Bytes (units of S per state and launch) this implementation's kernels
+    must move: reads and writes of state-sized arrays, FP64."""
+    h = nhist(scheme)
+    return {
+        "f_pred": (h + 1) + 4,   # read y, f_1..f_h; write y_p, c, T (= D_x y_p), f_1 (pending <H> fix-up)
+        "f_corr": 5,             # read y_p, H y_p, c; write y, T
+        "f_hyz": 3,              # read y, T; write H y
+        "densities": 1,          # read y
+        "f_fixup": 2,
+    }.get(name, 0)
+
+
+def step_units(scheme):
+    return nhist(scheme) + 8     # B_qp: 11, 12, 13 S for AB3AM4, AB4AM5, AB5AM5
 
 
 def run_gpu(args):
@@ -69,7 +84,6 @@ def run_gpu(args):
     del w["wf"]
     R.initial_measure()
     ctx.sync()
-    e0 = R.energy(0)
     setup_s = time.time() - t0
     it = 0
     meas = args.meas_every
@@ -85,6 +99,10 @@ def run_gpu(args):
 
     run_steps(args.warmup)
     ctx.sync()
+    # energy at the start of the timed loop: the warm-up ended on a gradient
+    # step, so densities (tau included) and potentials belong to the state
+    it_e0 = it
+    e0 = R.energy(it)
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(ctx.stream_ptr())
     ev0 = torch.cuda.Event(enable_timing=True)
@@ -105,17 +123,19 @@ def run_gpu(args):
     e1 = R.energy(it)
     N = ctx.N
     qps = n * args.steps / (ms / 1000.0)
-    # dominant kernel
+    # dominant kernel: largest total time in the timed loop
     dom = max(rep.items(), key=lambda kv: kv[1][0])
     dname, (dtot, dcnt) = dom
     davg = dtot / dcnt / 1000.0
-    dbytes = algorithmic_bytes(dname, n, N, p.scheme, prec)
+    S = 2 * N * 16
+    dbytes = algorithmic_units(dname, p.scheme) * S * n
     achieved = dbytes / davg / 1e9 if dbytes else None
-    step_bytes = {34: 11, 45: 12, 55: 13}[p.scheme] * 2 * N * 16 * n
+    moved = design_units(dname, p.scheme) * S * n
+    step_bytes = step_units(p.scheme) * S * n
     out = dict(qps=qps, ms=ms, wall=wall, n=n, N=N, rep=rep, dname=dname, davg=davg, dbytes=dbytes,
-               achieved=achieved, step_bytes=step_bytes, setup_s=setup_s, scheme=p.scheme,
-               e0=e0.tolist(), e1=e1.tolist(), name=w.get("name"), params=p, uniform=w["uniform"],
-               pots=w["pots"], fbeta=w["fbeta"], R=R, ctx=ctx)
+               achieved=achieved, moved=moved, step_bytes=step_bytes, setup_s=setup_s, scheme=p.scheme,
+               e0=e0.tolist(), e1=e1.tolist(), it_e0=it_e0, it_e1=it, name=w.get("name"), params=p,
+               uniform=w["uniform"], pots=w["pots"], fbeta=w["fbeta"], R=R, ctx=ctx)
     return out
 
 
@@ -166,6 +186,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-bytes", type=float, default=0.0,
+                    help="HBM bytes per launch of the dominant kernel from FETCH_SIZE + WRITE_SIZE counter runs "
+                         "(reported as roofline.traffic)")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (a plain copy kernel "
+                         "of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
@@ -193,15 +219,30 @@ def main():
         "roofline": {"bound": "hbm", "kernel": g["dname"],
                      "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (g["achieved"] / HBM_PEAK_GBS) if g["achieved"] else None,
-                     "traffic": None,
+                     "traffic": args.traffic_bytes if args.traffic_bytes else None,
                      "avg_launch_ms": g["davg"] * 1e3, "algorithmic_bytes_per_launch": g["dbytes"],
+                     "design_bytes_per_launch": g["moved"],
+                     "step_algorithmic_bytes": g["step_bytes"],
                      "step_level_frac": g["step_bytes"] / (g["ms"] / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(g["rep"].items(), key=lambda kv: -kv[1][0])},
-        "energy_drift": abs(sum(g["e1"][:7]) - sum(g["e0"][:7])) / abs(sum(g["e0"][:7])),
     }
+    e0, e1 = sum(g["e0"][:7]), sum(g["e1"][:7])
+    drift = abs(e1 - e0) / abs(e0)
+    res["energy_drift"] = {"E_start": e0, "E_end": e1, "steps": g["it_e1"] - g["it_e0"], "rel": drift,
+                           "rel_per_1000_steps": drift * 1000.0 / max(1, g["it_e1"] - g["it_e0"])}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(g, args)
     g["ctx"].close()
+    if args.copy_calib_mib:
+        import torch
+        nel = args.copy_calib_mib * 1024 * 1024 // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        res["copy_calib_bytes"] = nel * 8
+        del a, b
     print(json.dumps(res))
 
 
