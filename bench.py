@@ -50,10 +50,20 @@ def algorithmic_units(name, scheme):
     }.get(name, 0)
 
 
-def design_units(name, scheme):
+def design_units(name, scheme, split=False):
     """Bytes (units of S per state and launch) this implementation's kernels
-    must move: reads and writes of state-sized arrays, FP64."""
+    must move: reads and writes of state-sized arrays (FP64 S; the split
+    form's history, partial-corrector increment and T / H scratch are FP32,
+    S/2)."""
     h = nhist(scheme)
+    if split:
+        return {
+            "f_pred": 1 + 0.5 * h + 1 + 0.5 + 0.5 + 0.5,   # y, f_1..f_h; y_p, c - y_p, T, f_1 fix-up
+            "f_corr": 1 + 0.5 + 0.5 + 1 + 0.5,             # y_p, H y_p, c - y_p; y, T
+            "f_hyz": 1 + 0.5 + 0.5,                         # y, T; H y
+            "densities": 1,
+            "f_fixup": 2,
+        }.get(name, 0)
     return {
         "f_pred": (h + 1) + 4,   # read y, f_1..f_h; write y_p, c, T (= D_x y_p), f_1 (pending <H> fix-up)
         "f_corr": 5,             # read y_p, H y_p, c; write y, T
@@ -67,13 +77,12 @@ def step_units(scheme):
     return nhist(scheme) + 8     # B_qp: 11, 12, 13 S for AB3AM4, AB4AM5, AB5AM5
 
 
-def run_gpu(args):
+def run_gpu(args, prec):
     import numpy as np
     import torch
     from wslda_amd import problems, td
 
     dev = "cuda:0"
-    prec = td.SPLIT if args.mode == "split" else td.FP64
     t0 = time.time()
     w = problems.config2(torch, precision=prec, n_states=args.states, device=dev)
     p = w["params"]
@@ -130,11 +139,12 @@ def run_gpu(args):
     S = 2 * N * 16
     dbytes = algorithmic_units(dname, p.scheme) * S * n
     achieved = dbytes / davg / 1e9 if dbytes else None
-    moved = design_units(dname, p.scheme) * S * n
+    moved = design_units(dname, p.scheme, prec == td.SPLIT) * S * n
     step_bytes = step_units(p.scheme) * S * n
     out = dict(qps=qps, ms=ms, wall=wall, n=n, N=N, rep=rep, dname=dname, davg=davg, dbytes=dbytes,
                achieved=achieved, moved=moved, step_bytes=step_bytes, setup_s=setup_s, scheme=p.scheme,
                e0=e0.tolist(), e1=e1.tolist(), it_e0=it_e0, it_e1=it, name=w.get("name"), params=p,
+               shape=(p.nx, p.ny, p.nz),
                uniform=w["uniform"], pots=w["pots"], fbeta=w["fbeta"], R=R, ctx=ctx)
     return out
 
@@ -172,13 +182,68 @@ def cpu_baseline(g, args):
                        % ("x".join(str(x) for x in (cfg.nx, cfg.ny, cfg.nz)), ns, n, k, threads, dt))
 
 
+# HBM bytes per launch of the dominant kernel, from the FETCH_SIZE and
+# WRITE_SIZE counter runs of profiles/fused_fp64_24cubed_16384states.md
+# (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000 by the copy-kernel calibration),
+# keyed by (kernel, lattice, states, scheme, precision)
+MEASURED_TRAFFIC = {
+    ("f_pred", (24, 24, 24), 16384, 55, "fp64"): 72486234112,
+}
+
+
+def summarize(g, args, mode):
+    S = 2 * g["N"] * 16
+    out = {
+        "value": g["qps"],
+        "ms_per_step": g["ms"] / args.steps,
+        "roofline": {"bound": "hbm", "kernel": g["dname"],
+                     "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (g["achieved"] / HBM_PEAK_GBS) if g["achieved"] else None,
+                     "traffic": MEASURED_TRAFFIC.get((g["dname"], tuple(g["shape"]), g["n"], g["scheme"], mode)),
+                     "avg_launch_ms": g["davg"] * 1e3, "algorithmic_bytes_per_launch": g["dbytes"],
+                     "design_bytes_per_launch": g["moved"],
+                     "step_algorithmic_bytes": g["step_bytes"],
+                     "step_level_frac": g["step_bytes"] / (g["ms"] / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(g["rep"].items(), key=lambda kv: -kv[1][0])},
+    }
+    if args.traffic_bytes:
+        out["roofline"]["traffic"] = args.traffic_bytes
+    e0, e1 = sum(g["e0"][:7]), sum(g["e1"][:7])
+    drift = abs(e1 - e0) / abs(e0)
+    out["energy_drift"] = {"E_start": e0, "E_end": e1, "steps": g["it_e1"] - g["it_e0"], "rel": drift,
+                           "rel_per_1000_steps": drift * 1000.0 / max(1, g["it_e1"] - g["it_e0"])}
+    return out
+
+
+def density_errors(d, ref, N):
+    """Relative error per density field (max-abs difference over max-abs
+    value; nu as a complex field)."""
+    import numpy as np
+    out = {}
+    nu, nur = d[0:2 * N:2] + 1j * d[1:2 * N:2], ref[0:2 * N:2] + 1j * ref[1:2 * N:2]
+    out["nu"] = float(np.abs(nu - nur).max() / np.abs(nur).max())
+    for k, i in (("rho_a", 2), ("tau_a", 3), ("rho_b", 7), ("tau_b", 8)):
+        r = ref[i * N:(i + 1) * N]
+        out[k] = float(np.abs(d[i * N:(i + 1) * N] - r).max() / np.abs(r).max())
+    # currents vanish in a uniform state: measured against the current scale
+    # of the gas, rho * kF (or their own maximum, if larger)
+    n = ref[2 * N:3 * N] + ref[7 * N:8 * N]
+    jscale = n.max() * (3 * np.pi ** 2 * n.max()) ** (1.0 / 3.0)
+    for k, i0 in (("j_a", 4), ("j_b", 9)):
+        r = ref[i0 * N:(i0 + 3) * N]
+        out[k] = float(np.abs(d[i0 * N:(i0 + 3) * N] - r).max() / max(np.abs(r).max(), jscale))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--states", type=int, default=16384)
-    ap.add_argument("--mode", choices=["fp64", "split"], default="fp64")
+    ap.add_argument("--mode", choices=["both", "fp64", "split"], default="both",
+                    help="both: the FP64 run (value) and the split-precision run on the same initial state, "
+                         "with the split run's density errors against FP64")
     ap.add_argument("--fused", type=int, default=1)
     ap.add_argument("--meas-every", type=int, default=100,
                     help="steps between measurements (the corrector of the last step of each computes gradients)")
@@ -188,7 +253,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-bytes", type=float, default=0.0,
                     help="HBM bytes per launch of the dominant kernel from FETCH_SIZE + WRITE_SIZE counter runs "
-                         "(reported as roofline.traffic)")
+                         "(overrides the measured table)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a plain copy kernel "
                          "of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -197,40 +262,50 @@ def main():
         sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
                          "GPUs is a design, not code); run with --gpus 1\n")
         sys.exit(2)
-    g = run_gpu(args)
-    n, N = g["n"], g["N"]
+    from wslda_amd import td
+    primary = "split" if args.mode == "split" else "fp64"
+    g = run_gpu(args, td.SPLIT if primary == "split" else td.FP64)
+    n = g["n"]
+    head = summarize(g, args, primary)
     res = {
         "metric": "qp-wavefunction steps/s, FP64 vs split-prec at rel density err <=1e-6",
-        "value": g["qps"],
+        "value": head["value"],
         "unit": "qp-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": g["ms"] / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.mode == "fp64" else "f32-operator/f64-state",
+        "dtype": "f64" if primary == "fp64" else "f32-operator/f64-state",
         "data": "synthetic",
         "config": {"workload": "td-SLDA 3D uniform, %dx%dx%d lattice, %d qp states, AB5AM5, plane-wave BdG "
                                "+ seeded 1e-6 noise, gradients every %d steps" % (24, 24, 24, n, args.meas_every),
-                   "lattice": [24, 24, 24], "states": n, "scheme": "AB5AM5", "precision": args.mode,
+                   "lattice": [24, 24, 24], "states": n, "scheme": "AB5AM5", "precision": primary,
                    "fused": bool(args.fused)},
-        "roofline": {"bound": "hbm", "kernel": g["dname"],
-                     "achieved": g["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (g["achieved"] / HBM_PEAK_GBS) if g["achieved"] else None,
-                     "traffic": args.traffic_bytes if args.traffic_bytes else None,
-                     "avg_launch_ms": g["davg"] * 1e3, "algorithmic_bytes_per_launch": g["dbytes"],
-                     "design_bytes_per_launch": g["moved"],
-                     "step_algorithmic_bytes": g["step_bytes"],
-                     "step_level_frac": g["step_bytes"] / (g["ms"] / args.steps / 1e3) / 1e9 / HBM_PEAK_GBS},
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(g["rep"].items(), key=lambda kv: -kv[1][0])},
+        "roofline": head["roofline"],
+        "kernels_ms_per_step": head["kernels_ms_per_step"],
+        "energy_drift": head["energy_drift"],
     }
-    e0, e1 = sum(g["e0"][:7]), sum(g["e1"][:7])
-    drift = abs(e1 - e0) / abs(e0)
-    res["energy_drift"] = {"E_start": e0, "E_end": e1, "steps": g["it_e1"] - g["it_e0"], "rel": drift,
-                           "rel_per_1000_steps": drift * 1000.0 / max(1, g["it_e1"] - g["it_e0"])}
-    if not args.no_cpu:
+    if args.mode == "both":
+        gs = run_gpu(args, td.SPLIT)
+        sp = summarize(gs, args, "split")
+        dens64 = g["R"].dens.cpu().numpy()
+        denss = gs["R"].dens.cpu().numpy()
+        err = density_errors(denss, dens64, g["N"])
+        e64, es = sum(g["e1"][:7]), sum(gs["e1"][:7])
+        res["split"] = {"value": sp["value"], "ms_per_step": sp["ms_per_step"],
+                        "speedup_vs_fp64": sp["value"] / res["value"],
+                        "dtype": "f32-operator/f64-state",
+                        "density_rel_err_vs_fp64": err, "max_density_rel_err": max(err.values()),
+                        "energy_rel_diff_vs_fp64": abs(es - e64) / abs(e64),
+                        "after_steps": gs["it_e1"],
+                        "energy_drift": sp["energy_drift"], "roofline": sp["roofline"],
+                        "kernels_ms_per_step": sp["kernels_ms_per_step"]}
+        gs["ctx"].close()
+        del gs
+    if not args.no_cpu and primary == "fp64":
         res["cpu_baseline"] = cpu_baseline(g, args)
     g["ctx"].close()
     if args.copy_calib_mib:

@@ -156,12 +156,63 @@ int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
  * normalisation and <H> subtraction are applied by the next fused step, or by
  * tdw_finalize(), which every other entry point calls first.  Densities,
  * potentials and qpe are exact after every step.  Lattices the fused kernels
- * are not built for run the modular step (tdw_fused_supported() says which). */
+ * are not built for run the modular step (tdw_fused_supported() says which).
+ *
+ * Split precision (params.precision == TDW_SPLIT): the x / yz FFTs and the
+ * kinetic term run in FP32 complex, and f[0..4] hold FP32 complex values
+ * (8 bytes each; a buffer of 2*n*N*8 bytes is enough).  The state wf, every
+ * ABM update of it (the AM partial corrector is kept as the FP32 increment
+ * c - y_p and added in FP64), the densities, norms and <H> stay FP64;
+ * ddx and lap are used as FP32 scratch between steps.  Gradient steps
+ * (grad_last) compute derivatives, tau, j and H psi in FP64.  tdw_step and
+ * lattices without fused kernels return TDW_EINVAL in this mode. */
 int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last);
 int tdw_finalize(tdw_ctx *c, tdw_state *s);
 int tdw_fused_supported(tdw_ctx *c);
 /* pca.c:939-975: normalise, derivatives, densities, tau (potentials kept). */
 int tdw_initial_measure(tdw_ctx *c, tdw_state *s);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint (reverse-mode) time stepping.  No counterpart in the reference:
+ * gradients of a final-time observable with respect to the external-potential
+ * parameters, for SLDA in FP64 through the modular step's non-gradient form.
+ *
+ * tdw_adj_forward_step: tdw_step (grad_last = 0) that also records what the
+ *   backward step needs: a copy of y at the step's start (y_copy), a copy of
+ *   the new history entry (fnew_copy) and the tape fields below.
+ * tdw_adj_backward_step: applies the transpose of that step's Jacobian.
+ *   y0, f[0..h-1]: y and the history at the step's start; y1, fnew: y and
+ *   the new history entry after it; pots_prev: the 12*N potentials the step
+ *   started from.  a->ly holds dL/dy after the step on entry and dL/dy
+ *   before it on return; a->lf[0..h-1] likewise for the history (rotated in
+ *   place by pointer swaps); a->lpots (4*N: dL/d(Va, Vb, Re D, Im D)) holds
+ *   the gradient of the potentials the step left on entry and of pots_prev
+ *   on return.  dens_grad (4*N, or NULL): dL/d(rho_a, rho_b, Re nu, Im nu)
+ *   of the densities the step leaves.  theta_grad accumulates
+ *   dL/d(amp, cx, cy, cz, wx, wy, wz) of each Gaussian of params.vext.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int n;
+    double *scal;     /* 4*n: norm^2 of y_p, <H> of y_p, norm^2 of y_c, <H> of y' */
+    double *dens_p;   /* 12*N densities of the normalised prediction            */
+    double *pots_p;   /* 12*N potentials after the predictor's update           */
+    double *dens_c;   /* 12*N densities of the corrected state                  */
+    double *pots_c;   /* 12*N potentials after the corrector's update           */
+} tdw_tape_step;
+typedef struct {
+    int n;
+    tdw_complex *ly;
+    tdw_complex *lf[5];
+    double *lpots;        /* 4*N */
+    tdw_complex *w[5];    /* scratch, state-sized */
+    double *lgrid;        /* scratch, 12*N */
+    double *theta_grad;   /* 7*TDW_MAX_GAUSS, accumulated */
+} tdw_adj_state;
+int tdw_adj_forward_step(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_copy, tdw_complex *fnew_copy,
+                         tdw_tape_step *t);
+int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *a, int it, const tdw_complex *y0,
+                          const tdw_complex *const *f, const tdw_complex *y1, const tdw_complex *fnew,
+                          const tdw_tape_step *t, const double *pots_prev, const double *dens_grad);
 
 /* Uniform SLDA solution and plane-wave BdG states (host arrays).
  * tdw_fbeta: fbeta (pca_uniform.h:70-78).

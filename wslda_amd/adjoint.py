@@ -1,0 +1,237 @@
+"""Gradients of a final-time observable with respect to the external-potential
+parameters, by the adjoint (reverse-mode) PECE step of libwslda_td.so.
+
+The forward run is the modular step (pca.c:1401-1570, non-gradient form) from
+a start state with zero ABM history.  Every K steps it keeps a checkpoint (y,
+history, potentials).  The backward sweep takes the checkpoints in reverse:
+each K-step segment is recomputed with the taped step (which also records y,
+the new history entry, the norms, <H> values, densities and potentials of
+every step), then the adjoint steps run backwards through the segment.
+
+The observable is a linear functional of the final densities,
+O = sum_r g(r) . (rho_a, rho_b, Re nu, Im nu)(r), given by the 4*N array g.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import td
+
+NPAR = 7   # per Gaussian: amp, cx, cy, cz, wx, wy, wz
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class AdjointProblem:
+    """Forward run with checkpoints and the adjoint sweep on one context.
+
+    params: td.Params (SLDA, FP64); wf0: [2n, N] complex128 start state
+    (normalised here); pots0: 12*N start potentials; fbeta/kt/cnt: state
+    weights; nsteps: steps; K: checkpoint interval; it0: step index of the
+    start; dens_grad: 4*N observable weights (see module doc)."""
+
+    def __init__(self, params, wf0, pots0, fbeta, kt=None, cnt=None, nsteps=100, K=10, it0=0, dens_grad=None,
+                 device=0):
+        import torch
+        self.torch = torch
+        self.params = params
+        self.n = wf0.shape[0] // 2
+        self.ctx = td.TDContext(params, self.n, device)
+        self.ctx.set_states(fbeta, kt, cnt)
+        self.dev = "cuda:%d" % device
+        self.wf0 = torch.as_tensor(wf0).to(self.dev).to(torch.complex128).clone()
+        self.pots0 = torch.as_tensor(np.ascontiguousarray(pots0, dtype=np.float64)).to(self.dev)
+        self.nsteps, self.K, self.it0 = nsteps, K, it0
+        self.N = self.ctx.N
+        self.h = {td.AB3AM4: 3, td.AB4AM5: 4, td.AB5AM5: 5}[params.scheme]
+        self.dens_grad = None if dens_grad is None else torch.as_tensor(
+            np.ascontiguousarray(dens_grad, dtype=np.float64)).to(self.dev)
+        self.ckpts = []
+        self.run = None
+
+    # -- parameters -------------------------------------------------------
+    def set_vext(self, vext):
+        self.params.vext = vext
+        self.ctx.set_params(self.params)
+
+    # -- forward ----------------------------------------------------------
+    def _new_run(self, wf, pots, hist=None):
+        R = td.TDRun(self.ctx, wf.clone(), pots.clone().cpu().numpy())
+        if hist is not None:
+            for j in range(self.h):
+                R.f[j].copy_(hist[j])
+        return R
+
+    def forward(self, keep_checkpoints=True):
+        """Runs nsteps steps; returns the run (final y, densities, potentials)."""
+        torch = self.torch
+        R = self._new_run(self.wf0, self.pots0)
+        R.initial_measure()
+        self.ckpts = []
+        for k in range(self.nsteps):
+            if keep_checkpoints and k % self.K == 0:
+                self.ckpts.append((k, R.wf.clone(), [t.clone() for t in R.history()[:self.h]], R.pots.clone()))
+            R.step(self.it0 + k, grad_last=False)
+        self.ctx.sync()
+        self.run = R
+        return R
+
+    def observable(self, R=None):
+        R = R or self.run
+        d = R.dens
+        N = self.N
+        g = self.dens_grad
+        return float((g[:N] * d[2 * N:3 * N]).sum() + (g[N:2 * N] * d[7 * N:8 * N]).sum()
+                     + (g[2 * N:3 * N] * d[0:2 * N:2]).sum() + (g[3 * N:4 * N] * d[1:2 * N:2]).sum())
+
+    # -- backward ---------------------------------------------------------
+    def backward(self):
+        """dO/d(amp, cx, cy, cz, wx, wy, wz) of every Gaussian slot, shape
+        (TDW_MAX_GAUSS, 7); needs forward() first."""
+        torch = self.torch
+        n, N, h, K = self.n, self.N, self.h, self.K
+        dev = self.dev
+        z = lambda: torch.zeros((2 * n, N), dtype=torch.complex128, device=dev)
+        ly = z()
+        lf = [z() for _ in range(5)]
+        w = [z() for _ in range(5)]
+        lpots = torch.zeros(4 * N, dtype=torch.float64, device=dev)
+        lgrid = torch.zeros(12 * N, dtype=torch.float64, device=dev)
+        theta = torch.zeros(7 * td.MAX_GAUSS, dtype=torch.float64, device=dev)
+        A = td.AdjState()
+        A.n = n
+        A.ly = ly.data_ptr()
+        for j in range(5):
+            A.lf[j] = lf[j].data_ptr()
+            A.w[j] = w[j].data_ptr()
+        A.lpots, A.lgrid, A.theta_grad = lpots.data_ptr(), lgrid.data_ptr(), theta.data_ptr()
+        # tape of one segment
+        ty = [z() for _ in range(K + 1)]
+        tf = [z() for _ in range(K)]
+        tscal = torch.zeros((K, 4 * n), dtype=torch.float64, device=dev)
+        tgrid = torch.zeros((K, 4, 12 * N), dtype=torch.float64, device=dev)
+        tapes = []
+        for r in range(K):
+            t = td.TapeStep()
+            t.n = n
+            t.scal = tscal[r].data_ptr()
+            t.dens_p, t.pots_p = tgrid[r, 0].data_ptr(), tgrid[r, 1].data_ptr()
+            t.dens_c, t.pots_c = tgrid[r, 2].data_ptr(), tgrid[r, 3].data_ptr()
+            tapes.append(t)
+        L = td.lib()
+        last = True
+        for (k0, y0, hist0, pots0) in reversed(self.ckpts):
+            ks = min(K, self.nsteps - k0)
+            # taped recompute of the segment
+            R = self._new_run(y0, pots0, hist0)
+            for r in range(ks):
+                td.check(L.tdw_adj_forward_step(self.ctx.h, C.byref(R.st), self.it0 + k0 + r, _p(ty[r]), _p(tf[r]),
+                                                C.byref(tapes[r])), "tdw_adj_forward_step")
+            ty[ks].copy_(R.wf)
+            # adjoint sweep through the segment
+            for r in reversed(range(ks)):
+                hist = [(tf[r - j] if r - j >= 0 else hist0[j - r - 1]) for j in range(1, h + 1)]
+                fp = (C.c_void_p * h)(*[t.data_ptr() for t in hist])
+                pots_prev = tgrid[r - 1, 3] if r > 0 else pots0
+                dg = self.dens_grad if (last and r == ks - 1) else None
+                td.check(L.tdw_adj_backward_step(self.ctx.h, C.byref(A), self.it0 + k0 + r, _p(ty[r]), fp,
+                                                 _p(ty[r + 1]), _p(tf[r]), C.byref(tapes[r]), _p(pots_prev),
+                                                 _p(dg)), "tdw_adj_backward_step")
+            last = False
+            del R
+        self.ctx.sync()
+        self.lambda_y0 = ly
+        return theta.cpu().numpy().reshape(td.MAX_GAUSS, NPAR)
+
+    def close(self):
+        self.ckpts = []
+        self.run = None
+        self.ctx.close()
+
+
+def beams_vext(theta, switch=None):
+    """Gaussian beams exp(-((x-x_k)^2 + (y-y_k)^2) / (2 s_k^2)) with
+    theta = (A_1, x_1, y_1, s_1, A_2, x_2, y_2, s_2, ...)."""
+    th = np.asarray(theta, dtype=np.float64).reshape(-1, 4)
+    gs = []
+    for A, x, y, s in th:
+        w = -0.5 / (s * s)
+        gs.append((A, (x, y, 0.0), (w, w, 0.0)))
+    return td.gauss_vext(gs, switch=switch)
+
+
+def beams_grad(theta, g):
+    """Chain rule from the Gaussian-slot gradient g (MAX_GAUSS x 7) to theta."""
+    th = np.asarray(theta, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros_like(th)
+    for k, (A, x, y, s) in enumerate(th):
+        dw_ds = 1.0 / (s ** 3)
+        out[k] = (g[k, 0], g[k, 1], g[k, 2], (g[k, 4] + g[k, 5]) * dw_ds)
+    return out.ravel()
+
+
+def _make_function():
+    import torch
+
+    class VextObservable(torch.autograd.Function):
+        """O(theta): the observable of an AdjointProblem after its forward run
+        with the Gaussian beams theta; backward by the adjoint sweep."""
+
+        @staticmethod
+        def forward(fctx, theta, problem):
+            th = theta.detach().cpu().numpy()
+            problem.set_vext(beams_vext(th))
+            problem.forward()
+            fctx.problem = problem
+            fctx.theta = th
+            return torch.tensor(problem.observable(), dtype=torch.float64)
+
+        @staticmethod
+        def backward(fctx, grad_out):
+            g = fctx.problem.backward()
+            gt = torch.as_tensor(beams_grad(fctx.theta, g), dtype=torch.float64)
+            return grad_out * gt, None
+
+    return VextObservable
+
+
+_fn = None
+
+
+def vext_observable(theta, problem):
+    """torch.autograd entry point: O = vext_observable(theta, problem)."""
+    global _fn
+    if _fn is None:
+        _fn = _make_function()
+    return _fn.apply(theta, problem)
+
+
+def disk_weights(params, center, radius):
+    """dens_grad of the particle number inside a disk (2D) / ball (3D) /
+    interval (1D): VE * 1_{|r - c| < radius} on rho_a and rho_b."""
+    dim = params.dim
+    nx, ny, nz = params.nx, params.ny, params.nz
+    x = params.dx * (np.arange(nx) - nx // 2)
+    if dim == 1:
+        r2 = (x - center[0]) ** 2
+        ve = params.dx * params.dy * params.dz * ny * nz
+    elif dim == 2:
+        y = params.dy * (np.arange(ny) - ny // 2)
+        X, Y = np.meshgrid(x, y, indexing="ij")
+        r2 = ((X - center[0]) ** 2 + (Y - center[1]) ** 2).ravel()
+        ve = params.dx * params.dy * params.dz * nz
+    else:
+        y = params.dy * (np.arange(ny) - ny // 2)
+        zz = params.dz * (np.arange(nz) - nz // 2)
+        X, Y, Z = np.meshgrid(x, y, zz, indexing="ij")
+        r2 = ((X - center[0]) ** 2 + (Y - center[1]) ** 2 + (Z - center[2]) ** 2).ravel()
+        ve = params.dx * params.dy * params.dz
+    m = (r2 < radius * radius).astype(np.float64) * ve
+    N = m.size
+    g = np.zeros(4 * N)
+    g[:N] = m
+    g[N:2 * N] = m
+    return g

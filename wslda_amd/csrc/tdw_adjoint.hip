@@ -1,0 +1,538 @@
+// tdw_adjoint.hip -- reverse-mode (adjoint) PECE step and the taped forward
+// step it differentiates.
+//
+// The forward map is the modular step of pca.c:1401-1570 (SLDA, FAST
+// constant-mass mode, no quantum friction, non-gradient steps):
+//   y_p = y - i dt sum b_j f_j ;  c = y - i dt sum a_j f_j      (abm_step1)
+//   yh_p = y_p / n_p ; D_p = dens(yh_p) ; U_p = Phi(D_p, U_prev, V_ext)
+//   F_p = H(U_p) yh_p - e_p yh_p ,  e_p = dV Re<yh_p, H yh_p>
+//   y_c = -i dt a0 F_p + c                                         (abm_step4)
+//   y'  = y_c / n_c ; D' = dens(y') ; U' = Phi(D', U_p, V_ext)
+//   f_new = H(U') y' - e' y' ; history rotated
+// The backward step applies the transpose of its Jacobian (as a real-linear
+// map; gradients are paired by Re sum_r conj(a) b) in reverse order.  H is
+// self-adjoint; Phi is differentiated pointwise with dual numbers; the
+// derivative of the V_ext parameters is accumulated on the way.
+#include "tdw_functional_ad.hpp"
+#include "tdw_fused.hpp"
+#include "tdw_internal.hpp"
+#include "tdw_physics.hpp"
+
+namespace tdw {
+
+static constexpr int AT = 256;
+static constexpr int NTHETA = 7 * TDW_MAX_GAUSS;   // amp, cx, cy, cz, wx, wy, wz per Gaussian
+
+struct AbmCoef {
+    int h;
+    double b[5], a[5], a0;
+};
+
+// tdwslda_ode_integrator.h:26-97, 138-220
+static AbmCoef abm_coef(int scheme)
+{
+    AbmCoef k{};
+    if (scheme == TDW_AB3AM4) {
+        k.h = 3;
+        const double b[3] = {23. / 12., -16. / 12., 5. / 12.}, a[3] = {19. / 24., -5. / 24., 1. / 24.};
+        for (int j = 0; j < 3; j++) { k.b[j] = b[j]; k.a[j] = a[j]; }
+        k.a0 = 9. / 24.;
+    } else if (scheme == TDW_AB4AM5) {
+        k.h = 4;
+        const double b[4] = {55. / 24., -59. / 24., 37. / 24., -9. / 24.};
+        const double a[4] = {646. / 720., -264. / 720., 106. / 720., -19. / 720.};
+        for (int j = 0; j < 4; j++) { k.b[j] = b[j]; k.a[j] = a[j]; }
+        k.a0 = 251. / 720.;
+    } else {
+        k.h = 5;
+        const double b[5] = {1901. / 720., -2774. / 720., 2616. / 720., -1274. / 720., 251. / 720.};
+        const double a[4] = {646. / 720., -264. / 720., 106. / 720., -19. / 720.};
+        for (int j = 0; j < 5; j++) k.b[j] = b[j];
+        for (int j = 0; j < 4; j++) k.a[j] = a[j];
+        k.a0 = 251. / 720.;
+    }
+    return k;
+}
+
+// ---------------------------------------------------------------------------
+// per-state sums: part[a * gridDim.x + blk] over a chunk of points
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(AT)
+k_sdot_part(const double2 *__restrict__ x, const double2 *__restrict__ y, int n, long N, long chunk,
+            double *__restrict__ part)
+{
+    __shared__ double red[AT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    const long ou = (long)a * N, ov = ((long)n + a) * N;
+    double s = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += AT)
+        s += cmulc(x[ou + i], y[ou + i]).x + cmulc(x[ov + i], y[ov + i]).x;
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_rows(const double *__restrict__ part, int n, int nchunk, double scale, double *__restrict__ out)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < nchunk; k++) s += part[(long)a * nchunk + k];
+    out[a] = s * scale;
+}
+
+// ---------------------------------------------------------------------------
+// out = acc + hterm - 2 s_a (g1 + e_a g2)  (elementwise), and the potential
+// gradient of H: with nu = mu - s_a psi,
+//   dL/dVa = sum Re(nu_u* psi_u), dL/dVb = -sum Re(nu_v* psi_v),
+//   dL/dRe D = sum Re(nu_u* psi_v + nu_v* psi_u),
+//   dL/dIm D = sum (-Im(nu_u* psi_v) + Im(nu_v* psi_u))
+// summed over a chunk of states per point (point-major: coalesced rows).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(AT)
+k_cross_upd(double2 *out, const double2 *acc, const double2 *hterm, const double2 *__restrict__ g1,
+            const double2 *__restrict__ g2, const double2 *__restrict__ mu, const double2 *__restrict__ psi,
+            const double *__restrict__ sv, const double *__restrict__ ev, int n, long N, int chunk,
+            double *__restrict__ part)
+{
+    const long i = (long)blockIdx.x * AT + threadIdx.x;
+    if (i >= N) return;
+    const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
+    double gVa = 0.0, gVb = 0.0, gDr = 0.0, gDi = 0.0;
+    for (int a = a0; a < a1; a++) {
+        const long iu = (long)a * N + i, iv = ((long)n + a) * N + i;
+        const double s = sv[a];
+        const double e = ev ? ev[a] : 0.0;
+        double2 gu = g1[iu], gv = g1[iv];
+        if (g2) {
+            gu = gu + g2[iu] * e;
+            gv = gv + g2[iv] * e;
+        }
+        double2 ou = gu * (-2.0 * s), ov = gv * (-2.0 * s);
+        if (hterm) { ou = hterm[iu] + ou; ov = hterm[iv] + ov; }
+        if (acc) { ou = acc[iu] + ou; ov = acc[iv] + ov; }
+        const double2 pu = psi[iu], pv = psi[iv];
+        const double2 nu_u = mu[iu] - pu * s, nu_v = mu[iv] - pv * s;
+        out[iu] = ou;
+        out[iv] = ov;
+        gVa += cmulc(nu_u, pu).x;
+        gVb -= cmulc(nu_v, pv).x;
+        const double2 uv = cmulc(nu_u, pv), vu = cmulc(nu_v, pu);
+        gDr += uv.x + vu.x;
+        gDi += -uv.y + vu.y;
+    }
+    double *p = part + (long)blockIdx.y * 4 * N + i;
+    p[0] = gVa;
+    p[N] = gVb;
+    p[2 * N] = gDr;
+    p[3 * N] = gDi;
+}
+
+__global__ void k_cross_final(const double *__restrict__ part, int nchunk, long N, const double *__restrict__ add,
+                              double *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * AT + threadIdx.x;
+    if (i >= N) return;
+    for (int k = 0; k < 4; k++) {
+        double s = 0.0;
+        for (int c = 0; c < nchunk; c++) s += part[((long)c * 4 + k) * N + i];
+        if (add) s += add[k * N + i];
+        out[k * N + i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// transpose of the potential update at one point: dual numbers over the
+// inputs (rho_a, rho_b, Re nu, Im nu, u_a, u_b, Va_prev, Vb_prev); lU holds
+// dL/d(Va, Vb, Re D, Im D) in 4 planes of N.  Also accumulates dL/dtheta of
+// the Gaussian parameters through u_a, u_b (block partials).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(AT)
+k_pot_vjp(FParams f, VextDev V, long N, const double *__restrict__ dens, const double *__restrict__ pots_prev,
+          const double *__restrict__ lU, const double *__restrict__ dens_add, double *__restrict__ lD,
+          double *__restrict__ lUprev, double *__restrict__ gpart)
+{
+    typedef Dual<8> D8;
+    __shared__ double red[AT / 64];
+    const long i = (long)blockIdx.x * AT + threadIdx.x;
+    double lua = 0.0, lub = 0.0;
+    if (i < N) {
+        double ua, ub;
+        vext_point(V, i, ua, ub);
+        const D8 na = dvar<8>(dens[2 * N + i], 0), nb = dvar<8>(dens[7 * N + i], 1);
+        const Cx<D8> nu = cx<D8>(dvar<8>(dens[2 * i], 2), dvar<8>(dens[2 * i + 1], 3));
+        const D8 dua = dvar<8>(ua, 4), dub = dvar<8>(ub, 5);
+        D8 Va = dvar<8>(pots_prev[i], 6), Vb = dvar<8>(pots_prev[N + i], 7);
+        Cx<D8> del;
+        ad::slda_point(f, na, nb, nu, dua, dub, Va, Vb, del);
+        const double l0 = lU[i], l1 = lU[N + i], l2 = lU[2 * N + i], l3 = lU[3 * N + i];
+        double g[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) g[k] = l0 * Va.d[k] + l1 * Vb.d[k] + l2 * del.x.d[k] + l3 * del.y.d[k];
+        double d0 = g[0], d1 = g[1], d2 = g[2], d3 = g[3];
+        if (dens_add) {
+            d0 += dens_add[i];
+            d1 += dens_add[N + i];
+            d2 += dens_add[2 * N + i];
+            d3 += dens_add[3 * N + i];
+        }
+        lD[i] = d0;
+        lD[N + i] = d1;
+        lD[2 * N + i] = d2;
+        lD[3 * N + i] = d3;
+        lUprev[i] = g[6];
+        lUprev[N + i] = g[7];
+        lUprev[2 * N + i] = 0.0;
+        lUprev[3 * N + i] = 0.0;
+        lua = g[4];
+        lub = g[5];
+    }
+    // d(u_a, u_b)/dtheta of each Gaussian (same coordinates as vext_point)
+    int ix = 0, iy = 0, iz = 0;
+    if (i < N) decode(V.dim, V.ny, V.nz, i, ix, iy, iz);
+    const double x = V.dx * (ix - V.nx / 2);
+    const double y = V.dim >= 2 ? V.dy * (iy - V.ny / 2) : 0.0;
+    const double z = V.dim >= 3 ? V.dz * (iz - V.nz / 2) : 0.0;
+    const double lg = lua + (V.v.flip_b ? -lub : lub);   // dL/dg, g = the Gaussian sum
+    for (int k = 0; k < TDW_MAX_GAUSS; k++) {
+        double t[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (k < V.v.ngauss && i < N) {
+            const double X = x - V.v.cx[k], Y = y - V.v.cy[k], Z = z - V.v.cz[k];
+            const double E = V.s * exp(V.v.wx[k] * (X * X) + V.v.wy[k] * (Y * Y) + V.v.wz[k] * (Z * Z));
+            const double A = V.v.amp[k];
+            t[0] = lg * E;
+            t[1] = lg * A * E * (-2.0 * V.v.wx[k] * X);
+            t[2] = lg * A * E * (-2.0 * V.v.wy[k] * Y);
+            t[3] = lg * A * E * (-2.0 * V.v.wz[k] * Z);
+            t[4] = lg * A * E * (X * X);
+            t[5] = lg * A * E * (Y * Y);
+            t[6] = lg * A * E * (Z * Z);
+        }
+        for (int q = 0; q < 7; q++) {
+            const double s = block_sum(t[q], red);
+            if (threadIdx.x == 0) gpart[(long)blockIdx.x * NTHETA + 7 * k + q] = s;
+        }
+    }
+}
+
+__global__ void k_gsum(const double *__restrict__ gpart, int nblk, double *__restrict__ out)
+{
+    const int p = threadIdx.x;
+    if (p >= NTHETA) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += gpart[(long)b * NTHETA + p];
+    out[p] += s;
+}
+
+// ---------------------------------------------------------------------------
+// out += dens^T(lD) at psi (per state), and part = Re<psi, out> partials
+//   rho_a = sum wa |u|^2, rho_b = sum wb |v|^2, nu = sum cn u v*   (per state
+//   weights of k_dens_part: fbeta, transverse degeneracy, 1/transverse length)
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void __launch_bounds__(AT)
+k_dens_t(const double2 *__restrict__ psi, double2 *out, const double *__restrict__ lD,
+         const double *__restrict__ fbeta, const double *__restrict__ kt, const int *__restrict__ cnt, int n, long N,
+         long chunk, double tr, int cubic, double piz, double *__restrict__ part)
+{
+    __shared__ double red[AT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    double w = 1.0;
+    if (DIM == 1) w = (double)cnt[a];
+    else if (DIM == 2) {
+        const double kz = kt[a];
+        w = (fabs(kz) < 1.0e-12) ? 1.0 : 2.0;
+        if (cubic && fabs(kz + piz) < 1.0e-12) w = 1.0;
+    }
+    const double fb = fbeta[a];
+    const double wa = fb * w / tr, wb = (1.0 - fb) * w / tr, cn = (1.0 - 2.0 * fb) / 2.0 * w / tr;
+    const long ou = (long)a * N, ov = ((long)n + a) * N;
+    double s = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += AT) {
+        const double2 u = psi[ou + i], v = psi[ov + i];
+        const double2 ln = make_double2(lD[2 * N + i], lD[3 * N + i]);
+        const double2 gu = out[ou + i] + u * (2.0 * wa * lD[i]) + cmul(ln, v) * cn;
+        const double2 gv = out[ov + i] + v * (2.0 * wb * lD[N + i]) + cmul(make_double2(ln.x, -ln.y), u) * cn;
+        out[ou + i] = gu;
+        out[ov + i] = gv;
+        s += cmulc(u, gu).x + cmulc(v, gv).x;
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// corrector normalisation back, and the predictor quantities rebuilt from the
+// tape:  ly <- (ly - t1 y1)/n_c (= dL/dy_c = dL/dc) ; w0 <- i dt a0 ly (dL/dF_p) ;
+//   w3 <- yh_p = (y0 - i dt sum b f)/n_p ;
+//   w4 <- H(U_p) yh_p = i (n_c y1 - c)/(dt a0) + e_p yh_p ;  part: Re<w0, w3>
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(AT)
+k_recon(double2 *ly, const double2 *__restrict__ y1, const double2 *__restrict__ y0, const double2 *__restrict__ f0,
+        const double2 *__restrict__ f1, const double2 *__restrict__ f2, const double2 *__restrict__ f3,
+        const double2 *__restrict__ f4, double2 *__restrict__ w0, double2 *__restrict__ w3, double2 *__restrict__ w4,
+        const double *__restrict__ t1, const double *__restrict__ scal, AbmCoef cf, double dt, int n, long N,
+        long chunk, double *__restrict__ part)
+{
+    __shared__ double red[AT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    const double np = sqrt(scal[a]), ep = scal[n + a], nc = sqrt(scal[2 * n + a]);
+    const double tt = t1[a];
+    double s = 0.0;
+    for (int comp = 0; comp < 2; comp++) {
+        const long o = ((long)comp * n + a) * N;
+        for (long i = i0 + threadIdx.x; i < i1; i += AT) {
+            const long idx = o + i;
+            const double2 Y1 = y1[idx];
+            const double2 lyc = (ly[idx] - Y1 * tt) * (1.0 / nc);
+            ly[idx] = lyc;
+            const double2 mu = imul(lyc, dt * cf.a0);
+            w0[idx] = mu;
+            const double2 F[5] = {f0[idx], f1[idx], f2[idx], cf.h >= 4 ? f3[idx] : make_double2(0, 0),
+                                  cf.h >= 5 ? f4[idx] : make_double2(0, 0)};
+            double2 sb = make_double2(0, 0), sa = make_double2(0, 0);
+            for (int j = 0; j < cf.h; j++) {
+                sb = sb + F[j] * cf.b[j];
+                sa = sa + F[j] * cf.a[j];
+            }
+            const double2 Y0 = y0[idx];
+            const double2 yh = (Y0 + imul(sb, -dt)) * (1.0 / np);
+            const double2 cc = Y0 + imul(sa, -dt);
+            const double2 G = imul(Y1 * nc - cc, 1.0 / (dt * cf.a0)) + yh * ep;
+            w3[idx] = yh;
+            w4[idx] = G;
+            s += cmulc(mu, yh).x;
+        }
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = s;
+}
+
+// predictor normalisation back and the ABM transpose:
+//   lyp = (w1 - t2 w3)/n_p ; lf_j <- lf_j (0 for the last) + i dt (b_j lyp + a_j ly) ; ly <- ly + lyp
+__global__ void __launch_bounds__(AT)
+k_hist_back(double2 *ly, const double2 *__restrict__ w1, const double2 *__restrict__ w3, double2 *lf0, double2 *lf1,
+            double2 *lf2, double2 *lf3, double2 *lf4, const double *__restrict__ t2, const double *__restrict__ scal,
+            AbmCoef cf, double dt, int n, long N)
+{
+    const long idx = (long)blockIdx.x * AT + threadIdx.x;
+    if (idx >= 2L * n * N) return;
+    const long a = (idx / N) % n;
+    const double2 lyp = (w1[idx] - w3[idx] * t2[a]) * (1.0 / sqrt(scal[a]));
+    const double2 lc = ly[idx];
+    double2 *lf[5] = {lf0, lf1, lf2, lf3, lf4};
+    for (int j = 0; j < cf.h; j++) {
+        const double2 base = (j == cf.h - 1) ? make_double2(0.0, 0.0) : lf[j][idx];
+        lf[j][idx] = base + imul(lyp * cf.b[j] + lc * cf.a[j], dt);
+    }
+    ly[idx] = lc + lyp;
+}
+
+__global__ void k_copy_d(const double *__restrict__ s, double *__restrict__ d, long n)
+{
+    const long i = (long)blockIdx.x * AT + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static long dot_chunk(const tdw_ctx *c, int *nchunk)
+{
+    const long N = (long)c->N;
+    long chunk = 4096;
+    if (chunk > N) chunk = N;
+    *nchunk = (int)((N + chunk - 1) / chunk);
+    return chunk;
+}
+
+static int sdot(tdw_ctx *c, int n, const double2 *x, const double2 *y, double scale, double *out)
+{
+    int nchunk;
+    const long chunk = dot_chunk(c, &nchunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    hipLaunchKernelGGL(k_sdot_part, dim3(nchunk, n), dim3(AT), 0, c->stream, x, y, n, (long)c->N, chunk, c->d_part);
+    hipLaunchKernelGGL(k_rows, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk, scale, out);
+    return (int)hipGetLastError();
+}
+
+static int cross_upd(tdw_ctx *c, int n, double2 *out, const double2 *acc, const double2 *hterm, const double2 *g1,
+                     const double2 *g2, const double2 *mu, const double2 *psi, const double *sv, const double *ev,
+                     const double *add, double *lU)
+{
+    TDW_TIMED(c, "adj_cross");
+    const long N = (long)c->N;
+    const long nblk = (N + AT - 1) / AT;
+    int chunk = 128;
+    while (chunk > 8 && nblk * ((n + chunk - 1) / chunk) < 2048) chunk >>= 1;
+    const int nchunk = (n + chunk - 1) / chunk;
+    TDW_TRY(ensure_part(c, (size_t)nchunk * 4 * N));
+    hipLaunchKernelGGL(k_cross_upd, dim3(nblk, nchunk), dim3(AT), 0, c->stream, out, acc, hterm, g1, g2, mu, psi, sv,
+                       ev, n, N, chunk, c->d_part);
+    hipLaunchKernelGGL(k_cross_final, dim3(nblk), dim3(AT), 0, c->stream, c->d_part, nchunk, N, add, lU);
+    return (int)hipGetLastError();
+}
+
+static int pot_vjp(tdw_ctx *c, int it, const double *dens, const double *pots_prev, const double *lU,
+                   const double *dens_add, double *lD, double *lUprev, double *theta_grad)
+{
+    TDW_TIMED(c, "adj_pot_vjp");
+    const long N = (long)c->N;
+    const int nblk = (int)((N + AT - 1) / AT);
+    TDW_TRY(ensure_part(c, (size_t)nblk * NTHETA));
+    hipLaunchKernelGGL(k_pot_vjp, dim3(nblk), dim3(AT), 0, c->stream, fparams(c), make_vext(c, it), N, dens,
+                       pots_prev, lU, dens_add, lD, lUprev, c->d_part);
+    hipLaunchKernelGGL(k_gsum, dim3(1), dim3(64), 0, c->stream, c->d_part, nblk, theta_grad);
+    return (int)hipGetLastError();
+}
+
+static int dens_t(tdw_ctx *c, int n, const double2 *psi, double2 *out, const double *lD, double *tvec)
+{
+    TDW_TIMED(c, "adj_dens_t");
+    int nchunk;
+    const long chunk = dot_chunk(c, &nchunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    const double tr = dens_transverse(c), piz = M_PI / c->p.dz;
+    const int cubic = c->p.regularization == TDW_CUBIC_CUTOFF;
+    const long N = (long)c->N;
+    dim3 g(nchunk, n);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_dens_t<1>, g, dim3(AT), 0, c->stream, psi, out, lD, c->d_fbeta, c->d_kt, c->d_cnt, n, N,
+                           chunk, tr, cubic, piz, c->d_part);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_dens_t<2>, g, dim3(AT), 0, c->stream, psi, out, lD, c->d_fbeta, c->d_kt, c->d_cnt, n, N,
+                           chunk, tr, cubic, piz, c->d_part);
+    else
+        hipLaunchKernelGGL(k_dens_t<3>, g, dim3(AT), 0, c->stream, psi, out, lD, c->d_fbeta, c->d_kt, c->d_cnt, n, N,
+                           chunk, tr, cubic, piz, c->d_part);
+    hipLaunchKernelGGL(k_rows, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk, dvol_norm(c),
+                       tvec);
+    return (int)hipGetLastError();
+}
+
+static int copy_d(tdw_ctx *c, const double *s, double *d, long n)
+{
+    hipLaunchKernelGGL(k_copy_d, dim3((n + AT - 1) / AT), dim3(AT), 0, c->stream, s, d, n);
+    return (int)hipGetLastError();
+}
+
+static int adj_supported(const tdw_ctx *c)
+{
+    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64;
+}
+
+}  // namespace tdw
+
+using namespace tdw;
+
+#define D2(p) ((double2 *)(p))
+#define CD2(p) ((const double2 *)(p))
+
+extern "C" int tdw_adj_forward_step(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_copy, tdw_complex *fnew_copy,
+                                    tdw_tape_step *t)
+{
+    if (!c || !s || !t || s->n > c->nwf || t->n != s->n) return TDW_EINVAL;
+    if (!adj_supported(c)) return TDW_EINVAL;
+    TDW_TRY(fused_finalize(c, s));
+    const int n = s->n;
+    const AbmCoef cf = abm_coef(c->p.scheme);
+    const int k = cf.h;
+    const long N = (long)c->N;
+    const size_t sbytes = sizeof(double2) * 2 * (size_t)n * N;
+    double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    if (y_copy) TDW_HIP(hipMemcpyAsync(y_copy, s->wf, sbytes, hipMemcpyDeviceToDevice, c->stream));
+    // the call sequence of tdw_step (pca.c:1401-1570), non-gradient form
+    TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
+    TDW_TRY(normalize(c, n, D2(s->wf), t->scal));
+    TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
+                      s->dens, 0));
+    TDW_TRY(copy_d(c, s->dens, t->dens_p, 12 * N));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(copy_d(c, s->pots, t->pots_p, 12 * N));
+    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + n));
+    TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
+    TDW_TRY(normalize(c, n, D2(s->wf), t->scal + 2 * n));
+    TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
+                      s->dens, 0));
+    TDW_TRY(copy_d(c, s->dens, t->dens_c, 12 * N));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(copy_d(c, s->pots, t->pots_c, 12 * N));
+    tdw_complex *tmp = s->f[k - 1];
+    for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+    s->f[0] = tmp;
+    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + 3 * n));
+    TDW_HIP(hipMemcpyAsync(qpe, t->scal + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    if (fnew_copy) TDW_HIP(hipMemcpyAsync(fnew_copy, s->f[0], sbytes, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const tdw_complex *y0,
+                                     const tdw_complex *const *f, const tdw_complex *y1, const tdw_complex *fnew,
+                                     const tdw_tape_step *t, const double *pots_prev, const double *dens_grad)
+{
+    if (!c || !A || !t || !f || A->n > c->nwf || t->n != A->n) return TDW_EINVAL;
+    if (!adj_supported(c)) return TDW_EINVAL;
+    const int n = A->n;
+    const long N = (long)c->N;
+    const AbmCoef cf = abm_coef(c->p.scheme);
+    const int h = cf.h;
+    for (int j = 0; j < h; j++)
+        if (!f[j] || !A->lf[j]) return TDW_EINVAL;
+    const double dv = dvol_norm(c);
+    if (!c->d_adjv) TDW_HIP(hipMalloc(&c->d_adjv, sizeof(double) * 4 * (size_t)c->nwf));
+    double *sv1 = c->d_adjv, *t1 = c->d_adjv + c->nwf, *svp = c->d_adjv + 2 * c->nwf, *t2 = c->d_adjv + 3 * c->nwf;
+    const double *nrm_p = t->scal, *e_p = t->scal + n, *e_n = t->scal + 3 * n;
+    (void)nrm_p;
+    double2 *LY = D2(A->ly), *W0 = D2(A->w[0]), *W1 = D2(A->w[1]), *W2 = D2(A->w[2]), *W3 = D2(A->w[3]),
+            *W4 = D2(A->w[4]);
+    double2 *MU = D2(A->lf[0]);
+    double *lUc = A->lgrid, *lD = A->lgrid + 4 * N, *lUo = A->lgrid + 8 * N;
+    TimerScope ts(c, "adj_step");
+
+    // history refresh f_new = H(U') y' - e' y'  (mu = dL/df_new)
+    TDW_TRY(sdot(c, n, MU, CD2(y1), dv, sv1));
+    TDW_TRY(laplace(c, 2 * n, MU, W2));
+    TDW_TRY(apply_h(c, it + 1, n, MU, W1, W2, t->pots_c, c->d_kt, e_n, nullptr));
+    TDW_TRY(cross_upd(c, n, LY, LY, W1, CD2(fnew), CD2(y1), MU, CD2(y1), sv1, e_n, A->lpots, lUc));
+    // U' = Phi(D', U_p, V_ext)
+    TDW_TRY(pot_vjp(c, it + 1, t->dens_c, t->pots_p, lUc, dens_grad, lD, lUo, A->theta_grad));
+    // D' = dens(y'), then y' = y_c / n_c
+    TDW_TRY(dens_t(c, n, CD2(y1), LY, lD, t1));
+    // rebuild the predictor side; LY becomes dL/dy_c = dL/dc, W0 = dL/dF_p
+    {
+        int nchunk;
+        const long chunk = dot_chunk(c, &nchunk);
+        TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+        const double2 *F[5] = {CD2(f[0]), CD2(f[1]), CD2(f[2]), h >= 4 ? CD2(f[3]) : nullptr,
+                               h >= 5 ? CD2(f[4]) : nullptr};
+        TDW_TIMED(c, "adj_recon");
+        hipLaunchKernelGGL(k_recon, dim3(nchunk, n), dim3(AT), 0, c->stream, LY, CD2(y1), CD2(y0), F[0], F[1], F[2],
+                           F[3], F[4], W0, W3, W4, t1, t->scal, cf, c->p.dt, n, N, chunk, c->d_part);
+        hipLaunchKernelGGL(k_rows, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk, dv, svp);
+        TDW_HIP(hipGetLastError());
+    }
+    // F_p = H(U_p) yh_p - e_p yh_p
+    TDW_TRY(laplace(c, 2 * n, W0, W2));
+    TDW_TRY(apply_h(c, it + 1, n, W0, W1, W2, t->pots_p, c->d_kt, e_p, nullptr));
+    TDW_TRY(cross_upd(c, n, W1, nullptr, W1, W4, nullptr, W0, W3, svp, nullptr, lUo, lUc));
+    // U_p = Phi(D_p, U_prev, V_ext)
+    TDW_TRY(pot_vjp(c, it + 1, t->dens_p, pots_prev, lUc, nullptr, lD, A->lpots, A->theta_grad));
+    TDW_TRY(dens_t(c, n, W3, W1, lD, t2));
+    // yh_p = y_p / n_p, then the ABM predictor / partial corrector
+    tdw_complex *mu_store = A->lf[0];
+    for (int j = 0; j < h - 1; j++) A->lf[j] = A->lf[j + 1];
+    A->lf[h - 1] = mu_store;
+    {
+        TDW_TIMED(c, "adj_hist");
+        const long tot = 2L * n * N;
+        hipLaunchKernelGGL(k_hist_back, dim3((tot + AT - 1) / AT), dim3(AT), 0, c->stream, LY, W1, W3, D2(A->lf[0]),
+                           D2(A->lf[1]), D2(A->lf[2]), h >= 4 ? D2(A->lf[3]) : nullptr,
+                           h >= 5 ? D2(A->lf[4]) : nullptr, t2, t->scal, cf, c->p.dt, n, N);
+        TDW_HIP(hipGetLastError());
+    }
+    return 0;
+}

@@ -210,7 +210,7 @@ extern "C" int tdw_destroy(tdw_ctx *c)
         if (c->d_tw32[k]) hipFree(c->d_tw32[k]);
     }
     void *bufs[] = {c->d_fbeta, c->d_kt, c->d_cnt, c->d_part, c->d_nrm, c->d_qpe, c->d_esum,
-                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale};
+                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale, c->d_adjv};
     for (void *b : bufs)
         if (b) hipFree(b);
     fused_release(c);
@@ -391,6 +391,9 @@ static int nhist(const tdw_ctx *c)
 extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 {
     if (!c || !s || s->n > c->nwf) return TDW_EINVAL;
+    // the split-precision form keeps FP32 history entries and exists only as
+    // the fused step
+    if (c->p.precision == TDW_SPLIT) return TDW_EINVAL;
     TDW_TRY(fused_finalize(c, s));
     const int n = s->n, k = nhist(c);
     double *qpe = s->qpe ? s->qpe : c->d_qpe;

@@ -23,9 +23,16 @@
 #include "tdw_physics.hpp"
 #include "tdw_regfft.hpp"
 
+#include <type_traits>
+
 namespace tdw {
 
 static constexpr int FT = 256;
+
+// Storage type of the history entries, the partial corrector and the T / H
+// scratch: FP64 in the plain form; FP32 in the split-precision form, where
+// the state y stays FP64 and every update of it is done in FP64.
+template <bool SP> using hc_t = typename std::conditional<SP, float2, double2>::type;
 
 // ---------------------------------------------------------------------------
 // ABM coefficient sums (tdwslda_ode_integrator.h:26-97)
@@ -46,17 +53,18 @@ __device__ __forceinline__ void abm_sums(double2 a, double2 b, double2 d, double
     }
 }
 
-template <int LX>
-__device__ __forceinline__ void dxx_inplace(double2 (&r)[LX], double kscale)
+template <int LX, typename C>
+__device__ __forceinline__ void dxx_inplace(C (&r)[LX], double kscale)
 {
     // D_x = F^-1 (-k^2/LX) F ; kscale = (2 pi / Lx)^2 / LX
-    rf::fft<LX, -1, double2>(r);
+    typedef decltype(r[0].x) R;
+    rf::fft<LX, -1, C>(r);
 #pragma unroll
     for (int i = 0; i < LX; i++) {
         const int m = (i < LX / 2) ? i : i - LX;
-        r[i] = r[i] * (-kscale * (double)(m * m));
+        r[i] = r[i] * (R)(-kscale * (double)(m * m));
     }
-    rf::fft<LX, +1, double2>(r);
+    rf::fft<LX, +1, C>(r);
 }
 
 // wave: S % 64 == 0, so every wave lies inside one component and Ptot % 64 == 0
@@ -72,14 +80,18 @@ __device__ __forceinline__ void put_norm_part(double nrm, bool wave, long P, lon
 
 // ---------------------------------------------------------------------------
 // predictor x-pass
+//   plain: fold <- c = y_k - i dt sum_j a_j f_j            (FP64)
+//   split: fold <- c - y_p = -i dt sum_j (a_j - b_j) f_j   (FP32, a dt-sized
+//          increment; the corrector adds it to the FP64 y_p)
 // ---------------------------------------------------------------------------
-template <int LX, int SCH>
+template <int LX, int SCH, bool SP>
 __global__ void __launch_bounds__(FT)
-f_pred(double2 *__restrict__ y, double2 *f1, const double2 *f2, const double2 *f3, const double2 *f4,
-       const double2 *f5, double2 *fold, double2 *__restrict__ T, const double *__restrict__ sk,
+f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP> *f3, const hc_t<SP> *f4,
+       const hc_t<SP> *f5, hc_t<SP> *fold, hc_t<SP> *__restrict__ T, const double *__restrict__ sk,
        const double *__restrict__ ek, int pending, int n, long N, int S, double dt, double kscale, int wave,
        double *__restrict__ part)
 {
+    typedef hc_t<SP> HC;
     const long P = (long)blockIdx.x * FT + threadIdx.x;
     const long Ptot = 2L * n * S;
     const bool act = P < Ptot;
@@ -90,25 +102,26 @@ f_pred(double2 *__restrict__ y, double2 *f1, const double2 *f2, const double2 *f
         const double s = pending ? sk[a] : 1.0;
         const double e = pending ? ek[a] : 0.0;
         const long base = a2 * N + j;
-        double2 r[LX];
+        HC r[LX];
 #pragma unroll
         for (int i = 0; i < LX; i++) {
             const long idx = base + (long)i * S;
             const double2 yk = y[idx] * s;
-            double2 F1 = f1[idx];
+            double2 F1 = to64(f1[idx]);
             if (pending) {
                 F1 = F1 - yk * e;
-                f1[idx] = F1;
+                f1[idx] = cvt<HC>(F1);
             }
-            const double2 F2 = f2[idx], F3 = f3[idx];
-            const double2 F4 = (SCH >= 45) ? f4[idx] : make_double2(0.0, 0.0);
-            const double2 F5 = (SCH >= 55) ? f5[idx] : make_double2(0.0, 0.0);
+            const double2 F2 = to64(f2[idx]), F3 = to64(f3[idx]);
+            const double2 F4 = (SCH >= 45) ? to64(f4[idx]) : make_double2(0.0, 0.0);
+            const double2 F5 = (SCH >= 55) ? to64(f5[idx]) : make_double2(0.0, 0.0);
             double2 sp, sc;
             abm_sums<SCH>(F1, F2, F3, F4, F5, sp, sc);
             const double2 yp = yk + imul(sp, -dt);
             y[idx] = yp;
-            fold[idx] = yk + imul(sc, -dt);
-            r[i] = yp;
+            if constexpr (SP) fold[idx] = to32(imul(sc - sp, -dt));
+            else fold[idx] = yk + imul(sc, -dt);
+            r[i] = cvt<HC>(yp);
             nrm += cnorm(yp);
         }
         dxx_inplace<LX>(r, kscale);
@@ -119,14 +132,15 @@ f_pred(double2 *__restrict__ y, double2 *f1, const double2 *f2, const double2 *f
 }
 
 // ---------------------------------------------------------------------------
-// corrector x-pass
+// corrector x-pass: y = -i dt a0 (H - <H>) psi_p + c
 // ---------------------------------------------------------------------------
-template <int LX>
+template <int LX, bool SP>
 __global__ void __launch_bounds__(FT)
-f_corr(double2 *__restrict__ y, const double2 *__restrict__ hin, const double2 *__restrict__ cin,
-       double2 *__restrict__ T, const double *__restrict__ sp_, const double *__restrict__ ep_, int n, long N,
+f_corr(double2 *__restrict__ y, const hc_t<SP> *__restrict__ hin, const hc_t<SP> *__restrict__ cin,
+       hc_t<SP> *__restrict__ T, const double *__restrict__ sp_, const double *__restrict__ ep_, int n, long N,
        int S, double dta0, double kscale, int wave, double *__restrict__ part)
 {
+    typedef hc_t<SP> HC;
     const long P = (long)blockIdx.x * FT + threadIdx.x;
     const long Ptot = 2L * n * S;
     const bool act = P < Ptot;
@@ -136,15 +150,18 @@ f_corr(double2 *__restrict__ y, const double2 *__restrict__ hin, const double2 *
         const int a = (int)(a2 < n ? a2 : a2 - n);
         const double s = sp_[a], e = ep_[a];
         const long base = a2 * N + j;
-        double2 r[LX];
+        HC r[LX];
 #pragma unroll
         for (int i = 0; i < LX; i++) {
             const long idx = base + (long)i * S;
-            const double2 psi = y[idx] * s;
-            const double2 fp = hin[idx] - psi * e;          // (H - <H>) psi_p
-            const double2 yn = imul(fp, -dta0) + cin[idx];  // (-i dt a0) fp + c
+            const double2 yp = y[idx];
+            const double2 psi = yp * s;
+            const double2 fp = to64(hin[idx]) - psi * e;        // (H - <H>) psi_p
+            double2 yn;
+            if constexpr (SP) yn = imul(fp, -dta0) + (yp + to64(cin[idx]));
+            else yn = imul(fp, -dta0) + cin[idx];
             y[idx] = yn;
-            r[i] = yn;
+            r[i] = cvt<HC>(yn);
             nrm += cnorm(yn);
         }
         dxx_inplace<LX>(r, kscale);
@@ -173,8 +190,8 @@ __global__ void f_norm_final(const double *__restrict__ part, int n, int per, do
 // ---------------------------------------------------------------------------
 // yz-plane pass: lap = s (T + D_y y + D_z y); H psi; <psi|H psi> partials
 // ---------------------------------------------------------------------------
-template <int L, int R, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN>
-__device__ __forceinline__ void cstage(const double2 *src, double2 *dst, const double2 *tw, int tid)
+template <int L, int R, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, typename C>
+__device__ __forceinline__ void cstage(const C *src, C *dst, const C *tw, int tid)
 {
     constexpr int nb = L / R;
     constexpr int tstep = L / (Ns * R);
@@ -182,37 +199,32 @@ __device__ __forceinline__ void cstage(const double2 *src, double2 *dst, const d
         const int q = b / nb, j = b - q * nb;
         const int k = j % Ns;
         const int off = (q / QD) * QH + (q % QD) * QL;
-        double2 v[R];
+        C v[R];
 #pragma unroll
         for (int t = 0; t < R; t++) v[t] = src[off + (j + t * nb) * IS];
 #pragma unroll
         for (int t = 1; t < R; t++) {
-            const double2 w = tw[t * k * tstep];
-            v[t] = cmul(v[t], SIGN > 0 ? make_double2(w.x, -w.y) : w);
+            const C w = tw[t * k * tstep];
+            v[t] = cmul(v[t], SIGN > 0 ? mk(w.x, -w.y) : w);
         }
-        rf::small_dft<R, SIGN, double2>::run(v);
+        rf::small_dft<R, SIGN, C>::run(v);
         const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
         for (int t = 0; t < R; t++) dst[off + (base + t * Ns) * IS] = v[t];
     }
 }
 
-// Compile-time Stockham plan: radices 8, 4, 3, 2 (largest first).
-template <int L> struct splan {
-    static constexpr int R0 = (L % 8 == 0 && L >= 8) ? 8 : (L % 4 == 0) ? 4 : (L % 3 == 0) ? 3 : 2;
-};
-
-template <int L, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN>
-__device__ __forceinline__ double2 *cfft(double2 *a, double2 *b, const double2 *tw, int tid)
+template <int L, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, typename C>
+__device__ __forceinline__ C *cfft(C *a, C *b, const C *tw, int tid)
 {
     if constexpr (Ns >= L) {
         return a;
     } else {
         constexpr int rem = L / Ns;
         constexpr int R = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 3 == 0) ? 3 : 2;
-        cstage<L, R, Ns, P, QD, QH, QL, IS, SIGN>(a, b, tw, tid);
+        cstage<L, R, Ns, P, QD, QH, QL, IS, SIGN, C>(a, b, tw, tid);
         __syncthreads();
-        return cfft<L, Ns * R, P, QD, QH, QL, IS, SIGN>(b, a, tw, tid);
+        return cfft<L, Ns * R, P, QD, QH, QL, IS, SIGN, C>(b, a, tw, tid);
     }
 }
 
@@ -221,32 +233,34 @@ struct HParams {
     int bdg;
 };
 
-template <int LY, int LZ>
+template <int LY, int LZ, bool SP>
 __global__ void __launch_bounds__(FT)
-f_hyz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__restrict__ hout,
+f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
       const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
-      const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double kys, double kzs, HParams hp,
+      const hc_t<SP> *__restrict__ twy_g, const hc_t<SP> *__restrict__ twz_g, double kys, double kzs, HParams hp,
       double *__restrict__ part)
 {
+    typedef hc_t<SP> C;
+    typedef decltype(C().x) R;
     constexpr int E = LY * LZ;              // points of one plane
     constexpr int KM = (E + FT - 1) / FT;   // owned points per thread
-    __shared__ double2 twy[LY], twz[LZ];
+    __shared__ C twy[LY], twz[LZ];
     __shared__ double red[FT / 64];
-    extern __shared__ double2 lds[];
-    double2 *A = lds, *B = lds + 2 * E;
+    extern __shared__ double2 lds_raw[];
+    C *A = (C *)lds_raw, *B = A + 2 * E;
     const int tid = threadIdx.x;
     for (int k = tid; k < LY; k += FT) twy[k] = twy_g[k];
     for (int k = tid; k < LZ; k += FT) twz[k] = twz_g[k];
     const int a = blockIdx.x / NX;
     const int x = blockIdx.x - a * NX;
     const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
-    double2 Xu[KM], Xv[KM], Ou[KM], Ov[KM];
+    C Xu[KM], Xv[KM], Ou[KM], Ov[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * FT;
         if (e < E) {
-            Xu[k] = y[gu + e];
-            Xv[k] = y[gv + e];
+            Xu[k] = cvt<C>(y[gu + e]);
+            Xv[k] = cvt<C>(y[gv + e]);
             Ou[k] = T[gu + e];
             Ov[k] = T[gv + e];
         }
@@ -264,22 +278,22 @@ f_hyz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
             }
         }
         __syncthreads();
-        double2 *F;
-        if (ax == 0) F = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, -1>(A, B, twz, tid);
-        else F = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, -1>(A, B, twy, tid);
-        double2 *G = (F == A) ? B : A;
+        C *F;
+        if (ax == 0) F = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, -1, C>(A, B, twz, tid);
+        else F = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, -1, C>(A, B, twy, tid);
+        C *G = (F == A) ? B : A;
         for (int e2 = tid; e2 < 2 * E; e2 += FT) {
             const int e = e2 < E ? e2 : e2 - E;
             int m;
             double ks;
             if (ax == 0) { const int i = e % LZ; m = (i < LZ / 2) ? i : i - LZ; ks = kzs; }
             else { const int i = e / LZ; m = (i < LY / 2) ? i : i - LY; ks = kys; }
-            F[e2] = F[e2] * (-ks * (double)(m * m));
+            F[e2] = F[e2] * (R)(-ks * (double)(m * m));
         }
         __syncthreads();
-        double2 *Rr;
-        if (ax == 0) Rr = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, +1>(F, G, twz, tid);
-        else Rr = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, +1>(F, G, twy, tid);
+        C *Rr;
+        if (ax == 0) Rr = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, +1, C>(F, G, twz, tid);
+        else Rr = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, +1, C>(F, G, twy, tid);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * FT;
@@ -289,7 +303,7 @@ f_hyz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
             }
         }
     }
-    // H psi on owned points
+    // H psi on owned points (pointwise terms and the sums in FP64)
     const double s = sc[a];
     double ssum = 0.0;
 #pragma unroll
@@ -297,16 +311,16 @@ f_hyz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
         const int e = tid + k * FT;
         if (e < E) {
             const long gi = (long)x * E + e;
-            const double2 pu = Xu[k] * s, pv = Xv[k] * s;
-            const double2 lu = Ou[k] * s, lv = Ov[k] * s;
+            const double2 pu = to64(Xu[k]) * s, pv = to64(Xv[k]) * s;
+            const double2 lu = to64(Ou[k]) * s, lv = to64(Ov[k]) * s;
             const double aa = hp.bdg ? -0.5 : pots[4 * N + gi] * -0.5;
             const double ab = hp.bdg ? -0.5 : pots[5 * N + gi] * -0.5;
             const double Va = pots[gi] - hp.mu_a, Vb = pots[N + gi] - hp.mu_b;
             const double2 D = make_double2(pots[2 * N + 2 * gi], pots[2 * N + 2 * gi + 1]);
             const double2 hu = pu * Va + cmul(pv, D) + lu * aa;
             const double2 hv = pv * (-Vb) + cmulc(D, pu) - lv * ab;
-            hout[gu + e] = hu;
-            hout[gv + e] = hv;
+            hout[gu + e] = cvt<C>(hu);
+            hout[gv + e] = cvt<C>(hv);
             ssum += cmulc(pu, hu).x + cmulc(pv, hv).x;
         }
     }
@@ -326,8 +340,9 @@ __global__ void f_e_final(const double *__restrict__ part, int n, int per, doubl
 }
 
 // y *= s ; f0 -= e * y   (applies the pending normalisation and <H> subtraction)
+template <bool SP>
 __global__ void __launch_bounds__(FT)
-f_fixup(double2 *__restrict__ y, double2 *__restrict__ f0, const double *__restrict__ s,
+f_fixup(double2 *__restrict__ y, hc_t<SP> *__restrict__ f0, const double *__restrict__ s,
         const double *__restrict__ e, int n, long N, int do_f)
 {
     const long i = (long)blockIdx.x * FT + threadIdx.x;
@@ -335,7 +350,14 @@ f_fixup(double2 *__restrict__ y, double2 *__restrict__ f0, const double *__restr
     const long a = (i / N) % n;
     const double2 v = y[i] * s[a];
     y[i] = v;
-    if (do_f) f0[i] = f0[i] - v * e[a];
+    if (do_f) f0[i] = cvt<hc_t<SP>>(to64(f0[i]) - v * e[a]);
+}
+
+// FP64 -> FP32 copy (the measurement step's exact H psi into the split history)
+__global__ void __launch_bounds__(FT) f_to32(const double2 *__restrict__ src, float2 *__restrict__ dst, long tot)
+{
+    const long i = (long)blockIdx.x * FT + threadIdx.x;
+    if (i < tot) dst[i] = to32(src[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -372,68 +394,83 @@ void fused_release(tdw_ctx *c)
     c->d_sp = c->d_s2 = c->d_ep = nullptr;
 }
 
-typedef void (*pred_fn)(double2 *, double2 *, const double2 *, const double2 *, const double2 *, const double2 *,
-                        double2 *, double2 *, const double *, const double *, int, int, long, int, double, double,
-                        int, double *);
-typedef void (*corr_fn)(double2 *, const double2 *, const double2 *, double2 *, const double *, const double *, int,
-                        long, int, double, double, int, double *);
-typedef void (*hyz_fn)(const double2 *, const double2 *, double2 *, const double *, const double *, int, long, int,
-                       const double2 *, const double2 *, double, double, HParams, double *);
+template <bool SP> struct FusedFns {
+    typedef hc_t<SP> HC;
+    typedef void (*pred_fn)(double2 *, HC *, const HC *, const HC *, const HC *, const HC *, HC *, HC *,
+                            const double *, const double *, int, int, long, int, double, double, int, double *);
+    typedef void (*corr_fn)(double2 *, const HC *, const HC *, HC *, const double *, const double *, int, long, int,
+                            double, double, int, double *);
+    typedef void (*hyz_fn)(const double2 *, const HC *, HC *, const double *, const double *, int, long, int,
+                           const HC *, const HC *, double, double, HParams, double *);
 
-template <int LX> static pred_fn pick_pred(int sch)
-{
-    if (sch == TDW_AB3AM4) return f_pred<LX, 34>;
-    if (sch == TDW_AB4AM5) return f_pred<LX, 45>;
-    return f_pred<LX, 55>;
-}
-
-static bool xpass_fns(int nx, int sch, pred_fn *p, corr_fn *cf)
-{
-    switch (nx) {
-    case 8: *p = pick_pred<8>(sch); *cf = f_corr<8>; return true;
-    case 12: *p = pick_pred<12>(sch); *cf = f_corr<12>; return true;
-    case 16: *p = pick_pred<16>(sch); *cf = f_corr<16>; return true;
-    case 24: *p = pick_pred<24>(sch); *cf = f_corr<24>; return true;
-    case 32: *p = pick_pred<32>(sch); *cf = f_corr<32>; return true;
-    default: return false;
+    template <int LX> static pred_fn pick_pred(int sch)
+    {
+        if (sch == TDW_AB3AM4) return f_pred<LX, 34, SP>;
+        if (sch == TDW_AB4AM5) return f_pred<LX, 45, SP>;
+        return f_pred<LX, 55, SP>;
     }
-}
 
-static bool hyz_fn_for(int ny, int nz, hyz_fn *h)
+    static bool xpass(int nx, int sch, pred_fn *p, corr_fn *cf)
+    {
+        switch (nx) {
+        case 8: *p = pick_pred<8>(sch); *cf = f_corr<8, SP>; return true;
+        case 12: *p = pick_pred<12>(sch); *cf = f_corr<12, SP>; return true;
+        case 16: *p = pick_pred<16>(sch); *cf = f_corr<16, SP>; return true;
+        case 24: *p = pick_pred<24>(sch); *cf = f_corr<24, SP>; return true;
+        case 32: *p = pick_pred<32>(sch); *cf = f_corr<32, SP>; return true;
+        default: return false;
+        }
+    }
+
+    static bool hyz(int ny, int nz, hyz_fn *h)
+    {
+        if (ny == 12 && nz == 12) { *h = f_hyz<12, 12, SP>; return true; }
+        if (ny == 24 && nz == 24) { *h = f_hyz<24, 24, SP>; return true; }
+        if (ny == 32 && nz == 32) { *h = f_hyz<32, 32, SP>; return true; }
+        if (ny == 16 && nz == 16) { *h = f_hyz<16, 16, SP>; return true; }
+        if (ny == 8 && nz == 8) { *h = f_hyz<8, 8, SP>; return true; }
+        if (ny == 8 && nz == 12) { *h = f_hyz<8, 12, SP>; return true; }
+        if (ny == 12 && nz == 6) { *h = f_hyz<12, 6, SP>; return true; }
+        return false;
+    }
+};
+
+static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
+
+static size_t hyz_lds(const tdw_ctx *c)
 {
-    if (ny == 12 && nz == 12) { *h = f_hyz<12, 12>; return true; }
-    if (ny == 24 && nz == 24) { *h = f_hyz<24, 24>; return true; }
-    if (ny == 32 && nz == 32) { *h = f_hyz<32, 32>; return true; }
-    if (ny == 16 && nz == 16) { *h = f_hyz<16, 16>; return true; }
-    if (ny == 8 && nz == 8) { *h = f_hyz<8, 8>; return true; }
-    if (ny == 8 && nz == 12) { *h = f_hyz<8, 12>; return true; }
-    if (ny == 12 && nz == 6) { *h = f_hyz<12, 6>; return true; }
-    return false;
+    return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 4 * (size_t)c->ny * c->nz;
 }
 
 bool fused_supported(const tdw_ctx *c)
 {
-    if (c->dim != 3 || c->p.precision != TDW_FP64) return false;
-    pred_fn p;
-    corr_fn cf;
-    hyz_fn h;
-    if (!xpass_fns(c->nx, c->p.scheme, &p, &cf)) return false;
-    if (!hyz_fn_for(c->ny, c->nz, &h)) return false;
-    size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
-    return lds <= (size_t)c->max_lds;
+    if (c->dim != 3) return false;
+    if (c->p.precision != TDW_FP64 && c->p.precision != TDW_SPLIT) return false;
+    FusedFns<false>::pred_fn p;
+    FusedFns<false>::corr_fn cf;
+    FusedFns<false>::hyz_fn h;
+    if (!FusedFns<false>::xpass(c->nx, c->p.scheme, &p, &cf)) return false;
+    if (!FusedFns<false>::hyz(c->ny, c->nz, &h)) return false;
+    return hyz_lds(c) <= (size_t)c->max_lds;
 }
 
 static int nhist(const tdw_ctx *c) { return c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5; }
 
+template <bool SP> static int fixup(tdw_ctx *c, double2 *y, void *f0, int n, int do_f)
+{
+    const long tot = 2L * n * (long)c->N;
+    TDW_TIMED(c, "f_fixup");
+    hipLaunchKernelGGL(f_fixup<SP>, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, y, (hc_t<SP> *)f0,
+                       c->d_scale, c->d_qpe, n, (long)c->N, do_f);
+    return (int)hipGetLastError();
+}
+
 int fused_finalize(tdw_ctx *c, tdw_state *s)
 {
     if (!c->pending) return 0;
-    const long tot = 2L * s->n * (long)c->N;
-    TDW_TIMED(c, "f_fixup");
-    hipLaunchKernelGGL(f_fixup, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, (double2 *)s->wf,
-                       (double2 *)s->f[0], c->d_scale, c->d_qpe, s->n, (long)c->N, 1);
     c->pending = 0;
-    return (int)hipGetLastError();
+    if (split_mode(c)) return fixup<true>(c, (double2 *)s->wf, s->f[0], s->n, 1);
+    return fixup<false>(c, (double2 *)s->wf, s->f[0], s->n, 1);
 }
 
 static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave)
@@ -445,20 +482,24 @@ static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave
     return (int)hipGetLastError();
 }
 
-static int hyz(tdw_ctx *c, int n, const double2 *y, const double2 *T, double2 *out, const double *pots,
+template <bool SP>
+static int hyz(tdw_ctx *c, int n, const double2 *y, const hc_t<SP> *T, hc_t<SP> *out, const double *pots,
                const double *scale, double *e_out, double *e_out2)
 {
-    hyz_fn h;
-    hyz_fn_for(c->ny, c->nz, &h);
-    const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+    typename FusedFns<SP>::hyz_fn h;
+    FusedFns<SP>::hyz(c->ny, c->nz, &h);
+    const size_t lds = hyz_lds(c);
     HParams hp{c->p.mu_a, c->p.mu_b, c->p.functional == TDW_BDG};
     const double kys = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
     const double kzs = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
+    const hc_t<SP> *twy, *twz;
+    if constexpr (SP) { twy = c->d_tw32[1]; twz = c->d_tw32[2]; }
+    else { twy = c->d_tw[1]; twz = c->d_tw[2]; }
     // the norm partials are consumed already: reuse the partial buffer
     {
         TDW_TIMED(c, "f_hyz");
         hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(FT), lds, c->stream, y, T, out, pots, scale, n,
-                           (long)c->N, c->nx, c->d_tw[1], c->d_tw[2], kys, kzs, hp, c->d_fpart);
+                           (long)c->N, c->nx, twy, twz, kys, kzs, hp, c->d_fpart);
         TDW_HIP(hipGetLastError());
     }
     TDW_TIMED(c, "f_e");
@@ -467,12 +508,9 @@ static int hyz(tdw_ctx *c, int n, const double2 *y, const double2 *T, double2 *o
     return (int)hipGetLastError();
 }
 
-int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 {
-    if (!fused_supported(c)) {
-        TDW_TRY(fused_finalize(c, s));
-        return tdw_step(c, s, it, grad_last);
-    }
+    typedef hc_t<SP> HC;
     const int n = s->n, h = nhist(c);
     const long N = (long)c->N;
     const int S = c->S;
@@ -480,13 +518,16 @@ int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(ensure_state_vecs(c));
     TDW_TRY(ensure_f(c, wave ? (size_t)2 * n * (S / 64) : (size_t)2 * n * S));
     TDW_TRY(ensure_f(c, (size_t)n * c->nx));
-    pred_fn fp;
-    corr_fn fc;
-    xpass_fns(c->nx, c->p.scheme, &fp, &fc);
+    typename FusedFns<SP>::pred_fn fp;
+    typename FusedFns<SP>::corr_fn fc;
+    FusedFns<SP>::xpass(c->nx, c->p.scheme, &fp, &fc);
     const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
     const long nblk = (2L * n * S + FT - 1) / FT;
-    double2 *y = (double2 *)s->wf, *T = (double2 *)s->ddx, *H = (double2 *)s->lap;
-    double2 **f = (double2 **)s->f;
+    // T (D_x of the x-pass output) lives in ddx, H of the prediction in lap;
+    // in the split form both hold FP32 values
+    double2 *y = (double2 *)s->wf;
+    HC *T = (HC *)s->ddx, *H = (HC *)s->lap;
+    HC **f = (HC **)s->f;
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
 
     // ---- predictor ----
@@ -500,7 +541,7 @@ int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(xpass_norm(c, n, c->d_sp, c->d_s2, wave));
     TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt, s->dens, 0));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
-    TDW_TRY(hyz(c, n, y, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
+    TDW_TRY(hyz<SP>(c, n, y, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
 
     // ---- corrector ----
     const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
@@ -518,19 +559,13 @@ int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
         tdw_complex *tmp = s->f[h - 1];
         for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
         s->f[0] = tmp;
-        TDW_TRY(hyz(c, n, y, T, (double2 *)s->f[0], s->pots, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
+        TDW_TRY(hyz<SP>(c, n, y, T, (HC *)s->f[0], s->pots, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
         c->pending = 1;
         return 0;
     }
     // measurement step: normalise in place, then the reference's derivative /
-    // density / tau / potential / H sequence
-    {
-        const long tot = 2L * n * N;
-        TDW_TIMED(c, "f_fixup");
-        hipLaunchKernelGGL(f_fixup, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, y, (double2 *)nullptr,
-                           c->d_scale, c->d_qpe, n, N, 0);
-        TDW_HIP(hipGetLastError());
-    }
+    // density / tau / potential / H sequence, in FP64 in both forms
+    TDW_TRY(fixup<SP>(c, y, nullptr, n, 0));
     TDW_TRY(derivatives(c, 2 * n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap));
     TDW_TRY(densities(c, n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap,
                       c->d_fbeta, nullptr, c->d_kt, c->d_cnt, s->dens, 1));
@@ -539,8 +574,31 @@ int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     tdw_complex *tmp = s->f[h - 1];
     for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
-    TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, s->pots, c->d_kt, nullptr, qpe));
+    if constexpr (SP) {
+        // exact H psi - <H> psi into ddx (free after the densities), then FP32 history
+        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->ddx, (double2 *)s->lap, s->pots, c->d_kt, nullptr, qpe));
+        const long tot = 2L * n * N;
+        TDW_TIMED(c, "f_to32");
+        hipLaunchKernelGGL(f_to32, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, (const double2 *)s->ddx,
+                           (float2 *)s->f[0], tot);
+        TDW_HIP(hipGetLastError());
+    } else {
+        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, s->pots, c->d_kt, nullptr, qpe));
+    }
     return 0;
+}
+
+int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    if (!fused_supported(c)) {
+        // the split form exists only as the fused step: its history arrays
+        // hold FP32 values, which the FP64 step cannot continue from
+        if (split_mode(c)) return TDW_EINVAL;
+        TDW_TRY(fused_finalize(c, s));
+        return tdw_step(c, s, it, grad_last);
+    }
+    if (split_mode(c)) return fused_step_t<true>(c, s, it, grad_last);
+    return fused_step_t<false>(c, s, it, grad_last);
 }
 
 }  // namespace tdw

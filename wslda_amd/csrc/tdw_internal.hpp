@@ -56,6 +56,8 @@ struct tdw_ctx {
     double *d_sp, *d_s2, *d_ep;          // n: predictor scale, scale^2, predictor <H>
     double *d_fpart;  size_t fpart_cap;  // partial sums of the fused kernels
     int pending;                         // y and f[0] carry a pending scale / <H> fix-up
+    // adjoint
+    double *d_adjv;                      // 4n per-state scalars of the backward step
     // per-kernel timing
     int timing;
     std::vector<tdw_timer_rec> *timers;  // pending records

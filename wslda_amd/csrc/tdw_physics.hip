@@ -330,42 +330,6 @@ int density_tau(tdw_ctx *c, double *dens)
 // ---------------------------------------------------------------------------
 // external potential of the built-in problem
 // ---------------------------------------------------------------------------
-struct VextDev {
-    tdw_vext v;
-    double s;          // switch value at this time
-    int dim, nx, ny, nz;
-    double dx, dy, dz;
-};
-
-__device__ __forceinline__ void decode(int dim, int ny, int nz, long i, int &ix, int &iy, int &iz)
-{
-    if (dim == 1) { ix = (int)i; iy = 0; iz = 0; }
-    else if (dim == 2) { ix = (int)(i / ny); iy = (int)(i - (long)ix * ny); iz = 0; }
-    else {
-        ix = (int)(i / ((long)ny * nz));
-        long r = i - (long)ix * ny * nz;
-        iy = (int)(r / nz);
-        iz = (int)(r - (long)iy * nz);
-    }
-}
-
-__device__ __forceinline__ void vext_point(const VextDev &V, long i, double &va, double &vb)
-{
-    int ix, iy, iz;
-    decode(V.dim, V.ny, V.nz, i, ix, iy, iz);
-    const double x = V.dx * (ix - V.nx / 2);
-    const double y = V.dim >= 2 ? V.dy * (iy - V.ny / 2) : 0.0;
-    const double z = V.dim >= 3 ? V.dz * (iz - V.nz / 2) : 0.0;
-    double g = 0.0;
-    for (int k = 0; k < V.v.ngauss; k++) {
-        const double X = x - V.v.cx[k], Y = y - V.v.cy[k], Z = z - V.v.cz[k];
-        g += V.v.amp[k] * V.s * exp(V.v.wx[k] * (X * X) + V.v.wy[k] * (Y * Y) + V.v.wz[k] * (Z * Z));
-    }
-    const double trap = 0.5 * V.v.omega * V.v.omega * (x * x + y * y + z * z);
-    va = g + trap;
-    vb = (V.v.flip_b ? -g : g) + trap;
-}
-
 static double switch_function(double t, double T, double alpha)
 {
     return 0.5 * (1.0 + tanh(alpha * tan(M_PI_2 * (2.0 * t / T - 1.0))));
@@ -378,7 +342,7 @@ static double smooth_step(double t, double t0, double t1, double T, double alpha
     return 1.0 - switch_function(t - t1 + T, T, alpha);
 }
 
-static VextDev make_vext(const tdw_ctx *c, int it)
+VextDev make_vext(const tdw_ctx *c, int it)
 {
     VextDev V;
     V.v = c->p.vext;

@@ -140,3 +140,59 @@ def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, devi
     eF = 0.5
     return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
                 name="3D uniform SLDA %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
+
+
+def select_states(wf, En, kt, cnt, n_states, dim):
+    """The n_states lowest-|E| states of a plane-wave set (kept in their
+    original order), with their transverse momenta / degeneracies."""
+    nall = En.size
+    order = np.sort(np.argsort(np.abs(En), kind="stable")[:n_states])
+    wfs = np.concatenate([wf[order], wf[nall + order]])
+    kts = None
+    if kt is not None:
+        kts = kt[order] if dim == 2 else np.concatenate([kt[order], kt[nall + order]])
+    cnts = None if cnt is None else cnt[order]
+    return wfs, En[order], kts, cnts
+
+
+def uniform_start(dim, shape, dx=1.0, n_states=None, scheme=td.AB5AM5, dt_eF=0.005, T=1.0e-9):
+    """Self-consistent uniform SLDA state at kF = 1 on a 1D/2D/3D lattice
+    (transverse box from shape for dim < 3) and its plane-wave states (all,
+    or the n_states of lowest |E|), host arrays: params, wf [2n, N], pots
+    (12N), fbeta, kt, cnt."""
+    nx, ny, nz = shape
+    p = _base_params(dim, nx, ny, nz, dx, scheme=scheme, dt_eF=dt_eF)
+    u = uniform_slda(p, T=T)
+    p.mu_a, p.mu_b, p.ec = u.mu_a, u.mu_b, u.ec
+    wf, En, kt, cnt = td.uniform_states(p, u)
+    if n_states is not None and n_states < En.size:
+        wf, En, kt, cnt = select_states(wf, En, kt, cnt, n_states, dim)
+    N = wf.shape[1]
+    pots = np.zeros(12 * N)
+    pots[:N] = u.V_a
+    pots[N:2 * N] = u.V_b
+    pots[2 * N:4 * N:2] = u.delta
+    pots[4 * N:6 * N] = 1.0
+    fb = np.array([td.fbeta(e, u.beta) for e in En])
+    return dict(params=p, wf=wf, pots=pots, fbeta=fb, kt=kt, cnt=cnt, uniform=u, En=En)
+
+
+def config4(n_states=4096, seed=4, shape=(64, 64, 8), dx=1.0):
+    """2D SLDA on a 64x64 lattice (transverse box of 8 points), the 4096
+    lowest-|E| plane-wave states of the uniform solution, two Gaussian beams
+    with parameters (amplitude, x, y, width) drawn uniformly from the
+    physical range (seeded), and the particle number inside a disk of radius
+    10 dx as the observable."""
+    s = uniform_start(2, shape, dx=dx, n_states=n_states)
+    p = s["params"]
+    eF = 0.5
+    rng = np.random.default_rng(seed)
+    L = shape[0] * dx
+    theta = []
+    for _ in range(2):
+        theta += [rng.uniform(0.05, 0.2) * eF, rng.uniform(-0.25, 0.25) * L, rng.uniform(-0.25, 0.25) * L,
+                  rng.uniform(2.0, 5.0) * dx]
+    s["theta"] = np.array(theta)
+    s["center"] = (0.0, 0.0)
+    s["radius"] = 10.0 * dx
+    return s

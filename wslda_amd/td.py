@@ -20,6 +20,7 @@ SLDA, BDG = 111, 113
 SPHERICAL_CUTOFF, CUBIC_CUTOFF = 88, 89
 AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
 FP64, SPLIT = 0, 1
+TDW_EINVAL, TDW_ENOCTX, TDW_ESIZE, TDW_ENOMEM = -1001, -1002, -1003, -1004
 EKIN, EPOT, EPAIR, ECURRENT, EPOTEXT, EPAIREXT, EVELEXT, NPARTA, NPARTB, LZA, LZB = range(11)
 NITEMS = 11
 MAX_GAUSS = 4
@@ -64,6 +65,16 @@ class State(C.Structure):
                 ("ddy", C.c_void_p), ("ddz", C.c_void_p),
                 ("f", C.c_void_p * 5),
                 ("dens", C.c_void_p), ("pots", C.c_void_p), ("qpe", C.c_void_p)]
+
+
+class TapeStep(C.Structure):
+    _fields_ = [("n", C.c_int), ("scal", C.c_void_p), ("dens_p", C.c_void_p), ("pots_p", C.c_void_p),
+                ("dens_c", C.c_void_p), ("pots_c", C.c_void_p)]
+
+
+class AdjState(C.Structure):
+    _fields_ = [("n", C.c_int), ("ly", C.c_void_p), ("lf", C.c_void_p * 5), ("lpots", C.c_void_p),
+                ("w", C.c_void_p * 5), ("lgrid", C.c_void_p), ("theta_grad", C.c_void_p)]
 
 
 def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
@@ -131,6 +142,9 @@ def _sigs():
         "tdw_finalize": (I, [P, C.POINTER(State)]),
         "tdw_fused_supported": (I, [P]),
         "tdw_initial_measure": (I, [P, C.POINTER(State)]),
+        "tdw_adj_forward_step": (I, [P, C.POINTER(State), I, P, P, C.POINTER(TapeStep)]),
+        "tdw_adj_backward_step": (I, [P, C.POINTER(AdjState), I, P, C.POINTER(C.c_void_p), P, P,
+                                      C.POINTER(TapeStep), P, P]),
         "tdw_uniform_solve": (I, [C.POINTER(Params), D, D, D, D, D, D, D, D, D, I, I, C.POINTER(Uniform)]),
         "tdw_uniform_count": (I, [C.POINTER(Params), C.POINTER(Uniform)]),
         "tdw_uniform_states": (I, [C.POINTER(Params), C.POINTER(Uniform), I, I, P, P, P, P]),
@@ -394,7 +408,12 @@ class TDRun:
         self.ddx = z()
         self.ddy = z() if ctx.params.dim >= 2 else None
         self.ddz = z() if ctx.params.dim >= 3 else None
-        self.f = [z() for _ in range(5)]
+        # split precision: the history entries hold FP32 values
+        self.split = ctx.params.precision == SPLIT
+        if self.split:
+            self.f = [torch.zeros((2 * n, N), dtype=torch.complex64, device=dev) for _ in range(5)]
+        else:
+            self.f = [z() for _ in range(5)]
         self.dens = torch.zeros(12 * N, dtype=torch.float64, device=dev)
         if pots is None:
             pots = np.zeros(12 * N)
