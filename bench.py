@@ -235,6 +235,67 @@ def density_errors(d, ref, N):
     return out
 
 
+def run_adjoint(args):
+    """2D SLDA 64x64 with 4096 states, two Gaussian beams, particle number in
+    a disk of radius 10 dx after args.adj_steps steps, checkpoints every
+    args.adj_k steps: time of the plain forward run, of the checkpointed
+    forward + adjoint sweep, and a central-difference check (h = 1e-6) of
+    two beam parameters."""
+    import numpy as np
+    import torch
+    from wslda_amd import adjoint, problems
+    s = problems.config4(n_states=args.adj_states)
+    p = s["params"]
+    theta = s["theta"]
+    g = adjoint.disk_weights(p, s["center"], s["radius"])
+    prob = adjoint.AdjointProblem(p, s["wf"], s["pots"], s["fbeta"], s["kt"], s["cnt"], nsteps=args.adj_steps,
+                                  K=args.adj_k, dens_grad=g)
+    prob.set_vext(adjoint.beams_vext(theta))
+    prob.forward(keep_checkpoints=False)     # warm-up
+    torch.cuda.synchronize()
+    t0 = time.time()
+    prob.forward(keep_checkpoints=False)
+    torch.cuda.synchronize()
+    t_fwd = time.time() - t0
+    O = prob.observable()
+    t0 = time.time()
+    prob.forward(keep_checkpoints=True)
+    torch.cuda.synchronize()
+    t_fwd_ck = time.time() - t0
+    t0 = time.time()
+    gslot = prob.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.time() - t0
+    grad = adjoint.beams_grad(theta, gslot)
+    fd = {}
+    h = 1e-6
+    for idx, name in ((0, "A_1"), (3, "width_1"), (5, "x_2")):
+        vals = []
+        for sg in (1, -1):
+            th = theta.copy()
+            th[idx] += sg * h
+            prob.set_vext(adjoint.beams_vext(th))
+            prob.forward(keep_checkpoints=False)
+            vals.append(prob.observable())
+        fd[name] = {"adjoint": float(grad[idx]), "fd": (vals[0] - vals[1]) / (2 * h)}
+    gmax = float(np.abs(grad).max())
+    err = max(abs(v["adjoint"] - v["fd"]) for v in fd.values()) / gmax
+    n = prob.n
+    prob.close()
+    return {
+        "metric": "adjoint gradient cost / forward cost, and gradient error vs central finite differences",
+        "value": (t_fwd_ck + t_bwd) / t_fwd, "unit": "x forward", "higher_is_better": False,
+        "n_gpus": 1, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "td-SLDA 2D 64x64 lattice (8-point transverse box), %d qp states, AB5AM5, two "
+                               "Gaussian beams (seeded), N in a disk r < 10 dx after %d steps, checkpoints every %d"
+                               % (n, args.adj_steps, args.adj_k)},
+        "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "backward_s": t_bwd,
+        "backward_over_forward": t_bwd / t_fwd, "observable": O,
+        "grad_theta": [float(x) for x in grad], "fd_check": fd, "max_rel_err_vs_fd": err,
+        "qp_steps_per_s_forward": n * args.adj_steps / t_fwd,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,11 +318,19 @@ def main():
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a plain copy kernel "
                          "of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="instead of the step benchmark: the adjoint workload (2D 64x64, 4096 states, two beams)")
+    ap.add_argument("--adj-steps", type=int, default=500)
+    ap.add_argument("--adj-k", type=int, default=50)
+    ap.add_argument("--adj-states", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
                          "GPUs is a design, not code); run with --gpus 1\n")
         sys.exit(2)
+    if args.adjoint:
+        print(json.dumps(run_adjoint(args)))
+        return
     from wslda_amd import td
     primary = "split" if args.mode == "split" else "fp64"
     g = run_gpu(args, td.SPLIT if primary == "split" else td.FP64)
