@@ -296,6 +296,69 @@ def run_adjoint(args):
     }
 
 
+def run_trap(args):
+    """Split precision on the trapped vortex (32^3, 24576 states): throughput
+    of the full workload, and the split run's density / energy errors against
+    an FP64 run of the same steps on its lowest 4096 states (both forms of
+    the full workload do not fit one GPU's HBM side by side)."""
+    import numpy as np
+    import torch
+    from wslda_amd import problems, td
+
+    def run(prec, nst, steps, timed):
+        w = problems.config3(torch, precision=prec, n_states=nst)
+        p = w["params"]
+        n = w["wf"].shape[0] // 2
+        ctx = td.TDContext(p, n, 0)
+        ctx.set_states(w["fbeta"])
+        R = td.TDRun(ctx, w["wf"], w["pots"], fused=True)
+        del w["wf"]
+        R.initial_measure()
+        for it in range(3):
+            R.step(it, grad_last=(it == 2))
+        ctx.sync()
+        e0 = R.energy(3)
+        stream = torch.cuda.ExternalStream(ctx.stream_ptr())
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if timed:
+            ctx.timing(True)
+            ctx.timing_report()
+        ev0.record(stream)
+        for it in range(3, 3 + steps):
+            R.step(it, grad_last=((it + 1) % args.meas_every == 0) or it == 2 + steps)
+        R.finalize()
+        ev1.record(stream)
+        ev1.synchronize()
+        ms = ev0.elapsed_time(ev1)
+        rep = ctx.timing_report() if timed else {}
+        ctx.timing(False)
+        e1 = R.energy(3 + steps)
+        d = R.dens.cpu().numpy()
+        N = ctx.N
+        ctx.close()
+        return dict(qps=n * steps / (ms / 1e3), ms=ms / steps, n=n, N=N, dens=d, e0=e0, e1=e1, rep=rep)
+
+    full = run(td.SPLIT, args.trap_states, args.steps, True)
+    sub64 = run(td.FP64, 4096, args.steps, False)
+    subsp = run(td.SPLIT, 4096, args.steps, False)
+    err = density_errors(subsp["dens"], sub64["dens"], sub64["N"])
+    E64, Esp = sub64["e1"][:7].sum(), subsp["e1"][:7].sum()
+    drift = lambda r: abs(r["e1"][:7].sum() - r["e0"][:7].sum()) / abs(r["e0"][:7].sum())
+    return {
+        "metric": "qp-wavefunction steps/s, split precision, trapped vortex",
+        "value": full["qps"], "unit": "qp-steps/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": full["ms"],
+        "higher_is_better": True, "dtype": "f32-operator/f64-state", "data": "synthetic",
+        "config": {"workload": "td-SLDA 3D 32^3, harmonic trap omega = 0.1 eF, vortex line imprinted on u, "
+                               "%d Gaussian-envelope plane-wave BdG states (seeded noise), AB5AM5, split precision"
+                               % full["n"]},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(full["rep"].items(), key=lambda kv: -kv[1][0])},
+        "errors_vs_fp64_on_4096_states": {"density_rel_err": err, "max_density_rel_err": max(err.values()),
+                                          "energy_rel_diff": abs(Esp - E64) / abs(E64),
+                                          "fp64_qps_4096": sub64["qps"], "split_qps_4096": subsp["qps"],
+                                          "energy_drift_fp64": drift(sub64), "energy_drift_split": drift(subsp)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +386,9 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--adj-k", type=int, default=50)
     ap.add_argument("--adj-states", type=int, default=4096)
+    ap.add_argument("--trap", action="store_true",
+                    help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
+    ap.add_argument("--trap-states", type=int, default=24576)
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
@@ -330,6 +396,9 @@ def main():
         sys.exit(2)
     if args.adjoint:
         print(json.dumps(run_adjoint(args)))
+        return
+    if args.trap:
+        print(json.dumps(run_trap(args)))
         return
     from wslda_amd import td
     primary = "split" if args.mode == "split" else "fp64"

@@ -196,3 +196,53 @@ def config4(n_states=4096, seed=4, shape=(64, 64, 8), dx=1.0):
     s["center"] = (0.0, 0.0)
     s["radius"] = 10.0 * dx
     return s
+
+
+def config3(torch, precision=td.SPLIT, n_states=24576, seed=3, device="cuda:0", shape=(32, 32, 32), dx=1.0,
+            omega_eF=0.1, envelope_frac=0.3, noise=1.0e-3):
+    """3D SLDA in a harmonic trap with an imprinted vortex line: 32^3
+    lattice, trap 0.5 omega^2 r^2 with omega = 0.1 eF, the n_states lowest-|E|
+    plane-wave BdG states of the uniform solution multiplied by a Gaussian
+    envelope exp(-r^2 / (2 R^2)) (R = envelope_frac * L) with seeded
+    random-phase amplitude noise, and the phase exp(i phi) of a vortex line
+    along z imprinted on the u components (so Delta ~ exp(i phi))."""
+    nx, ny, nz = shape
+    eF = 0.5
+    vext = td.gauss_vext([], omega=omega_eF * eF)
+    p = _base_params(3, nx, ny, nz, dx, precision=precision, vext=vext)
+    u = uniform_slda(p)
+    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf)
+    order = np.sort(np.argsort(np.abs(E), kind="stable")[:n_states])
+    K, E, U, V = K[order], E[order], U[order], V[order]
+    X = (torch.arange(nx, dtype=torch.float64, device=device) - nx // 2) * dx
+    Y = (torch.arange(ny, dtype=torch.float64, device=device) - ny // 2) * dx
+    Z = (torch.arange(nz, dtype=torch.float64, device=device) - nz // 2) * dx
+    gx, gy, gz = torch.meshgrid(X, Y, Z, indexing="ij")
+    R = envelope_frac * nx * dx
+    env = torch.exp(-(gx ** 2 + gy ** 2 + gz ** 2) / (2 * R * R)).reshape(-1)
+    # phase of a vortex line through the lattice centre, offset by half a
+    # cell so that no lattice point sits on the core
+    phi = torch.atan2(gy + 0.5 * dx, gx + 0.5 * dx).reshape(-1)
+    phase = torch.polar(torch.ones_like(phi), phi)
+    wf = states_on_gpu(torch, p, K, U, V, device=device, envelope=env.to(torch.complex128),
+                       phase=phase)
+    if noise > 0:
+        g = torch.Generator(device=device)
+        g.manual_seed(seed)
+        vol = (nx * dx) * (ny * dx) * (nz * dx)
+        chunk = 2048
+        for a0 in range(0, wf.shape[0], chunk):
+            rows = wf[a0:a0 + chunk]
+            th = torch.rand(rows.shape, generator=g, dtype=torch.float64, device=device) * (2 * PI)
+            rows += torch.polar(torch.full_like(th, noise / math.sqrt(vol)), th) * env
+            del th
+    p.mu_a, p.mu_b, p.ec = u.mu_a, u.mu_b, u.ec
+    N = nx * ny * nz
+    pots = np.zeros(12 * N)
+    pots[:N] = u.V_a
+    pots[N:2 * N] = u.V_b
+    pots[2 * N:4 * N:2] = u.delta
+    pots[4 * N:6 * N] = 1.0
+    fb = np.array([td.fbeta(e, u.beta) for e in E])
+    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
+                name="3D trapped SLDA with a vortex line %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
