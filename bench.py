@@ -249,7 +249,7 @@ def run_adjoint(args):
     theta = s["theta"]
     g = adjoint.disk_weights(p, s["center"], s["radius"])
     prob = adjoint.AdjointProblem(p, s["wf"], s["pots"], s["fbeta"], s["kt"], s["cnt"], nsteps=args.adj_steps,
-                                  K=args.adj_k, dens_grad=g)
+                                  K=args.adj_k, dens_grad=g, taped_segments=args.adj_taped)
     prob.set_vext(adjoint.beams_vext(theta))
     prob.forward(keep_checkpoints=False)     # warm-up
     torch.cuda.synchronize()
@@ -287,8 +287,9 @@ def run_adjoint(args):
         "value": (t_fwd_ck + t_bwd) / t_fwd, "unit": "x forward", "higher_is_better": False,
         "n_gpus": 1, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "td-SLDA 2D 64x64 lattice (8-point transverse box), %d qp states, AB5AM5, two "
-                               "Gaussian beams (seeded), N in a disk r < 10 dx after %d steps, checkpoints every %d"
-                               % (n, args.adj_steps, args.adj_k)},
+                               "Gaussian beams (seeded), N in a disk r < 10 dx after %d steps, checkpoints every %d, "
+                               "last %d segments taped in the forward run" % (n, args.adj_steps, args.adj_k,
+                                                                             args.adj_taped)},
         "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "backward_s": t_bwd,
         "backward_over_forward": t_bwd / t_fwd, "observable": O,
         "grad_theta": [float(x) for x in grad], "fd_check": fd, "max_rel_err_vs_fd": err,
@@ -306,6 +307,9 @@ def run_trap(args):
     from wslda_amd import problems, td
 
     def run(prec, nst, steps, timed):
+        sys.stderr.write("trap-vortex: %s, %d states, %d steps\n" % ("split" if prec == td.SPLIT else "fp64", nst,
+                                                                     steps))
+        sys.stderr.flush()
         w = problems.config3(torch, precision=prec, n_states=nst)
         p = w["params"]
         n = w["wf"].shape[0] // 2
@@ -386,6 +390,8 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--adj-k", type=int, default=50)
     ap.add_argument("--adj-states", type=int, default=4096)
+    ap.add_argument("--adj-taped", type=int, default=3,
+                    help="segments the forward run tapes directly (each 2K+1 state arrays of HBM)")
     ap.add_argument("--trap", action="store_true",
                     help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
     ap.add_argument("--trap-states", type=int, default=24576)

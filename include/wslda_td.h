@@ -172,6 +172,26 @@ int tdw_fused_supported(tdw_ctx *c);
 /* pca.c:939-975: normalise, derivatives, densities, tau (potentials kept). */
 int tdw_initial_measure(tdw_ctx *c, tdw_state *s);
 
+/* td checkpoint files (hpc-engine/tdwslda_checkpoint.c:118-364), byte layout
+ * of a single-rank run: header (t0, mu[2], ec, kF, nwf, eF, Effg, beta), then
+ * fbeta, kz (dim <= 2) and ky (dim 1) sections, wf, fkm1..fkm3 [fkm4]
+ * [fkm5], and the 12*N potentials.  Per-state sections are sized for nwf + 1
+ * states, as tdwslda_checkpoint.c sizes them for one rank.  f[0..4] of the
+ * state are written as fkm1..fkm5; save applies a pending fused-step fix-up
+ * first.  FP64 runs only.
+ *   tdw_checkpoint_nwf   -- load_nwf (tdwslda_checkpoint.c:22-59)
+ *   tdw_checkpoint_load  -- load_checkpoint (:61-220), into s (s->n == nwf)
+ *   tdw_checkpoint_save  -- save_checkpoint (:223-364)
+ * h_kt: the state's transverse momenta, as passed to tdw_set_states. */
+typedef struct {
+    double mu_a, mu_b, ec, kF, eF, Effg, beta;
+} tdw_checkpoint_scalars;
+int tdw_checkpoint_nwf(const char *path, int *nwf);
+int tdw_checkpoint_save(tdw_ctx *c, const char *path, tdw_state *s, double t0, const double *h_fbeta,
+                        const double *h_kt, const tdw_checkpoint_scalars *sc);
+int tdw_checkpoint_load(tdw_ctx *c, const char *path, tdw_state *s, double *t0, double *h_fbeta, double *h_kt,
+                        tdw_checkpoint_scalars *sc);
+
 /* ---------------------------------------------------------------------------
  * Adjoint (reverse-mode) time stepping.  No counterpart in the reference:
  * gradients of a final-time observable with respect to the external-potential

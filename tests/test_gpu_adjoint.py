@@ -24,14 +24,14 @@ def torch():
 
 
 CASES = [
-    # dim, shape, states, scheme, steps, K
-    (2, (16, 16, 8), 64, 55, 30, 8),
-    (3, (8, 8, 8), 48, 45, 20, 6),
-    (1, (32, 8, 8), 48, 34, 24, 10),
+    # dim, shape, states, scheme, steps, K, segments taped by the forward run
+    (2, (16, 16, 8), 64, 55, 30, 8, 2),
+    (3, (8, 8, 8), 48, 45, 20, 6, 1),
+    (1, (32, 8, 8), 48, 34, 24, 10, 1),
 ]
 
 
-def setup_problem(dim, shape, nst, scheme, steps, K):
+def setup_problem(dim, shape, nst, scheme, steps, K, taped=1):
     from wslda_amd import adjoint, problems
     s = problems.uniform_start(dim, shape, n_states=nst, scheme=scheme)
     p = s["params"]
@@ -43,7 +43,7 @@ def setup_problem(dim, shape, nst, scheme, steps, K):
         theta[[2, 6]] = 0.0
     g = adjoint.disk_weights(p, (0.0, 0.0, 0.0), 0.3 * L)
     prob = adjoint.AdjointProblem(p, s["wf"], s["pots"], s["fbeta"], s["kt"], s["cnt"], nsteps=steps, K=K,
-                                  dens_grad=g)
+                                  dens_grad=g, taped_segments=taped)
     return prob, theta
 
 
@@ -66,12 +66,11 @@ def observable_at(prob, v):
     return prob.observable()
 
 
-@pytest.mark.parametrize("dim,shape,nst,scheme,steps,K", CASES)
-def test_adjoint_vs_finite_differences(torch, dim, shape, nst, scheme, steps, K):
-    say("case: adjoint vs FD, dim=%d shape=%s states=%d scheme=%d steps=%d K=%d" % (dim, shape, nst, scheme,
-                                                                                      steps, K))
-    from wslda_amd import td
-    prob, theta = setup_problem(dim, shape, nst, scheme, steps, K)
+@pytest.mark.parametrize("dim,shape,nst,scheme,steps,K,taped", CASES)
+def test_adjoint_vs_finite_differences(torch, dim, shape, nst, scheme, steps, K, taped):
+    say("case: adjoint vs FD, dim=%d shape=%s states=%d scheme=%d steps=%d K=%d taped=%d" % (
+        dim, shape, nst, scheme, steps, K, taped))
+    prob, theta = setup_problem(dim, shape, nst, scheme, steps, K, taped)
     v0 = gauss_slots(theta, dim)
     prob.set_vext(v0)
     prob.forward()

@@ -31,10 +31,13 @@ class AdjointProblem:
     params: td.Params (SLDA, FP64); wf0: [2n, N] complex128 start state
     (normalised here); pots0: 12*N start potentials; fbeta/kt/cnt: state
     weights; nsteps: steps; K: checkpoint interval; it0: step index of the
-    start; dens_grad: 4*N observable weights (see module doc)."""
+    start; dens_grad: 4*N observable weights (see module doc);
+    taped_segments: how many of the last K-step segments the forward run
+    tapes directly (each holds 2K+1 state arrays), so the backward sweep
+    recomputes only the others."""
 
     def __init__(self, params, wf0, pots0, fbeta, kt=None, cnt=None, nsteps=100, K=10, it0=0, dens_grad=None,
-                 device=0):
+                 device=0, taped_segments=1):
         import torch
         self.torch = torch
         self.params = params
@@ -51,6 +54,9 @@ class AdjointProblem:
             np.ascontiguousarray(dens_grad, dtype=np.float64)).to(self.dev)
         self.ckpts = []
         self.run = None
+        self.taped = {}
+        self._tps = []
+        self.taped_segments = taped_segments
 
     # -- parameters -------------------------------------------------------
     def set_vext(self, vext):
@@ -65,16 +71,54 @@ class AdjointProblem:
                 R.f[j].copy_(hist[j])
         return R
 
+    def _tape(self, i=0):
+        """Tape buffer i: one K-step segment (allocated on first use)."""
+        while len(self._tps) <= i:
+            torch = self.torch
+            n, N, K = self.n, self.N, self.K
+            z = lambda: torch.zeros((2 * n, N), dtype=torch.complex128, device=self.dev)
+            ty = [z() for _ in range(K + 1)]
+            tf = [z() for _ in range(K)]
+            tscal = torch.zeros((K, 4 * n), dtype=torch.float64, device=self.dev)
+            tgrid = torch.zeros((K, 4, 12 * N), dtype=torch.float64, device=self.dev)
+            tapes = []
+            for r in range(K):
+                t = td.TapeStep()
+                t.n = n
+                t.scal = tscal[r].data_ptr()
+                t.dens_p, t.pots_p = tgrid[r, 0].data_ptr(), tgrid[r, 1].data_ptr()
+                t.dens_c, t.pots_c = tgrid[r, 2].data_ptr(), tgrid[r, 3].data_ptr()
+                tapes.append(t)
+            self._tps.append((ty, tf, tscal, tgrid, tapes))
+        return self._tps[i]
+
+    def _taped_step(self, R, k, r, i=0):
+        ty, tf, _, _, tapes = self._tape(i)
+        td.check(td.lib().tdw_adj_forward_step(self.ctx.h, C.byref(R.st), self.it0 + k, _p(ty[r]), _p(tf[r]),
+                                               C.byref(tapes[r])), "tdw_adj_forward_step")
+
     def forward(self, keep_checkpoints=True):
-        """Runs nsteps steps; returns the run (final y, densities, potentials)."""
-        torch = self.torch
+        """Runs nsteps steps; returns the run (final y, densities, potentials).
+        With checkpoints, the last taped_segments segments run with the taped
+        step, so the backward sweep does not recompute them."""
         R = self._new_run(self.wf0, self.pots0)
         R.initial_measure()
         self.ckpts = []
+        self.taped = {}
+        K = self.K
+        nseg = (self.nsteps + K - 1) // K
+        first_taped = max(0, nseg - self.taped_segments)
         for k in range(self.nsteps):
-            if keep_checkpoints and k % self.K == 0:
+            seg, r = divmod(k, K)
+            if keep_checkpoints and r == 0:
                 self.ckpts.append((k, R.wf.clone(), [t.clone() for t in R.history()[:self.h]], R.pots.clone()))
-            R.step(self.it0 + k, grad_last=False)
+            if keep_checkpoints and seg >= first_taped:
+                self._taped_step(R, k, r, seg - first_taped)
+                if r == K - 1 or k == self.nsteps - 1:
+                    self._tape(seg - first_taped)[0][r + 1].copy_(R.wf)
+                    self.taped[seg] = seg - first_taped
+            else:
+                R.step(self.it0 + k, grad_last=False)
         self.ctx.sync()
         self.run = R
         return R
@@ -108,29 +152,22 @@ class AdjointProblem:
             A.lf[j] = lf[j].data_ptr()
             A.w[j] = w[j].data_ptr()
         A.lpots, A.lgrid, A.theta_grad = lpots.data_ptr(), lgrid.data_ptr(), theta.data_ptr()
-        # tape of one segment
-        ty = [z() for _ in range(K + 1)]
-        tf = [z() for _ in range(K)]
-        tscal = torch.zeros((K, 4 * n), dtype=torch.float64, device=dev)
-        tgrid = torch.zeros((K, 4, 12 * N), dtype=torch.float64, device=dev)
-        tapes = []
-        for r in range(K):
-            t = td.TapeStep()
-            t.n = n
-            t.scal = tscal[r].data_ptr()
-            t.dens_p, t.pots_p = tgrid[r, 0].data_ptr(), tgrid[r, 1].data_ptr()
-            t.dens_c, t.pots_c = tgrid[r, 2].data_ptr(), tgrid[r, 3].data_ptr()
-            tapes.append(t)
         L = td.lib()
         last = True
         for (k0, y0, hist0, pots0) in reversed(self.ckpts):
             ks = min(K, self.nsteps - k0)
-            # taped recompute of the segment
-            R = self._new_run(y0, pots0, hist0)
-            for r in range(ks):
-                td.check(L.tdw_adj_forward_step(self.ctx.h, C.byref(R.st), self.it0 + k0 + r, _p(ty[r]), _p(tf[r]),
-                                                C.byref(tapes[r])), "tdw_adj_forward_step")
-            ty[ks].copy_(R.wf)
+            seg = k0 // K
+            if seg in self.taped:
+                ti = self.taped.pop(seg)
+            else:
+                # taped recompute of the segment (every taped buffer is free by now)
+                ti = 0
+                R = self._new_run(y0, pots0, hist0)
+                for r in range(ks):
+                    self._taped_step(R, k0 + r, r, ti)
+                self._tape(ti)[0][ks].copy_(R.wf)
+                del R
+            ty, tf, tscal, tgrid, tapes = self._tape(ti)
             # adjoint sweep through the segment
             for r in reversed(range(ks)):
                 hist = [(tf[r - j] if r - j >= 0 else hist0[j - r - 1]) for j in range(1, h + 1)]
@@ -141,7 +178,6 @@ class AdjointProblem:
                                                  _p(ty[r + 1]), _p(tf[r]), C.byref(tapes[r]), _p(pots_prev),
                                                  _p(dg)), "tdw_adj_backward_step")
             last = False
-            del R
         self.ctx.sync()
         self.lambda_y0 = ly
         return theta.cpu().numpy().reshape(td.MAX_GAUSS, NPAR)
@@ -149,6 +185,7 @@ class AdjointProblem:
     def close(self):
         self.ckpts = []
         self.run = None
+        self._tps = []
         self.ctx.close()
 
 

@@ -82,7 +82,7 @@ __global__ void k_rows(const double *__restrict__ part, int n, int nchunk, doubl
 }
 
 // ---------------------------------------------------------------------------
-// out = acc + hterm - 2 s_a (g1 + e_a g2)  (elementwise), and the potential
+// out = acc + (hterm - em_a mu) - 2 s_a (g1 + e_a g2)  (elementwise), and the potential
 // gradient of H: with nu = mu - s_a psi,
 //   dL/dVa = sum Re(nu_u* psi_u), dL/dVb = -sum Re(nu_v* psi_v),
 //   dL/dRe D = sum Re(nu_u* psi_v + nu_v* psi_u),
@@ -92,8 +92,8 @@ __global__ void k_rows(const double *__restrict__ part, int n, int nchunk, doubl
 __global__ void __launch_bounds__(AT)
 k_cross_upd(double2 *out, const double2 *acc, const double2 *hterm, const double2 *__restrict__ g1,
             const double2 *__restrict__ g2, const double2 *__restrict__ mu, const double2 *__restrict__ psi,
-            const double *__restrict__ sv, const double *__restrict__ ev, int n, long N, int chunk,
-            double *__restrict__ part)
+            const double *__restrict__ sv, const double *__restrict__ ev, const double *__restrict__ em, int n, long N,
+            int chunk, double *__restrict__ part)
 {
     const long i = (long)blockIdx.x * AT + threadIdx.x;
     if (i >= N) return;
@@ -108,11 +108,16 @@ k_cross_upd(double2 *out, const double2 *acc, const double2 *hterm, const double
             gu = gu + g2[iu] * e;
             gv = gv + g2[iv] * e;
         }
+        const double2 mu_u = mu[iu], mu_v = mu[iv];
         double2 ou = gu * (-2.0 * s), ov = gv * (-2.0 * s);
-        if (hterm) { ou = hterm[iu] + ou; ov = hterm[iv] + ov; }
+        if (hterm) {
+            const double m = em ? em[a] : 0.0;
+            ou = hterm[iu] - mu_u * m + ou;
+            ov = hterm[iv] - mu_v * m + ov;
+        }
         if (acc) { ou = acc[iu] + ou; ov = acc[iv] + ov; }
         const double2 pu = psi[iu], pv = psi[iv];
-        const double2 nu_u = mu[iu] - pu * s, nu_v = mu[iv] - pv * s;
+        const double2 nu_u = mu_u - pu * s, nu_v = mu_v - pv * s;
         out[iu] = ou;
         out[iv] = ov;
         gVa += cmulc(nu_u, pu).x;
@@ -360,7 +365,7 @@ static int sdot(tdw_ctx *c, int n, const double2 *x, const double2 *y, double sc
 
 static int cross_upd(tdw_ctx *c, int n, double2 *out, const double2 *acc, const double2 *hterm, const double2 *g1,
                      const double2 *g2, const double2 *mu, const double2 *psi, const double *sv, const double *ev,
-                     const double *add, double *lU)
+                     const double *em, const double *add, double *lU)
 {
     TDW_TIMED(c, "adj_cross");
     const long N = (long)c->N;
@@ -370,7 +375,7 @@ static int cross_upd(tdw_ctx *c, int n, double2 *out, const double2 *acc, const 
     const int nchunk = (n + chunk - 1) / chunk;
     TDW_TRY(ensure_part(c, (size_t)nchunk * 4 * N));
     hipLaunchKernelGGL(k_cross_upd, dim3(nblk, nchunk), dim3(AT), 0, c->stream, out, acc, hterm, g1, g2, mu, psi, sv,
-                       ev, n, N, chunk, c->d_part);
+                       ev, em, n, N, chunk, c->d_part);
     hipLaunchKernelGGL(k_cross_final, dim3(nblk), dim3(AT), 0, c->stream, c->d_part, nchunk, N, add, lU);
     return (int)hipGetLastError();
 }
@@ -496,8 +501,8 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
     // history refresh f_new = H(U') y' - e' y'  (mu = dL/df_new)
     TDW_TRY(sdot(c, n, MU, CD2(y1), dv, sv1));
     TDW_TRY(laplace(c, 2 * n, MU, W2));
-    TDW_TRY(apply_h(c, it + 1, n, MU, W1, W2, t->pots_c, c->d_kt, e_n, nullptr));
-    TDW_TRY(cross_upd(c, n, LY, LY, W1, CD2(fnew), CD2(y1), MU, CD2(y1), sv1, e_n, A->lpots, lUc));
+    TDW_TRY(apply_h_only(c, n, MU, W1, W2, t->pots_c, c->d_kt));
+    TDW_TRY(cross_upd(c, n, LY, LY, W1, CD2(fnew), CD2(y1), MU, CD2(y1), sv1, e_n, e_n, A->lpots, lUc));
     // U' = Phi(D', U_p, V_ext)
     TDW_TRY(pot_vjp(c, it + 1, t->dens_c, t->pots_p, lUc, dens_grad, lD, lUo, A->theta_grad));
     // D' = dens(y'), then y' = y_c / n_c
@@ -517,8 +522,8 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
     }
     // F_p = H(U_p) yh_p - e_p yh_p
     TDW_TRY(laplace(c, 2 * n, W0, W2));
-    TDW_TRY(apply_h(c, it + 1, n, W0, W1, W2, t->pots_p, c->d_kt, e_p, nullptr));
-    TDW_TRY(cross_upd(c, n, W1, nullptr, W1, W4, nullptr, W0, W3, svp, nullptr, lUo, lUc));
+    TDW_TRY(apply_h_only(c, n, W0, W1, W2, t->pots_p, c->d_kt));
+    TDW_TRY(cross_upd(c, n, W1, nullptr, W1, W4, nullptr, W0, W3, svp, nullptr, e_p, lUo, lUc));
     // U_p = Phi(D_p, U_prev, V_ext)
     TDW_TRY(pot_vjp(c, it + 1, t->dens_p, pots_prev, lUc, nullptr, lD, A->lpots, A->theta_grad));
     TDW_TRY(dens_t(c, n, W3, W1, lD, t2));

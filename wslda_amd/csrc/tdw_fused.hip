@@ -328,6 +328,141 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
     if (tid == 0) part[blockIdx.x] = ssum;
 }
 
+// ---------------------------------------------------------------------------
+// square yz planes (LY = LZ = L): register-pencil pass.  A workgroup of
+// 2*P*L threads takes P (state, x) planes, both components: a coalesced load
+// into a padded LDS tile (row stride L+1), then thread (component-plane,
+// j) transforms column j (D_y) and row j (D_z, written back over the tile),
+// adds the two and T, and the H epilogue runs point-major over the tile; psi
+// for the pointwise terms is read again from global memory (the tile was
+// loaded moments before).
+// ---------------------------------------------------------------------------
+// D_z of row j of a tile (row stride L+1), in place
+template <int L, typename C>
+__device__ __forceinline__ void zpass_row(C *Ap, int j, double kzs)
+{
+    typedef decltype(C().x) R;
+    constexpr int LP = L + 1;
+    C r[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) r[i] = Ap[j * LP + i];
+    rf::fft<L, -1, C>(r);
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+        const int m = (i < L / 2) ? i : i - L;
+        r[i] = r[i] * (R)(-kzs * (double)(m * m));
+    }
+    rf::fft<L, +1, C>(r);
+#pragma unroll
+    for (int i = 0; i < L; i++) Ap[j * LP + i] = r[i];
+}
+
+// RR: FP64 form; keeps one pencil live (z pass first, then the y pass on
+// column j read again from global memory)
+template <int L, bool SP, int P, bool RR = !SP>
+__global__ void __launch_bounds__(2 * P * L)
+f_hyz2(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
+       const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX, double kys, double kzs,
+       HParams hp, double *__restrict__ part)
+{
+    typedef hc_t<SP> C;
+    typedef decltype(C().x) R;
+    constexpr int B = 2 * P * L, E = L * L, LP = L + 1, PE = L * LP;
+    extern __shared__ double2 lds_raw[];
+    __shared__ double red[B / 64];
+    C *A = (C *)lds_raw;
+    const int tid = threadIdx.x;
+    const long plane0 = (long)blockIdx.x * P;
+    const long nplanes = (long)n * NX;
+    for (int q = tid; q < 2 * P * E; q += B) {
+        const int pc = q / E, e = q - pc * E;
+        const long pl = plane0 + (pc >> 1);
+        if (pl < nplanes) {
+            const long a = pl / NX, x = pl - a * NX;
+            const long g = ((pc & 1) ? ((long)n + a) : a) * N + x * E + e;
+            A[pc * PE + (e / L) * LP + (e % L)] = cvt<C>(y[g]);
+        }
+    }
+    __syncthreads();
+    const int pc = tid / L, j = tid - pc * L;
+    C *Ap = A + pc * PE;
+    const long plj = plane0 + (pc >> 1);
+    const bool live = plj < nplanes;
+    // global index of (row 0, column j) of this thread's plane-component
+    auto col0 = [&]() -> long {
+        const long a = plj / NX, x = plj - a * NX;
+        return ((pc & 1) ? ((long)n + a) : a) * N + x * E + j;
+    };
+    if constexpr (!RR) {
+        // D_y of column j from the tile, kept in registers across the z pass
+        C dy[L];
+#pragma unroll
+        for (int i = 0; i < L; i++) dy[i] = Ap[i * LP + j];
+        rf::fft<L, -1, C>(dy);
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const int m = (i < L / 2) ? i : i - L;
+            dy[i] = dy[i] * (R)(-kys * (double)(m * m));
+        }
+        rf::fft<L, +1, C>(dy);
+        __syncthreads();
+        zpass_row<L>(Ap, j, kzs);
+        __syncthreads();
+        if (live) {
+            const long gcol = col0();
+#pragma unroll
+            for (int i = 0; i < L; i++) Ap[i * LP + j] = dy[i] + Ap[i * LP + j] + T[gcol + (long)i * L];
+        }
+    } else {
+        // one pencil live at a time: D_z over the tile first, then column j
+        // of the input again from global memory (just loaded, cache-hot)
+        zpass_row<L>(Ap, j, kzs);
+        __syncthreads();
+        if (live) {
+            const long gcol = col0();
+            C dy[L];
+#pragma unroll
+            for (int i = 0; i < L; i++) dy[i] = cvt<C>(y[gcol + (long)i * L]);
+            rf::fft<L, -1, C>(dy);
+#pragma unroll
+            for (int i = 0; i < L; i++) {
+                const int m = (i < L / 2) ? i : i - L;
+                dy[i] = dy[i] * (R)(-kys * (double)(m * m));
+            }
+            rf::fft<L, +1, C>(dy);
+#pragma unroll
+            for (int i = 0; i < L; i++) Ap[i * LP + j] = dy[i] + Ap[i * LP + j] + T[gcol + (long)i * L];
+        }
+    }
+    __syncthreads();
+    // H psi and <psi|H psi> per plane
+    for (int p = 0; p < P; p++) {
+        const long pl = plane0 + p;
+        if (pl >= nplanes) break;   // uniform over the workgroup
+        const long a = pl / NX, x = pl - a * NX;
+        const double s = sc[a];
+        const C *Au = A + (2 * p) * PE, *Av = A + (2 * p + 1) * PE;
+        double ssum = 0.0;
+        for (int e = tid; e < E; e += B) {
+            const int i = e / L, k = e - i * L;
+            const long gi = x * E + e;
+            const double2 pu = y[a * N + gi] * s, pv = y[((long)n + a) * N + gi] * s;
+            const double2 lu = to64(Au[i * LP + k]) * s, lv = to64(Av[i * LP + k]) * s;
+            const double aa = hp.bdg ? -0.5 : pots[4 * N + gi] * -0.5;
+            const double ab = hp.bdg ? -0.5 : pots[5 * N + gi] * -0.5;
+            const double Va = pots[gi] - hp.mu_a, Vb = pots[N + gi] - hp.mu_b;
+            const double2 D = make_double2(pots[2 * N + 2 * gi], pots[2 * N + 2 * gi + 1]);
+            const double2 hu = pu * Va + cmul(pv, D) + lu * aa;
+            const double2 hv = pv * (-Vb) + cmulc(D, pu) - lv * ab;
+            hout[a * N + gi] = cvt<C>(hu);
+            hout[((long)n + a) * N + gi] = cvt<C>(hv);
+            ssum += cmulc(pu, hu).x + cmulc(pv, hv).x;
+        }
+        ssum = block_sum(ssum, red);
+        if (tid == 0) part[pl] = ssum;
+    }
+}
+
 __global__ void f_e_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ e,
                           double *__restrict__ e2)
 {
@@ -442,6 +577,36 @@ static size_t hyz_lds(const tdw_ctx *c)
     return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 4 * (size_t)c->ny * c->nz;
 }
 
+// square-plane pass: kernel, planes per workgroup, workgroup size, LDS bytes
+struct Hyz2 {
+    void (*fn)(const double2 *, const void *, void *, const double *, const double *, int, long, int, double, double,
+               HParams, double *);
+    int P, B;
+    size_t lds;
+};
+
+template <int L, bool SP, int P> static void set_hyz2(Hyz2 *h)
+{
+    h->fn = (decltype(h->fn))f_hyz2<L, SP, P>;
+    h->P = P;
+    h->B = 2 * P * L;
+    h->lds = (SP ? sizeof(float2) : sizeof(double2)) * 2 * P * (size_t)L * (L + 1);
+}
+
+template <bool SP> static bool hyz2_for(const tdw_ctx *c, Hyz2 *h)
+{
+    if (c->ny != c->nz) return false;
+    switch (c->ny) {
+    case 8: set_hyz2<8, SP, 16>(h); break;
+    case 12: set_hyz2<12, SP, 8>(h); break;
+    case 16: set_hyz2<16, SP, 8>(h); break;
+    case 24: set_hyz2<24, SP, 4>(h); break;
+    case 32: if (SP) set_hyz2<32, SP, 4>(h); else set_hyz2<32, SP, 2>(h); break;
+    default: return false;
+    }
+    return h->lds <= (size_t)c->max_lds;
+}
+
 bool fused_supported(const tdw_ctx *c)
 {
     if (c->dim != 3) return false;
@@ -450,6 +615,8 @@ bool fused_supported(const tdw_ctx *c)
     FusedFns<false>::corr_fn cf;
     FusedFns<false>::hyz_fn h;
     if (!FusedFns<false>::xpass(c->nx, c->p.scheme, &p, &cf)) return false;
+    Hyz2 h2;
+    if (split_mode(c) ? hyz2_for<true>(c, &h2) : hyz2_for<false>(c, &h2)) return true;
     if (!FusedFns<false>::hyz(c->ny, c->nz, &h)) return false;
     return hyz_lds(c) <= (size_t)c->max_lds;
 }
@@ -486,12 +653,27 @@ template <bool SP>
 static int hyz(tdw_ctx *c, int n, const double2 *y, const hc_t<SP> *T, hc_t<SP> *out, const double *pots,
                const double *scale, double *e_out, double *e_out2)
 {
-    typename FusedFns<SP>::hyz_fn h;
-    FusedFns<SP>::hyz(c->ny, c->nz, &h);
-    const size_t lds = hyz_lds(c);
     HParams hp{c->p.mu_a, c->p.mu_b, c->p.functional == TDW_BDG};
     const double kys = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
     const double kzs = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
+    Hyz2 h2;
+    if (hyz2_for<SP>(c, &h2)) {
+        const long nplanes = (long)n * c->nx;
+        {
+            TDW_TIMED(c, "f_hyz");
+            hipLaunchKernelGGL(h2.fn, dim3((nplanes + h2.P - 1) / h2.P), dim3(h2.B), h2.lds, c->stream, y,
+                               (const void *)T, (void *)out, pots, scale, n, (long)c->N, c->nx, kys, kzs, hp,
+                               c->d_fpart);
+            TDW_HIP(hipGetLastError());
+        }
+        TDW_TIMED(c, "f_e");
+        hipLaunchKernelGGL(f_e_final, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_fpart, n, c->nx,
+                           dvol_norm(c), e_out, e_out2);
+        return (int)hipGetLastError();
+    }
+    typename FusedFns<SP>::hyz_fn h;
+    FusedFns<SP>::hyz(c->ny, c->nz, &h);
+    const size_t lds = hyz_lds(c);
     const hc_t<SP> *twy, *twz;
     if constexpr (SP) { twy = c->d_tw32[1]; twz = c->d_tw32[2]; }
     else { twy = c->d_tw[1]; twz = c->d_tw[2]; }

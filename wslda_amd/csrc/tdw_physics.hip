@@ -484,6 +484,21 @@ int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const do
     return (int)hipGetLastError();
 }
 
+// out = H in, without the <H> sums and subtraction (the adjoint applies H to
+// its own vectors and subtracts in its next pass)
+int apply_h_only(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *lap, const double *pots,
+                 const double *kt)
+{
+    TDW_TIMED(c, "apply_h");
+    const long N = (long)c->N;
+    int nchunk;
+    long chunk = pick_chunk(c, &nchunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    hipLaunchKernelGGL(k_apply_h, dim3(nchunk, n), dim3(NT), 0, c->stream, in, out, lap, pots, kt, n, N, chunk, c->dim,
+                       c->p.functional == TDW_BDG, c->p.mu_a, c->p.mu_b, c->d_part);
+    return (int)hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // energy
 // ---------------------------------------------------------------------------

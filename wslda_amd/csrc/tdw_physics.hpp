@@ -90,6 +90,8 @@ int vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 int potentials(tdw_ctx *c, int it, const double *dens, double *pots);
 int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const double2 *lap,
             const double *pots, const double *kt, const double *useqpe, double *d_qpe);
+int apply_h_only(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *lap, const double *pots,
+                 const double *kt);
 int energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e);
 int taylor(tdw_ctx *c, int order, double dt, int n, const double2 *hpsi, double2 *upd, double2 *contr);
 

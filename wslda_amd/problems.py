@@ -246,3 +246,16 @@ def config3(torch, precision=td.SPLIT, n_states=24576, seed=3, device="cuda:0", 
     fb = np.array([td.fbeta(e, u.beta) for e in E])
     return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
                 name="3D trapped SLDA with a vortex line %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
+
+
+def config1(n_states=128, shape=(64, 8, 8), dx=1.0, amp_eF=0.1, width_dx=4.0):
+    """1D uniform SLDA at unitarity: 64-point periodic lattice (8x8
+    transverse box, which the uniform solution needs to reach kF = 1), the
+    128 lowest-|E| plane-wave BdG states, Delta ~ 0.5 eF, and a Gaussian
+    V_ext of amplitude 0.1 eF and width 4 dx on from t = 0 (host arrays)."""
+    s = uniform_start(1, shape, dx=dx, n_states=n_states)
+    p = s["params"]
+    eF = 0.5
+    w = -0.5 / (width_dx * dx) ** 2
+    p.vext = td.gauss_vext([(amp_eF * eF, (0.0, 0.0, 0.0), (w, 0.0, 0.0))])
+    return s

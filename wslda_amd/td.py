@@ -67,6 +67,10 @@ class State(C.Structure):
                 ("dens", C.c_void_p), ("pots", C.c_void_p), ("qpe", C.c_void_p)]
 
 
+class CheckpointScalars(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("mu_a", "mu_b", "ec", "kF", "eF", "Effg", "beta")]
+
+
 class TapeStep(C.Structure):
     _fields_ = [("n", C.c_int), ("scal", C.c_void_p), ("dens_p", C.c_void_p), ("pots_p", C.c_void_p),
                 ("dens_c", C.c_void_p), ("pots_c", C.c_void_p)]
@@ -143,6 +147,10 @@ def _sigs():
         "tdw_fused_supported": (I, [P]),
         "tdw_initial_measure": (I, [P, C.POINTER(State)]),
         "tdw_adj_forward_step": (I, [P, C.POINTER(State), I, P, P, C.POINTER(TapeStep)]),
+        "tdw_checkpoint_nwf": (I, [C.c_char_p, C.POINTER(C.c_int)]),
+        "tdw_checkpoint_save": (I, [P, C.c_char_p, C.POINTER(State), D, P, P, C.POINTER(CheckpointScalars)]),
+        "tdw_checkpoint_load": (I, [P, C.c_char_p, C.POINTER(State), C.POINTER(C.c_double), P, P,
+                                    C.POINTER(CheckpointScalars)]),
         "tdw_adj_backward_step": (I, [P, C.POINTER(AdjState), I, P, C.POINTER(C.c_void_p), P, P,
                                       C.POINTER(TapeStep), P, P]),
         "tdw_uniform_solve": (I, [C.POINTER(Params), D, D, D, D, D, D, D, D, D, I, I, C.POINTER(Uniform)]),
@@ -449,3 +457,29 @@ class TDRun:
 
     def energy(self, it):
         return self.ctx.compute_energy(it, self.dens, self.pots)
+
+    def save_checkpoint(self, path, t0, fbeta, kt=None, **scalars):
+        """Writes the run as a td checkpoint file (tdwslda_checkpoint.c
+        layout); scalars: mu_a, mu_b, ec, kF, eF, Effg, beta."""
+        sc = CheckpointScalars(**scalars)
+        fb = np.ascontiguousarray(fbeta, dtype=np.float64)
+        ktv = None if kt is None else np.ascontiguousarray(kt, dtype=np.float64)
+        check(lib().tdw_checkpoint_save(self.ctx.h, path.encode(), C.byref(self.st), float(t0),
+                                        fb.ctypes.data_as(C.c_void_p),
+                                        None if ktv is None else ktv.ctypes.data_as(C.c_void_p), C.byref(sc)),
+              "tdw_checkpoint_save")
+
+    def load_checkpoint(self, path):
+        """Reads a td checkpoint file into the run; returns (t0, fbeta, kt,
+        scalars dict)."""
+        n = self.n
+        t0 = C.c_double()
+        fb = np.zeros(n)
+        kt = np.zeros(2 * n)
+        sc = CheckpointScalars()
+        check(lib().tdw_checkpoint_load(self.ctx.h, path.encode(), C.byref(self.st), C.byref(t0),
+                                        fb.ctypes.data_as(C.c_void_p), kt.ctypes.data_as(C.c_void_p), C.byref(sc)),
+              "tdw_checkpoint_load")
+        dim = self.ctx.params.dim
+        ktr = None if dim == 3 else (kt[:n].copy() if dim == 2 else kt)
+        return t0.value, fb, ktr, {k: getattr(sc, k) for k, _ in CheckpointScalars._fields_}
