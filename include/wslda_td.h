@@ -239,7 +239,9 @@ int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *a, int it, const tdw_comple
  * tdw_uniform_solve: solve_uniform_problem (pca_uniform.h:86-502); mu_rule 0
  *   is the chemical-potential update of pca_uniform.h:427-428, mu_rule 1 the
  *   one kept commented at :425-426 (the rule of the code that produced the
- *   testsuite's td-gauss goldens).  Returns 1 if maxiter was reached.
+ *   testsuite's td-gauss goldens).  With p->functional == TDW_BDG it is
+ *   solve_uniform_problem_bdg (pca_uniform.h:1105-1470; scattering length
+ *   p->sclgth, mu_rule ignored).  Returns 1 if maxiter was reached.
  * tdw_uniform_states: create_uniform_wf (pca_uniform.h:636-850) and its 2D /
  *   1D twins (:2044-2287, :2321-2560): states [idxfrom, idxto) in the
  *   reference's order, energies E_n, transverse momenta and degeneracies. */

@@ -978,10 +978,132 @@ static void bdg_branch(double eta_a, double eta_b, double delta, int branch, dou
     *ek = e; *uk = u; *vk = v;
 }
 
+/* solve_uniform_problem_bdg (pca_uniform.h:1105-1470): BdG functional with
+ * bare coupling 4*pi*a (a = sclgth), alpha = 1, no density functional in the
+ * mean fields (only the mean-field correction of the regularisation), and
+ * the chemical-potential update mu += muchange*(n0 - n) (:1431-1432). */
+static int uniform_solve_bdg(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in, double Tstart,
+                             double Tstop, double dT, double eps, double scmix, double muchange, int maxiter,
+                             tdwo_uniform *U)
+{
+    int NX = c->nx, NY = c->ny, NZ = c->nz;
+    double LX = c->dx * NX, LY = c->dy * NY, LZ = c->dz * NZ, LXYZ = LX * LY * LZ;
+    double *kkx = klist(NX, LX), *kky = klist(NY, LY), *kkz = klist(NZ, LZ);
+    size_t NXYZ = (size_t)NX * NY * NZ;
+    double *kk2 = (double *)malloc(sizeof(double) * NXYZ);
+    size_t i = 0;
+    for (int ix = 0; ix < NX; ix++) for (int iy = 0; iy < NY; iy++) for (int iz = 0; iz < NZ; iz++, i++)
+        kk2[i] = kkx[ix] * kkx[ix] + kky[iy] * kky[iy] + kkz[iz] * kkz[iz];
+    double eF_a = pow(6.0 * M_PI * M_PI * n0_a, 2.0 / 3.0) / 2.0;
+    double eF_b = pow(6.0 * M_PI * M_PI * n0_b, 2.0 / 3.0) / 2.0;
+    double eF_avg = pow(3.0 * M_PI * M_PI * (n0_a + n0_b), 2.0 / 3.0) / 2.0;
+    double gbare = 4.0 * M_PI * c->sclgth;
+    double kc = kc_in;
+    const double alph_a = 1.0, alph_b = 1.0;
+    double mu_a = 0.96087287 * eF_a, mu_b = 0.96087287 * eF_b;
+    double tau_a = pow(6.0 * M_PI * M_PI * n0_a, 5.0 / 3.0) / (10.0 * M_PI * M_PI);
+    double tau_b = pow(6.0 * M_PI * M_PI * n0_b, 5.0 / 3.0) / (10.0 * M_PI * M_PI);
+    double delta = (n0_a < n0_b) ? 0.21 * pow(6.0 * M_PI * M_PI * n0_a, 2.0 / 3.0) / 2.0
+                                 : 0.21 * pow(6.0 * M_PI * M_PI * n0_b, 2.0 / 3.0) / 2.0;
+    double nu = -1.0 * delta / gbare;
+    double V_a = 0.0, V_b = 0.0, V_a_old, V_b_old, tau_a_old, tau_b_old, delta_old, nu_old;
+    double n_a = 0, n_b = 0, S = 0, beta = 1.0, T;
+    double ec = c->ec;
+    double R0, R1;
+    reg_coeffs(c, &R0, &R1);
+    int iter = 0, nwf = 0;
+    double dsT = 1.0, TT;
+    double Ts = fabs(Tstart), Tp = fabs(Tstop), DT = fabs(dT);
+    if (Ts < Tp) dsT = -1.0;
+    for (TT = Ts * dsT; TT >= Tp * dsT; TT -= DT) {
+        T = TT * dsT;
+        if (T < 0.0) break;
+        beta = (T <= 1.0e-16) ? 1.0e16 : 1.0 / (T * eF_avg);
+        for (iter = 0; iter < maxiter; iter++) {
+            tau_a_old = tau_a; tau_b_old = tau_b; delta_old = delta; nu_old = nu;
+            V_a_old = V_a; V_b_old = V_b;
+            V_a = 0.0;
+            V_b = 0.0;
+            double mu_p = (mu_a - V_a_old + mu_b - V_b_old) / 2.0;
+            zc p0 = csqrt(CMPLX(2.0 * mu_p, 0.0));
+            if (cimag(p0) < 0.) p0 *= -1.;
+            if (c->regularization == TDWO_CUBIC) kc = M_PI / c->dx;
+            else ec = kc * kc / 2.0 - mu_p;
+            zc wz = clog((kc + p0) / (kc - p0));
+            if (cimag(wz) < 0.) wz += I * 2. * M_PI;
+            wz = kc * R0 * (1. - (p0 / kc) * R1 * wz);
+            double g_eff = creal(1.0 / (1.0 / gbare - wz));
+            delta = -1.0 * g_eff * nu;
+            if (c->mf_correction) {
+                double kF = pow(3.0 * M_PI * M_PI * (n0_a + n0_b), 1.0 / 3.0);
+                double bcoeff = creal(p0) / (kF + 1.0e-12);
+                double x = bcoeff * kF * c->dx / M_PI;
+                if (g_eff < -1.0e-10 && x > 1.0e-10) {
+                    double Lam_0 = bcoeff * R0 / x * (1.0 - R1 * x * log((1.0 + x) / (1.0 - x)));
+                    double dLam_0_dx = (-2. * bcoeff * R0 * R1) / (1. - x * x) - bcoeff * R0 / x / x;
+                    double dLam_dn = (kF / (3. * (n0_a + n0_b))) * (Lam_0 + dLam_0_dx * bcoeff * c->dx * kF / M_PI);
+                    V_a += dLam_dn * pow(delta, 2);
+                    V_b += dLam_dn * pow(delta, 2);
+                }
+            }
+            S = 0.0; n_a = 0.0; n_b = 0.0; tau_a = 0.0; tau_b = 0.0; nu = 0.0; nwf = 0;
+            size_t ixyz = 0;
+            for (int ix = 0; ix < NX; ix++) for (int iy = 0; iy < NY; iy++) for (int iz = 0; iz < NZ; iz++, ixyz++) {
+                if (c->regularization != TDWO_CUBIC) {
+                    if ((c->dim == 2 || c->dim == 1) && iz == NZ / 2) continue;
+                    if (c->dim == 1 && iy == NY / 2) continue;
+                }
+                double eta_a = alph_a * kk2[ixyz] / 2.0 + V_a - mu_a;
+                double eta_b = alph_b * kk2[ixyz] / 2.0 + V_b - mu_b;
+                for (int br = 0; br < 2; br++) {
+                    double ek, uk, vk;
+                    bdg_branch(eta_a, eta_b, delta, br, &ek, &uk, &vk);
+                    if (ek > -ec && ek < ec) {
+                        double fp = tdwo_fbeta(ek, beta), fm = tdwo_fbeta(-1.0 * ek, beta);
+                        n_a += uk * uk * fp;
+                        tau_a += kk2[ixyz] * uk * uk * fp;
+                        n_b += vk * vk * fm;
+                        tau_b += kk2[ixyz] * vk * vk * fm;
+                        nu += uk * vk * (fm - fp);
+                        if (fp > 1.0e-16) S -= fp * log(fp);
+                        if (fm > 1.0e-16) S -= fm * log(fm);
+                        nwf++;
+                    }
+                }
+            }
+            n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            int conv = 1;
+            if (fabs(n0_a - n_a) > eps) conv = 0;
+            if (fabs(n0_b - n_b) > eps) conv = 0;
+            if (fabs(delta - delta_old) > eps) conv = 0;
+            if (conv) break;
+            tau_a = (1. - scmix) * tau_a_old + scmix * tau_a;
+            tau_b = (1. - scmix) * tau_b_old + scmix * tau_b;
+            delta = (1. - scmix) * delta_old + scmix * delta;
+            nu = (1. - scmix) * nu_old + scmix * nu;
+            mu_a += muchange * (n0_a - n_a);
+            mu_b += muchange * (n0_b - n_b);
+        }
+        eF_a = pow(6.0 * M_PI * M_PI * n_a, 2.0 / 3.0) / 2.0;
+        eF_b = pow(6.0 * M_PI * M_PI * n_b, 2.0 / 3.0) / 2.0;
+        eF_avg = pow(3.0 * M_PI * M_PI * (n_a + n_b), 2.0 / 3.0) / 2.0;
+    }
+    U->n0_a = n_a; U->n0_b = n_b; U->mu_a = mu_a; U->mu_b = mu_b; U->V_a = V_a; U->V_b = V_b;
+    U->alph_a = alph_a; U->alph_b = alph_b; U->kc = kc; U->ec = ec; U->nu = nu; U->delta = delta;
+    U->tau_a = tau_a; U->tau_b = tau_b; U->beta = beta; U->S = S; U->nwf = nwf; U->iters = iter;
+    U->ekin = (0.5 * alph_a * tau_a + 0.5 * alph_b * tau_b) * LXYZ;
+    U->epot = 0.0;
+    U->epair = -1.0 * delta * nu * LXYZ;
+    free(kkx); free(kky); free(kkz); free(kk2);
+    return iter == maxiter ? 1 : 0;
+}
+
 int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in,
                        double Tstart, double Tstop, double dT, double eps,
                        double scmix, double muchange, int maxiter, int mu_rule, tdwo_uniform *U)
 {
+    if (c->functional == TDWO_BDG)
+        return uniform_solve_bdg(c, n0_a, n0_b, kc_in, Tstart, Tstop, dT, eps, scmix, muchange, maxiter, U);
     int NX = c->nx, NY = c->ny, NZ = c->nz;
     double LX = c->dx * NX, LY = c->dy * NY, LZ = c->dz * NZ, LXYZ = LX * LY * LZ;
     double *kkx = klist(NX, LX), *kky = klist(NY, LY), *kkz = klist(NZ, LZ);

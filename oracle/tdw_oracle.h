@@ -16,11 +16,10 @@
  *     potentials, H application, qpe subtraction and the energy sums together.
  *   - closed forms: plane-wave eigenstates (spectral derivatives of e^{ikx}),
  *     stationarity of the uniform solution, ABM order on y'=-i*lam*y,
- *     Parseval for the FFT, brute-force DFT on tiny lattices.
- *   - parity unpinned: none of the BdG-functional paths are covered by a
- *     golden in this build (the td-gauss-bdg golden needs the BdG uniform
- *     solver, which is not restated); BdG potentials are pinned only by the
- *     stationarity check of a self-consistent plane-wave state.
+ *     brute-force DFT on small mixed-radix lattices.
+ *   - td-gauss-bdg goldens (testsuite/td-gauss-bdg/ref/test.ss0-1d.wlog.ref,
+ *     all 101 rows; ss0-2d first rows): pin the BdG uniform solver, the BdG
+ *     potentials and the BdG H application in the same way.
  */
 #ifndef TDW_ORACLE_H
 #define TDW_ORACLE_H
@@ -166,7 +165,9 @@ typedef struct {
     double nu, delta, tau_a, tau_b, beta, S, ekin, epot, epair;
     int nwf, iters;
 } tdwo_uniform;
-/* solve_uniform_problem (pca_uniform.h:86-502) for SLDA.
+/* solve_uniform_problem (pca_uniform.h:86-502) for SLDA; for c->functional
+ * == TDWO_BDG, solve_uniform_problem_bdg (pca_uniform.h:1105-1470, scattering
+ * length c->sclgth, mu_rule ignored).
  * mu_rule 0: chemical-potential update of pca_uniform.h:427-428 (relative
  * density error times eF); mu_rule 1: the update kept commented out at
  * pca_uniform.h:425-426 (absolute density error times Tstart/T), which is the

@@ -1,20 +1,22 @@
-"""Extract the td-gauss-slda golden trajectories into a small JSON fixture.
+"""Extract the td-gauss-slda and td-gauss-bdg golden trajectories into small
+JSON fixtures.
 
 Reads the reference testsuite's own expected outputs (data, not code):
-  testsuite/td-gauss-slda/ref/test.ss0-{1d,2d,3d}.wlog.ref
-and the test inputs (testsuite/td-gauss-slda/input.txt.{1,2,3},
-predefines.h: 12^3 lattice, DX=1.15, SLDA, spherical cutoff, AB4AM5 default,
-UNIFORM_TEST_MODE, params7=1 appended by test.desc for the ss0 runs).
+  testsuite/td-gauss-{slda,bdg}/ref/test.ss0-{1d,2d,3d}.wlog.ref
+and the test inputs (testsuite/td-gauss-{slda,bdg}/input.txt.{1,2,3},
+predefines.h: 12^3 lattice, DX=1.15, SLDA or BDG, spherical cutoff, AB4AM5
+default, UNIFORM_TEST_MODE, params7=1 appended by test.desc for the ss0
+runs; aBdG = -1.1 is the scattering length of the BDG runs).
 
-Run in a tree where the reference is mounted; the JSON it writes is the
-committed fixture tests/golden/td_gauss_slda.json.
+Run in a tree where the reference is mounted; the JSON it writes are the
+committed fixtures tests/golden/td_gauss_slda.json and td_gauss_bdg.json.
 """
 import json
 import os
 import sys
 
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "td_gauss_slda.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def rows(path):
@@ -37,15 +39,6 @@ def header(path):
     return h
 
 
-data = {"columns": ["id", "t_eF", "Na", "Nb", "N", "Etot/Effg", "Ekin/Effg", "Epot/Effg",
-                    "Epair/Effg", "Ecurr/Effg", "Eext/Effg", "Edext/Effg", "Evext/Effg"],
-        "inputs": {"NX": 12, "NY": 12, "NZ": 12, "DX": 1.15, "functional": "SLDA",
-                   "regularization": "SPHERICAL_CUTOFF", "scheme": "AB4AM5",
-                   "temperature": 0.05, "init0Tstart": 0.05, "init0muchange": 0.5,
-                   "init0scmix": 0.5, "init0eps": 1e-9, "init0maxiter": 10000,
-                   "dt_eF": 0.0035, "timesteps": 286, "measurements": 100,
-                   },
-        "runs": {}}
 def params(path):
     pr = [0.0] * 8
     for line in open(path):
@@ -56,9 +49,20 @@ def params(path):
     return pr
 
 
-for d, tag in ((1, "ss0-1d"), (2, "ss0-2d"), (3, "ss0-3d")):
-    p = os.path.join(REF, "testsuite/td-gauss-slda/ref/test.%s.wlog.ref" % tag)
-    inp = os.path.join(REF, "testsuite/td-gauss-slda/input.txt.%d" % d)
-    data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp)}
-json.dump(data, open(OUT, "w"), indent=0)
-print("wrote", OUT)
+for suite, functional in (("slda", "SLDA"), ("bdg", "BDG")):
+    data = {"columns": ["id", "t_eF", "Na", "Nb", "N", "Etot/Effg", "Ekin/Effg", "Epot/Effg",
+                        "Epair/Effg", "Ecurr/Effg", "Eext/Effg", "Edext/Effg", "Evext/Effg"],
+            "inputs": {"NX": 12, "NY": 12, "NZ": 12, "DX": 1.15, "functional": functional,
+                       "regularization": "SPHERICAL_CUTOFF", "scheme": "AB4AM5",
+                       "temperature": 0.05, "init0Tstart": 0.05, "init0muchange": 0.5,
+                       "init0scmix": 0.5, "init0eps": 1e-9, "init0maxiter": 10000,
+                       "dt_eF": 0.0035, "timesteps": 286, "measurements": 100, "aBdG": -1.1,
+                       },
+            "runs": {}}
+    for d, tag in ((1, "ss0-1d"), (2, "ss0-2d"), (3, "ss0-3d")):
+        p = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.wlog.ref" % (suite, tag))
+        inp = os.path.join(REF, "testsuite/td-gauss-%s/input.txt.%d" % (suite, d))
+        data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp)}
+    out = os.path.join(HERE, "td_gauss_%s.json" % suite)
+    json.dump(data, open(out, "w"), indent=0)
+    print("wrote", out)

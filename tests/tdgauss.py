@@ -11,20 +11,24 @@ import numpy as np
 
 from oracle import pyoracle as po
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "td_gauss_slda.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def golden():
-    return json.load(open(GOLDEN))
+def golden(functional=po.SLDA):
+    name = "td_gauss_bdg.json" if functional == po.BDG else "td_gauss_slda.json"
+    return json.load(open(os.path.join(HERE, "golden", name)))
 
 
-def setup(dim, scheme=po.AB4AM5):
-    g = golden()["inputs"]
+def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
+    """td-gauss-slda (default) or td-gauss-bdg (functional=BDG, scattering
+    length aBdG, BdG uniform solution) problem."""
+    g = golden(functional)["inputs"]
     NX, DXv = g["NX"], g["DX"]
     LXYZ = (NX * DXv) ** 3
     Na = math.ceil(1.0 / (6.0 * math.pi ** 2) * LXYZ)
     Nb = Na + 1
-    cfg = po.make_cfg(dim, NX, NX, NX, dx=DXv, scheme=scheme)
+    cfg = po.make_cfg(dim, NX, NX, NX, dx=DXv, scheme=scheme, functional=functional,
+                      sclgth=g["aBdG"] if functional == po.BDG else 0.0)
     kc = 0.999999 * math.pi / DXv
     u, rc = po.uniform_solve(cfg, Na / LXYZ, Nb / LXYZ, kc, g["temperature"], eps=g["init0eps"],
                              scmix=g["init0scmix"], muchange=g["init0muchange"],
@@ -36,7 +40,7 @@ def setup(dim, scheme=po.AB4AM5):
     Effg = 0.6 * n * eF * LXYZ
     wf, En, kt, cnt = po.uniform_states(cfg, u)
     fb = np.array([po.fbeta(E, u.beta) for E in En])
-    p = golden()["runs"]["ss0-%dd" % dim]["params"]
+    p = golden(functional)["runs"]["ss0-%dd" % dim]["params"]
     A = p[0] * eF
 
     def wcoef(s):

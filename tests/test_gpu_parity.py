@@ -206,14 +206,15 @@ def test_pece_steps(torch, dim, nsteps):
     ctx.close()
 
 
-def test_td_gauss_1d_golden_on_gpu(torch):
+@pytest.mark.parametrize("functional", [po.SLDA, po.BDG])
+def test_td_gauss_1d_golden_on_gpu(torch, functional):
     """The whole 28,600-step 1D trajectory on the GPU against the
-    reference's golden rows (10 printed digits)."""
-    say("case: td-gauss 1D golden trajectory on GPU")
-    s = tdgauss.setup(1)
+    reference's golden rows (10 printed digits), SLDA and BdG."""
+    say("case: td-gauss 1D golden trajectory on GPU, functional %d" % functional)
+    s = tdgauss.setup(1, functional=functional)
     ctx, R = gpu_run(torch, s)
     R.initial_measure()
-    ref = tdgauss.golden()["runs"]["ss0-1d"]["rows"]
+    ref = tdgauss.golden(functional)["runs"]["ss0-1d"]["rows"]
     it = 0
     bad = []
     for m in range(101):

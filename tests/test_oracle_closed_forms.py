@@ -228,3 +228,26 @@ def test_product_uniform_solver_matches_oracle():
     assert wg.shape == wr.shape
     assert np.abs(Eg - Er).max() < 1e-12
     assert np.abs(wg - wr).max() < 1e-12 * np.abs(wr).max()
+
+
+def test_product_bdg_uniform_solver_matches_oracle():
+    """The BdG branch of the library's uniform solver (td-gauss-bdg inputs:
+    12^3, DX 1.15, aBdG -1.1, T 0.05 eF) against the oracle's."""
+    from wslda_amd import td
+    from tests import tdgauss
+    s = tdgauss.setup(3, functional=po.BDG)
+    cfg = s["cfg"]
+    p = tdgauss.to_params(cfg)
+    g = tdgauss.golden(po.BDG)["inputs"]
+    L3 = (g["NX"] * g["DX"]) ** 3
+    na, nb = s["u"].n0_a, s["u"].n0_b
+    kc = 0.999999 * PI / g["DX"]
+    ug, rc = td.uniform_solve(p, na, nb, kc, g["temperature"], eps=g["init0eps"], scmix=g["init0scmix"],
+                              muchange=g["init0muchange"], maxiter=g["init0maxiter"])
+    ur, rcr = po.uniform_solve(cfg, na, nb, kc, g["temperature"], eps=g["init0eps"], scmix=g["init0scmix"],
+                               muchange=g["init0muchange"], maxiter=g["init0maxiter"])
+    assert rc == 0 and rcr == 0
+    for k in ("mu_a", "mu_b", "V_a", "V_b", "delta", "nu", "tau_a", "tau_b", "ekin", "epair", "ec"):
+        a, b = getattr(ug, k), getattr(ur, k)
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), k
+    assert ug.nwf == ur.nwf and ug.epot == 0.0

@@ -1,5 +1,6 @@
 """The oracle against the reference testsuite's own golden trajectories
-(testsuite/td-gauss-slda/ref/test.ss0-*.wlog.ref, 10 significant digits)."""
+(testsuite/td-gauss-{slda,bdg}/ref/test.ss0-*.wlog.ref, 10 significant
+digits)."""
 import pytest
 
 from oracle import pyoracle as po
@@ -9,11 +10,11 @@ from tests import tdgauss
 CHECKED = [(2, "Na"), (3, "Nb"), (4, "N"), (5, "Etot"), (6, "Ekin"), (7, "Epot"), (8, "Epair"), (10, "Eext")]
 
 
-def run_and_compare(dim, tag, nmeas):
-    s = tdgauss.setup(dim)
+def run_and_compare(dim, tag, nmeas, functional=po.SLDA):
+    s = tdgauss.setup(dim, functional=functional)
     P = po.Problem(s["cfg"], s["wf"], s["fbeta"], kt=s["kt"], cnt=s["cnt"], pots=s["pots"])
     P.initial_measure()
-    ref = tdgauss.golden()["runs"][tag]["rows"]
+    ref = tdgauss.golden(functional)["runs"][tag]["rows"]
     it = 0
     bad = []
     for m in range(nmeas + 1):
@@ -38,9 +39,10 @@ def run_and_compare(dim, tag, nmeas):
     assert not bad, bad[:10]
 
 
-def test_uniform_header_1d():
-    s = tdgauss.setup(1)
-    h = tdgauss.golden()["runs"]["ss0-1d"]["header"]
+@pytest.mark.parametrize("functional", [po.SLDA, po.BDG])
+def test_uniform_header_1d(functional):
+    s = tdgauss.setup(1, functional=functional)
+    h = tdgauss.golden(functional)["runs"]["ss0-1d"]["header"]
     assert abs(s["kF"] - float(h["kF"])) < 1e-5
     assert abs(s["eF"] - float(h["eF"])) < 1e-6
     assert abs(s["Effg"] - float(h["Effg"])) < 1e-4
@@ -63,3 +65,13 @@ def test_td_gauss_2d_first_rows():
 @pytest.mark.slow
 def test_td_gauss_3d_first_rows():
     run_and_compare(3, "ss0-3d", 2)
+
+
+def test_td_gauss_bdg_1d_full_trajectory():
+    """BdG functional: all 101 rows of the 1D spin-imbalanced run
+    (testsuite/td-gauss-bdg, aBdG = -1.1)."""
+    run_and_compare(1, "ss0-1d", 100, functional=po.BDG)
+
+
+def test_td_gauss_bdg_2d_first_rows():
+    run_and_compare(2, "ss0-2d", 11, functional=po.BDG)

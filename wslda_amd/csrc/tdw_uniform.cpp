@@ -105,7 +105,11 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
                                  double Tstop, double dT, double eps, double scmix, double muchange, int maxiter,
                                  int mu_rule, tdw_uniform *U)
 {
-    if (!c || !U || c->functional != TDW_SLDA) return TDW_EINVAL;
+    if (!c || !U || (c->functional != TDW_SLDA && c->functional != TDW_BDG)) return TDW_EINVAL;
+    // BdG (solve_uniform_problem_bdg, pca_uniform.h:1105-1470): bare coupling
+    // 4 pi a, no density functional in the mean fields, mu += muchange (n0 - n)
+    const bool bdg = c->functional == TDW_BDG;
+    const double gbare = 4.0 * PI * c->sclgth;
     const int NX = c->nx, NY = c->ny, NZ = c->nz;
     const double LX = c->dx * NX, LY = c->dy * NY, LZ = c->dz * NZ, LXYZ = LX * LY * LZ;
     std::vector<double> kk2((size_t)NX * NY * NZ);
@@ -130,12 +134,13 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     double eF_b = pow(6.0 * PI * PI * n0_b, 2.0 / 3.0) / 2.0;
     double eF_avg = pow(3.0 * PI * PI * (n0_a + n0_b), 2.0 / 3.0) / 2.0;
     double kc = kc_in, ec = c->ec;
-    double mu_a = 0.37 * eF_a, mu_b = 0.37 * eF_b;
+    const double mu0 = bdg ? 0.96087287 : 0.37, d0 = bdg ? 0.21 : 0.5;
+    double mu_a = mu0 * eF_a, mu_b = mu0 * eF_b;
     double tau_a = pow(6.0 * PI * PI * n0_a, 5.0 / 3.0) / (10.0 * PI * PI);
     double tau_b = pow(6.0 * PI * PI * n0_b, 5.0 / 3.0) / (10.0 * PI * PI);
-    double delta = (n0_a < n0_b) ? 0.5 * pow(6.0 * PI * PI * n0_a, 2.0 / 3.0) / 2.0
-                                 : 0.5 * pow(6.0 * PI * PI * n0_b, 2.0 / 3.0) / 2.0;
-    double nu = -1.0 * delta * tC / alph_plus;
+    double delta = (n0_a < n0_b) ? d0 * pow(6.0 * PI * PI * n0_a, 2.0 / 3.0) / 2.0
+                                 : d0 * pow(6.0 * PI * PI * n0_b, 2.0 / 3.0) / 2.0;
+    double nu = bdg ? -1.0 * delta / gbare : -1.0 * delta * tC / alph_plus;
     double V_a = 0, V_b = 0, n_a = 0, n_b = 0, S = 0, beta = 1.0;
     int iter = 0, nwf = 0;
     const double Ts = fabs(Tstart), Tp = fabs(Tstop), DT = fabs(dT);
@@ -147,8 +152,8 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
         for (iter = 0; iter < maxiter; iter++) {
             double tau_a_old = tau_a, tau_b_old = tau_b, delta_old = delta, nu_old = nu;
             double V_a_old = V_a, V_b_old = V_b;
-            V_a = -dtC_dna * delta * delta / alph_plus + dD_dna;
-            V_b = -dtC_dnb * delta * delta / alph_plus + dD_dnb;
+            V_a = bdg ? 0.0 : -dtC_dna * delta * delta / alph_plus + dD_dna;
+            V_b = bdg ? 0.0 : -dtC_dnb * delta * delta / alph_plus + dD_dnb;
             double mu_p = (mu_a - V_a_old + mu_b - V_b_old) / 2.0;
             zc p0 = std::sqrt(zc(2.0 * mu_p / alph_plus, 0.0));
             if (p0.imag() < 0.) p0 *= -1.;
@@ -157,7 +162,7 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
             zc wz = std::log((kc + p0) / (kc - p0));
             if (wz.imag() < 0.) wz += zc(0.0, 2. * PI);
             wz = kc * R0 * (1. - (p0 / kc) * R1 * wz);
-            double g_eff = (alph_plus / (tC - wz)).real();
+            double g_eff = bdg ? (1.0 / (1.0 / gbare - wz)).real() : (alph_plus / (tC - wz)).real();
             delta = -1.0 * g_eff * nu;
             if (c->mf_correction) {
                 double kF = pow(3.0 * PI * PI * (n0_a + n0_b), 1.0 / 3.0);
@@ -206,7 +211,10 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
             tau_b = (1. - scmix) * tau_b_old + scmix * tau_b;
             delta = (1. - scmix) * delta_old + scmix * delta;
             nu = (1. - scmix) * nu_old + scmix * nu;
-            if (mu_rule == 1) {
+            if (bdg) {
+                mu_a += muchange * (n0_a - n_a);
+                mu_b += muchange * (n0_b - n_b);
+            } else if (mu_rule == 1) {
                 mu_a += muchange * (Ts / T) * (n0_a - n_a);
                 mu_b += muchange * (Ts / T) * (n0_b - n_b);
             } else {
@@ -222,7 +230,7 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     U->alph_a = alph_a; U->alph_b = alph_b; U->kc = kc; U->ec = ec; U->nu = nu; U->delta = delta;
     U->tau_a = tau_a; U->tau_b = tau_b; U->beta = beta; U->S = S; U->nwf = nwf; U->iters = iter;
     U->ekin = (0.5 * alph_a * tau_a + 0.5 * alph_b * tau_b) * LXYZ;
-    U->epot = Dn * LXYZ;
+    U->epot = bdg ? 0.0 : Dn * LXYZ;
     U->epair = -1.0 * delta * nu * LXYZ;
     return iter == maxiter ? 1 : 0;
 }
