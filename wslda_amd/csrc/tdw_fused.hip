@@ -190,12 +190,17 @@ __global__ void f_norm_final(const double *__restrict__ part, int n, int per, do
 // ---------------------------------------------------------------------------
 // yz-plane pass: lap = s (T + D_y y + D_z y); H psi; <psi|H psi> partials
 // ---------------------------------------------------------------------------
-template <int L, int R, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, typename C>
-__device__ __forceinline__ void cstage(const C *src, C *dst, const C *tw, int tid)
+// One radix-R Stockham stage over P pencils of length L (pencil q at offset
+// (q/QD)*QH + (q%QD)*QL, element stride IS).  The last stage of a forward
+// transform also applies the kinetic factor -ks*m^2 of its output frequency.
+template <int L, int R, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, int NT, typename C>
+__device__ __forceinline__ void cstage(const C *src, C *dst, const C *tw, int tid, double ks)
 {
+    typedef decltype(C().x) Rl;
     constexpr int nb = L / R;
     constexpr int tstep = L / (Ns * R);
-    for (int b = tid; b < P * nb; b += FT) {
+    constexpr bool scale = SIGN < 0 && Ns * R == L;
+    for (int b = tid; b < P * nb; b += NT) {
         const int q = b / nb, j = b - q * nb;
         const int k = j % Ns;
         const int off = (q / QD) * QH + (q % QD) * QL;
@@ -210,22 +215,36 @@ __device__ __forceinline__ void cstage(const C *src, C *dst, const C *tw, int ti
         rf::small_dft<R, SIGN, C>::run(v);
         const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-        for (int t = 0; t < R; t++) dst[off + (base + t * Ns) * IS] = v[t];
+        for (int t = 0; t < R; t++) {
+            if constexpr (scale) {
+                const int f = j + t * Ns;                 // output frequency (j < Ns here)
+                const int m = (f < L / 2) ? f : f - L;
+                v[t] = v[t] * (Rl)(-ks * (double)(m * m));
+            }
+            dst[off + (base + t * Ns) * IS] = v[t];
+        }
     }
 }
 
-template <int L, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, typename C>
-__device__ __forceinline__ C *cfft(C *a, C *b, const C *tw, int tid)
+template <int L, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, int NT, typename C>
+__device__ __forceinline__ C *cfft(C *a, C *b, const C *tw, int tid, double ks)
 {
     if constexpr (Ns >= L) {
         return a;
     } else {
         constexpr int rem = L / Ns;
         constexpr int R = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 3 == 0) ? 3 : 2;
-        cstage<L, R, Ns, P, QD, QH, QL, IS, SIGN, C>(a, b, tw, tid);
+        cstage<L, R, Ns, P, QD, QH, QL, IS, SIGN, NT, C>(a, b, tw, tid, ks);
         __syncthreads();
-        return cfft<L, Ns * R, P, QD, QH, QL, IS, SIGN, C>(b, a, tw, tid);
+        return cfft<L, Ns * R, P, QD, QH, QL, IS, SIGN, NT, C>(b, a, tw, tid, ks);
     }
+}
+
+// threads of the yz-plane pass for a plane of E points: a multiple of 64
+// that leaves few idle lanes when each thread owns E/NT points
+constexpr int nt_hyz(int E)
+{
+    return E % 256 == 0 ? 256 : E % 192 == 0 ? 192 : E <= 64 ? 64 : E <= 128 ? 128 : E <= 192 ? 192 : 256;
 }
 
 struct HParams {
@@ -233,8 +252,8 @@ struct HParams {
     int bdg;
 };
 
-template <int LY, int LZ, bool SP>
-__global__ void __launch_bounds__(FT)
+template <int LY, int LZ, bool SP, int NT = nt_hyz(LY * LZ)>
+__global__ void __launch_bounds__(NT)
 f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
       const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
       const hc_t<SP> *__restrict__ twy_g, const hc_t<SP> *__restrict__ twz_g, double kys, double kzs, HParams hp,
@@ -243,21 +262,21 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
     typedef hc_t<SP> C;
     typedef decltype(C().x) R;
     constexpr int E = LY * LZ;              // points of one plane
-    constexpr int KM = (E + FT - 1) / FT;   // owned points per thread
+    constexpr int KM = (E + NT - 1) / NT;   // owned points per thread
     __shared__ C twy[LY], twz[LZ];
-    __shared__ double red[FT / 64];
+    __shared__ double red[NT / 64];
     extern __shared__ double2 lds_raw[];
     C *A = (C *)lds_raw, *B = A + 2 * E;
     const int tid = threadIdx.x;
-    for (int k = tid; k < LY; k += FT) twy[k] = twy_g[k];
-    for (int k = tid; k < LZ; k += FT) twz[k] = twz_g[k];
+    for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
+    for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
     const int a = blockIdx.x / NX;
     const int x = blockIdx.x - a * NX;
     const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
     C Xu[KM], Xv[KM], Ou[KM], Ov[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
-        const int e = tid + k * FT;
+        const int e = tid + k * NT;
         if (e < E) {
             Xu[k] = cvt<C>(y[gu + e]);
             Xv[k] = cvt<C>(y[gv + e]);
@@ -271,32 +290,24 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KM; k++) {
-            const int e = tid + k * FT;
+            const int e = tid + k * NT;
             if (e < E) {
                 A[e] = Xu[k];
                 A[E + e] = Xv[k];
             }
         }
         __syncthreads();
+        // forward transform (kinetic factor applied by its last stage), inverse
         C *F;
-        if (ax == 0) F = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, -1, C>(A, B, twz, tid);
-        else F = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, -1, C>(A, B, twy, tid);
+        if (ax == 0) F = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, -1, NT, C>(A, B, twz, tid, kzs);
+        else F = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, -1, NT, C>(A, B, twy, tid, kys);
         C *G = (F == A) ? B : A;
-        for (int e2 = tid; e2 < 2 * E; e2 += FT) {
-            const int e = e2 < E ? e2 : e2 - E;
-            int m;
-            double ks;
-            if (ax == 0) { const int i = e % LZ; m = (i < LZ / 2) ? i : i - LZ; ks = kzs; }
-            else { const int i = e / LZ; m = (i < LY / 2) ? i : i - LY; ks = kys; }
-            F[e2] = F[e2] * (R)(-ks * (double)(m * m));
-        }
-        __syncthreads();
         C *Rr;
-        if (ax == 0) Rr = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, +1, C>(F, G, twz, tid);
-        else Rr = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, +1, C>(F, G, twy, tid);
+        if (ax == 0) Rr = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, +1, NT, C>(F, G, twz, tid, 0.0);
+        else Rr = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, +1, NT, C>(F, G, twy, tid, 0.0);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
-            const int e = tid + k * FT;
+            const int e = tid + k * NT;
             if (e < E) {
                 Ou[k] = Ou[k] + Rr[e];
                 Ov[k] = Ov[k] + Rr[E + e];
@@ -308,7 +319,7 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; k++) {
-        const int e = tid + k * FT;
+        const int e = tid + k * NT;
         if (e < E) {
             const long gi = (long)x * E + e;
             const double2 pu = to64(Xu[k]) * s, pv = to64(Xv[k]) * s;
@@ -326,141 +337,6 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
     }
     ssum = block_sum(ssum, red);
     if (tid == 0) part[blockIdx.x] = ssum;
-}
-
-// ---------------------------------------------------------------------------
-// square yz planes (LY = LZ = L): register-pencil pass.  A workgroup of
-// 2*P*L threads takes P (state, x) planes, both components: a coalesced load
-// into a padded LDS tile (row stride L+1), then thread (component-plane,
-// j) transforms column j (D_y) and row j (D_z, written back over the tile),
-// adds the two and T, and the H epilogue runs point-major over the tile; psi
-// for the pointwise terms is read again from global memory (the tile was
-// loaded moments before).
-// ---------------------------------------------------------------------------
-// D_z of row j of a tile (row stride L+1), in place
-template <int L, typename C>
-__device__ __forceinline__ void zpass_row(C *Ap, int j, double kzs)
-{
-    typedef decltype(C().x) R;
-    constexpr int LP = L + 1;
-    C r[L];
-#pragma unroll
-    for (int i = 0; i < L; i++) r[i] = Ap[j * LP + i];
-    rf::fft<L, -1, C>(r);
-#pragma unroll
-    for (int i = 0; i < L; i++) {
-        const int m = (i < L / 2) ? i : i - L;
-        r[i] = r[i] * (R)(-kzs * (double)(m * m));
-    }
-    rf::fft<L, +1, C>(r);
-#pragma unroll
-    for (int i = 0; i < L; i++) Ap[j * LP + i] = r[i];
-}
-
-// RR: FP64 form; keeps one pencil live (z pass first, then the y pass on
-// column j read again from global memory)
-template <int L, bool SP, int P, bool RR = !SP>
-__global__ void __launch_bounds__(2 * P * L)
-f_hyz2(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
-       const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX, double kys, double kzs,
-       HParams hp, double *__restrict__ part)
-{
-    typedef hc_t<SP> C;
-    typedef decltype(C().x) R;
-    constexpr int B = 2 * P * L, E = L * L, LP = L + 1, PE = L * LP;
-    extern __shared__ double2 lds_raw[];
-    __shared__ double red[B / 64];
-    C *A = (C *)lds_raw;
-    const int tid = threadIdx.x;
-    const long plane0 = (long)blockIdx.x * P;
-    const long nplanes = (long)n * NX;
-    for (int q = tid; q < 2 * P * E; q += B) {
-        const int pc = q / E, e = q - pc * E;
-        const long pl = plane0 + (pc >> 1);
-        if (pl < nplanes) {
-            const long a = pl / NX, x = pl - a * NX;
-            const long g = ((pc & 1) ? ((long)n + a) : a) * N + x * E + e;
-            A[pc * PE + (e / L) * LP + (e % L)] = cvt<C>(y[g]);
-        }
-    }
-    __syncthreads();
-    const int pc = tid / L, j = tid - pc * L;
-    C *Ap = A + pc * PE;
-    const long plj = plane0 + (pc >> 1);
-    const bool live = plj < nplanes;
-    // global index of (row 0, column j) of this thread's plane-component
-    auto col0 = [&]() -> long {
-        const long a = plj / NX, x = plj - a * NX;
-        return ((pc & 1) ? ((long)n + a) : a) * N + x * E + j;
-    };
-    if constexpr (!RR) {
-        // D_y of column j from the tile, kept in registers across the z pass
-        C dy[L];
-#pragma unroll
-        for (int i = 0; i < L; i++) dy[i] = Ap[i * LP + j];
-        rf::fft<L, -1, C>(dy);
-#pragma unroll
-        for (int i = 0; i < L; i++) {
-            const int m = (i < L / 2) ? i : i - L;
-            dy[i] = dy[i] * (R)(-kys * (double)(m * m));
-        }
-        rf::fft<L, +1, C>(dy);
-        __syncthreads();
-        zpass_row<L>(Ap, j, kzs);
-        __syncthreads();
-        if (live) {
-            const long gcol = col0();
-#pragma unroll
-            for (int i = 0; i < L; i++) Ap[i * LP + j] = dy[i] + Ap[i * LP + j] + T[gcol + (long)i * L];
-        }
-    } else {
-        // one pencil live at a time: D_z over the tile first, then column j
-        // of the input again from global memory (just loaded, cache-hot)
-        zpass_row<L>(Ap, j, kzs);
-        __syncthreads();
-        if (live) {
-            const long gcol = col0();
-            C dy[L];
-#pragma unroll
-            for (int i = 0; i < L; i++) dy[i] = cvt<C>(y[gcol + (long)i * L]);
-            rf::fft<L, -1, C>(dy);
-#pragma unroll
-            for (int i = 0; i < L; i++) {
-                const int m = (i < L / 2) ? i : i - L;
-                dy[i] = dy[i] * (R)(-kys * (double)(m * m));
-            }
-            rf::fft<L, +1, C>(dy);
-#pragma unroll
-            for (int i = 0; i < L; i++) Ap[i * LP + j] = dy[i] + Ap[i * LP + j] + T[gcol + (long)i * L];
-        }
-    }
-    __syncthreads();
-    // H psi and <psi|H psi> per plane
-    for (int p = 0; p < P; p++) {
-        const long pl = plane0 + p;
-        if (pl >= nplanes) break;   // uniform over the workgroup
-        const long a = pl / NX, x = pl - a * NX;
-        const double s = sc[a];
-        const C *Au = A + (2 * p) * PE, *Av = A + (2 * p + 1) * PE;
-        double ssum = 0.0;
-        for (int e = tid; e < E; e += B) {
-            const int i = e / L, k = e - i * L;
-            const long gi = x * E + e;
-            const double2 pu = y[a * N + gi] * s, pv = y[((long)n + a) * N + gi] * s;
-            const double2 lu = to64(Au[i * LP + k]) * s, lv = to64(Av[i * LP + k]) * s;
-            const double aa = hp.bdg ? -0.5 : pots[4 * N + gi] * -0.5;
-            const double ab = hp.bdg ? -0.5 : pots[5 * N + gi] * -0.5;
-            const double Va = pots[gi] - hp.mu_a, Vb = pots[N + gi] - hp.mu_b;
-            const double2 D = make_double2(pots[2 * N + 2 * gi], pots[2 * N + 2 * gi + 1]);
-            const double2 hu = pu * Va + cmul(pv, D) + lu * aa;
-            const double2 hv = pv * (-Vb) + cmulc(D, pu) - lv * ab;
-            hout[a * N + gi] = cvt<C>(hu);
-            hout[((long)n + a) * N + gi] = cvt<C>(hv);
-            ssum += cmulc(pu, hu).x + cmulc(pv, hv).x;
-        }
-        ssum = block_sum(ssum, red);
-        if (tid == 0) part[pl] = ssum;
-    }
 }
 
 __global__ void f_e_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ e,
@@ -577,36 +453,6 @@ static size_t hyz_lds(const tdw_ctx *c)
     return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 4 * (size_t)c->ny * c->nz;
 }
 
-// square-plane pass: kernel, planes per workgroup, workgroup size, LDS bytes
-struct Hyz2 {
-    void (*fn)(const double2 *, const void *, void *, const double *, const double *, int, long, int, double, double,
-               HParams, double *);
-    int P, B;
-    size_t lds;
-};
-
-template <int L, bool SP, int P> static void set_hyz2(Hyz2 *h)
-{
-    h->fn = (decltype(h->fn))f_hyz2<L, SP, P>;
-    h->P = P;
-    h->B = 2 * P * L;
-    h->lds = (SP ? sizeof(float2) : sizeof(double2)) * 2 * P * (size_t)L * (L + 1);
-}
-
-template <bool SP> static bool hyz2_for(const tdw_ctx *c, Hyz2 *h)
-{
-    if (c->ny != c->nz) return false;
-    switch (c->ny) {
-    case 8: set_hyz2<8, SP, 16>(h); break;
-    case 12: set_hyz2<12, SP, 8>(h); break;
-    case 16: set_hyz2<16, SP, 8>(h); break;
-    case 24: set_hyz2<24, SP, 4>(h); break;
-    case 32: if (SP) set_hyz2<32, SP, 4>(h); else set_hyz2<32, SP, 2>(h); break;
-    default: return false;
-    }
-    return h->lds <= (size_t)c->max_lds;
-}
-
 bool fused_supported(const tdw_ctx *c)
 {
     if (c->dim != 3) return false;
@@ -615,8 +461,6 @@ bool fused_supported(const tdw_ctx *c)
     FusedFns<false>::corr_fn cf;
     FusedFns<false>::hyz_fn h;
     if (!FusedFns<false>::xpass(c->nx, c->p.scheme, &p, &cf)) return false;
-    Hyz2 h2;
-    if (split_mode(c) ? hyz2_for<true>(c, &h2) : hyz2_for<false>(c, &h2)) return true;
     if (!FusedFns<false>::hyz(c->ny, c->nz, &h)) return false;
     return hyz_lds(c) <= (size_t)c->max_lds;
 }
@@ -656,21 +500,6 @@ static int hyz(tdw_ctx *c, int n, const double2 *y, const hc_t<SP> *T, hc_t<SP> 
     HParams hp{c->p.mu_a, c->p.mu_b, c->p.functional == TDW_BDG};
     const double kys = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
     const double kzs = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
-    Hyz2 h2;
-    if (hyz2_for<SP>(c, &h2)) {
-        const long nplanes = (long)n * c->nx;
-        {
-            TDW_TIMED(c, "f_hyz");
-            hipLaunchKernelGGL(h2.fn, dim3((nplanes + h2.P - 1) / h2.P), dim3(h2.B), h2.lds, c->stream, y,
-                               (const void *)T, (void *)out, pots, scale, n, (long)c->N, c->nx, kys, kzs, hp,
-                               c->d_fpart);
-            TDW_HIP(hipGetLastError());
-        }
-        TDW_TIMED(c, "f_e");
-        hipLaunchKernelGGL(f_e_final, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_fpart, n, c->nx,
-                           dvol_norm(c), e_out, e_out2);
-        return (int)hipGetLastError();
-    }
     typename FusedFns<SP>::hyz_fn h;
     FusedFns<SP>::hyz(c->ny, c->nz, &h);
     const size_t lds = hyz_lds(c);
@@ -680,7 +509,8 @@ static int hyz(tdw_ctx *c, int n, const double2 *y, const hc_t<SP> *T, hc_t<SP> 
     // the norm partials are consumed already: reuse the partial buffer
     {
         TDW_TIMED(c, "f_hyz");
-        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(FT), lds, c->stream, y, T, out, pots, scale, n,
+        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(nt_hyz(c->ny * c->nz)), lds, c->stream, y, T, out, pots,
+                           scale, n,
                            (long)c->N, c->nx, twy, twz, kys, kzs, hp, c->d_fpart);
         TDW_HIP(hipGetLastError());
     }
