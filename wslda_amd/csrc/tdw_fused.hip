@@ -190,61 +190,101 @@ __global__ void f_norm_final(const double *__restrict__ part, int n, int per, do
 // ---------------------------------------------------------------------------
 // yz-plane pass: lap = s (T + D_y y + D_z y); H psi; <psi|H psi> partials
 // ---------------------------------------------------------------------------
-// One radix-R Stockham stage over P pencils of length L (pencil q at offset
-// (q/QD)*QH + (q%QD)*QL, element stride IS).  The last stage of a forward
-// transform also applies the kinetic factor -ks*m^2 of its output frequency.
-template <int L, int R, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, int NT, typename C>
-__device__ __forceinline__ void cstage(const C *src, C *dst, const C *tw, int tid, double ks)
-{
-    typedef decltype(C().x) Rl;
-    constexpr int nb = L / R;
-    constexpr int tstep = L / (Ns * R);
-    constexpr bool scale = SIGN < 0 && Ns * R == L;
-    for (int b = tid; b < P * nb; b += NT) {
-        const int q = b / nb, j = b - q * nb;
-        const int k = j % Ns;
-        const int off = (q / QD) * QH + (q % QD) * QL;
-        C v[R];
-#pragma unroll
-        for (int t = 0; t < R; t++) v[t] = src[off + (j + t * nb) * IS];
-#pragma unroll
-        for (int t = 1; t < R; t++) {
-            const C w = tw[t * k * tstep];
-            v[t] = cmul(v[t], SIGN > 0 ? mk(w.x, -w.y) : w);
-        }
-        rf::small_dft<R, SIGN, C>::run(v);
-        const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-        for (int t = 0; t < R; t++) {
-            if constexpr (scale) {
-                const int f = j + t * Ns;                 // output frequency (j < Ns here)
-                const int m = (f < L / 2) ? f : f - L;
-                v[t] = v[t] * (Rl)(-ks * (double)(m * m));
-            }
-            dst[off + (base + t * Ns) * IS] = v[t];
-        }
-    }
-}
-
-template <int L, int Ns, int P, int QD, int QH, int QL, int IS, int SIGN, int NT, typename C>
-__device__ __forceinline__ C *cfft(C *a, C *b, const C *tw, int tid, double ks)
-{
-    if constexpr (Ns >= L) {
-        return a;
-    } else {
-        constexpr int rem = L / Ns;
-        constexpr int R = (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 3 == 0) ? 3 : 2;
-        cstage<L, R, Ns, P, QD, QH, QL, IS, SIGN, NT, C>(a, b, tw, tid, ks);
-        __syncthreads();
-        return cfft<L, Ns * R, P, QD, QH, QL, IS, SIGN, NT, C>(b, a, tw, tid, ks);
-    }
-}
-
 // threads of the yz-plane pass for a plane of E points: a multiple of 64
 // that leaves few idle lanes when each thread owns E/NT points
 constexpr int nt_hyz(int E)
 {
     return E % 256 == 0 ? 256 : E % 192 == 0 ? 192 : E <= 64 ? 64 : E <= 128 ? 128 : E <= 192 ? 192 : 256;
+}
+
+// In-place second derivative along one axis of P pencils held in LDS, for
+// L = R1*R2: with n = n1*R2 + n2 and f = k1 + R1*k2,
+//   X[f] = sum_n2 W_R2^{n2 k2} W_L^{n2 k1} sum_n1 x[n1 R2 + n2] W_R1^{n1 k1},
+// stage 1 (per n2): R1-point DFT over n1, twiddle W_L^{n2 k1}, stored at k1*R2 + n2;
+// stage 2 (per k1): R2-point DFT over n2 -> X[k1 + R1 k2], times -ks*m(f)^2,
+//   inverse R2-point DFT and the conjugate twiddle, stored at k1*R2 + n2;
+// stage 3 (per n2): inverse R1-point DFT over k1, stored at n1*R2 + n2.
+// Every stage reads and writes the same positions, so one buffer suffices.
+template <int L> struct fac2 {
+    static constexpr int R1 = (L == 8 || L == 4 || L == 3 || L == 2) ? L
+                            : (L % 8 == 0) ? 8 : (L % 4 == 0) ? 4 : (L % 3 == 0) ? 3 : 2;
+    static constexpr int R2 = L / R1;
+};
+
+template <int L, int P, int QD, int QH, int QL, int IS, int NT, typename C>
+__device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
+{
+    typedef decltype(C().x) Rl;
+    constexpr int R1 = fac2<L>::R1, R2 = fac2<L>::R2;
+    // stage 1
+    if constexpr (R2 > 1) {
+        for (int b = tid; b < P * R2; b += NT) {
+            const int q = b / R2, n2 = b - q * R2;
+            C *pa = A + (q / QD) * QH + (q % QD) * QL;
+            C v[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; n1++) v[n1] = pa[(n1 * R2 + n2) * IS];
+            rf::small_dft<R1, -1, C>::run(v);
+#pragma unroll
+            for (int k1 = 1; k1 < R1; k1++) v[k1] = cmul(v[k1], tw[(n2 * k1) % L]);
+#pragma unroll
+            for (int k1 = 0; k1 < R1; k1++) pa[(k1 * R2 + n2) * IS] = v[k1];
+        }
+        __syncthreads();
+    }
+    // stage 2 (for R2 == 1 this is the whole transform)
+    for (int b = tid; b < P * R1; b += NT) {
+        const int q = b / R1, k1 = b - q * R1;
+        C *pa = A + (q / QD) * QH + (q % QD) * QL;
+        if constexpr (R2 > 1) {
+            C v[R2];
+#pragma unroll
+            for (int n2 = 0; n2 < R2; n2++) v[n2] = pa[(k1 * R2 + n2) * IS];
+            rf::small_dft<R2, -1, C>::run(v);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; k2++) {
+                const int f = k1 + R1 * k2;
+                const int m = (f < L / 2) ? f : f - L;
+                v[k2] = v[k2] * (Rl)(-ks * (double)(m * m));
+            }
+            rf::small_dft<R2, +1, C>::run(v);
+#pragma unroll
+            for (int n2 = 0; n2 < R2; n2++) {
+                C w = tw[(n2 * k1) % L];
+                w.y = -w.y;
+                pa[(k1 * R2 + n2) * IS] = n2 == 0 ? v[n2] : cmul(v[n2], w);
+            }
+        } else if (k1 == 0) {
+            // R2 == 1: one group per pencil holds the whole pencil
+            C v[R1];
+#pragma unroll
+            for (int n = 0; n < R1; n++) v[n] = pa[n * IS];
+            rf::small_dft<R1, -1, C>::run(v);
+#pragma unroll
+            for (int f = 0; f < R1; f++) {
+                const int m = (f < L / 2) ? f : f - L;
+                v[f] = v[f] * (Rl)(-ks * (double)(m * m));
+            }
+            rf::small_dft<R1, +1, C>::run(v);
+#pragma unroll
+            for (int n = 0; n < R1; n++) pa[n * IS] = v[n];
+        }
+    }
+    __syncthreads();
+    // stage 3
+    if constexpr (R2 > 1) {
+        for (int b = tid; b < P * R2; b += NT) {
+            const int q = b / R2, n2 = b - q * R2;
+            C *pa = A + (q / QD) * QH + (q % QD) * QL;
+            C v[R1];
+#pragma unroll
+            for (int k1 = 0; k1 < R1; k1++) v[k1] = pa[(k1 * R2 + n2) * IS];
+            rf::small_dft<R1, +1, C>::run(v);
+#pragma unroll
+            for (int n1 = 0; n1 < R1; n1++) pa[(n1 * R2 + n2) * IS] = v[n1];
+        }
+        __syncthreads();
+    }
 }
 
 struct HParams {
@@ -266,7 +306,7 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
     __shared__ C twy[LY], twz[LZ];
     __shared__ double red[NT / 64];
     extern __shared__ double2 lds_raw[];
-    C *A = (C *)lds_raw, *B = A + 2 * E;
+    C *A = (C *)lds_raw;
     const int tid = threadIdx.x;
     for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
     for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
@@ -297,14 +337,10 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
             }
         }
         __syncthreads();
-        // forward transform (kinetic factor applied by its last stage), inverse
-        C *F;
-        if (ax == 0) F = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, -1, NT, C>(A, B, twz, tid, kzs);
-        else F = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, -1, NT, C>(A, B, twy, tid, kys);
-        C *G = (F == A) ? B : A;
-        C *Rr;
-        if (ax == 0) Rr = cfft<LZ, 1, 2 * LY, 1, LZ, 0, 1, +1, NT, C>(F, G, twz, tid, 0.0);
-        else Rr = cfft<LY, 1, 2 * LZ, LZ, E, 1, LZ, +1, NT, C>(F, G, twy, tid, 0.0);
+        // second derivative along the axis, in place
+        if (ax == 0) lap_axis<LZ, 2 * LY, 1, LZ, 0, 1, NT, C>(A, twz, tid, kzs);
+        else lap_axis<LY, 2 * LZ, LZ, E, 1, LZ, NT, C>(A, twy, tid, kys);
+        const C *Rr = A;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
@@ -450,7 +486,7 @@ static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
 
 static size_t hyz_lds(const tdw_ctx *c)
 {
-    return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 4 * (size_t)c->ny * c->nz;
+    return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 2 * (size_t)c->ny * c->nz;
 }
 
 bool fused_supported(const tdw_ctx *c)
