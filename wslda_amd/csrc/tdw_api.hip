@@ -67,12 +67,19 @@ int ensure_T(tdw_ctx *c, size_t ne)
     return 0;
 }
 
-// Laplacian of m arrays.  1D: one outer pass.  2D/3D: outer pass into T
-// (the output array when it differs from the input), inner pass into out.
+// Laplacian of m arrays.  1D: one outer pass.  2D planes that fit in LDS:
+// one pass.  Otherwise outer pass into T (the output array when it differs
+// from the input), inner pass into out.
 int laplace(tdw_ctx *c, int m, const double2 *in, double2 *out)
 {
     if (m <= 0) return 0;
     if (c->dim == 1) return launch_outer(c, m, in, out, nullptr, 0);
+    if (c->dim == 2) {
+        // one pass per plane where the plane fits in LDS (in place allowed)
+        int done = 0;
+        TDW_TRY(lap2d(c, m, in, out, &done));
+        if (done) return 0;
+    }
     if (out != in) {
         TDW_TRY(launch_outer(c, m, in, out, nullptr, 0));
         return launch_inner(c, m, in, out, out, nullptr, nullptr, 0);

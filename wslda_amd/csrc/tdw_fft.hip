@@ -8,6 +8,7 @@
 // the z- and y-transforms run in LDS and out = T + D_y(in) + D_z(in).
 // 1D problems have no inner axis: the outer pass writes the Laplacian.
 #include "tdw_fft.hpp"
+#include "tdw_regfft.hpp"
 
 namespace tdw {
 
@@ -234,6 +235,152 @@ int launch_inner(tdw_ctx *c, int m, const double2 *in, const double2 *T, double2
     while (TB * c->S < 256 && TB < 64) TB <<= 1;
     if (mode == 0) return launch_inner_k<0, 1>(c, nblk, TB, in, T, out, ddy, ddz);
     return launch_inner_k<1, 1>(c, nblk, TB, in, T, out, ddy, ddz);
+}
+
+
+// --------------------------------------------------------------------------
+// 2D Laplacian in one pass: one workgroup per NX x NY plane held in LDS
+// (row stride NY+1), lap = F^-1[-(kx^2 + ky^2)/N F in] with in-place
+// two-factor transforms (L = R1*R2, n = n1*R2 + n2, frequency k1 + R1*k2 at
+// position k1*R2 + k2 after the forward stages).
+// --------------------------------------------------------------------------
+template <int L, int P, int PS, int IS>
+__device__ __forceinline__ void fwd2(double2 *A, const double2 *tw, int tid)
+{
+    constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
+    for (int b = tid; b < P * R2; b += NTHR) {
+        const int n2 = b / P, q = b - n2 * P;
+        double2 *pa = A + q * PS;
+        double2 v[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) v[n1] = pa[(n1 * R2 + n2) * IS];
+        rf::small_dft<R1, -1, double2>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; k1++) v[k1] = cmul(v[k1], tw[(n2 * k1) % L]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) pa[(k1 * R2 + n2) * IS] = v[k1];
+    }
+    __syncthreads();
+    if constexpr (R2 == 1) return;   // the R1-point DFT was the whole transform
+    for (int b = tid; b < P * R1; b += NTHR) {
+        const int k1 = b / P, q = b - k1 * P;
+        double2 *pa = A + q * PS;
+        double2 v[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = pa[(k1 * R2 + n2) * IS];
+        rf::small_dft<R2, -1, double2>::run(v);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) pa[(k1 * R2 + k2) * IS] = v[k2];
+    }
+    __syncthreads();
+}
+
+template <int L, int P, int PS, int IS>
+__device__ __forceinline__ void inv2(double2 *A, const double2 *tw, int tid)
+{
+    constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
+    if constexpr (R2 > 1) {
+        for (int b = tid; b < P * R1; b += NTHR) {
+            const int k1 = b / P, q = b - k1 * P;
+            double2 *pa = A + q * PS;
+            double2 v[R2];
+#pragma unroll
+            for (int k2 = 0; k2 < R2; k2++) v[k2] = pa[(k1 * R2 + k2) * IS];
+            rf::small_dft<R2, +1, double2>::run(v);
+#pragma unroll
+            for (int n2 = 0; n2 < R2; n2++) {
+                const double2 w = tw[(n2 * k1) % L];
+                pa[(k1 * R2 + n2) * IS] = n2 == 0 ? v[n2] : cmul(v[n2], make_double2(w.x, -w.y));
+            }
+        }
+        __syncthreads();
+    }
+    for (int b = tid; b < P * R2; b += NTHR) {
+        const int n2 = b / P, q = b - n2 * P;
+        double2 *pa = A + q * PS;
+        double2 v[R1];
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) v[k1] = pa[(k1 * R2 + n2) * IS];
+        rf::small_dft<R1, +1, double2>::run(v);
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) pa[(n1 * R2 + n2) * IS] = v[n1];
+    }
+    __syncthreads();
+}
+
+// frequency index held at position p of an axis after fwd2
+template <int L> __device__ __forceinline__ int perm_freq(int p)
+{
+    constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
+    const int k1 = p / R2, k2 = p - k1 * R2;
+    return k1 + R1 * k2;
+}
+
+template <int NX, int NY>
+__global__ void __launch_bounds__(NTHR)
+k_lap2d(const double2 *in, double2 *out, long N, const double2 *__restrict__ twx_g,
+        const double2 *__restrict__ twy_g, double ksx, double ksy)
+{
+    constexpr int NYP = NY + 1;
+    extern __shared__ double2 lds2[];
+    __shared__ double2 twx[NX], twy[NY];
+    double2 *A = lds2;
+    const int tid = threadIdx.x;
+    const long r0 = (long)blockIdx.x * N;
+    for (int k = tid; k < NX; k += NTHR) twx[k] = twx_g[k];
+    for (int k = tid; k < NY; k += NTHR) twy[k] = twy_g[k];
+    for (int e = tid; e < NX * NY; e += NTHR) A[(e / NY) * NYP + (e % NY)] = in[r0 + e];
+    __syncthreads();
+    // y transforms (NX pencils, contiguous), forward
+    fwd2<NY, NX, NYP, 1>(A, twy, tid);
+    // x transforms (NY pencils, stride NYP), forward, -(kx^2 + ky^2), inverse
+    fwd2<NX, NY, 1, NYP>(A, twx, tid);
+    for (int e = tid; e < NX * NY; e += NTHR) {
+        const int px = e / NY, py = e - px * NY;
+        const int fx = perm_freq<NX>(px), fy = perm_freq<NY>(py);
+        const int mx = (fx < NX / 2) ? fx : fx - NX, my = (fy < NY / 2) ? fy : fy - NY;
+        A[px * NYP + py] = A[px * NYP + py] * (-(ksx * (double)(mx * mx) + ksy * (double)(my * my)));
+    }
+    __syncthreads();
+    inv2<NX, NY, 1, NYP>(A, twx, tid);
+    inv2<NY, NX, NYP, 1>(A, twy, tid);
+    for (int e = tid; e < NX * NY; e += NTHR) out[r0 + e] = A[(e / NY) * NYP + (e % NY)];
+}
+
+typedef void (*lap2d_fn)(const double2 *, double2 *, long, const double2 *, const double2 *, double, double);
+
+static bool lap2d_for(const tdw_ctx *c, lap2d_fn *f, size_t *lds)
+{
+    if (c->dim != 2 || c->nx != c->ny) return false;
+    switch (c->nx) {
+    case 8: *f = k_lap2d<8, 8>; break;
+    case 12: *f = k_lap2d<12, 12>; break;
+    case 16: *f = k_lap2d<16, 16>; break;
+    case 24: *f = k_lap2d<24, 24>; break;
+    case 32: *f = k_lap2d<32, 32>; break;
+    case 64: *f = k_lap2d<64, 64>; break;
+    default: return false;
+    }
+    *lds = sizeof(double2) * (size_t)c->nx * (c->ny + 1);
+    return *lds <= (size_t)c->max_lds;
+}
+
+// 2D Laplacian of m arrays in one pass; *done = 0 if the lattice has no
+// one-pass kernel (the caller then runs the outer / inner passes)
+int lap2d(tdw_ctx *c, int m, const double2 *in, double2 *out, int *done)
+{
+    lap2d_fn f;
+    size_t lds;
+    *done = 0;
+    if (!lap2d_for(c, &f, &lds)) return 0;
+    *done = 1;
+    TDW_TIMED(c, "lap2d");
+    const double N = (double)c->N;
+    const double ksx = pow(2.0 * M_PI / c->Lphys[0], 2) / N, ksy = pow(2.0 * M_PI / c->Lphys[1], 2) / N;
+    hipLaunchKernelGGL(f, dim3(m), dim3(NTHR), lds, c->stream, in, out, (long)c->N, c->d_tw[0], c->d_tw[1], ksx,
+                       ksy);
+    TDW_HIP(hipGetLastError());
+    return 0;
 }
 
 }  // namespace tdw

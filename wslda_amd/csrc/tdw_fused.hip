@@ -205,17 +205,12 @@ constexpr int nt_hyz(int E)
 //   inverse R2-point DFT and the conjugate twiddle, stored at k1*R2 + n2;
 // stage 3 (per n2): inverse R1-point DFT over k1, stored at n1*R2 + n2.
 // Every stage reads and writes the same positions, so one buffer suffices.
-template <int L> struct fac2 {
-    static constexpr int R1 = (L == 8 || L == 4 || L == 3 || L == 2) ? L
-                            : (L % 8 == 0) ? 8 : (L % 4 == 0) ? 4 : (L % 3 == 0) ? 3 : 2;
-    static constexpr int R2 = L / R1;
-};
 
 template <int L, int P, int QD, int QH, int QL, int IS, int NT, typename C>
 __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
 {
     typedef decltype(C().x) Rl;
-    constexpr int R1 = fac2<L>::R1, R2 = fac2<L>::R2;
+    constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
     // stage 1
     if constexpr (R2 > 1) {
         for (int b = tid; b < P * R2; b += NT) {

@@ -85,6 +85,7 @@ int ensure_T(tdw_ctx *c, size_t nelem);
 // FFT launchers (tdw_fft.hip).  mode 0: laplacian, 1: derivatives.
 // Outer (x) pass: T = D_x(in) [and ddx = G_x(in)] for m arrays.
 int launch_outer(tdw_ctx *c, int m, const double2 *in, double2 *T, double2 *ddx, int mode);
+int lap2d(tdw_ctx *c, int m, const double2 *in, double2 *out, int *done);
 // Inner (y[,z]) pass: out = T + D_y(in) + D_z(in) [ddy, ddz].  T may be NULL.
 int launch_inner(tdw_ctx *c, int m, const double2 *in, const double2 *T, double2 *out,
                  double2 *ddy, double2 *ddz, int mode);

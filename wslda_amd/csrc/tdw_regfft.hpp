@@ -208,6 +208,14 @@ template <int L, int SIGN, typename C> __device__ __forceinline__ void fft(C (&x
     for (int i = 0; i < L; i++) x[i] = y[i];
 }
 
+// L = R1 * R2 with both factors small DFTs (2, 3, 4, 8): the two factors of
+// the in-place transforms in LDS
+template <int L> struct fac2 {
+    static constexpr int R1 = (L == 8 || L == 4 || L == 3 || L == 2) ? L
+                            : (L % 8 == 0) ? 8 : (L % 4 == 0) ? 4 : (L % 3 == 0) ? 3 : 2;
+    static constexpr int R2 = L / R1;
+};
+
 // -k_i^2 / L for index i of an axis of L points and physical length Lp
 __device__ __forceinline__ double k2fac(int i, int L, double Lp)
 {
