@@ -30,17 +30,26 @@ def nhist(scheme):
     return {34: 3, 45: 4, 55: 5}[scheme]
 
 
-def algorithmic_units(name, scheme):
+def algorithmic_units(name, scheme, split=False):
     """A kernel's share of the compulsory traffic of one PECE step, in units
     of S (one state's u and v in FP64) per state and launch.
 
-    The compulsory traffic of a step (AB5AM5) is B_qp = 13 S: the predictor
-    reads y, f_1..f_5 and writes y_p and the partial corrector c (8 S); the
-    corrector reads y_p and c and writes y (3 S); the history refresh reads
-    y and writes the newest f (2 S).  FFTs, densities, norms and <H> are
-    taken as fused into those passes, so whatever a kernel moves beyond its
-    share counts against its achieved bandwidth."""
+    The compulsory traffic of an FP64 step (AB5AM5) is B_qp = 13 S: the
+    predictor reads y, f_1..f_5 and writes y_p and the partial corrector c
+    (8 S); the corrector reads y_p and c and writes y (3 S); the history
+    refresh reads y and writes the newest f (2 S).  In the split form the
+    history entries and the partial corrector are FP32 (S/2 each), so the
+    same passes need 5 S, 2.5 S and 1.5 S (B_qp = 9 S at AB5AM5).  FFTs,
+    densities, norms and <H> are taken as fused into those passes, so
+    whatever a kernel moves beyond its share counts against its achieved
+    bandwidth."""
     h = nhist(scheme)
+    if split:
+        return {
+            "f_pred": 1 + 0.5 * h + 1 + 0.5,
+            "f_corr": 1 + 0.5 + 1,
+            "f_hyz": 0.75,       # two launches per step: H of y_p (share 0), history refresh (share 1.5)
+        }.get(name, 0)
     return {
         "f_pred": (h + 1) + 2,
         "f_corr": 3,
@@ -73,7 +82,9 @@ def design_units(name, scheme, split=False):
     }.get(name, 0)
 
 
-def step_units(scheme):
+def step_units(scheme, split=False):
+    if split:
+        return 0.5 * nhist(scheme) + 6.5   # 8, 8.5, 9 S
     return nhist(scheme) + 8     # B_qp: 11, 12, 13 S for AB3AM4, AB4AM5, AB5AM5
 
 
@@ -137,10 +148,10 @@ def run_gpu(args, prec):
     dname, (dtot, dcnt) = dom
     davg = dtot / dcnt / 1000.0
     S = 2 * N * 16
-    dbytes = algorithmic_units(dname, p.scheme) * S * n
+    dbytes = algorithmic_units(dname, p.scheme, prec == td.SPLIT) * S * n
     achieved = dbytes / davg / 1e9 if dbytes else None
     moved = design_units(dname, p.scheme, prec == td.SPLIT) * S * n
-    step_bytes = step_units(p.scheme) * S * n
+    step_bytes = step_units(p.scheme, prec == td.SPLIT) * S * n
     out = dict(qps=qps, ms=ms, wall=wall, n=n, N=N, rep=rep, dname=dname, davg=davg, dbytes=dbytes,
                achieved=achieved, moved=moved, step_bytes=step_bytes, setup_s=setup_s, scheme=p.scheme,
                e0=e0.tolist(), e1=e1.tolist(), it_e0=it_e0, it_e1=it, name=w.get("name"), params=p,

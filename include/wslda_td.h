@@ -71,12 +71,14 @@ typedef struct { float re, im; } tdw_complex32;
 #define TDW_ENOCTX (-1002)  /* drop-in call before tdw_bind()           */
 #define TDW_ESIZE (-1003)   /* lattice axis not a product of 2,3,4,8    */
 #define TDW_ENOMEM (-1004)
+#define TDW_EEXIST (-1005)  /* output file exists and overwrite is off   */
+#define TDW_EIO (-1006)     /* file could not be opened / written       */
 
 /* External potential of the built-in problem: a sum of Gaussians
  * amp*s(t)*exp(wx*(x-cx)^2 + wy*(y-cy)^2 + wz*(z-cz)^2), a harmonic trap
  * 0.5*omega^2*r^2 and the smooth on/off switch s(t) of the testsuite's
  * problem-definition.h; x = DX*(ix-NX/2) etc.  A project build replaces it
- * with the user's v_ext() callback (INTEGRATION.md). */
+ * with the user's v_ext() callback. */
 #define TDW_MAX_GAUSS 4
 typedef struct {
     int ngauss;
@@ -191,6 +193,45 @@ int tdw_checkpoint_save(tdw_ctx *c, const char *path, tdw_state *s, double t0, c
                         const double *h_kt, const tdw_checkpoint_scalars *sc);
 int tdw_checkpoint_load(tdw_ctx *c, const char *path, tdw_state *s, double *t0, double *h_fbeta, double *h_kt,
                         tdw_checkpoint_scalars *sc);
+
+/* W-data measurement output (lib/wdata/doc/README.md): <outprefix>.wtxt and
+ * <outprefix>_<var>.wdat, one datablock appended per measurement.
+ *   tdw_wdata_create   -- create_wdata_metadata (wslda_writevars.c:109-253):
+ *                         variables from the writevar keys (density|rho,
+ *                         current|j, tau, V, V_ext, velocity_ext, delta,
+ *                         delta_ext, nu, alpha, A, "all", "default"; none ->
+ *                         rho delta j, pca_utils.c:495-566), _b variables as
+ *                         links when spinsymmetry; datadim = the code's dim
+ *   tdw_wdata_setconst -- wdata_setconst (lib/wdata/c/wdata.c:1444-1462)
+ *   tdw_wdata_begin    -- clear_files + write_wdata_metadata_file
+ *                         (wslda_writevars.c:258-288, pca.c:1033-1038);
+ *                         TDW_EEXIST if overwrite is 0 and a file exists
+ *   tdw_wdata_write    -- write_measurments (wslda_writevars.c:298-452) of
+ *                         the device densities / potentials at step it, then
+ *                         the cycle count and the .wtxt rewrite
+ *                         (pca.c:1623-1630).  V is written as V - V_ext(it)
+ *                         (except at it == 0 for inittype 1..3), delta_ext as
+ *                         delta and velocity_ext as zero (the built-in V_ext
+ *                         has no pairing or velocity field); writeprec 'f'
+ *                         stores FP32 casts. */
+typedef struct {
+    const char *outprefix;   /* md.outprefix                                    */
+    const char *writevar;    /* writevar keys, separated by spaces or commas   */
+    char writeprec;          /* 'd' or 'f'                                      */
+    int spinsymmetry;
+    int overwrite;
+    int inittype;
+    double t0;               /* time of the first cycle                         */
+    double dt;               /* time between cycles (timesteps * dt)            */
+    const char *codename;    /* NULL: td-wslda-{1,2,3}d                         */
+} tdw_wdata_params;
+typedef struct tdw_wdata tdw_wdata;
+int tdw_wdata_create(tdw_ctx *c, const tdw_wdata_params *p, tdw_wdata **out);
+int tdw_wdata_setconst(tdw_wdata *w, const char *name, double value, const char *unit /* NULL: none */);
+int tdw_wdata_begin(tdw_wdata *w);
+int tdw_wdata_write(tdw_wdata *w, int it, const double *dens, const double *pots);
+int tdw_wdata_cycles(const tdw_wdata *w);
+int tdw_wdata_destroy(tdw_wdata *w);
 
 /* ---------------------------------------------------------------------------
  * Adjoint (reverse-mode) time stepping.  No counterpart in the reference:
@@ -333,6 +374,8 @@ int taylor_expansion_contribution(int it, double dt, int n, tdw_complex *wf_hpsi
                                   tdw_complex *wf_update, tdw_complex *wf_contr,
                                   int nthreads);                     /* pca_kernels.cu:1076 */
 int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kernels.cu:1106 (host data) */
+int get_delta_ext(int datadim, int it, void *deltain, void *data);  /* pca_kernels.cu:1135 (device in, host out) */
+int get_velocity_ext(int datadim, int spin, int it, double *data); /* pca_kernels.cu:1166 (host data, 3 blocks) */
 
 #if defined(TDW_CODEDIM) && TDW_CODEDIM == 1
 #define apply_hamiltonian apply_hamiltonian_1d

@@ -81,6 +81,12 @@ class AdjState(C.Structure):
                 ("w", C.c_void_p * 5), ("lgrid", C.c_void_p), ("theta_grad", C.c_void_p)]
 
 
+class WDataParams(C.Structure):
+    _fields_ = [("outprefix", C.c_char_p), ("writevar", C.c_char_p), ("writeprec", C.c_char),
+                ("spinsymmetry", C.c_int), ("overwrite", C.c_int), ("inittype", C.c_int),
+                ("t0", C.c_double), ("dt", C.c_double), ("codename", C.c_char_p)]
+
+
 def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
                 regularization=SPHERICAL_CUTOFF, mf_correction=0, scheme=AB4AM5,
                 mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
@@ -151,6 +157,12 @@ def _sigs():
         "tdw_checkpoint_save": (I, [P, C.c_char_p, C.POINTER(State), D, P, P, C.POINTER(CheckpointScalars)]),
         "tdw_checkpoint_load": (I, [P, C.c_char_p, C.POINTER(State), C.POINTER(C.c_double), P, P,
                                     C.POINTER(CheckpointScalars)]),
+        "tdw_wdata_create": (I, [P, C.POINTER(WDataParams), C.POINTER(C.c_void_p)]),
+        "tdw_wdata_setconst": (I, [P, C.c_char_p, D, C.c_char_p]),
+        "tdw_wdata_begin": (I, [P]),
+        "tdw_wdata_write": (I, [P, I, P, P]),
+        "tdw_wdata_cycles": (I, [P]),
+        "tdw_wdata_destroy": (I, [P]),
         "tdw_adj_backward_step": (I, [P, C.POINTER(AdjState), I, P, C.POINTER(C.c_void_p), P, P,
                                       C.POINTER(TapeStep), P, P]),
         "tdw_uniform_solve": (I, [C.POINTER(Params), D, D, D, D, D, D, D, D, D, I, I, C.POINTER(Uniform)]),
@@ -193,6 +205,8 @@ def _sigs():
         "abm_step4": (I, [I, P, P, P, I, I]),
         "taylor_expansion_contribution": (I, [I, D, I, P, P, P, I]),
         "get_v_ext": (I, [I, I, I, P]),
+        "get_delta_ext": (I, [I, I, P, P]),
+        "get_velocity_ext": (I, [I, I, I, P]),
     }
 
 
@@ -483,3 +497,45 @@ class TDRun:
         dim = self.ctx.params.dim
         ktr = None if dim == 3 else (kt[:n].copy() if dim == 2 else kt)
         return t0.value, fb, ktr, {k: getattr(sc, k) for k, _ in CheckpointScalars._fields_}
+
+
+class WData:
+    """W-data measurement output of a run (tdw_wdata_*: <outprefix>.wtxt and
+    <outprefix>_<var>.wdat, one datablock per measurement), as the td codes
+    write it (wslda_writevars.c).  consts: {name: value} written to the
+    metadata before the first cycle (pca.c:1026-1030 sets kF, eF, mu_a, mu_b)."""
+
+    def __init__(self, ctx, outprefix, writevar="default", writeprec="d", spinsymmetry=0, overwrite=1,
+                 inittype=0, t0=0.0, dt=1.0, codename=None, consts=None):
+        self.ctx = ctx
+        wp = WDataParams()
+        wp.outprefix = str(outprefix).encode()
+        wp.writevar = writevar.encode()
+        wp.writeprec = writeprec.encode()[:1]
+        wp.spinsymmetry, wp.overwrite, wp.inittype = int(spinsymmetry), int(overwrite), int(inittype)
+        wp.t0, wp.dt = float(t0), float(dt)
+        wp.codename = codename.encode() if codename else None
+        h = C.c_void_p()
+        check(lib().tdw_wdata_create(ctx.h, C.byref(wp), C.byref(h)), "tdw_wdata_create")
+        self.h = h
+        for k, v in (consts or {}).items():
+            check(lib().tdw_wdata_setconst(self.h, k.encode(), float(v), None), "tdw_wdata_setconst")
+        check(lib().tdw_wdata_begin(self.h), "tdw_wdata_begin")
+
+    def write(self, it, dens, pots):
+        check(lib().tdw_wdata_write(self.h, int(it), _ptr(dens), _ptr(pots)), "tdw_wdata_write")
+
+    @property
+    def cycles(self):
+        return lib().tdw_wdata_cycles(self.h)
+
+    def close(self):
+        if self.h:
+            lib().tdw_wdata_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
