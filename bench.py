@@ -194,7 +194,7 @@ def cpu_baseline(g, args):
 
 
 # HBM bytes per launch of the dominant kernel, from the FETCH_SIZE and
-# WRITE_SIZE counter runs of profiles/fused_fp64_24cubed_16384states.md
+# WRITE_SIZE counter runs of profiles/fused_step_24cubed_16384states.md
 # (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000 by the copy-kernel calibration),
 # keyed by (kernel, lattice, states, scheme, precision)
 MEASURED_TRAFFIC = {

@@ -271,6 +271,15 @@ typedef struct {
 } tdw_adj_state;
 int tdw_adj_forward_step(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_copy, tdw_complex *fnew_copy,
                          tdw_tape_step *t);
+/* The taped step without copies, for a caller whose tape holds y and the
+ * history entries themselves: y at the step's start is read from s->wf and
+ * kept; y after the step is written to y_next and s->wf set to it; the new
+ * history entry is written to f_next and becomes s->f[0] (the others shift
+ * by pointer; no history entry is written); s->ddx is used as scratch for
+ * the partial corrector.  y_next, f_next, s->ddx and s->lap must differ from
+ * s->wf, s->f[0..h-1] and from each other. */
+int tdw_adj_forward_step_to(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_next, tdw_complex *f_next,
+                            tdw_tape_step *t);
 int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *a, int it, const tdw_complex *y0,
                           const tdw_complex *const *f, const tdw_complex *y1, const tdw_complex *fnew,
                           const tdw_tape_step *t, const double *pots_prev, const double *dens_grad);

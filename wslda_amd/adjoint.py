@@ -93,9 +93,44 @@ class AdjointProblem:
         return self._tps[i]
 
     def _taped_step(self, R, k, r, i=0):
+        """Step k (row r of tape buffer i) with y and the new history entry
+        written straight into the tape (tdw_adj_forward_step_to).  At r == 0
+        the run's y is first copied into the tape's row 0."""
         ty, tf, _, _, tapes = self._tape(i)
-        td.check(td.lib().tdw_adj_forward_step(self.ctx.h, C.byref(R.st), self.it0 + k, _p(ty[r]), _p(tf[r]),
-                                               C.byref(tapes[r])), "tdw_adj_forward_step")
+        if r == 0 and R.st.wf != ty[0].data_ptr():
+            ty[0].copy_(self._tensor_at(R, R.st.wf))
+            R.st.wf = ty[0].data_ptr()
+        td.check(td.lib().tdw_adj_forward_step_to(self.ctx.h, C.byref(R.st), self.it0 + k, _p(ty[r + 1]),
+                                                  _p(tf[r]), C.byref(tapes[r])), "tdw_adj_forward_step_to")
+
+    def _tensor_at(self, R, ptr):
+        """The tensor (the run's own or a tape row) whose data is at ptr."""
+        for t in [R.wf] + list(R.f):
+            if t.data_ptr() == ptr:
+                return t
+        for (ty, tf, _, _, _) in self._tps:
+            for t in list(ty) + list(tf):
+                if t.data_ptr() == ptr:
+                    return t
+        raise td.TDWError("state pointer outside the run and its tape")
+
+    def _restore(self, R):
+        """Points the run's y and history back at its own tensors (copying the
+        values out of the tape), so that it can step and be checkpointed."""
+        own = {t.data_ptr() for t in R.f}
+        if R.st.wf != R.wf.data_ptr():
+            R.wf.copy_(self._tensor_at(R, R.st.wf))
+            R.st.wf = R.wf.data_ptr()
+        ptrs = [R.st.f[j] for j in range(5)]
+        if all(p in own for p in ptrs):
+            return
+        # values of every slot first (a slot may still hold one of the run's
+        # own tensors in a shifted position), then write them back in order
+        vals = [self._tensor_at(R, p).clone() for p in ptrs]
+        for j in range(5):
+            R.f[j].copy_(vals[j])
+            R.st.f[j] = R.f[j].data_ptr()
+        del vals
 
     def forward(self, keep_checkpoints=True):
         """Runs nsteps steps; returns the run (final y, densities, potentials).
@@ -115,8 +150,8 @@ class AdjointProblem:
             if keep_checkpoints and seg >= first_taped:
                 self._taped_step(R, k, r, seg - first_taped)
                 if r == K - 1 or k == self.nsteps - 1:
-                    self._tape(seg - first_taped)[0][r + 1].copy_(R.wf)
                     self.taped[seg] = seg - first_taped
+                    self._restore(R)
             else:
                 R.step(self.it0 + k, grad_last=False)
         self.ctx.sync()
@@ -165,7 +200,6 @@ class AdjointProblem:
                 R = self._new_run(y0, pots0, hist0)
                 for r in range(ks):
                     self._taped_step(R, k0 + r, r, ti)
-                self._tape(ti)[0][ks].copy_(R.wf)
                 del R
             ty, tf, tscal, tgrid, tapes = self._tape(ti)
             # adjoint sweep through the segment

@@ -133,17 +133,18 @@ k_cross_upd(double2 *out, const double2 *acc, const double2 *hterm, const double
     p[3 * N] = gDi;
 }
 
+// one thread per (gradient plane k, point i): 4N threads, each summing the
+// nchunk state-chunk partials in chunk order
 __global__ void k_cross_final(const double *__restrict__ part, int nchunk, long N, const double *__restrict__ add,
                               double *__restrict__ out)
 {
-    const long i = (long)blockIdx.x * AT + threadIdx.x;
-    if (i >= N) return;
-    for (int k = 0; k < 4; k++) {
-        double s = 0.0;
-        for (int c = 0; c < nchunk; c++) s += part[((long)c * 4 + k) * N + i];
-        if (add) s += add[k * N + i];
-        out[k * N + i] = s;
-    }
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * N) return;
+    const long k = t / N, i = t - k * N;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += part[((long)c * 4 + k) * N + i];
+    if (add) s += add[k * N + i];
+    out[k * N + i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -159,7 +160,7 @@ k_pot_vjp(FParams f, VextDev V, long N, const double *__restrict__ dens, const d
 {
     typedef Dual<8> D8;
     __shared__ double red[AT / 64];
-    const long i = (long)blockIdx.x * AT + threadIdx.x;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double lua = 0.0, lub = 0.0;
     if (i < N) {
         double ua, ub;
@@ -273,6 +274,7 @@ k_dens_t(const double2 *__restrict__ psi, double2 *out, const double *__restrict
 //   w3 <- yh_p = (y0 - i dt sum b f)/n_p ;
 //   w4 <- H(U_p) yh_p = i (n_c y1 - c)/(dt a0) + e_p yh_p ;  part: Re<w0, w3>
 // ---------------------------------------------------------------------------
+template <int H>
 __global__ void __launch_bounds__(AT)
 k_recon(double2 *ly, const double2 *__restrict__ y1, const double2 *__restrict__ y0, const double2 *__restrict__ f0,
         const double2 *__restrict__ f1, const double2 *__restrict__ f2, const double2 *__restrict__ f3,
@@ -295,10 +297,17 @@ k_recon(double2 *ly, const double2 *__restrict__ y1, const double2 *__restrict__
             ly[idx] = lyc;
             const double2 mu = imul(lyc, dt * cf.a0);
             w0[idx] = mu;
-            const double2 F[5] = {f0[idx], f1[idx], f2[idx], cf.h >= 4 ? f3[idx] : make_double2(0, 0),
-                                  cf.h >= 5 ? f4[idx] : make_double2(0, 0)};
+            // H (the scheme's history length) is a template parameter so that
+            // F stays in registers
+            double2 F[H];
+            F[0] = f0[idx];
+            F[1] = f1[idx];
+            F[2] = f2[idx];
+            if constexpr (H >= 4) F[3] = f3[idx];
+            if constexpr (H >= 5) F[4] = f4[idx];
             double2 sb = make_double2(0, 0), sa = make_double2(0, 0);
-            for (int j = 0; j < cf.h; j++) {
+#pragma unroll
+            for (int j = 0; j < H; j++) {
                 sb = sb + F[j] * cf.b[j];
                 sa = sa + F[j] * cf.a[j];
             }
@@ -376,7 +385,7 @@ static int cross_upd(tdw_ctx *c, int n, double2 *out, const double2 *acc, const 
     TDW_TRY(ensure_part(c, (size_t)nchunk * 4 * N));
     hipLaunchKernelGGL(k_cross_upd, dim3(nblk, nchunk), dim3(AT), 0, c->stream, out, acc, hterm, g1, g2, mu, psi, sv,
                        ev, em, n, N, chunk, c->d_part);
-    hipLaunchKernelGGL(k_cross_final, dim3(nblk), dim3(AT), 0, c->stream, c->d_part, nchunk, N, add, lU);
+    hipLaunchKernelGGL(k_cross_final, dim3((4 * N + 63) / 64), dim3(64), 0, c->stream, c->d_part, nchunk, N, add, lU);
     return (int)hipGetLastError();
 }
 
@@ -385,9 +394,12 @@ static int pot_vjp(tdw_ctx *c, int it, const double *dens, const double *pots_pr
 {
     TDW_TIMED(c, "adj_pot_vjp");
     const long N = (long)c->N;
-    const int nblk = (int)((N + AT - 1) / AT);
+    // one wave per workgroup: the kernel is register-bound (one wave per
+    // SIMD), so small grids spread over more CUs
+    constexpr int PT = 64;
+    const int nblk = (int)((N + PT - 1) / PT);
     TDW_TRY(ensure_part(c, (size_t)nblk * NTHETA));
-    hipLaunchKernelGGL(k_pot_vjp, dim3(nblk), dim3(AT), 0, c->stream, fparams(c), make_vext(c, it), N, dens,
+    hipLaunchKernelGGL(k_pot_vjp, dim3(nblk), dim3(PT), 0, c->stream, fparams(c), make_vext(c, it), N, dens,
                        pots_prev, lU, dens_add, lD, lUprev, c->d_part);
     hipLaunchKernelGGL(k_gsum, dim3(1), dim3(64), 0, c->stream, c->d_part, nblk, theta_grad);
     return (int)hipGetLastError();
@@ -435,44 +447,90 @@ using namespace tdw;
 #define D2(p) ((double2 *)(p))
 #define CD2(p) ((const double2 *)(p))
 
+// The taped step.  y_out: where the new y goes (NULL: in place); cbuf: the
+// partial corrector's buffer (NULL: the oldest history entry, as abm_step1
+// does); f_next: the new history entry's buffer (NULL: the dropped oldest
+// entry).  With y_out, cbuf and f_next the step writes no history entry and
+// leaves y at its start unchanged.
+static int adj_forward(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_out, double2 *cbuf, tdw_complex *f_next,
+                       tdw_tape_step *t)
+{
+    TDW_TRY(fused_finalize(c, s));
+    const int n = s->n;
+    const AbmCoef cf = abm_coef(c->p.scheme);
+    const int k = cf.h;
+    const long N = (long)c->N;
+    double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    // the call sequence of tdw_step (pca.c:1401-1570), non-gradient form;
+    // the densities of a non-gradient step do not read the Laplacian, and 2D
+    // lattices with a plane kernel form it inside the H application
+    const bool plane = h2d_supported(c);
+    if (y_out) {
+        TDW_TRY(abm1_to(c, n, CD2(s->wf), D2(y_out), cbuf, D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]),
+                        D2(s->f[4])));
+        s->wf = y_out;
+    } else {
+        TDW_TRY(abm1_to(c, n, CD2(s->wf), D2(s->wf), cbuf, D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]),
+                        D2(s->f[4])));
+    }
+    const double2 *flast = cbuf ? cbuf : CD2(s->f[k - 1]);
+    TDW_TRY(normalize(c, n, D2(s->wf), t->scal));
+    if (!plane) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
+                      s->dens, 0));
+    TDW_TRY(copy_d(c, s->dens, t->dens_p, 12 * N));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(copy_d(c, s->pots, t->pots_p, 12 * N));
+    if (plane) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->lap), s->pots, c->d_kt, nullptr, t->scal + n));
+    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + n));
+    TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), flast));
+    TDW_TRY(normalize(c, n, D2(s->wf), t->scal + 2 * n));
+    if (!plane) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
+                      s->dens, 0));
+    TDW_TRY(copy_d(c, s->dens, t->dens_c, 12 * N));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(copy_d(c, s->pots, t->pots_c, 12 * N));
+    tdw_complex *tmp = f_next ? f_next : s->f[k - 1];
+    for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+    s->f[0] = tmp;
+    if (plane)
+        TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->f[0]), s->pots, c->d_kt, nullptr, t->scal + 3 * n));
+    else
+        TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + 3 * n));
+    TDW_HIP(hipMemcpyAsync(qpe, t->scal + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
 extern "C" int tdw_adj_forward_step(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_copy, tdw_complex *fnew_copy,
                                     tdw_tape_step *t)
 {
     if (!c || !s || !t || s->n > c->nwf || t->n != s->n) return TDW_EINVAL;
     if (!adj_supported(c)) return TDW_EINVAL;
     TDW_TRY(fused_finalize(c, s));
-    const int n = s->n;
-    const AbmCoef cf = abm_coef(c->p.scheme);
-    const int k = cf.h;
-    const long N = (long)c->N;
-    const size_t sbytes = sizeof(double2) * 2 * (size_t)n * N;
-    double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    const size_t sbytes = sizeof(double2) * 2 * (size_t)s->n * c->N;
     if (y_copy) TDW_HIP(hipMemcpyAsync(y_copy, s->wf, sbytes, hipMemcpyDeviceToDevice, c->stream));
-    // the call sequence of tdw_step (pca.c:1401-1570), non-gradient form
-    TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
-    TDW_TRY(normalize(c, n, D2(s->wf), t->scal));
-    TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
-    TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
-                      s->dens, 0));
-    TDW_TRY(copy_d(c, s->dens, t->dens_p, 12 * N));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
-    TDW_TRY(copy_d(c, s->pots, t->pots_p, 12 * N));
-    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + n));
-    TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
-    TDW_TRY(normalize(c, n, D2(s->wf), t->scal + 2 * n));
-    TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
-    TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
-                      s->dens, 0));
-    TDW_TRY(copy_d(c, s->dens, t->dens_c, 12 * N));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
-    TDW_TRY(copy_d(c, s->pots, t->pots_c, 12 * N));
-    tdw_complex *tmp = s->f[k - 1];
-    for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
-    s->f[0] = tmp;
-    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + 3 * n));
-    TDW_HIP(hipMemcpyAsync(qpe, t->scal + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    TDW_TRY(adj_forward(c, s, it, nullptr, nullptr, nullptr, t));
     if (fnew_copy) TDW_HIP(hipMemcpyAsync(fnew_copy, s->f[0], sbytes, hipMemcpyDeviceToDevice, c->stream));
     return 0;
+}
+
+extern "C" int tdw_adj_forward_step_to(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_next, tdw_complex *f_next,
+                                       tdw_tape_step *t)
+{
+    if (!c || !s || !t || !y_next || !f_next || !s->ddx || s->n > c->nwf || t->n != s->n) return TDW_EINVAL;
+    if (!adj_supported(c)) return TDW_EINVAL;
+    const int h = abm_coef(c->p.scheme).h;
+    // every buffer the step writes must differ from those it reads
+    const tdw_complex *bufs[4] = {y_next, f_next, s->ddx, s->lap};
+    for (int i = 0; i < 4; i++) {
+        if (bufs[i] == s->wf) return TDW_EINVAL;
+        for (int j = 0; j < h; j++)
+            if (bufs[i] == s->f[j]) return TDW_EINVAL;
+        for (int j = i + 1; j < 4; j++)
+            if (bufs[i] == bufs[j]) return TDW_EINVAL;
+    }
+    return adj_forward(c, s, it, y_next, D2(s->ddx), f_next, t);
 }
 
 extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const tdw_complex *y0,
@@ -499,9 +557,13 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
     TimerScope ts(c, "adj_step");
 
     // history refresh f_new = H(U') y' - e' y'  (mu = dL/df_new)
-    TDW_TRY(sdot(c, n, MU, CD2(y1), dv, sv1));
-    TDW_TRY(laplace(c, 2 * n, MU, W2));
-    TDW_TRY(apply_h_only(c, n, MU, W1, W2, t->pots_c, c->d_kt));
+    const bool plane = h2d_supported(c);
+    if (plane) TDW_TRY(apply_h_only_plane(c, n, MU, W1, t->pots_c, c->d_kt, CD2(y1), dv, sv1));
+    else {
+        TDW_TRY(sdot(c, n, MU, CD2(y1), dv, sv1));
+        TDW_TRY(laplace(c, 2 * n, MU, W2));
+        TDW_TRY(apply_h_only(c, n, MU, W1, W2, t->pots_c, c->d_kt));
+    }
     TDW_TRY(cross_upd(c, n, LY, LY, W1, CD2(fnew), CD2(y1), MU, CD2(y1), sv1, e_n, e_n, A->lpots, lUc));
     // U' = Phi(D', U_p, V_ext)
     TDW_TRY(pot_vjp(c, it + 1, t->dens_c, t->pots_p, lUc, dens_grad, lD, lUo, A->theta_grad));
@@ -515,14 +577,18 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
         const double2 *F[5] = {CD2(f[0]), CD2(f[1]), CD2(f[2]), h >= 4 ? CD2(f[3]) : nullptr,
                                h >= 5 ? CD2(f[4]) : nullptr};
         TDW_TIMED(c, "adj_recon");
-        hipLaunchKernelGGL(k_recon, dim3(nchunk, n), dim3(AT), 0, c->stream, LY, CD2(y1), CD2(y0), F[0], F[1], F[2],
-                           F[3], F[4], W0, W3, W4, t1, t->scal, cf, c->p.dt, n, N, chunk, c->d_part);
+        auto kr = h == 3 ? k_recon<3> : h == 4 ? k_recon<4> : k_recon<5>;
+        hipLaunchKernelGGL(kr, dim3(nchunk, n), dim3(AT), 0, c->stream, LY, CD2(y1), CD2(y0), F[0], F[1], F[2], F[3],
+                           F[4], W0, W3, W4, t1, t->scal, cf, c->p.dt, n, N, chunk, c->d_part);
         hipLaunchKernelGGL(k_rows, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk, dv, svp);
         TDW_HIP(hipGetLastError());
     }
     // F_p = H(U_p) yh_p - e_p yh_p
-    TDW_TRY(laplace(c, 2 * n, W0, W2));
-    TDW_TRY(apply_h_only(c, n, W0, W1, W2, t->pots_p, c->d_kt));
+    if (plane) TDW_TRY(apply_h_only_plane(c, n, W0, W1, t->pots_p, c->d_kt));
+    else {
+        TDW_TRY(laplace(c, 2 * n, W0, W2));
+        TDW_TRY(apply_h_only(c, n, W0, W1, W2, t->pots_p, c->d_kt));
+    }
     TDW_TRY(cross_upd(c, n, W1, nullptr, W1, W4, nullptr, W0, W3, svp, nullptr, e_p, lUo, lUc));
     // U_p = Phi(D_p, U_prev, V_ext)
     TDW_TRY(pot_vjp(c, it + 1, t->dens_p, pots_prev, lUc, nullptr, lD, A->lpots, A->theta_grad));

@@ -379,11 +379,13 @@ extern "C" int tdw_vext_eval(tdw_ctx *c, int it, double *va, double *vb)
 // ---------------------------------------------------------------------------
 // modular PECE step: the call sequence of pca.c:1401-1570
 // ---------------------------------------------------------------------------
-static int half_densities(tdw_ctx *c, tdw_state *s, int grad)
+// skip_lap: a non-gradient half-step whose H application forms the
+// Laplacian itself (apply_h_plane); the densities do not read it then
+static int half_densities(tdw_ctx *c, tdw_state *s, int grad, bool skip_lap = false)
 {
     const int n = s->n;
     if (grad) TDW_TRY(derivatives(c, 2 * n, D2(s->wf), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap)));
-    else TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    else if (!skip_lap) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
     TDW_TRY(densities(c, n, D2(s->wf), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), c->d_fbeta, nullptr,
                       c->d_kt, c->d_cnt, s->dens, grad));
     if (grad) TDW_TRY(density_tau(c, s->dens));
@@ -404,21 +406,28 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(fused_finalize(c, s));
     const int n = s->n, k = nhist(c);
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    // 2D lattices with a one-pass plane kernel apply H with the Laplacian
+    // formed in LDS on non-gradient half-steps (lap then holds H y_p, not the
+    // Laplacian, after the step)
+    const bool plane = h2d_supported(c);
     // predictor
     TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
-    TDW_TRY(half_densities(c, s, 0));
+    TDW_TRY(half_densities(c, s, 0, plane));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
-    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    if (plane) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
     // corrector
     TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
-    TDW_TRY(half_densities(c, s, grad_last));
+    const bool plane_c = plane && !grad_last;
+    TDW_TRY(half_densities(c, s, grad_last, plane_c));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
     tdw_complex *tmp = s->f[k - 1];
     for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
-    TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    if (plane_c) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->f[0]), s->pots, c->d_kt, nullptr, qpe));
+    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
     return 0;
 }
 

@@ -116,20 +116,12 @@ k_inner(const double2 *in, const double2 *T, double2 *out,
             if (T) O[k] = T[off + e];
         }
     }
-#pragma unroll
-    for (int axi = 0; axi < NAX; axi++) {
-        // axis geometry: 3D: axi 0 = z (contiguous), axi 1 = y (stride nz)
-        //                2D: axi 0 = y (contiguous)
-        const bool isz = (NAX == 2 && axi == 0);
-        const tdw_axis &ax = isz ? axz : axy;
-        const double2 *tw = isz ? tz : ty;
+    // one axis: forward FFT, spectral factor(s), inverse FFT(s).  The axis
+    // plan is passed as the kernel argument itself (not a runtime selection
+    // between the two), so that its stage tables stay in kernel-argument
+    // memory instead of a private copy.
+    auto run_axis = [&](const tdw_axis &ax, const double2 *tw, double Lp, int P, int ps, int is, bool isz) {
         const int L = ax.L;
-        const double Lp = isz ? Lz : Ly;
-        int P, ps, is;
-        if (NAX == 2) {
-            if (isz) { P = ny; ps = nz; is = 1; }
-            else { P = nz; ps = 1; is = nz; }
-        } else { P = TB; ps = ny; is = 1; }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KM; k++) {
@@ -155,10 +147,17 @@ k_inner(const double2 *in, const double2 *T, double2 *out,
         if (MODE == 1) {
             __syncthreads();
             double2 *R2 = lds_fft(D, A, P, ps, is, ax, tw, +1, tid, NTHR);
-            double2 *dd = (NAX == 2 && isz) ? ddz : ddy;
+            double2 *dd = isz ? ddz : ddy;
             for (int e = tid; e < E; e += NTHR)
                 if (off + e < lim) dd[off + e] = R2[e];
         }
+    };
+    // axis geometry: 3D: z (contiguous) then y (stride nz); 2D: y (contiguous)
+    if (NAX == 2) {
+        run_axis(axz, tz, Lz, ny, nz, 1, true);
+        run_axis(axy, ty, Ly, nz, 1, nz, false);
+    } else {
+        run_axis(axy, ty, Ly, TB, ny, 1, false);
     }
 #pragma unroll
     for (int k = 0; k < KM; k++) {
@@ -316,6 +315,39 @@ template <int L> __device__ __forceinline__ int perm_freq(int p)
     return k1 + R1 * k2;
 }
 
+// The 2D Laplacian of the plane held in LDS (row stride NY + 1), in place:
+// forward transforms along y then x, -(ksx mx^2 + ksy my^2) at the permuted
+// frequencies, inverse along x then y.
+template <int NX, int NY>
+__device__ __forceinline__ void lap_plane(double2 *A, const double2 *twx, const double2 *twy, int tid, double ksx,
+                                          double ksy)
+{
+    constexpr int NYP = NY + 1;
+    fwd2<NY, NX, NYP, 1>(A, twy, tid);
+    fwd2<NX, NY, 1, NYP>(A, twx, tid);
+    for (int e = tid; e < NX * NY; e += NTHR) {
+        const int px = e / NY, py = e - px * NY;
+        const int fx = perm_freq<NX>(px), fy = perm_freq<NY>(py);
+        const int mx = (fx < NX / 2) ? fx : fx - NX, my = (fy < NY / 2) ? fy : fy - NY;
+        A[px * NYP + py] = A[px * NYP + py] * (-(ksx * (double)(mx * mx) + ksy * (double)(my * my)));
+    }
+    __syncthreads();
+    inv2<NX, NY, 1, NYP>(A, twx, tid);
+    inv2<NY, NX, NYP, 1>(A, twy, tid);
+}
+
+template <int NX, int NY>
+__device__ __forceinline__ void load_plane(double2 *A, double2 *twx, double2 *twy, const double2 *src,
+                                           const double2 *__restrict__ twx_g, const double2 *__restrict__ twy_g,
+                                           int tid)
+{
+    constexpr int NYP = NY + 1;
+    for (int k = tid; k < NX; k += NTHR) twx[k] = twx_g[k];
+    for (int k = tid; k < NY; k += NTHR) twy[k] = twy_g[k];
+    for (int e = tid; e < NX * NY; e += NTHR) A[(e / NY) * NYP + (e % NY)] = src[e];
+    __syncthreads();
+}
+
 template <int NX, int NY>
 __global__ void __launch_bounds__(NTHR)
 k_lap2d(const double2 *in, double2 *out, long N, const double2 *__restrict__ twx_g,
@@ -327,24 +359,87 @@ k_lap2d(const double2 *in, double2 *out, long N, const double2 *__restrict__ twx
     double2 *A = lds2;
     const int tid = threadIdx.x;
     const long r0 = (long)blockIdx.x * N;
+    load_plane<NX, NY>(A, twx, twy, in + r0, twx_g, twy_g, tid);
+    lap_plane<NX, NY>(A, twx, twy, tid, ksx, ksy);
+    for (int e = tid; e < NX * NY; e += NTHR) out[r0 + e] = A[(e / NY) * NYP + (e % NY)];
+}
+
+// H applied to one plane (one component of one state) per workgroup, with
+// the Laplacian kept in LDS: plane p < n is u_p, p >= n is v_{p-n}; the
+// other component is read pointwise.  The pointwise part is that of
+// apply_hamiltonian_2d (as in k_apply_h), with kz the state's transverse
+// momentum and alpha = 1 for BdG:
+//   (H psi)_u =  (V_a - mu_a + alpha_a kz^2/2) u + Delta v  - alpha_a/2 lap u
+//   (H psi)_v = -(V_b - mu_b + alpha_b kz^2/2) v + Delta* u + alpha_b/2 lap v
+// part[p] = Re sum_i conj(psi_p) (H psi)_p; with dotw, also
+// part2[p] = Re sum_i conj(psi_p) dotw_p.  out must not alias in (the other
+// component is read from in).
+template <int NX, int NY>
+__global__ void __launch_bounds__(NTHR)
+k_h2d(const double2 *__restrict__ in, double2 *__restrict__ out, long N, int n, const double2 *__restrict__ twx_g,
+      const double2 *__restrict__ twy_g, double ksx, double ksy, const double *__restrict__ pots,
+      const double *__restrict__ kt, int bdg, double mu_a, double mu_b, double *__restrict__ part,
+      const double2 *__restrict__ dotw, double *__restrict__ part2)
+{
+    constexpr int NYP = NY + 1, E = NX * NY, KM = (E + NTHR - 1) / NTHR;
+    extern __shared__ double2 lds2[];
+    __shared__ double2 twx[NX], twy[NY];
+    __shared__ double red[NTHR / 64];
+    double2 *A = lds2;
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x;
+    const int comp = p >= n ? 1 : 0;
+    const int a = p - comp * n;
+    const long own = (long)p * N, oth = (long)(comp ? a : n + a) * N;
     for (int k = tid; k < NX; k += NTHR) twx[k] = twx_g[k];
     for (int k = tid; k < NY; k += NTHR) twy[k] = twy_g[k];
-    for (int e = tid; e < NX * NY; e += NTHR) A[(e / NY) * NYP + (e % NY)] = in[r0 + e];
-    __syncthreads();
-    // y transforms (NX pencils, contiguous), forward
-    fwd2<NY, NX, NYP, 1>(A, twy, tid);
-    // x transforms (NY pencils, stride NYP), forward, -(kx^2 + ky^2), inverse
-    fwd2<NX, NY, 1, NYP>(A, twx, tid);
-    for (int e = tid; e < NX * NY; e += NTHR) {
-        const int px = e / NY, py = e - px * NY;
-        const int fx = perm_freq<NX>(px), fy = perm_freq<NY>(py);
-        const int mx = (fx < NX / 2) ? fx : fx - NX, my = (fy < NY / 2) ? fy : fy - NY;
-        A[px * NYP + py] = A[px * NYP + py] * (-(ksx * (double)(mx * mx) + ksy * (double)(my * my)));
+    // the plane stays in registers for the pointwise part; the other
+    // component is fetched while the transforms run
+    double2 X[KM], Y[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NTHR;
+        if (e < E) {
+            X[k] = in[own + e];
+            A[(e / NY) * NYP + (e % NY)] = X[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NTHR;
+        if (e < E) Y[k] = in[oth + e];
     }
     __syncthreads();
-    inv2<NX, NY, 1, NYP>(A, twx, tid);
-    inv2<NY, NX, NYP, 1>(A, twy, tid);
-    for (int e = tid; e < NX * NY; e += NTHR) out[r0 + e] = A[(e / NY) * NYP + (e % NY)];
+    lap_plane<NX, NY>(A, twx, twy, tid, ksx, ksy);
+    const double k2t = kt ? kt[a] * kt[a] : 0.0;
+    double s = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NTHR;
+        if (e >= E) continue;
+        const double2 lp = A[(e / NY) * NYP + (e % NY)];
+        const double2 x = X[k], y = Y[k];
+        const double2 D = make_double2(pots[2 * N + 2 * e], pots[2 * N + 2 * e + 1]);
+        double2 r;
+        if (!comp) {
+            const double aa = bdg ? -0.5 : pots[4 * N + e] * -0.5;
+            const double Va = pots[e] - mu_a + (-1.0 * aa * k2t);
+            r = x * Va + cmul(y, D) + lp * aa;
+        } else {
+            const double ab = bdg ? -0.5 : pots[5 * N + e] * -0.5;
+            const double Vb = pots[N + e] - mu_b + (-1.0 * ab * k2t);
+            r = x * (-Vb) + cmulc(D, y) - lp * ab;
+        }
+        out[own + e] = r;
+        s += cmulc(x, r).x;
+        if (dotw) s2 += cmulc(x, dotw[own + e]).x;
+    }
+    s = block_sum(s, red);
+    if (tid == 0) part[p] = s;
+    if (dotw) {
+        s2 = block_sum(s2, red);
+        if (tid == 0) part2[p] = s2;
+    }
 }
 
 typedef void (*lap2d_fn)(const double2 *, double2 *, long, const double2 *, const double2 *, double, double);
@@ -379,6 +474,47 @@ int lap2d(tdw_ctx *c, int m, const double2 *in, double2 *out, int *done)
     const double ksx = pow(2.0 * M_PI / c->Lphys[0], 2) / N, ksy = pow(2.0 * M_PI / c->Lphys[1], 2) / N;
     hipLaunchKernelGGL(f, dim3(m), dim3(NTHR), lds, c->stream, in, out, (long)c->N, c->d_tw[0], c->d_tw[1], ksx,
                        ksy);
+    TDW_HIP(hipGetLastError());
+    return 0;
+}
+
+typedef void (*h2d_fn)(const double2 *, double2 *, long, int, const double2 *, const double2 *, double, double,
+                       const double *, const double *, int, double, double, double *, const double2 *, double *);
+
+static bool h2d_for(const tdw_ctx *c, h2d_fn *f, size_t *lds)
+{
+    if (c->dim != 2 || c->nx != c->ny) return false;
+    switch (c->nx) {
+    case 8: *f = k_h2d<8, 8>; break;
+    case 12: *f = k_h2d<12, 12>; break;
+    case 16: *f = k_h2d<16, 16>; break;
+    case 24: *f = k_h2d<24, 24>; break;
+    case 32: *f = k_h2d<32, 32>; break;
+    case 64: *f = k_h2d<64, 64>; break;
+    default: return false;
+    }
+    *lds = sizeof(double2) * (size_t)c->nx * (c->ny + 1);
+    return *lds <= (size_t)c->max_lds;
+}
+
+bool h2d_supported(const tdw_ctx *c)
+{
+    h2d_fn f;
+    size_t lds;
+    return h2d_for(c, &f, &lds);
+}
+
+int h2d(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, const double *kt, double *part,
+        const double2 *dotw, double *part2)
+{
+    h2d_fn f;
+    size_t lds;
+    if (!h2d_for(c, &f, &lds) || in == out) return TDW_EINVAL;
+    TDW_TIMED(c, "h2d");
+    const double N = (double)c->N;
+    const double ksx = pow(2.0 * M_PI / c->Lphys[0], 2) / N, ksy = pow(2.0 * M_PI / c->Lphys[1], 2) / N;
+    hipLaunchKernelGGL(f, dim3(2 * n), dim3(NTHR), lds, c->stream, in, out, (long)c->N, n, c->d_tw[0], c->d_tw[1],
+                       ksx, ksy, pots, kt, c->p.functional == TDW_BDG, c->p.mu_a, c->p.mu_b, part, dotw, part2);
     TDW_HIP(hipGetLastError());
     return 0;
 }

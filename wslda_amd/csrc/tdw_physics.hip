@@ -17,12 +17,12 @@ static constexpr int NT = 256;
 // ---------------------------------------------------------------------------
 template <int SCH>
 __global__ void __launch_bounds__(NT)
-k_abm1(long tot, double dt, double2 *__restrict__ y, double2 *__restrict__ f1, double2 *__restrict__ f2,
-       double2 *__restrict__ f3, double2 *__restrict__ f4, double2 *__restrict__ f5)
+k_abm1(long tot, double dt, const double2 *yin, double2 *y, double2 *cout, double2 *f1, double2 *f2, double2 *f3,
+       double2 *f4, double2 *f5)
 {
     long i = (long)blockIdx.x * NT + threadIdx.x;
     if (i >= tot) return;
-    const double2 yk = y[i];
+    const double2 yk = yin[i];
     const double2 a = f1[i], b = f2[i], d = f3[i];
     double2 sp, sc;
     if (SCH == 34) {
@@ -39,8 +39,9 @@ k_abm1(long tot, double dt, double2 *__restrict__ y, double2 *__restrict__ f1, d
     }
     // y + (-i dt) * s
     y[i] = yk + imul(sp, -dt);
-    double2 c = yk + imul(sc, -dt);
-    if (SCH == 34) f3[i] = c;
+    const double2 c = yk + imul(sc, -dt);
+    if (cout) cout[i] = c;
+    else if (SCH == 34) f3[i] = c;
     else if (SCH == 45) f4[i] = c;
     else f5[i] = c;
 }
@@ -55,19 +56,27 @@ k_abm4(long tot, double dta0, const double2 *__restrict__ hy, double2 *__restric
     y[i] = imul(hy[i], -dta0) + fl[i];
 }
 
-int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, double2 *f4, double2 *f5)
+int abm1_to(tdw_ctx *c, int n, const double2 *yin, double2 *y, double2 *cout, double2 *f1, double2 *f2, double2 *f3,
+            double2 *f4, double2 *f5)
 {
     TDW_TIMED(c, "abm_step1");
     long tot = 2L * n * (long)c->N;
     dim3 g((tot + NT - 1) / NT);
     double dt = c->p.dt;
+#define TDW_A1(S) hipLaunchKernelGGL(k_abm1<S>, g, dim3(NT), 0, c->stream, tot, dt, yin, y, cout, f1, f2, f3, f4, f5)
     switch (c->p.scheme) {
-    case TDW_AB3AM4: hipLaunchKernelGGL(k_abm1<34>, g, dim3(NT), 0, c->stream, tot, dt, y, f1, f2, f3, f4, f5); break;
-    case TDW_AB4AM5: hipLaunchKernelGGL(k_abm1<45>, g, dim3(NT), 0, c->stream, tot, dt, y, f1, f2, f3, f4, f5); break;
-    case TDW_AB5AM5: hipLaunchKernelGGL(k_abm1<55>, g, dim3(NT), 0, c->stream, tot, dt, y, f1, f2, f3, f4, f5); break;
+    case TDW_AB3AM4: TDW_A1(34); break;
+    case TDW_AB4AM5: TDW_A1(45); break;
+    case TDW_AB5AM5: TDW_A1(55); break;
     default: return TDW_EINVAL;
     }
+#undef TDW_A1
     return (int)hipGetLastError();
+}
+
+int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, double2 *f4, double2 *f5)
+{
+    return abm1_to(c, n, y, y, nullptr, f1, f2, f3, f4, f5);
 }
 
 int abm4(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl)
@@ -481,6 +490,46 @@ int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const do
     }
     long tot = 2L * n * N;
     hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);
+    return (int)hipGetLastError();
+}
+
+// qpe[a] = dv (part[a] + part[n + a]): <H> from the per-plane sums of h2d
+__global__ void k_plane_pairs(const double *__restrict__ part, int n, double dv, double *__restrict__ qpe)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) qpe[a] = dv * (part[a] + part[n + a]);
+}
+
+// apply_h for 2D lattices with a one-pass plane kernel: the Laplacian of in
+// is formed in LDS and not stored (lap of apply_h is not needed)
+int apply_h_plane(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const double *pots, const double *kt,
+                  const double *useqpe, double *d_qpe)
+{
+    (void)it;
+    TDW_TRY(ensure_part(c, (size_t)2 * n));
+    TDW_TRY(h2d(c, n, in, out, pots, kt, c->d_part));
+    TDW_TIMED(c, "apply_h");
+    const double *q = useqpe;
+    if (!q) {
+        hipLaunchKernelGGL(k_plane_pairs, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, dvol_norm(c),
+                           d_qpe);
+        q = d_qpe;
+    } else if (d_qpe && d_qpe != useqpe) {
+        TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const long N = (long)c->N, tot = 2L * n * N;
+    hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);
+    return (int)hipGetLastError();
+}
+
+int apply_h_only_plane(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, const double *kt,
+                       const double2 *dotw, double dscale, double *d_dot)
+{
+    TDW_TRY(ensure_part(c, (size_t)4 * n));
+    TDW_TRY(h2d(c, n, in, out, pots, kt, c->d_part, dotw, dotw ? c->d_part + 2 * n : nullptr));
+    if (dotw)
+        hipLaunchKernelGGL(k_plane_pairs, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part + 2 * n, n, dscale,
+                           d_dot);
     return (int)hipGetLastError();
 }
 

@@ -76,6 +76,10 @@ inline double dens_transverse(const tdw_ctx *c)
 }
 
 int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, double2 *f4, double2 *f5);
+// abm1 reading y from yin and writing the prediction to y (may equal yin);
+// with cout the partial corrector goes there and the history is only read
+int abm1_to(tdw_ctx *c, int n, const double2 *yin, double2 *y, double2 *cout, double2 *f1, double2 *f2, double2 *f3,
+            double2 *f4, double2 *f5);
 int abm4(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl);
 int norms(tdw_ctx *c, int n, const double2 *wf, double *d_nrm);
 int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm);
@@ -92,6 +96,13 @@ int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const do
             const double *pots, const double *kt, const double *useqpe, double *d_qpe);
 int apply_h_only(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *lap, const double *pots,
                  const double *kt);
+// the same two on 2D lattices with a one-pass plane kernel (h2d_supported):
+// the Laplacian of in is formed in LDS; out must differ from in
+int apply_h_plane(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const double *pots, const double *kt,
+                  const double *useqpe, double *d_qpe);
+// With dotw, also d_dot[a] = dscale Re<in_a, dotw_a> (both components).
+int apply_h_only_plane(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, const double *kt,
+                       const double2 *dotw = nullptr, double dscale = 0.0, double *d_dot = nullptr);
 int energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e);
 int taylor(tdw_ctx *c, int order, double dt, int n, const double2 *hpsi, double2 *upd, double2 *contr);
 
