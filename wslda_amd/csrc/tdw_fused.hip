@@ -18,6 +18,13 @@
 // e = <H>) into the next pass instead of extra passes over the state: the
 // stored y stays unnormalised and the newest history entry keeps H psi until
 // the next predictor (or tdw_finalize) applies them.
+//
+// Split precision: the history, T and H are FP32; y and every ABM update of
+// it stay FP64.  The predictor writes the FP64 partial corrector c into y
+// and an FP32 copy of the prediction into the history slot it consumed; the
+// densities and the yz pass read that copy.  The corrector reads c and the
+// copy, writes the FP64 y and an FP32 copy of it (into H's buffer, which it
+// has consumed) for the second density and yz passes.
 #include "tdw_fft.hpp"
 #include "tdw_fused.hpp"
 #include "tdw_physics.hpp"
@@ -80,9 +87,8 @@ __device__ __forceinline__ void put_norm_part(double nrm, bool wave, long P, lon
 
 // ---------------------------------------------------------------------------
 // predictor x-pass
-//   plain: fold <- c = y_k - i dt sum_j a_j f_j            (FP64)
-//   split: fold <- c - y_p = -i dt sum_j (a_j - b_j) f_j   (FP32, a dt-sized
-//          increment; the corrector adds it to the FP64 y_p)
+//   plain: y <- y_p (FP64), fold <- c = y_k - i dt sum_j a_j f_j (FP64)
+//   split: y <- c (FP64),   fold <- y_p (FP32 copy)
 // ---------------------------------------------------------------------------
 template <int LX, int SCH, bool SP>
 __global__ void __launch_bounds__(FT)
@@ -118,9 +124,13 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
             double2 sp, sc;
             abm_sums<SCH>(F1, F2, F3, F4, F5, sp, sc);
             const double2 yp = yk + imul(sp, -dt);
-            y[idx] = yp;
-            if constexpr (SP) fold[idx] = to32(imul(sc - sp, -dt));
-            else fold[idx] = yk + imul(sc, -dt);
+            if constexpr (SP) {
+                y[idx] = yk + imul(sc, -dt);
+                fold[idx] = to32(yp);
+            } else {
+                y[idx] = yp;
+                fold[idx] = yk + imul(sc, -dt);
+            }
             r[i] = cvt<HC>(yp);
             nrm += cnorm(yp);
         }
@@ -133,12 +143,15 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
 
 // ---------------------------------------------------------------------------
 // corrector x-pass: y = -i dt a0 (H - <H>) psi_p + c
+//   plain: y holds y_p, cin c;  split: y holds c, cin the FP32 y_p, and the
+//   FP32 copy of the new y goes to y32 (H's buffer: each element of hin is
+//   read by the thread that then writes it)
 // ---------------------------------------------------------------------------
 template <int LX, bool SP>
 __global__ void __launch_bounds__(FT)
-f_corr(double2 *__restrict__ y, const hc_t<SP> *__restrict__ hin, const hc_t<SP> *__restrict__ cin,
+f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict__ cin,
        hc_t<SP> *__restrict__ T, const double *__restrict__ sp_, const double *__restrict__ ep_, int n, long N,
-       int S, double dta0, double kscale, int wave, double *__restrict__ part)
+       int S, double dta0, double kscale, int wave, double *__restrict__ part, float2 *y32)
 {
     typedef hc_t<SP> HC;
     const long P = (long)blockIdx.x * FT + threadIdx.x;
@@ -154,14 +167,14 @@ f_corr(double2 *__restrict__ y, const hc_t<SP> *__restrict__ hin, const hc_t<SP>
 #pragma unroll
         for (int i = 0; i < LX; i++) {
             const long idx = base + (long)i * S;
-            const double2 yp = y[idx];
+            const double2 yp = SP ? to64(cin[idx]) : y[idx];
+            const double2 cc = SP ? y[idx] : to64(cin[idx]);
             const double2 psi = yp * s;
             const double2 fp = to64(hin[idx]) - psi * e;        // (H - <H>) psi_p
-            double2 yn;
-            if constexpr (SP) yn = imul(fp, -dta0) + (yp + to64(cin[idx]));
-            else yn = imul(fp, -dta0) + cin[idx];
+            const double2 yn = imul(fp, -dta0) + cc;
             y[idx] = yn;
             r[i] = cvt<HC>(yn);
+            if constexpr (SP) y32[idx] = to32(yn);
             nrm += cnorm(yn);
         }
         dxx_inplace<LX>(r, kscale);
@@ -289,7 +302,7 @@ struct HParams {
 
 template <int LY, int LZ, bool SP, int NT = nt_hyz(LY * LZ)>
 __global__ void __launch_bounds__(NT)
-f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
+f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
       const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
       const hc_t<SP> *__restrict__ twy_g, const hc_t<SP> *__restrict__ twz_g, double kys, double kzs, HParams hp,
       double *__restrict__ part)
@@ -313,8 +326,8 @@ f_hyz(const double2 *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *_
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * NT;
         if (e < E) {
-            Xu[k] = cvt<C>(y[gu + e]);
-            Xv[k] = cvt<C>(y[gv + e]);
+            Xu[k] = y[gu + e];
+            Xv[k] = y[gv + e];
             Ou[k] = T[gu + e];
             Ov[k] = T[gv + e];
         }
@@ -441,8 +454,8 @@ template <bool SP> struct FusedFns {
     typedef void (*pred_fn)(double2 *, HC *, const HC *, const HC *, const HC *, const HC *, HC *, HC *,
                             const double *, const double *, int, int, long, int, double, double, int, double *);
     typedef void (*corr_fn)(double2 *, const HC *, const HC *, HC *, const double *, const double *, int, long, int,
-                            double, double, int, double *);
-    typedef void (*hyz_fn)(const double2 *, const HC *, HC *, const double *, const double *, int, long, int,
+                            double, double, int, double *, float2 *);
+    typedef void (*hyz_fn)(const HC *, const HC *, HC *, const double *, const double *, int, long, int,
                            const HC *, const HC *, double, double, HParams, double *);
 
     template <int LX> static pred_fn pick_pred(int sch)
@@ -525,7 +538,7 @@ static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave
 }
 
 template <bool SP>
-static int hyz(tdw_ctx *c, int n, const double2 *y, const hc_t<SP> *T, hc_t<SP> *out, const double *pots,
+static int hyz(tdw_ctx *c, int n, const hc_t<SP> *y, const hc_t<SP> *T, hc_t<SP> *out, const double *pots,
                const double *scale, double *e_out, double *e_out2)
 {
     HParams hp{c->p.mu_a, c->p.mu_b, c->p.functional == TDW_BDG};
@@ -582,27 +595,39 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     }
     c->pending = 0;
     TDW_TRY(xpass_norm(c, n, c->d_sp, c->d_s2, wave));
-    TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt, s->dens, 0));
+    // the prediction: y itself (plain) or its FP32 copy in the consumed
+    // history slot (split)
+    const HC *yp = SP ? (const HC *)f[h - 1] : (const HC *)y;
+    if constexpr (SP)
+        TDW_TRY(densities32(c, n, (const float2 *)yp, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt, s->dens));
+    else
+        TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
+                          s->dens, 0));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
-    TDW_TRY(hyz<SP>(c, n, y, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
+    TDW_TRY(hyz<SP>(c, n, yp, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
 
     // ---- corrector ----
     const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
     {
         TDW_TIMED(c, "f_corr");
         hipLaunchKernelGGL(fc, dim3(nblk), dim3(FT), 0, c->stream, y, H, f[h - 1], T, c->d_sp, c->d_ep, n, N, S,
-                           c->p.dt * a0, kxs, wave, c->d_fpart);
+                           c->p.dt * a0, kxs, wave, c->d_fpart, SP ? (float2 *)H : nullptr);
         TDW_HIP(hipGetLastError());
     }
     TDW_TRY(xpass_norm(c, n, c->d_scale, c->d_s2, wave));
     if (!grad_last) {
-        TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
-                          s->dens, 0));
+        // the corrected state: y (plain) or its FP32 copy in H's buffer (split)
+        const HC *yc = SP ? (const HC *)H : (const HC *)y;
+        if constexpr (SP)
+            TDW_TRY(densities32(c, n, (const float2 *)yc, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt, s->dens));
+        else
+            TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
+                              s->dens, 0));
         TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
         tdw_complex *tmp = s->f[h - 1];
         for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
         s->f[0] = tmp;
-        TDW_TRY(hyz<SP>(c, n, y, T, (HC *)s->f[0], s->pots, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
+        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], s->pots, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
         c->pending = 1;
         return 0;
     }

@@ -160,9 +160,9 @@ int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm)
 // densities: partial sums over state chunks, then a fixed-order reduction.
 // NV = 4 (rho_a, rho_b, nu) or 12 (+ tau_a, tau_b, j_a, j_b).
 // ---------------------------------------------------------------------------
-template <int DIM, bool GRAD>
+template <int DIM, bool GRAD, typename W = double2>
 __global__ void __launch_bounds__(NT)
-k_dens_part(const double2 *__restrict__ wf, const double2 *__restrict__ ddx, const double2 *__restrict__ ddy,
+k_dens_part(const W *__restrict__ wf, const double2 *__restrict__ ddx, const double2 *__restrict__ ddy,
             const double2 *__restrict__ ddz, const double2 *__restrict__ lap, const double *__restrict__ fbeta,
             const double *__restrict__ wgt, const double *__restrict__ kt, const int *__restrict__ cnt,
             int n, long N, int chunk, int cubic, double piz, double *__restrict__ part)
@@ -187,7 +187,7 @@ k_dens_part(const double2 *__restrict__ wf, const double2 *__restrict__ ddx, con
             w = (fabs(kz) < 1.0e-12) ? 1.0 : 2.0;
             if (cubic && fabs(kz + piz) < 1.0e-12) w = 1.0;
         }
-        const double2 u = wf[(long)a * N + i], v = wf[((long)n + a) * N + i];
+        const double2 u = to64(wf[(long)a * N + i]), v = to64(wf[((long)n + a) * N + i]);
         const double fw = fb * w, fmw = fbm * w;
         na += cnorm(u) * fw;
         nb += cnorm(v) * fmw;
@@ -256,9 +256,10 @@ k_dens_final(const double *__restrict__ part, int nchunk, long N, double tr, dou
     dens[11 * N + i] = GRAD ? s[11] / sc / tr : 0.0;
 }
 
-int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const double2 *ddy,
-              const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
-              const double *kt, const int *cnt, double *dens, int grad)
+template <typename W>
+static int densities_t(tdw_ctx *c, int n, const W *wf, const double2 *ddx, const double2 *ddy,
+                       const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
+                       const double *kt, const int *cnt, double *dens, int grad)
 {
     TDW_TIMED(c, "densities");
     const long N = (long)c->N;
@@ -272,8 +273,8 @@ int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const do
     double piz = M_PI / c->p.dz;
     int cubic = c->p.regularization == TDW_CUBIC_CUTOFF;
     dim3 g(nblk, nchunk);
-#define TDW_DP(D, G)                                                                                   \
-    hipLaunchKernelGGL((k_dens_part<D, G>), g, dim3(NT), 0, c->stream, wf, ddx, ddy, ddz, lap, fbeta, wgt, \
+#define TDW_DP(D, G)                                                                                      \
+    hipLaunchKernelGGL((k_dens_part<D, G, W>), g, dim3(NT), 0, c->stream, wf, ddx, ddy, ddz, lap, fbeta, wgt, \
                        kt, cnt, n, N, chunk, cubic, piz, c->d_part)
     if (c->dim == 1) { if (grad) TDW_DP(1, true); else TDW_DP(1, false); }
     else if (c->dim == 2) { if (grad) TDW_DP(2, true); else TDW_DP(2, false); }
@@ -283,6 +284,19 @@ int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const do
     if (grad) hipLaunchKernelGGL(k_dens_final<true>, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, tr, dens);
     else hipLaunchKernelGGL(k_dens_final<false>, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, tr, dens);
     return (int)hipGetLastError();
+}
+
+int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const double2 *ddy,
+              const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
+              const double *kt, const int *cnt, double *dens, int grad)
+{
+    return densities_t<double2>(c, n, wf, ddx, ddy, ddz, lap, fbeta, wgt, kt, cnt, dens, grad);
+}
+
+int densities32(tdw_ctx *c, int n, const float2 *wf, const double *fbeta, const double *wgt, const double *kt,
+                const int *cnt, double *dens)
+{
+    return densities_t<float2>(c, n, wf, nullptr, nullptr, nullptr, nullptr, fbeta, wgt, kt, cnt, dens, 0);
 }
 
 // ---------------------------------------------------------------------------

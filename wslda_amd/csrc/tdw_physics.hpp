@@ -89,6 +89,9 @@ int laplace_real(tdw_ctx *c, const double *f, double *lap);
 int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const double2 *ddy,
               const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
               const double *kt, const int *cnt, double *dens, int grad);
+// rho, nu (no gradients) from FP32 wavefunctions, sums in FP64
+int densities32(tdw_ctx *c, int n, const float2 *wf, const double *fbeta, const double *wgt, const double *kt,
+                const int *cnt, double *dens);
 int density_tau(tdw_ctx *c, double *dens);
 int vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 int potentials(tdw_ctx *c, int it, const double *dens, double *pots);
