@@ -62,15 +62,15 @@ def algorithmic_units(name, scheme, split=False):
 def design_units(name, scheme, split=False):
     """Bytes (units of S per state and launch) this implementation's kernels
     must move: reads and writes of state-sized arrays (FP64 S; the split
-    form's history, partial-corrector increment and T / H scratch are FP32,
-    S/2)."""
+    form's history, T / H scratch and the FP32 copies of the prediction and
+    of the corrected state are S/2)."""
     h = nhist(scheme)
     if split:
         return {
-            "f_pred": 1 + 0.5 * h + 1 + 0.5 + 0.5 + 0.5,   # y, f_1..f_h; y_p, c - y_p, T, f_1 fix-up
-            "f_corr": 1 + 0.5 + 0.5 + 1 + 0.5,             # y_p, H y_p, c - y_p; y, T
-            "f_hyz": 1 + 0.5 + 0.5,                         # y, T; H y
-            "densities": 1,
+            "f_pred": 1 + 0.5 * h + 1 + 0.5 + 0.5 + 0.5,   # y, f_1..f_h; c, y_p (FP32), T, f_1 fix-up
+            "f_corr": 1 + 0.5 + 0.5 + 1 + 0.5 + 0.5,       # c, H y_p, y_p (FP32); y, T, y (FP32)
+            "f_hyz": 0.5 + 0.5 + 0.5,                       # y (FP32), T; H y
+            "densities": 0.5,                               # y (FP32)
             "f_fixup": 2,
         }.get(name, 0)
     return {

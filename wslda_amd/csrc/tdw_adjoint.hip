@@ -344,6 +344,150 @@ k_hist_back(double2 *ly, const double2 *__restrict__ w1, const double2 *__restri
     ly[idx] = lc + lyp;
 }
 
+// ---------------------------------------------------------------------------
+// The density transpose fused into the pass that consumes it, for states
+// small enough that one workgroup holds a whole state (N <= RT * KP): phase
+// 1 forms g = out + dens^T(lD)(psi) in registers and reduces t = dV Re<psi, g>
+// over the state; phase 2 uses g and t.  Same arithmetic as k_dens_t
+// followed by k_recon / k_hist_back.
+// ---------------------------------------------------------------------------
+static constexpr int RT = 1024;
+
+// per-state density weights of k_dens_part (occupation, transverse
+// degeneracy, 1/transverse length): rho_a, rho_b and nu factors
+__device__ __forceinline__ void dens_wts(int dim, int a, const double *__restrict__ fbeta,
+                                         const double *__restrict__ kt, const int *__restrict__ cnt, double tr,
+                                         int cubic, double piz, double &wa, double &wb, double &cn)
+{
+    double w = 1.0;
+    if (dim == 1) w = (double)cnt[a];
+    else if (dim == 2) {
+        const double kz = kt[a];
+        w = (fabs(kz) < 1.0e-12) ? 1.0 : 2.0;
+        if (cubic && fabs(kz + piz) < 1.0e-12) w = 1.0;
+    }
+    const double fb = fbeta[a];
+    wa = fb * w / tr;
+    wb = (1.0 - fb) * w / tr;
+    cn = (1.0 - 2.0 * fb) / 2.0 * w / tr;
+}
+
+// phase 1 for the points of one thread: gu/gv = out + dens^T(lD) at (u, v)
+template <int KP>
+__device__ __forceinline__ double dens_t_regs(const double2 *__restrict__ psi, const double2 *out,
+                                              const double *__restrict__ lD, long ou, long ov, long N, double wa,
+                                              double wb, double cn, double2 (&gu)[KP], double2 (&gv)[KP])
+{
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const long i = threadIdx.x + (long)k * RT;
+        if (i < N) {
+            const double2 u = psi[ou + i], v = psi[ov + i];
+            const double2 ln = make_double2(lD[2 * N + i], lD[3 * N + i]);
+            gu[k] = out[ou + i] + u * (2.0 * wa * lD[i]) + cmul(ln, v) * cn;
+            gv[k] = out[ov + i] + v * (2.0 * wb * lD[N + i]) + cmul(make_double2(ln.x, -ln.y), u) * cn;
+            s += cmulc(u, gu[k]).x + cmulc(v, gv[k]).x;
+        }
+    }
+    return s;
+}
+
+// k_dens_t(y1 -> ly) then k_recon, one workgroup per state; svp[a] is
+// written directly (no partials)
+template <int H, int KP>
+__global__ void __launch_bounds__(RT)
+k_recon_dt(double2 *ly, const double2 *__restrict__ y1, const double2 *__restrict__ y0,
+           const double2 *__restrict__ f0, const double2 *__restrict__ f1, const double2 *__restrict__ f2,
+           const double2 *__restrict__ f3, const double2 *__restrict__ f4, double2 *__restrict__ w0,
+           double2 *__restrict__ w3, double2 *__restrict__ w4, const double *__restrict__ lD,
+           const double *__restrict__ fbeta, const double *__restrict__ kt, const int *__restrict__ cnt, int dim,
+           double tr, int cubic, double piz, const double *__restrict__ scal, AbmCoef cf, double dt, int n, long N,
+           double dv, double *__restrict__ t1out, double *__restrict__ svp)
+{
+    __shared__ double red[RT / 64];
+    const int a = blockIdx.x;
+    const long ou = (long)a * N, ov = ((long)n + a) * N;
+    double wa, wb, cn;
+    dens_wts(dim, a, fbeta, kt, cnt, tr, cubic, piz, wa, wb, cn);
+    double2 gu[KP], gv[KP];
+    const double tt = dv * block_sum(dens_t_regs<KP>(y1, ly, lD, ou, ov, N, wa, wb, cn, gu, gv), red);
+    if (threadIdx.x == 0) t1out[a] = tt;
+    const double np = sqrt(scal[a]), ep = scal[n + a], nc = sqrt(scal[2 * n + a]);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const long i = threadIdx.x + (long)k * RT;
+        if (i >= N) continue;
+#pragma unroll
+        for (int comp = 0; comp < 2; comp++) {
+            const long idx = (comp ? ov : ou) + i;
+            const double2 Y1 = y1[idx];
+            const double2 lyc = ((comp ? gv[k] : gu[k]) - Y1 * tt) * (1.0 / nc);
+            ly[idx] = lyc;
+            const double2 mu = imul(lyc, dt * cf.a0);
+            w0[idx] = mu;
+            double2 F[H];
+            F[0] = f0[idx];
+            F[1] = f1[idx];
+            F[2] = f2[idx];
+            if constexpr (H >= 4) F[3] = f3[idx];
+            if constexpr (H >= 5) F[4] = f4[idx];
+            double2 sb = make_double2(0, 0), sa = make_double2(0, 0);
+#pragma unroll
+            for (int j = 0; j < H; j++) {
+                sb = sb + F[j] * cf.b[j];
+                sa = sa + F[j] * cf.a[j];
+            }
+            const double2 Y0 = y0[idx];
+            const double2 yh = (Y0 + imul(sb, -dt)) * (1.0 / np);
+            const double2 cc = Y0 + imul(sa, -dt);
+            const double2 G = imul(Y1 * nc - cc, 1.0 / (dt * cf.a0)) + yh * ep;
+            w3[idx] = yh;
+            w4[idx] = G;
+            s += cmulc(mu, yh).x;
+        }
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) svp[a] = s * dv;
+}
+
+// k_dens_t(w3 -> w1) then k_hist_back, one workgroup per state
+template <int H, int KP>
+__global__ void __launch_bounds__(RT)
+k_hist_dt(double2 *ly, const double2 *__restrict__ w1, const double2 *__restrict__ w3, double2 *lf0, double2 *lf1,
+          double2 *lf2, double2 *lf3, double2 *lf4, const double *__restrict__ lD, const double *__restrict__ fbeta,
+          const double *__restrict__ kt, const int *__restrict__ cnt, int dim, double tr, int cubic, double piz,
+          const double *__restrict__ scal, AbmCoef cf, double dt, int n, long N, double dv)
+{
+    __shared__ double red[RT / 64];
+    const int a = blockIdx.x;
+    const long ou = (long)a * N, ov = ((long)n + a) * N;
+    double wa, wb, cn;
+    dens_wts(dim, a, fbeta, kt, cnt, tr, cubic, piz, wa, wb, cn);
+    double2 gu[KP], gv[KP];
+    const double t2 = dv * block_sum(dens_t_regs<KP>(w3, w1, lD, ou, ov, N, wa, wb, cn, gu, gv), red);
+    const double inp = 1.0 / sqrt(scal[a]);
+    double2 *lf[5] = {lf0, lf1, lf2, lf3, lf4};
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+        const long i = threadIdx.x + (long)k * RT;
+        if (i >= N) continue;
+#pragma unroll
+        for (int comp = 0; comp < 2; comp++) {
+            const long idx = (comp ? ov : ou) + i;
+            const double2 lyp = ((comp ? gv[k] : gu[k]) - w3[idx] * t2) * inp;
+            const double2 lc = ly[idx];
+#pragma unroll
+            for (int j = 0; j < H; j++) {
+                const double2 base = (j == H - 1) ? make_double2(0.0, 0.0) : lf[j][idx];
+                lf[j][idx] = base + imul(lyp * cf.b[j] + lc * cf.a[j], dt);
+            }
+            ly[idx] = lc + lyp;
+        }
+    }
+}
+
 __global__ void k_copy_d(const double *__restrict__ s, double *__restrict__ d, long n)
 {
     const long i = (long)blockIdx.x * AT + threadIdx.x;
@@ -433,6 +577,40 @@ static int copy_d(tdw_ctx *c, const double *s, double *d, long n)
 {
     hipLaunchKernelGGL(k_copy_d, dim3((n + AT - 1) / AT), dim3(AT), 0, c->stream, s, d, n);
     return (int)hipGetLastError();
+}
+
+// points per thread of the fused one-workgroup-per-state kernels; 0: the
+// state does not fit one workgroup (the separate passes run)
+static int fused_kp(long N)
+{
+    for (int kp : {1, 2, 4, 8})
+        if (N <= (long)RT * kp) return kp;
+    return 0;
+}
+
+typedef void (*recon_dt_t)(double2 *, const double2 *, const double2 *, const double2 *, const double2 *,
+                           const double2 *, const double2 *, const double2 *, double2 *, double2 *, double2 *,
+                           const double *, const double *, const double *, const int *, int, double, int, double,
+                           const double *, AbmCoef, double, int, long, double, double *, double *);
+typedef void (*hist_dt_t)(double2 *, const double2 *, const double2 *, double2 *, double2 *, double2 *, double2 *,
+                          double2 *, const double *, const double *, const double *, const int *, int, double, int,
+                          double, const double *, AbmCoef, double, int, long, double);
+
+template <int H> static recon_dt_t recon_dt_kp(int kp)
+{
+    return kp == 1 ? k_recon_dt<H, 1> : kp == 2 ? k_recon_dt<H, 2> : kp == 4 ? k_recon_dt<H, 4> : k_recon_dt<H, 8>;
+}
+static recon_dt_t recon_dt_fn(int h, int kp)
+{
+    return h == 3 ? recon_dt_kp<3>(kp) : h == 4 ? recon_dt_kp<4>(kp) : recon_dt_kp<5>(kp);
+}
+template <int H> static hist_dt_t hist_dt_kp(int kp)
+{
+    return kp == 1 ? k_hist_dt<H, 1> : kp == 2 ? k_hist_dt<H, 2> : kp == 4 ? k_hist_dt<H, 4> : k_hist_dt<H, 8>;
+}
+static hist_dt_t hist_dt_fn(int h, int kp)
+{
+    return h == 3 ? hist_dt_kp<3>(kp) : h == 4 ? hist_dt_kp<4>(kp) : hist_dt_kp<5>(kp);
 }
 
 static int adj_supported(const tdw_ctx *c)
@@ -567,10 +745,22 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
     TDW_TRY(cross_upd(c, n, LY, LY, W1, CD2(fnew), CD2(y1), MU, CD2(y1), sv1, e_n, e_n, A->lpots, lUc));
     // U' = Phi(D', U_p, V_ext)
     TDW_TRY(pot_vjp(c, it + 1, t->dens_c, t->pots_p, lUc, dens_grad, lD, lUo, A->theta_grad));
-    // D' = dens(y'), then y' = y_c / n_c
-    TDW_TRY(dens_t(c, n, CD2(y1), LY, lD, t1));
-    // rebuild the predictor side; LY becomes dL/dy_c = dL/dc, W0 = dL/dF_p
-    {
+    // D' = dens(y'), then y' = y_c / n_c; rebuild the predictor side: LY
+    // becomes dL/dy_c = dL/dc, W0 = dL/dF_p
+    const int kp = fused_kp(N);
+    const double tr = dens_transverse(c), piz = M_PI / c->p.dz;
+    const int cubic = c->p.regularization == TDW_CUBIC_CUTOFF;
+    if (kp) {
+        TDW_TIMED(c, "adj_recon");
+        const double2 *F[5] = {CD2(f[0]), CD2(f[1]), CD2(f[2]), h >= 4 ? CD2(f[3]) : nullptr,
+                               h >= 5 ? CD2(f[4]) : nullptr};
+        auto kr = recon_dt_fn(h, kp);
+        hipLaunchKernelGGL(kr, dim3(n), dim3(RT), 0, c->stream, LY, CD2(y1), CD2(y0), F[0], F[1], F[2], F[3], F[4],
+                           W0, W3, W4, lD, c->d_fbeta, c->d_kt, c->d_cnt, c->dim, tr, cubic, piz, t->scal, cf,
+                           c->p.dt, n, N, dv, t1, svp);
+        TDW_HIP(hipGetLastError());
+    } else {
+        TDW_TRY(dens_t(c, n, CD2(y1), LY, lD, t1));
         int nchunk;
         const long chunk = dot_chunk(c, &nchunk);
         TDW_TRY(ensure_part(c, (size_t)n * nchunk));
@@ -592,12 +782,19 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
     TDW_TRY(cross_upd(c, n, W1, nullptr, W1, W4, nullptr, W0, W3, svp, nullptr, e_p, lUo, lUc));
     // U_p = Phi(D_p, U_prev, V_ext)
     TDW_TRY(pot_vjp(c, it + 1, t->dens_p, pots_prev, lUc, nullptr, lD, A->lpots, A->theta_grad));
-    TDW_TRY(dens_t(c, n, W3, W1, lD, t2));
     // yh_p = y_p / n_p, then the ABM predictor / partial corrector
+    if (!kp) TDW_TRY(dens_t(c, n, W3, W1, lD, t2));
     tdw_complex *mu_store = A->lf[0];
     for (int j = 0; j < h - 1; j++) A->lf[j] = A->lf[j + 1];
     A->lf[h - 1] = mu_store;
-    {
+    if (kp) {
+        TDW_TIMED(c, "adj_hist");
+        auto kh = hist_dt_fn(h, kp);
+        hipLaunchKernelGGL(kh, dim3(n), dim3(RT), 0, c->stream, LY, W1, W3, D2(A->lf[0]), D2(A->lf[1]), D2(A->lf[2]),
+                           h >= 4 ? D2(A->lf[3]) : nullptr, h >= 5 ? D2(A->lf[4]) : nullptr, lD, c->d_fbeta,
+                           c->d_kt, c->d_cnt, c->dim, tr, cubic, piz, t->scal, cf, c->p.dt, n, N, dv);
+        TDW_HIP(hipGetLastError());
+    } else {
         TDW_TIMED(c, "adj_hist");
         const long tot = 2L * n * N;
         hipLaunchKernelGGL(k_hist_back, dim3((tot + AT - 1) / AT), dim3(AT), 0, c->stream, LY, W1, W3, D2(A->lf[0]),
