@@ -88,6 +88,36 @@ def step_units(scheme, split=False):
     return nhist(scheme) + 8     # B_qp: 11, 12, 13 S for AB3AM4, AB4AM5, AB5AM5
 
 
+def copy_calibration(args):
+    """One device copy of args.copy_calib_mib MiB at the end of a run: the
+    reference dispatch for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    if not args.copy_calib_mib:
+        return 0
+    import torch
+    nel = args.copy_calib_mib * 1024 * 1024 // 8
+    a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    return nel * 8
+
+
+def reserve_release(gb):
+    """Allocates, touches and frees gb of device memory before the run's own
+    allocations.  On a fresh box the first large allocations of a process
+    have landed where the fused predictor runs ~15% slower (14.2 vs 12.2 ms
+    per launch at 24^3 x 16384, constant over the run); allocations made after
+    one reserve-and-release cycle have not.  No timed work is changed."""
+    import torch
+    x = torch.empty(int(gb * 1e9), dtype=torch.uint8, device="cuda:0")
+    x.fill_(0)
+    torch.cuda.synchronize()
+    del x
+    torch.cuda.empty_cache()
+
+
 def run_gpu(args, prec):
     import numpy as np
     import torch
@@ -280,7 +310,7 @@ def run_adjoint(args):
     grad = adjoint.beams_grad(theta, gslot)
     fd = {}
     h = 1e-6
-    for idx, name in ((0, "A_1"), (3, "width_1"), (5, "x_2")):
+    for idx, name in ((0, "A_1"), (3, "width_1"), (5, "x_2")) if args.adj_fd else ():
         vals = []
         for sg in (1, -1):
             th = theta.copy()
@@ -290,9 +320,10 @@ def run_adjoint(args):
             vals.append(prob.observable())
         fd[name] = {"adjoint": float(grad[idx]), "fd": (vals[0] - vals[1]) / (2 * h)}
     gmax = float(np.abs(grad).max())
-    err = max(abs(v["adjoint"] - v["fd"]) for v in fd.values()) / gmax
+    err = max(abs(v["adjoint"] - v["fd"]) for v in fd.values()) / gmax if fd else None
     n = prob.n
     prob.close()
+    copy_calibration(args)
     return {
         "metric": "adjoint gradient cost / forward cost, and gradient error vs central finite differences",
         "value": (t_fwd_ck + t_bwd) / t_fwd, "unit": "x forward", "higher_is_better": False,
@@ -384,6 +415,9 @@ def main():
                     help="both: the FP64 run (value) and the split-precision run on the same initial state, "
                          "with the split run's density errors against FP64")
     ap.add_argument("--fused", type=int, default=1)
+    ap.add_argument("--reserve-gb", type=float, default=120.0,
+                    help="reserve, touch and release this much device memory before the run's arrays are "
+                         "allocated (0: off)")
     ap.add_argument("--meas-every", type=int, default=100,
                     help="steps between measurements (the corrector of the last step of each computes gradients)")
     ap.add_argument("--cpu-states", type=int, default=256)
@@ -401,6 +435,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--adj-k", type=int, default=50)
     ap.add_argument("--adj-states", type=int, default=4096)
+    ap.add_argument("--adj-fd", type=int, default=1, help="central-difference check of three parameters (0: off)")
     ap.add_argument("--adj-taped", type=int, default=3,
                     help="segments the forward run tapes directly (each 2K+1 state arrays of HBM)")
     ap.add_argument("--trap", action="store_true",
@@ -419,6 +454,8 @@ def main():
         return
     from wslda_amd import td
     primary = "split" if args.mode == "split" else "fp64"
+    if args.reserve_gb > 0:
+        reserve_release(args.reserve_gb)
     g = run_gpu(args, td.SPLIT if primary == "split" else td.FP64)
     n = g["n"]
     head = summarize(g, args, primary)
@@ -463,16 +500,9 @@ def main():
     if not args.no_cpu and primary == "fp64":
         res["cpu_baseline"] = cpu_baseline(g, args)
     g["ctx"].close()
-    if args.copy_calib_mib:
-        import torch
-        nel = args.copy_calib_mib * 1024 * 1024 // 8
-        a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        res["copy_calib_bytes"] = nel * 8
-        del a, b
+    nb = copy_calibration(args)
+    if nb:
+        res["copy_calib_bytes"] = nb
     print(json.dumps(res))
 
 

@@ -89,6 +89,9 @@ __global__ void k_rows(const double *__restrict__ part, int n, int nchunk, doubl
 //   dL/dIm D = sum (-Im(nu_u* psi_v) + Im(nu_v* psi_u))
 // summed over a chunk of states per point (point-major: coalesced rows).
 // ---------------------------------------------------------------------------
+// States are taken CU at a time, all loads of the group issued before its
+// stores, so that a wave keeps several states' reads in flight.
+template <int CU>
 __global__ void __launch_bounds__(AT)
 k_cross_upd(double2 *out, const double2 *acc, const double2 *hterm, const double2 *__restrict__ g1,
             const double2 *__restrict__ g2, const double2 *__restrict__ mu, const double2 *__restrict__ psi,
@@ -99,32 +102,52 @@ k_cross_upd(double2 *out, const double2 *acc, const double2 *hterm, const double
     if (i >= N) return;
     const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
     double gVa = 0.0, gVb = 0.0, gDr = 0.0, gDi = 0.0;
-    for (int a = a0; a < a1; a++) {
-        const long iu = (long)a * N + i, iv = ((long)n + a) * N + i;
-        const double s = sv[a];
-        const double e = ev ? ev[a] : 0.0;
-        double2 gu = g1[iu], gv = g1[iv];
-        if (g2) {
-            gu = gu + g2[iu] * e;
-            gv = gv + g2[iv] * e;
+    for (int a = a0; a < a1; a += CU) {
+        double2 Gu[CU], Gv[CU], Mu[CU], Mv[CU], Hu[CU], Hv[CU], Au[CU], Av[CU], Pu[CU], Pv[CU];
+#pragma unroll
+        for (int q = 0; q < CU; q++) {
+            const int b = a + q;
+            if (b >= a1) break;
+            const long iu = (long)b * N + i, iv = ((long)n + b) * N + i;
+            Pu[q] = psi[iu];
+            Pv[q] = psi[iv];
+            Gu[q] = g1[iu];
+            Gv[q] = g1[iv];
+            if (g2) {
+                // g2 is psi in the history-refresh call: reuse the loaded values
+                const double e = ev ? ev[b] : 0.0;
+                Gu[q] = Gu[q] + (g2 == psi ? Pu[q] : g2[iu]) * e;
+                Gv[q] = Gv[q] + (g2 == psi ? Pv[q] : g2[iv]) * e;
+            }
+            Mu[q] = mu[iu];
+            Mv[q] = mu[iv];
+            Hu[q] = hterm ? hterm[iu] : make_double2(0.0, 0.0);
+            Hv[q] = hterm ? hterm[iv] : make_double2(0.0, 0.0);
+            Au[q] = acc ? acc[iu] : make_double2(0.0, 0.0);
+            Av[q] = acc ? acc[iv] : make_double2(0.0, 0.0);
         }
-        const double2 mu_u = mu[iu], mu_v = mu[iv];
-        double2 ou = gu * (-2.0 * s), ov = gv * (-2.0 * s);
-        if (hterm) {
-            const double m = em ? em[a] : 0.0;
-            ou = hterm[iu] - mu_u * m + ou;
-            ov = hterm[iv] - mu_v * m + ov;
+#pragma unroll
+        for (int q = 0; q < CU; q++) {
+            const int b = a + q;
+            if (b >= a1) break;
+            const long iu = (long)b * N + i, iv = ((long)n + b) * N + i;
+            const double s = sv[b];
+            double2 ou = Gu[q] * (-2.0 * s), ov = Gv[q] * (-2.0 * s);
+            if (hterm) {
+                const double m = em ? em[b] : 0.0;
+                ou = Hu[q] - Mu[q] * m + ou;
+                ov = Hv[q] - Mv[q] * m + ov;
+            }
+            if (acc) { ou = Au[q] + ou; ov = Av[q] + ov; }
+            const double2 nu_u = Mu[q] - Pu[q] * s, nu_v = Mv[q] - Pv[q] * s;
+            out[iu] = ou;
+            out[iv] = ov;
+            gVa += cmulc(nu_u, Pu[q]).x;
+            gVb -= cmulc(nu_v, Pv[q]).x;
+            const double2 uv = cmulc(nu_u, Pv[q]), vu = cmulc(nu_v, Pu[q]);
+            gDr += uv.x + vu.x;
+            gDi += -uv.y + vu.y;
         }
-        if (acc) { ou = acc[iu] + ou; ov = acc[iv] + ov; }
-        const double2 pu = psi[iu], pv = psi[iv];
-        const double2 nu_u = mu_u - pu * s, nu_v = mu_v - pv * s;
-        out[iu] = ou;
-        out[iv] = ov;
-        gVa += cmulc(nu_u, pu).x;
-        gVb -= cmulc(nu_v, pv).x;
-        const double2 uv = cmulc(nu_u, pv), vu = cmulc(nu_v, pu);
-        gDr += uv.x + vu.x;
-        gDi += -uv.y + vu.y;
     }
     double *p = part + (long)blockIdx.y * 4 * N + i;
     p[0] = gVa;
@@ -527,7 +550,7 @@ static int cross_upd(tdw_ctx *c, int n, double2 *out, const double2 *acc, const 
     while (chunk > 8 && nblk * ((n + chunk - 1) / chunk) < 2048) chunk >>= 1;
     const int nchunk = (n + chunk - 1) / chunk;
     TDW_TRY(ensure_part(c, (size_t)nchunk * 4 * N));
-    hipLaunchKernelGGL(k_cross_upd, dim3(nblk, nchunk), dim3(AT), 0, c->stream, out, acc, hterm, g1, g2, mu, psi, sv,
+    hipLaunchKernelGGL(k_cross_upd<2>, dim3(nblk, nchunk), dim3(AT), 0, c->stream, out, acc, hterm, g1, g2, mu, psi, sv,
                        ev, em, n, N, chunk, c->d_part);
     hipLaunchKernelGGL(k_cross_final, dim3((4 * N + 63) / 64), dim3(64), 0, c->stream, c->d_part, nchunk, N, add, lU);
     return (int)hipGetLastError();
