@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libtdw_oracle.so")
 
-SLDA, BDG = 111, 113
+SLDA, ASLDA, BDG = 111, 112, 113
 SPHERICAL, CUBIC = 88, 89
 AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
 EKIN, EPOT, EPAIR, ECURRENT, EPOTEXT, EPAIREXT, EVELEXT, NPARTA, NPARTB, LZA, LZB = range(11)
@@ -88,6 +88,8 @@ def lib():
             "tdwo_vext_eval": (None, [cp, I, P, P]),
             "tdwo_potentials": (None, [cp, I, P, P]),
             "tdwo_apply_h": (None, [cp, I, P, P, P, P, P, P, P]),
+            "tdwo_apply_h_full": (None, [cp, I, P, P, P, P, P, P, P, P, P, P, P]),
+            "tdwo_alpha_wf": (None, [cp, I, P, P, P]),
             "tdwo_energy": (None, [cp, I, P, P, P]),
             "tdwo_step": (I, [cp, C.POINTER(State), I, I]),
             "tdwo_initial_measure": (None, [cp, C.POINTER(State)]),

@@ -26,11 +26,16 @@
 
 typedef double complex zc;
 
-/* Functional constants for SLDA / BdG with the A0=1 effective mass
- * (pca_settings.h:80-110, SLDA_FORCE_A1). */
-#define A0c 1.000
-#define A1c 0.0
-#define A2c 0.0
+/* Functional constants (pca_settings.h:80-110): SLDA and BdG force the
+ * effective mass to A0 = 1 (SLDA_FORCE_A1, FAST_CONST_EFFECTIVE_MASS_MODE);
+ * ASLDA uses A0 = 1.094, A1 = 0.156, A2 = -0.532 with CURRENT_CORRECTIONS
+ * and the density-dependent effective mass. */
+static double A0of(const tdwo_cfg *c) { return c->functional == TDWO_ASLDA ? 1.094 : 1.000; }
+static double A1of(const tdwo_cfg *c) { return c->functional == TDWO_ASLDA ? 0.156 : 0.0; }
+static double A2of(const tdwo_cfg *c) { return c->functional == TDWO_ASLDA ? -0.532 : 0.0; }
+#define A0c A0of(c)
+#define A1c A1of(c)
+#define A2c A2of(c)
 #define G0c 0.357
 #define G1c 0.642
 #define P_ALPHAc 1.0
@@ -566,12 +571,13 @@ static double der_pol_nb(const tdwo_cfg *c, double na, double nb)
     if (dpr != 0.0) r += dpr * (na - nb) / np;
     return r;
 }
-static double alpha_f(double p)
+static double alpha_f(const tdwo_cfg *c, double p)
 {
     double p2 = p * p, p4 = p2 * p2;
     return A0c + A1c * p * (1.0 - (2.0 / 3.0) * p2 + (1.0 / 5.0) * p4) + A2c * p2 * (1.0 - p2 + (1.0 / 3.0) * p4);
 }
-static double alpha_plus(double p) { return 0.5 * (alpha_f(p) + alpha_f(-p)); }
+static double alpha_plus(const tdwo_cfg *c, double p) { return 0.5 * (alpha_f(c, p) + alpha_f(c, -p)); }
+static double alpha_minus(const tdwo_cfg *c, double p) { return 0.5 * (alpha_f(c, p) - alpha_f(c, -p)); }
 static double der_alpha_plus_na(const tdwo_cfg *c, double na, double nb)
 {
     double p = polarization(c, na, nb), p2 = p * p, p4 = p2 * p2;
@@ -582,25 +588,37 @@ static double der_alpha_plus_nb(const tdwo_cfg *c, double na, double nb)
     double p = polarization(c, na, nb), p2 = p * p, p4 = p2 * p2;
     return (2.0 * A2c * p * (1.0 - p2 + (1.0 / 3.0) * p4) - 2.0 * A2c * p * p2 * (1.0 - (2.0 / 3.0) * p2)) * der_pol_nb(c, na, nb);
 }
+static double der_alpha_minus_na(const tdwo_cfg *c, double na, double nb)
+{
+    double p = polarization(c, na, nb), p2 = p * p, p4 = p2 * p2;
+    return (A1c * (1.0 - (2.0 / 3.0) * p2 + (1.0 / 5.0) * p4) - 4.0 * A1c * p2 * ((1.0 / 3.0) - (1.0 / 5.0) * p2)) *
+           der_pol_na(c, na, nb);
+}
+static double der_alpha_minus_nb(const tdwo_cfg *c, double na, double nb)
+{
+    double p = polarization(c, na, nb), p2 = p * p, p4 = p2 * p2;
+    return (A1c * (1.0 - (2.0 / 3.0) * p2 + (1.0 / 5.0) * p4) - 4.0 * A1c * p2 * ((1.0 / 3.0) - (1.0 / 5.0) * p2)) *
+           der_pol_nb(c, na, nb);
+}
 static double funG(double p) { return G0c + G1c * p * p; }
 static double funD(const tdwo_cfg *c, double na, double nb)
 {
     double p = polarization(c, na, nb);
     return (1.0 / (20.0 * M_PI * M_PI)) * pow(6.0 * M_PI * M_PI * (na + nb), 5.0 / 3.0) *
-           (funG(p) - alpha_f(p) * pow((1.0 + p) / 2.0, 5.0 / 3.0) - alpha_f(-1.0 * p) * pow((1.0 - p) / 2.0, 5.0 / 3.0));
+           (funG(p) - alpha_f(c, p) * pow((1.0 + p) / 2.0, 5.0 / 3.0) - alpha_f(c, -1.0 * p) * pow((1.0 - p) / 2.0, 5.0 / 3.0));
 }
 static double der_funD_common(const tdwo_cfg *c, double na, double nb, double dpol)
 {
     double p = polarization(c, na, nb), p2 = p * p, p4 = p2 * p2;
-    double base = funG(p) - alpha_f(p) * pow((1.0 + p) / 2.0, 5.0 / 3.0) - alpha_f(-1.0 * p) * pow((1.0 - p) / 2.0, 5.0 / 3.0);
+    double base = funG(p) - alpha_f(c, p) * pow((1.0 + p) / 2.0, 5.0 / 3.0) - alpha_f(c, -1.0 * p) * pow((1.0 - p) / 2.0, 5.0 / 3.0);
     double da_p = A1c * (1.0 - (2.0 / 3.0) * p2 + (1.0 / 5.0) * p4) - 4.0 * A1c * p2 * ((1.0 / 3.0) - (1.0 / 5.0) * p2) +
                   2.0 * A2c * p * (1.0 - p2 + (1.0 / 3.0) * p4) - 2.0 * A2c * p * p2 * (1.0 - (2.0 / 3.0) * p2);
     double da_m = -1.0 * A1c * (1.0 - (2.0 / 3.0) * p2 + (1.0 / 5.0) * p4) + 4.0 * A1c * p2 * ((1.0 / 3.0) - (1.0 / 5.0) * p2) +
                   2.0 * A2c * p * (1.0 - p2 + (1.0 / 3.0) * p4) - 2.0 * A2c * p * p2 * (1.0 - (2.0 / 3.0) * p2);
     return (1.0 / 2.0) * pow(6.0 * M_PI * M_PI * (na + nb), 2.0 / 3.0) * base +
            (1.0 / (20.0 * M_PI * M_PI)) * pow(6.0 * M_PI * M_PI * (na + nb), 5.0 / 3.0) *
-               (2.0 * G1c * p - da_p * pow((1.0 + p) / 2.0, 5.0 / 3.0) - (5.0 / 6.0) * alpha_f(p) * pow((1.0 + p) / 2.0, 2.0 / 3.0) -
-                da_m * pow((1.0 - p) / 2.0, 5.0 / 3.0) + (5.0 / 6.0) * alpha_f(-1.0 * p) * pow((1.0 - p) / 2.0, 2.0 / 3.0)) *
+               (2.0 * G1c * p - da_p * pow((1.0 + p) / 2.0, 5.0 / 3.0) - (5.0 / 6.0) * alpha_f(c, p) * pow((1.0 + p) / 2.0, 2.0 / 3.0) -
+                da_m * pow((1.0 - p) / 2.0, 5.0 / 3.0) + (5.0 / 6.0) * alpha_f(c, -1.0 * p) * pow((1.0 - p) / 2.0, 2.0 / 3.0)) *
                dpol;
 }
 static double der_funD_na(const tdwo_cfg *c, double na, double nb) { return der_funD_common(c, na, nb, der_pol_na(c, na, nb)); }
@@ -608,19 +626,19 @@ static double der_funD_nb(const tdwo_cfg *c, double na, double nb) { return der_
 static double tildeC(const tdwo_cfg *c, double na, double nb)
 {
     double p = polarization(c, na, nb);
-    return alpha_plus(p) * pow(na + nb + c->denseps, 1.0 / 3.0) / (GAMMA0c * 1.0);
+    return alpha_plus(c, p) * pow(na + nb + c->denseps, 1.0 / 3.0) / (GAMMA0c * 1.0);
 }
 static double der_tildeC_na(const tdwo_cfg *c, double na, double nb)
 {
     double p = polarization(c, na, nb);
     return (der_alpha_plus_na(c, na, nb) * pow(na + nb + c->denseps, 1.0 / 3.0) +
-            (1.0 / 3.0) * alpha_plus(p) * pow(na + nb + c->denseps, -2.0 / 3.0)) / (GAMMA0c * 1.0);
+            (1.0 / 3.0) * alpha_plus(c, p) * pow(na + nb + c->denseps, -2.0 / 3.0)) / (GAMMA0c * 1.0);
 }
 static double der_tildeC_nb(const tdwo_cfg *c, double na, double nb)
 {
     double p = polarization(c, na, nb);
     return (der_alpha_plus_nb(c, na, nb) * pow(na + nb + c->denseps, 1.0 / 3.0) +
-            (1.0 / 3.0) * alpha_plus(p) * pow(na + nb + c->denseps, -2.0 / 3.0)) / (GAMMA0c * 1.0);
+            (1.0 / 3.0) * alpha_plus(c, p) * pow(na + nb + c->denseps, -2.0 / 3.0)) / (GAMMA0c * 1.0);
 }
 
 /* Regularisation constants (pca_settings.h:226-239). */
@@ -690,14 +708,57 @@ void tdwo_potentials(const tdwo_cfg *c, int it, const double *dens, double *pots
     for (size_t i = 0; i < N; i++) {
         double na = rho_a[i], nb = rho_b[i];
         zc lnu = nu[i];
-        if (c->functional == TDWO_SLDA) {
+        if (c->functional == TDWO_SLDA || c->functional == TDWO_ASLDA) {
             double t1p = polarization(c, na, nb);
-            double alph_plus = alpha_plus(t1p);
+            double alph_plus = alpha_plus(c, t1p);
             double Va_const = ua[i], Vb_const = ub[i];
-            /* FAST_CONST_EFFECTIVE_MASS_MODE: d(alpha)/dn = 0 */
+            /* SLDA: FAST_CONST_EFFECTIVE_MASS_MODE, d(alpha)/dn = 0 */
             double dalphp_dna = 0.0, dalphp_dnb = 0.0;
-            Va_const += der_funD_na(c, na, nb);
-            Vb_const += der_funD_nb(c, na, nb);
+            if (c->functional == TDWO_ASLDA) {
+                /* CURRENT_CORRECTIONS and the density-dependent effective
+                 * mass (tdwslda_functionals.h:57-227, cccoeff = 1) */
+                double alph_minus = alpha_minus(c, t1p);
+                double aA = alpha_f(c, t1p), aB = alpha_f(c, -t1p);
+                al_a[i] = aA;
+                al_b[i] = aB;
+                const double *ja = dens + 4 * N, *jb = dens + 9 * N;
+                double *Aa = pots + 6 * N, *Ab = pots + 9 * N;
+                double fa = p_reg(c, na), fb = p_reg(c, nb);
+                for (int d = 0; d < c->dim; d++) {
+                    Aa[d * N + i] = fa == 0.0 ? 0.0 : fa * (1. - aA) * ja[d * N + i] / na;
+                    Ab[d * N + i] = fb == 0.0 ? 0.0 : fb * (1. - aB) * jb[d * N + i] / nb;
+                }
+                double taua = dens[3 * N + i], taub = dens[8 * N + i];
+                double dalphm_dna = der_alpha_minus_na(c, na, nb), dalphm_dnb = der_alpha_minus_nb(c, na, nb);
+                Va_const += dalphm_dna * (taua - taub) / 2.0;
+                Vb_const += dalphm_dnb * (taua - taub) / 2.0;
+                dalphp_dna = der_alpha_plus_na(c, na, nb);
+                dalphp_dnb = der_alpha_plus_nb(c, na, nb);
+                Va_const += dalphp_dna * (taua + taub) / 2.0;
+                Vb_const += dalphp_dnb * (taua + taub) / 2.0;
+                Va_const += der_funD_na(c, na, nb);
+                Vb_const += der_funD_nb(c, na, nb);
+                double j2a = 0.0, j2b = 0.0;
+                for (int d = 0; d < c->dim; d++) {
+                    j2a += ja[d * N + i] * ja[d * N + i];
+                    j2b += jb[d * N + i] * jb[d * N + i];
+                }
+                if (fa != 0.0) {
+                    double t7 = fa * j2a / (2.0 * na);
+                    Va_const += ((alph_plus + alph_minus - 1.0) / na - (dalphp_dna + dalphm_dna)) * t7;
+                    Va_const -= der_p_reg(c, na) * (alph_plus + alph_minus - 1.0) * j2a / (2.0 * na);
+                    Vb_const -= (dalphp_dnb + dalphm_dnb) * t7;
+                }
+                if (fb != 0.0) {
+                    double t7 = fb * j2b / (2.0 * nb);
+                    Va_const -= (dalphp_dna - dalphm_dna) * t7;
+                    Vb_const += ((alph_plus - alph_minus - 1.0) / nb - (dalphp_dnb - dalphm_dnb)) * t7;
+                    Vb_const -= der_p_reg(c, nb) * (alph_plus - alph_minus - 1.0) * j2b / (2.0 * nb);
+                }
+            } else {
+                Va_const += der_funD_na(c, na, nb);
+                Vb_const += der_funD_nb(c, na, nb);
+            }
             double t1 = dalphp_dna / alph_plus, t2 = dalphp_dnb / alph_plus;
             double t3 = der_tildeC_na(c, na, nb) / alph_plus;
             double t4 = der_tildeC_nb(c, na, nb) / alph_plus;
@@ -842,6 +903,121 @@ void tdwo_apply_h(const tdwo_cfg *c, int n, const double *in, double *out,
 }
 
 /* =========================================================================
+ * Hamiltonian with the density-dependent effective mass and the current
+ * corrections (ASLDA; pca_kernels.cu:233-391 and 674-790 without
+ * FAST_CONST_EFFECTIVE_MASS_MODE, with CURRENT_CORRECTIONS; 1D / 2D twins
+ * ccpca_kernels.cu:209-340, cpca_kernels.cu:231-370):
+ *   (H psi)_u = (V_a - mu_a + alpha_a k_t^2/2 - i/2 div A_a) u + Delta v
+ *               + lap(alpha_a) u / 4 - i A_a . grad u
+ *               - alpha_a lap(u) / 4 - lap(alpha_a u) / 4
+ * and the conjugate structure for v.  alap holds lap(alpha psi) (the
+ * reference's alphawf_laplace), ddx..ddz the gradients of psi along the
+ * evolved axes.  Other functionals: tdwo_apply_h.
+ * ========================================================================= */
+void tdwo_apply_h_full(const tdwo_cfg *c, int n, const double *in, double *out, const double *ddx,
+                       const double *ddy, const double *ddz, const double *lap, const double *alap,
+                       const double *pots, const double *kt, const double *useqpe, double *qpe)
+{
+    if (c->functional != TDWO_ASLDA) {
+        tdwo_apply_h(c, n, in, out, lap, pots, kt, useqpe, qpe);
+        return;
+    }
+    size_t N = tdwo_npts(c);
+    const zc *W = (const zc *)in, *L = (const zc *)lap, *AL = (const zc *)alap;
+    const zc *G[3] = {(const zc *)ddx, (const zc *)ddy, (const zc *)ddz};
+    zc *O = (zc *)out;
+    const double *Va = pots, *Vb = pots + N, *al_a = pots + 4 * N, *al_b = pots + 5 * N;
+    const double *Aa = pots + 6 * N, *Ab = pots + 9 * N;
+    const zc *D = (const zc *)(pots + 2 * N);
+    double dv = dvol_norm(c);
+    /* laplace of the effective masses and div A (compute_laplace_real_f,
+     * compute_derivative_real_vector_f: Nyquist derivatives zero) */
+    double *lal = (double *)malloc(sizeof(double) * 2 * N), *cj = (double *)malloc(sizeof(double) * 2 * N);
+    tdwo_laplace_real(c, al_a, lal);
+    tdwo_laplace_real(c, al_b, lal + N);
+    {
+        double *z = (double *)calloc(2 * N, sizeof(double));
+        double *d3[3];
+        for (int d = 0; d < 3; d++) d3[d] = (double *)malloc(sizeof(double) * 2 * N);
+        double *lz = (double *)malloc(sizeof(double) * 2 * N);
+        for (int sp = 0; sp < 2; sp++) {
+            const double *A = sp ? Ab : Aa;
+            for (size_t i = 0; i < N; i++) cj[sp * N + i] = 0.0;
+            for (int d = 0; d < c->dim; d++) {
+                for (size_t i = 0; i < N; i++) { z[2 * i] = A[d * N + i]; z[2 * i + 1] = 0.0; }
+                tdwo_derivatives(c, 1, z, d3[0], d3[1], d3[2], lz);
+                for (size_t i = 0; i < N; i++) cj[sp * N + i] += d3[d][2 * i];
+            }
+            for (size_t i = 0; i < N; i++) cj[sp * N + i] *= -0.5;
+        }
+        free(z); free(lz);
+        for (int d = 0; d < 3; d++) free(d3[d]);
+    }
+    double *e = (double *)malloc(sizeof(double) * (size_t)n);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) OMP_NT
+#endif
+    for (int a = 0; a < n; a++) {
+        double k2t = 0.0;
+        if (c->dim == 1) k2t = kt[n + a] * kt[n + a] + kt[a] * kt[a];
+        else if (c->dim == 2) k2t = kt[a] * kt[a];
+        double ssum = 0.0;
+        for (size_t i = 0; i < N; i++) {
+            double aa = al_a[i] * -0.25, ab = al_b[i] * -0.25;
+            double la = 0.25 * lal[i], lb = 0.25 * lal[N + i];
+            double ja = -2.0 * aa * k2t, jb = -2.0 * ab * k2t;
+            size_t iu = (size_t)a * N + i, iv = ((size_t)n + a) * N + i;
+            zc tu = W[iu], tv = W[iv];
+            zc u = 0, v = 0;
+            u += tu * CMPLX(Va[i] - c->mu_a + ja, cj[i]);
+            v -= tv * CMPLX(Vb[i] - c->mu_b + jb, -1.0 * cj[N + i]);
+            u += tv * D[i];
+            v += tu * conj(D[i]);
+            u += tu * la;
+            v -= tv * lb;
+            for (int d = 0; d < c->dim; d++) {
+                u += G[d][iu] * CMPLX(0.0, -1. * Aa[d * N + i]);
+                v -= G[d][iv] * CMPLX(0.0, 1. * Ab[d * N + i]);
+            }
+            u += L[iu] * aa;
+            v -= L[iv] * ab;
+            u += AL[iu] * (-0.25);
+            v -= AL[iv] * (-0.25);
+            O[iu] = u;
+            O[iv] = v;
+            if (!useqpe) ssum += creal(conj(tu) * u + conj(tv) * v) * dv;
+        }
+        e[a] = useqpe ? useqpe[a] : ssum;
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) OMP_NT
+#endif
+    for (int a = 0; a < n; a++) {
+        double ea = e[a];
+        for (size_t i = 0; i < N; i++) {
+            O[(size_t)a * N + i] -= W[(size_t)a * N + i] * ea;
+            O[((size_t)n + a) * N + i] -= W[((size_t)n + a) * N + i] * ea;
+        }
+    }
+    if (qpe) memcpy(qpe, e, sizeof(double) * (size_t)n);
+    free(e); free(lal); free(cj);
+}
+
+/* alpha_a u, alpha_b v (multiply_wf_by_alpha, pca_kernels.cu:981-1026) */
+void tdwo_alpha_wf(const tdwo_cfg *c, int n, const double *in, const double *pots, double *out)
+{
+    size_t N = tdwo_npts(c);
+    const zc *W = (const zc *)in;
+    zc *O = (zc *)out;
+    const double *al_a = pots + 4 * N, *al_b = pots + 5 * N;
+    for (int a = 0; a < n; a++)
+        for (size_t i = 0; i < N; i++) {
+            O[(size_t)a * N + i] = W[(size_t)a * N + i] * al_a[i];
+            O[((size_t)n + a) * N + i] = W[((size_t)n + a) * N + i] * al_b[i];
+        }
+}
+
+/* =========================================================================
  * Energy (tdwslda_energy_generic.h:21-110, tdwslda_functionals.h:308-366,
  * :455-510, pca_kernels.cu:121-209)
  * ========================================================================= */
@@ -865,9 +1041,9 @@ void tdwo_energy(const tdwo_cfg *c, int it, const double *dens, const double *po
         double tz1 = c->dim >= 3 ? jaz[i] : 0.0, tz2 = c->dim >= 3 ? jbz[i] : 0.0;
         double ta = tau_a[i] - p_reg(c, na) * (tx1 * tx1 + ty1 * ty1 + tz1 * tz1) / na;
         double tb = tau_b[i] - p_reg(c, nb) * (tx2 * tx2 + ty2 * ty2 + tz2 * tz2) / nb;
-        if (c->functional == TDWO_SLDA) {
+        if (c->functional == TDWO_SLDA || c->functional == TDWO_ASLDA) {
             double p = polarization(c, na, nb);
-            s[TDWO_EKIN] += 0.5 * (alpha_f(p) * ta + alpha_f(-1.0 * p) * tb) * VE;
+            s[TDWO_EKIN] += 0.5 * (alpha_f(c, p) * ta + alpha_f(c, -1.0 * p) * tb) * VE;
             s[TDWO_EPOT] += funD(c, na, nb) * VE;
         } else {
             s[TDWO_EKIN] += 0.5 * (al_a[i] * ta + al_b[i] * tb) * VE;
@@ -916,23 +1092,50 @@ static void half_densities(const tdwo_cfg *c, tdwo_state *s, int grad)
     if (grad) tdwo_density_tau(c, s->dens);
 }
 
+/* One H application of a half-step.  ASLDA (not FAST_CONST_EFFECTIVE_MASS
+ * _MODE, pca.c:1443-1465): gradients every half-step, then
+ * lap(alpha psi) and the full Hamiltonian. */
+static void half_h(const tdwo_cfg *c, tdwo_state *s, double *out)
+{
+    int n = s->n;
+    if (c->functional == TDWO_ASLDA) {
+        size_t N = tdwo_npts(c);
+        double *al = (double *)malloc(sizeof(double) * 4 * (size_t)n * N);
+        tdwo_alpha_wf(c, n, s->wf, s->pots, al);
+        tdwo_laplace(c, 2 * n, al, al);
+        tdwo_apply_h_full(c, n, s->wf, out, s->ddx, s->ddy, s->ddz, s->lap, al, s->pots, s->kt, NULL, s->qpe);
+        free(al);
+    } else {
+        tdwo_apply_h(c, n, s->wf, out, s->lap, s->pots, s->kt, NULL, s->qpe);
+    }
+}
+
 int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
 {
     int n = s->n;
     int last = c->scheme == TDWO_AB3AM4 ? 2 : c->scheme == TDWO_AB4AM5 ? 3 : 4;
+    const int full = c->functional == TDWO_ASLDA;
     /* predictor */
     tdwo_abm_step1(c, n, s->wf, s->f[0], s->f[1], s->f[2], s->f[3], s->f[4]);
     tdwo_normalize(c, n, s->wf);
-    half_densities(c, s, 0);
+    half_densities(c, s, full);
     tdwo_potentials(c, it + 1, s->dens, s->pots);
-    tdwo_apply_h(c, n, s->wf, s->lap, s->lap, s->pots, s->kt, NULL, s->qpe);
+    {
+        /* the H output goes to lap (the reference's buffer), which the
+         * full Hamiltonian still reads: stage it */
+        size_t N = tdwo_npts(c);
+        double *h = (double *)malloc(sizeof(double) * 4 * (size_t)n * N);
+        half_h(c, s, h);
+        memcpy(s->lap, h, sizeof(double) * 4 * (size_t)n * N);
+        free(h);
+    }
     /* corrector */
     tdwo_abm_step4(c, n, s->lap, s->wf, s->f[last]);
     tdwo_normalize(c, n, s->wf);
-    half_densities(c, s, grad_last);
+    half_densities(c, s, grad_last || full);
     tdwo_potentials(c, it + 1, s->dens, s->pots);
     rotate_history(c, s);
-    tdwo_apply_h(c, n, s->wf, s->f[0], s->lap, s->pots, s->kt, NULL, s->qpe);
+    half_h(c, s, s->f[0]);
     return 0;
 }
 
@@ -1113,9 +1316,11 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
     for (int ix = 0; ix < NX; ix++) for (int iy = 0; iy < NY; iy++) for (int iz = 0; iz < NZ; iz++, i++)
         kk2[i] = kkx[ix] * kkx[ix] + kky[iy] * kky[iy] + kkz[iz] * kkz[iz];
     double p = polarization(c, n0_a, n0_b);
-    double alph_a = alpha_f(p), alph_b = alpha_f(-p);
-    double alph_plus = alpha_plus(p);
-    double dalphm_dna = 0, dalphm_dnb = 0, dalphp_dna = 0, dalphp_dnb = 0; /* A1=A2=0 */
+    double alph_a = alpha_f(c, p), alph_b = alpha_f(c, -p);
+    double alph_plus = alpha_plus(c, p);
+    /* pca_uniform.h:157-163 (all zero for SLDA's A1 = A2 = 0) */
+    double dalphm_dna = der_alpha_minus_na(c, n0_a, n0_b), dalphm_dnb = der_alpha_minus_nb(c, n0_a, n0_b);
+    double dalphp_dna = der_alpha_plus_na(c, n0_a, n0_b), dalphp_dnb = der_alpha_plus_nb(c, n0_a, n0_b);
     double dtC_dna = der_tildeC_na(c, n0_a, n0_b), dtC_dnb = der_tildeC_nb(c, n0_a, n0_b);
     double dD_dna = der_funD_na(c, n0_a, n0_b), dD_dnb = der_funD_nb(c, n0_a, n0_b);
     double tC = tildeC(c, n0_a, n0_b), Dn = funD(c, n0_a, n0_b);

@@ -20,6 +20,17 @@
  *   - td-gauss-bdg goldens (testsuite/td-gauss-bdg/ref/test.ss0-1d.wlog.ref,
  *     all 101 rows; ss0-2d first rows): pin the BdG uniform solver, the BdG
  *     potentials and the BdG H application in the same way.
+ *   - td-gauss-aslda goldens (testsuite/td-gauss-aslda/ref/test.ss0-1d.wlog.ref
+ *     and .cmp.ref; ss0-2d first rows): pin the ASLDA uniform solver (alpha,
+ *     its density derivatives), the ASLDA potentials with the tau and current
+ *     terms, the H application with grad(alpha), div(A), A.grad, and the
+ *     every-half-step gradient schedule. Rows 0-7 (through the V_ext switch-on)
+ *     and the 2D rows 0-3 match all 10 printed digits; rows 8-100 of the 1D run
+ *     agree to 2.5e-6 relative (largest while the strong V_ext acts, t*eF =
+ *     20..40, 5e-8 at the end); the final state equals .cmp.ref to all printed
+ *     digits.  The 2.5e-6 band is systematic (a 1e-13 perturbation of the
+ *     start moves it by nothing comparable) and its source is not identified:
+ *     ASLDA parity at rows 8-100 is pinned to that tolerance only.
  */
 #ifndef TDW_ORACLE_H
 #define TDW_ORACLE_H
@@ -32,6 +43,7 @@ extern "C" {
 
 /* Functional ids (pca_settings.h:63-67). */
 #define TDWO_SLDA 111
+#define TDWO_ASLDA 112
 #define TDWO_BDG  113
 /* Regularisation ids (pca_settings.h:223-224). */
 #define TDWO_SPHERICAL 88
@@ -76,7 +88,7 @@ typedef struct {
     int dim;              /* 1, 2 or 3 (CODEDIM)                          */
     int nx, ny, nz;       /* NX, NY, NZ; for dim<3 the extra ones are transverse */
     double dx, dy, dz;
-    int functional;       /* TDWO_SLDA / TDWO_BDG                          */
+    int functional;       /* TDWO_SLDA / TDWO_ASLDA / TDWO_BDG             */
     int regularization;   /* TDWO_SPHERICAL / TDWO_CUBIC                   */
     int mf_correction;    /* INCLUDE_MF_CORRECTION_FROM_REG_SCHEME         */
     int scheme;           /* TDWO_AB3AM4 / AB4AM5 / AB5AM5                 */
@@ -138,6 +150,12 @@ void tdwo_apply_h(const tdwo_cfg *c, int n, const double *in, double *out,
                   const double *useqpe, double *qpe);
 
 /* --- energy (tdwslda_energy_generic.h:21-110) ----------------------------- */
+/* ASLDA Hamiltonian (effective mass and current terms); other functionals
+ * fall through to tdwo_apply_h.  alap = lap(alpha psi). */
+void tdwo_apply_h_full(const tdwo_cfg *c, int n, const double *in, double *out, const double *ddx,
+                       const double *ddy, const double *ddz, const double *lap, const double *alap,
+                       const double *pots, const double *kt, const double *useqpe, double *qpe);
+void tdwo_alpha_wf(const tdwo_cfg *c, int n, const double *in, const double *pots, double *out);
 void tdwo_energy(const tdwo_cfg *c, int it, const double *dens, const double *pots,
                  double *e);
 

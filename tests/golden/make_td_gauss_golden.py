@@ -1,15 +1,16 @@
-"""Extract the td-gauss-slda and td-gauss-bdg golden trajectories into small
-JSON fixtures.
+"""Extract the td-gauss-slda, td-gauss-aslda and td-gauss-bdg golden
+trajectories into small JSON fixtures.
 
 Reads the reference testsuite's own expected outputs (data, not code):
-  testsuite/td-gauss-{slda,bdg}/ref/test.ss0-{1d,2d,3d}.wlog.ref
-and the test inputs (testsuite/td-gauss-{slda,bdg}/input.txt.{1,2,3},
-predefines.h: 12^3 lattice, DX=1.15, SLDA or BDG, spherical cutoff, AB4AM5
+  testsuite/td-gauss-{slda,aslda,bdg}/ref/test.ss0-{1d,2d,3d}.wlog.ref
+  testsuite/td-gauss-{slda,aslda,bdg}/ref/test.ss0-{1d,2d,3d}.cmp.ref
+and the test inputs (testsuite/td-gauss-{slda,aslda,bdg}/input.txt.{1,2,3},
+predefines.h: 12^3 lattice, DX=1.15, SLDA, ASLDA or BDG, spherical cutoff, AB4AM5
 default, UNIFORM_TEST_MODE, params7=1 appended by test.desc for the ss0
 runs; aBdG = -1.1 is the scattering length of the BDG runs).
 
 Run in a tree where the reference is mounted; the JSON it writes are the
-committed fixtures tests/golden/td_gauss_slda.json and td_gauss_bdg.json.
+committed fixtures tests/golden/td_gauss_{slda,aslda,bdg}.json.
 """
 import json
 import os
@@ -39,6 +40,16 @@ def header(path):
     return h
 
 
+def final_values(path):
+    """test.*.cmp.ref: name, value, tolerance of the run's final state."""
+    out = {}
+    for line in open(path):
+        t = line.split()
+        if len(t) == 3:
+            out[t[0]] = [float(t[1]), float(t[2])]
+    return out
+
+
 def params(path):
     pr = [0.0] * 8
     for line in open(path):
@@ -49,7 +60,7 @@ def params(path):
     return pr
 
 
-for suite, functional in (("slda", "SLDA"), ("bdg", "BDG")):
+for suite, functional in (("slda", "SLDA"), ("aslda", "ASLDA"), ("bdg", "BDG")):
     data = {"columns": ["id", "t_eF", "Na", "Nb", "N", "Etot/Effg", "Ekin/Effg", "Epot/Effg",
                         "Epair/Effg", "Ecurr/Effg", "Eext/Effg", "Edext/Effg", "Evext/Effg"],
             "inputs": {"NX": 12, "NY": 12, "NZ": 12, "DX": 1.15, "functional": functional,
@@ -62,7 +73,9 @@ for suite, functional in (("slda", "SLDA"), ("bdg", "BDG")):
     for d, tag in ((1, "ss0-1d"), (2, "ss0-2d"), (3, "ss0-3d")):
         p = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.wlog.ref" % (suite, tag))
         inp = os.path.join(REF, "testsuite/td-gauss-%s/input.txt.%d" % (suite, d))
-        data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp)}
+        c = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.cmp.ref" % (suite, tag))
+        data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp),
+                             "final": final_values(c)}
     out = os.path.join(HERE, "td_gauss_%s.json" % suite)
     json.dump(data, open(out, "w"), indent=0)
     print("wrote", out)

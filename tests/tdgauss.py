@@ -15,13 +15,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def golden(functional=po.SLDA):
-    name = "td_gauss_bdg.json" if functional == po.BDG else "td_gauss_slda.json"
+    name = {po.BDG: "td_gauss_bdg.json", po.ASLDA: "td_gauss_aslda.json"}.get(functional, "td_gauss_slda.json")
     return json.load(open(os.path.join(HERE, "golden", name)))
 
 
+def mu_rule_of(functional):
+    """The uniform solver's chemical-potential update of the code that wrote
+    each golden: the SLDA and BdG logs come from the 2021.11.06 code (the
+    rule now kept commented at pca_uniform.h:425-426), the ASLDA log from
+    the 2026.09.19 code (the active rule, pca_uniform.h:427-428)."""
+    return 0 if functional == po.ASLDA else 1
+
+
 def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
-    """td-gauss-slda (default) or td-gauss-bdg (functional=BDG, scattering
-    length aBdG, BdG uniform solution) problem."""
+    """td-gauss-slda (default), td-gauss-aslda (functional=ASLDA: the
+    uniform alpha_a, alpha_b go into the effective-mass slots of pots) or
+    td-gauss-bdg (functional=BDG, scattering length aBdG, BdG uniform
+    solution) problem."""
     g = golden(functional)["inputs"]
     NX, DXv = g["NX"], g["DX"]
     LXYZ = (NX * DXv) ** 3
@@ -32,7 +42,7 @@ def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
     kc = 0.999999 * math.pi / DXv
     u, rc = po.uniform_solve(cfg, Na / LXYZ, Nb / LXYZ, kc, g["temperature"], eps=g["init0eps"],
                              scmix=g["init0scmix"], muchange=g["init0muchange"],
-                             maxiter=g["init0maxiter"], mu_rule=1)
+                             maxiter=g["init0maxiter"], mu_rule=mu_rule_of(functional))
     assert rc == 0
     n = u.n0_a + u.n0_b
     eF = (3 * math.pi ** 2 * n) ** (2 / 3) / 2
@@ -56,7 +66,11 @@ def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
     pots[:N] = u.V_a
     pots[N:2 * N] = u.V_b
     pots[2 * N:4 * N:2] = u.delta
-    pots[4 * N:6 * N] = 1.0
+    if functional == po.ASLDA:
+        pots[4 * N:5 * N] = u.alph_a
+        pots[5 * N:6 * N] = u.alph_b
+    else:
+        pots[4 * N:6 * N] = 1.0
     return dict(cfg=cfg, u=u, wf=wf, fbeta=fb, kt=kt, cnt=cnt, pots=pots, eF=eF, kF=kF,
                 Effg=Effg, timesteps=g["timesteps"])
 
