@@ -42,6 +42,7 @@ typedef struct { float re, im; } tdw_complex32;
 
 /* ids shared with the reference's predefines (pca_settings.h:63-67, 201-203, 223-224) */
 #define TDW_SLDA 111
+#define TDW_ASLDA 112
 #define TDW_BDG 113
 #define TDW_SPHERICAL_CUTOFF 88
 #define TDW_CUBIC_CUTOFF 89
@@ -95,7 +96,7 @@ typedef struct {
     int dim;             /* 1, 2, 3 (CODEDIM)                                 */
     int nx, ny, nz;      /* NX, NY, NZ; for dim<3 the extra axes are transverse */
     double dx, dy, dz;   /* DX, DY, DZ                                        */
-    int functional;      /* TDW_SLDA or TDW_BDG                               */
+    int functional;      /* TDW_SLDA, TDW_ASLDA or TDW_BDG                    */
     int regularization;  /* TDW_SPHERICAL_CUTOFF or TDW_CUBIC_CUTOFF          */
     int mf_correction;   /* INCLUDE_MF_CORRECTION_FROM_REG_SCHEME             */
     int scheme;          /* TDW_AB3AM4 / TDW_AB4AM5 / TDW_AB5AM5              */
@@ -136,14 +137,32 @@ int tdw_densities(tdw_ctx *c, int n, const tdw_complex *wf, const tdw_complex *d
                   const double *weights /* n or NULL */, double *dens, int gradients);
 int tdw_density_tau(tdw_ctx *c, double *dens);
 int tdw_potentials(tdw_ctx *c, int it, const double *dens, double *pots);
+/* SLDA / BdG: out = H in - <H> in from in and its Laplacian (TDW_EINVAL for
+ * ASLDA, whose H needs tdw_apply_h_full). */
 int tdw_apply_h(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
                 const tdw_complex *lap, const double *pots, const double *useqpe, double *qpe);
+/* Any functional: the H application of apply_hamiltonian (pca_kernels.cu:674)
+ * with the gradients ddx..ddz of in along the evolved axes and alap =
+ * lap(alpha psi) (tdw_alpha_wf, then tdw_laplace; NULL: formed in a context
+ * array).  ASLDA adds the effective-mass and current terms
+ * (pca_kernels.cu:233-391 without FAST_CONST_EFFECTIVE_MASS_MODE); the
+ * others ignore ddx..ddz and alap.  out may equal lap or alap. */
+int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
+                     const tdw_complex *ddx, const tdw_complex *ddy, const tdw_complex *ddz,
+                     const tdw_complex *lap, const tdw_complex *alap, const double *pots,
+                     const double *useqpe, double *qpe);
+/* out = alpha_a u, alpha_b v (multiply_wf_by_alpha, pca_kernels.cu:1016);
+ * in place allowed. */
+int tdw_alpha_wf(tdw_ctx *c, int n, const tdw_complex *in, const double *pots, tdw_complex *out);
 int tdw_energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e /* 11 */);
 int tdw_vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 
 /* One PECE step of pca.c:1401-1570 on the arrays of a run.  f[0..4] are
  * fkm1..fkm5 and are rotated in place (pointer swaps) as pca.c:1543-1562
- * does.  grad_last: full derivatives (tau, j) on this step's corrector. */
+ * does.  grad_last: full derivatives (tau, j) on this step's corrector.
+ * ASLDA computes the derivatives on both half-steps (the reference's path
+ * without FAST_CONST_EFFECTIVE_MASS_MODE), so ddx..ddz of the evolved axes
+ * must be set; grad_last changes nothing there. */
 typedef struct {
     int n;
     tdw_complex *wf, *lap, *ddx, *ddy, *ddz;
@@ -158,7 +177,8 @@ int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
  * normalisation and <H> subtraction are applied by the next fused step, or by
  * tdw_finalize(), which every other entry point calls first.  Densities,
  * potentials and qpe are exact after every step.  Lattices the fused kernels
- * are not built for run the modular step (tdw_fused_supported() says which).
+ * are not built for, and ASLDA, run the modular step (tdw_fused_supported()
+ * says which).
  *
  * Split precision (params.precision == TDW_SPLIT): the x / yz FFTs and the
  * kinetic term run in FP32 complex, and f[0..4] hold FP32 complex values
@@ -374,6 +394,8 @@ int apply_hamiltonian_2d(int it, int n, tdw_complex *wf_in, tdw_complex *wf_out,
                          tdw_complex *wf_laplace, tdw_complex *alphawf_laplace,
                          double *d_densities, double *d_potentials, double qfswitch,
                          double *useqpe, double pccoeff, int nthreads);
+int multiply_wf_by_alpha(int n, tdw_complex *wf_in, tdw_complex *wf_out, double *d_potentials,
+                         int nthreads);                              /* pca_kernels.cu:1016 */
 int normalize_wf(int n, tdw_complex *wf, int nthreads);              /* pca_kernels.cu:955 */
 int abm_step1(int n, tdw_complex *ykm1, tdw_complex *fkm1, tdw_complex *fkm2, tdw_complex *fkm3,
               tdw_complex *fkm4, tdw_complex *fkm5, int abm_scheme, int nthreads); /* tdwslda_ode_integrator.h:106 */

@@ -157,7 +157,7 @@ extern "C" int tdw_create(const tdw_params *p, int nwf, int device, tdw_ctx **ou
 {
     if (!p || !out || nwf <= 0) return TDW_EINVAL;
     if (p->dim < 1 || p->dim > 3 || p->nx < 2) return TDW_EINVAL;
-    if (p->functional != TDW_SLDA && p->functional != TDW_BDG) return TDW_EINVAL;
+    if (p->functional != TDW_SLDA && p->functional != TDW_ASLDA && p->functional != TDW_BDG) return TDW_EINVAL;
     if (p->scheme != TDW_AB3AM4 && p->scheme != TDW_AB4AM5 && p->scheme != TDW_AB5AM5) return TDW_EINVAL;
     TDW_HIP(hipSetDevice(device));
     tdw_ctx *c = (tdw_ctx *)calloc(1, sizeof(tdw_ctx));
@@ -217,7 +217,7 @@ extern "C" int tdw_destroy(tdw_ctx *c)
         if (c->d_tw32[k]) hipFree(c->d_tw32[k]);
     }
     void *bufs[] = {c->d_fbeta, c->d_kt, c->d_cnt, c->d_part, c->d_nrm, c->d_qpe, c->d_esum,
-                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale, c->d_adjv};
+                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale, c->d_adjv, c->d_hx, c->d_ascr, c->d_aw};
     for (void *b : bufs)
         if (b) hipFree(b);
     fused_release(c);
@@ -363,7 +363,27 @@ extern "C" int tdw_apply_h(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw
                            const tdw_complex *lap, const double *pots, const double *useqpe, double *qpe)
 {
     if (!c || n > c->nwf || in == out) return TDW_EINVAL;
+    // ASLDA's H needs the gradients and lap(alpha psi): tdw_apply_h_full
+    if (c->p.functional == TDW_ASLDA) return TDW_EINVAL;
     return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
+}
+extern "C" int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
+                                const tdw_complex *ddx, const tdw_complex *ddy, const tdw_complex *ddz,
+                                const tdw_complex *lap, const tdw_complex *alap, const double *pots,
+                                const double *useqpe, double *qpe)
+{
+    if (!c || n > c->nwf || in == out) return TDW_EINVAL;
+    if (c->p.functional != TDW_ASLDA)
+        return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
+    if (!ddx || (c->dim >= 2 && !ddy) || (c->dim >= 3 && !ddz) || out == ddx || out == ddy || out == ddz)
+        return TDW_EINVAL;
+    return apply_h_aslda(c, n, CD2(in), D2(out), CD2(ddx), CD2(ddy), CD2(ddz), CD2(lap), CD2(alap), pots, c->d_kt,
+                         useqpe, qpe ? qpe : c->d_qpe);
+}
+extern "C" int tdw_alpha_wf(tdw_ctx *c, int n, const tdw_complex *in, const double *pots, tdw_complex *out)
+{
+    if (!c || n > c->nwf) return TDW_EINVAL;
+    return alpha_wf(c, n, CD2(in), D2(out), pots);
 }
 extern "C" int tdw_energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e)
 {
@@ -397,6 +417,32 @@ static int nhist(const tdw_ctx *c)
     return c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5;
 }
 
+// The ASLDA step (not FAST_CONST_EFFECTIVE_MASS_MODE, pca.c:1443-1465 and
+// :1510-1535): every half-step computes the derivatives, tau and j, the
+// potentials with the effective masses and vector potentials, lap(alpha psi)
+// and the full Hamiltonian.
+static int step_aslda(tdw_ctx *c, tdw_state *s, int it, int k, double *qpe)
+{
+    const int n = s->n;
+    // predictor
+    TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
+    TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
+    TDW_TRY(half_densities(c, s, 1));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(apply_h_aslda(c, n, D2(s->wf), D2(s->lap), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
+                          s->pots, c->d_kt, nullptr, qpe));
+    // corrector
+    TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
+    TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
+    TDW_TRY(half_densities(c, s, 1));
+    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    tdw_complex *tmp = s->f[k - 1];
+    for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+    s->f[0] = tmp;
+    return apply_h_aslda(c, n, D2(s->wf), D2(s->f[0]), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
+                         s->pots, c->d_kt, nullptr, qpe);
+}
+
 extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 {
     if (!c || !s || s->n > c->nwf) return TDW_EINVAL;
@@ -406,6 +452,10 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(fused_finalize(c, s));
     const int n = s->n, k = nhist(c);
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    if (c->p.functional == TDW_ASLDA) {
+        if (!s->ddx || (c->dim >= 2 && !s->ddy) || (c->dim >= 3 && !s->ddz)) return TDW_EINVAL;
+        return step_aslda(c, s, it, k, qpe);
+    }
     // 2D lattices with a one-pass plane kernel apply H with the Laplacian
     // formed in LDS on non-gradient half-steps (lap then holds H y_p, not the
     // Laplacian, after the step)
@@ -581,9 +631,8 @@ extern "C" int density_caculate_tau(double *dens, int nthreads)
 extern "C" int compute_potentials(int it, double *dens, double *pots, double cccoeff, int nthreads)
 {
     BOUND();
-    (void)cccoeff;
     (void)nthreads;
-    return potentials(c, it, dens, pots);
+    return potentials(c, it, dens, pots, cccoeff);
 }
 extern "C" int compute_energy(int it, double *dens, double *pots, double *work, int nthreads)
 {
@@ -595,11 +644,18 @@ extern "C" int compute_energy(int it, double *dens, double *pots, double *work, 
     return 0;
 }
 
-static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *lap,
-                       const double *kt, double *pots, double qfswitch, double *useqpe)
+static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx,
+                       tdw_complex *dy, tdw_complex *dz, tdw_complex *lap, tdw_complex *alap, const double *kt,
+                       double *pots, double qfswitch, double *useqpe)
 {
     if (qfswitch > 0.0) return TDW_EINVAL;   // quantum friction is not part of this path
-    TDW_TRY(apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, kt, useqpe, c->d_qpe));
+    if (c->p.functional == TDW_ASLDA) {
+        if (!dx || (c->dim >= 2 && !dy) || (c->dim >= 3 && !dz) || !alap || in == out) return TDW_EINVAL;
+        TDW_TRY(apply_h_aslda(c, n, CD2(in), D2(out), CD2(dx), CD2(dy), CD2(dz), CD2(lap), CD2(alap), pots, kt,
+                              useqpe, c->d_qpe));
+    } else {
+        TDW_TRY(apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, kt, useqpe, c->d_qpe));
+    }
     // <H> is left at the head of the work area (pca_kernels.cu:812-822)
     if (c->ref_workarea)
         TDW_HIP(hipMemcpyAsync(c->ref_workarea, useqpe ? useqpe : c->d_qpe, sizeof(double) * n,
@@ -611,24 +667,30 @@ extern "C" int apply_hamiltonian(int it, int n, tdw_complex *in, tdw_complex *ou
                                  double qfswitch, double *useqpe, double pccoeff, int nthreads)
 {
     BOUND();
-    (void)dx; (void)dy; (void)dz; (void)alap; (void)dens; (void)pccoeff; (void)nthreads;
-    return apply_h_ref(c, it, n, in, out, lap, c->d_kt, pots, qfswitch, useqpe);
+    (void)dens; (void)pccoeff; (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, dx, dy, dz, lap, alap, c->d_kt, pots, qfswitch, useqpe);
 }
 extern "C" int apply_hamiltonian_1d(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx, double *kkyz,
                                     tdw_complex *lap, tdw_complex *alap, double *dens, double *pots, double qfswitch,
                                     double *useqpe, double pccoeff, int nthreads)
 {
     BOUND();
-    (void)dx; (void)alap; (void)dens; (void)pccoeff; (void)nthreads;
-    return apply_h_ref(c, it, n, in, out, lap, kkyz, pots, qfswitch, useqpe);
+    (void)dens; (void)pccoeff; (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, dx, nullptr, nullptr, lap, alap, kkyz, pots, qfswitch, useqpe);
 }
 extern "C" int apply_hamiltonian_2d(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx,
                                     tdw_complex *dy, double *kkz, tdw_complex *lap, tdw_complex *alap, double *dens,
                                     double *pots, double qfswitch, double *useqpe, double pccoeff, int nthreads)
 {
     BOUND();
-    (void)dx; (void)dy; (void)alap; (void)dens; (void)pccoeff; (void)nthreads;
-    return apply_h_ref(c, it, n, in, out, lap, kkz, pots, qfswitch, useqpe);
+    (void)dens; (void)pccoeff; (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, dx, dy, nullptr, lap, alap, kkz, pots, qfswitch, useqpe);
+}
+extern "C" int multiply_wf_by_alpha(int n, tdw_complex *wf_in, tdw_complex *wf_out, double *pots, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    return alpha_wf(c, n, CD2(wf_in), D2(wf_out), pots);
 }
 extern "C" int normalize_wf(int n, tdw_complex *wf, int nthreads)
 {

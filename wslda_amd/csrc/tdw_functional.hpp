@@ -1,9 +1,11 @@
-// tdw_functional.hpp -- pointwise SLDA / BdG functional on the device.
+// tdw_functional.hpp -- pointwise SLDA / ASLDA / BdG functional on the device.
 //
-// Restates hpc-engine/pca_edf.h:17-352 (with the A0=1 effective mass of
-// SLDA_FORCE_A1, pca_settings.h:80-90) and the potential loops of
-// tdwslda_functionals.h:47-306 (SLDA) and :373-452 (BdG).
+// Restates hpc-engine/pca_edf.h:17-352 (SLDA with the A0=1 effective mass of
+// SLDA_FORCE_A1, pca_settings.h:80-90; ASLDA with the polarization-dependent
+// effective mass of tdw_aslda.hpp) and the potential loops of
+// tdwslda_functionals.h:47-306 (SLDA, ASLDA) and :373-452 (BdG).
 #pragma once
+#include "tdw_aslda.hpp"
 #include "tdw_device.hpp"
 
 namespace tdw {
@@ -21,7 +23,7 @@ struct FParams {
     double p_nmin, p_nmax, denseps;
     double mu_a, mu_b, ec, dx;
     double R0, R1;
-    int cubic, mf, bdg;
+    int cubic, mf, bdg, aslda;
     double sclgth;
 };
 
@@ -168,6 +170,83 @@ __device__ __forceinline__ void slda_point(const FParams &f, double na, double n
         double Vbnew = Vb_const - t4 * t7b;
         if (f.mf) {
             double dl = mf_corr(f, p0, g.x, lkF, na + nb, alph_plus);
+            Vanew += dl * t7b;
+            Vbnew += dl * t7b;
+        }
+        Va = TDW_UD_MIX * Vanew + (1.0 - TDW_UD_MIX) * Va;
+        Vb = TDW_UD_MIX * Vbnew + (1.0 - TDW_UD_MIX) * Vb;
+    }
+    delta = ld;
+}
+
+// ASLDA potentials at one point (tdwslda_functionals.h:47-306 without
+// FAST_CONST_EFFECTIVE_MASS_MODE, with CURRENT_CORRECTIONS): V_a, V_b, Delta,
+// the effective masses alpha_a, alpha_b and the vector potentials
+// A_s = cc f_r(n_s) (1 - alpha_s) j_s / n_s (dim components; cc is the
+// current-corrections coefficient).  tau and j are the densities of the same
+// half-step.
+__device__ __forceinline__ void aslda_point(const FParams &f, double na, double nb, double2 nu, double taua,
+                                            double taub, const double *ja, const double *jb, int dim, double cc,
+                                            double ua, double ub, double &Va, double &Vb, double2 &delta,
+                                            double &ala, double &alb, double *Aa, double *Ab)
+{
+    const double n = na + nb;
+    const double p = polar(f, na, nb);
+    const double dpa = der_pol(f, na, nb, 0), dpb = der_pol(f, na, nb, 1);
+    const double alph_plus = aslda::alpha_plus(p), alph_minus = aslda::alpha_minus(p);
+    const double aA = aslda::alpha(p), aB = aslda::alpha(-p);
+    ala = aA;
+    alb = aB;
+    const double fa = p_reg(f, na) * cc, fb = p_reg(f, nb) * cc;
+    double j2a = 0.0, j2b = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= dim) break;
+        Aa[d] = fa == 0.0 ? 0.0 : fa * (1. - aA) * ja[d] / na;
+        Ab[d] = fb == 0.0 ? 0.0 : fb * (1. - aB) * jb[d] / nb;
+        j2a += ja[d] * ja[d];
+        j2b += jb[d] * jb[d];
+    }
+    const double dalphp_dna = aslda::dalpha_plus_dp(p) * dpa, dalphp_dnb = aslda::dalpha_plus_dp(p) * dpb;
+    const double dalphm_dna = aslda::dalpha_minus_dp(p) * dpa, dalphm_dnb = aslda::dalpha_minus_dp(p) * dpb;
+    double Va_const = ua, Vb_const = ub;
+    Va_const += dalphm_dna * (taua - taub) / 2.0;
+    Vb_const += dalphm_dnb * (taua - taub) / 2.0;
+    Va_const += dalphp_dna * (taua + taub) / 2.0;
+    Vb_const += dalphp_dnb * (taua + taub) / 2.0;
+    Va_const += aslda::der_funD(n, p, dpa);
+    Vb_const += aslda::der_funD(n, p, dpb);
+    if (fa != 0.0) {
+        const double t7 = fa * j2a / (2.0 * na);
+        Va_const += ((alph_plus + alph_minus - 1.0) / na - (dalphp_dna + dalphm_dna)) * t7;
+        Va_const -= cc * der_p_reg(f, na) * (alph_plus + alph_minus - 1.0) * j2a / (2.0 * na);
+        Vb_const -= (dalphp_dnb + dalphm_dnb) * t7;
+    }
+    if (fb != 0.0) {
+        const double t7 = fb * j2b / (2.0 * nb);
+        Va_const -= (dalphp_dna - dalphm_dna) * t7;
+        Vb_const += ((alph_plus - alph_minus - 1.0) / nb - (dalphp_dnb - dalphm_dnb)) * t7;
+        Vb_const -= cc * der_p_reg(f, nb) * (alph_plus - alph_minus - 1.0) * j2b / (2.0 * nb);
+    }
+    const double t1 = dalphp_dna / alph_plus, t2 = dalphp_dnb / alph_plus;
+    const double t3 = aslda::der_tildeC(n, p, dpa, f.denseps) / alph_plus;
+    const double t4 = aslda::der_tildeC(n, p, dpb, f.denseps) / alph_plus;
+    const double t5 = aslda::tildeC(n, p, f.denseps);
+    double lkF = 0.0;
+    if (f.mf) lkF = pow(3.0 * M_PI * M_PI * n, 1.0 / 3.0);
+    double2 ld = make_double2(0.0, 0.0);
+    for (int k = 0; k < TDW_UD_SCITERS; k++) {
+        double t7 = (f.mu_a - Va + f.mu_b - Vb) / 2.0;
+        double2 wz, p0;
+        reg_wz(f, t7, alph_plus, alph_plus, wz, p0);
+        double2 g = cdiv(make_double2(alph_plus, 0.0), make_double2(t5 - wz.x, -wz.y));
+        ld = nu * (-1.0 * g.x);
+        const double t6 = ld.x * nu.x + ld.y * nu.y;   // Re(conj(delta) * nu)
+        const double t7b = ld.x * ld.x + ld.y * ld.y;
+        double Vanew = Va_const - t1 * t6 - t3 * t7b;
+        double Vbnew = Vb_const - t2 * t6 - t4 * t7b;
+        if (f.mf) {
+            double dl = mf_corr(f, p0, g.x, lkF, n, alph_plus);
             Vanew += dl * t7b;
             Vbnew += dl * t7b;
         }

@@ -58,6 +58,10 @@ struct tdw_ctx {
     int pending;                         // y and f[0] carry a pending scale / <H> fix-up
     // adjoint
     double *d_adjv;                      // 4n per-state scalars of the backward step
+    // ASLDA (allocated on first use)
+    double *d_hx;                        // 4N: lap alpha_a, lap alpha_b, div A_a, div A_b
+    double2 *d_ascr;                     // 4N complex: derivative scratch of the real fields
+    double2 *d_aw;   size_t aw_cap;      // lap(alpha psi) (complex elements)
     // per-kernel timing
     int timing;
     std::vector<tdw_timer_rec> *timers;  // pending records

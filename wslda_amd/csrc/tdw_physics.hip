@@ -406,12 +406,14 @@ FParams fparams(const tdw_ctx *c)
     else { f.R0 = 1. / (2.0 * M_PI * M_PI); f.R1 = 0.5; }
     f.mf = c->p.mf_correction;
     f.bdg = c->p.functional == TDW_BDG;
+    f.aslda = c->p.functional == TDW_ASLDA;
     f.sclgth = c->p.sclgth;
     return f;
 }
 
 __global__ void __launch_bounds__(NT)
-k_potentials(FParams f, VextDev V, long N, const double *__restrict__ dens, double *__restrict__ pots)
+k_potentials(FParams f, VextDev V, long N, int dim, double cc, const double *__restrict__ dens,
+             double *__restrict__ pots)
 {
     long i = (long)blockIdx.x * NT + threadIdx.x;
     if (i >= N) return;
@@ -421,7 +423,25 @@ k_potentials(FParams f, VextDev V, long N, const double *__restrict__ dens, doub
     vext_point(V, i, ua, ub);
     double Va = pots[i], Vb = pots[N + i];
     double2 d;
-    if (f.bdg) bdg_point(f, na, nb, nu, ua, ub, pots[4 * N + i], pots[5 * N + i], Va, Vb, d);
+    if (f.aslda) {
+        double ja[3] = {0.0, 0.0, 0.0}, jb[3] = {0.0, 0.0, 0.0}, Aa[3], Ab[3], ala, alb;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (k < dim) {
+                ja[k] = dens[(4 + k) * N + i];
+                jb[k] = dens[(9 + k) * N + i];
+            }
+        aslda_point(f, na, nb, nu, dens[3 * N + i], dens[8 * N + i], ja, jb, dim, cc, ua, ub, Va, Vb, d, ala, alb,
+                    Aa, Ab);
+        pots[4 * N + i] = ala;
+        pots[5 * N + i] = alb;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if (k < dim) {
+                pots[(6 + k) * N + i] = Aa[k];
+                pots[(9 + k) * N + i] = Ab[k];
+            }
+    } else if (f.bdg) bdg_point(f, na, nb, nu, ua, ub, pots[4 * N + i], pots[5 * N + i], Va, Vb, d);
     else slda_point(f, na, nb, nu, ua, ub, Va, Vb, d);
     pots[i] = Va;
     pots[N + i] = Vb;
@@ -429,12 +449,12 @@ k_potentials(FParams f, VextDev V, long N, const double *__restrict__ dens, doub
     pots[2 * N + 2 * i + 1] = d.y;
 }
 
-int potentials(tdw_ctx *c, int it, const double *dens, double *pots)
+int potentials(tdw_ctx *c, int it, const double *dens, double *pots, double cccoeff)
 {
     TDW_TIMED(c, "potentials");
     long N = (long)c->N;
     hipLaunchKernelGGL(k_potentials, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, fparams(c),
-                       make_vext(c, it), N, dens, pots);
+                       make_vext(c, it), N, c->dim, cccoeff, dens, pots);
     return (int)hipGetLastError();
 }
 
@@ -563,6 +583,165 @@ int apply_h_only(tdw_ctx *c, int n, const double2 *in, double2 *out, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// ASLDA: Hamiltonian with the density-dependent effective mass and the
+// current terms (pca_kernels.cu:233-391 and 674-822 without
+// FAST_CONST_EFFECTIVE_MASS_MODE, with CURRENT_CORRECTIONS; 1D / 2D twins
+// ccpca_kernels.cu:209-350, cpca_kernels.cu:231-385):
+//   (H psi)_u = (V_a - mu_a + alpha_a k_t^2/2 - i/2 div A_a) u + Delta v
+//               + lap(alpha_a) u / 4 - i A_a . grad u
+//               - alpha_a lap(u) / 4 - lap(alpha_a u) / 4
+// and the conjugate structure for v, then the <H> subtraction.
+// ---------------------------------------------------------------------------
+// out = alpha_s psi_s (multiply_wf_by_alpha, pca_kernels.cu:981-1026); in
+// place allowed
+__global__ void __launch_bounds__(NT)
+k_alpha_wf(const double2 *in, double2 *out, const double *__restrict__ pots, int n, long N)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    const long tot = 2L * n * N;
+    if (i >= tot) return;
+    const long j = i % N;
+    out[i] = in[i] * pots[(i < (long)n * N ? 4 : 5) * N + j];
+}
+
+int alpha_wf(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots)
+{
+    TDW_TIMED(c, "alpha_wf");
+    const long tot = 2L * n * (long)c->N;
+    hipLaunchKernelGGL(k_alpha_wf, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, pots, n, (long)c->N);
+    return (int)hipGetLastError();
+}
+
+// acc = Re(dd) (first) or acc += Re(dd)
+__global__ void k_re_acc(const double2 *__restrict__ dd, double *__restrict__ acc, long N, int first)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    acc[i] = first ? dd[i].x : acc[i] + dd[i].x;
+}
+
+static int ensure_aslda(tdw_ctx *c)
+{
+    const size_t N = c->N;
+    if (!c->d_hx) TDW_HIP(hipMalloc(&c->d_hx, sizeof(double) * 4 * N));
+    if (!c->d_ascr) TDW_HIP(hipMalloc(&c->d_ascr, sizeof(double2) * 4 * N));
+    return 0;
+}
+
+// hx = [lap alpha_a, lap alpha_b, div A_a, div A_b] of the potentials
+// (compute_laplace_real_f and compute_derivative_real_vector_f with the
+// Nyquist derivative set to zero, pca_kernels.cu:700-760)
+int aslda_fields(tdw_ctx *c, const double *pots)
+{
+    TDW_TRY(ensure_aslda(c));
+    const long N = (long)c->N;
+    TDW_TRY(laplace_real(c, pots + 4 * N, c->d_hx));
+    TDW_TRY(laplace_real(c, pots + 5 * N, c->d_hx + N));
+    double2 *g = c->d_ascr;
+    for (int sp = 0; sp < 2; sp++) {
+        const double *A = pots + (sp ? 9 : 6) * N;
+        double *acc = c->d_hx + (2 + sp) * N;
+        for (int d = 0; d < c->dim; d++) {
+            hipLaunchKernelGGL(k_real2cplx, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, A + d * N, c->d_rtmp, N);
+            TDW_TRY(derivatives(c, 1, c->d_rtmp, g, g + N, g + 2 * N, g + 3 * N));
+            hipLaunchKernelGGL(k_re_acc, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, g + d * N, acc, N, d == 0);
+        }
+    }
+    return (int)hipGetLastError();
+}
+
+__global__ void __launch_bounds__(NT)
+k_apply_h_aslda(const double2 *__restrict__ in, double2 *out, const double2 *__restrict__ gx,
+                const double2 *__restrict__ gy, const double2 *__restrict__ gz, const double2 *lap,
+                const double2 *alap, const double *__restrict__ pots, const double *__restrict__ hx,
+                const double *__restrict__ kt, int n, long N, long chunk, int dim, double mu_a, double mu_b,
+                double *__restrict__ part)
+{
+    __shared__ double red[NT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    double k2t = 0.0;
+    if (dim == 1) k2t = kt[n + a] * kt[n + a] + kt[a] * kt[a];
+    else if (dim == 2) k2t = kt[a] * kt[a];
+    const long ou = (long)a * N, ov = ((long)n + a) * N;
+    const double2 *G[3] = {gx, gy, gz};
+    double s = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += NT) {
+        const double aa = pots[4 * N + i] * -0.25, ab = pots[5 * N + i] * -0.25;
+        const double la = 0.25 * hx[i], lb = 0.25 * hx[N + i];
+        const double ca = -0.5 * hx[2 * N + i], cb = -0.5 * hx[3 * N + i];
+        const double ja = -2.0 * aa * k2t, jb = -2.0 * ab * k2t;
+        const double2 D = make_double2(pots[2 * N + 2 * i], pots[2 * N + 2 * i + 1]);
+        const double2 tu = in[ou + i], tv = in[ov + i];
+        const double2 lu = lap[ou + i], lv = lap[ov + i];
+        const double2 alu = alap[ou + i], alv = alap[ov + i];
+        double2 u = cmul(tu, make_double2(pots[i] - mu_a + ja, ca));
+        double2 v = make_double2(0.0, 0.0) - cmul(tv, make_double2(pots[N + i] - mu_b + jb, -1.0 * cb));
+        u = u + cmul(tv, D);
+        v = v + cmulc(D, tu);
+        u = u + tu * la;
+        v = v - tv * lb;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (d >= dim) break;
+            const double Aa = pots[(6 + d) * N + i], Ab = pots[(9 + d) * N + i];
+            u = u + cmul(G[d][ou + i], make_double2(0.0, -1.0 * Aa));
+            v = v - cmul(G[d][ov + i], make_double2(0.0, Ab));
+        }
+        u = u + lu * aa;
+        v = v - lv * ab;
+        u = u + alu * (-0.25);
+        v = v - alv * (-0.25);
+        out[ou + i] = u;
+        out[ov + i] = v;
+        s += cmulc(tu, u).x + cmulc(tv, v).x;
+    }
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = s;
+}
+
+int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *ddx, const double2 *ddy,
+                  const double2 *ddz, const double2 *lap, const double2 *alap, const double *pots, const double *kt,
+                  const double *useqpe, double *d_qpe)
+{
+    const long N = (long)c->N;
+    if (!alap) {
+        // lap(alpha psi) into the context's array
+        const size_t need = (size_t)2 * n * N;
+        if (need > c->aw_cap) {
+            TDW_HIP(hipStreamSynchronize(c->stream));
+            if (c->d_aw) TDW_HIP(hipFree(c->d_aw));
+            c->d_aw = nullptr;
+            c->aw_cap = 0;
+            TDW_HIP(hipMalloc(&c->d_aw, need * sizeof(double2)));
+            c->aw_cap = need;
+        }
+        TDW_TRY(alpha_wf(c, n, in, c->d_aw, pots));
+        TDW_TRY(laplace(c, 2 * n, c->d_aw, c->d_aw));
+        alap = c->d_aw;
+    }
+    TDW_TRY(aslda_fields(c, pots));
+    TDW_TIMED(c, "apply_h");
+    int nchunk;
+    long chunk = pick_chunk(c, &nchunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    hipLaunchKernelGGL(k_apply_h_aslda, dim3(nchunk, n), dim3(NT), 0, c->stream, in, out, ddx,
+                       c->dim >= 2 ? ddy : ddx, c->dim >= 3 ? ddz : ddx, lap, alap, pots, c->d_hx, kt, n, N, chunk,
+                       c->dim, c->p.mu_a, c->p.mu_b, c->d_part);
+    const double *q = useqpe;
+    if (!q) {
+        hipLaunchKernelGGL(k_rowsum, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk,
+                           dvol_norm(c), d_qpe);
+        q = d_qpe;
+    } else if (d_qpe && d_qpe != useqpe) {
+        TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const long tot = 2L * n * N;
+    hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // energy
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT)
@@ -582,7 +761,11 @@ k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restr
         const double ja2 = tx1 * tx1 + ty1 * ty1 + tz1 * tz1, jb2 = tx2 * tx2 + ty2 * ty2 + tz2 * tz2;
         const double ta = dens[3 * N + i] - p_reg(f, na) * ja2 / na;
         const double tb = dens[8 * N + i] - p_reg(f, nb) * jb2 / nb;
-        if (!f.bdg) {
+        if (f.aslda) {
+            const double p = polar(f, na, nb);
+            e[TDW_EKIN] = 0.5 * (aslda::alpha(p) * ta + aslda::alpha(-1.0 * p) * tb) * VE;
+            e[TDW_EPOT] = aslda::funD(na + nb, p) * VE;
+        } else if (!f.bdg) {
             const double p = polar(f, na, nb);
             e[TDW_EKIN] = 0.5 * (alpha_f(p) * ta + alpha_f(-1.0 * p) * tb) * VE;
             e[TDW_EPOT] = funD(f, na, nb) * VE;

@@ -94,7 +94,9 @@ int densities32(tdw_ctx *c, int n, const float2 *wf, const double *fbeta, const 
                 const int *cnt, double *dens);
 int density_tau(tdw_ctx *c, double *dens);
 int vext_eval(tdw_ctx *c, int it, double *va, double *vb);
-int potentials(tdw_ctx *c, int it, const double *dens, double *pots);
+// cccoeff: the current-corrections coefficient of the ASLDA vector
+// potentials and current terms (1 after the reference's default ramp)
+int potentials(tdw_ctx *c, int it, const double *dens, double *pots, double cccoeff = 1.0);
 int apply_h(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, const double2 *lap,
             const double *pots, const double *kt, const double *useqpe, double *d_qpe);
 int apply_h_only(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *lap, const double *pots,
@@ -106,6 +108,14 @@ int apply_h_plane(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, co
 // With dotw, also d_dot[a] = dscale Re<in_a, dotw_a> (both components).
 int apply_h_only_plane(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, const double *kt,
                        const double2 *dotw = nullptr, double dscale = 0.0, double *d_dot = nullptr);
+// ASLDA: out = alpha_s psi_s (in place allowed)
+int alpha_wf(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots);
+// ASLDA H application with the <H> subtraction: ddx..ddz the gradients of in
+// along the evolved axes, lap its Laplacian, alap lap(alpha psi) (NULL: formed
+// here).  out may equal lap or alap (pointwise), not in.
+int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *ddx, const double2 *ddy,
+                  const double2 *ddz, const double2 *lap, const double2 *alap, const double *pots, const double *kt,
+                  const double *useqpe, double *d_qpe);
 int energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e);
 int taylor(tdw_ctx *c, int order, double dt, int n, const double2 *hpsi, double2 *upd, double2 *contr);
 

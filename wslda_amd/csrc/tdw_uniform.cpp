@@ -1,7 +1,9 @@
-// tdw_uniform.cpp -- uniform SLDA solution and plane-wave BdG states (host).
+// tdw_uniform.cpp -- uniform SLDA / ASLDA solution and plane-wave BdG states
+// (host).
 //
 // Restates hpc-engine/pca_uniform.h: fbeta (:70-78), solve_uniform_problem
-// (:86-502, SLDA with the A0=1 effective mass, no spin symmetry),
+// (:86-502, SLDA with the A0=1 effective mass or ASLDA with the
+// polarization-dependent one of tdw_aslda.hpp, no spin symmetry),
 // create_uniform_wf (:636-850), create_uniform_wf_2d_generic (:2044-2287),
 // create_uniform_wf_1d_generic (:2321-2560) and the 1D transverse-mode table
 // of wslda_wavevectors.h:21-80.
@@ -11,6 +13,7 @@
 #include <string.h>
 #include <vector>
 
+#include "tdw_aslda.hpp"
 #include "wslda_td.h"
 
 typedef std::complex<double> zc;
@@ -105,7 +108,8 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
                                  double Tstop, double dT, double eps, double scmix, double muchange, int maxiter,
                                  int mu_rule, tdw_uniform *U)
 {
-    if (!c || !U || (c->functional != TDW_SLDA && c->functional != TDW_BDG)) return TDW_EINVAL;
+    if (!c || !U || (c->functional != TDW_SLDA && c->functional != TDW_ASLDA && c->functional != TDW_BDG))
+        return TDW_EINVAL;
     // BdG (solve_uniform_problem_bdg, pca_uniform.h:1105-1470): bare coupling
     // 4 pi a, no density functional in the mean fields, mu += muchange (n0 - n)
     const bool bdg = c->functional == TDW_BDG;
@@ -126,10 +130,24 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     const bool cubic = c->regularization == TDW_CUBIC_CUTOFF;
     const double R0 = cubic ? 2.442749607806335 / (4.0 * PI * PI) : 1. / (2.0 * PI * PI);
     const double R1 = cubic ? 0.31786000 : 0.5;
-    const double alph_plus = 1.0, alph_a = 1.0, alph_b = 1.0;
-    const double dtC_dna = f.dtC(n0_a, n0_b), dtC_dnb = dtC_dna;
-    const double dD_dna = f.dD(n0_a, n0_b, 0), dD_dnb = f.dD(n0_a, n0_b, 1);
-    const double tC = f.tC(n0_a, n0_b), Dn = f.D(n0_a, n0_b);
+    // ASLDA: alpha and its density derivatives at the target densities
+    // (pca_uniform.h:157-163); SLDA / BdG: alpha = 1
+    const bool as = c->functional == TDW_ASLDA;
+    const double p0r = as ? f.pol(n0_a, n0_b) : 0.0;
+    const double dpa = as ? f.dpol(n0_a, n0_b, 0) : 0.0, dpb = as ? f.dpol(n0_a, n0_b, 1) : 0.0;
+    const double alph_plus = as ? tdw::aslda::alpha_plus(p0r) : 1.0;
+    const double alph_a = as ? tdw::aslda::alpha(p0r) : 1.0, alph_b = as ? tdw::aslda::alpha(-p0r) : 1.0;
+    const double dalphm_dna = as ? tdw::aslda::dalpha_minus_dp(p0r) * dpa : 0.0;
+    const double dalphm_dnb = as ? tdw::aslda::dalpha_minus_dp(p0r) * dpb : 0.0;
+    const double dalphp_dna = as ? tdw::aslda::dalpha_plus_dp(p0r) * dpa : 0.0;
+    const double dalphp_dnb = as ? tdw::aslda::dalpha_plus_dp(p0r) * dpb : 0.0;
+    const double n0 = n0_a + n0_b;
+    const double dtC_dna = as ? tdw::aslda::der_tildeC(n0, p0r, dpa, c->denseps) : f.dtC(n0_a, n0_b);
+    const double dtC_dnb = as ? tdw::aslda::der_tildeC(n0, p0r, dpb, c->denseps) : dtC_dna;
+    const double dD_dna = as ? tdw::aslda::der_funD(n0, p0r, dpa) : f.dD(n0_a, n0_b, 0);
+    const double dD_dnb = as ? tdw::aslda::der_funD(n0, p0r, dpb) : f.dD(n0_a, n0_b, 1);
+    const double tC = as ? tdw::aslda::tildeC(n0, p0r, c->denseps) : f.tC(n0_a, n0_b);
+    const double Dn = as ? tdw::aslda::funD(n0, p0r) : f.D(n0_a, n0_b);
     double eF_a = pow(6.0 * PI * PI * n0_a, 2.0 / 3.0) / 2.0;
     double eF_b = pow(6.0 * PI * PI * n0_b, 2.0 / 3.0) / 2.0;
     double eF_avg = pow(3.0 * PI * PI * (n0_a + n0_b), 2.0 / 3.0) / 2.0;
@@ -152,8 +170,16 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
         for (iter = 0; iter < maxiter; iter++) {
             double tau_a_old = tau_a, tau_b_old = tau_b, delta_old = delta, nu_old = nu;
             double V_a_old = V_a, V_b_old = V_b;
-            V_a = bdg ? 0.0 : -dtC_dna * delta * delta / alph_plus + dD_dna;
-            V_b = bdg ? 0.0 : -dtC_dnb * delta * delta / alph_plus + dD_dnb;
+            if (as) {
+                const double tau_p = tau_a + tau_b, tau_m = tau_a - tau_b;
+                V_a = dalphm_dna * tau_m / 2.0 + dalphp_dna * (tau_p / 2.0 - delta * nu / alph_plus) -
+                      dtC_dna * delta * delta / alph_plus + dD_dna;
+                V_b = dalphm_dnb * tau_m / 2.0 + dalphp_dnb * (tau_p / 2.0 - delta * nu / alph_plus) -
+                      dtC_dnb * delta * delta / alph_plus + dD_dnb;
+            } else {
+                V_a = bdg ? 0.0 : -dtC_dna * delta * delta / alph_plus + dD_dna;
+                V_b = bdg ? 0.0 : -dtC_dnb * delta * delta / alph_plus + dD_dnb;
+            }
             double mu_p = (mu_a - V_a_old + mu_b - V_b_old) / 2.0;
             zc p0 = std::sqrt(zc(2.0 * mu_p / alph_plus, 0.0));
             if (p0.imag() < 0.) p0 *= -1.;

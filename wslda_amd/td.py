@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libwslda_td.so")
 
-SLDA, BDG = 111, 113
+SLDA, ASLDA, BDG = 111, 112, 113
 SPHERICAL_CUTOFF, CUBIC_CUTOFF = 88, 89
 AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
 FP64, SPLIT = 0, 1
@@ -145,6 +145,8 @@ def _sigs():
         "tdw_density_tau": (I, [P, P]),
         "tdw_potentials": (I, [P, I, P, P]),
         "tdw_apply_h": (I, [P, I, I, P, P, P, P, P, P]),
+        "tdw_apply_h_full": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P]),
+        "tdw_alpha_wf": (I, [P, I, P, P, P]),
         "tdw_energy": (I, [P, I, P, P, P]),
         "tdw_vext_eval": (I, [P, I, P, P]),
         "tdw_step": (I, [P, C.POINTER(State), I, I]),
@@ -201,6 +203,7 @@ def _sigs():
         "apply_hamiltonian": (I, [I, I, P, P, P, P, P, P, P, P, P, D, P, D, I]),
         "apply_hamiltonian_1d": (I, [I, I, P, P, P, P, P, P, P, P, D, P, D, I]),
         "apply_hamiltonian_2d": (I, [I, I, P, P, P, P, P, P, P, P, P, D, P, D, I]),
+        "multiply_wf_by_alpha": (I, [I, P, P, P, I]),
         "normalize_wf": (I, [I, P, I]),
         "abm_step1": (I, [I, P, P, P, P, P, P, I, I]),
         "abm_step4": (I, [I, P, P, P, I, I]),
@@ -336,6 +339,19 @@ class TDContext:
         check(lib().tdw_apply_h(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(lap),
                                 _ptr(pots), _ptr(useqpe), _ptr(qpe)), "apply_hamiltonian")
 
+    def apply_hamiltonian_full(self, it, wf_in, wf_out, ddx, ddy, ddz, lap, alap, pots, useqpe=None, qpe=None):
+        """H application with the gradients and lap(alpha psi) (alap None:
+        formed by the library); the form ASLDA needs."""
+        self._pre()
+        check(lib().tdw_apply_h_full(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(ddx),
+                                     _ptr(ddy), _ptr(ddz), _ptr(lap), _ptr(alap), _ptr(pots), _ptr(useqpe),
+                                     _ptr(qpe)), "apply_hamiltonian_full")
+
+    def multiply_wf_by_alpha(self, wf_in, wf_out, pots):
+        self._pre()
+        check(lib().tdw_alpha_wf(self.h, self.nwf_of(wf_in), _ptr(wf_in), _ptr(pots), _ptr(wf_out)),
+              "multiply_wf_by_alpha")
+
     def compute_energy(self, it, dens, pots):
         self._pre()
         e = np.zeros(NITEMS)
@@ -380,7 +396,8 @@ class TDContext:
 
 def uniform_solve(params, n0_a, n0_b, kc, T, eps=1e-9, scmix=0.5, muchange=0.5, maxiter=10000,
                   Tstart=None, dT=0.01, mu_rule=0):
-    """solve_uniform_problem of pca_uniform.h (SLDA); returns (Uniform, rc)."""
+    """solve_uniform_problem of pca_uniform.h (SLDA, ASLDA; the BdG twin for
+    BDG); returns (Uniform, rc)."""
     u = Uniform()
     rc = lib().tdw_uniform_solve(C.byref(params), n0_a, n0_b, kc, T if Tstart is None else Tstart, T, dT,
                                  eps, scmix, muchange, maxiter, mu_rule, C.byref(u))
