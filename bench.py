@@ -82,6 +82,18 @@ def design_units(name, scheme, split=False):
     }.get(name, 0)
 
 
+def modular_units(name):
+    """The ASLDA step's kernels (3D, FP64): each kernel's own compulsory
+    reads and writes of state-sized arrays, in S per state and launch (its
+    algorithmic bytes given the reference's kernel boundaries).  Per step
+    (AB5AM5): abm_step1 8, abm_step4 3, and twice: norms 1 + scale 2,
+    derivatives 3 + 5, densities 5, alpha_wf 2, lap(alpha psi) 2 + 3,
+    apply_h 7 + the <H> subtraction 3 -- 77 S against the 13 S compulsory
+    for a whole step."""
+    return {"abm_step1": 8, "abm_step4": 3, "norms": 1, "scale": 2, "fft_outer_deriv": 3, "fft_inner_deriv": 5,
+            "densities": 5, "alpha_wf": 2, "fft_outer": 2, "fft_inner": 3, "apply_h": 10}.get(name, 0)
+
+
 def step_units(scheme, split=False):
     if split:
         return 0.5 * nhist(scheme) + 6.5   # 8, 8.5, 9 S
@@ -125,7 +137,8 @@ def run_gpu(args, prec):
 
     dev = "cuda:0"
     t0 = time.time()
-    w = problems.config2(torch, precision=prec, n_states=args.states, device=dev)
+    w = problems.config2(torch, precision=prec, n_states=args.states, device=dev,
+                         functional=td.ASLDA if args.aslda else td.SLDA)
     p = w["params"]
     n = w["wf"].shape[0] // 2
     ctx = td.TDContext(p, n, 0)
@@ -178,9 +191,13 @@ def run_gpu(args, prec):
     dname, (dtot, dcnt) = dom
     davg = dtot / dcnt / 1000.0
     S = 2 * N * 16
-    dbytes = algorithmic_units(dname, p.scheme, prec == td.SPLIT) * S * n
+    if args.aslda:
+        dbytes = modular_units(dname) * S * n
+        moved = dbytes
+    else:
+        dbytes = algorithmic_units(dname, p.scheme, prec == td.SPLIT) * S * n
+        moved = design_units(dname, p.scheme, prec == td.SPLIT) * S * n
     achieved = dbytes / davg / 1e9 if dbytes else None
-    moved = design_units(dname, p.scheme, prec == td.SPLIT) * S * n
     step_bytes = step_units(p.scheme, prec == td.SPLIT) * S * n
     out = dict(qps=qps, ms=ms, wall=wall, n=n, N=N, rep=rep, dname=dname, davg=davg, dbytes=dbytes,
                achieved=achieved, moved=moved, step_bytes=step_bytes, setup_s=setup_s, scheme=p.scheme,
@@ -405,6 +422,35 @@ def run_trap(args):
     }
 
 
+def run_aslda(args):
+    """ASLDA (density-dependent effective mass, current corrections) on the
+    3D uniform workload: the modular step with gradients on both half-steps,
+    FP64, its dominant kernel against the HBM roofline, and the oracle's
+    ASLDA step on a sample of the same states."""
+    from wslda_amd import td
+    if args.reserve_gb > 0:
+        reserve_release(args.reserve_gb)
+    g = run_gpu(args, td.FP64)
+    head = summarize(g, args, "fp64")
+    roof = head["roofline"]
+    roof["unit_note"] = ("achieved = the kernel's own compulsory state-array reads and writes per launch "
+                         "/ average launch time")
+    res = {
+        "metric": "qp-wavefunction steps/s, ASLDA (gradients every half-step), FP64",
+        "value": head["value"], "unit": "qp-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "td-ASLDA 3D uniform, 24x24x24 lattice, %d qp states, AB5AM5, plane-wave BdG "
+                               "+ seeded 1e-6 noise, modular step" % g["n"]},
+        "roofline": roof, "kernels_ms_per_step": head["kernels_ms_per_step"],
+        "energy_drift": head["energy_drift"],
+        "step_state_traffic_S": 77, "step_compulsory_S": step_units(g["scheme"]),
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(g, args)
+    g["ctx"].close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -441,6 +487,8 @@ def main():
     ap.add_argument("--trap", action="store_true",
                     help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
     ap.add_argument("--trap-states", type=int, default=24576)
+    ap.add_argument("--aslda", action="store_true",
+                    help="instead of the step benchmark: the ASLDA step on the 3D uniform workload (24^3)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
@@ -451,6 +499,9 @@ def main():
         return
     if args.trap:
         print(json.dumps(run_trap(args)))
+        return
+    if args.aslda:
+        print(json.dumps(run_aslda(args)))
         return
     from wslda_amd import td
     primary = "split" if args.mode == "split" else "fp64"

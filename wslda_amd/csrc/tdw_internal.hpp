@@ -80,6 +80,15 @@ struct TimerScope {
     TimerScope(tdw_ctx *cc, const char *name) : c(cc), idx(-1) { if (c->timing) idx = timer_begin(c, name); }
     ~TimerScope() { if (idx >= 0) timer_end(c, idx); }
 };
+// Suspends the per-kernel records inside a timed helper, so that the
+// grid-sized transforms it launches are not averaged with the state-sized
+// launches of the same kernels.
+struct TimingOff {
+    tdw_ctx *c;
+    int saved;
+    explicit TimingOff(tdw_ctx *cc) : c(cc), saved(cc->timing) { c->timing = 0; }
+    ~TimingOff() { c->timing = saved; }
+};
 #define TDW_CAT2(a, b) a##b
 #define TDW_CAT(a, b) TDW_CAT2(a, b)
 #define TDW_TIMED(ctx, name) tdw::TimerScope TDW_CAT(_tdw_ts_, __LINE__)(ctx, name)

@@ -340,6 +340,7 @@ int laplace_real(tdw_ctx *c, const double *f, double *lap)
 int density_tau(tdw_ctx *c, double *dens)
 {
     TDW_TIMED(c, "tau");
+    TimingOff quiet(c);
     long N = (long)c->N;
     TDW_TRY(ensure_T(c, (size_t)N));
     // b component first, then a (pca_dens.cu:317-333)
@@ -634,6 +635,8 @@ static int ensure_aslda(tdw_ctx *c)
 int aslda_fields(tdw_ctx *c, const double *pots)
 {
     TDW_TRY(ensure_aslda(c));
+    TDW_TIMED(c, "aslda_fields");
+    TimingOff quiet(c);
     const long N = (long)c->N;
     TDW_TRY(laplace_real(c, pots + 4 * N, c->d_hx));
     TDW_TRY(laplace_real(c, pots + 5 * N, c->d_hx + N));

@@ -91,14 +91,16 @@ def states_on_gpu(torch, params, K, U, V, device="cuda:0", chunk=2048, envelope=
     return wf
 
 
-def _base_params(dim, nx, ny, nz, dx, scheme=td.AB5AM5, precision=td.FP64, vext=None, dt_eF=0.005):
+def _base_params(dim, nx, ny, nz, dx, scheme=td.AB5AM5, precision=td.FP64, vext=None, dt_eF=0.005,
+                 functional=td.SLDA):
     eF = 0.5
-    return td.make_params(dim, nx, ny, nz, dx=dx, functional=td.SLDA, regularization=td.SPHERICAL_CUTOFF,
+    return td.make_params(dim, nx, ny, nz, dx=dx, functional=functional, regularization=td.SPHERICAL_CUTOFF,
                           mf_correction=1, scheme=scheme, dt=dt_eF / eF, vext=vext, precision=precision)
 
 
 def uniform_slda(params, T=1.0e-9):
-    """Self-consistent uniform SLDA state at kF = 1 (n_a = n_b = n/2)."""
+    """Self-consistent uniform SLDA (or ASLDA, by params.functional) state at
+    kF = 1 (n_a = n_b = n/2)."""
     n0 = 1.0 / (3.0 * PI ** 2)
     kc = 0.999999 * PI / max(params.dx, params.dy, params.dz)
     u, rc = td.uniform_solve(params, n0 / 2, n0 / 2, kc, T, eps=1e-9, scmix=0.5, muchange=0.5, maxiter=10000)
@@ -108,16 +110,18 @@ def uniform_slda(params, T=1.0e-9):
 
 
 def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, device="cuda:0", scheme=td.AB5AM5,
-            shape=(24, 24, 24), dx=1.0):
+            shape=(24, 24, 24), dx=1.0, functional=td.SLDA):
     """Plain 3D uniform SLDA: 24^3 periodic lattice, the n_states lowest-|E|
     plane-wave BdG states of the self-consistent uniform solution (Delta of
-    the solution ~ 0.5 eF), seeded random-phase noise of amplitude 1e-6."""
+    the solution ~ 0.5 eF), seeded random-phase noise of amplitude 1e-6.
+    functional=ASLDA: the same with the ASLDA solution (effective mass
+    alpha = 1.094 at n_a = n_b)."""
     nx, ny, nz = shape
-    p = _base_params(3, nx, ny, nz, dx, scheme=scheme, precision=precision)
+    p = _base_params(3, nx, ny, nz, dx, scheme=scheme, precision=precision, functional=functional)
     u = uniform_slda(p)
     # all plane-wave states of the lattice, the n_states of lowest |E| (a few
     # may lie above the solution's cutoff when n_states exceeds its count)
-    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf)
+    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf, alph=u.alph_a)
     order = np.argsort(np.abs(E), kind="stable")[:n_states]
     order.sort()   # keep create_uniform_wf's order among the selected states
     K, E, U, V = K[order], E[order], U[order], V[order]
@@ -135,11 +139,13 @@ def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, devi
     pots[:N] = u.V_a
     pots[N:2 * N] = u.V_b
     pots[2 * N:4 * N:2] = u.delta
-    pots[4 * N:6 * N] = 1.0
+    pots[4 * N:5 * N] = u.alph_a
+    pots[5 * N:6 * N] = u.alph_b
     fb = np.array([td.fbeta(e, u.beta) for e in E])
     eF = 0.5
     return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
-                name="3D uniform SLDA %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
+                name="3D uniform %s %dx%dx%d, %d states" % ("ASLDA" if functional == td.ASLDA else "SLDA", nx, ny,
+                                                            nz, len(E)))
 
 
 def select_states(wf, En, kt, cnt, n_states, dim):
