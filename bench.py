@@ -85,13 +85,16 @@ def design_units(name, scheme, split=False):
 def modular_units(name):
     """The ASLDA step's kernels (3D, FP64): each kernel's own compulsory
     reads and writes of state-sized arrays, in S per state and launch (its
-    algorithmic bytes given the reference's kernel boundaries).  Per step
-    (AB5AM5): abm_step1 8, abm_step4 3, and twice: norms 1 + scale 2,
+    algorithmic bytes at its boundaries).  On lattices with the two-pass
+    kernels a step (AB5AM5) is abm_step1 8, abm_step4 3, and twice: norms 1
+    + scale 2, the density x and yz passes 1 + 1 (gradients kept in LDS),
+    the H x and yz passes 2 + 3, the <H> subtraction 3 -- 37 S against the
+    13 S compulsory for a whole step.  The modular kernels of other lattices:
     derivatives 3 + 5, densities 5, alpha_wf 2, lap(alpha psi) 2 + 3,
-    apply_h 7 + the <H> subtraction 3 -- 77 S against the 13 S compulsory
-    for a whole step."""
-    return {"abm_step1": 8, "abm_step4": 3, "norms": 1, "scale": 2, "fft_outer_deriv": 3, "fft_inner_deriv": 5,
-            "densities": 5, "alpha_wf": 2, "fft_outer": 2, "fft_inner": 3, "apply_h": 10}.get(name, 0)
+    apply_h 7 + 3."""
+    return {"abm_step1": 8, "abm_step4": 3, "norms": 1, "scale": 2, "dens_gx": 1, "dens_gyz": 1, "ah_x": 2,
+            "ah_yz": 3, "sub_qpe": 3, "fft_outer_deriv": 3, "fft_inner_deriv": 5, "densities": 5, "alpha_wf": 2,
+            "fft_outer": 2, "fft_inner": 3, "apply_h": 10}.get(name, 0)
 
 
 def step_units(scheme, split=False):
@@ -443,7 +446,8 @@ def run_aslda(args):
                                "+ seeded 1e-6 noise, modular step" % g["n"]},
         "roofline": roof, "kernels_ms_per_step": head["kernels_ms_per_step"],
         "energy_drift": head["energy_drift"],
-        "step_state_traffic_S": 77, "step_compulsory_S": step_units(g["scheme"]),
+        "step_state_traffic_S": sum(modular_units(k) * v[1] for k, v in g["rep"].items()) / args.steps,
+        "step_compulsory_S": step_units(g["scheme"]),
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(g, args)

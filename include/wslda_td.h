@@ -132,6 +132,10 @@ int tdw_laplace(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *out);
 int tdw_derivatives(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *ddx,
                     tdw_complex *ddy, tdw_complex *ddz, tdw_complex *lap);
 int tdw_laplace_real(tdw_ctx *c, const double *f, double *lap);
+/* calculate_densities (pca_dens.cu:220).  With gradients = 1 and all four
+ * derivative arrays NULL, the gradients are formed inside the density passes
+ * and not stored (3D lattices with the two-pass kernels; TDW_EINVAL
+ * elsewhere). */
 int tdw_densities(tdw_ctx *c, int n, const tdw_complex *wf, const tdw_complex *ddx,
                   const tdw_complex *ddy, const tdw_complex *ddz, const tdw_complex *lap,
                   const double *weights /* n or NULL */, double *dens, int gradients);
@@ -162,7 +166,9 @@ int tdw_vext_eval(tdw_ctx *c, int it, double *va, double *vb);
  * does.  grad_last: full derivatives (tau, j) on this step's corrector.
  * ASLDA computes the derivatives on both half-steps (the reference's path
  * without FAST_CONST_EFFECTIVE_MASS_MODE), so ddx..ddz of the evolved axes
- * must be set; grad_last changes nothing there. */
+ * must be set; grad_last changes nothing there.  On 3D lattices with the
+ * two-pass kernels the ASLDA step forms the gradients inside its density
+ * and H passes and leaves ddx..ddz untouched. */
 typedef struct {
     int n;
     tdw_complex *wf, *lap, *ddx, *ddy, *ddz;

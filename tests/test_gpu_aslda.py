@@ -218,3 +218,102 @@ def test_aslda_uses_modular_step(torch):
     R3.initial_measure()
     assert td.lib().tdw_step_fused(ctx3.h, C.byref(R3.st), 0, 0) == td.TDW_EINVAL
     ctx3.close()
+
+
+@pytest.mark.parametrize("shape", [(12, 12, 12), (16, 16, 16), (24, 24, 24), (32, 32, 32)])
+def test_aslda_h_two_pass_equals_modular(torch, shape):
+    """The two-pass 3D ASLDA H (x-pencil and yz-plane passes from psi, alpha
+    and A) equals the modular form (alpha psi, its batched Laplacian and the
+    pointwise H kernel over the stored gradients) on random states and
+    smooth random alpha, A fields, and both equal the oracle."""
+    from wslda_amd import td
+    say("case: ASLDA two-pass H vs modular, shape %s" % (shape,))
+    nx, ny, nz = shape
+    cfg = po.make_cfg(3, nx, ny, nz, dx=1.1, functional=po.ASLDA)
+    cfg.mu_a, cfg.mu_b = 0.3, 0.25
+    N = po.npts(cfg)
+    n = 16
+    rng = np.random.default_rng(sum(shape))
+    wf = (rng.standard_normal((2 * n, N)) + 1j * rng.standard_normal((2 * n, N))) * 0.05
+    X, Y, Z = np.meshgrid(*[np.arange(k) * 2 * np.pi / k for k in shape], indexing="ij")
+    smooth = lambda p: (np.sin(X + p) * np.cos(2 * Y - p) + 0.5 * np.cos(Z + 2 * p)).ravel()
+    pots = np.zeros(12 * N)
+    pots[:N] = 0.1 * smooth(0.3)
+    pots[N:2 * N] = 0.1 * smooth(0.7)
+    pots[2 * N:4 * N:2] = 0.2 + 0.05 * smooth(1.1)
+    pots[2 * N + 1:4 * N:2] = 0.03 * smooth(1.9)
+    pots[4 * N:5 * N] = 1.0 + 0.1 * smooth(2.3)
+    pots[5 * N:6 * N] = 1.0 + 0.1 * smooth(2.9)
+    for d in range(3):
+        pots[(6 + d) * N:(7 + d) * N] = 0.05 * smooth(3.1 + d)
+        pots[(9 + d) * N:(10 + d) * N] = 0.05 * smooth(4.3 + d)
+    P = po.Problem(cfg, wf.copy(), np.ones(n), pots=pots)
+    po.lib().tdwo_derivatives(C.byref(cfg), 2 * n, po.ptr(P.wf), po.ptr(P.ddx), po.ptr(P.ddy), po.ptr(P.ddz),
+                              po.ptr(P.lap))
+    alap_ref = np.zeros_like(P.wf)
+    po.lib().tdwo_alpha_wf(C.byref(cfg), n, po.ptr(P.wf), po.ptr(P.pots), po.ptr(alap_ref))
+    po.lib().tdwo_laplace(C.byref(cfg), 2 * n, po.ptr(alap_ref), po.ptr(alap_ref))
+    out_ref = np.zeros_like(P.wf)
+    q_ref = np.zeros(n)
+    po.lib().tdwo_apply_h_full(C.byref(cfg), n, po.ptr(P.wf), po.ptr(out_ref), po.ptr(P.ddx), po.ptr(P.ddy),
+                               po.ptr(P.ddz), po.ptr(P.lap), po.ptr(alap_ref), po.ptr(P.pots), None, None,
+                               po.ptr(q_ref))
+    ctx = td.TDContext(tdgauss.to_params(cfg), n, 0)
+    W = torch.as_tensor(wf).to("cuda:0")
+    Pt = torch.as_tensor(pots).to("cuda:0")
+    D = [torch.zeros_like(W) for _ in range(4)]
+    ctx.compute_derivatives(W, D[0], D[1], D[2], D[3])
+    aw = torch.zeros_like(W)
+    ctx.multiply_wf_by_alpha(W, aw, Pt)
+    ctx.compute_laplace(aw, aw)
+    outs = []
+    for alap in (aw, None):
+        out = torch.zeros_like(W)
+        q = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+        ctx.apply_hamiltonian_full(0, W, out, D[0], D[1], D[2], D[3], alap, Pt, qpe=q)
+        ctx.sync()
+        outs.append(out.cpu().numpy())
+        assert rel(q.cpu().numpy(), q_ref) < TOL
+        assert rel(out.cpu().numpy(), out_ref) < TOL
+    assert rel(outs[1], outs[0]) < TOL
+    ctx.close()
+
+
+@pytest.mark.parametrize("shape,n", [((12, 12, 12), 40), ((16, 16, 16), 24), ((24, 24, 24), 70), ((32, 32, 32), 8)])
+def test_densities_gradients_in_passes(torch, shape, n):
+    """Densities with gradients formed inside the x / yz density passes
+    (tdw_densities with no derivative arrays) equal the oracle's, and the
+    modular form over stored gradients, at 1e-12."""
+    from wslda_amd import td
+    say("case: densities with in-pass gradients, shape %s, %d states" % (shape, n))
+    nx, ny, nz = shape
+    cfg = po.make_cfg(3, nx, ny, nz, dx=0.9)
+    N = po.npts(cfg)
+    rng = np.random.default_rng(7 + nx)
+    X = np.arange(N) // (N // nx)
+    wf = (rng.standard_normal((2 * n, N)) + 1j * rng.standard_normal((2 * n, N))) * 0.05
+    wf *= np.exp(1j * 2 * np.pi * X / nx)[None, :]   # currents along x as well
+    fb = rng.uniform(0.0, 1.0, n)
+    P = po.Problem(cfg, wf.copy(), fb)
+    po.lib().tdwo_derivatives(C.byref(cfg), 2 * n, po.ptr(P.wf), po.ptr(P.ddx), po.ptr(P.ddy), po.ptr(P.ddz),
+                              po.ptr(P.lap))
+    dref = np.zeros(12 * N)
+    po.lib().tdwo_densities(C.byref(cfg), n, po.ptr(P.wf), po.ptr(P.ddx), po.ptr(P.ddy), po.ptr(P.ddz),
+                            po.ptr(P.lap), po.ptr(P.fbeta), None, None, po.ptr(dref), 1)
+    ctx = td.TDContext(tdgauss.to_params(cfg), n, 0)
+    ctx.set_states(fb, None, None)
+    W = torch.as_tensor(wf).to("cuda:0")
+    d1 = torch.zeros(12 * N, dtype=torch.float64, device="cuda:0")
+    ctx.calculate_densities(W, None, None, None, None, d1, 1)
+    D = [torch.zeros_like(W) for _ in range(4)]
+    ctx.compute_derivatives(W, D[0], D[1], D[2], D[3])
+    d2 = torch.zeros_like(d1)
+    ctx.calculate_densities(W, D[0], D[1], D[2], D[3], d2, 1)
+    ctx.sync()
+    a, b = d1.cpu().numpy(), d2.cpu().numpy()
+    for k, name in [(slice(0, 2 * N), "nu"), (slice(2 * N, 3 * N), "rho_a"), (slice(3 * N, 4 * N), "tau_acc_a"),
+                    (slice(4 * N, 7 * N), "j_a"), (slice(7 * N, 8 * N), "rho_b"), (slice(8 * N, 9 * N), "tau_acc_b"),
+                    (slice(9 * N, 12 * N), "j_b")]:
+        assert rel(a[k], dref[k]) < TOL, name
+        assert rel(a[k], b[k]) < TOL, name
+    ctx.close()

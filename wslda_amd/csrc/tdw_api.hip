@@ -346,6 +346,11 @@ extern "C" int tdw_densities(tdw_ctx *c, int n, const tdw_complex *wf, const tdw
                              const double *weights, double *dens, int gradients)
 {
     if (!c || n > c->nwf) return TDW_EINVAL;
+    if (gradients && !ddx && !ddy && !ddz && !lap) {
+        // gradients formed inside the density passes
+        if (!dens_grad_fused_supported(c)) return TDW_EINVAL;
+        return dens_grad_fused(c, n, CD2(wf), c->d_fbeta, weights, dens);
+    }
     return densities(c, n, CD2(wf), CD2(ddx), CD2(ddy), CD2(ddz), CD2(lap), c->d_fbeta, weights, c->d_kt,
                      c->d_cnt, dens, gradients);
 }
@@ -421,20 +426,31 @@ static int nhist(const tdw_ctx *c)
 // :1510-1535): every half-step computes the derivatives, tau and j, the
 // potentials with the effective masses and vector potentials, lap(alpha psi)
 // and the full Hamiltonian.
+// 3D lattices with the two-pass kernels form the gradients inside the
+// density and H passes (tdw_aslda.hip) and store none of them.
+static int aslda_densities(tdw_ctx *c, tdw_state *s)
+{
+    if (aslda_h_fused_supported(c) && dens_grad_fused_supported(c)) {
+        TDW_TRY(dens_grad_fused(c, s->n, D2(s->wf), c->d_fbeta, nullptr, s->dens));
+        return density_tau(c, s->dens);
+    }
+    return half_densities(c, s, 1);
+}
+
 static int step_aslda(tdw_ctx *c, tdw_state *s, int it, int k, double *qpe)
 {
     const int n = s->n;
     // predictor
     TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
-    TDW_TRY(half_densities(c, s, 1));
+    TDW_TRY(aslda_densities(c, s));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(apply_h_aslda(c, n, D2(s->wf), D2(s->lap), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
                           s->pots, c->d_kt, nullptr, qpe));
     // corrector
     TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
-    TDW_TRY(half_densities(c, s, 1));
+    TDW_TRY(aslda_densities(c, s));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
     tdw_complex *tmp = s->f[k - 1];
     for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];

@@ -1,0 +1,687 @@
+// tdw_aslda.hip -- the ASLDA Hamiltonian on 3D lattices in two passes.
+//
+// H of the current-correction path (pca_kernels.cu:233-391 without
+// FAST_CONST_EFFECTIVE_MASS_MODE, with CURRENT_CORRECTIONS):
+//   (H psi)_u = (V_a - mu_a - i/2 div A_a + lap(alpha_a)/4) u + Delta v
+//               + sum_d K_d,   K_d = -(alpha_a d_d^2 u + d_d^2(alpha_a u))/4 - i A_a,d d_d u
+//   (H psi)_v = -(V_b - mu_b + i/2 div A_b + lap(alpha_b)/4) v + conj(Delta) u
+//               + sum_d K'_d,  K'_d = +(alpha_b d_d^2 v + d_d^2(alpha_b v))/4 - i A_b,d d_d v
+// Every K_d needs only spectral derivatives along axis d of psi and of
+// alpha psi, so the x terms are formed in an x-pencil pass (blocks of
+// pencils, in-place two-factor FFT in LDS) and stored in T, and the y, z
+// terms with the pointwise terms in a yz-plane pass (the same FFT) that
+// reads psi and T and writes H psi with the <psi|H psi> partials.  This
+// replaces alpha*psi, the batched lap(alpha psi) through a scratch array,
+// and the H kernel that reads psi, its three gradients, its Laplacian and
+// lap(alpha psi): 5 S per state instead of 14 S.  The gradient of psi along
+// an axis has its Nyquist mode set to zero, as the reference's derivatives
+// do (pca_derivative.cu:169-171).
+#include "tdw_fft.hpp"
+#include "tdw_physics.hpp"
+#include "tdw_regfft.hpp"
+
+namespace tdw {
+
+static constexpr int AT = 256;
+
+// ---------------------------------------------------------------------------
+// In-place derivatives along one axis of P pencils held in LDS (L = R1*R2,
+// n = n1*R2 + n2, frequency f = k1 + R1*k2 at position k1*R2 + n2 between
+// the stages).  A receives the second derivative (factor -k2s m^2, the
+// inverse transform's 1/L folded into k2s); with GRAD, B (same layout)
+// receives the first derivative (factor i k1s m, Nyquist mode zero).  Pencil q starts at
+// (q / QD) * QH + (q % QD) * QL, element stride IS.
+// ---------------------------------------------------------------------------
+template <int L, int P, int QD, int QH, int QL, int IS, int NT, bool GRAD>
+__device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *tw, int tid, double k1s, double k2s)
+{
+    constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
+    static_assert(R2 > 1, "axis_d12 needs L = R1 * R2 with R2 > 1");
+    // stage 1: R1-point DFTs over n1, twiddle W_L^{n2 k1}
+    for (int b = tid; b < P * R2; b += NT) {
+        const int q = b / R2, n2 = b - q * R2;
+        double2 *pa = A + (q / QD) * QH + (q % QD) * QL;
+        double2 v[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) v[n1] = pa[(n1 * R2 + n2) * IS];
+        rf::small_dft<R1, -1, double2>::run(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; k1++) v[k1] = cmul(v[k1], tw[(n2 * k1) % L]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) pa[(k1 * R2 + n2) * IS] = v[k1];
+    }
+    __syncthreads();
+    // stage 2: R2-point DFTs over n2, the derivative factors, inverse R2 DFTs
+    for (int b = tid; b < P * R1; b += NT) {
+        const int q = b / R1, k1 = b - q * R1;
+        const int off = (q / QD) * QH + (q % QD) * QL;
+        double2 *pa = A + off;
+        double2 v[R2], g[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) v[n2] = pa[(k1 * R2 + n2) * IS];
+        rf::small_dft<R2, -1, double2>::run(v);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) {
+            const int f = k1 + R1 * k2;
+            const int m = (f < L / 2) ? f : f - L;
+            if constexpr (GRAD) g[k2] = (f == L / 2) ? make_double2(0.0, 0.0) : imul(v[k2], k1s * (double)m);
+            v[k2] = v[k2] * (-k2s * (double)(m * m));
+        }
+        rf::small_dft<R2, +1, double2>::run(v);
+        if constexpr (GRAD) rf::small_dft<R2, +1, double2>::run(g);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) {
+            double2 wc = tw[(n2 * k1) % L];
+            wc.y = -wc.y;
+            pa[(k1 * R2 + n2) * IS] = n2 == 0 ? v[n2] : cmul(v[n2], wc);
+            if constexpr (GRAD) B[off + (k1 * R2 + n2) * IS] = n2 == 0 ? g[n2] : cmul(g[n2], wc);
+        }
+    }
+    __syncthreads();
+    // stage 3: inverse R1-point DFTs over k1
+    for (int b = tid; b < P * R2; b += NT) {
+        const int q = b / R2, n2 = b - q * R2;
+        const int off = (q / QD) * QH + (q % QD) * QL;
+        double2 v[R1];
+        double2 *pa = A + off;
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) v[k1] = pa[(k1 * R2 + n2) * IS];
+        rf::small_dft<R1, +1, double2>::run(v);
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) pa[(n1 * R2 + n2) * IS] = v[n1];
+        if constexpr (GRAD) {
+            double2 *pb = B + off;
+#pragma unroll
+            for (int k1 = 0; k1 < R1; k1++) v[k1] = pb[(k1 * R2 + n2) * IS];
+            rf::small_dft<R1, +1, double2>::run(v);
+#pragma unroll
+            for (int n1 = 0; n1 < R1; n1++) pb[(n1 * R2 + n2) * IS] = v[n1];
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// x pass: T = K_x (u) / K'_x (v).  One workgroup per (state component, block
+// of PX consecutive x pencils); the block sits in LDS as A[q + i*PX] (pencil
+// q, point i), so the loads and stores of consecutive lanes are consecutive
+// in memory.  k1s = 2 pi / (Lx * LX), k2s = (2 pi / Lx)^2 / LX.
+// ---------------------------------------------------------------------------
+template <int LX, int PX>
+__global__ void __launch_bounds__(AT)
+ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__restrict__ pots, int n, long N, int S,
+     const double2 *__restrict__ twx_g, double k1s, double k2s)
+{
+    constexpr int E = LX * PX;
+    constexpr int KM = (E + AT - 1) / AT;
+    __shared__ double2 twx[LX];
+    __shared__ double2 A[E], B[E];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < LX; k += AT) twx[k] = twx_g[k];
+    const int nb = S / PX;
+    const long a2 = blockIdx.x / nb;
+    const int jb = (int)(blockIdx.x - a2 * nb);
+    const bool vcomp = a2 >= n;
+    const long base = a2 * N + (long)jb * PX;
+    const long fb = (long)jb * PX;
+    const double *alpha = pots + (vcomp ? 5 : 4) * N;
+    const double *Ax = pots + (vcomp ? 9 : 6) * N;
+    const double sg = vcomp ? 0.25 : -0.25;
+    double2 X[KM], acc[KM];
+    double al[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * AT;
+        if (e < E) {
+            const int i = e / PX, q = e - i * PX;
+            const long off = (long)i * S + q;
+            X[k] = y[base + off];
+            al[k] = alpha[fb + off];
+            A[e] = X[k];
+        }
+    }
+    __syncthreads();
+    axis_d12<LX, PX, PX, 0, 1, PX, AT, true>(A, B, twx, tid, k1s, k2s);
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * AT;
+        if (e < E) {
+            const int i = e / PX, q = e - i * PX;
+            acc[k] = A[e] * (sg * al[k]) + imul(B[e], -Ax[fb + (long)i * S + q]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * AT;
+        if (e < E) A[e] = X[k] * al[k];
+    }
+    __syncthreads();
+    axis_d12<LX, PX, PX, 0, 1, PX, AT, false>(A, nullptr, twx, tid, k1s, k2s);
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * AT;
+        if (e < E) {
+            const int i = e / PX, q = e - i * PX;
+            T[base + (long)i * S + q] = acc[k] + A[e] * sg;
+        }
+    }
+}
+
+constexpr int nt_ayz(int E)
+{
+    return E % 256 == 0 ? 256 : E % 192 == 0 ? 192 : E <= 64 ? 64 : E <= 128 ? 128 : E <= 192 ? 192 : 256;
+}
+
+// ---------------------------------------------------------------------------
+// yz pass: one workgroup per (state, x plane); both components of the plane
+// in LDS (A: 2E, B: 2E).  out = H psi, part[block] = Re <psi|H psi> of the
+// plane.  hx = [lap alpha_a, lap alpha_b, div A_a, div A_b].
+// ---------------------------------------------------------------------------
+template <int LY, int LZ>
+__global__ void __launch_bounds__(nt_ayz(LY *LZ))
+ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__restrict__ out,
+      const double *__restrict__ pots, const double *__restrict__ hx, int n, long N, int NX,
+      const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y, double k1z,
+      double k2z, double mu_a, double mu_b, double *__restrict__ part)
+{
+    constexpr int E = LY * LZ;
+    constexpr int NT = nt_ayz(E);
+    constexpr int KM = (E + NT - 1) / NT;
+    __shared__ double2 twy[LY], twz[LZ];
+    __shared__ double red[NT / 64];
+    extern __shared__ double2 lds_raw[];
+    double2 *A = lds_raw, *B = lds_raw + 2 * E;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
+    for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
+    const int a = blockIdx.x / NX;
+    const int x = blockIdx.x - a * NX;
+    const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
+    const long g0 = (long)x * E;
+    double2 Xu[KM], Xv[KM], Ou[KM], Ov[KM];
+    double al[KM], bl[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NT;
+        if (e < E) {
+            Xu[k] = y[gu + e];
+            Xv[k] = y[gv + e];
+            Ou[k] = T[gu + e];
+            Ov[k] = T[gv + e];
+            al[k] = pots[4 * N + g0 + e];
+            bl[k] = pots[5 * N + g0 + e];
+        }
+    }
+    // axis 0: z (pencils (c, iy): offset q*LZ, stride 1); axis 1: y (pencils
+    // (c, iz): offset (q/LZ)*E + q%LZ, stride LZ)
+#pragma unroll
+    for (int ax = 0; ax < 2; ax++) {
+        const int d = ax == 0 ? 2 : 1;   // lattice axis of the vector potential component
+        // psi: d^2 into A, d into B
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            if (e < E) {
+                A[e] = Xu[k];
+                A[E + e] = Xv[k];
+            }
+        }
+        __syncthreads();
+        if (ax == 0) axis_d12<LZ, 2 * LY, 1, LZ, 0, 1, NT, true>(A, B, twz, tid, k1z, k2z);
+        else axis_d12<LY, 2 * LZ, LZ, E, 1, LZ, NT, true>(A, B, twy, tid, k1y, k2y);
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            if (e < E) {
+                const double Aa = pots[(6 + d) * N + g0 + e], Ab = pots[(9 + d) * N + g0 + e];
+                Ou[k] = Ou[k] + A[e] * (-0.25 * al[k]) + imul(B[e], -Aa);
+                Ov[k] = Ov[k] + A[E + e] * (0.25 * bl[k]) + imul(B[E + e], -Ab);
+            }
+        }
+        // alpha psi: d^2 into A
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            if (e < E) {
+                A[e] = Xu[k] * al[k];
+                A[E + e] = Xv[k] * bl[k];
+            }
+        }
+        __syncthreads();
+        if (ax == 0) axis_d12<LZ, 2 * LY, 1, LZ, 0, 1, NT, false>(A, nullptr, twz, tid, k1z, k2z);
+        else axis_d12<LY, 2 * LZ, LZ, E, 1, LZ, NT, false>(A, nullptr, twy, tid, k1y, k2y);
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            if (e < E) {
+                Ou[k] = Ou[k] + A[e] * -0.25;
+                Ov[k] = Ov[k] + A[E + e] * 0.25;
+            }
+        }
+    }
+    double ssum = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NT;
+        if (e < E) {
+            const long gi = g0 + e;
+            const double la = 0.25 * hx[gi], lb = 0.25 * hx[N + gi];
+            const double ca = -0.5 * hx[2 * N + gi], cb = -0.5 * hx[3 * N + gi];
+            const double2 D = make_double2(pots[2 * N + 2 * gi], pots[2 * N + 2 * gi + 1]);
+            const double2 tu = Xu[k], tv = Xv[k];
+            double2 u = cmul(tu, make_double2(pots[gi] - mu_a, ca));
+            double2 v = make_double2(0.0, 0.0) - cmul(tv, make_double2(pots[N + gi] - mu_b, -1.0 * cb));
+            u = u + cmul(tv, D);
+            v = v + cmulc(D, tu);
+            u = u + tu * la;
+            v = v - tv * lb;
+            u = u + Ou[k];
+            v = v + Ov[k];
+            out[gu + e] = u;
+            out[gv + e] = v;
+            ssum += cmulc(tu, u).x + cmulc(tv, v).x;
+        }
+    }
+    ssum = block_sum(ssum, red);
+    if (tid == 0) part[blockIdx.x] = ssum;
+}
+
+// ---------------------------------------------------------------------------
+// Densities with gradients without storing the gradients: an x pass and a
+// yz pass over the state, each workgroup looping over a chunk of G states
+// and summing, per lattice point, the terms of calculate_densities
+// (pca_dens.cu:220-267: rho, nu, Re psi* lap psi for tau, Im psi* grad psi
+// for j; weights f*1e4 and (1-f)*1e4) that its axis contributes.  Partial
+// sums per chunk, then a fixed-order reduction over the chunks.
+// ---------------------------------------------------------------------------
+// x pass: both components of PX consecutive x pencils in LDS (u at A[0..E),
+// v at A[E..2E), pencil q, point i at q + i*PX).  Fields per chunk:
+// rho_a, rho_b, Re nu, Im nu, tau_a (x), tau_b (x), j_a,x, j_b,x.
+template <int LX, int PX>
+__global__ void __launch_bounds__(AT)
+dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
+     int S, int G, const double2 *__restrict__ twx_g, double k1s, double k2s, double *__restrict__ part)
+{
+    constexpr int E = LX * PX;
+    constexpr int KM = (E + AT - 1) / AT;
+    __shared__ double2 twx[LX];
+    __shared__ double2 A[2 * E], B[2 * E];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < LX; k += AT) twx[k] = twx_g[k];
+    const int nb = S / PX;
+    const int ch = blockIdx.x / nb;
+    const int jb = blockIdx.x - ch * nb;
+    const int a0 = ch * G, a1 = min(n, a0 + G);
+    const long fb0 = (long)jb * PX;
+    double na[KM], nbv[KM], nux[KM], nuy[KM], ta[KM], tb[KM], ja[KM], jbx[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) na[k] = nbv[k] = nux[k] = nuy[k] = ta[k] = tb[k] = ja[k] = jbx[k] = 0.0;
+    // the next state's pencils are loaded while this one is transformed
+    double2 Xu[KM], Xv[KM], Nu[KM], Nv[KM];
+    auto load = [&](int a, double2 *U, double2 *V) {
+        const long bu = (long)a * N + fb0, bv = ((long)n + a) * N + fb0;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * AT;
+            if (e < E) {
+                const int i = e / PX, q = e - i * PX;
+                U[k] = y[bu + (long)i * S + q];
+                V[k] = y[bv + (long)i * S + q];
+            }
+        }
+    };
+    if (a0 < a1) load(a0, Xu, Xv);
+    for (int a = a0; a < a1; a++) {
+        double fb = fbeta[a] * 10000.;
+        double fbm = 10000. - fb;
+        if (wgt) { fb *= wgt[a]; fbm *= wgt[a]; }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * AT;
+            if (e < E) {
+                A[e] = Xu[k];
+                A[E + e] = Xv[k];
+            }
+        }
+        if (a + 1 < a1) load(a + 1, Nu, Nv);
+        __syncthreads();
+        axis_d12<LX, 2 * PX, PX, E, 1, PX, AT, true>(A, B, twx, tid, k1s, k2s);
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * AT;
+            if (e < E) {
+                const double2 u = Xu[k], v = Xv[k];
+                na[k] += cnorm(u) * fb;
+                nbv[k] += cnorm(v) * fbm;
+                const double2 uv = cmul(u, make_double2(v.x, -v.y)) * ((fbm - fb) / 2.0);
+                nux[k] += uv.x;
+                nuy[k] += uv.y;
+                ta[k] += cmulc(u, A[e]).x * fb;
+                tb[k] += cmulc(v, A[E + e]).x * fbm;
+                ja[k] += cmulc(u, B[e]).y * fb;
+                jbx[k] -= cmulc(v, B[E + e]).y * fbm;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            Xu[k] = Nu[k];
+            Xv[k] = Nv[k];
+        }
+    }
+    double *p = part + (long)ch * 8 * N;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * AT;
+        if (e < E) {
+            const int i = e / PX, q = e - i * PX;
+            const long g = fb0 + (long)i * S + q;
+            p[0 * N + g] = na[k];
+            p[1 * N + g] = nbv[k];
+            p[2 * N + g] = nux[k];
+            p[3 * N + g] = nuy[k];
+            p[4 * N + g] = ta[k];
+            p[5 * N + g] = tb[k];
+            p[6 * N + g] = ja[k];
+            p[7 * N + g] = jbx[k];
+        }
+    }
+}
+
+// yz pass: one workgroup per (chunk, x plane), both components of the plane
+// in LDS.  Fields per chunk: tau_a (y+z), tau_b (y+z), j_a,y, j_a,z, j_b,y,
+// j_b,z.
+template <int LY, int LZ>
+__global__ void __launch_bounds__(nt_ayz(LY *LZ))
+dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
+      int NX, int G, const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y,
+      double k1z, double k2z, double *__restrict__ part)
+{
+    constexpr int E = LY * LZ;
+    constexpr int NT = nt_ayz(E);
+    constexpr int KM = (E + NT - 1) / NT;
+    __shared__ double2 twy[LY], twz[LZ];
+    extern __shared__ double2 lds_raw[];
+    double2 *A = lds_raw, *B = lds_raw + 2 * E;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
+    for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
+    const int ch = blockIdx.x / NX;
+    const int x = blockIdx.x - ch * NX;
+    const int a0 = ch * G, a1 = min(n, a0 + G);
+    const long g0 = (long)x * E;
+    double ta[KM], tb[KM], jay[KM], jaz[KM], jby[KM], jbz[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) ta[k] = tb[k] = jay[k] = jaz[k] = jby[k] = jbz[k] = 0.0;
+    // the next state's plane is loaded while this one is transformed
+    double2 Xu[KM], Xv[KM], Nu[KM], Nv[KM];
+    auto load = [&](int a, double2 *U, double2 *V) {
+        const long gu = (long)a * N + g0, gv = ((long)n + a) * N + g0;
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            if (e < E) {
+                U[k] = y[gu + e];
+                V[k] = y[gv + e];
+            }
+        }
+    };
+    if (a0 < a1) load(a0, Xu, Xv);
+    for (int a = a0; a < a1; a++) {
+        double fb = fbeta[a] * 10000.;
+        double fbm = 10000. - fb;
+        if (wgt) { fb *= wgt[a]; fbm *= wgt[a]; }
+        if (a + 1 < a1) load(a + 1, Nu, Nv);
+#pragma unroll
+        for (int ax = 0; ax < 2; ax++) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                const int e = tid + k * NT;
+                if (e < E) {
+                    A[e] = Xu[k];
+                    A[E + e] = Xv[k];
+                }
+            }
+            __syncthreads();
+            if (ax == 0) axis_d12<LZ, 2 * LY, 1, LZ, 0, 1, NT, true>(A, B, twz, tid, k1z, k2z);
+            else axis_d12<LY, 2 * LZ, LZ, E, 1, LZ, NT, true>(A, B, twy, tid, k1y, k2y);
+#pragma unroll
+            for (int k = 0; k < KM; k++) {
+                const int e = tid + k * NT;
+                if (e < E) {
+                    ta[k] += cmulc(Xu[k], A[e]).x * fb;
+                    tb[k] += cmulc(Xv[k], A[E + e]).x * fbm;
+                    const double gu_ = cmulc(Xu[k], B[e]).y * fb, gv_ = cmulc(Xv[k], B[E + e]).y * fbm;
+                    if (ax == 0) { jaz[k] += gu_; jbz[k] -= gv_; }
+                    else { jay[k] += gu_; jby[k] -= gv_; }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            Xu[k] = Nu[k];
+            Xv[k] = Nv[k];
+        }
+    }
+    double *p = part + (long)ch * 6 * N + g0;
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NT;
+        if (e < E) {
+            p[0 * N + e] = ta[k];
+            p[1 * N + e] = tb[k];
+            p[2 * N + e] = jay[k];
+            p[3 * N + e] = jaz[k];
+            p[4 * N + e] = jby[k];
+            p[5 * N + e] = jbz[k];
+        }
+    }
+}
+
+// dens (the 12*N block) from the chunk partials of dg_x and dg_yz
+__global__ void __launch_bounds__(AT)
+dg_final(const double *__restrict__ px, const double *__restrict__ pyz, int nchunk, long N, double tr,
+         double *__restrict__ dens)
+{
+    const long i = (long)blockIdx.x * AT + threadIdx.x;
+    if (i >= N) return;
+    double s[8], t[6];
+#pragma unroll
+    for (int v = 0; v < 8; v++) s[v] = 0.0;
+#pragma unroll
+    for (int v = 0; v < 6; v++) t[v] = 0.0;
+    for (int k = 0; k < nchunk; k++) {
+#pragma unroll
+        for (int v = 0; v < 8; v++) s[v] += px[((long)k * 8 + v) * N + i];
+#pragma unroll
+        for (int v = 0; v < 6; v++) t[v] += pyz[((long)k * 6 + v) * N + i];
+    }
+    const double sc = 10000.;
+    dens[2 * i] = s[2] / sc / tr;
+    dens[2 * i + 1] = s[3] / sc / tr;
+    dens[2 * N + i] = s[0] / sc / tr;
+    dens[7 * N + i] = s[1] / sc / tr;
+    dens[3 * N + i] = (s[4] + t[0]) / sc / tr;
+    dens[8 * N + i] = (s[5] + t[1]) / sc / tr;
+    dens[4 * N + i] = s[6] / sc / tr;
+    dens[5 * N + i] = t[2] / sc / tr;
+    dens[6 * N + i] = t[3] / sc / tr;
+    dens[9 * N + i] = s[7] / sc / tr;
+    dens[10 * N + i] = t[4] / sc / tr;
+    dens[11 * N + i] = t[5] / sc / tr;
+}
+
+// qpe[a] = dv * sum of the state's NX plane partials
+__global__ void ah_qpe(const double *__restrict__ part, int n, int NX, double dv, double *__restrict__ qpe)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < NX; k++) s += part[(long)a * NX + k];
+    qpe[a] = dv * s;
+}
+
+typedef void (*ahx_fn)(const double2 *, double2 *, const double *, int, long, int, const double2 *, double, double);
+typedef void (*ahyz_fn)(const double2 *, const double2 *, double2 *, const double *, const double *, int, long, int,
+                        const double2 *, const double2 *, double, double, double, double, double, double, double *);
+
+// x pass for nx points and PX pencils per workgroup (S must be a multiple
+// of PX)
+static bool ahx_for(int nx, int S, ahx_fn *f, int *px)
+{
+    if (S % 32 == 0) {
+        *px = 32;
+        switch (nx) {
+        case 12: *f = ah_x<12, 32>; return true;
+        case 16: *f = ah_x<16, 32>; return true;
+        case 24: *f = ah_x<24, 32>; return true;
+        case 32: *f = ah_x<32, 32>; return true;
+        default: return false;
+        }
+    }
+    if (S % 16 == 0) {
+        *px = 16;
+        switch (nx) {
+        case 12: *f = ah_x<12, 16>; return true;
+        case 16: *f = ah_x<16, 16>; return true;
+        case 24: *f = ah_x<24, 16>; return true;
+        case 32: *f = ah_x<32, 16>; return true;
+        default: return false;
+        }
+    }
+    return false;
+}
+
+static bool ahyz_for(int ny, int nz, ahyz_fn *f, int *nt)
+{
+    if (ny == 12 && nz == 12) { *f = ah_yz<12, 12>; *nt = nt_ayz(144); return true; }
+    if (ny == 16 && nz == 16) { *f = ah_yz<16, 16>; *nt = nt_ayz(256); return true; }
+    if (ny == 24 && nz == 24) { *f = ah_yz<24, 24>; *nt = nt_ayz(576); return true; }
+    if (ny == 32 && nz == 32) { *f = ah_yz<32, 32>; *nt = nt_ayz(1024); return true; }
+    return false;
+}
+
+typedef void (*dgx_fn)(const double2 *, const double *, const double *, int, long, int, int, const double2 *, double,
+                       double, double *);
+typedef void (*dgyz_fn)(const double2 *, const double *, const double *, int, long, int, int, const double2 *,
+                        const double2 *, double, double, double, double, double *);
+
+static bool dgx_for(int nx, int S, dgx_fn *f, int *px)
+{
+    if (S % 32 == 0) {
+        *px = 32;
+        switch (nx) {
+        case 12: *f = dg_x<12, 32>; return true;
+        case 16: *f = dg_x<16, 32>; return true;
+        case 24: *f = dg_x<24, 32>; return true;
+        case 32: *f = dg_x<32, 16>; *px = 16; return true;   // 2 x 2 x 32 x 32 points would exceed 64 KB
+        default: return false;
+        }
+    }
+    if (S % 16 == 0) {
+        *px = 16;
+        switch (nx) {
+        case 12: *f = dg_x<12, 16>; return true;
+        case 16: *f = dg_x<16, 16>; return true;
+        case 24: *f = dg_x<24, 16>; return true;
+        case 32: *f = dg_x<32, 16>; return true;
+        default: return false;
+        }
+    }
+    return false;
+}
+
+static bool dgyz_for(int ny, int nz, dgyz_fn *f, int *nt)
+{
+    if (ny == 12 && nz == 12) { *f = dg_yz<12, 12>; *nt = nt_ayz(144); return true; }
+    if (ny == 16 && nz == 16) { *f = dg_yz<16, 16>; *nt = nt_ayz(256); return true; }
+    if (ny == 24 && nz == 24) { *f = dg_yz<24, 24>; *nt = nt_ayz(576); return true; }
+    if (ny == 32 && nz == 32) { *f = dg_yz<32, 32>; *nt = nt_ayz(1024); return true; }
+    return false;
+}
+
+bool dens_grad_fused_supported(const tdw_ctx *c)
+{
+    if (c->dim != 3) return false;
+    dgx_fn fx;
+    dgyz_fn fy;
+    int nt, px;
+    if (!dgx_for(c->nx, c->S, &fx, &px) || !dgyz_for(c->ny, c->nz, &fy, &nt)) return false;
+    return sizeof(double2) * 4 * (size_t)c->ny * c->nz <= (size_t)c->max_lds;
+}
+
+int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, const double *wgt, double *dens)
+{
+    dgx_fn fx;
+    dgyz_fn fy;
+    int nt, px;
+    if (!dgx_for(c->nx, c->S, &fx, &px) || !dgyz_for(c->ny, c->nz, &fy, &nt)) return TDW_ESIZE;
+    const long N = (long)c->N;
+    const int S = c->S;
+    // states per chunk: keep >= ~4096 workgroups in the x pass
+    int G = 64;
+    while (G > 4 && (long)(S / px) * ((n + G - 1) / G) < 4096) G >>= 1;
+    const int nchunk = (n + G - 1) / G;
+    TDW_TRY(ensure_part(c, (size_t)nchunk * 14 * N));
+    double *px_part = c->d_part, *pyz_part = c->d_part + (size_t)nchunk * 8 * N;
+    const double k1 = 2.0 * M_PI / c->Lphys[0] / c->nx, k2 = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
+    {
+        TDW_TIMED(c, "dens_gx");
+        hipLaunchKernelGGL(fx, dim3((unsigned)((long)nchunk * (S / px))), dim3(AT), 0, c->stream, wf, fbeta, wgt, n, N,
+                           S, G, c->d_tw[0], k1, k2, px_part);
+    }
+    TDW_TIMED(c, "dens_gyz");
+    const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+    const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
+    const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
+    hipLaunchKernelGGL(fy, dim3((unsigned)((long)nchunk * c->nx)), dim3(nt), lds, c->stream, wf, fbeta, wgt, n, N,
+                       c->nx, G, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, pyz_part);
+    hipLaunchKernelGGL(dg_final, dim3((unsigned)((N + AT - 1) / AT)), dim3(AT), 0, c->stream, px_part, pyz_part, nchunk,
+                       N, dens_transverse(c), dens);
+    return (int)hipGetLastError();
+}
+
+bool aslda_h_fused_supported(const tdw_ctx *c)
+{
+    if (c->dim != 3) return false;
+    ahx_fn fx;
+    ahyz_fn fy;
+    int nt, px;
+    if (!ahx_for(c->nx, c->S, &fx, &px) || !ahyz_for(c->ny, c->nz, &fy, &nt)) return false;
+    return sizeof(double2) * 4 * (size_t)c->ny * c->nz <= (size_t)c->max_lds;
+}
+
+// out = H in with <H> into qpe (no subtraction); T: 2*n*N complex scratch;
+// hx: the fields of aslda_fields; part: n*NX doubles
+int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots,
+                  const double *hx, double *part, double *d_qpe)
+{
+    ahx_fn fx;
+    ahyz_fn fy;
+    int nt, px;
+    if (!ahx_for(c->nx, c->S, &fx, &px) || !ahyz_for(c->ny, c->nz, &fy, &nt)) return TDW_ESIZE;
+    const long N = (long)c->N;
+    const int S = c->S;
+    {
+        TDW_TIMED(c, "ah_x");
+        const long nblk = 2L * n * (S / px);
+        const double k1 = 2.0 * M_PI / c->Lphys[0] / c->nx, k2 = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
+        hipLaunchKernelGGL(fx, dim3((unsigned)nblk), dim3(AT), 0, c->stream, in, T, pots, n, N, S, c->d_tw[0], k1, k2);
+    }
+    {
+        TDW_TIMED(c, "ah_yz");
+        const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+        const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
+        const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
+        hipLaunchKernelGGL(fy, dim3((unsigned)((long)n * c->nx)), dim3(nt), lds, c->stream, in, T, out, pots, hx, n, N,
+                           c->nx, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, c->p.mu_a, c->p.mu_b, part);
+        hipLaunchKernelGGL(ah_qpe, dim3((n + 255) / 256), dim3(256), 0, c->stream, part, n, c->nx, dvol_norm(c), d_qpe);
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace tdw

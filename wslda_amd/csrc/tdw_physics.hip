@@ -708,6 +708,30 @@ int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const doub
                   const double *useqpe, double *d_qpe)
 {
     const long N = (long)c->N;
+    if (!alap && aslda_h_fused_supported(c)) {
+        // two passes from psi, alpha and A (the context's state-sized array
+        // holds the x terms)
+        const size_t need = (size_t)2 * n * N;
+        if (need > c->aw_cap) {
+            TDW_HIP(hipStreamSynchronize(c->stream));
+            if (c->d_aw) TDW_HIP(hipFree(c->d_aw));
+            c->d_aw = nullptr;
+            c->aw_cap = 0;
+            TDW_HIP(hipMalloc(&c->d_aw, need * sizeof(double2)));
+            c->aw_cap = need;
+        }
+        TDW_TRY(aslda_fields(c, pots));
+        TDW_TRY(ensure_part(c, (size_t)n * c->nx + n));
+        double *qcalc = useqpe ? c->d_part + (size_t)n * c->nx : d_qpe;
+        TDW_TRY(aslda_h_fused(c, n, in, out, c->d_aw, pots, c->d_hx, c->d_part, qcalc));
+        const double *q = useqpe ? useqpe : d_qpe;
+        if (useqpe && d_qpe && d_qpe != useqpe)
+            TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        TDW_TIMED(c, "sub_qpe");
+        const long tot = 2L * n * N;
+        hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);
+        return (int)hipGetLastError();
+    }
     if (!alap) {
         // lap(alpha psi) into the context's array
         const size_t need = (size_t)2 * n * N;

@@ -116,6 +116,17 @@ int alpha_wf(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *p
 int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *ddx, const double2 *ddy,
                   const double2 *ddz, const double2 *lap, const double2 *alap, const double *pots, const double *kt,
                   const double *useqpe, double *d_qpe);
+// ASLDA H on 3D lattices in an x-pencil and a yz-plane pass (tdw_aslda.hip):
+// out = H in without the subtraction, <H> into d_qpe; T state-sized scratch,
+// hx the grid fields of aslda_fields, part n*NX doubles
+bool aslda_h_fused_supported(const tdw_ctx *c);
+int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots,
+                  const double *hx, double *part, double *d_qpe);
+// Densities with gradients (rho, nu, tau before its lap(rho) correction, j)
+// on 3D lattices in an x and a yz pass that keep the gradients in LDS
+// (tdw_aslda.hip); the same dens block as densities(..., grad = 1)
+bool dens_grad_fused_supported(const tdw_ctx *c);
+int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, const double *wgt, double *dens);
 int energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e);
 int taylor(tdw_ctx *c, int order, double dt, int n, const double2 *hpsi, double2 *upd, double2 *contr);
 
