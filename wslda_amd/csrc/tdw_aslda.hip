@@ -27,7 +27,9 @@ static constexpr int AT = 256;
 // ---------------------------------------------------------------------------
 // In-place derivatives along one axis of P pencils held in LDS (L = R1*R2,
 // n = n1*R2 + n2, frequency f = k1 + R1*k2 at position k1*R2 + n2 between
-// the stages).  A receives the second derivative (factor -k2s m^2, the
+// the stages; consecutive lanes take consecutive pencils, so layouts whose
+// pencils start one element apart, or a padded row apart, are read without
+// LDS bank conflicts).  A receives the second derivative (factor -k2s m^2, the
 // inverse transform's 1/L folded into k2s); with GRAD, B (same layout)
 // receives the first derivative (factor i k1s m, Nyquist mode zero).  Pencil q starts at
 // (q / QD) * QH + (q % QD) * QL, element stride IS.
@@ -39,7 +41,7 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
     static_assert(R2 > 1, "axis_d12 needs L = R1 * R2 with R2 > 1");
     // stage 1: R1-point DFTs over n1, twiddle W_L^{n2 k1}
     for (int b = tid; b < P * R2; b += NT) {
-        const int q = b / R2, n2 = b - q * R2;
+        const int n2 = b / P, q = b - n2 * P;
         double2 *pa = A + (q / QD) * QH + (q % QD) * QL;
         double2 v[R1];
 #pragma unroll
@@ -53,7 +55,7 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
     __syncthreads();
     // stage 2: R2-point DFTs over n2, the derivative factors, inverse R2 DFTs
     for (int b = tid; b < P * R1; b += NT) {
-        const int q = b / R1, k1 = b - q * R1;
+        const int k1 = b / P, q = b - k1 * P;
         const int off = (q / QD) * QH + (q % QD) * QL;
         double2 *pa = A + off;
         double2 v[R2], g[R2];
@@ -80,7 +82,7 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
     __syncthreads();
     // stage 3: inverse R1-point DFTs over k1
     for (int b = tid; b < P * R2; b += NT) {
-        const int q = b / R2, n2 = b - q * R2;
+        const int n2 = b / P, q = b - n2 * P;
         const int off = (q / QD) * QH + (q % QD) * QL;
         double2 v[R1];
         double2 *pa = A + off;
@@ -175,23 +177,25 @@ constexpr int nt_ayz(int E)
 
 // ---------------------------------------------------------------------------
 // yz pass: one workgroup per (state, x plane); both components of the plane
-// in LDS (A: 2E, B: 2E).  out = H psi, part[block] = Re <psi|H psi> of the
+// in LDS (A and B: 2 planes of row stride LZ + 1).  out = H psi,
+// part[block] = Re <psi|H psi> of the
 // plane.  hx = [lap alpha_a, lap alpha_b, div A_a, div A_b].
 // ---------------------------------------------------------------------------
 template <int LY, int LZ>
-__global__ void __launch_bounds__(nt_ayz(LY *LZ))
+__global__ void __launch_bounds__(nt_ayz(LY *LZ), (LY * LZ <= 576) ? 3 : 2)
 ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__restrict__ out,
       const double *__restrict__ pots, const double *__restrict__ hx, int n, long N, int NX,
       const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y, double k1z,
       double k2z, double mu_a, double mu_b, double *__restrict__ part)
 {
     constexpr int E = LY * LZ;
+    constexpr int EP = LY * (LZ + 1);   // padded plane in LDS: row stride LZ + 1
     constexpr int NT = nt_ayz(E);
     constexpr int KM = (E + NT - 1) / NT;
     __shared__ double2 twy[LY], twz[LZ];
     __shared__ double red[NT / 64];
     extern __shared__ double2 lds_raw[];
-    double2 *A = lds_raw, *B = lds_raw + 2 * E;
+    double2 *A = lds_raw, *B = lds_raw + 2 * EP;
     const int tid = threadIdx.x;
     for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
     for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
@@ -224,20 +228,22 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
-                A[e] = Xu[k];
-                A[E + e] = Xv[k];
+                const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                A[pe] = Xu[k];
+                A[EP + pe] = Xv[k];
             }
         }
         __syncthreads();
-        if (ax == 0) axis_d12<LZ, 2 * LY, 1, LZ, 0, 1, NT, true>(A, B, twz, tid, k1z, k2z);
-        else axis_d12<LY, 2 * LZ, LZ, E, 1, LZ, NT, true>(A, B, twy, tid, k1y, k2y);
+        if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, true>(A, B, twz, tid, k1z, k2z);
+        else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, true>(A, B, twy, tid, k1y, k2y);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
+                const int pe = (e / LZ) * (LZ + 1) + e % LZ;
                 const double Aa = pots[(6 + d) * N + g0 + e], Ab = pots[(9 + d) * N + g0 + e];
-                Ou[k] = Ou[k] + A[e] * (-0.25 * al[k]) + imul(B[e], -Aa);
-                Ov[k] = Ov[k] + A[E + e] * (0.25 * bl[k]) + imul(B[E + e], -Ab);
+                Ou[k] = Ou[k] + A[pe] * (-0.25 * al[k]) + imul(B[pe], -Aa);
+                Ov[k] = Ov[k] + A[EP + pe] * (0.25 * bl[k]) + imul(B[EP + pe], -Ab);
             }
         }
         // alpha psi: d^2 into A
@@ -246,19 +252,21 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
-                A[e] = Xu[k] * al[k];
-                A[E + e] = Xv[k] * bl[k];
+                const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                A[pe] = Xu[k] * al[k];
+                A[EP + pe] = Xv[k] * bl[k];
             }
         }
         __syncthreads();
-        if (ax == 0) axis_d12<LZ, 2 * LY, 1, LZ, 0, 1, NT, false>(A, nullptr, twz, tid, k1z, k2z);
-        else axis_d12<LY, 2 * LZ, LZ, E, 1, LZ, NT, false>(A, nullptr, twy, tid, k1y, k2y);
+        if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, false>(A, nullptr, twz, tid, k1z, k2z);
+        else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, false>(A, nullptr, twy, tid, k1y, k2y);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
-                Ou[k] = Ou[k] + A[e] * -0.25;
-                Ov[k] = Ov[k] + A[E + e] * 0.25;
+                const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                Ou[k] = Ou[k] + A[pe] * -0.25;
+                Ov[k] = Ov[k] + A[EP + pe] * 0.25;
             }
         }
     }
@@ -301,7 +309,7 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
 // v at A[E..2E), pencil q, point i at q + i*PX).  Fields per chunk:
 // rho_a, rho_b, Re nu, Im nu, tau_a (x), tau_b (x), j_a,x, j_b,x.
 template <int LX, int PX>
-__global__ void __launch_bounds__(AT)
+__global__ void __launch_bounds__(AT, (LX * PX <= 512) ? 3 : 2)
 dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
      int S, int G, const double2 *__restrict__ twx_g, double k1s, double k2s, double *__restrict__ part)
 {
@@ -319,8 +327,7 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
     double na[KM], nbv[KM], nux[KM], nuy[KM], ta[KM], tb[KM], ja[KM], jbx[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) na[k] = nbv[k] = nux[k] = nuy[k] = ta[k] = tb[k] = ja[k] = jbx[k] = 0.0;
-    // the next state's pencils are loaded while this one is transformed
-    double2 Xu[KM], Xv[KM], Nu[KM], Nv[KM];
+    double2 Xu[KM], Xv[KM];
     auto load = [&](int a, double2 *U, double2 *V) {
         const long bu = (long)a * N + fb0, bv = ((long)n + a) * N + fb0;
 #pragma unroll
@@ -333,11 +340,11 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
             }
         }
     };
-    if (a0 < a1) load(a0, Xu, Xv);
     for (int a = a0; a < a1; a++) {
         double fb = fbeta[a] * 10000.;
         double fbm = 10000. - fb;
         if (wgt) { fb *= wgt[a]; fbm *= wgt[a]; }
+        load(a, Xu, Xv);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KM; k++) {
@@ -347,7 +354,6 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
                 A[E + e] = Xv[k];
             }
         }
-        if (a + 1 < a1) load(a + 1, Nu, Nv);
         __syncthreads();
         axis_d12<LX, 2 * PX, PX, E, 1, PX, AT, true>(A, B, twx, tid, k1s, k2s);
 #pragma unroll
@@ -365,11 +371,6 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
                 ja[k] += cmulc(u, B[e]).y * fb;
                 jbx[k] -= cmulc(v, B[E + e]).y * fbm;
             }
-        }
-#pragma unroll
-        for (int k = 0; k < KM; k++) {
-            Xu[k] = Nu[k];
-            Xv[k] = Nv[k];
         }
     }
     double *p = part + (long)ch * 8 * N;
@@ -395,17 +396,18 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
 // in LDS.  Fields per chunk: tau_a (y+z), tau_b (y+z), j_a,y, j_a,z, j_b,y,
 // j_b,z.
 template <int LY, int LZ>
-__global__ void __launch_bounds__(nt_ayz(LY *LZ))
+__global__ void __launch_bounds__(nt_ayz(LY *LZ), (LY * LZ <= 576) ? 3 : 2)
 dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
       int NX, int G, const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y,
       double k1z, double k2z, double *__restrict__ part)
 {
     constexpr int E = LY * LZ;
+    constexpr int EP = LY * (LZ + 1);   // padded plane in LDS: row stride LZ + 1
     constexpr int NT = nt_ayz(E);
     constexpr int KM = (E + NT - 1) / NT;
     __shared__ double2 twy[LY], twz[LZ];
     extern __shared__ double2 lds_raw[];
-    double2 *A = lds_raw, *B = lds_raw + 2 * E;
+    double2 *A = lds_raw, *B = lds_raw + 2 * EP;
     const int tid = threadIdx.x;
     for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
     for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
@@ -416,8 +418,7 @@ dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const dou
     double ta[KM], tb[KM], jay[KM], jaz[KM], jby[KM], jbz[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) ta[k] = tb[k] = jay[k] = jaz[k] = jby[k] = jbz[k] = 0.0;
-    // the next state's plane is loaded while this one is transformed
-    double2 Xu[KM], Xv[KM], Nu[KM], Nv[KM];
+    double2 Xu[KM], Xv[KM];
     auto load = [&](int a, double2 *U, double2 *V) {
         const long gu = (long)a * N + g0, gv = ((long)n + a) * N + g0;
 #pragma unroll
@@ -429,12 +430,11 @@ dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const dou
             }
         }
     };
-    if (a0 < a1) load(a0, Xu, Xv);
     for (int a = a0; a < a1; a++) {
         double fb = fbeta[a] * 10000.;
         double fbm = 10000. - fb;
         if (wgt) { fb *= wgt[a]; fbm *= wgt[a]; }
-        if (a + 1 < a1) load(a + 1, Nu, Nv);
+        load(a, Xu, Xv);
 #pragma unroll
         for (int ax = 0; ax < 2; ax++) {
             __syncthreads();
@@ -442,29 +442,26 @@ dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const dou
             for (int k = 0; k < KM; k++) {
                 const int e = tid + k * NT;
                 if (e < E) {
-                    A[e] = Xu[k];
-                    A[E + e] = Xv[k];
+                    const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                    A[pe] = Xu[k];
+                    A[EP + pe] = Xv[k];
                 }
             }
             __syncthreads();
-            if (ax == 0) axis_d12<LZ, 2 * LY, 1, LZ, 0, 1, NT, true>(A, B, twz, tid, k1z, k2z);
-            else axis_d12<LY, 2 * LZ, LZ, E, 1, LZ, NT, true>(A, B, twy, tid, k1y, k2y);
+            if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, true>(A, B, twz, tid, k1z, k2z);
+            else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, true>(A, B, twy, tid, k1y, k2y);
 #pragma unroll
             for (int k = 0; k < KM; k++) {
                 const int e = tid + k * NT;
                 if (e < E) {
-                    ta[k] += cmulc(Xu[k], A[e]).x * fb;
-                    tb[k] += cmulc(Xv[k], A[E + e]).x * fbm;
-                    const double gu_ = cmulc(Xu[k], B[e]).y * fb, gv_ = cmulc(Xv[k], B[E + e]).y * fbm;
+                    const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                    ta[k] += cmulc(Xu[k], A[pe]).x * fb;
+                    tb[k] += cmulc(Xv[k], A[EP + pe]).x * fbm;
+                    const double gu_ = cmulc(Xu[k], B[pe]).y * fb, gv_ = cmulc(Xv[k], B[EP + pe]).y * fbm;
                     if (ax == 0) { jaz[k] += gu_; jbz[k] -= gv_; }
                     else { jay[k] += gu_; jby[k] -= gv_; }
                 }
             }
-        }
-#pragma unroll
-        for (int k = 0; k < KM; k++) {
-            Xu[k] = Nu[k];
-            Xv[k] = Nv[k];
         }
     }
     double *p = part + (long)ch * 6 * N + g0;
@@ -577,7 +574,7 @@ static bool dgx_for(int nx, int S, dgx_fn *f, int *px)
         switch (nx) {
         case 12: *f = dg_x<12, 32>; return true;
         case 16: *f = dg_x<16, 32>; return true;
-        case 24: *f = dg_x<24, 32>; return true;
+        case 24: *f = dg_x<24, 16>; *px = 16; return true;   // fewer registers, three waves per SIMD
         case 32: *f = dg_x<32, 16>; *px = 16; return true;   // 2 x 2 x 32 x 32 points would exceed 64 KB
         default: return false;
         }
@@ -611,7 +608,7 @@ bool dens_grad_fused_supported(const tdw_ctx *c)
     dgyz_fn fy;
     int nt, px;
     if (!dgx_for(c->nx, c->S, &fx, &px) || !dgyz_for(c->ny, c->nz, &fy, &nt)) return false;
-    return sizeof(double2) * 4 * (size_t)c->ny * c->nz <= (size_t)c->max_lds;
+    return sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1) <= (size_t)c->max_lds;
 }
 
 int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, const double *wgt, double *dens)
@@ -635,7 +632,7 @@ int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, c
                            S, G, c->d_tw[0], k1, k2, px_part);
     }
     TDW_TIMED(c, "dens_gyz");
-    const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+    const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1);
     const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
     const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
     hipLaunchKernelGGL(fy, dim3((unsigned)((long)nchunk * c->nx)), dim3(nt), lds, c->stream, wf, fbeta, wgt, n, N,
@@ -652,7 +649,7 @@ bool aslda_h_fused_supported(const tdw_ctx *c)
     ahyz_fn fy;
     int nt, px;
     if (!ahx_for(c->nx, c->S, &fx, &px) || !ahyz_for(c->ny, c->nz, &fy, &nt)) return false;
-    return sizeof(double2) * 4 * (size_t)c->ny * c->nz <= (size_t)c->max_lds;
+    return sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1) <= (size_t)c->max_lds;
 }
 
 // out = H in with <H> into qpe (no subtraction); T: 2*n*N complex scratch;
@@ -674,7 +671,7 @@ int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
     }
     {
         TDW_TIMED(c, "ah_yz");
-        const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * c->nz;
+        const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1);
         const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
         const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
         hipLaunchKernelGGL(fy, dim3((unsigned)((long)n * c->nx)), dim3(nt), lds, c->stream, in, T, out, pots, hx, n, N,
