@@ -218,6 +218,8 @@ constexpr int nt_hyz(int E)
 //   inverse R2-point DFT and the conjugate twiddle, stored at k1*R2 + n2;
 // stage 3 (per n2): inverse R1-point DFT over k1, stored at n1*R2 + n2.
 // Every stage reads and writes the same positions, so one buffer suffices.
+// Consecutive lanes take consecutive pencils: with pencils one element or one
+// padded row apart, a wave's LDS accesses fall in distinct banks.
 
 template <int L, int P, int QD, int QH, int QL, int IS, int NT, typename C>
 __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
@@ -227,7 +229,7 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
     // stage 1
     if constexpr (R2 > 1) {
         for (int b = tid; b < P * R2; b += NT) {
-            const int q = b / R2, n2 = b - q * R2;
+            const int n2 = b / P, q = b - n2 * P;
             C *pa = A + (q / QD) * QH + (q % QD) * QL;
             C v[R1];
 #pragma unroll
@@ -242,7 +244,7 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
     }
     // stage 2 (for R2 == 1 this is the whole transform)
     for (int b = tid; b < P * R1; b += NT) {
-        const int q = b / R1, k1 = b - q * R1;
+        const int k1 = b / P, q = b - k1 * P;
         C *pa = A + (q / QD) * QH + (q % QD) * QL;
         if constexpr (R2 > 1) {
             C v[R2];
@@ -282,7 +284,7 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
     // stage 3
     if constexpr (R2 > 1) {
         for (int b = tid; b < P * R2; b += NT) {
-            const int q = b / R2, n2 = b - q * R2;
+            const int n2 = b / P, q = b - n2 * P;
             C *pa = A + (q / QD) * QH + (q % QD) * QL;
             C v[R1];
 #pragma unroll
@@ -310,6 +312,7 @@ f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *
     typedef hc_t<SP> C;
     typedef decltype(C().x) R;
     constexpr int E = LY * LZ;              // points of one plane
+    constexpr int EP = LY * (LZ + 1);       // the plane in LDS, row stride LZ + 1
     constexpr int KM = (E + NT - 1) / NT;   // owned points per thread
     __shared__ C twy[LY], twz[LZ];
     __shared__ double red[NT / 64];
@@ -332,7 +335,8 @@ f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *
             Ov[k] = T[gv + e];
         }
     }
-    // z axis: pencils (c, iy), offset q*LZ, stride 1; y axis: pencils (c, iz), offset (q/LZ)*E + q%LZ, stride LZ
+    // z axis: pencils (c, iy), offset c*EP + iy*(LZ+1), stride 1; y axis:
+    // pencils (c, iz), offset c*EP + iz, stride LZ+1
 #pragma unroll
     for (int ax = 0; ax < 2; ax++) {
         __syncthreads();
@@ -340,21 +344,23 @@ f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
-                A[e] = Xu[k];
-                A[E + e] = Xv[k];
+                const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                A[pe] = Xu[k];
+                A[EP + pe] = Xv[k];
             }
         }
         __syncthreads();
         // second derivative along the axis, in place
-        if (ax == 0) lap_axis<LZ, 2 * LY, 1, LZ, 0, 1, NT, C>(A, twz, tid, kzs);
-        else lap_axis<LY, 2 * LZ, LZ, E, 1, LZ, NT, C>(A, twy, tid, kys);
+        if (ax == 0) lap_axis<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, C>(A, twz, tid, kzs);
+        else lap_axis<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, C>(A, twy, tid, kys);
         const C *Rr = A;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
-                Ou[k] = Ou[k] + Rr[e];
-                Ov[k] = Ov[k] + Rr[E + e];
+                const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+                Ou[k] = Ou[k] + Rr[pe];
+                Ov[k] = Ov[k] + Rr[EP + pe];
             }
         }
     }
@@ -494,7 +500,7 @@ static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
 
 static size_t hyz_lds(const tdw_ctx *c)
 {
-    return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 2 * (size_t)c->ny * c->nz;
+    return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 2 * (size_t)c->ny * (c->nz + 1);
 }
 
 bool fused_supported(const tdw_ctx *c)
