@@ -479,37 +479,31 @@ dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const dou
     }
 }
 
-// dens (the 12*N block) from the chunk partials of dg_x and dg_yz
+// dens (the 12*N block) from the chunk partials of dg_x and dg_yz: one
+// thread per (output field, point), each summing its field over the chunks
+// in chunk order
 __global__ void __launch_bounds__(AT)
 dg_final(const double *__restrict__ px, const double *__restrict__ pyz, int nchunk, long N, double tr,
          double *__restrict__ dens)
 {
-    const long i = (long)blockIdx.x * AT + threadIdx.x;
-    if (i >= N) return;
-    double s[8], t[6];
-#pragma unroll
-    for (int v = 0; v < 8; v++) s[v] = 0.0;
-#pragma unroll
-    for (int v = 0; v < 6; v++) t[v] = 0.0;
-    for (int k = 0; k < nchunk; k++) {
-#pragma unroll
-        for (int v = 0; v < 8; v++) s[v] += px[((long)k * 8 + v) * N + i];
-#pragma unroll
-        for (int v = 0; v < 6; v++) t[v] += pyz[((long)k * 6 + v) * N + i];
-    }
-    const double sc = 10000.;
-    dens[2 * i] = s[2] / sc / tr;
-    dens[2 * i + 1] = s[3] / sc / tr;
-    dens[2 * N + i] = s[0] / sc / tr;
-    dens[7 * N + i] = s[1] / sc / tr;
-    dens[3 * N + i] = (s[4] + t[0]) / sc / tr;
-    dens[8 * N + i] = (s[5] + t[1]) / sc / tr;
-    dens[4 * N + i] = s[6] / sc / tr;
-    dens[5 * N + i] = t[2] / sc / tr;
-    dens[6 * N + i] = t[3] / sc / tr;
-    dens[9 * N + i] = s[7] / sc / tr;
-    dens[10 * N + i] = t[4] / sc / tr;
-    dens[11 * N + i] = t[5] / sc / tr;
+    const long t = (long)blockIdx.x * AT + threadIdx.x;
+    if (t >= 12 * N) return;
+    const int f = (int)(t / N);
+    const long i = t - (long)f * N;
+    // source field in the x partials (8 per chunk) and the yz partials (6 per chunk), -1: none
+    // output f: 0 Re nu, 1 Im nu, 2 rho_a, 3 tau_a, 4 j_a,x, 5 j_a,y, 6 j_a,z, 7 rho_b, 8 tau_b, 9 j_b,x,
+    //           10 j_b,y, 11 j_b,z
+    constexpr int sx[12] = {2, 3, 0, 4, 6, -1, -1, 1, 5, 7, -1, -1};
+    constexpr int sy[12] = {-1, -1, -1, 0, -1, 2, 3, -1, 1, -1, 4, 5};
+    const int fx = sx[f], fy = sy[f];
+    double a = 0.0, b = 0.0;
+    if (fx >= 0)
+        for (int k = 0; k < nchunk; k++) a += px[((long)k * 8 + fx) * N + i];
+    if (fy >= 0)
+        for (int k = 0; k < nchunk; k++) b += pyz[((long)k * 6 + fy) * N + i];
+    const double v = (a + b) / 10000. / tr;
+    if (f < 2) dens[2 * i + f] = v;
+    else dens[(long)f * N + i] = v;
 }
 
 // qpe[a] = dv * sum of the state's NX plane partials
@@ -637,8 +631,8 @@ int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, c
     const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
     hipLaunchKernelGGL(fy, dim3((unsigned)((long)nchunk * c->nx)), dim3(nt), lds, c->stream, wf, fbeta, wgt, n, N,
                        c->nx, G, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, pyz_part);
-    hipLaunchKernelGGL(dg_final, dim3((unsigned)((N + AT - 1) / AT)), dim3(AT), 0, c->stream, px_part, pyz_part, nchunk,
-                       N, dens_transverse(c), dens);
+    hipLaunchKernelGGL(dg_final, dim3((unsigned)((12 * N + AT - 1) / AT)), dim3(AT), 0, c->stream, px_part, pyz_part,
+                       nchunk, N, dens_transverse(c), dens);
     return (int)hipGetLastError();
 }
 
