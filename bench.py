@@ -86,13 +86,15 @@ def modular_units(name):
     """The ASLDA step's kernels (3D, FP64): each kernel's own compulsory
     reads and writes of state-sized arrays, in S per state and launch (its
     algorithmic bytes at its boundaries).  On lattices with the two-pass
-    kernels a step (AB5AM5) is abm_step1 8, abm_step4 3, and twice: norms 1
-    + scale 2, the density x and yz passes 1 + 1 (gradients kept in LDS),
-    the H x and yz passes 2 + 3, the <H> subtraction 3 -- 37 S against the
-    13 S compulsory for a whole step.  The modular kernels of other lattices:
-    derivatives 3 + 5, densities 5, alpha_wf 2, lap(alpha psi) 2 + 3,
-    apply_h 7 + 3."""
-    return {"abm_step1": 8, "abm_step4": 3, "norms": 1, "scale": 2, "dens_gx": 1, "dens_gyz": 1, "ah_x": 2,
+    kernels a step (AB5AM5) is abm_step1 8, the corrector with the
+    predictor's <H> subtraction folded in 4, and twice: norms 1 + scale 2,
+    the density x and yz passes 1 + 1 (gradients kept in LDS), the H x and
+    yz passes 2 + 3; plus the corrector H's <H> subtraction 3 -- 35 S
+    against the 13 S compulsory for a whole step.  The modular kernels of
+    other lattices: derivatives 3 + 5, densities 5, alpha_wf 2,
+    lap(alpha psi) 2 + 3, apply_h 7 + 3, abm_step4 3."""
+    return {"abm_step1": 8, "abm_step4": 3, "abm_step4q": 4, "norms": 1, "scale": 2, "dens_gx": 1, "dens_gyz": 1,
+            "ah_x": 2,
             "ah_yz": 3, "sub_qpe": 3, "fft_outer_deriv": 3, "fft_inner_deriv": 5, "densities": 5, "alpha_wf": 2,
             "fft_outer": 2, "fft_inner": 3, "apply_h": 10}.get(name, 0)
 

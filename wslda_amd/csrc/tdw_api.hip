@@ -445,10 +445,13 @@ static int step_aslda(tdw_ctx *c, tdw_state *s, int it, int k, double *qpe)
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(aslda_densities(c, s));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    // two-pass lattices fold the predictor's <H> subtraction into the corrector
+    const bool fold = aslda_h_fused_supported(c);
     TDW_TRY(apply_h_aslda(c, n, D2(s->wf), D2(s->lap), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
-                          s->pots, c->d_kt, nullptr, qpe));
+                          s->pots, c->d_kt, nullptr, qpe, !fold));
     // corrector
-    TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
+    if (fold) TDW_TRY(abm4q(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1]), qpe));
+    else TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(aslda_densities(c, s));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));

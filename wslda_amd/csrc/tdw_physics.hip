@@ -79,6 +79,28 @@ int abm1(tdw_ctx *c, int n, double2 *y, double2 *f1, double2 *f2, double2 *f3, d
     return abm1_to(c, n, y, y, nullptr, f1, f2, f3, f4, f5);
 }
 
+// corrector with the <H> subtraction of the predictor's H folded in:
+// y = -i dt a0 (hy - qpe y) + fl, y holding the normalised prediction
+__global__ void __launch_bounds__(NT)
+k_abm4q(long tot, double dta0, const double2 *__restrict__ hy, double2 *y, const double2 *__restrict__ fl,
+        const double *__restrict__ qpe, int n, long N)
+{
+    long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= tot) return;
+    const long a = (i / N) % n;
+    y[i] = imul(hy[i] - y[i] * qpe[a], -dta0) + fl[i];
+}
+
+int abm4q(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl, const double *qpe)
+{
+    TDW_TIMED(c, "abm_step4q");
+    long tot = 2L * n * (long)c->N;
+    double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
+    hipLaunchKernelGGL(k_abm4q, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, tot, c->p.dt * a0, hy, y, fl, qpe, n,
+                       (long)c->N);
+    return (int)hipGetLastError();
+}
+
 int abm4(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl)
 {
     TDW_TIMED(c, "abm_step4");
@@ -705,7 +727,7 @@ k_apply_h_aslda(const double2 *__restrict__ in, double2 *out, const double2 *__r
 
 int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *ddx, const double2 *ddy,
                   const double2 *ddz, const double2 *lap, const double2 *alap, const double *pots, const double *kt,
-                  const double *useqpe, double *d_qpe)
+                  const double *useqpe, double *d_qpe, bool subtract)
 {
     const long N = (long)c->N;
     if (!alap && aslda_h_fused_supported(c)) {
@@ -727,6 +749,7 @@ int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const doub
         const double *q = useqpe ? useqpe : d_qpe;
         if (useqpe && d_qpe && d_qpe != useqpe)
             TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        if (!subtract) return (int)hipGetLastError();
         TDW_TIMED(c, "sub_qpe");
         const long tot = 2L * n * N;
         hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);

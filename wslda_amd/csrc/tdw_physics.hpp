@@ -113,9 +113,14 @@ int alpha_wf(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *p
 // ASLDA H application with the <H> subtraction: ddx..ddz the gradients of in
 // along the evolved axes, lap its Laplacian, alap lap(alpha psi) (NULL: formed
 // here).  out may equal lap or alap (pointwise), not in.
+// subtract = false (two-pass form only, alap NULL on a supported lattice):
+// out = H in without the subtraction, <H> in d_qpe
 int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const double2 *ddx, const double2 *ddy,
                   const double2 *ddz, const double2 *lap, const double2 *alap, const double *pots, const double *kt,
-                  const double *useqpe, double *d_qpe);
+                  const double *useqpe, double *d_qpe, bool subtract = true);
+// the corrector with the predictor's <H> subtraction folded in:
+// y = -i dt a0 (hy - qpe y) + fl
+int abm4q(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl, const double *qpe);
 // ASLDA H on 3D lattices in an x-pencil and a yz-plane pass (tdw_aslda.hip):
 // out = H in without the subtraction, <H> into d_qpe; T state-sized scratch,
 // hx the grid fields of aslda_fields, part n*NX doubles
