@@ -220,6 +220,32 @@ int tdw_checkpoint_save(tdw_ctx *c, const char *path, tdw_state *s, double t0, c
 int tdw_checkpoint_load(tdw_ctx *c, const char *path, tdw_state *s, double *t0, double *h_fbeta, double *h_kt,
                         tdw_checkpoint_scalars *sc);
 
+/* Static-solver initial states (inittype 3, pca.c:628-725): the st-wslda-3d
+ * output in <prefix>/ -- wf.info (or s3dpca.info), wf.%04d.info per file,
+ * wf.%04d_{un,vn}.wdat and wf.%04d_en.dat (or s3dpca.%04d.{wfu,wfv,en}),
+ * wf.pud (or s3dpca.pud), raw binary as the static solver writes them.
+ *   tdw_st3d_info -- check_comptibility + read_checkpoint_info_pca +
+ *                    scan_s3dpca_info_files: the run constants and the
+ *                    number of states in each of the iogroups files
+ *   tdw_st3d_load -- read_s3dpca_wf of states [lidx, uidx) into h_wf (host,
+ *                    [u_lidx..u_uidx-1, v_lidx..v_uidx-1]), their
+ *                    occupations f(E) into h_fbeta, the 12*N potentials into
+ *                    h_pots (may be NULL) and mu, ec, kF, eF, beta and Effg
+ *                    (0.6 N eF, N = N_a + N_b with spinsymmetry == 1, else
+ *                    N_a, as pca.c:716-718) into sc; TDW_EINVAL when the
+ *                    files' lattice is not p's.
+ * Return codes: the reference readers' (negative from the wave-function
+ * reader, 10002 / 10006 / 10007 / 10030-10033 from the info files). */
+typedef struct {
+    int nwf, nx, ny, nz;
+    double dx, dy, dz, kF, mu_a, mu_b, ec, beta;
+    int new_format;          /* 1: wf.* names, 0: s3dpca.* names */
+} tdw_st_info;
+int tdw_st3d_info(const char *prefix, int iogroups, tdw_st_info *info, int *nwf_per_file /* iogroups */);
+int tdw_st3d_load(const tdw_params *p, const char *prefix, int iogroups, const int *nwf_per_file,
+                  const tdw_st_info *info, int lidx, int uidx, int spinsymmetry, tdw_complex *h_wf,
+                  double *h_fbeta, double *h_pots, tdw_checkpoint_scalars *sc);
+
 /* W-data measurement output (lib/wdata/doc/README.md): <outprefix>.wtxt and
  * <outprefix>_<var>.wdat, one datablock appended per measurement.
  *   tdw_wdata_create   -- create_wdata_metadata (wslda_writevars.c:109-253):
@@ -411,6 +437,15 @@ int taylor_expansion_contribution(int it, double dt, int n, tdw_complex *wf_hpsi
                                   tdw_complex *wf_update, tdw_complex *wf_contr,
                                   int nthreads);                     /* pca_kernels.cu:1076 */
 int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kernels.cu:1106 (host data) */
+/* st-wslda-3d readers of the driver (host; pca_io.h) */
+int check_comptibility(char *prefix, int requested_dim, char *suffix);          /* pca_io.h:382 */
+int read_checkpoint_info_pca(const char *file_name, int *nwf, int *nx, int *ny, int *nz, double *dx,
+                             double *dy, double *dz, double *kF, double *mu, double *ec,
+                             double *beta);                                   /* pca_io.h:340 */
+int scan_s3dpca_info_files(const char *prefix, int number_of_files, int *nwf,
+                           int *nwf_per_file);                                /* pca_io.h:1519 */
+int read_s3dpca_wf(const char *prefix, int number_of_files, int *nwf_per_file, int mylidx, int myuidx,
+                   tdw_complex *h_wavefun, double *h_fbetaEn);                /* pca_io.h:1562, N of the bound context */
 int get_delta_ext(int datadim, int it, void *deltain, void *data);  /* pca_kernels.cu:1135 (device in, host out) */
 int get_velocity_ext(int datadim, int spin, int it, double *data); /* pca_kernels.cu:1166 (host data, 3 blocks) */
 

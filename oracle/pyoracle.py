@@ -158,20 +158,22 @@ class Problem:
         self.N = N
         n = wf.shape[0] // 2 if wf.ndim == 2 else wf.size // (2 * N)
         self.n = n
-        self.wf = np.ascontiguousarray(wf, dtype=np.complex128).reshape(2 * n, N)
+        # own copies: the oracle updates its arrays in place, and callers
+        # start GPU runs from the same inputs
+        self.wf = np.array(wf, dtype=np.complex128, copy=True).reshape(2 * n, N)
         self.lap = np.zeros_like(self.wf)
         self.ddx = np.zeros_like(self.wf)
         self.ddy = np.zeros_like(self.wf)
         self.ddz = np.zeros_like(self.wf)
         self.f = [np.zeros_like(self.wf) for _ in range(5)]
-        self.fbeta = np.ascontiguousarray(fbeta, dtype=np.float64)
-        self.kt = None if kt is None else np.ascontiguousarray(kt, dtype=np.float64)
-        self.cnt = None if cnt is None else np.ascontiguousarray(cnt, dtype=np.int32)
+        self.fbeta = np.array(fbeta, dtype=np.float64, copy=True)
+        self.kt = None if kt is None else np.array(kt, dtype=np.float64, copy=True)
+        self.cnt = None if cnt is None else np.array(cnt, dtype=np.int32, copy=True)
         self.dens = np.zeros(12 * N)
         if pots is None:
             pots = np.zeros(12 * N)
             pots[4 * N:6 * N] = 1.0
-        self.pots = np.ascontiguousarray(pots, dtype=np.float64)
+        self.pots = np.array(pots, dtype=np.float64, copy=True)
         self.qpe = np.zeros(n)
         self.st = State()
         self.st.n = n

@@ -71,6 +71,13 @@ class CheckpointScalars(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("mu_a", "mu_b", "ec", "kF", "eF", "Effg", "beta")]
 
 
+class StInfo(C.Structure):
+    """tdw_st_info: the run constants of a static-solver (st-wslda-3d) output."""
+    _fields_ = [("nwf", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int)] + \
+               [(k, C.c_double) for k in ("dx", "dy", "dz", "kF", "mu_a", "mu_b", "ec", "beta")] + \
+               [("new_format", C.c_int)]
+
+
 class TapeStep(C.Structure):
     _fields_ = [("n", C.c_int), ("scal", C.c_void_p), ("dens_p", C.c_void_p), ("pots_p", C.c_void_p),
                 ("dens_c", C.c_void_p), ("pots_c", C.c_void_p)]
@@ -211,6 +218,13 @@ def _sigs():
         "get_v_ext": (I, [I, I, I, P]),
         "get_delta_ext": (I, [I, I, P, P]),
         "get_velocity_ext": (I, [I, I, I, P]),
+        "tdw_st3d_info": (I, [C.c_char_p, I, C.POINTER(StInfo), P]),
+        "tdw_st3d_load": (I, [C.POINTER(Params), C.c_char_p, I, P, C.POINTER(StInfo), I, I, I, P, P, P,
+                              C.POINTER(CheckpointScalars)]),
+        "check_comptibility": (I, [C.c_char_p, I, C.c_char_p]),
+        "read_checkpoint_info_pca": (I, [C.c_char_p, P, P, P, P, P, P, P, P, P, P, P]),
+        "scan_s3dpca_info_files": (I, [C.c_char_p, I, P, P]),
+        "read_s3dpca_wf": (I, [C.c_char_p, I, P, I, I, P, P]),
     }
 
 
@@ -404,6 +418,37 @@ def uniform_solve(params, n0_a, n0_b, kc, T, eps=1e-9, scmix=0.5, muchange=0.5, 
     if rc < 0:
         raise TDWError("tdw_uniform_solve failed with code %d" % rc)
     return u, rc
+
+
+def st3d_info(prefix, iogroups):
+    """Run constants and states per file of a static-solver 3D output
+    (tdw_st3d_info); returns (StInfo, nwf_per_file)."""
+    info = StInfo()
+    per = np.zeros(iogroups, dtype=np.int32)
+    rc = lib().tdw_st3d_info(prefix.encode(), iogroups, C.byref(info), per.ctypes.data_as(C.c_void_p))
+    if rc != 0:
+        raise TDWError("tdw_st3d_info failed with code %d" % rc)
+    return info, per
+
+
+def st3d_load(params, prefix, iogroups, info, per, lidx=0, uidx=None, spinsymmetry=0, potentials=True):
+    """States [lidx, uidx) of a static-solver 3D output (tdw_st3d_load):
+    returns wf [2n, N] (complex128, u rows then v rows), their occupations
+    f(E), the 12*N potentials (or None) and the run scalars."""
+    uidx = info.nwf if uidx is None else uidx
+    n = uidx - lidx
+    N = params.nx * params.ny * params.nz
+    wf = np.zeros((2 * n, N), dtype=np.complex128)
+    fb = np.zeros(n)
+    pots = np.zeros(12 * N) if potentials else None
+    sc = CheckpointScalars()
+    rc = lib().tdw_st3d_load(C.byref(params), prefix.encode(), iogroups, per.ctypes.data_as(C.c_void_p),
+                             C.byref(info), lidx, uidx, spinsymmetry, wf.ctypes.data_as(C.c_void_p),
+                             fb.ctypes.data_as(C.c_void_p),
+                             pots.ctypes.data_as(C.c_void_p) if pots is not None else None, C.byref(sc))
+    if rc != 0:
+        raise TDWError("tdw_st3d_load failed with code %d" % rc)
+    return wf, fb, pots, sc
 
 
 def uniform_states(params, u):
