@@ -104,6 +104,9 @@ typedef struct {
     double sclgth;       /* memcopy_const_sclgth(), BdG scattering length      */
     double p_nmin, p_nmax, denseps; /* stabilisation densities, DENSEPSILON   */
     tdw_vext vext;
+    int spinsymmetry;    /* 1: spin-symmetric mode (SPINSYMMETRY_MODE): only
+                            E > 0 states are kept; densities sum both
+                            components into spin b and copy it to a       */
     int precision;       /* TDW_FP64 or TDW_SPLIT                              */
 } tdw_params;
 
@@ -436,6 +439,7 @@ int abm_step4(int n, tdw_complex *ykm1_in, tdw_complex *ykm1_out, tdw_complex *f
 int taylor_expansion_contribution(int it, double dt, int n, tdw_complex *wf_hpsi,
                                   tdw_complex *wf_update, tdw_complex *wf_contr,
                                   int nthreads);                     /* pca_kernels.cu:1076 */
+int symmetrize_densities_device(double *d_densities);                /* pca_dens.cu:343 */
 int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kernels.cu:1106 (host data) */
 /* st-wslda-3d readers of the driver (host; pca_io.h) */
 int check_comptibility(char *prefix, int requested_dim, char *suffix);          /* pca_io.h:382 */

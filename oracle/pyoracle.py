@@ -38,7 +38,7 @@ class Cfg(C.Structure):
                 ("mu_a", C.c_double), ("mu_b", C.c_double), ("ec", C.c_double),
                 ("dt", C.c_double), ("t0", C.c_double), ("sclgth", C.c_double),
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
-                ("vext", Vext)]
+                ("vext", Vext), ("spinsymmetry", C.c_int)]
 
 
 class Uniform(C.Structure):
@@ -115,7 +115,7 @@ def ptr(a):
 def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
              regularization=SPHERICAL, mf_correction=0, scheme=AB4AM5,
              mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
-             p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None):
+             p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, spinsymmetry=0):
     c = Cfg()
     c.dim, c.nx, c.ny, c.nz = dim, nx, ny, nz
     c.dx = dx
@@ -126,6 +126,7 @@ def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
     c.p_nmin, c.p_nmax, c.denseps = p_nmin, p_nmax, denseps
     if vext is not None:
         c.vext = vext
+    c.spinsymmetry = spinsymmetry
     return c
 
 

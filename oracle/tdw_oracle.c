@@ -467,6 +467,30 @@ void tdwo_densities(const tdwo_cfg *c, int n, const double *wf,
             }
             (void)piy;
             zc u = W[(size_t)a * N + i], v = W[((size_t)n + a) * N + i];
+            if (c->spinsymmetry) {
+                /* SPINSYMMETRY_MODE (pca_dens.cu:55-58, 71-90): both
+                 * components go to spin b, nu without the 1/2 */
+                nb += (nrm2(v) * fbm + nrm2(u) * fb) * w;
+                nuu += u * conj(v) * (fbm - fb) * w;
+                if (gradients) {
+                    zc ux = DX[(size_t)a * N + i], vx = DX[((size_t)n + a) * N + i];
+                    zc lu = L[(size_t)a * N + i], lv = L[((size_t)n + a) * N + i];
+                    tb += creal(conj(u) * (lu - u * k2t)) * fb * w + creal(conj(v) * (lv - v * k2t)) * fbm * w;
+                    jbx += cimag(conj(u) * ux) * fb * w;
+                    jbx -= cimag(conj(v) * vx) * fbm * w;
+                    if (c->dim >= 2) {
+                        zc uy = DY[(size_t)a * N + i], vy = DY[((size_t)n + a) * N + i];
+                        jby += cimag(conj(u) * uy) * fb * w;
+                        jby -= cimag(conj(v) * vy) * fbm * w;
+                    }
+                    if (c->dim == 3) {
+                        zc uz = DZ[(size_t)a * N + i], vz = DZ[((size_t)n + a) * N + i];
+                        jbz += cimag(conj(u) * uz) * fb * w;
+                        jbz -= cimag(conj(v) * vz) * fbm * w;
+                    }
+                }
+                continue;
+            }
             if (c->dim == 3) {
                 na += nrm2(u) * fb;
                 nb += nrm2(v) * fbm;
@@ -506,6 +530,13 @@ void tdwo_densities(const tdwo_cfg *c, int n, const double *wf,
                 }
             }
         }
+        if (c->spinsymmetry) {
+            na = nb;
+            ta = tb;
+            jax = jbx;
+            jay = jby;
+            jaz = jbz;
+        }
         double sc = DENS_FACTOR_M;
         rho_a[i] = na / sc / tr;
         rho_b[i] = nb / sc / tr;
@@ -525,6 +556,11 @@ void tdwo_density_tau(const tdwo_cfg *c, double *dens)
     double *rho[2] = {dens + 7 * N, dens + 2 * N};
     double *tau[2] = {dens + 8 * N, dens + 3 * N};
     for (int s = 0; s < 2; s++) {
+        if (s == 1 && c->spinsymmetry) {
+            /* SPINSYMMETRY_MODE: tau_a is a copy of tau_b (pca_dens.cu:329-331) */
+            memcpy(tau[1], tau[0], sizeof(double) * N);
+            break;
+        }
         tdwo_laplace_real(c, rho[s], lr);
         for (size_t i = 0; i < N; i++) {
             double t = 0.5 * lr[i] - tau[s][i];
@@ -1261,7 +1297,19 @@ static int uniform_solve_bdg(const tdwo_cfg *c, double n0_a, double n0_b, double
                 for (int br = 0; br < 2; br++) {
                     double ek, uk, vk;
                     bdg_branch(eta_a, eta_b, delta, br, &ek, &uk, &vk);
-                    if (ek > -ec && ek < ec) {
+                    if (c->spinsymmetry) {
+                        /* spin-symmetric mode keeps the E > 0 half of the
+                         * spectrum and folds spin a into b (pca_uniform.h:325-337) */
+                        if (ek > 0.0 && ek < ec) {
+                            double fp = tdwo_fbeta(ek, beta), fm = tdwo_fbeta(-1.0 * ek, beta);
+                            n_b += vk * vk * fm + uk * uk * fp;
+                            tau_b += kk2[ixyz] * (vk * vk * fm + uk * uk * fp);
+                            nu += uk * vk * (fm - fp) * 2.0;
+                            if (fp > 1.0e-16) S -= 2. * fp * log(fp);
+                            if (fm > 1.0e-16) S -= 2. * fm * log(fm);
+                            nwf++;
+                        }
+                    } else if (ek > -ec && ek < ec) {
                         double fp = tdwo_fbeta(ek, beta), fm = tdwo_fbeta(-1.0 * ek, beta);
                         n_a += uk * uk * fp;
                         tau_a += kk2[ixyz] * uk * uk * fp;
@@ -1274,7 +1322,11 @@ static int uniform_solve_bdg(const tdwo_cfg *c, double n0_a, double n0_b, double
                     }
                 }
             }
-            n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            if (c->spinsymmetry) {  /* pca_uniform.h:397-402 */
+                n_b /= LXYZ; n_a = n_b; tau_b /= LXYZ; tau_a = tau_b; nu /= 2.0 * LXYZ;
+            } else {
+                n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            }
             int conv = 1;
             if (fabs(n0_a - n_a) > eps) conv = 0;
             if (fabs(n0_b - n_b) > eps) conv = 0;
@@ -1286,6 +1338,7 @@ static int uniform_solve_bdg(const tdwo_cfg *c, double n0_a, double n0_b, double
             nu = (1. - scmix) * nu_old + scmix * nu;
             mu_a += muchange * (n0_a - n_a);
             mu_b += muchange * (n0_b - n_b);
+            if (c->spinsymmetry) mu_b = mu_a;  /* pca_uniform.h:1432 */
         }
         eF_a = pow(6.0 * M_PI * M_PI * n_a, 2.0 / 3.0) / 2.0;
         eF_b = pow(6.0 * M_PI * M_PI * n_b, 2.0 / 3.0) / 2.0;
@@ -1388,7 +1441,19 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
                 for (int br = 0; br < 2; br++) {
                     double ek, uk, vk;
                     bdg_branch(eta_a, eta_b, delta, br, &ek, &uk, &vk);
-                    if (ek > -ec && ek < ec) {
+                    if (c->spinsymmetry) {
+                        /* spin-symmetric mode keeps the E > 0 half of the
+                         * spectrum and folds spin a into b (pca_uniform.h:325-337) */
+                        if (ek > 0.0 && ek < ec) {
+                            double fp = tdwo_fbeta(ek, beta), fm = tdwo_fbeta(-1.0 * ek, beta);
+                            n_b += vk * vk * fm + uk * uk * fp;
+                            tau_b += kk2[ixyz] * (vk * vk * fm + uk * uk * fp);
+                            nu += uk * vk * (fm - fp) * 2.0;
+                            if (fp > 1.0e-16) S -= 2. * fp * log(fp);
+                            if (fm > 1.0e-16) S -= 2. * fm * log(fm);
+                            nwf++;
+                        }
+                    } else if (ek > -ec && ek < ec) {
                         double fp = tdwo_fbeta(ek, beta), fm = tdwo_fbeta(-1.0 * ek, beta);
                         n_a += uk * uk * fp;
                         tau_a += kk2[ixyz] * uk * uk * fp;
@@ -1401,7 +1466,11 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
                     }
                 }
             }
-            n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            if (c->spinsymmetry) {  /* pca_uniform.h:397-402 */
+                n_b /= LXYZ; n_a = n_b; tau_b /= LXYZ; tau_a = tau_b; nu /= 2.0 * LXYZ;
+            } else {
+                n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            }
             int conv = 1;
             if (fabs(n0_a - n_a) > eps) conv = 0;
             if (fabs(n0_b - n_b) > eps) conv = 0;
@@ -1418,6 +1487,7 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
                 mu_a += muchange * (n0_a - n_a) / n0_a * eF_a;
                 mu_b += muchange * (n0_b - n_b) / n0_b * eF_b;
             }
+            if (c->spinsymmetry) mu_b = mu_a;  /* pca_uniform.h:429 */
         }
         eF_a = pow(6.0 * M_PI * M_PI * n_a, 2.0 / 3.0) / 2.0;
         eF_b = pow(6.0 * M_PI * M_PI * n_b, 2.0 / 3.0) / 2.0;
@@ -1489,7 +1559,9 @@ static int uniform_wf_generic(const tdwo_cfg *c, const tdwo_uniform *U, int idxf
         for (int br = 0; br < 2; br++) {
             double ek, uk, vk;
             bdg_branch(eta_a, eta_b, U->delta, br, &ek, &uk, &vk);
-            if (!(ek > -U->ec && ek < U->ec)) continue;
+            /* spin-symmetric mode: E > 0 states only (pca_uniform.h:727-733) */
+            double elo = c->spinsymmetry ? 0.0 : -U->ec;
+            if (!(ek > elo && ek < U->ec)) continue;
             nwf++;
             if (countonly || nwf < idxfrom || nwf >= idxto) continue;
             int l = nwf - idxfrom;

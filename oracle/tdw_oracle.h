@@ -31,6 +31,13 @@
  *     digits.  The 2.5e-6 band is systematic (a 1e-13 perturbation of the
  *     start moves it by nothing comparable) and its source is not identified:
  *     ASLDA parity at rows 8-100 is pinned to that tolerance only.
+ *   - spin-symmetric mode (spinsymmetry = 1): no golden of the reference
+ *     runs it with a functional built here (its runs are SLDAE).  Pinned by
+ *     the partner-state identity: the half-spectrum densities of any states
+ *     (u, v, f) equal the ordinary densities of {(u, v, f), (v*, -u*, 1-f)}
+ *     in every field (1D, 2D, 3D, with and without gradients, and tau), and
+ *     the balanced spin-symmetric uniform solution (SLDA, ASLDA, BdG) and its
+ *     E > 0 plane waves equal the ordinary balanced solution and spectrum.
  */
 #ifndef TDW_ORACLE_H
 #define TDW_ORACLE_H
@@ -96,6 +103,7 @@ typedef struct {
     double sclgth;        /* BdG scattering length (memcopy_const_sclgth)  */
     double p_nmin, p_nmax, denseps;
     tdwo_vext vext;
+    int spinsymmetry;     /* 1: SPINSYMMETRY_MODE (E > 0 states only)      */
 } tdwo_cfg;
 
 /* Number of evolved lattice points (NUMBER_ELEMENT). */

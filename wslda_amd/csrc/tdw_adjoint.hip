@@ -638,7 +638,7 @@ static hist_dt_t hist_dt_fn(int h, int kp)
 
 static int adj_supported(const tdw_ctx *c)
 {
-    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64;
+    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64 && !c->p.spinsymmetry;
 }
 
 }  // namespace tdw

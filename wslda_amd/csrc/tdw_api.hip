@@ -647,6 +647,15 @@ extern "C" int density_caculate_tau(double *dens, int nthreads)
     (void)nthreads;
     return density_tau(c, dens);
 }
+// rho_b, tau_b, j_b <- rho_a, tau_a, j_a (pca_dens.cu:343-359: 5 N doubles
+// from d_densities + 2N to d_densities + 7N)
+extern "C" int symmetrize_densities_device(double *dens)
+{
+    BOUND();
+    if (!dens) return TDW_EINVAL;
+    const long N = (long)c->N;
+    return copy_f64(c, dens + 2 * N, dens + 7 * N, 5 * N);
+}
 extern "C" int compute_potentials(int it, double *dens, double *pots, double cccoeff, int nthreads)
 {
     BOUND();

@@ -633,7 +633,8 @@ int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, c
                        c->nx, G, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, pyz_part);
     hipLaunchKernelGGL(dg_final, dim3((unsigned)((12 * N + AT - 1) / AT)), dim3(AT), 0, c->stream, px_part, pyz_part,
                        nchunk, N, dens_transverse(c), dens);
-    return (int)hipGetLastError();
+    TDW_TRY((int)hipGetLastError());
+    return spin_fold(c, dens);
 }
 
 bool aslda_h_fused_supported(const tdw_ctx *c)

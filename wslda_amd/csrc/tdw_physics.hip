@@ -278,6 +278,46 @@ k_dens_final(const double *__restrict__ part, int nchunk, long N, double tr, dou
     dens[11 * N + i] = GRAD ? s[11] / sc / tr : 0.0;
 }
 
+// spin-symmetric mode (SPINSYMMETRY_MODE, pca_dens.cu:55-58, 71-90, 112-118):
+// the reference accumulates both components of every state into spin b, nu
+// without the factor 1/2, and copies b to a.  The partial sums above keep the
+// u and v contributions apart, so the fold is b = a + b for rho, tau_acc and
+// j, nu *= 2 (exact), then a = b.
+__global__ void k_spin_fold(double *__restrict__ dens, long N)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    dens[2 * i] *= 2.0;
+    dens[2 * i + 1] *= 2.0;
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+        const double v = dens[(2 + f) * N + i] + dens[(7 + f) * N + i];
+        dens[(2 + f) * N + i] = v;
+        dens[(7 + f) * N + i] = v;
+    }
+}
+
+int spin_fold(tdw_ctx *c, double *dens)
+{
+    if (!c->p.spinsymmetry) return 0;
+    const long N = (long)c->N;
+    hipLaunchKernelGGL(k_spin_fold, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, dens, N);
+    return (int)hipGetLastError();
+}
+
+// a <- b copy of n doubles (symmetrize_densities_device, tau in spin-symmetric mode)
+__global__ void k_copy_f64(const double *__restrict__ src, double *__restrict__ dst, long n)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+int copy_f64(tdw_ctx *c, const double *src, double *dst, long n)
+{
+    hipLaunchKernelGGL(k_copy_f64, dim3((n + NT - 1) / NT), dim3(NT), 0, c->stream, src, dst, n);
+    return (int)hipGetLastError();
+}
+
 template <typename W>
 static int densities_t(tdw_ctx *c, int n, const W *wf, const double2 *ddx, const double2 *ddy,
                        const double2 *ddz, const double2 *lap, const double *fbeta, const double *wgt,
@@ -305,7 +345,8 @@ static int densities_t(tdw_ctx *c, int n, const W *wf, const double2 *ddx, const
     double tr = dens_transverse(c);
     if (grad) hipLaunchKernelGGL(k_dens_final<true>, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, tr, dens);
     else hipLaunchKernelGGL(k_dens_final<false>, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, tr, dens);
-    return (int)hipGetLastError();
+    TDW_TRY((int)hipGetLastError());
+    return spin_fold(c, dens);
 }
 
 int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const double2 *ddy,
@@ -365,9 +406,11 @@ int density_tau(tdw_ctx *c, double *dens)
     TimingOff quiet(c);
     long N = (long)c->N;
     TDW_TRY(ensure_T(c, (size_t)N));
-    // b component first, then a (pca_dens.cu:317-333)
+    // b component first, then a (pca_dens.cu:317-333); in spin-symmetric
+    // mode tau_a is a copy of tau_b (pca_dens.cu:329-331)
     TDW_TRY(laplace_real_c(c, dens + 7 * N, c->d_T));
     hipLaunchKernelGGL(k_tau, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, c->d_T, dens + 8 * N, N);
+    if (c->p.spinsymmetry) return copy_f64(c, dens + 8 * N, dens + 3 * N, N);
     TDW_TRY(laplace_real_c(c, dens + 2 * N, c->d_T));
     hipLaunchKernelGGL(k_tau, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, c->d_T, dens + 3 * N, N);
     return (int)hipGetLastError();

@@ -93,6 +93,10 @@ int densities(tdw_ctx *c, int n, const double2 *wf, const double2 *ddx, const do
 int densities32(tdw_ctx *c, int n, const float2 *wf, const double *fbeta, const double *wgt, const double *kt,
                 const int *cnt, double *dens);
 int density_tau(tdw_ctx *c, double *dens);
+// spin-symmetric mode: fold the u contributions kept in spin a into spin b,
+// double nu, copy b to a (no-op when the mode is off)
+int spin_fold(tdw_ctx *c, double *dens);
+int copy_f64(tdw_ctx *c, const double *src, double *dst, long n);
 int vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 // cccoeff: the current-corrections coefficient of the ASLDA vector
 // potentials and current terms (1 after the reference's default ramp)

@@ -3,7 +3,7 @@
 //
 // Restates hpc-engine/pca_uniform.h: fbeta (:70-78), solve_uniform_problem
 // (:86-502, SLDA with the A0=1 effective mass or ASLDA with the
-// polarization-dependent one of tdw_aslda.hpp, no spin symmetry),
+// polarization-dependent one of tdw_aslda.hpp, with or without spin symmetry),
 // create_uniform_wf (:636-850), create_uniform_wf_2d_generic (:2044-2287),
 // create_uniform_wf_1d_generic (:2321-2560) and the 1D transverse-mode table
 // of wslda_wavevectors.h:21-80.
@@ -113,6 +113,7 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     // BdG (solve_uniform_problem_bdg, pca_uniform.h:1105-1470): bare coupling
     // 4 pi a, no density functional in the mean fields, mu += muchange (n0 - n)
     const bool bdg = c->functional == TDW_BDG;
+    const bool ss = c->spinsymmetry != 0;
     const double gbare = 4.0 * PI * c->sclgth;
     const int NX = c->nx, NY = c->ny, NZ = c->nz;
     const double LX = c->dx * NX, LY = c->dy * NY, LZ = c->dz * NZ, LXYZ = LX * LY * LZ;
@@ -218,7 +219,20 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
                         for (int br = 0; br < 2; br++) {
                             double E, uk, vk;
                             branch(ea, eb, delta, br, &E, &uk, &vk);
-                            if (E > -ec && E < ec) {
+                            if (ss) {
+                                // spin-symmetric mode: the E > 0 half of the
+                                // spectrum, spin a folded into b
+                                // (pca_uniform.h:325-337)
+                                if (E > 0.0 && E < ec) {
+                                    double fp = tdw_fbeta(E, beta), fm = tdw_fbeta(-E, beta);
+                                    n_b += vk * vk * fm + uk * uk * fp;
+                                    tau_b += kk2[ixyz] * (vk * vk * fm + uk * uk * fp);
+                                    nu += uk * vk * (fm - fp) * 2.0;
+                                    if (fp > 1.0e-16) S -= 2. * fp * log(fp);
+                                    if (fm > 1.0e-16) S -= 2. * fm * log(fm);
+                                    nwf++;
+                                }
+                            } else if (E > -ec && E < ec) {
                                 double fp = tdw_fbeta(E, beta), fm = tdw_fbeta(-E, beta);
                                 n_a += uk * uk * fp;
                                 tau_a += kk2[ixyz] * uk * uk * fp;
@@ -231,7 +245,11 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
                             }
                         }
                     }
-            n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            if (ss) {  // pca_uniform.h:397-402
+                n_b /= LXYZ; n_a = n_b; tau_b /= LXYZ; tau_a = tau_b; nu /= 2.0 * LXYZ;
+            } else {
+                n_a /= LXYZ; n_b /= LXYZ; tau_a /= LXYZ; tau_b /= LXYZ; nu /= 2.0 * LXYZ;
+            }
             if (fabs(n0_a - n_a) <= eps && fabs(n0_b - n_b) <= eps && fabs(delta - delta_old) <= eps) break;
             tau_a = (1. - scmix) * tau_a_old + scmix * tau_a;
             tau_b = (1. - scmix) * tau_b_old + scmix * tau_b;
@@ -247,6 +265,7 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
                 mu_a += muchange * (n0_a - n_a) / n0_a * eF_a;
                 mu_b += muchange * (n0_b - n_b) / n0_b * eF_b;
             }
+            if (ss) mu_b = mu_a;  // pca_uniform.h:429, 1432
         }
         eF_a = pow(6.0 * PI * PI * n_a, 2.0 / 3.0) / 2.0;
         eF_b = pow(6.0 * PI * PI * n_b, 2.0 / 3.0) / 2.0;
@@ -327,7 +346,9 @@ int states_generic(const tdw_params *c, const tdw_uniform *U, int idxfrom, int i
                 for (int br = 0; br < 2; br++) {
                     double E, uk, vk;
                     branch(ea, eb, U->delta, br, &E, &uk, &vk);
-                    if (!(E > -U->ec && E < U->ec)) continue;
+                    // spin-symmetric mode: E > 0 states only (pca_uniform.h:727-733)
+                    const double elo = c->spinsymmetry ? 0.0 : -U->ec;
+                    if (!(E > elo && E < U->ec)) continue;
                     nwf++;
                     if (countonly || nwf < idxfrom || nwf >= idxto) continue;
                     const int l = nwf - idxfrom;

@@ -49,7 +49,7 @@ class Params(C.Structure):
                 ("mu_a", C.c_double), ("mu_b", C.c_double), ("ec", C.c_double),
                 ("dt", C.c_double), ("t0", C.c_double), ("sclgth", C.c_double),
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
-                ("vext", Vext), ("precision", C.c_int)]
+                ("vext", Vext), ("spinsymmetry", C.c_int), ("precision", C.c_int)]
 
 
 class Uniform(C.Structure):
@@ -97,7 +97,7 @@ class WDataParams(C.Structure):
 def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
                 regularization=SPHERICAL_CUTOFF, mf_correction=0, scheme=AB4AM5,
                 mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
-                p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, precision=FP64):
+                p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, precision=FP64, spinsymmetry=0):
     p = Params()
     p.dim, p.nx, p.ny, p.nz = dim, nx, ny, nz
     p.dx = dx
@@ -108,6 +108,7 @@ def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
     p.p_nmin, p.p_nmax, p.denseps = p_nmin, p_nmax, denseps
     if vext is not None:
         p.vext = vext
+    p.spinsymmetry = spinsymmetry
     p.precision = precision
     return p
 
@@ -218,6 +219,7 @@ def _sigs():
         "get_v_ext": (I, [I, I, I, P]),
         "get_delta_ext": (I, [I, I, P, P]),
         "get_velocity_ext": (I, [I, I, I, P]),
+        "symmetrize_densities_device": (I, [P]),
         "tdw_st3d_info": (I, [C.c_char_p, I, C.POINTER(StInfo), P]),
         "tdw_st3d_load": (I, [C.POINTER(Params), C.c_char_p, I, P, C.POINTER(StInfo), I, I, I, P, P, P,
                               C.POINTER(CheckpointScalars)]),
