@@ -397,6 +397,23 @@ int compute_derivatives(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_comple
                         tdw_complex *wf_d_dz, tdw_complex *wf_laplace, int nthreads); /* pca_derivative.cu:217 */
 int compute_laplace(int n, tdw_complex *wf, tdw_complex *wf_laplace, int nthreads); /* pca_derivative.cu:332 */
 int compute_laplace_real_f(double *f, double *laplace_f, int nthreads);            /* pca_derivative.cu:640 */
+/* Real-field derivatives along the evolved axes (1D: x only, 2D: x, y; the
+ * other arguments are ignored, as in ccpca_ / cpca_derivative.cu); the
+ * Nyquist derivative is zero; outputs may alias inputs. */
+int compute_gradient_real_f(double *f, double *df_dx, double *df_dy, double *df_dz,
+                            int nthreads);                           /* pca_derivative.cu:451 */
+int compute_derivative_real_vector_f(double *fx, double *fy, double *fz, double *dfx_dx,
+                                     double *dfy_dy, double *dfz_dz,
+                                     int nthreads);                  /* pca_derivative.cu:549 */
+int compute_divergence_real_vector_f(double *fx, double *fy, double *fz, double *divf,
+                                     int nthreads);                  /* pca_derivative.cu:735 */
+/* out = F^-1[FD(k^2/2) F[in]], FD(e) = 1/(exp((e - fd_mu)/fd_T) + 1); out may be in */
+int high_frequency_filter_d(double *in, double *out, double fd_mu, double fd_T,
+                            int nthreads);                           /* pca_derivative.cu:829 */
+int high_frequency_filter_c(tdw_complex *in, tdw_complex *out, double fd_mu, double fd_T,
+                            int nthreads);                           /* pca_derivative.cu:906 */
+int high_frequency_filter_massive_d(int n, double *in, double *out, double fd_mu, double fd_T,
+                                    int nthreads);                   /* pca_derivative.cu:950 */
 
 int calculate_densities(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_complex *wf_d_dy,
                         tdw_complex *wf_d_dz, tdw_complex *d_wf_laplace, double *d_fbetaEn,
@@ -440,6 +457,27 @@ int taylor_expansion_contribution(int it, double dt, int n, tdw_complex *wf_hpsi
                                   tdw_complex *wf_update, tdw_complex *wf_contr,
                                   int nthreads);                     /* pca_kernels.cu:1076 */
 int symmetrize_densities_device(double *d_densities);                /* pca_dens.cu:343 */
+/* <wf1_a|wf2_a> = dV sum (u1* u2 + v1* v2) for a < n into overlap_re[a] and
+ * overlap_im[a] (either may be NULL; device arrays); workarea is unused.
+ * The reference copies the real part into overlap_im too
+ * (pca_kernels.cu:903-906); here overlap_im is the imaginary part its
+ * documentation names. */
+int compute_ovelap(int n, tdw_complex *wf1, tdw_complex *wf2, double *overlap_re,
+                   double *overlap_im, double *workarea, int nthreads); /* pca_kernels.cu:877 */
+/* sum of size device doubles into partial_sums[0] (mode 0 only) */
+int local_reductionR(double *array, int size, double *partial_sums, int threads,
+                     int mode);                                      /* wslda_cuda_utils.h:250 */
+int memcopy_extra_data(size_t extra_data_size, void *extra_data);   /* wslda_cuda_utils.h:47 */
+/* Views of the work area given to set_workspace_for_cufftPlan:
+ * [kernels 24 n doubles][derivatives 8 n doubles][fft ONE]
+ * (tdwslda_memory_management.cu:31-94) */
+size_t mm_get_size_of_kernels_workspace(size_t n);
+size_t mm_get_size_of_derivatives_workspace(size_t n);
+size_t mm_get_size_of_total_workspace(size_t n);
+void *mm_get_pointer_to_kernels_workspace(size_t n);
+void *mm_get_pointer_to_derivatives_workspace(size_t n);
+void *mm_get_pointer_to_total_workspace(size_t n);
+void *mm_get_pointer_to_fftONE_workspace(size_t n);
 int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kernels.cu:1106 (host data) */
 /* st-wslda-3d readers of the driver (host; pca_io.h) */
 int check_comptibility(char *prefix, int requested_dim, char *suffix);          /* pca_io.h:382 */

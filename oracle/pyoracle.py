@@ -79,6 +79,12 @@ def lib():
             "tdwo_laplace": (None, [cp, I, P, P]),
             "tdwo_derivatives": (None, [cp, I, P, P, P, P, P]),
             "tdwo_laplace_real": (None, [cp, P, P]),
+            "tdwo_gradient_real": (None, [cp, P, P, P, P]),
+            "tdwo_dvec_real": (None, [cp, P, P, P, P, P, P]),
+            "tdwo_div_real": (None, [cp, P, P, P, P]),
+            "tdwo_hf_filter_c": (None, [cp, P, P, D, D]),
+            "tdwo_hf_filter_d": (None, [cp, I, P, P, D, D]),
+            "tdwo_overlap": (None, [cp, I, P, P, P, P]),
             "tdwo_abm_step1": (None, [cp, I, P, P, P, P, P, P]),
             "tdwo_abm_step4": (None, [cp, I, P, P, P]),
             "tdwo_norms": (None, [cp, I, P, P]),

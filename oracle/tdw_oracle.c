@@ -345,6 +345,107 @@ void tdwo_laplace_real(const tdwo_cfg *c, const double *f, double *lap)
     free(z);
 }
 
+/* Real-field derivatives along the evolved axes, Nyquist derivative zero
+ * (pca_derivative.cu:385-488, 490-598, 673-782): the D2Z / Z2D pairs equal
+ * the real part of the complex transform of the real field. */
+static void real_derivs(const tdwo_cfg *c, const double *f, zc *g /* 4N */)
+{
+    size_t N = tdwo_npts(c);
+    zc *z = (zc *)malloc(sizeof(zc) * N);
+    for (size_t i = 0; i < N; i++) z[i] = f[i];
+    tdwo_derivatives(c, 1, (const double *)z, (double *)g, (double *)(g + N), (double *)(g + 2 * N),
+                     (double *)(g + 3 * N));
+    free(z);
+}
+
+void tdwo_gradient_real(const tdwo_cfg *c, const double *f, double *dx, double *dy, double *dz)
+{
+    size_t N = tdwo_npts(c);
+    zc *g = (zc *)malloc(sizeof(zc) * 4 * N);
+    real_derivs(c, f, g);
+    double *out[3] = {dx, dy, dz};
+    for (int d = 0; d < c->dim; d++)
+        for (size_t i = 0; i < N; i++) out[d][i] = creal(g[d * N + i]);
+    free(g);
+}
+
+void tdwo_dvec_real(const tdwo_cfg *c, const double *fx, const double *fy, const double *fz,
+                    double *dfx, double *dfy, double *dfz)
+{
+    size_t N = tdwo_npts(c);
+    zc *g = (zc *)malloc(sizeof(zc) * 4 * N);
+    const double *f[3] = {fx, fy, fz};
+    double *out[3] = {dfx, dfy, dfz};
+    for (int d = 0; d < c->dim; d++) {
+        real_derivs(c, f[d], g);
+        for (size_t i = 0; i < N; i++) out[d][i] = creal(g[d * N + i]);
+    }
+    free(g);
+}
+
+void tdwo_div_real(const tdwo_cfg *c, const double *fx, const double *fy, const double *fz, double *div)
+{
+    size_t N = tdwo_npts(c);
+    zc *g = (zc *)malloc(sizeof(zc) * 4 * N);
+    double *acc = (double *)calloc(N, sizeof(double));
+    const double *f[3] = {fx, fy, fz};
+    for (int d = 0; d < c->dim; d++) {
+        real_derivs(c, f[d], g);
+        for (size_t i = 0; i < N; i++) acc[i] += creal(g[d * N + i]);
+    }
+    memcpy(div, acc, sizeof(double) * N);
+    free(acc);
+    free(g);
+}
+
+/* Fermi-Dirac high-frequency filter of one complex field: F^-1[FD(k^2/2)/N F]
+ * with FD(e) = 1/(exp((e - mu)/T) + 1) (pca_derivative.cu:784-963). */
+void tdwo_hf_filter_c(const tdwo_cfg *c, const double *in, double *out, double mu, double T)
+{
+    size_t N = tdwo_npts(c);
+    zc *z = (zc *)malloc(sizeof(zc) * N);
+    memcpy(z, in, sizeof(zc) * N);
+    prepare_plans(c);
+    fft_one(c, z, -1);
+    for (size_t i = 0; i < N; i++) {
+        double k2n, a, b, d;
+        kfactors(c, i, &k2n, &a, &b, &d);
+        double ek = -0.5 * k2n * (double)N;   /* k^2 / 2 */
+        z[i] *= 1.0 / (exp((ek - mu) / T) + 1.0) / (double)N;
+    }
+    fft_one(c, z, +1);
+    memcpy(out, z, sizeof(zc) * N);
+    free(z);
+}
+
+void tdwo_hf_filter_d(const tdwo_cfg *c, int n, const double *in, double *out, double mu, double T)
+{
+    size_t N = tdwo_npts(c);
+    zc *z = (zc *)malloc(sizeof(zc) * N);
+    for (int a = 0; a < n; a++) {
+        for (size_t i = 0; i < N; i++) z[i] = in[(size_t)a * N + i];
+        tdwo_hf_filter_c(c, (const double *)z, (double *)z, mu, T);
+        for (size_t i = 0; i < N; i++) out[(size_t)a * N + i] = creal(z[i]);
+    }
+    free(z);
+}
+
+/* <w1_a|w2_a> dV = dV sum (conj(u1) u2 + conj(v1) v2) (pca_kernels.cu:853-911) */
+void tdwo_overlap(const tdwo_cfg *c, int n, const double *w1, const double *w2, double *re, double *im)
+{
+    size_t N = tdwo_npts(c);
+    const zc *A = (const zc *)w1, *B = (const zc *)w2;
+    double dv = c->dx * (c->dim >= 2 ? c->dy : 1.0) * (c->dim >= 3 ? c->dz : 1.0);
+    for (int a = 0; a < n; a++) {
+        zc s = 0;
+        for (size_t i = 0; i < N; i++)
+            s += conj(A[(size_t)a * N + i]) * B[(size_t)a * N + i] +
+                 conj(A[((size_t)n + a) * N + i]) * B[((size_t)n + a) * N + i];
+        if (re) re[a] = creal(s) * dv;
+        if (im) im[a] = cimag(s) * dv;
+    }
+}
+
 /* =========================================================================
  * ABM integrator (tdwslda_ode_integrator.h:26-220)
  * ========================================================================= */

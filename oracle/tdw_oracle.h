@@ -31,6 +31,10 @@
  *     digits.  The 2.5e-6 band is systematic (a 1e-13 perturbation of the
  *     start moves it by nothing comparable) and its source is not identified:
  *     ASLDA parity at rows 8-100 is pinned to that tolerance only.
+ *   - grid-field helpers (tdwo_gradient_real, tdwo_dvec_real, tdwo_div_real,
+ *     tdwo_hf_filter_*, tdwo_overlap): closed forms on sums of plane waves
+ *     (exact derivatives below Nyquist, zero at Nyquist, FD(k^2/2) times
+ *     each mode) and a numpy overlap (tests/test_oracle_grid.py).
  *   - spin-symmetric mode (spinsymmetry = 1): no golden of the reference
  *     runs it with a functional built here (its runs are SLDAE).  Pinned by
  *     the partner-state identity: the half-spectrum densities of any states
@@ -121,6 +125,15 @@ void tdwo_derivatives(const tdwo_cfg *c, int m, const double *in,
                       double *ddx, double *ddy, double *ddz, double *lap);
 /* compute_laplace_real_f (pca_derivative.cu:601-668). */
 void tdwo_laplace_real(const tdwo_cfg *c, const double *f, double *lap);
+/* grid-field helpers: gradient, per-component derivative and divergence of
+ * real fields (evolved axes), the Fermi-Dirac filter, state overlaps */
+void tdwo_gradient_real(const tdwo_cfg *c, const double *f, double *dx, double *dy, double *dz);
+void tdwo_dvec_real(const tdwo_cfg *c, const double *fx, const double *fy, const double *fz,
+                    double *dfx, double *dfy, double *dfz);
+void tdwo_div_real(const tdwo_cfg *c, const double *fx, const double *fy, const double *fz, double *div);
+void tdwo_hf_filter_c(const tdwo_cfg *c, const double *in, double *out, double mu, double T);
+void tdwo_hf_filter_d(const tdwo_cfg *c, int n, const double *in, double *out, double mu, double T);
+void tdwo_overlap(const tdwo_cfg *c, int n, const double *w1, const double *w2, double *re, double *im);
 
 /* --- ODE integrator (tdwslda_ode_integrator.h) ---------------------------- */
 void tdwo_abm_step1(const tdwo_cfg *c, int n, double *y, double *f1, double *f2,

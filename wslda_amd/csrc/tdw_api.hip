@@ -217,7 +217,8 @@ extern "C" int tdw_destroy(tdw_ctx *c)
         if (c->d_tw32[k]) hipFree(c->d_tw32[k]);
     }
     void *bufs[] = {c->d_fbeta, c->d_kt, c->d_cnt, c->d_part, c->d_nrm, c->d_qpe, c->d_esum,
-                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale, c->d_adjv, c->d_hx, c->d_ascr, c->d_aw};
+                    c->d_T, c->d_rtmp, c->d_vext, c->d_scale, c->d_adjv, c->d_hx, c->d_ascr, c->d_aw,
+                    c->d_gscr};
     for (void *b : bufs)
         if (b) hipFree(b);
     fused_release(c);
@@ -646,6 +647,116 @@ extern "C" int density_caculate_tau(double *dens, int nthreads)
     BOUND();
     (void)nthreads;
     return density_tau(c, dens);
+}
+// Grid-field helpers of the derivative layer (pca_derivative.cu:385-963 and
+// the 1D / 2D twins): only the evolved axes' arguments are used, as in the
+// 1D / 2D reference files; outputs may alias inputs.
+extern "C" int compute_gradient_real_f(double *f, double *df_dx, double *df_dy, double *df_dz, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    double *const df[3] = {df_dx, df_dy, df_dz};
+    return grid_gradient(c, f, df);
+}
+extern "C" int compute_derivative_real_vector_f(double *fx, double *fy, double *fz, double *dfx_dx, double *dfy_dy,
+                                                double *dfz_dz, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    const double *const f[3] = {fx, fy, fz};
+    double *const df[3] = {dfx_dx, dfy_dy, dfz_dz};
+    return grid_dvec(c, f, df);
+}
+extern "C" int compute_divergence_real_vector_f(double *fx, double *fy, double *fz, double *divf, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    const double *const f[3] = {fx, fy, fz};
+    return grid_div(c, f, divf);
+}
+extern "C" int high_frequency_filter_d(double *in, double *out, double fd_mu, double fd_T, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (!in || !out) return TDW_EINVAL;
+    return grid_filter_d(c, 1, in, out, fd_mu, fd_T);
+}
+extern "C" int high_frequency_filter_c(tdw_complex *in, tdw_complex *out, double fd_mu, double fd_T, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (!in || !out) return TDW_EINVAL;
+    return grid_filter_c(c, CD2(in), D2(out), fd_mu, fd_T);
+}
+extern "C" int high_frequency_filter_massive_d(int n, double *in, double *out, double fd_mu, double fd_T,
+                                               int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (n < 0 || ((!in || !out) && n > 0)) return TDW_EINVAL;
+    return grid_filter_d(c, n, in, out, fd_mu, fd_T);
+}
+// overlaps of state pairs (pca_kernels.cu:877-911); workarea is not needed
+// (the sums run in the context), overlap_im receives the imaginary parts
+extern "C" int compute_ovelap(int n, tdw_complex *wf1, tdw_complex *wf2, double *overlap_re, double *overlap_im,
+                              double *workarea, int nthreads)
+{
+    BOUND();
+    (void)workarea;
+    (void)nthreads;
+    if (n < 0 || n > c->nwf || ((!wf1 || !wf2) && n > 0)) return TDW_EINVAL;
+    return grid_overlap(c, n, CD2(wf1), CD2(wf2), overlap_re, overlap_im);
+}
+// sum of size doubles into partial_sums[0] (wslda_cuda_utils.h:250-278)
+extern "C" int local_reductionR(double *array, int size, double *partial_sums, int threads, int mode)
+{
+    BOUND();
+    (void)threads;
+    if (mode != 0) return TDW_EINVAL;   // the reference defines mode 0 (sum) only
+    return grid_sum(c, array, (long)size, partial_sums);
+}
+// the caller's block of extra device data for its own code
+// (wslda_cuda_utils.h:47-52); kept on the bound context
+extern "C" int memcopy_extra_data(size_t extra_data_size, void *extra_data)
+{
+    BOUND();
+    c->extra_data = extra_data;
+    c->extra_size = extra_data_size;
+    return 0;
+}
+// workspace views of the work area given to set_workspace_for_cufftPlan
+// (tdwslda_memory_management.cu:31-94): [kernels 24 n doubles]
+// [derivatives 8 n doubles][fft ONE]; the caller allocates at least
+// mm_get_size_of_total_workspace(N) bytes, as pca.c:99, 928-930 does
+extern "C" size_t mm_get_size_of_kernels_workspace(size_t n) { return sizeof(double) * n * 24; }
+extern "C" size_t mm_get_size_of_derivatives_workspace(size_t n) { return sizeof(double) * n * 8; }
+extern "C" size_t mm_get_size_of_total_workspace(size_t n)
+{
+    return mm_get_size_of_derivatives_workspace(n) + mm_get_size_of_kernels_workspace(n);
+}
+static double *workarea_of_bound()
+{
+    return g_bound ? (double *)g_bound->ref_workarea : nullptr;
+}
+extern "C" void *mm_get_pointer_to_kernels_workspace(size_t n)
+{
+    (void)n;
+    return workarea_of_bound();
+}
+extern "C" void *mm_get_pointer_to_derivatives_workspace(size_t n)
+{
+    double *w = workarea_of_bound();
+    return w ? (void *)(w + 24 * n) : nullptr;
+}
+extern "C" void *mm_get_pointer_to_total_workspace(size_t n)
+{
+    (void)n;
+    return workarea_of_bound();
+}
+extern "C" void *mm_get_pointer_to_fftONE_workspace(size_t n)
+{
+    double *w = workarea_of_bound();
+    return w ? (void *)(w + 32 * n) : nullptr;
 }
 // rho_b, tau_b, j_b <- rho_a, tau_a, j_a (pca_dens.cu:343-359: 5 N doubles
 // from d_densities + 2N to d_densities + 7N)

@@ -62,6 +62,11 @@ struct tdw_ctx {
     double *d_hx;                        // 4N: lap alpha_a, lap alpha_b, div A_a, div A_b
     double2 *d_ascr;                     // 4N complex: derivative scratch of the real fields
     double2 *d_aw;   size_t aw_cap;      // lap(alpha psi) (complex elements)
+    // grid-field helpers (tdw_grid.hip, allocated on first use)
+    double2 *d_gscr;                     // 6N complex
+    // memcopy_extra_data(): the caller's device block, kept for its own code
+    void *extra_data;
+    size_t extra_size;
     // per-kernel timing
     int timing;
     std::vector<tdw_timer_rec> *timers;  // pending records

@@ -97,6 +97,17 @@ int density_tau(tdw_ctx *c, double *dens);
 // double nu, copy b to a (no-op when the mode is off)
 int spin_fold(tdw_ctx *c, double *dens);
 int copy_f64(tdw_ctx *c, const double *src, double *dst, long n);
+// grid-field helpers (tdw_grid.hip): spectral gradient, per-component
+// derivative and divergence of real fields (evolved axes only), the
+// Fermi-Dirac filter FD(k^2/2) of n real fields or one complex field, and the
+// per-state overlaps <w1_a | w2_a> dV
+int grid_gradient(tdw_ctx *c, const double *f, double *const df[3]);
+int grid_dvec(tdw_ctx *c, const double *const f[3], double *const df[3]);
+int grid_div(tdw_ctx *c, const double *const f[3], double *div);
+int grid_filter_d(tdw_ctx *c, int n, const double *in, double *out, double mu, double T);
+int grid_filter_c(tdw_ctx *c, const double2 *in, double2 *out, double mu, double T);
+int grid_overlap(tdw_ctx *c, int n, const double2 *w1, const double2 *w2, double *re, double *im);
+int grid_sum(tdw_ctx *c, const double *a, long n, double *out);
 int vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 // cccoeff: the current-corrections coefficient of the ASLDA vector
 // potentials and current terms (1 after the reference's default ramp)

@@ -220,6 +220,22 @@ def _sigs():
         "get_delta_ext": (I, [I, I, P, P]),
         "get_velocity_ext": (I, [I, I, I, P]),
         "symmetrize_densities_device": (I, [P]),
+        "compute_gradient_real_f": (I, [P, P, P, P, I]),
+        "compute_derivative_real_vector_f": (I, [P, P, P, P, P, P, I]),
+        "compute_divergence_real_vector_f": (I, [P, P, P, P, I]),
+        "high_frequency_filter_d": (I, [P, P, D, D, I]),
+        "high_frequency_filter_c": (I, [P, P, D, D, I]),
+        "high_frequency_filter_massive_d": (I, [I, P, P, D, D, I]),
+        "compute_ovelap": (I, [I, P, P, P, P, P, I]),
+        "local_reductionR": (I, [P, I, P, I, I]),
+        "memcopy_extra_data": (I, [C.c_size_t, P]),
+        "mm_get_size_of_kernels_workspace": (C.c_size_t, [C.c_size_t]),
+        "mm_get_size_of_derivatives_workspace": (C.c_size_t, [C.c_size_t]),
+        "mm_get_size_of_total_workspace": (C.c_size_t, [C.c_size_t]),
+        "mm_get_pointer_to_kernels_workspace": (P, [C.c_size_t]),
+        "mm_get_pointer_to_derivatives_workspace": (P, [C.c_size_t]),
+        "mm_get_pointer_to_total_workspace": (P, [C.c_size_t]),
+        "mm_get_pointer_to_fftONE_workspace": (P, [C.c_size_t]),
         "tdw_st3d_info": (I, [C.c_char_p, I, C.POINTER(StInfo), P]),
         "tdw_st3d_load": (I, [C.POINTER(Params), C.c_char_p, I, P, C.POINTER(StInfo), I, I, I, P, P, P,
                               C.POINTER(CheckpointScalars)]),
