@@ -107,6 +107,8 @@ typedef struct {
     int spinsymmetry;    /* 1: spin-symmetric mode (SPINSYMMETRY_MODE): only
                             E > 0 states are kept; densities sum both
                             components into spin b and copy it to a       */
+    double kF;           /* reference kF (memcopy_const); n_F = kF^3/(3 pi^2) */
+    double qfalpha, qfbeta, qfgamma; /* quantum-friction strengths (md.qf*) */
     int precision;       /* TDW_FP64 or TDW_SPLIT                              */
 } tdw_params;
 
@@ -427,6 +429,22 @@ int calculate_densities_2d(int n, tdw_complex *wf, tdw_complex *wf_d_dx, tdw_com
                            double *weights, double *d_densities, int gradients_computed,
                            int nthreads);                            /* cpca_dens.h */
 int density_caculate_tau(double *d_densities, int nthreads);         /* pca_dens.cu:300 */
+/* Quantum-friction densities [U_a, U_b, D] (4 N doubles; the driver passes
+ * d_densities + 12 N).  apply_hamiltonian with qfswitch > 0 reads them there,
+ * adds the friction terms to V_a, V_b, Delta (strengths p.qfalpha,
+ * p.qfbeta, p.qfgamma times qfswitch, qfgamma also times pccoeff, n_F from
+ * p.kF / memcopy_const), applies H, restores the potentials and leaves the
+ * unmodified V_a, V_b, Delta in those 4 N doubles, as the reference does. */
+int calculate_quantum_friction_densities(int n, tdw_complex *wf, tdw_complex *d_wf_laplace,
+                                         double *d_fbetaEn, double *weights, double *d_qf_densities,
+                                         int nthreads);              /* pca_dens.cu:459 */
+int calculate_quantum_friction_densities_1d(int n, tdw_complex *wf, tdw_complex *d_wf_laplace,
+                                            double *kkyz, double *d_fbetaEn, double *weights,
+                                            int *cnt, double *d_qf_densities,
+                                            int nthreads);           /* ccpca_dens.cu:470 */
+int calculate_quantum_friction_densities_2d(int n, tdw_complex *wf, tdw_complex *d_wf_laplace,
+                                            double *kkz, double *d_fbetaEn, double *weights,
+                                            double *d_qf_densities, int nthreads); /* cpca_dens.cu:471 */
 
 int compute_potentials(int it, double *d_densities, double *d_potentials, double cccoeff,
                        int nthreads);                                /* pca_kernels.cu:81 */

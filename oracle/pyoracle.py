@@ -38,7 +38,8 @@ class Cfg(C.Structure):
                 ("mu_a", C.c_double), ("mu_b", C.c_double), ("ec", C.c_double),
                 ("dt", C.c_double), ("t0", C.c_double), ("sclgth", C.c_double),
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
-                ("vext", Vext), ("spinsymmetry", C.c_int)]
+                ("vext", Vext), ("spinsymmetry", C.c_int),
+                ("kF", C.c_double), ("qfalpha", C.c_double), ("qfbeta", C.c_double), ("qfgamma", C.c_double)]
 
 
 class Uniform(C.Structure):
@@ -85,6 +86,9 @@ def lib():
             "tdwo_hf_filter_c": (None, [cp, P, P, D, D]),
             "tdwo_hf_filter_d": (None, [cp, I, P, P, D, D]),
             "tdwo_overlap": (None, [cp, I, P, P, P, P]),
+            "tdwo_qf_densities": (None, [cp, I, P, P, P, P, P, P]),
+            "tdwo_qf_add": (None, [cp, P, P, D, D]),
+            "tdwo_qf_remove": (None, [cp, P, P]),
             "tdwo_abm_step1": (None, [cp, I, P, P, P, P, P, P]),
             "tdwo_abm_step4": (None, [cp, I, P, P, P]),
             "tdwo_norms": (None, [cp, I, P, P]),
@@ -121,7 +125,8 @@ def ptr(a):
 def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
              regularization=SPHERICAL, mf_correction=0, scheme=AB4AM5,
              mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
-             p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, spinsymmetry=0):
+             p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, spinsymmetry=0, kF=1.0,
+             qf=(0.0, 0.0, 0.0)):
     c = Cfg()
     c.dim, c.nx, c.ny, c.nz = dim, nx, ny, nz
     c.dx = dx
@@ -133,6 +138,8 @@ def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
     if vext is not None:
         c.vext = vext
     c.spinsymmetry = spinsymmetry
+    c.kF = kF
+    c.qfalpha, c.qfbeta, c.qfgamma = qf
     return c
 
 

@@ -673,6 +673,73 @@ void tdwo_density_tau(const tdwo_cfg *c, double *dens)
 }
 
 /* =========================================================================
+ * Quantum friction (pca_dens.cu:360-480, cpca_dens.cu:385-490,
+ * ccpca_dens.cu:384-500; pca_kernels.cu:497-539)
+ * ========================================================================= */
+void tdwo_qf_densities(const tdwo_cfg *c, int n, const double *wf, const double *lap, const double *fbeta,
+                       const double *kt, const int *cnt, double *out)
+{
+    size_t N = tdwo_npts(c);
+    const zc *W = (const zc *)wf, *L = (const zc *)lap;
+    double tr = dens_transverse(c);
+    zc *D = (zc *)(out + 2 * N);
+    for (size_t i = 0; i < N; i++) {
+        double Ua = 0.0, Ub = 0.0;
+        zc Dl = 0.0;
+        for (int a = 0; a < n; a++) {
+            double fb = fbeta[a], fbm = 1.0 - fb, w = 1.0, k2t = 0.0;
+            if (c->dim == 1) {
+                k2t = kt[n + a] * kt[n + a] + kt[a] * kt[a];
+                w = (double)cnt[a];
+            } else if (c->dim == 2) {
+                k2t = kt[a] * kt[a];
+                w = fabs(kt[a]) < 1.0e-12 ? 1.0 : 2.0;
+            }
+            zc u = W[(size_t)a * N + i], v = W[((size_t)n + a) * N + i];
+            zc lu = L[(size_t)a * N + i] - u * k2t, lv = L[((size_t)n + a) * N + i] - v * k2t;
+            if (c->spinsymmetry) {
+                Ub += (cimag(conj(u) * lu) * fb - cimag(conj(v) * lv) * fbm) * w;
+                Dl += (conj(v) * lu + conj(lv) * u) * (fbm - fb) * w * 2.0;
+            } else {
+                Ua += cimag(conj(u) * lu) * fb * w;
+                Ub -= cimag(conj(v) * lv) * fbm * w;
+                Dl += (conj(v) * lu + conj(lv) * u) * (fbm - fb) * w;
+            }
+        }
+        if (c->spinsymmetry) Ua = Ub;
+        out[i] = Ua / tr;
+        out[N + i] = Ub / tr;
+        D[i] = -0.5 * Dl / tr;
+    }
+}
+
+void tdwo_qf_add(const tdwo_cfg *c, double *qf, double *pots, double qfswitch, double pccoeff)
+{
+    size_t N = tdwo_npts(c);
+    double qa = c->qfalpha * qfswitch, qb = c->qfbeta * qfswitch, qg = c->qfgamma * qfswitch * pccoeff;
+    double nF = c->kF * c->kF * c->kF / (3. * M_PI * M_PI);
+    zc *dl = (zc *)(pots + 2 * N), *dq = (zc *)(qf + 2 * N);
+    for (size_t i = 0; i < N; i++) {
+        double phd = carg(dl[i]), phn = carg(dq[i]);
+        double va = pots[i], vb = pots[N + i];
+        zc del = dl[i];
+        pots[i] -= qf[i] * qa / nF;
+        pots[N + i] -= qf[N + i] * qa / nF;
+        dl[i] += qb * cabs(del) * (cos(phd) + I * sin(phd)) * sin(phd - phn);
+        dl[i] += I * qg * del;
+        qf[i] = va;
+        qf[N + i] = vb;
+        dq[i] = del;
+    }
+}
+
+void tdwo_qf_remove(const tdwo_cfg *c, const double *qf, double *pots)
+{
+    size_t N = tdwo_npts(c);
+    memcpy(pots, qf, sizeof(double) * 4 * N);
+}
+
+/* =========================================================================
  * Energy density functional helpers (pca_edf.h:17-352), SLDA with A0=1.
  * ========================================================================= */
 static double p_reg(const tdwo_cfg *c, double n)

@@ -35,6 +35,12 @@
  *     tdwo_hf_filter_*, tdwo_overlap): closed forms on sums of plane waves
  *     (exact derivatives below Nyquist, zero at Nyquist, FD(k^2/2) times
  *     each mode) and a numpy overlap (tests/test_oracle_grid.py).
+ *   - quantum friction (tdwo_qf_*): U_a, U_b equal the spectral divergence
+ *     of the currents for band-limited states (Im(u* lap u) = div Im(u* grad
+ *     u)); D of plane-wave states is sum (k^2 + k_t^2) u v* (1 - 2f) w / tr;
+ *     the friction terms' special cases (phase of D equal to / a quarter
+ *     turn from that of Delta, qfgamma alone, qfalpha alone) and the exact
+ *     restore (tests/test_oracle_qf.py).
  *   - spin-symmetric mode (spinsymmetry = 1): no golden of the reference
  *     runs it with a functional built here (its runs are SLDAE).  Pinned by
  *     the partner-state identity: the half-spectrum densities of any states
@@ -108,6 +114,8 @@ typedef struct {
     double p_nmin, p_nmax, denseps;
     tdwo_vext vext;
     int spinsymmetry;     /* 1: SPINSYMMETRY_MODE (E > 0 states only)      */
+    double kF;            /* reference kF: n_F = kF^3/(3 pi^2) (dc_nF)      */
+    double qfalpha, qfbeta, qfgamma; /* quantum-friction strengths        */
 } tdwo_cfg;
 
 /* Number of evolved lattice points (NUMBER_ELEMENT). */
@@ -134,6 +142,12 @@ void tdwo_div_real(const tdwo_cfg *c, const double *fx, const double *fy, const 
 void tdwo_hf_filter_c(const tdwo_cfg *c, const double *in, double *out, double mu, double T);
 void tdwo_hf_filter_d(const tdwo_cfg *c, int n, const double *in, double *out, double mu, double T);
 void tdwo_overlap(const tdwo_cfg *c, int n, const double *w1, const double *w2, double *re, double *im);
+/* quantum friction: densities [U_a, U_b, D] (4N doubles) and the friction
+ * terms added to / removed from V_a, V_b, Delta around H */
+void tdwo_qf_densities(const tdwo_cfg *c, int n, const double *wf, const double *lap, const double *fbeta,
+                       const double *kt, const int *cnt, double *out);
+void tdwo_qf_add(const tdwo_cfg *c, double *qf, double *pots, double qfswitch, double pccoeff);
+void tdwo_qf_remove(const tdwo_cfg *c, const double *qf, double *pots);
 
 /* --- ODE integrator (tdwslda_ode_integrator.h) ---------------------------- */
 void tdwo_abm_step1(const tdwo_cfg *c, int n, double *y, double *f1, double *f2,

@@ -562,7 +562,7 @@ extern "C" int memcopy_gpu2gpu(void *src, void *dst, size_t size)
 extern "C" int memcopy_const(double mu_a, double mu_b, double ec, double t0, double dt, double kF)
 {
     BOUND();
-    (void)kF;
+    c->p.kF = kF;
     c->p.mu_a = mu_a;
     c->p.mu_b = mu_b;
     c->p.ec = ec;
@@ -641,6 +641,34 @@ extern "C" int calculate_densities_2d(int n, tdw_complex *wf, tdw_complex *dx, t
     BOUND();
     (void)nthreads;
     return densities(c, n, CD2(wf), CD2(dx), CD2(dy), nullptr, CD2(lap), fbeta, weights, kkz, c->d_cnt, dens, grad);
+}
+// quantum-friction densities into d_qf_densities = [U_a, U_b, D] (4 N
+// doubles, the driver passes d_densities + 12 N): pca_dens.cu:459-480,
+// 1D ccpca_dens.cu:470-495 (kkyz, cnt), 2D cpca_dens.cu:471-490 (kkz)
+extern "C" int calculate_quantum_friction_densities(int n, tdw_complex *wf, tdw_complex *lap, double *fbeta,
+                                                    double *weights, double *qf, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (n < 0 || n > c->nwf || !wf || !lap || !fbeta || !qf) return TDW_EINVAL;
+    return qf_densities(c, n, CD2(wf), CD2(lap), fbeta, weights, c->d_kt, c->d_cnt, qf);
+}
+extern "C" int calculate_quantum_friction_densities_1d(int n, tdw_complex *wf, tdw_complex *lap, double *kkyz,
+                                                       double *fbeta, double *weights, int *cnt, double *qf,
+                                                       int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (c->dim != 1 || n < 0 || n > c->nwf || !wf || !lap || !fbeta || !qf || !kkyz || !cnt) return TDW_EINVAL;
+    return qf_densities(c, n, CD2(wf), CD2(lap), fbeta, weights, kkyz, cnt, qf);
+}
+extern "C" int calculate_quantum_friction_densities_2d(int n, tdw_complex *wf, tdw_complex *lap, double *kkz,
+                                                       double *fbeta, double *weights, double *qf, int nthreads)
+{
+    BOUND();
+    (void)nthreads;
+    if (c->dim != 2 || n < 0 || n > c->nwf || !wf || !lap || !fbeta || !qf || !kkz) return TDW_EINVAL;
+    return qf_densities(c, n, CD2(wf), CD2(lap), fbeta, weights, kkz, nullptr, qf);
 }
 extern "C" int density_caculate_tau(double *dens, int nthreads)
 {
@@ -785,9 +813,17 @@ extern "C" int compute_energy(int it, double *dens, double *pots, double *work, 
 
 static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx,
                        tdw_complex *dy, tdw_complex *dz, tdw_complex *lap, tdw_complex *alap, const double *kt,
-                       double *pots, double qfswitch, double *useqpe)
+                       double *pots, double qfswitch, double *useqpe, double *dens, double pccoeff)
 {
-    if (qfswitch > 0.0) return TDW_EINVAL;   // quantum friction is not part of this path
+    // quantum friction: the friction densities sit at dens + 12N
+    // (pca_kernels.cu:714-733, 835-845); V_a, V_b, Delta are modified for
+    // this H application and restored after it
+    double *qf = nullptr;
+    if (qfswitch > 0.0) {
+        if (!dens || c->p.kF <= 0.0) return TDW_EINVAL;
+        qf = dens + 12 * c->N;
+        TDW_TRY(qf_add(c, qf, pots, qfswitch, pccoeff));
+    }
     if (c->p.functional == TDW_ASLDA) {
         if (!dx || (c->dim >= 2 && !dy) || (c->dim >= 3 && !dz) || !alap || in == out) return TDW_EINVAL;
         TDW_TRY(apply_h_aslda(c, n, CD2(in), D2(out), CD2(dx), CD2(dy), CD2(dz), CD2(lap), CD2(alap), pots, kt,
@@ -795,6 +831,7 @@ static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *
     } else {
         TDW_TRY(apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, kt, useqpe, c->d_qpe));
     }
+    if (qf) TDW_TRY(qf_remove(c, qf, pots));
     // <H> is left at the head of the work area (pca_kernels.cu:812-822)
     if (c->ref_workarea)
         TDW_HIP(hipMemcpyAsync(c->ref_workarea, useqpe ? useqpe : c->d_qpe, sizeof(double) * n,
@@ -806,24 +843,25 @@ extern "C" int apply_hamiltonian(int it, int n, tdw_complex *in, tdw_complex *ou
                                  double qfswitch, double *useqpe, double pccoeff, int nthreads)
 {
     BOUND();
-    (void)dens; (void)pccoeff; (void)nthreads;
-    return apply_h_ref(c, it, n, in, out, dx, dy, dz, lap, alap, c->d_kt, pots, qfswitch, useqpe);
+    (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, dx, dy, dz, lap, alap, c->d_kt, pots, qfswitch, useqpe, dens, pccoeff);
 }
 extern "C" int apply_hamiltonian_1d(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx, double *kkyz,
                                     tdw_complex *lap, tdw_complex *alap, double *dens, double *pots, double qfswitch,
                                     double *useqpe, double pccoeff, int nthreads)
 {
     BOUND();
-    (void)dens; (void)pccoeff; (void)nthreads;
-    return apply_h_ref(c, it, n, in, out, dx, nullptr, nullptr, lap, alap, kkyz, pots, qfswitch, useqpe);
+    (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, dx, nullptr, nullptr, lap, alap, kkyz, pots, qfswitch, useqpe, dens,
+                       pccoeff);
 }
 extern "C" int apply_hamiltonian_2d(int it, int n, tdw_complex *in, tdw_complex *out, tdw_complex *dx,
                                     tdw_complex *dy, double *kkz, tdw_complex *lap, tdw_complex *alap, double *dens,
                                     double *pots, double qfswitch, double *useqpe, double pccoeff, int nthreads)
 {
     BOUND();
-    (void)dens; (void)pccoeff; (void)nthreads;
-    return apply_h_ref(c, it, n, in, out, dx, dy, nullptr, lap, alap, kkz, pots, qfswitch, useqpe);
+    (void)nthreads;
+    return apply_h_ref(c, it, n, in, out, dx, dy, nullptr, lap, alap, kkz, pots, qfswitch, useqpe, dens, pccoeff);
 }
 extern "C" int multiply_wf_by_alpha(int n, tdw_complex *wf_in, tdw_complex *wf_out, double *pots, int nthreads)
 {

@@ -363,6 +363,151 @@ int densities32(tdw_ctx *c, int n, const float2 *wf, const double *fbeta, const 
 }
 
 // ---------------------------------------------------------------------------
+// quantum-friction densities (pca_dens.cu:360-450, cpca_dens.cu:385-490,
+// ccpca_dens.cu:384-500), with the weights f and 1 - f (no 1e4 scale) and
+// L psi = lap psi - k_t^2 psi in 1D / 2D:
+//   U_a = sum f w Im(u* L u),  U_b = -sum (1 - f) w Im(v* L v),
+//   D = -1/2 sum (v* L u + (L v)* u)(1 - 2 f) w,
+// each divided by the transverse length.  w: 1 (3D), 1 or 2 by k_z (2D, no
+// cubic-cutoff exception in this kernel of the reference), cnt (1D).
+// Spin-symmetric mode: U_b = U_a + U_b, D doubled, U_a = U_b.
+// out: [U_a (N), U_b (N), D (N complex)].
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void __launch_bounds__(NT)
+k_qf_part(const double2 *__restrict__ wf, const double2 *__restrict__ lap, const double *__restrict__ fbeta,
+          const double *__restrict__ wgt, const double *__restrict__ kt, const int *__restrict__ cnt, int n, long N,
+          int chunk, double *__restrict__ part)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    const int a0 = blockIdx.y * chunk, a1 = min(n, a0 + chunk);
+    double ua = 0.0, ub = 0.0;
+    double2 d = make_double2(0.0, 0.0);
+    for (int a = a0; a < a1; a++) {
+        double fb = fbeta[a], fbm = 1.0 - fb;
+        if (wgt) { fb *= wgt[a]; fbm *= wgt[a]; }
+        double w = 1.0, k2t = 0.0;
+        if (DIM == 1) {
+            const double ky = kt[a], kz = kt[n + a];
+            k2t = kz * kz + ky * ky;
+            w = (double)cnt[a];
+        } else if (DIM == 2) {
+            const double kz = kt[a];
+            k2t = kz * kz;
+            w = (fabs(kz) < 1.0e-12) ? 1.0 : 2.0;
+        }
+        const double2 u = wf[(long)a * N + i], v = wf[((long)n + a) * N + i];
+        double2 lu = lap[(long)a * N + i], lv = lap[((long)n + a) * N + i];
+        if (DIM < 3) { lu = lu - u * k2t; lv = lv - v * k2t; }
+        ua += cmulc(u, lu).y * fb * w;
+        ub -= cmulc(v, lv).y * fbm * w;
+        d = d + (cmulc(v, lu) + cmulc(lv, u)) * ((fbm - fb) * w);
+    }
+    double *p = part + (long)blockIdx.y * 4 * N + i;
+    p[0] = ua;
+    p[N] = ub;
+    p[2 * N] = d.x;
+    p[3 * N] = d.y;
+}
+
+__global__ void __launch_bounds__(NT)
+k_qf_final(const double *__restrict__ part, int nchunk, long N, double tr, int ss, double *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nchunk; k++) {
+        const double *p = part + (long)k * 4 * N + i;
+#pragma unroll
+        for (int v = 0; v < 4; v++) s[v] += p[v * N];
+    }
+    if (ss) {
+        s[1] += s[0];
+        s[0] = s[1];
+        s[2] *= 2.0;
+        s[3] *= 2.0;
+    }
+    out[i] = s[0] / tr;
+    out[N + i] = s[1] / tr;
+    out[2 * N + 2 * i] = -0.5 * s[2] / tr;
+    out[2 * N + 2 * i + 1] = -0.5 * s[3] / tr;
+}
+
+int qf_densities(tdw_ctx *c, int n, const double2 *wf, const double2 *lap, const double *fbeta, const double *wgt,
+                 const double *kt, const int *cnt, double *out)
+{
+    TDW_TIMED(c, "qf_densities");
+    const long N = (long)c->N;
+    int chunk = 128;
+    const long nblk = (N + NT - 1) / NT;
+    while (chunk > 8 && nblk * ((n + chunk - 1) / chunk) < 2048) chunk >>= 1;
+    const int nchunk = (n + chunk - 1) / chunk;
+    TDW_TRY(ensure_part(c, (size_t)nchunk * 4 * N));
+    dim3 g(nblk, nchunk);
+    if (c->dim == 1)
+        hipLaunchKernelGGL(k_qf_part<1>, g, dim3(NT), 0, c->stream, wf, lap, fbeta, wgt, kt, cnt, n, N, chunk, c->d_part);
+    else if (c->dim == 2)
+        hipLaunchKernelGGL(k_qf_part<2>, g, dim3(NT), 0, c->stream, wf, lap, fbeta, wgt, kt, cnt, n, N, chunk, c->d_part);
+    else
+        hipLaunchKernelGGL(k_qf_part<3>, g, dim3(NT), 0, c->stream, wf, lap, fbeta, wgt, kt, cnt, n, N, chunk, c->d_part);
+    hipLaunchKernelGGL(k_qf_final, dim3(nblk), dim3(NT), 0, c->stream, c->d_part, nchunk, N, dens_transverse(c),
+                       c->p.spinsymmetry, out);
+    return (int)hipGetLastError();
+}
+
+// Friction terms into the mean fields before H and their removal after it
+// (pca_kernels.cu:497-539, Eq. (3) of arXiv:1305.6891):
+//   V_s -= U_s qfalpha / n_F,
+//   Delta += qfbeta |Delta| e^{i phi_Delta} sin(phi_Delta - phi_D) + i qfgamma Delta,
+// with the original V_a, V_b, Delta saved over the friction densities (as
+// the reference does) and copied back by k_qf_remove.
+__global__ void k_qf_add(double *__restrict__ qf, double *__restrict__ pots, long N, double qa, double qb,
+                         double qg, double nF)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    double2 *dl = reinterpret_cast<double2 *>(pots + 2 * N);
+    double2 *dq = reinterpret_cast<double2 *>(qf + 2 * N);
+    const double2 del = dl[i], dd = dq[i];
+    const double phd = atan2(del.y, del.x), phn = atan2(dd.y, dd.x);
+    const double va = pots[i], vb = pots[N + i];
+    pots[i] = va - qf[i] * qa / nF;
+    pots[N + i] = vb - qf[N + i] * qa / nF;
+    double2 nd = del + make_double2(cos(phd), sin(phd)) * (qb * hypot(del.x, del.y) * sin(phd - phn));
+    nd = nd + cmul(make_double2(0.0, qg), del);
+    dl[i] = nd;
+    qf[i] = va;
+    qf[N + i] = vb;
+    dq[i] = del;
+}
+__global__ void k_qf_remove(const double *__restrict__ qf, double *__restrict__ pots, long N)
+{
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    pots[i] = qf[i];
+    pots[N + i] = qf[N + i];
+    pots[2 * N + 2 * i] = qf[2 * N + 2 * i];
+    pots[2 * N + 2 * i + 1] = qf[2 * N + 2 * i + 1];
+}
+
+int qf_add(tdw_ctx *c, double *qf, double *pots, double qfswitch, double pccoeff)
+{
+    const long N = (long)c->N;
+    const double kF = c->p.kF;
+    const double nF = kF * kF * kF / (3. * M_PI * M_PI);
+    hipLaunchKernelGGL(k_qf_add, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, qf, pots, N,
+                       c->p.qfalpha * qfswitch, c->p.qfbeta * qfswitch, c->p.qfgamma * qfswitch * pccoeff, nF);
+    return (int)hipGetLastError();
+}
+int qf_remove(tdw_ctx *c, const double *qf, double *pots)
+{
+    const long N = (long)c->N;
+    hipLaunchKernelGGL(k_qf_remove, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, qf, pots, N);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // tau = max(0, lap(rho)/2 - tau_acc)
 // ---------------------------------------------------------------------------
 __global__ void k_real2cplx(const double *__restrict__ f, double2 *__restrict__ z, long N)

@@ -108,6 +108,12 @@ int grid_filter_d(tdw_ctx *c, int n, const double *in, double *out, double mu, d
 int grid_filter_c(tdw_ctx *c, const double2 *in, double2 *out, double mu, double T);
 int grid_overlap(tdw_ctx *c, int n, const double2 *w1, const double2 *w2, double *re, double *im);
 int grid_sum(tdw_ctx *c, const double *a, long n, double *out);
+// quantum friction: the densities U_a, U_b, D (4N doubles) and the friction
+// terms added to / removed from V_a, V_b, Delta around an H application
+int qf_densities(tdw_ctx *c, int n, const double2 *wf, const double2 *lap, const double *fbeta, const double *wgt,
+                 const double *kt, const int *cnt, double *out);
+int qf_add(tdw_ctx *c, double *qf, double *pots, double qfswitch, double pccoeff);
+int qf_remove(tdw_ctx *c, const double *qf, double *pots);
 int vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 // cccoeff: the current-corrections coefficient of the ASLDA vector
 // potentials and current terms (1 after the reference's default ramp)

@@ -49,7 +49,9 @@ class Params(C.Structure):
                 ("mu_a", C.c_double), ("mu_b", C.c_double), ("ec", C.c_double),
                 ("dt", C.c_double), ("t0", C.c_double), ("sclgth", C.c_double),
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
-                ("vext", Vext), ("spinsymmetry", C.c_int), ("precision", C.c_int)]
+                ("vext", Vext), ("spinsymmetry", C.c_int),
+                ("kF", C.c_double), ("qfalpha", C.c_double), ("qfbeta", C.c_double), ("qfgamma", C.c_double),
+                ("precision", C.c_int)]
 
 
 class Uniform(C.Structure):
@@ -97,7 +99,8 @@ class WDataParams(C.Structure):
 def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
                 regularization=SPHERICAL_CUTOFF, mf_correction=0, scheme=AB4AM5,
                 mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
-                p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, precision=FP64, spinsymmetry=0):
+                p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, precision=FP64, spinsymmetry=0,
+                kF=1.0, qf=(0.0, 0.0, 0.0)):
     p = Params()
     p.dim, p.nx, p.ny, p.nz = dim, nx, ny, nz
     p.dx = dx
@@ -109,6 +112,8 @@ def make_params(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
     if vext is not None:
         p.vext = vext
     p.spinsymmetry = spinsymmetry
+    p.kF = kF
+    p.qfalpha, p.qfbeta, p.qfgamma = qf
     p.precision = precision
     return p
 
@@ -220,6 +225,9 @@ def _sigs():
         "get_delta_ext": (I, [I, I, P, P]),
         "get_velocity_ext": (I, [I, I, I, P]),
         "symmetrize_densities_device": (I, [P]),
+        "calculate_quantum_friction_densities": (I, [I, P, P, P, P, P, I]),
+        "calculate_quantum_friction_densities_1d": (I, [I, P, P, P, P, P, P, P, I]),
+        "calculate_quantum_friction_densities_2d": (I, [I, P, P, P, P, P, P, I]),
         "compute_gradient_real_f": (I, [P, P, P, P, I]),
         "compute_derivative_real_vector_f": (I, [P, P, P, P, P, P, I]),
         "compute_divergence_real_vector_f": (I, [P, P, P, P, I]),
