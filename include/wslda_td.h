@@ -44,6 +44,7 @@ typedef struct { float re, im; } tdw_complex32;
 #define TDW_SLDA 111
 #define TDW_ASLDA 112
 #define TDW_BDG 113
+#define TDW_SLDAE 115
 #define TDW_SPHERICAL_CUTOFF 88
 #define TDW_CUBIC_CUTOFF 89
 #define TDW_AB3AM4 34
@@ -96,7 +97,7 @@ typedef struct {
     int dim;             /* 1, 2, 3 (CODEDIM)                                 */
     int nx, ny, nz;      /* NX, NY, NZ; for dim<3 the extra axes are transverse */
     double dx, dy, dz;   /* DX, DY, DZ                                        */
-    int functional;      /* TDW_SLDA, TDW_ASLDA or TDW_BDG                    */
+    int functional;      /* TDW_SLDA, TDW_ASLDA, TDW_BDG or TDW_SLDAE         */
     int regularization;  /* TDW_SPHERICAL_CUTOFF or TDW_CUBIC_CUTOFF          */
     int mf_correction;   /* INCLUDE_MF_CORRECTION_FROM_REG_SCHEME             */
     int scheme;          /* TDW_AB3AM4 / TDW_AB4AM5 / TDW_AB5AM5              */

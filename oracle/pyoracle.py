@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libtdw_oracle.so")
 
-SLDA, ASLDA, BDG = 111, 112, 113
+SLDA, ASLDA, BDG, SLDAE = 111, 112, 113, 115
 SPHERICAL, CUBIC = 88, 89
 AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
 EKIN, EPOT, EPAIR, ECURRENT, EPOTEXT, EPAIREXT, EVELEXT, NPARTA, NPARTB, LZA, LZB = range(11)

@@ -894,6 +894,68 @@ void tdwo_vext_eval(const tdwo_cfg *c, int it, double *va, double *vb)
 /* =========================================================================
  * Potentials (tdwslda_functionals.h:47-306 SLDA, :373-452 BdG)
  * ========================================================================= */
+/* SLDAE functional parameters (sldae_functional.c:14-330, default set of
+ * sldae_parameters.h:1-75): Pade[4/4] fits in x = |a kF|. */
+static double se_pade(double x, const double *q /* a1..a4, b1..b4 */)
+{
+    double x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+    return (q[0] * x + q[1] * x2 + q[2] * x3 + q[3] * x4) / (1. + q[4] * x + q[5] * x2 + q[6] * x3 + q[7] * x4);
+}
+static double se_pade_d(double x, const double *q)
+{
+    double x2 = x * x, x3 = x2 * x, x4 = x3 * x;
+    double N_ = q[0] * x + q[1] * x2 + q[2] * x3 + q[3] * x4;
+    double D_ = 1. + q[4] * x + q[5] * x2 + q[6] * x3 + q[7] * x4;
+    double dN = q[0] + 2. * q[1] * x + 3. * q[2] * x2 + 4. * q[3] * x3;
+    double dD = q[4] + 2. * q[5] * x + 3. * q[6] * x2 + 4. * q[7] * x3;
+    return -(dD * N_) / (D_ * D_) + dN / D_;
+}
+static const double SE_ALPHA[8] = {0.0, -0.2081560, -0.1358830, -0.0162071, 2.09838, 1.738, 0.668549, 0.101508};
+static const double SE_BETA[8] = {1.58442, 17.2973, 0.820671, 3.48966, 13.9448, 10.1914, 11.9429, 3.48966};
+static const double SE_IGAM[8] = {6.15853, 0.156722, 0.76668, 0.0823379, 1.48431, 2.07468, 0.737836, 0.0823379};
+static const double SE_B[8] = {1.29871, 6.02212, 0.376541, 0.810972, 6.47079, 3.73389, 3.47301, 0.810972};
+static const double SE_C[8] = {0.267952, 0.754923, 0.330699, 0.0900897, 2.61695, 2.71855, 0.550587, 0.0900897};
+#define SE_BF (-0.281863)
+#define SE_CF (-14.95950)
+static double se_alpha(double x) { return 1. + se_pade(x, SE_ALPHA); }
+static double se_alpha_d(double x) { return se_pade_d(x, SE_ALPHA); }
+static double se_beta(double x) { return SE_BF * se_pade(x, SE_BETA); }
+static double se_igamma(double x) { return (1. / SE_CF) * se_pade(x, SE_IGAM); }
+static double se_bfun(double x) { return SE_BF * se_pade(x, SE_B); }
+static double se_cfun(double x) { return SE_CF * se_pade(x, SE_C); }
+static double se_mu_ufg(double x)
+{
+    double U = 5. / 24., V = U / tan(3. * M_PI / 16. * (1. - 0.3582341));
+    double s0 = atan(x * U / (1. + x * V));
+    double s1 = U / (1. + 2. * V * x + (U * U + V * V) * x * x);
+    return 1. - 16. / (3. * M_PI) * s0 - 16. / (15. * M_PI) * x * s1;
+}
+/* af, af', beta, eF, C~, C~', B at total density nt (tdwslda_functionals.h:580-618) */
+typedef struct { double af, af_p, beta, eF, ct, ct_p, bf; } se_local;
+static se_local se_coeffs(double nt, double ntreg, double as)
+{
+    se_local L;
+    double kF = pow(3. * M_PI * M_PI * nt, 1. / 3.);
+    L.eF = kF * kF / 2.;
+    double x = fabs(as * kF);
+    double dx = ntreg > 0.0 ? ntreg * x / (3. * nt) : 0.0;
+    L.beta = se_beta(x);
+    double ig = se_igamma(x);
+    L.af = se_alpha(x);
+    L.bf = se_bfun(x);
+    double cf = se_cfun(x), cf_p = 0.0;
+    L.af_p = 0.0;
+    if (ntreg > 0.0) {
+        L.af_p = dx * se_alpha_d(x);
+        cf_p = ntreg * cf / (3. * nt) * (1. - cf * ig);
+    }
+    L.ct = L.af * pow(nt, 1. / 3.) / cf;
+    double igeff = (1 / cf) * (1. - 3. * nt / cf * cf_p);
+    L.ct_p = ntreg > 0. ? pow(ntreg, 2. / 3.) * igeff * L.af / (3. * pow(nt, 2. / 3.)) : 0.;
+    L.ct_p += L.af_p * pow(nt, 1. / 3.) / cf;
+    return L;
+}
+
 void tdwo_potentials(const tdwo_cfg *c, int it, const double *dens, double *pots)
 {
     size_t N = tdwo_npts(c);
@@ -912,6 +974,76 @@ void tdwo_potentials(const tdwo_cfg *c, int it, const double *dens, double *pots
     for (size_t i = 0; i < N; i++) {
         double na = rho_a[i], nb = rho_b[i];
         zc lnu = nu[i];
+        if (c->functional == TDWO_SLDAE) {
+            /* tdwslda_functionals.h:530-812, CURRENT_CORRECTIONS, cccoeff = 1 */
+            double nt = na + nb;
+            se_local L = se_coeffs(nt, p_reg(c, nt), c->sclgth);
+            al_a[i] = L.af;
+            al_b[i] = L.af;
+            const double *ja = dens + 4 * N, *jb = dens + 9 * N;
+            double *Aa = pots + 6 * N, *Ab = pots + 9 * N;
+            double fa = p_reg(c, na), fb = p_reg(c, nb), j2a = 0.0, j2b = 0.0;
+            for (int d = 0; d < c->dim; d++) {
+                Aa[d * N + i] = fa == 0.0 ? 0.0 : fa * (1. - L.af) * ja[d * N + i] / na;
+                Ab[d * N + i] = fb == 0.0 ? 0.0 : fb * (1. - L.af) * jb[d * N + i] / nb;
+                j2a += ja[d * N + i] * ja[d * N + i];
+                j2b += jb[d * N + i] * jb[d * N + i];
+            }
+            double taus = dens[3 * N + i] + dens[8 * N + i];
+            double Va_const = ua[i] + L.af_p * taus / 2. + L.beta * L.eF;
+            double Vb_const = ub[i] + L.af_p * taus / 2. + L.beta * L.eF;
+            if (fa != 0.0) {
+                double t7 = fa * j2a / (2.0 * na);
+                Va_const += ((L.af - 1.0) / na - L.af_p) * t7;
+                Va_const -= der_p_reg(c, na) * (L.af - 1.0) * j2a / (2.0 * na);
+                Vb_const -= L.af_p * t7;
+            }
+            if (fb != 0.0) {
+                double t7 = fb * j2b / (2.0 * nb);
+                Va_const -= L.af_p * t7;
+                Vb_const += ((L.af - 1.0) / nb - L.af_p) * t7;
+                Vb_const -= der_p_reg(c, nb) * (L.af - 1.0) * j2b / (2.0 * nb);
+            }
+            double Va = Va_[i], Vb = Vb_[i];
+            zc ldelta = 0;
+            double lkF = pow(3.0 * M_PI * M_PI * nt, 1.0 / 3.0);
+            for (int k = 0; k < UD_SCITERS; k++) {
+                double lmu = (c->mu_a - Va + c->mu_b - Vb) / 2.;
+                zc p0 = csqrt(CMPLX(2.0 * lmu / L.af, 0.0));
+                if (cimag(p0) < 0.) p0 *= -1.;
+                zc kc;
+                if (c->regularization == TDWO_CUBIC) kc = M_PI / DXs;
+                else {
+                    kc = csqrt(CMPLX(2.0 * (c->ec + lmu) / L.af, 0.0));
+                    if (cimag(kc) < 0.) kc *= -1.;
+                }
+                zc wz = clog((kc + p0) / (kc - p0));
+                if (cimag(wz) < 0.) wz += CMPLX(0.0, 2. * M_PI);
+                wz = kc * R0 * (1. - (p0 / kc) * R1 * wz);
+                double g_eff = L.af / (L.ct - creal(wz));
+                ldelta = -lnu * g_eff;
+                double dn = creal(conj(ldelta) * lnu), d2 = nrm2(ldelta);
+                double Vanew = Va_const - (L.af_p / L.af) * dn - L.ct_p / L.af * d2;
+                double Vbnew = Vb_const - (L.af_p / L.af) * dn - L.ct_p / L.af * d2;
+                if (c->mf_correction) {
+                    double bcoeff = creal(p0 / (lkF + 1.0e-12));
+                    double lx = bcoeff * lkF * DXs / M_PI;
+                    if (g_eff < -1.0e-10 && lx > 1.0e-10) {
+                        double Lam0 = bcoeff * R0 / lx * (1.0 - R1 * lx * log((1.0 + lx) / (1.0 - lx)));
+                        double dLam0 = (-2. * bcoeff * R0 * R1) / (1. - lx * lx) - bcoeff * R0 / lx / lx;
+                        double dLam_dn = (lkF / (3. * nt * L.af)) * (Lam0 + dLam0 * bcoeff * DXs * lkF / M_PI);
+                        Vanew += dLam_dn * d2;
+                        Vbnew += dLam_dn * d2;
+                    }
+                }
+                Va = UD_MIX_COEFF * Vanew + (1. - UD_MIX_COEFF) * Va;
+                Vb = UD_MIX_COEFF * Vbnew + (1. - UD_MIX_COEFF) * Vb;
+            }
+            Va_[i] = Va;
+            Vb_[i] = Vb;
+            delta[i] = ldelta;
+            continue;
+        }
         if (c->functional == TDWO_SLDA || c->functional == TDWO_ASLDA) {
             double t1p = polarization(c, na, nb);
             double alph_plus = alpha_plus(c, t1p);
@@ -1122,7 +1254,7 @@ void tdwo_apply_h_full(const tdwo_cfg *c, int n, const double *in, double *out, 
                        const double *ddy, const double *ddz, const double *lap, const double *alap,
                        const double *pots, const double *kt, const double *useqpe, double *qpe)
 {
-    if (c->functional != TDWO_ASLDA) {
+    if (c->functional != TDWO_ASLDA && c->functional != TDWO_SLDAE) {
         tdwo_apply_h(c, n, in, out, lap, pots, kt, useqpe, qpe);
         return;
     }
@@ -1245,7 +1377,12 @@ void tdwo_energy(const tdwo_cfg *c, int it, const double *dens, const double *po
         double tz1 = c->dim >= 3 ? jaz[i] : 0.0, tz2 = c->dim >= 3 ? jbz[i] : 0.0;
         double ta = tau_a[i] - p_reg(c, na) * (tx1 * tx1 + ty1 * ty1 + tz1 * tz1) / na;
         double tb = tau_b[i] - p_reg(c, nb) * (tx2 * tx2 + ty2 * ty2 + tz2 * tz2) / nb;
-        if (c->functional == TDWO_SLDA || c->functional == TDWO_ASLDA) {
+        if (c->functional == TDWO_SLDAE) {
+            /* tdwslda_functionals.h:815-889 */
+            double nt = na + nb, kF = pow(3. * M_PI * M_PI * nt, 1. / 3.), x = fabs(c->sclgth * kF);
+            s[TDWO_EKIN] += se_alpha(x) * (ta + tb) / 2. * VE;
+            s[TDWO_EPOT] += (3. / 5.) * se_bfun(x) * (kF * kF / 2.) * nt * VE;
+        } else if (c->functional == TDWO_SLDA || c->functional == TDWO_ASLDA) {
             double p = polarization(c, na, nb);
             s[TDWO_EKIN] += 0.5 * (alpha_f(c, p) * ta + alpha_f(c, -1.0 * p) * tb) * VE;
             s[TDWO_EPOT] += funD(c, na, nb) * VE;
@@ -1302,7 +1439,7 @@ static void half_densities(const tdwo_cfg *c, tdwo_state *s, int grad)
 static void half_h(const tdwo_cfg *c, tdwo_state *s, double *out)
 {
     int n = s->n;
-    if (c->functional == TDWO_ASLDA) {
+    if (c->functional == TDWO_ASLDA || c->functional == TDWO_SLDAE) {
         size_t N = tdwo_npts(c);
         double *al = (double *)malloc(sizeof(double) * 4 * (size_t)n * N);
         tdwo_alpha_wf(c, n, s->wf, s->pots, al);
@@ -1318,7 +1455,7 @@ int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
 {
     int n = s->n;
     int last = c->scheme == TDWO_AB3AM4 ? 2 : c->scheme == TDWO_AB4AM5 ? 3 : 4;
-    const int full = c->functional == TDWO_ASLDA;
+    const int full = c->functional == TDWO_ASLDA || c->functional == TDWO_SLDAE;
     /* predictor */
     tdwo_abm_step1(c, n, s->wf, s->f[0], s->f[1], s->f[2], s->f[3], s->f[4]);
     tdwo_normalize(c, n, s->wf);
@@ -1550,6 +1687,17 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
     double eF_avg = pow(3.0 * M_PI * M_PI * (n0_a + n0_b), 2.0 / 3.0) / 2.0;
     double kc = kc_in;
     double mu_a = 0.37 * eF_a, mu_b = 0.37 * eF_b;
+    /* SLDAE: solve_uniform_problem_sldae (pca_uniform.h:1532-2043) */
+    const int se = c->functional == TDWO_SLDAE;
+    se_local SL = se_coeffs(n0_a + n0_b, 1.0, c->sclgth);
+    if (se) {
+        double x = fabs(c->sclgth * pow(3.0 * M_PI * M_PI * (n0_a + n0_b), 1.0 / 3.0));
+        alph_a = alph_b = alph_plus = SL.af;
+        tC = SL.ct;
+        Dn = (3. / 5.) * SL.bf * (n0_a + n0_b) * SL.eF;
+        mu_a = se_mu_ufg(x) * eF_a;
+        mu_b = se_mu_ufg(x) * eF_b;
+    }
     double tau_a = pow(6.0 * M_PI * M_PI * n0_a, 5.0 / 3.0) / (10.0 * M_PI * M_PI);
     double tau_b = pow(6.0 * M_PI * M_PI * n0_b, 5.0 / 3.0) / (10.0 * M_PI * M_PI);
     double delta = (n0_a < n0_b) ? 0.5 * pow(6.0 * M_PI * M_PI * n0_a, 2.0 / 3.0) / 2.0
@@ -1575,6 +1723,10 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
             double tau_p = tau_a + tau_b, tau_m = tau_a - tau_b;
             V_a = dalphm_dna * tau_m / 2.0 + dalphp_dna * (tau_p / 2.0 - delta * nu / alph_plus) - dtC_dna * delta * delta / alph_plus + dD_dna;
             V_b = dalphm_dnb * tau_m / 2.0 + dalphp_dnb * (tau_p / 2.0 - delta * nu / alph_plus) - dtC_dnb * delta * delta / alph_plus + dD_dnb;
+            if (se) {
+                V_a = SL.af_p * tau_p / 2. + SL.beta * SL.eF - SL.ct_p * delta * delta / SL.af - SL.af_p * delta * nu / SL.af;
+                V_b = V_a;
+            }
             double mu_p = (mu_a - V_a_old + mu_b - V_b_old) / 2.0;
             zc p0 = csqrt(CMPLX(2.0 * mu_p / alph_plus, 0.0));
             if (cimag(p0) < 0.) p0 *= -1.;
@@ -1583,7 +1735,7 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
             zc wz = clog((kc + p0) / (kc - p0));
             if (cimag(wz) < 0.) wz += I * 2. * M_PI;
             wz = kc * R0 * (1. - (p0 / kc) * R1 * wz);
-            double g_eff = creal(alph_plus / (tC - wz));
+            double g_eff = se ? alph_plus / (tC - creal(wz)) : creal(alph_plus / (tC - wz));
             delta = -1.0 * g_eff * nu;
             if (c->mf_correction) {
                 double kF = pow(3.0 * M_PI * M_PI * (n0_a + n0_b), 1.0 / 3.0);

@@ -62,6 +62,7 @@ extern "C" {
 #define TDWO_SLDA 111
 #define TDWO_ASLDA 112
 #define TDWO_BDG  113
+#define TDWO_SLDAE 115
 /* Regularisation ids (pca_settings.h:223-224). */
 #define TDWO_SPHERICAL 88
 #define TDWO_CUBIC     89

@@ -1,5 +1,5 @@
-"""Extract the td-gauss-slda, td-gauss-aslda and td-gauss-bdg golden
-trajectories into small JSON fixtures.
+"""Extract the td-gauss-slda, td-gauss-aslda, td-gauss-bdg and
+td-gauss-sldae golden trajectories into small JSON fixtures.
 
 Reads the reference testsuite's own expected outputs (data, not code):
   testsuite/td-gauss-{slda,aslda,bdg}/ref/test.ss0-{1d,2d,3d}.wlog.ref
@@ -50,17 +50,21 @@ def final_values(path):
     return out
 
 
-def params(path):
+def params(path, flip):
     pr = [0.0] * 8
     for line in open(path):
         t = line.split()
         if t and t[0].startswith("params"):
             pr[int(t[0][6:])] = float(t[1])
-    pr[7] = 1.0  # test.desc appends "params7 1" for the ss0 runs
+    if flip:
+        pr[7] = 1.0  # test.desc appends "params7 1" for the ss0 runs
     return pr
 
 
-for suite, functional in (("slda", "SLDA"), ("aslda", "ASLDA"), ("bdg", "BDG")):
+# td-gauss-sldae: the spin-symmetric ss1 runs (Makefile.1.s.34:
+# SPINSYMMETRY_MODE, AB3AM4), N_a = N_b, sclgth = -1.1, no sign flip
+for suite, functional, ss in (("slda", "SLDA", "ss0"), ("aslda", "ASLDA", "ss0"), ("bdg", "BDG", "ss0"),
+                              ("sldae", "SLDAE", "ss1")):
     data = {"columns": ["id", "t_eF", "Na", "Nb", "N", "Etot/Effg", "Ekin/Effg", "Epot/Effg",
                         "Epair/Effg", "Ecurr/Effg", "Eext/Effg", "Edext/Effg", "Evext/Effg"],
             "inputs": {"NX": 12, "NY": 12, "NZ": 12, "DX": 1.15, "functional": functional,
@@ -70,11 +74,13 @@ for suite, functional in (("slda", "SLDA"), ("aslda", "ASLDA"), ("bdg", "BDG")):
                        "dt_eF": 0.0035, "timesteps": 286, "measurements": 100, "aBdG": -1.1,
                        },
             "runs": {}}
-    for d, tag in ((1, "ss0-1d"), (2, "ss0-2d"), (3, "ss0-3d")):
+    if ss == "ss1":
+        data["inputs"].update({"scheme": "AB3AM4", "spinsymmetry": 1, "sclgth": -1.1})
+    for d, tag in ((1, ss + "-1d"), (2, ss + "-2d"), (3, ss + "-3d")):
         p = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.wlog.ref" % (suite, tag))
         inp = os.path.join(REF, "testsuite/td-gauss-%s/input.txt.%d" % (suite, d))
         c = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.cmp.ref" % (suite, tag))
-        data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp),
+        data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp, ss == "ss0"),
                              "final": final_values(c)}
     out = os.path.join(HERE, "td_gauss_%s.json" % suite)
     json.dump(data, open(out, "w"), indent=0)

@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def golden(functional=po.SLDA):
-    name = {po.BDG: "td_gauss_bdg.json", po.ASLDA: "td_gauss_aslda.json"}.get(functional, "td_gauss_slda.json")
+    name = {po.BDG: "td_gauss_bdg.json", po.ASLDA: "td_gauss_aslda.json",
+            po.SLDAE: "td_gauss_sldae.json"}.get(functional, "td_gauss_slda.json")
     return json.load(open(os.path.join(HERE, "golden", name)))
 
 
@@ -24,7 +25,13 @@ def mu_rule_of(functional):
     each golden: the SLDA and BdG logs come from the 2021.11.06 code (the
     rule now kept commented at pca_uniform.h:425-426), the ASLDA log from
     the 2026.09.19 code (the active rule, pca_uniform.h:427-428)."""
-    return 0 if functional == po.ASLDA else 1
+    return 0 if functional in (po.ASLDA, po.SLDAE) else 1
+
+
+def run_tag(functional, dim):
+    """The golden run of each suite: td-gauss-sldae has only the
+    spin-symmetric ss1 runs."""
+    return ("ss1-%dd" if functional == po.SLDAE else "ss0-%dd") % dim
 
 
 def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
@@ -36,9 +43,12 @@ def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
     NX, DXv = g["NX"], g["DX"]
     LXYZ = (NX * DXv) ** 3
     Na = math.ceil(1.0 / (6.0 * math.pi ** 2) * LXYZ)
-    Nb = Na + 1
-    cfg = po.make_cfg(dim, NX, NX, NX, dx=DXv, scheme=scheme, functional=functional,
-                      sclgth=g["aBdG"] if functional == po.BDG else 0.0)
+    ss = int(g.get("spinsymmetry", 0))
+    Nb = Na if ss else Na + 1          # UNIFORM_TEST_MODE, pca.c:198-199
+    if functional == po.SLDAE:
+        scheme = po.AB3AM4              # Makefile.1.s.34
+    cfg = po.make_cfg(dim, NX, NX, NX, dx=DXv, scheme=scheme, functional=functional, spinsymmetry=ss,
+                      sclgth=g["aBdG"] if functional == po.BDG else g.get("sclgth", 0.0))
     kc = 0.999999 * math.pi / DXv
     u, rc = po.uniform_solve(cfg, Na / LXYZ, Nb / LXYZ, kc, g["temperature"], eps=g["init0eps"],
                              scmix=g["init0scmix"], muchange=g["init0muchange"],
@@ -50,7 +60,7 @@ def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
     Effg = 0.6 * n * eF * LXYZ
     wf, En, kt, cnt = po.uniform_states(cfg, u)
     fb = np.array([po.fbeta(E, u.beta) for E in En])
-    p = golden(functional)["runs"]["ss0-%dd" % dim]["params"]
+    p = golden(functional)["runs"][run_tag(functional, dim)]["params"]
     A = p[0] * eF
 
     def wcoef(s):
@@ -66,7 +76,7 @@ def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
     pots[:N] = u.V_a
     pots[N:2 * N] = u.V_b
     pots[2 * N:4 * N:2] = u.delta
-    if functional == po.ASLDA:
+    if functional in (po.ASLDA, po.SLDAE):
         pots[4 * N:5 * N] = u.alph_a
         pots[5 * N:6 * N] = u.alph_b
     else:

@@ -1,0 +1,122 @@
+"""SLDAE functional (spin-symmetric td-gauss-sldae set-up) on the HIP path
+against the oracle (pinned to the td-gauss-sldae goldens by
+tests/test_oracle_golden.py).  Bar: 1e-12 for steps and the uniform
+solution; the GPU's own 1D golden trajectory to the oracle's bar."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import tdgauss
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    d = np.abs(np.asarray(a) - np.asarray(b)).max()
+    s = np.abs(np.asarray(b)).max()
+    return d / s if s > 0 else d
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no HIP device")
+    return t
+
+
+def gpu_run(torch, s):
+    from wslda_amd import td
+    ctx = td.TDContext(tdgauss.to_params(s["cfg"]), s["wf"].shape[0] // 2, 0)
+    ctx.set_states(s["fbeta"], s["kt"], s["cnt"])
+    return ctx, td.TDRun(ctx, s["wf"], s["pots"])
+
+
+def test_sldae_uniform_solution(torch):
+    from wslda_amd import td
+    say("case: SLDAE uniform solution (library vs oracle)")
+    s = tdgauss.setup(3, functional=po.SLDAE)
+    g = tdgauss.golden(po.SLDAE)["inputs"]
+    LXYZ = (g["NX"] * g["DX"]) ** 3
+    n0 = math.ceil(LXYZ / (6 * math.pi ** 2)) / LXYZ
+    u, rc = td.uniform_solve(tdgauss.to_params(s["cfg"]), n0, n0, 0.999999 * math.pi / g["DX"], g["temperature"],
+                             eps=g["init0eps"], scmix=g["init0scmix"], muchange=g["init0muchange"],
+                             maxiter=g["init0maxiter"], mu_rule=0)
+    assert rc == 0
+    for k in ("n0_a", "n0_b", "mu_a", "mu_b", "V_a", "delta", "nu", "tau_a", "alph_a", "S", "ekin", "epot",
+              "epair"):
+        x, y = getattr(u, k), getattr(s["u"], k)
+        assert abs(x - y) <= TOL * max(abs(y), 1e-300), (k, x, y)
+    assert u.nwf == s["u"].nwf
+
+
+@pytest.mark.parametrize("dim,nsteps", [(1, 286), (2, 40), (3, 6)])
+def test_sldae_pece_steps(torch, dim, nsteps):
+    """PECE steps with V_ext switching on: densities, alpha, energies."""
+    say("case: SLDAE PECE steps dim=%d nsteps=%d" % (dim, nsteps))
+    s = tdgauss.setup(dim, functional=po.SLDAE)
+    it0 = int(round(6.0 / s["eF"] / s["cfg"].dt))
+    P = po.Problem(s["cfg"], s["wf"], s["fbeta"], kt=s["kt"], cnt=s["cnt"], pots=s["pots"])
+    P.initial_measure()
+    ctx, R = gpu_run(torch, s)
+    R.initial_measure()
+    for k in range(nsteps):
+        P.step(it0 + k, grad_last=0)
+        R.step(it0 + k, grad_last=False)
+    ctx.sync()
+    N = ctx.N
+    d = R.dens.cpu().numpy()
+    for sl, name in [(slice(0, 2 * N), "nu"), (slice(2 * N, 3 * N), "rho_a"), (slice(3 * N, 4 * N), "tau_a"),
+                     (slice(7 * N, 8 * N), "rho_b"), (slice(8 * N, 9 * N), "tau_b")]:
+        assert rel(d[sl], P.dens[sl]) < TOL, name
+    jscale = s["kF"] * np.abs(P.dens[2 * N:3 * N]).max()
+    for k in range(dim):
+        for base in (4, 9):
+            sl = slice((base + k) * N, (base + k + 1) * N)
+            assert np.abs(d[sl] - P.dens[sl]).max() <= TOL * jscale, (base, k)
+    assert rel(R.pots.cpu().numpy()[:4 * N], P.pots[:4 * N]) < TOL
+    assert rel(R.pots.cpu().numpy()[4 * N:6 * N], P.pots[4 * N:6 * N]) < TOL
+    assert rel(R.wf.cpu().numpy(), P.wf) < 1e-11
+    e_ref = P.energy(it0 + nsteps)
+    e = R.energy(it0 + nsteps)
+    for k in (0, 1, 2, 7, 8):
+        assert abs(e[k] - e_ref[k]) <= TOL * max(abs(e_ref[k]), abs(e_ref[0])), k
+    ctx.close()
+
+
+def test_sldae_td_gauss_1d_golden_on_gpu(torch):
+    """The 28,600-step 1D td-gauss-sldae ss1 trajectory on the GPU: rows 0-7
+    to all 10 printed digits, later rows to 4e-5 relative (the oracle's bar)."""
+    say("case: td-gauss-sldae 1D golden trajectory on GPU")
+    s = tdgauss.setup(1, functional=po.SLDAE)
+    ctx, R = gpu_run(torch, s)
+    R.initial_measure()
+    ref = tdgauss.golden(po.SLDAE)["runs"]["ss1-1d"]["rows"]
+    it = 0
+    bad = []
+    for m in range(101):
+        if m > 0:
+            for k in range(s["timesteps"]):
+                R.step(it, grad_last=(k == s["timesteps"] - 1))
+                it += 1
+        row = tdgauss.wlog_row(R.energy(it), s["Effg"])
+        for col, val in ((2, row[0]), (3, row[1]), (4, row[2]), (5, row[3]), (6, row[4]), (7, row[5]),
+                         (8, row[6]), (10, row[8])):
+            r = ref[m][col]
+            ok = abs(val - r) < 2e-11 if col == 10 else tdgauss.digits_match(val, r)
+            if not ok and m >= 8:
+                ok = abs(val - r) <= 4e-5 * max(abs(r), 1e-3)
+            if not ok:
+                bad.append((m, col, val, r))
+    assert not bad, bad[:10]
+    ctx.close()

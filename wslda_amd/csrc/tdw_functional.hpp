@@ -6,6 +6,7 @@
 // tdwslda_functionals.h:47-306 (SLDA, ASLDA) and :373-452 (BdG).
 #pragma once
 #include "tdw_aslda.hpp"
+#include "tdw_sldae.hpp"
 #include "tdw_device.hpp"
 
 namespace tdw {
@@ -23,7 +24,7 @@ struct FParams {
     double p_nmin, p_nmax, denseps;
     double mu_a, mu_b, ec, dx;
     double R0, R1;
-    int cubic, mf, bdg, aslda;
+    int cubic, mf, bdg, aslda, sldae;
     double sclgth;
 };
 
@@ -249,6 +250,71 @@ __device__ __forceinline__ void aslda_point(const FParams &f, double na, double 
             double dl = mf_corr(f, p0, g.x, lkF, n, alph_plus);
             Vanew += dl * t7b;
             Vbnew += dl * t7b;
+        }
+        Va = TDW_UD_MIX * Vanew + (1.0 - TDW_UD_MIX) * Va;
+        Vb = TDW_UD_MIX * Vbnew + (1.0 - TDW_UD_MIX) * Vb;
+    }
+    delta = ld;
+}
+
+// SLDAE potentials at one point (tdwslda_functionals.h:530-812, with
+// CURRENT_CORRECTIONS): alpha_a = alpha_b = A(x), x = |a k_F(n)|, the vector
+// potentials as in ASLDA, V_s = U_ext + A' (tau_a + tau_b)/2 + beta e_F
+// + current terms - (A'/A) Re(Delta* nu) - C~'/A |Delta|^2, and
+// Delta = -nu A / (C~ - Lambda) with Lambda = Re of the regularised integral.
+__device__ __forceinline__ void sldae_point(const FParams &f, double na, double nb, double2 nu, double taua,
+                                            double taub, const double *ja, const double *jb, int dim, double cc,
+                                            double ua, double ub, double &Va, double &Vb, double2 &delta,
+                                            double &ala, double &alb, double *Aa, double *Ab)
+{
+    const double n = na + nb;
+    const sldae::Local L = sldae::local(n, p_reg(f, n), f.sclgth);
+    const double af = L.af, af_p = L.af_p;
+    ala = af;
+    alb = af;
+    const double fa = p_reg(f, na) * cc, fb = p_reg(f, nb) * cc;
+    double j2a = 0.0, j2b = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= dim) break;
+        Aa[d] = fa == 0.0 ? 0.0 : fa * (1. - af) * ja[d] / na;
+        Ab[d] = fb == 0.0 ? 0.0 : fb * (1. - af) * jb[d] / nb;
+        j2a += ja[d] * ja[d];
+        j2b += jb[d] * jb[d];
+    }
+    double Va_const = ua + af_p * (taua + taub) / 2.;
+    double Vb_const = ub + af_p * (taua + taub) / 2.;
+    Va_const += L.beta * L.eF;
+    Vb_const += L.beta * L.eF;
+    if (fa != 0.0) {
+        const double t7 = fa * j2a / (2.0 * na);
+        Va_const += ((af - 1.0) / na - af_p) * t7;
+        Va_const -= cc * der_p_reg(f, na) * (af - 1.0) * j2a / (2.0 * na);
+        Vb_const -= af_p * t7;
+    }
+    if (fb != 0.0) {
+        const double t7 = fb * j2b / (2.0 * nb);
+        Va_const -= af_p * t7;
+        Vb_const += ((af - 1.0) / nb - af_p) * t7;
+        Vb_const -= cc * der_p_reg(f, nb) * (af - 1.0) * j2b / (2.0 * nb);
+    }
+    double lkF = 0.0;
+    if (f.mf) lkF = pow(3.0 * M_PI * M_PI * n, 1.0 / 3.0);
+    double2 ld = make_double2(0.0, 0.0);
+    for (int k = 0; k < TDW_UD_SCITERS; k++) {
+        const double lmu = (f.mu_a - Va + f.mu_b - Vb) / 2.0;
+        double2 wz, p0;
+        reg_wz(f, lmu, af, af, wz, p0);
+        const double g_eff = af / (L.ctilde - wz.x);
+        ld = nu * (-1.0 * g_eff);
+        const double dn = ld.x * nu.x + ld.y * nu.y;   // Re(conj(delta) * nu)
+        const double d2 = ld.x * ld.x + ld.y * ld.y;
+        double Vanew = Va_const - (af_p / af) * dn - L.ctilde_p / af * d2;
+        double Vbnew = Vb_const - (af_p / af) * dn - L.ctilde_p / af * d2;
+        if (f.mf) {
+            const double dl = mf_corr(f, p0, g_eff, lkF, n, af);
+            Vanew += dl * d2;
+            Vbnew += dl * d2;
         }
         Va = TDW_UD_MIX * Vanew + (1.0 - TDW_UD_MIX) * Va;
         Vb = TDW_UD_MIX * Vbnew + (1.0 - TDW_UD_MIX) * Vb;

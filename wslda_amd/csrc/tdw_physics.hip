@@ -618,6 +618,7 @@ FParams fparams(const tdw_ctx *c)
     f.mf = c->p.mf_correction;
     f.bdg = c->p.functional == TDW_BDG;
     f.aslda = c->p.functional == TDW_ASLDA;
+    f.sldae = c->p.functional == TDW_SLDAE;
     f.sclgth = c->p.sclgth;
     return f;
 }
@@ -634,7 +635,7 @@ k_potentials(FParams f, VextDev V, long N, int dim, double cc, const double *__r
     vext_point(V, i, ua, ub);
     double Va = pots[i], Vb = pots[N + i];
     double2 d;
-    if (f.aslda) {
+    if (f.aslda || f.sldae) {
         double ja[3] = {0.0, 0.0, 0.0}, jb[3] = {0.0, 0.0, 0.0}, Aa[3], Ab[3], ala, alb;
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -642,8 +643,12 @@ k_potentials(FParams f, VextDev V, long N, int dim, double cc, const double *__r
                 ja[k] = dens[(4 + k) * N + i];
                 jb[k] = dens[(9 + k) * N + i];
             }
-        aslda_point(f, na, nb, nu, dens[3 * N + i], dens[8 * N + i], ja, jb, dim, cc, ua, ub, Va, Vb, d, ala, alb,
-                    Aa, Ab);
+        if (f.sldae)
+            sldae_point(f, na, nb, nu, dens[3 * N + i], dens[8 * N + i], ja, jb, dim, cc, ua, ub, Va, Vb, d, ala,
+                        alb, Aa, Ab);
+        else
+            aslda_point(f, na, nb, nu, dens[3 * N + i], dens[8 * N + i], ja, jb, dim, cc, ua, ub, Va, Vb, d, ala,
+                        alb, Aa, Ab);
         pots[4 * N + i] = ala;
         pots[5 * N + i] = alb;
 #pragma unroll
@@ -999,7 +1004,14 @@ k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restr
         const double ja2 = tx1 * tx1 + ty1 * ty1 + tz1 * tz1, jb2 = tx2 * tx2 + ty2 * ty2 + tz2 * tz2;
         const double ta = dens[3 * N + i] - p_reg(f, na) * ja2 / na;
         const double tb = dens[8 * N + i] - p_reg(f, nb) * jb2 / nb;
-        if (f.aslda) {
+        if (f.sldae) {
+            // tdwslda_functionals.h:815-889: A (tau~_a + tau~_b)/2, 3/5 B e_F n
+            const double nt = na + nb;
+            const double kF = pow(3. * M_PI * M_PI * nt, 1. / 3.);
+            const double x = fabs(f.sclgth * kF);
+            e[TDW_EKIN] = sldae::alpha0(x) * (ta + tb) / 2. * VE;
+            e[TDW_EPOT] = (3. / 5.) * sldae::bfun0(x) * (kF * kF / 2.) * nt * VE;
+        } else if (f.aslda) {
             const double p = polar(f, na, nb);
             e[TDW_EKIN] = 0.5 * (aslda::alpha(p) * ta + aslda::alpha(-1.0 * p) * tb) * VE;
             e[TDW_EPOT] = aslda::funD(na + nb, p) * VE;

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "tdw_aslda.hpp"
+#include "tdw_sldae.hpp"
 #include "wslda_td.h"
 
 typedef std::complex<double> zc;
@@ -108,7 +109,8 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
                                  double Tstop, double dT, double eps, double scmix, double muchange, int maxiter,
                                  int mu_rule, tdw_uniform *U)
 {
-    if (!c || !U || (c->functional != TDW_SLDA && c->functional != TDW_ASLDA && c->functional != TDW_BDG))
+    if (!c || !U || (c->functional != TDW_SLDA && c->functional != TDW_ASLDA && c->functional != TDW_BDG &&
+                     c->functional != TDW_SLDAE))
         return TDW_EINVAL;
     // BdG (solve_uniform_problem_bdg, pca_uniform.h:1105-1470): bare coupling
     // 4 pi a, no density functional in the mean fields, mu += muchange (n0 - n)
@@ -136,8 +138,16 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     const bool as = c->functional == TDW_ASLDA;
     const double p0r = as ? f.pol(n0_a, n0_b) : 0.0;
     const double dpa = as ? f.dpol(n0_a, n0_b, 0) : 0.0, dpb = as ? f.dpol(n0_a, n0_b, 1) : 0.0;
-    const double alph_plus = as ? tdw::aslda::alpha_plus(p0r) : 1.0;
-    const double alph_a = as ? tdw::aslda::alpha(p0r) : 1.0, alph_b = as ? tdw::aslda::alpha(-p0r) : 1.0;
+    // SLDAE (solve_uniform_problem_sldae, pca_uniform.h:1532-2043): A, beta,
+    // C~ of x = |a k_F| at the total density, alpha_a = alpha_b = A
+    const bool se = c->functional == TDW_SLDAE;
+    const double nt0 = n0_a + n0_b;
+    const tdw::sldae::Local SL = tdw::sldae::local(nt0, 1.0, c->sclgth);
+    const double kFt = pow(3.0 * PI * PI * nt0, 1.0 / 3.0);
+    const double xse = fabs(c->sclgth * kFt);
+    const double alph_plus = as ? tdw::aslda::alpha_plus(p0r) : se ? SL.af : 1.0;
+    const double alph_a = as ? tdw::aslda::alpha(p0r) : se ? SL.af : 1.0;
+    const double alph_b = as ? tdw::aslda::alpha(-p0r) : se ? SL.af : 1.0;
     const double dalphm_dna = as ? tdw::aslda::dalpha_minus_dp(p0r) * dpa : 0.0;
     const double dalphm_dnb = as ? tdw::aslda::dalpha_minus_dp(p0r) * dpb : 0.0;
     const double dalphp_dna = as ? tdw::aslda::dalpha_plus_dp(p0r) * dpa : 0.0;
@@ -147,14 +157,18 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     const double dtC_dnb = as ? tdw::aslda::der_tildeC(n0, p0r, dpb, c->denseps) : dtC_dna;
     const double dD_dna = as ? tdw::aslda::der_funD(n0, p0r, dpa) : f.dD(n0_a, n0_b, 0);
     const double dD_dnb = as ? tdw::aslda::der_funD(n0, p0r, dpb) : f.dD(n0_a, n0_b, 1);
-    const double tC = as ? tdw::aslda::tildeC(n0, p0r, c->denseps) : f.tC(n0_a, n0_b);
-    const double Dn = as ? tdw::aslda::funD(n0, p0r) : f.D(n0_a, n0_b);
+    const double tC = as ? tdw::aslda::tildeC(n0, p0r, c->denseps) : se ? SL.ctilde : f.tC(n0_a, n0_b);
+    const double Dn = as ? tdw::aslda::funD(n0, p0r) : se ? (3. / 5.) * SL.bf * nt0 * SL.eF : f.D(n0_a, n0_b);
     double eF_a = pow(6.0 * PI * PI * n0_a, 2.0 / 3.0) / 2.0;
     double eF_b = pow(6.0 * PI * PI * n0_b, 2.0 / 3.0) / 2.0;
     double eF_avg = pow(3.0 * PI * PI * (n0_a + n0_b), 2.0 / 3.0) / 2.0;
     double kc = kc_in, ec = c->ec;
     const double mu0 = bdg ? 0.96087287 : 0.37, d0 = bdg ? 0.21 : 0.5;
     double mu_a = mu0 * eF_a, mu_b = mu0 * eF_b;
+    if (se) {
+        mu_a = tdw::sldae::chem_pot0(xse) * eF_a;
+        mu_b = tdw::sldae::chem_pot0(xse) * eF_b;
+    }
     double tau_a = pow(6.0 * PI * PI * n0_a, 5.0 / 3.0) / (10.0 * PI * PI);
     double tau_b = pow(6.0 * PI * PI * n0_b, 5.0 / 3.0) / (10.0 * PI * PI);
     double delta = (n0_a < n0_b) ? d0 * pow(6.0 * PI * PI * n0_a, 2.0 / 3.0) / 2.0
@@ -171,7 +185,11 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
         for (iter = 0; iter < maxiter; iter++) {
             double tau_a_old = tau_a, tau_b_old = tau_b, delta_old = delta, nu_old = nu;
             double V_a_old = V_a, V_b_old = V_b;
-            if (as) {
+            if (se) {
+                V_a = SL.af_p * (tau_a + tau_b) / 2. + SL.beta * SL.eF - SL.ctilde_p * delta * delta / SL.af -
+                      SL.af_p * delta * nu / SL.af;
+                V_b = V_a;
+            } else if (as) {
                 const double tau_p = tau_a + tau_b, tau_m = tau_a - tau_b;
                 V_a = dalphm_dna * tau_m / 2.0 + dalphp_dna * (tau_p / 2.0 - delta * nu / alph_plus) -
                       dtC_dna * delta * delta / alph_plus + dD_dna;
@@ -189,7 +207,8 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
             zc wz = std::log((kc + p0) / (kc - p0));
             if (wz.imag() < 0.) wz += zc(0.0, 2. * PI);
             wz = kc * R0 * (1. - (p0 / kc) * R1 * wz);
-            double g_eff = bdg ? (1.0 / (1.0 / gbare - wz)).real() : (alph_plus / (tC - wz)).real();
+            double g_eff = bdg ? (1.0 / (1.0 / gbare - wz)).real()
+                               : se ? alph_plus / (tC - wz.real()) : (alph_plus / (tC - wz)).real();
             delta = -1.0 * g_eff * nu;
             if (c->mf_correction) {
                 double kF = pow(3.0 * PI * PI * (n0_a + n0_b), 1.0 / 3.0);

@@ -16,7 +16,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libwslda_td.so")
 
-SLDA, ASLDA, BDG = 111, 112, 113
+SLDA, ASLDA, BDG, SLDAE = 111, 112, 113, 115
 SPHERICAL_CUTOFF, CUBIC_CUTOFF = 88, 89
 AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
 FP64, SPLIT = 0, 1
