@@ -110,6 +110,8 @@ typedef struct {
                             components into spin b and copy it to a       */
     double kF;           /* reference kF (memcopy_const); n_F = kF^3/(3 pi^2) */
     double qfalpha, qfbeta, qfgamma; /* quantum-friction strengths (md.qf*) */
+    int force_a1;        /* SLDAE built with SLDA_FORCE_A1: A = 1, the A1
+                            parameter set, constant-mass (FAST) step       */
     int precision;       /* TDW_FP64 or TDW_SPLIT                              */
 } tdw_params;
 

@@ -39,7 +39,8 @@ class Cfg(C.Structure):
                 ("dt", C.c_double), ("t0", C.c_double), ("sclgth", C.c_double),
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
                 ("vext", Vext), ("spinsymmetry", C.c_int),
-                ("kF", C.c_double), ("qfalpha", C.c_double), ("qfbeta", C.c_double), ("qfgamma", C.c_double)]
+                ("kF", C.c_double), ("qfalpha", C.c_double), ("qfbeta", C.c_double), ("qfgamma", C.c_double),
+                ("force_a1", C.c_int)]
 
 
 class Uniform(C.Structure):
@@ -126,7 +127,7 @@ def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
              regularization=SPHERICAL, mf_correction=0, scheme=AB4AM5,
              mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
              p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, spinsymmetry=0, kF=1.0,
-             qf=(0.0, 0.0, 0.0)):
+             qf=(0.0, 0.0, 0.0), force_a1=0):
     c = Cfg()
     c.dim, c.nx, c.ny, c.nz = dim, nx, ny, nz
     c.dx = dx
@@ -140,6 +141,7 @@ def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
     c.spinsymmetry = spinsymmetry
     c.kF = kF
     c.qfalpha, c.qfbeta, c.qfgamma = qf
+    c.force_a1 = force_a1
     return c
 
 

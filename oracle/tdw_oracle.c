@@ -917,12 +917,20 @@ static const double SE_B[8] = {1.29871, 6.02212, 0.376541, 0.810972, 6.47079, 3.
 static const double SE_C[8] = {0.267952, 0.754923, 0.330699, 0.0900897, 2.61695, 2.71855, 0.550587, 0.0900897};
 #define SE_BF (-0.281863)
 #define SE_CF (-14.95950)
-static double se_alpha(double x) { return 1. + se_pade(x, SE_ALPHA); }
-static double se_alpha_d(double x) { return se_pade_d(x, SE_ALPHA); }
-static double se_beta(double x) { return SE_BF * se_pade(x, SE_BETA); }
-static double se_igamma(double x) { return (1. / SE_CF) * se_pade(x, SE_IGAM); }
-static double se_bfun(double x) { return SE_BF * se_pade(x, SE_B); }
-static double se_cfun(double x) { return SE_CF * se_pade(x, SE_C); }
+/* SLDA_FORCE_A1 set (sldae_parameters_A1.h:1-36): A = 1 */
+static const double SE1_BETA[8] = {0.797400, 0.943961, 1.037043, 0.523445, 1.227062, 2.126210, 1.734795, 0.523445};
+static const double SE1_IGAM[8] = {2.713374, 1.441196, 3.700251, 1.877961, 0.634303, 3.522427, 3.701347, 1.877961};
+static const double SE1_B[8] = {0.698112, 0.034678, 0.455138, -0.031327, 0.353905, 0.806444, 0.403679, -0.031327};
+static const double SE1_C[8] = {0.370145, 1.041299, 1.737347, 1.085400, 2.458364, 5.604947, 3.828124, 1.085400};
+#define SE1_BF (-0.481506)
+#define SE1_CF (-10.532397)
+static int g_se_a1 = 0;   /* parameter set of the current call (set by the entry points) */
+static double se_alpha(double x) { return g_se_a1 ? 1. : 1. + se_pade(x, SE_ALPHA); }
+static double se_alpha_d(double x) { return g_se_a1 ? 0. : se_pade_d(x, SE_ALPHA); }
+static double se_beta(double x) { return g_se_a1 ? SE1_BF * se_pade(x, SE1_BETA) : SE_BF * se_pade(x, SE_BETA); }
+static double se_igamma(double x) { return g_se_a1 ? (1. / SE1_CF) * se_pade(x, SE1_IGAM) : (1. / SE_CF) * se_pade(x, SE_IGAM); }
+static double se_bfun(double x) { return g_se_a1 ? SE1_BF * se_pade(x, SE1_B) : SE_BF * se_pade(x, SE_B); }
+static double se_cfun(double x) { return g_se_a1 ? SE1_CF * se_pade(x, SE1_C) : SE_CF * se_pade(x, SE_C); }
 static double se_mu_ufg(double x)
 {
     double U = 5. / 24., V = U / tan(3. * M_PI / 16. * (1. - 0.3582341));
@@ -958,6 +966,7 @@ static se_local se_coeffs(double nt, double ntreg, double as)
 
 void tdwo_potentials(const tdwo_cfg *c, int it, const double *dens, double *pots)
 {
+    g_se_a1 = c->force_a1;
     size_t N = tdwo_npts(c);
     const zc *nu = (const zc *)dens;
     const double *rho_a = dens + 2 * N, *rho_b = dens + 7 * N;
@@ -978,12 +987,16 @@ void tdwo_potentials(const tdwo_cfg *c, int it, const double *dens, double *pots
             /* tdwslda_functionals.h:530-812, CURRENT_CORRECTIONS, cccoeff = 1 */
             double nt = na + nb;
             se_local L = se_coeffs(nt, p_reg(c, nt), c->sclgth);
-            al_a[i] = L.af;
-            al_b[i] = L.af;
             const double *ja = dens + 4 * N, *jb = dens + 9 * N;
             double *Aa = pots + 6 * N, *Ab = pots + 9 * N;
-            double fa = p_reg(c, na), fb = p_reg(c, nb), j2a = 0.0, j2b = 0.0;
-            for (int d = 0; d < c->dim; d++) {
+            /* SLDA_FORCE_A1: FAST_CONST_EFFECTIVE_MASS_MODE, no CURRENT_CORRECTIONS */
+            const int cc_on = !c->force_a1;
+            double fa = cc_on ? p_reg(c, na) : 0.0, fb = cc_on ? p_reg(c, nb) : 0.0, j2a = 0.0, j2b = 0.0;
+            if (cc_on) {
+                al_a[i] = L.af;
+                al_b[i] = L.af;
+            }
+            for (int d = 0; d < c->dim && cc_on; d++) {
                 Aa[d * N + i] = fa == 0.0 ? 0.0 : fa * (1. - L.af) * ja[d * N + i] / na;
                 Ab[d * N + i] = fb == 0.0 ? 0.0 : fb * (1. - L.af) * jb[d * N + i] / nb;
                 j2a += ja[d * N + i] * ja[d * N + i];
@@ -1254,7 +1267,7 @@ void tdwo_apply_h_full(const tdwo_cfg *c, int n, const double *in, double *out, 
                        const double *ddy, const double *ddz, const double *lap, const double *alap,
                        const double *pots, const double *kt, const double *useqpe, double *qpe)
 {
-    if (c->functional != TDWO_ASLDA && c->functional != TDWO_SLDAE) {
+    if (c->functional != TDWO_ASLDA && !(c->functional == TDWO_SLDAE && !c->force_a1)) {
         tdwo_apply_h(c, n, in, out, lap, pots, kt, useqpe, qpe);
         return;
     }
@@ -1359,6 +1372,7 @@ void tdwo_alpha_wf(const tdwo_cfg *c, int n, const double *in, const double *pot
  * ========================================================================= */
 void tdwo_energy(const tdwo_cfg *c, int it, const double *dens, const double *pots, double *E)
 {
+    g_se_a1 = c->force_a1;
     size_t N = tdwo_npts(c);
     const zc *nu = (const zc *)dens;
     const double *rho_a = dens + 2 * N, *tau_a = dens + 3 * N, *jax = dens + 4 * N, *jay = dens + 5 * N, *jaz = dens + 6 * N;
@@ -1439,7 +1453,7 @@ static void half_densities(const tdwo_cfg *c, tdwo_state *s, int grad)
 static void half_h(const tdwo_cfg *c, tdwo_state *s, double *out)
 {
     int n = s->n;
-    if (c->functional == TDWO_ASLDA || c->functional == TDWO_SLDAE) {
+    if (c->functional == TDWO_ASLDA || (c->functional == TDWO_SLDAE && !c->force_a1)) {
         size_t N = tdwo_npts(c);
         double *al = (double *)malloc(sizeof(double) * 4 * (size_t)n * N);
         tdwo_alpha_wf(c, n, s->wf, s->pots, al);
@@ -1455,7 +1469,7 @@ int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
 {
     int n = s->n;
     int last = c->scheme == TDWO_AB3AM4 ? 2 : c->scheme == TDWO_AB4AM5 ? 3 : 4;
-    const int full = c->functional == TDWO_ASLDA || c->functional == TDWO_SLDAE;
+    const int full = c->functional == TDWO_ASLDA || (c->functional == TDWO_SLDAE && !c->force_a1);
     /* predictor */
     tdwo_abm_step1(c, n, s->wf, s->f[0], s->f[1], s->f[2], s->f[3], s->f[4]);
     tdwo_normalize(c, n, s->wf);
@@ -1665,6 +1679,7 @@ int tdwo_uniform_solve(const tdwo_cfg *c, double n0_a, double n0_b, double kc_in
 {
     if (c->functional == TDWO_BDG)
         return uniform_solve_bdg(c, n0_a, n0_b, kc_in, Tstart, Tstop, dT, eps, scmix, muchange, maxiter, U);
+    g_se_a1 = c->force_a1;
     int NX = c->nx, NY = c->ny, NZ = c->nz;
     double LX = c->dx * NX, LY = c->dy * NY, LZ = c->dz * NZ, LXYZ = LX * LY * LZ;
     double *kkx = klist(NX, LX), *kky = klist(NY, LY), *kkz = klist(NZ, LZ);

@@ -117,6 +117,7 @@ typedef struct {
     int spinsymmetry;     /* 1: SPINSYMMETRY_MODE (E > 0 states only)      */
     double kF;            /* reference kF: n_F = kF^3/(3 pi^2) (dc_nF)      */
     double qfalpha, qfbeta, qfgamma; /* quantum-friction strengths        */
+    int force_a1;         /* SLDAE with SLDA_FORCE_A1: A = 1, FAST mode     */
 } tdwo_cfg;
 
 /* Number of evolved lattice points (NUMBER_ELEMENT). */

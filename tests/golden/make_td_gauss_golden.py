@@ -1,5 +1,5 @@
 """Extract the td-gauss-slda, td-gauss-aslda, td-gauss-bdg and
-td-gauss-sldae golden trajectories into small JSON fixtures.
+td-gauss-sldae(-a1) golden trajectories into small JSON fixtures.
 
 Reads the reference testsuite's own expected outputs (data, not code):
   testsuite/td-gauss-{slda,aslda,bdg}/ref/test.ss0-{1d,2d,3d}.wlog.ref
@@ -64,7 +64,7 @@ def params(path, flip):
 # td-gauss-sldae: the spin-symmetric ss1 runs (Makefile.1.s.34:
 # SPINSYMMETRY_MODE, AB3AM4), N_a = N_b, sclgth = -1.1, no sign flip
 for suite, functional, ss in (("slda", "SLDA", "ss0"), ("aslda", "ASLDA", "ss0"), ("bdg", "BDG", "ss0"),
-                              ("sldae", "SLDAE", "ss1")):
+                              ("sldae", "SLDAE", "ss1"), ("sldae-a1", "SLDAE", "ss1")):
     data = {"columns": ["id", "t_eF", "Na", "Nb", "N", "Etot/Effg", "Ekin/Effg", "Epot/Effg",
                         "Epair/Effg", "Ecurr/Effg", "Eext/Effg", "Edext/Effg", "Evext/Effg"],
             "inputs": {"NX": 12, "NY": 12, "NZ": 12, "DX": 1.15, "functional": functional,
@@ -75,13 +75,16 @@ for suite, functional, ss in (("slda", "SLDA", "ss0"), ("aslda", "ASLDA", "ss0")
                        },
             "runs": {}}
     if ss == "ss1":
-        data["inputs"].update({"scheme": "AB3AM4", "spinsymmetry": 1, "sclgth": -1.1})
+        # td-gauss-sldae: Makefile.1.s.34 (AB3AM4); td-gauss-sldae-a1:
+        # Makefile.1.s (default AB4AM5) with SLDA_FORCE_A1 in predefines.h
+        data["inputs"].update({"scheme": "AB4AM5" if suite == "sldae-a1" else "AB3AM4", "spinsymmetry": 1,
+                               "sclgth": -1.1, "force_a1": int(suite == "sldae-a1")})
     for d, tag in ((1, ss + "-1d"), (2, ss + "-2d"), (3, ss + "-3d")):
         p = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.wlog.ref" % (suite, tag))
         inp = os.path.join(REF, "testsuite/td-gauss-%s/input.txt.%d" % (suite, d))
         c = os.path.join(REF, "testsuite/td-gauss-%s/ref/test.%s.cmp.ref" % (suite, tag))
         data["runs"][tag] = {"header": header(p), "rows": rows(p), "params": params(inp, ss == "ss0"),
                              "final": final_values(c)}
-    out = os.path.join(HERE, "td_gauss_%s.json" % suite)
+    out = os.path.join(HERE, "td_gauss_%s.json" % suite.replace("-", "_"))
     json.dump(data, open(out, "w"), indent=0)
     print("wrote", out)

@@ -14,9 +14,10 @@ from oracle import pyoracle as po
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def golden(functional=po.SLDA):
+def golden(functional=po.SLDA, force_a1=0):
     name = {po.BDG: "td_gauss_bdg.json", po.ASLDA: "td_gauss_aslda.json",
-            po.SLDAE: "td_gauss_sldae.json"}.get(functional, "td_gauss_slda.json")
+            po.SLDAE: "td_gauss_sldae_a1.json" if force_a1 else "td_gauss_sldae.json"}.get(
+        functional, "td_gauss_slda.json")
     return json.load(open(os.path.join(HERE, "golden", name)))
 
 
@@ -34,25 +35,27 @@ def run_tag(functional, dim):
     return ("ss1-%dd" if functional == po.SLDAE else "ss0-%dd") % dim
 
 
-def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
+def setup(dim, scheme=po.AB4AM5, functional=po.SLDA, force_a1=0, mu_rule=None):
     """td-gauss-slda (default), td-gauss-aslda (functional=ASLDA: the
     uniform alpha_a, alpha_b go into the effective-mass slots of pots) or
     td-gauss-bdg (functional=BDG, scattering length aBdG, BdG uniform
     solution) problem."""
-    g = golden(functional)["inputs"]
+    G = golden(functional, force_a1)
+    g = G["inputs"]
     NX, DXv = g["NX"], g["DX"]
     LXYZ = (NX * DXv) ** 3
     Na = math.ceil(1.0 / (6.0 * math.pi ** 2) * LXYZ)
     ss = int(g.get("spinsymmetry", 0))
     Nb = Na if ss else Na + 1          # UNIFORM_TEST_MODE, pca.c:198-199
     if functional == po.SLDAE:
-        scheme = po.AB3AM4              # Makefile.1.s.34
+        scheme = po.AB4AM5 if force_a1 else po.AB3AM4   # Makefile.1.s / Makefile.1.s.34
     cfg = po.make_cfg(dim, NX, NX, NX, dx=DXv, scheme=scheme, functional=functional, spinsymmetry=ss,
-                      sclgth=g["aBdG"] if functional == po.BDG else g.get("sclgth", 0.0))
+                      sclgth=g["aBdG"] if functional == po.BDG else g.get("sclgth", 0.0), force_a1=force_a1)
     kc = 0.999999 * math.pi / DXv
     u, rc = po.uniform_solve(cfg, Na / LXYZ, Nb / LXYZ, kc, g["temperature"], eps=g["init0eps"],
                              scmix=g["init0scmix"], muchange=g["init0muchange"],
-                             maxiter=g["init0maxiter"], mu_rule=mu_rule_of(functional))
+                             maxiter=g["init0maxiter"],
+                             mu_rule=mu_rule_of(functional) if mu_rule is None else mu_rule)
     assert rc == 0
     n = u.n0_a + u.n0_b
     eF = (3 * math.pi ** 2 * n) ** (2 / 3) / 2
@@ -60,7 +63,7 @@ def setup(dim, scheme=po.AB4AM5, functional=po.SLDA):
     Effg = 0.6 * n * eF * LXYZ
     wf, En, kt, cnt = po.uniform_states(cfg, u)
     fb = np.array([po.fbeta(E, u.beta) for E in En])
-    p = golden(functional)["runs"][run_tag(functional, dim)]["params"]
+    p = G["runs"][run_tag(functional, dim)]["params"]
     A = p[0] * eF
 
     def wcoef(s):

@@ -35,11 +35,11 @@ def torch():
     return t
 
 
-def gpu_run(torch, s):
+def gpu_run(torch, s, fused=False):
     from wslda_amd import td
     ctx = td.TDContext(tdgauss.to_params(s["cfg"]), s["wf"].shape[0] // 2, 0)
     ctx.set_states(s["fbeta"], s["kt"], s["cnt"])
-    return ctx, td.TDRun(ctx, s["wf"], s["pots"])
+    return ctx, td.TDRun(ctx, s["wf"], s["pots"], fused=fused)
 
 
 def test_sldae_uniform_solution(torch):
@@ -119,4 +119,38 @@ def test_sldae_td_gauss_1d_golden_on_gpu(torch):
             if not ok:
                 bad.append((m, col, val, r))
     assert not bad, bad[:10]
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,nsteps,fused", [(1, 286, False), (2, 40, False), (3, 6, False), (3, 6, True)])
+def test_sldae_a1_pece_steps(torch, dim, nsteps, fused):
+    """SLDAE with SLDA_FORCE_A1 runs the constant-mass step (modular, and the
+    fused 3D step): densities and energies against the oracle."""
+    from wslda_amd import td
+    say("case: SLDAE-A1 PECE steps dim=%d nsteps=%d fused=%d" % (dim, nsteps, fused))
+    s = tdgauss.setup(dim, functional=po.SLDAE, force_a1=1)
+    it0 = int(round(6.0 / s["eF"] / s["cfg"].dt))
+    P = po.Problem(s["cfg"], s["wf"], s["fbeta"], kt=s["kt"], cnt=s["cnt"], pots=s["pots"])
+    P.initial_measure()
+    ctx, R = gpu_run(torch, s, fused=fused)
+    if fused:
+        assert td.lib().tdw_fused_supported(ctx.h) == 1
+    R.initial_measure()
+    for k in range(nsteps):
+        last = k == nsteps - 1
+        P.step(it0 + k, grad_last=last)
+        R.step(it0 + k, grad_last=last)
+    if fused:
+        R.finalize()
+    ctx.sync()
+    N = ctx.N
+    d = R.dens.cpu().numpy()
+    for sl, name in [(slice(0, 2 * N), "nu"), (slice(2 * N, 3 * N), "rho_a"), (slice(3 * N, 4 * N), "tau_a"),
+                     (slice(7 * N, 8 * N), "rho_b")]:
+        assert rel(d[sl], P.dens[sl]) < TOL, name
+    assert rel(R.wf.cpu().numpy(), P.wf) < 1e-11
+    e_ref = P.energy(it0 + nsteps)
+    e = R.energy(it0 + nsteps)
+    for k in (0, 1, 2, 7, 8):
+        assert abs(e[k] - e_ref[k]) <= TOL * max(abs(e_ref[k]), abs(e_ref[0])), k
     ctx.close()

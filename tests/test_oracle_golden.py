@@ -12,15 +12,15 @@ from tests import tdgauss
 CHECKED = [(2, "Na"), (3, "Nb"), (4, "N"), (5, "Etot"), (6, "Ekin"), (7, "Epot"), (8, "Epair"), (10, "Eext")]
 
 
-def run_and_compare(dim, tag, nmeas, functional=po.SLDA, exact_rows=None, rel_tol=None):
+def run_and_compare(dim, tag, nmeas, functional=po.SLDA, exact_rows=None, rel_tol=None, force_a1=0):
     """Steps the oracle through the first nmeas measurements of a golden run.
     Rows below exact_rows (all rows by default) must equal the golden to its
     10 printed digits; later rows must agree to rel_tol relative. Returns the
     oracle's last energy vector."""
-    s = tdgauss.setup(dim, functional=functional)
+    s = tdgauss.setup(dim, functional=functional, force_a1=force_a1)
     P = po.Problem(s["cfg"], s["wf"], s["fbeta"], kt=s["kt"], cnt=s["cnt"], pots=s["pots"])
     P.initial_measure()
-    ref = tdgauss.golden(functional)["runs"][tag]["rows"]
+    ref = tdgauss.golden(functional, force_a1)["runs"][tag]["rows"]
     it = 0
     bad = []
     for m in range(nmeas + 1):
@@ -140,3 +140,15 @@ def test_td_gauss_sldae_1d_full_trajectory():
 def test_td_gauss_sldae_2d_first_rows():
     """2D SLDAE: the first 8 rows to all 10 printed digits."""
     run_and_compare(2, "ss1-2d", 7, functional=po.SLDAE)
+
+
+def test_td_gauss_sldae_a1_1d_full_trajectory():
+    """SLDAE built with SLDA_FORCE_A1 (A = 1, the A1 parameter set, constant
+    mass, spin-symmetric, AB4AM5; testsuite/td-gauss-sldae-a1, code of
+    2022.01.31): all 101 rows of the 1D run to 1e-8 relative (most to all
+    10 printed digits; the largest departure is 5.4e-9)."""
+    run_and_compare(1, "ss1-1d", 100, functional=po.SLDAE, exact_rows=0, rel_tol=1e-8, force_a1=1)
+
+
+def test_td_gauss_sldae_a1_2d_first_rows():
+    run_and_compare(2, "ss1-2d", 7, functional=po.SLDAE, exact_rows=0, rel_tol=1e-8, force_a1=1)

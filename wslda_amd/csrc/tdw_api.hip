@@ -372,7 +372,7 @@ extern "C" int tdw_apply_h(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw
 {
     if (!c || n > c->nwf || in == out) return TDW_EINVAL;
     // ASLDA's H needs the gradients and lap(alpha psi): tdw_apply_h_full
-    if ((c->p.functional == TDW_ASLDA || c->p.functional == TDW_SLDAE)) return TDW_EINVAL;
+    if (varying_mass(c)) return TDW_EINVAL;
     return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
 }
 extern "C" int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
@@ -381,7 +381,7 @@ extern "C" int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in
                                 const double *useqpe, double *qpe)
 {
     if (!c || n > c->nwf || in == out) return TDW_EINVAL;
-    if (!(c->p.functional == TDW_ASLDA || c->p.functional == TDW_SLDAE))
+    if (!varying_mass(c))
         return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
     if (!ddx || (c->dim >= 2 && !ddy) || (c->dim >= 3 && !ddz) || out == ddx || out == ddy || out == ddz)
         return TDW_EINVAL;
@@ -474,7 +474,7 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(fused_finalize(c, s));
     const int n = s->n, k = nhist(c);
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
-    if ((c->p.functional == TDW_ASLDA || c->p.functional == TDW_SLDAE)) {
+    if (varying_mass(c)) {
         if (!s->ddx || (c->dim >= 2 && !s->ddy) || (c->dim >= 3 && !s->ddz)) return TDW_EINVAL;
         return step_aslda(c, s, it, k, qpe);
     }
@@ -826,7 +826,7 @@ static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *
         qf = dens + 12 * c->N;
         TDW_TRY(qf_add(c, qf, pots, qfswitch, pccoeff));
     }
-    if ((c->p.functional == TDW_ASLDA || c->p.functional == TDW_SLDAE)) {
+    if (varying_mass(c)) {
         if (!dx || (c->dim >= 2 && !dy) || (c->dim >= 3 && !dz) || !alap || in == out) return TDW_EINVAL;
         TDW_TRY(apply_h_aslda(c, n, CD2(in), D2(out), CD2(dx), CD2(dy), CD2(dz), CD2(lap), CD2(alap), pots, kt,
                               useqpe, c->d_qpe));

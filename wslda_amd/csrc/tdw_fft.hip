@@ -500,7 +500,7 @@ static bool h2d_for(const tdw_ctx *c, h2d_fn *f, size_t *lds)
 bool h2d_supported(const tdw_ctx *c)
 {
     // the plane kernel applies the constant-effective-mass Hamiltonian
-    if ((c->p.functional == TDW_ASLDA || c->p.functional == TDW_SLDAE)) return false;
+    if (varying_mass(c)) return false;
     h2d_fn f;
     size_t lds;
     return h2d_for(c, &f, &lds);

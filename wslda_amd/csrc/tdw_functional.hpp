@@ -24,7 +24,7 @@ struct FParams {
     double p_nmin, p_nmax, denseps;
     double mu_a, mu_b, ec, dx;
     double R0, R1;
-    int cubic, mf, bdg, aslda, sldae;
+    int cubic, mf, bdg, aslda, sldae, a1;
     double sclgth;
 };
 
@@ -268,11 +268,12 @@ __device__ __forceinline__ void sldae_point(const FParams &f, double na, double 
                                             double &ala, double &alb, double *Aa, double *Ab)
 {
     const double n = na + nb;
-    const sldae::Local L = sldae::local(n, p_reg(f, n), f.sclgth);
+    const sldae::Local L = sldae::local(n, p_reg(f, n), f.sclgth, f.a1);
     const double af = L.af, af_p = L.af_p;
     ala = af;
     alb = af;
-    const double fa = p_reg(f, na) * cc, fb = p_reg(f, nb) * cc;
+    // SLDA_FORCE_A1: FAST_CONST_EFFECTIVE_MASS_MODE, no current corrections
+    const double fa = f.a1 ? 0.0 : p_reg(f, na) * cc, fb = f.a1 ? 0.0 : p_reg(f, nb) * cc;
     double j2a = 0.0, j2b = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {

@@ -507,7 +507,7 @@ bool fused_supported(const tdw_ctx *c)
 {
     if (c->dim != 3) return false;
     // the fused passes apply the constant-effective-mass Hamiltonian
-    if ((c->p.functional == TDW_ASLDA || c->p.functional == TDW_SLDAE)) return false;
+    if (varying_mass(c)) return false;
     if (c->p.precision != TDW_FP64 && c->p.precision != TDW_SPLIT) return false;
     FusedFns<false>::pred_fn p;
     FusedFns<false>::corr_fn cf;

@@ -74,6 +74,13 @@ struct tdw_ctx {
 };
 
 namespace tdw {
+// The functionals whose effective mass varies in space: the step takes
+// derivatives on every half-step and H carries the alpha, A terms
+// (ASLDA; SLDAE unless built with SLDA_FORCE_A1).
+inline bool varying_mass(const tdw_ctx *c)
+{
+    return c->p.functional == TDW_ASLDA || (c->p.functional == TDW_SLDAE && !c->p.force_a1);
+}
 // Brackets one kernel launch with events when timing is enabled.
 // Records nest (a timed launcher may call other timed launchers): each scope
 // closes its own record.

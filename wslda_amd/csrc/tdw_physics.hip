@@ -619,6 +619,7 @@ FParams fparams(const tdw_ctx *c)
     f.bdg = c->p.functional == TDW_BDG;
     f.aslda = c->p.functional == TDW_ASLDA;
     f.sldae = c->p.functional == TDW_SLDAE;
+    f.a1 = c->p.force_a1 != 0;
     f.sclgth = c->p.sclgth;
     return f;
 }
@@ -649,14 +650,18 @@ k_potentials(FParams f, VextDev V, long N, int dim, double cc, const double *__r
         else
             aslda_point(f, na, nb, nu, dens[3 * N + i], dens[8 * N + i], ja, jb, dim, cc, ua, ub, Va, Vb, d, ala,
                         alb, Aa, Ab);
-        pots[4 * N + i] = ala;
-        pots[5 * N + i] = alb;
+        // the effective masses and vector potentials are saved only with
+        // CURRENT_CORRECTIONS (not in SLDA_FORCE_A1 builds)
+        if (!(f.sldae && f.a1)) {
+            pots[4 * N + i] = ala;
+            pots[5 * N + i] = alb;
 #pragma unroll
-        for (int k = 0; k < 3; k++)
-            if (k < dim) {
-                pots[(6 + k) * N + i] = Aa[k];
-                pots[(9 + k) * N + i] = Ab[k];
-            }
+            for (int k = 0; k < 3; k++)
+                if (k < dim) {
+                    pots[(6 + k) * N + i] = Aa[k];
+                    pots[(9 + k) * N + i] = Ab[k];
+                }
+        }
     } else if (f.bdg) bdg_point(f, na, nb, nu, ua, ub, pots[4 * N + i], pots[5 * N + i], Va, Vb, d);
     else slda_point(f, na, nb, nu, ua, ub, Va, Vb, d);
     pots[i] = Va;
@@ -1009,8 +1014,8 @@ k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restr
             const double nt = na + nb;
             const double kF = pow(3. * M_PI * M_PI * nt, 1. / 3.);
             const double x = fabs(f.sclgth * kF);
-            e[TDW_EKIN] = sldae::alpha0(x) * (ta + tb) / 2. * VE;
-            e[TDW_EPOT] = (3. / 5.) * sldae::bfun0(x) * (kF * kF / 2.) * nt * VE;
+            e[TDW_EKIN] = sldae::alpha0(x, f.a1) * (ta + tb) / 2. * VE;
+            e[TDW_EPOT] = (3. / 5.) * sldae::bfun0(x, f.a1) * (kF * kF / 2.) * nt * VE;
         } else if (f.aslda) {
             const double p = polar(f, na, nb);
             e[TDW_EKIN] = 0.5 * (aslda::alpha(p) * ta + aslda::alpha(-1.0 * p) * tb) * VE;

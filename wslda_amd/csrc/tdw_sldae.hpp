@@ -5,9 +5,9 @@
 // chemical potential that seeds the uniform solver.
 //
 // Restates hpc-engine/sldae_functional.c:14-330 with the default parameter
-// set of sldae_parameters.h:1-75 (the td-gauss-sldae build: neither
-// SLDAE_FORCE_A1 nor SLDA_FORCE_A1) and the constants of
-// sldae_functional.h:36-56.
+// set of sldae_parameters.h:1-75 (the td-gauss-sldae build) or, with a1,
+// the SLDA_FORCE_A1 set of sldae_parameters_A1.h:1-36 (td-gauss-sldae-a1),
+// and the constants of sldae_functional.h:36-56.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -51,12 +51,31 @@ __host__ __device__ inline Pade inv_gamma_fit() { return {6.15853, 0.156722, 0.7
 __host__ __device__ inline Pade b_fit() { return {1.29871, 6.02212, 0.376541, 0.810972, 6.47079, 3.73389, 3.47301, 0.810972}; }
 __host__ __device__ inline Pade c_fit() { return {0.267952, 0.754923, 0.330699, 0.0900897, 2.61695, 2.71855, 0.550587, 0.0900897}; }
 
-__host__ __device__ inline double alpha0(double x) { return 1. + pade0(alpha_fit(), x); }   // = A(x)
-__host__ __device__ inline double alpha1(double x) { return pade1(alpha_fit(), x); }
-__host__ __device__ inline double beta0(double x) { return BF_UFG * pade0(beta_fit(), x); }
-__host__ __device__ inline double inv_gamma0(double x) { return (1. / CF_UFG) * pade0(inv_gamma_fit(), x); }
-__host__ __device__ inline double bfun0(double x) { return BF_UFG * pade0(b_fit(), x); }
-__host__ __device__ inline double cfun0(double x) { return CF_UFG * pade0(c_fit(), x); }
+// SLDA_FORCE_A1 builds (sldae_parameters_A1.h:1-36): A = 1 and these fits
+constexpr double BF1_UFG = -0.481506, CF1_UFG = -10.532397;
+__host__ __device__ inline Pade beta_fit_a1() { return {0.797400, 0.943961, 1.037043, 0.523445, 1.227062, 2.126210, 1.734795, 0.523445}; }
+__host__ __device__ inline Pade inv_gamma_fit_a1() { return {2.713374, 1.441196, 3.700251, 1.877961, 0.634303, 3.522427, 3.701347, 1.877961}; }
+__host__ __device__ inline Pade b_fit_a1() { return {0.698112, 0.034678, 0.455138, -0.031327, 0.353905, 0.806444, 0.403679, -0.031327}; }
+__host__ __device__ inline Pade c_fit_a1() { return {0.370145, 1.041299, 1.737347, 1.085400, 2.458364, 5.604947, 3.828124, 1.085400}; }
+
+__host__ __device__ inline double alpha0(double x, bool a1) { return a1 ? 1. : 1. + pade0(alpha_fit(), x); }   // = A(x)
+__host__ __device__ inline double alpha1(double x, bool a1) { return a1 ? 0. : pade1(alpha_fit(), x); }
+__host__ __device__ inline double beta0(double x, bool a1)
+{
+    return a1 ? BF1_UFG * pade0(beta_fit_a1(), x) : BF_UFG * pade0(beta_fit(), x);
+}
+__host__ __device__ inline double inv_gamma0(double x, bool a1)
+{
+    return a1 ? (1. / CF1_UFG) * pade0(inv_gamma_fit_a1(), x) : (1. / CF_UFG) * pade0(inv_gamma_fit(), x);
+}
+__host__ __device__ inline double bfun0(double x, bool a1)
+{
+    return a1 ? BF1_UFG * pade0(b_fit_a1(), x) : BF_UFG * pade0(b_fit(), x);
+}
+__host__ __device__ inline double cfun0(double x, bool a1)
+{
+    return a1 ? CF1_UFG * pade0(c_fit_a1(), x) : CF_UFG * pade0(c_fit(), x);
+}
 
 // APS interpolation s(x) and the chemical potential mu/e_F
 // (sldae_functional.c:17-62)
@@ -82,7 +101,7 @@ __host__ __device__ inline double chem_pot0(double x)
 struct Local {
     double af, af_p, beta, eF, ctilde, ctilde_p, bf;
 };
-__host__ __device__ inline Local local(double n, double nreg, double a)
+__host__ __device__ inline Local local(double n, double nreg, double a, bool a1)
 {
     Local L;
     const double n13 = pow(n, 1. / 3.), n23 = pow(n, 2. / 3.);
@@ -90,13 +109,13 @@ __host__ __device__ inline Local local(double n, double nreg, double a)
     L.eF = kF * kF / 2.;
     const double x = fabs(a * kF);
     const double dx_dn = nreg > 0.0 ? nreg * x / (3. * n) : 0.0;
-    const double alpha = alpha0(x);
-    L.beta = beta0(x);
-    const double invg = inv_gamma0(x);
-    const double alpha_p = dx_dn * alpha1(x);
+    const double alpha = alpha0(x, a1);
+    L.beta = beta0(x, a1);
+    const double invg = inv_gamma0(x, a1);
+    const double alpha_p = dx_dn * alpha1(x, a1);
     L.af = alpha;
-    L.bf = bfun0(x);
-    const double cf = cfun0(x);
+    L.bf = bfun0(x, a1);
+    const double cf = cfun0(x, a1);
     double cf_p = 0.0;
     L.af_p = 0.0;
     if (nreg > 0.0) {

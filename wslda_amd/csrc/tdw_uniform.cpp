@@ -142,7 +142,7 @@ extern "C" int tdw_uniform_solve(const tdw_params *c, double n0_a, double n0_b, 
     // C~ of x = |a k_F| at the total density, alpha_a = alpha_b = A
     const bool se = c->functional == TDW_SLDAE;
     const double nt0 = n0_a + n0_b;
-    const tdw::sldae::Local SL = tdw::sldae::local(nt0, 1.0, c->sclgth);
+    const tdw::sldae::Local SL = tdw::sldae::local(nt0, 1.0, c->sclgth, c->force_a1 != 0);
     const double kFt = pow(3.0 * PI * PI * nt0, 1.0 / 3.0);
     const double xse = fabs(c->sclgth * kFt);
     const double alph_plus = as ? tdw::aslda::alpha_plus(p0r) : se ? SL.af : 1.0;
