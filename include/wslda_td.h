@@ -500,6 +500,25 @@ void *mm_get_pointer_to_derivatives_workspace(size_t n);
 void *mm_get_pointer_to_total_workspace(size_t n);
 void *mm_get_pointer_to_fftONE_workspace(size_t n);
 int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kernels.cu:1106 (host data) */
+/* st-wslda-1d starts (inittype 1; host).  tdw_kmode has the layout of the
+ * driver's wslda_kmode (wslda_wavevectors.h:104-109): a transverse mode
+ * (ky, kz) and its degeneracy.  tdw_st1d_kmodes lists the modes of p's
+ * (ny, nz) grid as fill_wslda_kmodes_1d does (wslda_wavevectors.h:25-100;
+ * kvecs may be NULL to count).  The readers take codedim 1 (each state
+ * evolved once, h_kkyz = (ky..., kz...), h_cnt = degeneracy) or 3 (each state
+ * expanded over its mode's (ky, kz)); codedim 2 returns TDW_EINVAL (the
+ * 1D -> 2D expansion is not built).  read_stwslda1d_wf takes the lattice
+ * from the bound context, whose dim must equal codedim. */
+typedef struct {
+    double ky, kz;
+    int weight;
+} tdw_kmode;
+int tdw_st1d_kmodes(const tdw_params *p, tdw_kmode *kvecs, int max, int *cnt);
+int scan_stwslda1d_info_files(const char *prefix, int codedim, int kvecs_to_consder, tdw_kmode *kvecs,
+                              int *nwf, int *nwf_per_kyz);                    /* pca_io.h:845 */
+int read_stwslda1d_wf(const char *prefix, int codedim, int kvecs_to_consder, tdw_kmode *kvecs,
+                      int *nwf_per_kyz, int mylidx, int myuidx, tdw_complex *h_wavefun,
+                      double *h_fbetaEn, double *h_kkyz, int *h_cnt);        /* pca_io.h:1177 */
 /* st-wslda-3d readers of the driver (host; pca_io.h) */
 int check_comptibility(char *prefix, int requested_dim, char *suffix);          /* pca_io.h:382 */
 int read_checkpoint_info_pca(const char *file_name, int *nwf, int *nx, int *ny, int *nz, double *dx,

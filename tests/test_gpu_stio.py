@@ -37,3 +37,71 @@ def test_dropin_read_s3dpca_wf(tmp_path):
     assert np.array_equal(en, ref["en"][lo:hi])
     L.tdw_bind(None)
     ctx.close()
+
+
+@pytest.mark.parametrize("codedim", [1, 3])
+def test_dropin_read_stwslda1d_wf(tmp_path, codedim):
+    """read_stwslda1d_wf on the bound context: a 1D run takes each state
+    once with its mode's degeneracy; a 3D run expands each state over its
+    mode's (ky, kz) as u(x) e^{i(ky y + kz z)} / sqrt(LY LZ)
+    (pca_io.h:1177-1372)."""
+    import math
+    from tests import st1dfiles
+    from wslda_amd import td
+    from tests.test_st1d import kmodes
+    nx, ny, nz, dx, dy, dz = 12, 6, 4, 0.9, 1.1, 0.7
+    p = td.make_params(codedim, nx, ny, nz, dx=dx, dy=dy, dz=dz)
+    arr = kmodes(p)
+    modes = [(m.ky, m.kz, m.weight) for m in arr][:3]
+    per = [2, 3, 1]
+    ref = st1dfiles.write_st1d(str(tmp_path), nx, modes, per, new=True, seed=4)
+    kv = (td.KMode * 3)(*[td.KMode(*m) for m in modes])
+    nwf = C.c_int(sum(n * m[2] for n, m in zip(per, modes)))
+    got = (C.c_int * 3)()
+    L = td.lib()
+    assert L.scan_stwslda1d_info_files(str(tmp_path).encode(), codedim, 3, C.cast(kv, C.c_void_p), C.byref(nwf),
+                                       C.cast(got, C.c_void_p)) == 0
+    n = nwf.value
+    N = nx if codedim == 1 else nx * ny * nz
+    ctx = td.TDContext(p, n, 0)
+    wf = np.zeros((2 * n, N), dtype=np.complex128)
+    fb = np.zeros(n)
+    kk = np.zeros(2 * n)
+    cnt = np.zeros(n, dtype=np.int32)
+    td.check(L.tdw_bind(ctx.h), "tdw_bind")
+    try:
+        rc = L.read_stwslda1d_wf(str(tmp_path).encode(), codedim, 3, C.cast(kv, C.c_void_p),
+                                 C.cast(got, C.c_void_p), 0, n, wf.ctypes.data_as(C.c_void_p),
+                                 fb.ctypes.data_as(C.c_void_p), kk.ctypes.data_as(C.c_void_p),
+                                 cnt.ctypes.data_as(C.c_void_p))
+    finally:
+        L.tdw_bind(None)
+    assert rc == 0
+    U, V, E = np.concatenate(ref["u"]), np.concatenate(ref["v"]), np.concatenate(ref["en"])
+    if codedim == 1:
+        assert np.array_equal(wf[:n], U) and np.array_equal(wf[n:], V) and np.array_equal(fb, E)
+        assert np.array_equal(kk[:n], ref["ky"]) and np.array_equal(kk[n:], ref["kz"])
+        w = {(m[0], m[1]): m[2] for m in modes}
+        assert list(cnt) == [w[(a, b)] for a, b in zip(ref["ky"], ref["kz"])]
+    else:
+        LY, LZ = ny * dy, nz * dz
+        ky_grid = [2 * math.pi / LY * (i if i < ny // 2 else i - ny) for i in range(ny)]
+        kz_grid = [2 * math.pi / LZ * (i if i < nz // 2 else i - nz) for i in range(nz)]
+        iy, iz = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
+        row, s0 = 0, 0
+        for (mky, mkz, wgt), cnt_m in zip(modes, per):
+            members = [(a, b) for i1, a in enumerate(ky_grid) for i2, b in enumerate(kz_grid)
+                       if i1 != ny // 2 and i2 != nz // 2 and abs(a * a + b * b - mky ** 2 - mkz ** 2) < 1e-12]
+            assert len(members) == wgt
+            for a, b in members:
+                ph = (np.exp(1j * (a * iy * dy + b * iz * dz)) / math.sqrt(LY * LZ)).ravel()
+                for k in range(cnt_m):
+                    want_u = (U[s0 + k][:, None] * ph[None, :]).ravel()
+                    want_v = (V[s0 + k][:, None] * ph[None, :]).ravel()
+                    assert np.abs(wf[row] - want_u).max() < 1e-14
+                    assert np.abs(wf[n + row] - want_v).max() < 1e-14
+                    assert fb[row] == E[s0 + k]
+                    row += 1
+            s0 += cnt_m
+        assert row == n
+    ctx.close()

@@ -247,3 +247,208 @@ extern "C" int tdw_st3d_load(const tdw_params *p, const char *prefix, int iogrou
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// st-wslda-1d starts (inittype 1): the transverse modes of the 1D solver
+// (wslda_wavevectors.h:25-100, 284-390) and its readers
+// scan_stwslda1d_info_files (pca_io.h:845-912) and read_stwslda1d_wf
+// (pca_io.h:1177-1372), for a 1D td run (codedim 1: each state evolved once
+// with its mode's degeneracy) or a 3D one (codedim 3: each state expanded
+// over the mode's (ky, kz) as u(x) e^{i(ky y + kz z)} / sqrt(LY LZ)).
+// Files <prefix>/wf.%04d.{info,_un.wdat,_vn.wdat,_kkyz.dat,_en.dat} or the
+// old s1dpca.%04d.{info,wfu,wfv,kkyz,en}, one set per transverse mode.
+// ---------------------------------------------------------------------------
+namespace {
+
+const double CMP_EPS = 1.0e-12;   // wslda_wavevectors.h:12
+const int ERR_SCAN_INFO_FILES_SUM_FAILED = 10026;   // wslda_errors.h:49
+
+double kgrid(int i, int n, double L)
+{
+    return (i < n / 2) ? 2. * M_PI / L * (double)i : 2. * M_PI / L * (double)(i - n);
+}
+
+// unique |k_perp|^2 classes of the (ky, kz) grid (Nyquist rows skipped
+// unless cubic cutoff), first representative and degeneracy
+int kmodes(const tdw_params *p, tdw_kmode *out, int max)
+{
+    const double LY = p->dy * p->ny, LZ = p->dz * p->nz;
+    const bool cubic = p->regularization == TDW_CUBIC_CUTOFF;
+    std::vector<double> k2s;
+    int cnt = 0;
+    for (int iy = 0; iy < p->ny; iy++)
+        for (int iz = 0; iz < p->nz; iz++) {
+            if (!cubic && p->ny > 1 && iy == p->ny / 2) continue;
+            if (!cubic && p->nz > 1 && iz == p->nz / 2) continue;
+            const double ky = kgrid(iy, p->ny, LY), kz = kgrid(iz, p->nz, LZ), k2 = ky * ky + kz * kz;
+            int i = 0;
+            for (; i < cnt; i++)
+                if (fabs(k2s[i] - k2) < CMP_EPS) break;
+            if (i < cnt) {
+                if (out && i < max) out[i].weight++;
+            } else {
+                if (out && cnt < max) out[cnt] = tdw_kmode{ky, kz, 1};
+                k2s.push_back(k2);
+                cnt++;
+            }
+        }
+    return cnt;
+}
+
+}  // namespace
+
+extern "C" int tdw_st1d_kmodes(const tdw_params *p, tdw_kmode *kvecs, int max, int *cnt)
+{
+    if (!p || !cnt) return TDW_EINVAL;
+    *cnt = kmodes(p, kvecs, kvecs ? max : 0);
+    return 0;
+}
+
+extern "C" int scan_stwslda1d_info_files(const char *prefix, int codedim, int kvecs_to_consder, tdw_kmode *kvecs,
+                                         int *nwf, int *nwf_per_kyz)
+{
+    char file_name[1024];
+    int tnwf = 0;
+    for (int ikz = 0; ikz < kvecs_to_consder; ikz++) {
+        snprintf(file_name, sizeof(file_name), "%s/wf.%04d.info", prefix, ikz);
+        if (!exists(file_name)) snprintf(file_name, sizeof(file_name), "%s/s1dpca.%04d.info", prefix, ikz);
+        FILE *f = fopen(file_name, "rb");
+        if (!f) return ERR_CANNOT_OPEN_FILE;
+        int info_dim = 0, i = 0;
+        size_t g1 = fread(&info_dim, sizeof(int), 1, f);
+        size_t g2 = fread(&i, sizeof(int), 1, f);
+        fclose(f);
+        if (g1 != 1 || g2 != 1) return ERR_CANNOT_OPEN_FILE;
+        if (info_dim == 2) return ERR_INITSTATE_FOR_2D;
+        if (info_dim == 3) return ERR_INITSTATE_FOR_3D;
+        nwf_per_kyz[ikz] = i;
+        tnwf += i * kvecs[ikz].weight;
+    }
+    if (tnwf != *nwf) return ERR_SCAN_INFO_FILES_SUM_FAILED;
+    if (codedim == 2) return TDW_EINVAL;   // the 1D -> 2D mode expansion is not built
+    tnwf = 0;
+    for (int ikz = 0; ikz < kvecs_to_consder; ikz++) tnwf += nwf_per_kyz[ikz] * (codedim == 3 ? kvecs[ikz].weight : 1);
+    *nwf = tnwf;
+    return 0;
+}
+
+// read_stwslda1d_wf: the lattice (NX and, for codedim 3, NY, NZ, DY, DZ and
+// the cutoff) from the bound context
+extern "C" int read_stwslda1d_wf(const char *prefix, int codedim, int kvecs_to_consder, tdw_kmode *kvecs,
+                                 int *nwf_per_kyz, int mylidx, int myuidx, tdw_complex *h_wavefun,
+                                 double *h_fbetaEn, double *h_kkyz, int *h_cnt)
+{
+    tdw_ctx *c = tdw_bound();
+    if (!c) return TDW_ENOCTX;
+    if (codedim != 1 && codedim != 3) return TDW_EINVAL;
+    if (codedim != c->dim) return TDW_EINVAL;
+    const tdw_params *p = &c->p;
+    const int NX = p->nx;
+    const size_t N = c->N;
+    const int nwfip = myuidx - mylidx;
+    std::vector<tdw_complex> u(NX), v(NX);
+    std::vector<tdw_kmode> table(p->ny * p->nz + 1);
+    const int ntab = kmodes(p, table.data(), (int)table.size());
+    char name[1024], fu_n[1024], fv_n[1024], fk_n[1024], fe_n[1024];
+    snprintf(name, sizeof(name), "%s/wf.%04d.info", prefix, 0);
+    const bool new_format = exists(name);
+    const double LY = p->dy * p->ny, LZ = p->dz * p->nz;
+    const bool cubic = p->regularization == TDW_CUBIC_CUTOFF;
+    int iwf = 0;
+    for (int ikz = 0; ikz < kvecs_to_consder; ikz++) {
+        // the (ky, kz) members of this mode (wslda_kmodes_1d_get_modes)
+        std::vector<double> mky, mkz;
+        if (codedim == 3) {
+            const double k2in = kvecs[ikz].ky * kvecs[ikz].ky + kvecs[ikz].kz * kvecs[ikz].kz;
+            for (int iy = 0; iy < p->ny; iy++)
+                for (int iz = 0; iz < p->nz; iz++) {
+                    if (!cubic && p->ny > 1 && iy == p->ny / 2) continue;
+                    if (!cubic && p->nz > 1 && iz == p->nz / 2) continue;
+                    const double ky = kgrid(iy, p->ny, LY), kz = kgrid(iz, p->nz, LZ);
+                    if (fabs(ky * ky + kz * kz - k2in) < CMP_EPS) {
+                        mky.push_back(ky);
+                        mkz.push_back(kz);
+                    }
+                }
+            if ((int)mky.size() != kvecs[ikz].weight) return -122;
+        }
+        const int dcoeff = codedim == 3 ? (int)mky.size() : 1;
+        FILE *fu = nullptr, *fv = nullptr, *fk = nullptr, *fe = nullptr;
+        int err = 0;
+        for (int dd = 0; dd < dcoeff && !err; dd++) {
+            if (fu && dd >= 1) {
+                if (fseek(fu, 0, SEEK_SET) || fseek(fv, 0, SEEK_SET) || fseek(fk, 0, SEEK_SET) || fseek(fe, 0, SEEK_SET))
+                    err = -111;
+            }
+            for (int ii = 0; ii < nwf_per_kyz[ikz] && !err; ii++, iwf++) {
+                if (iwf < mylidx || iwf >= myuidx) continue;
+                if (!fu) {
+                    if (new_format) {
+                        snprintf(fu_n, sizeof(fu_n), "%s/wf.%04d_un.wdat", prefix, ikz);
+                        snprintf(fv_n, sizeof(fv_n), "%s/wf.%04d_vn.wdat", prefix, ikz);
+                        snprintf(fk_n, sizeof(fk_n), "%s/wf.%04d_kkyz.dat", prefix, ikz);
+                        snprintf(fe_n, sizeof(fe_n), "%s/wf.%04d_en.dat", prefix, ikz);
+                    } else {
+                        snprintf(fu_n, sizeof(fu_n), "%s/s1dpca.%04d.wfu", prefix, ikz);
+                        snprintf(fv_n, sizeof(fv_n), "%s/s1dpca.%04d.wfv", prefix, ikz);
+                        snprintf(fk_n, sizeof(fk_n), "%s/s1dpca.%04d.kkyz", prefix, ikz);
+                        snprintf(fe_n, sizeof(fe_n), "%s/s1dpca.%04d.en", prefix, ikz);
+                    }
+                    fu = fopen(fu_n, "rb");
+                    fv = fopen(fv_n, "rb");
+                    fk = fopen(fk_n, "rb");
+                    fe = fopen(fe_n, "rb");
+                    if (!fu) err = -1;
+                    else if (!fv) err = -2;
+                    else if (!fk) err = -3;
+                    else if (!fe) err = -4;
+                    else if (fseek(fu, (long)(sizeof(tdw_complex) * NX * ii), SEEK_SET)) err = -11;
+                    else if (fseek(fv, (long)(sizeof(tdw_complex) * NX * ii), SEEK_SET)) err = -12;
+                    else if (fseek(fk, (long)(sizeof(double) * ii * 2), SEEK_SET)) err = -13;
+                    else if (fseek(fe, (long)(sizeof(double) * ii), SEEK_SET)) err = -14;
+                    if (err) break;
+                }
+                double en, ky, kz;
+                if (fread(u.data(), sizeof(tdw_complex) * NX, 1, fu) != 1) { err = -21; break; }
+                if (fread(v.data(), sizeof(tdw_complex) * NX, 1, fv) != 1) { err = -22; break; }
+                if (fread(&en, sizeof(double), 1, fe) != 1) { err = -23; break; }
+                if (fread(&ky, sizeof(double), 1, fk) != 1) { err = -23; break; }
+                if (fread(&kz, sizeof(double), 1, fk) != 1) { err = -24; break; }
+                const size_t slot = (size_t)(iwf - mylidx);
+                tdw_complex *U = h_wavefun + N * slot, *V = h_wavefun + N * slot + N * (size_t)nwfip;
+                if (codedim == 1) {
+                    for (int ix = 0; ix < NX; ix++) { U[ix] = u[ix]; V[ix] = v[ix]; }
+                    h_fbetaEn[slot] = en;
+                    h_kkyz[slot] = ky;
+                    h_kkyz[slot + nwfip] = kz;
+                    int w = 0;
+                    for (int t = 0; t < ntab; t++)
+                        if (fabs(table[t].ky - ky) < CMP_EPS && fabs(table[t].kz - kz) < CMP_EPS) { w = table[t].weight; break; }
+                    h_cnt[slot] = w;
+                } else {
+                    if (ky != kvecs[ikz].ky) { err = -54; break; }
+                    if (kz != kvecs[ikz].kz) { err = -55; break; }
+                    const double sky = mky[dd], skz = mkz[dd], norm = 1.0 / sqrt(LY * LZ);
+                    size_t ixyz = 0;
+                    for (int ix = 0; ix < NX; ix++)
+                        for (int iy = 0; iy < p->ny; iy++)
+                            for (int iz = 0; iz < p->nz; iz++, ixyz++) {
+                                const double ph = sky * iy * p->dy + skz * iz * p->dz;
+                                const double cr = cos(ph) * norm, ci = sin(ph) * norm;
+                                U[ixyz].re = u[ix].re * cr - u[ix].im * ci;
+                                U[ixyz].im = u[ix].re * ci + u[ix].im * cr;
+                                V[ixyz].re = v[ix].re * cr - v[ix].im * ci;
+                                V[ixyz].im = v[ix].re * ci + v[ix].im * cr;
+                            }
+                    h_fbetaEn[slot] = en;
+                }
+            }
+        }
+        if (fu) fclose(fu);
+        if (fv) fclose(fv);
+        if (fk) fclose(fk);
+        if (fe) fclose(fe);
+        if (err) return err;
+    }
+    return 0;
+}

@@ -54,6 +54,11 @@ class Params(C.Structure):
                 ("force_a1", C.c_int), ("precision", C.c_int)]
 
 
+class KMode(C.Structure):
+    """tdw_kmode: a transverse mode (ky, kz) and its degeneracy."""
+    _fields_ = [("ky", C.c_double), ("kz", C.c_double), ("weight", C.c_int)]
+
+
 class Uniform(C.Structure):
     _fields_ = [(k, C.c_double) for k in
                 ("n0_a", "n0_b", "mu_a", "mu_b", "V_a", "V_b", "alph_a", "alph_b", "kc", "ec",
@@ -247,6 +252,9 @@ def _sigs():
         "tdw_st3d_info": (I, [C.c_char_p, I, C.POINTER(StInfo), P]),
         "tdw_st3d_load": (I, [C.POINTER(Params), C.c_char_p, I, P, C.POINTER(StInfo), I, I, I, P, P, P,
                               C.POINTER(CheckpointScalars)]),
+        "tdw_st1d_kmodes": (I, [C.POINTER(Params), P, I, P]),
+        "scan_stwslda1d_info_files": (I, [C.c_char_p, I, I, P, P, P]),
+        "read_stwslda1d_wf": (I, [C.c_char_p, I, I, P, P, I, I, P, P, P, P]),
         "check_comptibility": (I, [C.c_char_p, I, C.c_char_p]),
         "read_checkpoint_info_pca": (I, [C.c_char_p, P, P, P, P, P, P, P, P, P, P, P]),
         "scan_s3dpca_info_files": (I, [C.c_char_p, I, P, P]),
