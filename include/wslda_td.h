@@ -505,10 +505,11 @@ int get_v_ext(int datadim, int spin, int it, double *data);          /* pca_kern
  * (ky, kz) and its degeneracy.  tdw_st1d_kmodes lists the modes of p's
  * (ny, nz) grid as fill_wslda_kmodes_1d does (wslda_wavevectors.h:25-100;
  * kvecs may be NULL to count).  The readers take codedim 1 (each state
- * evolved once, h_kkyz = (ky..., kz...), h_cnt = degeneracy) or 3 (each state
- * expanded over its mode's (ky, kz)); codedim 2 returns TDW_EINVAL (the
- * 1D -> 2D expansion is not built).  read_stwslda1d_wf takes the lattice
- * from the bound context, whose dim must equal codedim. */
+ * evolved once, h_kkyz = (ky..., kz...), h_cnt = degeneracy), 2 (each state
+ * expanded over its mode's ky with kz >= 0, h_kkyz = kz; the scan needs a
+ * bound context for the mode table) or 3 (each state expanded over its
+ * mode's (ky, kz)).  read_stwslda1d_wf takes the lattice from the bound
+ * context, whose dim must equal codedim. */
 typedef struct {
     double ky, kz;
     int weight;

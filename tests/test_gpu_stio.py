@@ -39,7 +39,7 @@ def test_dropin_read_s3dpca_wf(tmp_path):
     ctx.close()
 
 
-@pytest.mark.parametrize("codedim", [1, 3])
+@pytest.mark.parametrize("codedim", [1, 2, 3])
 def test_dropin_read_stwslda1d_wf(tmp_path, codedim):
     """read_stwslda1d_wf on the bound context: a 1D run takes each state
     once with its mode's degeneracy; a 3D run expands each state over its
@@ -59,10 +59,36 @@ def test_dropin_read_stwslda1d_wf(tmp_path, codedim):
     nwf = C.c_int(sum(n * m[2] for n, m in zip(per, modes)))
     got = (C.c_int * 3)()
     L = td.lib()
-    assert L.scan_stwslda1d_info_files(str(tmp_path).encode(), codedim, 3, C.cast(kv, C.c_void_p), C.byref(nwf),
-                                       C.cast(got, C.c_void_p)) == 0
+    LY, LZ = ny * dy, nz * dz
+    ky_grid = [2 * math.pi / LY * (i if i < ny // 2 else i - ny) for i in range(ny)]
+    kz_grid = [2 * math.pi / LZ * (i if i < nz // 2 else i - nz) for i in range(nz)]
+
+    def members(mky, mkz):
+        return [(a, b) for i1, a in enumerate(ky_grid) for i2, b in enumerate(kz_grid)
+                if i1 != ny // 2 and i2 != nz // 2 and abs(a * a + b * b - mky ** 2 - mkz ** 2) < 1e-12]
+
+    def members_2d(mky, mkz):
+        # grouped by kz in order of first appearance, kz >= 0 kept
+        groups = {}
+        for a, b in members(mky, mkz):
+            key = next((k for k in groups if abs(k - b) < 1e-12), b)
+            groups.setdefault(key, []).append(a)
+        return [(a, kz) for kz, ys in groups.items() if kz > -1e-12 for a in ys]
+
+    # the scan of a 2D run reads the mode table of the bound context
+    N = nx if codedim == 1 else nx * ny if codedim == 2 else nx * ny * nz
+    nbig = sum(per) * 8
+    ctx = td.TDContext(p, nbig, 0)
+    td.check(L.tdw_bind(ctx.h), "tdw_bind")
+    try:
+        assert L.scan_stwslda1d_info_files(str(tmp_path).encode(), codedim, 3, C.cast(kv, C.c_void_p),
+                                           C.byref(nwf), C.cast(got, C.c_void_p)) == 0
+    finally:
+        L.tdw_bind(None)
     n = nwf.value
-    N = nx if codedim == 1 else nx * ny * nz
+    if codedim == 2:
+        assert n == sum(c * len(members_2d(m[0], m[1])) for c, m in zip(per, modes))
+    ctx.close()
     ctx = td.TDContext(p, n, 0)
     wf = np.zeros((2 * n, N), dtype=np.complex128)
     fb = np.zeros(n)
@@ -83,17 +109,25 @@ def test_dropin_read_stwslda1d_wf(tmp_path, codedim):
         assert np.array_equal(kk[:n], ref["ky"]) and np.array_equal(kk[n:], ref["kz"])
         w = {(m[0], m[1]): m[2] for m in modes}
         assert list(cnt) == [w[(a, b)] for a, b in zip(ref["ky"], ref["kz"])]
+    elif codedim == 2:
+        row, s0 = 0, 0
+        for (mky, mkz, wgt), cnt_m in zip(modes, per):
+            for a, b in members_2d(mky, mkz):
+                ph = np.exp(1j * a * np.arange(ny) * dy) / math.sqrt(LY)
+                for k in range(cnt_m):
+                    assert np.abs(wf[row] - (U[s0 + k][:, None] * ph[None, :]).ravel()).max() < 1e-14
+                    assert np.abs(wf[n + row] - (V[s0 + k][:, None] * ph[None, :]).ravel()).max() < 1e-14
+                    assert fb[row] == E[s0 + k] and kk[row] == b
+                    row += 1
+            s0 += cnt_m
+        assert row == n
     else:
-        LY, LZ = ny * dy, nz * dz
-        ky_grid = [2 * math.pi / LY * (i if i < ny // 2 else i - ny) for i in range(ny)]
-        kz_grid = [2 * math.pi / LZ * (i if i < nz // 2 else i - nz) for i in range(nz)]
         iy, iz = np.meshgrid(np.arange(ny), np.arange(nz), indexing="ij")
         row, s0 = 0, 0
         for (mky, mkz, wgt), cnt_m in zip(modes, per):
-            members = [(a, b) for i1, a in enumerate(ky_grid) for i2, b in enumerate(kz_grid)
-                       if i1 != ny // 2 and i2 != nz // 2 and abs(a * a + b * b - mky ** 2 - mkz ** 2) < 1e-12]
-            assert len(members) == wgt
-            for a, b in members:
+            mem = members(mky, mkz)
+            assert len(mem) == wgt
+            for a, b in mem:
                 ph = (np.exp(1j * (a * iy * dy + b * iz * dz)) / math.sqrt(LY * LZ)).ravel()
                 for k in range(cnt_m):
                     want_u = (U[s0 + k][:, None] * ph[None, :]).ravel()

@@ -253,8 +253,10 @@ extern "C" int tdw_st3d_load(const tdw_params *p, const char *prefix, int iogrou
 // (wslda_wavevectors.h:25-100, 284-390) and its readers
 // scan_stwslda1d_info_files (pca_io.h:845-912) and read_stwslda1d_wf
 // (pca_io.h:1177-1372), for a 1D td run (codedim 1: each state evolved once
-// with its mode's degeneracy) or a 3D one (codedim 3: each state expanded
-// over the mode's (ky, kz) as u(x) e^{i(ky y + kz z)} / sqrt(LY LZ)).
+// with its mode's degeneracy), a 2D one (codedim 2: each state expanded over
+// the mode's ky with kz >= 0 as u(x) e^{i ky y} / sqrt(LY), kz kept as the
+// transverse momentum) or a 3D one (codedim 3: each state expanded over the
+// mode's (ky, kz) as u(x) e^{i(ky y + kz z)} / sqrt(LY LZ)).
 // Files <prefix>/wf.%04d.{info,_un.wdat,_vn.wdat,_kkyz.dat,_en.dat} or the
 // old s1dpca.%04d.{info,wfu,wfv,kkyz,en}, one set per transverse mode.
 // ---------------------------------------------------------------------------
@@ -295,6 +297,53 @@ int kmodes(const tdw_params *p, tdw_kmode *out, int max)
     return cnt;
 }
 
+// members (ky, kz) of the mode with |k|^2 = ky^2 + kz^2 in grid order
+// (wslda_kmodes_1d_get_modes, wslda_wavevectors.h:346-383)
+void mode_members(const tdw_params *p, double ky0, double kz0, std::vector<double> &mky, std::vector<double> &mkz)
+{
+    const double LY = p->dy * p->ny, LZ = p->dz * p->nz, k2in = ky0 * ky0 + kz0 * kz0;
+    const bool cubic = p->regularization == TDW_CUBIC_CUTOFF;
+    for (int iy = 0; iy < p->ny; iy++)
+        for (int iz = 0; iz < p->nz; iz++) {
+            if (!cubic && p->ny > 1 && iy == p->ny / 2) continue;
+            if (!cubic && p->nz > 1 && iz == p->nz / 2) continue;
+            const double ky = kgrid(iy, p->ny, LY), kz = kgrid(iz, p->nz, LZ);
+            if (fabs(ky * ky + kz * kz - k2in) < CMP_EPS) {
+                mky.push_back(ky);
+                mkz.push_back(kz);
+            }
+        }
+}
+
+// the (ky, kz) pairs a 2D run takes from one mode: the members grouped by kz
+// in order of first appearance, kz >= 0 only (kz = -pi/dz too with cubic
+// cutoff) (wslda_kmodes_1d_to_2d / _getcnt2d, wslda_wavevectors.h:390-470)
+int modes_2d(const tdw_params *p, double ky0, double kz0, std::vector<double> &oky, std::vector<double> &okz)
+{
+    std::vector<double> mky, mkz, gz;
+    std::vector<std::vector<double>> gy;
+    mode_members(p, ky0, kz0, mky, mkz);
+    for (size_t i = 0; i < mky.size(); i++) {
+        size_t j = 0;
+        for (; j < gz.size(); j++)
+            if (fabs(mkz[i] - gz[j]) < CMP_EPS) break;
+        if (j == gz.size()) {
+            gz.push_back(mkz[i]);
+            gy.push_back({});
+        }
+        gy[j].push_back(mky[i]);
+    }
+    const bool cubic = p->regularization == TDW_CUBIC_CUTOFF;
+    for (size_t j = 0; j < gz.size(); j++) {
+        if (!(gz[j] > -CMP_EPS || (cubic && fabs(gz[j] + M_PI / p->dz) < 1.0e-12))) continue;
+        for (double y : gy[j]) {
+            oky.push_back(y);
+            okz.push_back(gz[j]);
+        }
+    }
+    return (int)oky.size();
+}
+
 }  // namespace
 
 extern "C" int tdw_st1d_kmodes(const tdw_params *p, tdw_kmode *kvecs, int max, int *cnt)
@@ -325,9 +374,19 @@ extern "C" int scan_stwslda1d_info_files(const char *prefix, int codedim, int kv
         tnwf += i * kvecs[ikz].weight;
     }
     if (tnwf != *nwf) return ERR_SCAN_INFO_FILES_SUM_FAILED;
-    if (codedim == 2) return TDW_EINVAL;   // the 1D -> 2D mode expansion is not built
     tnwf = 0;
-    for (int ikz = 0; ikz < kvecs_to_consder; ikz++) tnwf += nwf_per_kyz[ikz] * (codedim == 3 ? kvecs[ikz].weight : 1);
+    for (int ikz = 0; ikz < kvecs_to_consder; ikz++) {
+        int mult = 1;
+        if (codedim == 3) mult = kvecs[ikz].weight;
+        if (codedim == 2) {
+            // the mode table of the bound context's transverse grid
+            tdw_ctx *c = tdw_bound();
+            if (!c) return TDW_ENOCTX;
+            std::vector<double> oky, okz;
+            mult = modes_2d(&c->p, kvecs[ikz].ky, kvecs[ikz].kz, oky, okz);
+        }
+        tnwf += nwf_per_kyz[ikz] * mult;
+    }
     *nwf = tnwf;
     return 0;
 }
@@ -340,7 +399,7 @@ extern "C" int read_stwslda1d_wf(const char *prefix, int codedim, int kvecs_to_c
 {
     tdw_ctx *c = tdw_bound();
     if (!c) return TDW_ENOCTX;
-    if (codedim != 1 && codedim != 3) return TDW_EINVAL;
+    if (codedim < 1 || codedim > 3) return TDW_EINVAL;
     if (codedim != c->dim) return TDW_EINVAL;
     const tdw_params *p = &c->p;
     const int NX = p->nx;
@@ -359,20 +418,12 @@ extern "C" int read_stwslda1d_wf(const char *prefix, int codedim, int kvecs_to_c
         // the (ky, kz) members of this mode (wslda_kmodes_1d_get_modes)
         std::vector<double> mky, mkz;
         if (codedim == 3) {
-            const double k2in = kvecs[ikz].ky * kvecs[ikz].ky + kvecs[ikz].kz * kvecs[ikz].kz;
-            for (int iy = 0; iy < p->ny; iy++)
-                for (int iz = 0; iz < p->nz; iz++) {
-                    if (!cubic && p->ny > 1 && iy == p->ny / 2) continue;
-                    if (!cubic && p->nz > 1 && iz == p->nz / 2) continue;
-                    const double ky = kgrid(iy, p->ny, LY), kz = kgrid(iz, p->nz, LZ);
-                    if (fabs(ky * ky + kz * kz - k2in) < CMP_EPS) {
-                        mky.push_back(ky);
-                        mkz.push_back(kz);
-                    }
-                }
+            mode_members(p, kvecs[ikz].ky, kvecs[ikz].kz, mky, mkz);
             if ((int)mky.size() != kvecs[ikz].weight) return -122;
+        } else if (codedim == 2) {
+            modes_2d(p, kvecs[ikz].ky, kvecs[ikz].kz, mky, mkz);
         }
-        const int dcoeff = codedim == 3 ? (int)mky.size() : 1;
+        const int dcoeff = codedim == 1 ? 1 : (int)mky.size();
         FILE *fu = nullptr, *fv = nullptr, *fk = nullptr, *fe = nullptr;
         int err = 0;
         for (int dd = 0; dd < dcoeff && !err; dd++) {
@@ -425,6 +476,22 @@ extern "C" int read_stwslda1d_wf(const char *prefix, int codedim, int kvecs_to_c
                     for (int t = 0; t < ntab; t++)
                         if (fabs(table[t].ky - ky) < CMP_EPS && fabs(table[t].kz - kz) < CMP_EPS) { w = table[t].weight; break; }
                     h_cnt[slot] = w;
+                } else if (codedim == 2) {
+                    if (ky != kvecs[ikz].ky) { err = -44; break; }
+                    if (kz != kvecs[ikz].kz) { err = -45; break; }
+                    const double sky = mky[dd], norm = 1.0 / sqrt(LY);
+                    size_t ixy = 0;
+                    for (int ix = 0; ix < NX; ix++)
+                        for (int iy = 0; iy < p->ny; iy++, ixy++) {
+                            const double ph = sky * iy * p->dy;
+                            const double cr = cos(ph) * norm, ci = sin(ph) * norm;
+                            U[ixy].re = u[ix].re * cr - u[ix].im * ci;
+                            U[ixy].im = u[ix].re * ci + u[ix].im * cr;
+                            V[ixy].re = v[ix].re * cr - v[ix].im * ci;
+                            V[ixy].im = v[ix].re * ci + v[ix].im * cr;
+                        }
+                    h_fbetaEn[slot] = en;
+                    h_kkyz[slot] = mkz[dd];
                 } else {
                     if (ky != kvecs[ikz].ky) { err = -54; break; }
                     if (kz != kvecs[ikz].kz) { err = -55; break; }
