@@ -488,7 +488,7 @@ def main():
     ap.add_argument("--adj-k", type=int, default=50)
     ap.add_argument("--adj-states", type=int, default=4096)
     ap.add_argument("--adj-fd", type=int, default=1, help="central-difference check of three parameters (0: off)")
-    ap.add_argument("--adj-taped", type=int, default=3,
+    ap.add_argument("--adj-taped", type=int, default=4,
                     help="segments the forward run tapes directly (each 2K+1 state arrays of HBM)")
     ap.add_argument("--trap", action="store_true",
                     help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
