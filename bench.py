@@ -143,7 +143,8 @@ def run_gpu(args, prec):
     dev = "cuda:0"
     t0 = time.time()
     w = problems.config2(torch, precision=prec, n_states=args.states, device=dev,
-                         functional=td.ASLDA if args.aslda else td.SLDA)
+                         functional=td.SLDAE if args.sldae else td.ASLDA if args.aslda else td.SLDA,
+                         sclgth=-1.1 if args.sldae else 0.0)
     p = w["params"]
     n = w["wf"].shape[0] // 2
     ctx = td.TDContext(p, n, 0)
@@ -441,11 +442,13 @@ def run_aslda(args):
     roof["unit_note"] = ("achieved = the kernel's own compulsory state-array reads and writes per launch "
                          "/ average launch time")
     res = {
-        "metric": "qp-wavefunction steps/s, ASLDA (gradients every half-step), FP64",
+        "metric": "qp-wavefunction steps/s, %s (gradients every half-step), FP64" % ("SLDAE" if args.sldae
+                                                                                    else "ASLDA"),
         "value": head["value"], "unit": "qp-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "td-ASLDA 3D uniform, 24x24x24 lattice, %d qp states, AB5AM5, plane-wave BdG "
-                               "+ seeded 1e-6 noise, modular step" % g["n"]},
+        "config": {"workload": "td-%s 3D uniform, 24x24x24 lattice, %d qp states, AB5AM5, plane-wave BdG "
+                               "+ seeded 1e-6 noise, modular step" % ("SLDAE (a kF = -1.1)" if args.sldae
+                                                                      else "ASLDA", g["n"])},
         "roofline": roof, "kernels_ms_per_step": head["kernels_ms_per_step"],
         "energy_drift": head["energy_drift"],
         "step_state_traffic_S": sum(modular_units(k) * v[1] for k, v in g["rep"].items()) / args.steps,
@@ -495,7 +498,11 @@ def main():
     ap.add_argument("--trap-states", type=int, default=24576)
     ap.add_argument("--aslda", action="store_true",
                     help="instead of the step benchmark: the ASLDA step on the 3D uniform workload (24^3)")
+    ap.add_argument("--sldae", action="store_true",
+                    help="as --aslda with the SLDAE functional (scattering length a kF = -1.1)")
     args = ap.parse_args()
+    if args.sldae:
+        args.aslda = True
     if args.gpus != 1:
         sys.stderr.write("bench.py: this build runs on one GPU only (the density all-reduce across "
                          "GPUs is a design, not code); run with --gpus 1\n")

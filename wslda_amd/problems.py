@@ -92,10 +92,11 @@ def states_on_gpu(torch, params, K, U, V, device="cuda:0", chunk=2048, envelope=
 
 
 def _base_params(dim, nx, ny, nz, dx, scheme=td.AB5AM5, precision=td.FP64, vext=None, dt_eF=0.005,
-                 functional=td.SLDA):
+                 functional=td.SLDA, sclgth=0.0):
     eF = 0.5
     return td.make_params(dim, nx, ny, nz, dx=dx, functional=functional, regularization=td.SPHERICAL_CUTOFF,
-                          mf_correction=1, scheme=scheme, dt=dt_eF / eF, vext=vext, precision=precision)
+                          mf_correction=1, scheme=scheme, dt=dt_eF / eF, vext=vext, precision=precision,
+                          sclgth=sclgth)
 
 
 def uniform_slda(params, T=1.0e-9):
@@ -110,14 +111,15 @@ def uniform_slda(params, T=1.0e-9):
 
 
 def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, device="cuda:0", scheme=td.AB5AM5,
-            shape=(24, 24, 24), dx=1.0, functional=td.SLDA):
+            shape=(24, 24, 24), dx=1.0, functional=td.SLDA, sclgth=0.0):
     """Plain 3D uniform SLDA: 24^3 periodic lattice, the n_states lowest-|E|
     plane-wave BdG states of the self-consistent uniform solution (Delta of
     the solution ~ 0.5 eF), seeded random-phase noise of amplitude 1e-6.
     functional=ASLDA: the same with the ASLDA solution (effective mass
-    alpha = 1.094 at n_a = n_b)."""
+    alpha = 1.094 at n_a = n_b); functional=SLDAE: the SLDAE solution at
+    scattering length sclgth (k_F = 1 units)."""
     nx, ny, nz = shape
-    p = _base_params(3, nx, ny, nz, dx, scheme=scheme, precision=precision, functional=functional)
+    p = _base_params(3, nx, ny, nz, dx, scheme=scheme, precision=precision, functional=functional, sclgth=sclgth)
     u = uniform_slda(p)
     # all plane-wave states of the lattice, the n_states of lowest |E| (a few
     # may lie above the solution's cutoff when n_states exceeds its count)
@@ -144,8 +146,8 @@ def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, devi
     fb = np.array([td.fbeta(e, u.beta) for e in E])
     eF = 0.5
     return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
-                name="3D uniform %s %dx%dx%d, %d states" % ("ASLDA" if functional == td.ASLDA else "SLDA", nx, ny,
-                                                            nz, len(E)))
+                name="3D uniform %s %dx%dx%d, %d states" % ({td.ASLDA: "ASLDA", td.SLDAE: "SLDAE"}.get(
+                    functional, "SLDA"), nx, ny, nz, len(E)))
 
 
 def select_states(wf, En, kt, cnt, n_states, dim):
