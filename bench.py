@@ -3,7 +3,9 @@
 Metric: quasiparticle-wavefunction steps per second (states x ABM steps /
 wall time of the step loop, init and measurements excluded), FP64 and
 split precision.  Default workload: the plain 3D uniform SLDA problem
-(24^3 lattice, 16384 states, AB5AM5, FP64).
+(24^3 lattice, the uniform solution's 16308 states below its cutoff,
+AB5AM5, FP64), then the same in split precision, then the trapped vortex
+(32^3, 24576 states) in split precision and FP64.
 
 Prints ONE JSON line (rank 0).  Extra objects:
   roofline     -- dominant kernel: algorithmic HBM bytes per launch over its
@@ -213,13 +215,33 @@ def run_gpu(args, prec):
     return out
 
 
+def host_cores():
+    """Every core the process may run on (the affinity mask, else
+    os.cpu_count())."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(g, args):
     """Oracle on the host: a sample of the same workload (the first states of
     the GPU problem), a few PECE steps."""
     import ctypes as C
     import numpy as np
     from oracle import pyoracle as po
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cores()
     po.lib().tdwo_set_threads(threads)
     p = g["params"]
     cfg = po.Cfg()
@@ -242,7 +264,9 @@ def cpu_baseline(g, args):
             break
     dt = time.time() - t
     return dict(value=ns * k / dt, unit="qp-steps/s", cores=threads, kind="port",
-                sample="%s lattice, %d of %d states, %d PECE steps after 1 warm-up, %d OpenMP threads, %.1f s"
+                host_nproc=os.cpu_count(), cpu_model=cpu_model(),
+                sample="%s lattice, %d of %d states, %d PECE steps after 1 warm-up, %d OpenMP threads "
+                       "(all host cores), %.1f s"
                        % ("x".join(str(x) for x in (cfg.nx, cfg.ny, cfg.nz)), ns, n, k, threads, dt))
 
 
@@ -279,9 +303,11 @@ def summarize(g, args, mode):
     return out
 
 
-def density_errors(d, ref, N):
-    """Relative error per density field (max-abs difference over max-abs
-    value; nu as a complex field)."""
+def density_errors(d, ref, N, currents=True):
+    """Relative error per density field: max-abs difference over the
+    field's own max-abs value (nu as a complex field, j = (j_x, j_y, j_z) as
+    one field).  currents=False leaves j out (a uniform state, whose currents
+    are zero up to noise)."""
     import numpy as np
     out = {}
     nu, nur = d[0:2 * N:2] + 1j * d[1:2 * N:2], ref[0:2 * N:2] + 1j * ref[1:2 * N:2]
@@ -289,13 +315,10 @@ def density_errors(d, ref, N):
     for k, i in (("rho_a", 2), ("tau_a", 3), ("rho_b", 7), ("tau_b", 8)):
         r = ref[i * N:(i + 1) * N]
         out[k] = float(np.abs(d[i * N:(i + 1) * N] - r).max() / np.abs(r).max())
-    # currents vanish in a uniform state: measured against the current scale
-    # of the gas, rho * kF (or their own maximum, if larger)
-    n = ref[2 * N:3 * N] + ref[7 * N:8 * N]
-    jscale = n.max() * (3 * np.pi ** 2 * n.max()) ** (1.0 / 3.0)
-    for k, i0 in (("j_a", 4), ("j_b", 9)):
-        r = ref[i0 * N:(i0 + 3) * N]
-        out[k] = float(np.abs(d[i0 * N:(i0 + 3) * N] - r).max() / max(np.abs(r).max(), jscale))
+    if currents:
+        for k, i0 in (("j_a", 4), ("j_b", 9)):
+            r = ref[i0 * N:(i0 + 3) * N]
+            out[k] = float(np.abs(d[i0 * N:(i0 + 3) * N] - r).max() / np.abs(r).max())
     return out
 
 
@@ -362,69 +385,89 @@ def run_adjoint(args):
     }
 
 
-def run_trap(args):
-    """Split precision on the trapped vortex (32^3, 24576 states): throughput
-    of the full workload, and the split run's density / energy errors against
-    an FP64 run of the same steps on its lowest 4096 states (both forms of
-    the full workload do not fit one GPU's HBM side by side)."""
-    import numpy as np
+def trap_leg(args, prec, nst, steps, timed):
+    """The trapped-vortex workload (32^3) in one precision: initial
+    measurement, three steps (the last a gradient step), then `steps` timed
+    steps with gradient steps every meas_every and at the end.  Every array
+    is released before returning."""
     import torch
     from wslda_amd import problems, td
+    sys.stderr.write("trap-vortex: %s, %d states, %d steps\n" % ("split" if prec == td.SPLIT else "fp64", nst,
+                                                                 steps))
+    sys.stderr.flush()
+    w = problems.config3(torch, precision=prec, n_states=nst)
+    p = w["params"]
+    n = w["wf"].shape[0] // 2
+    ctx = td.TDContext(p, n, 0)
+    ctx.set_states(w["fbeta"])
+    R = td.TDRun(ctx, w["wf"], w["pots"], fused=True)
+    del w["wf"]
+    R.initial_measure()
+    for it in range(3):
+        R.step(it, grad_last=(it == 2))
+    ctx.sync()
+    e0 = R.energy(3)
+    stream = torch.cuda.ExternalStream(ctx.stream_ptr())
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    if timed:
+        ctx.timing(True)
+        ctx.timing_report()
+    ev0.record(stream)
+    for it in range(3, 3 + steps):
+        R.step(it, grad_last=((it + 1) % args.meas_every == 0) or it == 2 + steps)
+    R.finalize()
+    ev1.record(stream)
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    rep = ctx.timing_report() if timed else {}
+    ctx.timing(False)
+    e1 = R.energy(3 + steps)
+    d = R.dens.cpu().numpy()
+    N = ctx.N
+    ctx.close()
+    del R, ctx
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return dict(qps=n * steps / (ms / 1e3), ms=ms / steps, n=n, N=N, dens=d, e0=e0, e1=e1, rep=rep)
 
-    def run(prec, nst, steps, timed):
-        sys.stderr.write("trap-vortex: %s, %d states, %d steps\n" % ("split" if prec == td.SPLIT else "fp64", nst,
-                                                                     steps))
-        sys.stderr.flush()
-        w = problems.config3(torch, precision=prec, n_states=nst)
-        p = w["params"]
-        n = w["wf"].shape[0] // 2
-        ctx = td.TDContext(p, n, 0)
-        ctx.set_states(w["fbeta"])
-        R = td.TDRun(ctx, w["wf"], w["pots"], fused=True)
-        del w["wf"]
-        R.initial_measure()
-        for it in range(3):
-            R.step(it, grad_last=(it == 2))
-        ctx.sync()
-        e0 = R.energy(3)
-        stream = torch.cuda.ExternalStream(ctx.stream_ptr())
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        if timed:
-            ctx.timing(True)
-            ctx.timing_report()
-        ev0.record(stream)
-        for it in range(3, 3 + steps):
-            R.step(it, grad_last=((it + 1) % args.meas_every == 0) or it == 2 + steps)
-        R.finalize()
-        ev1.record(stream)
-        ev1.synchronize()
-        ms = ev0.elapsed_time(ev1)
-        rep = ctx.timing_report() if timed else {}
-        ctx.timing(False)
-        e1 = R.energy(3 + steps)
-        d = R.dens.cpu().numpy()
-        N = ctx.N
-        ctx.close()
-        return dict(qps=n * steps / (ms / 1e3), ms=ms / steps, n=n, N=N, dens=d, e0=e0, e1=e1, rep=rep)
 
-    full = run(td.SPLIT, args.trap_states, args.steps, True)
-    sub64 = run(td.FP64, 4096, args.steps, False)
-    subsp = run(td.SPLIT, 4096, args.steps, False)
-    err = density_errors(subsp["dens"], sub64["dens"], sub64["N"])
-    E64, Esp = sub64["e1"][:7].sum(), subsp["e1"][:7].sum()
+def trap_pair(args):
+    """Split precision and FP64 on the whole trapped-vortex workload (32^3,
+    24576 states), one after the other (each alone fills most of the HBM):
+    both throughputs, the speedup, and the split run's density errors
+    against the FP64 run after the same steps from the same seeded start --
+    the current j included, measured against its own maximum (the vortex
+    carries real currents)."""
+    from wslda_amd import td
+    steps = args.trap_steps or args.steps
+    sp = trap_leg(args, td.SPLIT, args.trap_states, steps, True)
+    fp = trap_leg(args, td.FP64, args.trap_states, steps, True)
+    err = density_errors(sp["dens"], fp["dens"], fp["N"], currents=True)
+    E64, Esp = fp["e1"][:7].sum(), sp["e1"][:7].sum()
     drift = lambda r: abs(r["e1"][:7].sum() - r["e0"][:7].sum()) / abs(r["e0"][:7].sum())
+    N = fp["N"]
+    return {
+        "workload": "td-SLDA 3D 32^3, harmonic trap omega = 0.1 eF, vortex line imprinted on u, %d Gaussian-envelope "
+                    "plane-wave BdG states (seeded noise), AB5AM5, %d timed steps" % (sp["n"], steps),
+        "split_value": sp["qps"], "fp64_value": fp["qps"], "unit": "qp-steps/s",
+        "split_ms_per_step": sp["ms"], "fp64_ms_per_step": fp["ms"], "speedup_vs_fp64": sp["qps"] / fp["qps"],
+        "density_rel_err_vs_fp64": err, "max_density_rel_err": max(err.values()),
+        "j_inf_fp64": float(abs(fp["dens"][4 * N:7 * N]).max()),
+        "energy_rel_diff_vs_fp64": abs(Esp - E64) / abs(E64),
+        "energy_drift_fp64": drift(fp), "energy_drift_split": drift(sp), "drift_steps": steps,
+        "split_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(sp["rep"].items(), key=lambda kv: -kv[1][0])},
+        "fp64_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(fp["rep"].items(), key=lambda kv: -kv[1][0])},
+    }
+
+
+def run_trap(args):
+    """--trap: the trapped-vortex split / FP64 pair as its own JSON line."""
+    t = trap_pair(args)
     return {
         "metric": "qp-wavefunction steps/s, split precision, trapped vortex",
-        "value": full["qps"], "unit": "qp-steps/s", "n_gpus": 1, "steps": args.steps, "ms_per_step": full["ms"],
-        "higher_is_better": True, "dtype": "f32-operator/f64-state", "data": "synthetic",
-        "config": {"workload": "td-SLDA 3D 32^3, harmonic trap omega = 0.1 eF, vortex line imprinted on u, "
-                               "%d Gaussian-envelope plane-wave BdG states (seeded noise), AB5AM5, split precision"
-                               % full["n"]},
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in sorted(full["rep"].items(), key=lambda kv: -kv[1][0])},
-        "errors_vs_fp64_on_4096_states": {"density_rel_err": err, "max_density_rel_err": max(err.values()),
-                                          "energy_rel_diff": abs(Esp - E64) / abs(E64),
-                                          "fp64_qps_4096": sub64["qps"], "split_qps_4096": subsp["qps"],
-                                          "energy_drift_fp64": drift(sub64), "energy_drift_split": drift(subsp)},
+        "value": t["split_value"], "unit": "qp-steps/s", "n_gpus": 1, "steps": t["drift_steps"],
+        "ms_per_step": t["split_ms_per_step"], "higher_is_better": True, "dtype": "f32-operator/f64-state",
+        "data": "synthetic", "config": {"workload": t["workload"] + ", split precision"}, "trap_vortex": t,
     }
 
 
@@ -465,7 +508,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--states", type=int, default=16384)
+    ap.add_argument("--states", type=int, default=None,
+                    help="states of the 24^3 workload: the uniform solution's whole state set below its "
+                         "cutoff (16308) by default; another value must equal that set's size")
     ap.add_argument("--mode", choices=["both", "fp64", "split"], default="both",
                     help="both: the FP64 run (value) and the split-precision run on the same initial state, "
                          "with the split run's density errors against FP64")
@@ -495,7 +540,10 @@ def main():
                     help="segments the forward run tapes directly (each 2K+1 state arrays of HBM)")
     ap.add_argument("--trap", action="store_true",
                     help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
-    ap.add_argument("--trap-states", type=int, default=24576)
+    ap.add_argument("--trap-states", type=int, default=24576,
+                    help="states of the trapped-vortex pair in the default line (0: leave the pair out)")
+    ap.add_argument("--trap-steps", type=int, default=0, help="timed steps of the trapped-vortex pair "
+                                                              "(0: --steps)")
     ap.add_argument("--aslda", action="store_true",
                     help="instead of the step benchmark: the ASLDA step on the 3D uniform workload (24^3)")
     ap.add_argument("--sldae", action="store_true",
@@ -536,8 +584,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if primary == "fp64" else "f32-operator/f64-state",
         "data": "synthetic",
-        "config": {"workload": "td-SLDA 3D uniform, %dx%dx%d lattice, %d qp states, AB5AM5, plane-wave BdG "
-                               "+ seeded 1e-6 noise, gradients every %d steps" % (24, 24, 24, n, args.meas_every),
+        "config": {"workload": "td-SLDA 3D uniform, %dx%dx%d lattice, %d qp states (the uniform solution's whole "
+                               "state set below its cutoff; 16384 is not the size of any such set on this "
+                               "lattice), AB5AM5, plane-wave BdG + seeded 1e-6 noise, gradients every %d steps"
+                               % (24, 24, 24, n, args.meas_every),
                    "lattice": [24, 24, 24], "states": n, "scheme": "AB5AM5", "precision": primary,
                    "fused": bool(args.fused)},
         "roofline": head["roofline"],
@@ -549,7 +599,7 @@ def main():
         sp = summarize(gs, args, "split")
         dens64 = g["R"].dens.cpu().numpy()
         denss = gs["R"].dens.cpu().numpy()
-        err = density_errors(denss, dens64, g["N"])
+        err = density_errors(denss, dens64, g["N"], currents=False)
         e64, es = sum(g["e1"][:7]), sum(gs["e1"][:7])
         res["split"] = {"value": sp["value"], "ms_per_step": sp["ms_per_step"],
                         "speedup_vs_fp64": sp["value"] / res["value"],
@@ -564,6 +614,12 @@ def main():
     if not args.no_cpu and primary == "fp64":
         res["cpu_baseline"] = cpu_baseline(g, args)
     g["ctx"].close()
+    del g
+    if args.mode == "both" and args.trap_states > 0:
+        import torch
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        res["trap_vortex"] = trap_pair(args)
     nb = copy_calibration(args)
     if nb:
         res["copy_calib_bytes"] = nb

@@ -68,6 +68,14 @@ typedef struct { float re, im; } tdw_complex32;
 #define TDW_LZB 10
 #define TDW_NITEMS 11
 
+/* MAX_USER_PARAMS of the reference's predefines.h: length of the params
+ * array handed to the problem's callbacks (memcopy_const_params). */
+#define TDW_MAX_USER_PARAMS 32
+/* bytes per lattice point of the fftONE zone that create_cufftPlans adds
+ * behind the kernels and derivatives zones of the caller's work area
+ * (2 complex values per point) */
+#define TDW_FFTONE_BYTES_PER_POINT 32
+
 /* error codes (besides hipError_t values) */
 #define TDW_EINVAL (-1001)  /* bad argument / unsupported combination   */
 #define TDW_ENOCTX (-1002)  /* drop-in call before tdw_bind()           */
@@ -113,6 +121,15 @@ typedef struct {
     int force_a1;        /* SLDAE built with SLDA_FORCE_A1: A = 1, the A1
                             parameter set, constant-mass (FAST) step       */
     int precision;       /* TDW_FP64 or TDW_SPLIT                              */
+    /* Fermi-Dirac high-frequency filter of the mean fields (md.hkf_mode,
+     * md.hkf_mu, md.hkf_T; pca_utils.c:107-109).  hkf_mode >= 1: V_a, V_b
+     * (and, with a varying effective mass, alpha_a, alpha_b, A_a, A_b) are
+     * filtered before every H application; >= 2: Delta too.  The drop-in
+     * apply_hamiltonian filters V, Delta and A in place after the friction
+     * terms (pca_kernels.cu:733-764, 786); the whole-step entry points also
+     * filter alpha, as the driver does before alpha psi (pca.c:1448-1452). */
+    int hkf_mode;
+    double hkf_mu, hkf_T;
 } tdw_params;
 
 typedef struct tdw_ctx tdw_ctx;
@@ -127,6 +144,13 @@ int tdw_destroy(tdw_ctx *c);
 int tdw_sync(tdw_ctx *c);
 size_t tdw_npts(const tdw_ctx *c);
 int tdw_set_params(tdw_ctx *c, const tdw_params *p);   /* constants, V_ext; lattice fixed */
+/* The problem's user parameters (TDW_MAX_USER_PARAMS doubles, host pointer;
+ * copied).  The built-in problem ignores them; a project build
+ * (problem-definition.h compiled in) hands them to its callbacks as
+ * params / dc_params.  memcopy_const_params is this call on the bound
+ * context.  tdw_get_user_params copies them back. */
+int tdw_set_user_params(tdw_ctx *c, const double *h_params);
+int tdw_get_user_params(const tdw_ctx *c, double *h_params);
 /* per-state data: occupation weights f(E_n) (n doubles), transverse momenta
  * ([ky|kz], 2n doubles for dim 1; kz, n doubles for dim 2; NULL for dim 3) and
  * degeneracies (n ints, dim 1).  Host pointers; copied. */
@@ -393,7 +417,8 @@ int memcopy_host2gpu(void *h_src, void *d_dst, size_t size);       /* wslda_cuda
 int memcopy_gpu2host(void *d_src, void *h_dst, size_t size);
 int memcopy_gpu2gpu(void *d_src, void *d_dst, size_t size);
 int memcopy_const(double mu_a, double mu_b, double ec, double t0, double dt, double kF); /* :21-36 */
-int memcopy_const_params(double *params);                          /* :80-90 (no-op: params live in tdw_vext) */
+int memcopy_const_params(double *params);                          /* :80-90: TDW_MAX_USER_PARAMS doubles onto the
+                                                                      bound context (tdw_set_user_params) */
 int memcopy_const_sclgth(double sclgth);                           /* :92-97 */
 
 int create_cufftPlans(int batch_size, int nwfip, size_t *workSize); /* pca_derivative.h:12 */

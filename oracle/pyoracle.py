@@ -40,7 +40,8 @@ class Cfg(C.Structure):
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
                 ("vext", Vext), ("spinsymmetry", C.c_int),
                 ("kF", C.c_double), ("qfalpha", C.c_double), ("qfbeta", C.c_double), ("qfgamma", C.c_double),
-                ("force_a1", C.c_int)]
+                ("force_a1", C.c_int), ("precision", C.c_int),
+                ("hkf_mode", C.c_int), ("hkf_mu", C.c_double), ("hkf_T", C.c_double)]
 
 
 class Uniform(C.Structure):
@@ -127,7 +128,7 @@ def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
              regularization=SPHERICAL, mf_correction=0, scheme=AB4AM5,
              mu_a=0.0, mu_b=0.0, ec=0.0, dt=0.01, t0=0.0, sclgth=0.0,
              p_nmin=1.0e-7, p_nmax=1.0e-5, denseps=1.0e-8, vext=None, spinsymmetry=0, kF=1.0,
-             qf=(0.0, 0.0, 0.0), force_a1=0):
+             qf=(0.0, 0.0, 0.0), force_a1=0, hkf=(0, 0.0, 0.0)):
     c = Cfg()
     c.dim, c.nx, c.ny, c.nz = dim, nx, ny, nz
     c.dx = dx
@@ -142,6 +143,7 @@ def make_cfg(dim, nx, ny, nz, dx=1.0, dy=None, dz=None, functional=SLDA,
     c.kF = kF
     c.qfalpha, c.qfbeta, c.qfgamma = qf
     c.force_a1 = force_a1
+    c.hkf_mode, c.hkf_mu, c.hkf_T = hkf
     return c
 
 

@@ -1465,6 +1465,22 @@ static void half_h(const tdwo_cfg *c, tdwo_state *s, double *out)
     }
 }
 
+/* hkf_mode filtering of the mean fields before an H application: alpha_a,
+ * alpha_b in the varying-mass modes (pca.c:1448-1452), V_a, V_b
+ * (pca_kernels.cu:733-737), Delta for hkf_mode >= 2 (:758-762) and the
+ * vector potentials A_a, A_b with current corrections (:786-790), each in
+ * place with the Fermi-Dirac filter. */
+static void hkf_fields(const tdwo_cfg *c, double *pots)
+{
+    if (c->hkf_mode < 1) return;
+    const size_t N = tdwo_npts(c);
+    const int vm = c->functional == TDWO_ASLDA || (c->functional == TDWO_SLDAE && !c->force_a1);
+    if (vm) tdwo_hf_filter_d(c, 2, pots + 4 * N, pots + 4 * N, c->hkf_mu, c->hkf_T);
+    tdwo_hf_filter_d(c, 2, pots, pots, c->hkf_mu, c->hkf_T);
+    if (c->hkf_mode >= 2) tdwo_hf_filter_c(c, pots + 2 * N, pots + 2 * N, c->hkf_mu, c->hkf_T);
+    if (vm) tdwo_hf_filter_d(c, 6, pots + 6 * N, pots + 6 * N, c->hkf_mu, c->hkf_T);
+}
+
 int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
 {
     int n = s->n;
@@ -1475,6 +1491,7 @@ int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
     tdwo_normalize(c, n, s->wf);
     half_densities(c, s, full);
     tdwo_potentials(c, it + 1, s->dens, s->pots);
+    hkf_fields(c, s->pots);
     {
         /* the H output goes to lap (the reference's buffer), which the
          * full Hamiltonian still reads: stage it */
@@ -1489,6 +1506,7 @@ int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
     tdwo_normalize(c, n, s->wf);
     half_densities(c, s, grad_last || full);
     tdwo_potentials(c, it + 1, s->dens, s->pots);
+    hkf_fields(c, s->pots);
     rotate_history(c, s);
     half_h(c, s, s->f[0]);
     return 0;

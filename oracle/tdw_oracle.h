@@ -118,6 +118,10 @@ typedef struct {
     double kF;            /* reference kF: n_F = kF^3/(3 pi^2) (dc_nF)      */
     double qfalpha, qfbeta, qfgamma; /* quantum-friction strengths        */
     int force_a1;         /* SLDAE with SLDA_FORCE_A1: A = 1, FAST mode     */
+    int precision;        /* ignored: the oracle is FP64 throughout         */
+    int hkf_mode;         /* md.hkf_mode: >= 1 filter V (and alpha, A with a
+                             varying mass) before each H, >= 2 Delta too    */
+    double hkf_mu, hkf_T; /* Fermi-Dirac filter parameters                  */
 } tdwo_cfg;
 
 /* Number of evolved lattice points (NUMBER_ELEMENT). */

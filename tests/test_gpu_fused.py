@@ -74,7 +74,7 @@ def test_fused_vs_modular_24cubed(torch):
     modular steps agree, including a measurement step."""
     say("case: fused vs modular, 24^3, 512 states")
     from wslda_amd import problems, td
-    w = problems.config2(torch, n_states=512)
+    w = problems.config2(torch, first=512)
     p = w["params"]
     n = w["wf"].shape[0] // 2
     ctxs, runs = [], []
@@ -103,7 +103,7 @@ def test_fused_vs_modular_24cubed(torch):
 def test_timing_report(torch):
     say("case: per-kernel timing report")
     from wslda_amd import problems, td
-    w = problems.config2(torch, n_states=256)
+    w = problems.config2(torch, first=256)
     ctx = td.TDContext(w["params"], 256, 0)
     ctx.set_states(w["fbeta"])
     R = td.TDRun(ctx, w["wf"], w["pots"], fused=True)

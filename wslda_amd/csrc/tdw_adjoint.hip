@@ -638,7 +638,10 @@ static hist_dt_t hist_dt_fn(int h, int kp)
 
 static int adj_supported(const tdw_ctx *c)
 {
-    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64 && !c->p.spinsymmetry;
+    // the backward step differentiates the SLDA step as written: no
+    // spin-symmetric folding and no mean-field filter between the functional
+    // and H
+    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64 && !c->p.spinsymmetry && c->p.hkf_mode < 1;
 }
 
 }  // namespace tdw

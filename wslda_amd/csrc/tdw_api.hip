@@ -393,6 +393,18 @@ extern "C" int tdw_alpha_wf(tdw_ctx *c, int n, const tdw_complex *in, const doub
     if (!c || n > c->nwf) return TDW_EINVAL;
     return alpha_wf(c, n, CD2(in), D2(out), pots);
 }
+extern "C" int tdw_set_user_params(tdw_ctx *c, const double *h_params)
+{
+    if (!c || !h_params) return TDW_EINVAL;
+    for (int k = 0; k < TDW_MAX_USER_PARAMS; k++) c->user_params[k] = h_params[k];
+    return 0;
+}
+extern "C" int tdw_get_user_params(const tdw_ctx *c, double *h_params)
+{
+    if (!c || !h_params) return TDW_EINVAL;
+    for (int k = 0; k < TDW_MAX_USER_PARAMS; k++) h_params[k] = c->user_params[k];
+    return 0;
+}
 extern "C" int tdw_energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e)
 {
     if (!c) return TDW_EINVAL;
@@ -448,6 +460,7 @@ static int step_aslda(tdw_ctx *c, tdw_state *s, int it, int k, double *qpe)
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(aslda_densities(c, s));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
     // two-pass lattices fold the predictor's <H> subtraction into the corrector
     const bool fold = aslda_h_fused_supported(c);
     TDW_TRY(apply_h_aslda(c, n, D2(s->wf), D2(s->lap), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
@@ -458,6 +471,7 @@ static int step_aslda(tdw_ctx *c, tdw_state *s, int it, int k, double *qpe)
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(aslda_densities(c, s));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
     tdw_complex *tmp = s->f[k - 1];
     for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
@@ -487,6 +501,7 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(half_densities(c, s, 0, plane));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
     if (plane) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->lap), s->pots, c->d_kt, nullptr, qpe));
     else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
     // corrector
@@ -495,6 +510,7 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     const bool plane_c = plane && !grad_last;
     TDW_TRY(half_densities(c, s, grad_last, plane_c));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
     tdw_complex *tmp = s->f[k - 1];
     for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
@@ -572,10 +588,13 @@ extern "C" int memcopy_const(double mu_a, double mu_b, double ec, double t0, dou
     c->p.dt = dt;
     return 0;
 }
+// the user parameter array (MAX_USER_PARAMS doubles) of the problem's
+// callbacks (wslda_cuda_utils.h:80-90): kept on the bound context and, in a
+// project build, copied to the callbacks' dc_params
 extern "C" int memcopy_const_params(double *params)
 {
-    (void)params;
-    return 0;
+    BOUND();
+    return tdw_set_user_params(c, params);
 }
 extern "C" int memcopy_const_sclgth(double a)
 {
@@ -590,8 +609,14 @@ extern "C" int create_cufftPlans(int batch_size, int nwfip, size_t *workSize)
     (void)batch_size;
     if (nwfip > c->nwf) return TDW_EINVAL;
     // the caller's work area carries the <H> values and the energies back
-    // (pca.c:1575, :1590), so it must hold at least that much
-    if (workSize) *workSize = sizeof(double) * (size_t)(nwfip > TDW_NITEMS ? nwfip : TDW_NITEMS);
+    // (pca.c:1575, :1590) and holds the mm_get_* zones: kernels and
+    // derivatives (mm_get_size_of_total_workspace) followed by the fftONE
+    // zone, which the reference sizes as a one-array plan's work on top of
+    // the total (pca_derivative.cu:86-98); here that zone is 2 N complex
+    // values (TDW_FFTONE_BYTES_PER_POINT bytes per point)
+    const size_t ret = sizeof(double) * (size_t)(nwfip > TDW_NITEMS ? nwfip : TDW_NITEMS);
+    const size_t zones = mm_get_size_of_total_workspace(c->N) + (size_t)TDW_FFTONE_BYTES_PER_POINT * c->N;
+    if (workSize) *workSize = ret > zones ? ret : zones;
     return 0;
 }
 extern "C" int set_workspace_for_cufftPlan(void *workArea)
@@ -826,6 +851,9 @@ static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *
         qf = dens + 12 * c->N;
         TDW_TRY(qf_add(c, qf, pots, qfswitch, pccoeff));
     }
+    // mean-field filtering, in place, after the friction terms
+    // (pca_kernels.cu:733-764, 786-790); alpha is the driver's to filter
+    TDW_TRY(hkf_fields(c, pots, false));
     if (varying_mass(c)) {
         if (!dx || (c->dim >= 2 && !dy) || (c->dim >= 3 && !dz) || !alap || in == out) return TDW_EINVAL;
         TDW_TRY(apply_h_aslda(c, n, CD2(in), D2(out), CD2(dx), CD2(dy), CD2(dz), CD2(lap), CD2(alap), pots, kt,

@@ -612,6 +612,7 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
         TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
                           s->dens, 0));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
     TDW_TRY(hyz<SP>(c, n, yp, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
 
     // ---- corrector ----
@@ -632,6 +633,7 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
             TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
                               s->dens, 0));
         TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
         tdw_complex *tmp = s->f[h - 1];
         for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
         s->f[0] = tmp;
@@ -647,6 +649,7 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
                       c->d_fbeta, nullptr, c->d_kt, c->d_cnt, s->dens, 1));
     TDW_TRY(density_tau(c, s->dens));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(hkf_fields(c, s->pots, true));
     tdw_complex *tmp = s->f[h - 1];
     for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;

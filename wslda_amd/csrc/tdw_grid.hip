@@ -198,6 +198,24 @@ int grid_filter_d(tdw_ctx *c, int n, const double *in, double *out, double mu, d
     return (int)hipGetLastError();
 }
 
+// hkf_mode filtering of the mean fields, in place: V_a, V_b (hkf_mode >= 1,
+// pca_kernels.cu:733-737), Delta (>= 2, :758-762), and with a varying
+// effective mass the vector potentials A_a, A_b (:786-790) and, when the
+// caller stands in for the driver, alpha_a, alpha_b (pca.c:1448-1452)
+int hkf_fields(tdw_ctx *c, double *pots, bool with_alpha)
+{
+    if (c->p.hkf_mode < 1) return 0;
+    const long N = (long)c->N;
+    const double mu = c->p.hkf_mu, T = c->p.hkf_T;
+    if (!(T > 0.0)) return TDW_EINVAL;
+    const bool vm = varying_mass(c);
+    if (vm && with_alpha) TDW_TRY(grid_filter_d(c, 2, pots + 4 * N, pots + 4 * N, mu, T));
+    TDW_TRY(grid_filter_d(c, 2, pots, pots, mu, T));
+    if (c->p.hkf_mode >= 2) TDW_TRY(grid_filter_c(c, (const double2 *)(pots + 2 * N), (double2 *)(pots + 2 * N), mu, T));
+    if (vm) TDW_TRY(grid_filter_d(c, 6, pots + 6 * N, pots + 6 * N, mu, T));
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // overlaps <wf1_a | wf2_a> = dV sum (conj(u1) u2 + conj(v1) v2), one block per
 // state (pca_kernels.cu:853-911)

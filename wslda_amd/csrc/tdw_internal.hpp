@@ -67,6 +67,8 @@ struct tdw_ctx {
     // memcopy_extra_data(): the caller's device block, kept for its own code
     void *extra_data;
     size_t extra_size;
+    // memcopy_const_params(): the problem's user parameters
+    double user_params[TDW_MAX_USER_PARAMS];
     // per-kernel timing
     int timing;
     std::vector<tdw_timer_rec> *timers;  // pending records

@@ -106,6 +106,8 @@ int grid_dvec(tdw_ctx *c, const double *const f[3], double *const df[3]);
 int grid_div(tdw_ctx *c, const double *const f[3], double *div);
 int grid_filter_d(tdw_ctx *c, int n, const double *in, double *out, double mu, double T);
 int grid_filter_c(tdw_ctx *c, const double2 *in, double2 *out, double mu, double T);
+// hkf_mode filtering of V, Delta, A (and alpha with with_alpha) in place
+int hkf_fields(tdw_ctx *c, double *pots, bool with_alpha);
 int grid_overlap(tdw_ctx *c, int n, const double2 *w1, const double2 *w2, double *re, double *im);
 int grid_sum(tdw_ctx *c, const double *a, long n, double *out);
 // quantum friction: the densities U_a, U_b, D (4N doubles) and the friction

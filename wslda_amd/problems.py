@@ -92,41 +92,76 @@ def states_on_gpu(torch, params, K, U, V, device="cuda:0", chunk=2048, envelope=
 
 
 def _base_params(dim, nx, ny, nz, dx, scheme=td.AB5AM5, precision=td.FP64, vext=None, dt_eF=0.005,
-                 functional=td.SLDA, sclgth=0.0):
+                 functional=td.SLDA, sclgth=0.0, dperp=None):
     eF = 0.5
-    return td.make_params(dim, nx, ny, nz, dx=dx, functional=functional, regularization=td.SPHERICAL_CUTOFF,
+    dp = dx if dperp is None else dperp
+    return td.make_params(dim, nx, ny, nz, dx=dx, dy=dx if dim >= 2 else dp, dz=dx if dim == 3 else dp,
+                          functional=functional, regularization=td.SPHERICAL_CUTOFF,
                           mf_correction=1, scheme=scheme, dt=dt_eF / eF, vext=vext, precision=precision,
                           sclgth=sclgth)
 
 
-def uniform_slda(params, T=1.0e-9):
+def uniform_slda(params, T=1.0e-9, kc=None, eps=1.0e-14):
     """Self-consistent uniform SLDA (or ASLDA, by params.functional) state at
-    kF = 1 (n_a = n_b = n/2)."""
+    kF = 1 (n_a = n_b = n/2).  kc: momentum cutoff of the regularised
+    coupling (default 0.999999 pi / max(DX, DY, DZ), pca_utils.c:171); the
+    solution's energy cutoff is ec = alpha_+ kc^2 / 2 - mu_+
+    (pca_uniform.h:256-262).  eps: convergence tolerance on n_a, n_b and
+    Delta.  The td step recomputes the mean fields from the states' own
+    densities, so a start is stationary only to the tolerance the solution
+    was converged to: 1e-9 leaves a one-off energy offset of ~3e-10 after
+    the first step, 1e-14 one of ~3e-15."""
     n0 = 1.0 / (3.0 * PI ** 2)
-    kc = 0.999999 * PI / max(params.dx, params.dy, params.dz)
-    u, rc = td.uniform_solve(params, n0 / 2, n0 / 2, kc, T, eps=1e-9, scmix=0.5, muchange=0.5, maxiter=10000)
+    if kc is None:
+        kc = 0.999999 * PI / max(params.dx, params.dy, params.dz)
+    u, rc = td.uniform_solve(params, n0 / 2, n0 / 2, kc, T, eps=eps, scmix=0.5, muchange=0.5, maxiter=10000)
     if rc != 0:
         raise td.TDWError("uniform SLDA solution did not converge")
     return u
 
 
-def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, device="cuda:0", scheme=td.AB5AM5,
-            shape=(24, 24, 24), dx=1.0, functional=td.SLDA, sclgth=0.0):
-    """Plain 3D uniform SLDA: 24^3 periodic lattice, the n_states lowest-|E|
-    plane-wave BdG states of the self-consistent uniform solution (Delta of
-    the solution ~ 0.5 eF), seeded random-phase noise of amplitude 1e-6.
+# Momentum cutoffs at which the uniform solution's own state set (every state
+# with -ec < E < ec, as create_uniform_wf keeps them, pca_uniform.h:737) has
+# the size the workloads name.  The set grows by whole k-shells: on the 24^3
+# lattice (dx = 1) the shell counts step 16212 -> 16308 -> 16500, so 16384
+# is not the size of any such set and the 24^3 workload takes the 16308
+# states of the shells |m|^2 <= 157 (kc in [3.2825, 3.2925]; the middle of the
+# interval keeps every state away from the cutoff).  The 1D workload's 128
+# states form such a set on a 4x4 transverse box with DY = DZ = 2.5 for kc in
+# [1.400, 1.410].
+KC_24CUBED_16308 = 3.2875
+KC_1D_128 = 1.405
+
+
+def config2(torch, precision=td.FP64, n_states=None, seed=1, noise=1.0e-6, device="cuda:0", scheme=td.AB5AM5,
+            shape=(24, 24, 24), dx=1.0, functional=td.SLDA, sclgth=0.0, kc=None, first=None):
+    """Plain 3D uniform SLDA: 24^3 periodic lattice, the self-consistent
+    uniform solution's own state set -- every plane-wave BdG state with
+    -ec < E < ec, in create_uniform_wf's order (pca_uniform.h:636-850) --
+    plus seeded random-phase noise of amplitude `noise`.  Without noise the
+    start is stationary: (H - <H>) psi = 0 for every state.
+
+    kc: momentum cutoff; the default on the 24^3 lattice is
+    KC_24CUBED_16308 (16308 states, Delta = 0.485 eF), elsewhere the
+    reference's default.  n_states, if given, must equal the size of the
+    set (an error otherwise).  first: keep only the first `first` states of
+    the set (a slice for parity tests; not a stationary start).
     functional=ASLDA: the same with the ASLDA solution (effective mass
     alpha = 1.094 at n_a = n_b); functional=SLDAE: the SLDAE solution at
     scattering length sclgth (k_F = 1 units)."""
     nx, ny, nz = shape
     p = _base_params(3, nx, ny, nz, dx, scheme=scheme, precision=precision, functional=functional, sclgth=sclgth)
-    u = uniform_slda(p)
-    # all plane-wave states of the lattice, the n_states of lowest |E| (a few
-    # may lie above the solution's cutoff when n_states exceeds its count)
-    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf, alph=u.alph_a)
-    order = np.argsort(np.abs(E), kind="stable")[:n_states]
-    order.sort()   # keep create_uniform_wf's order among the selected states
-    K, E, U, V = K[order], E[order], U[order], V[order]
+    if kc is None and shape == (24, 24, 24) and dx == 1.0 and functional == td.SLDA:
+        kc = KC_24CUBED_16308
+    u = uniform_slda(p, kc=kc)
+    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, u.ec, alph=u.alph_a)
+    if len(E) != u.nwf:
+        raise td.TDWError("plane-wave set (%d) differs from the solver's state count (%d)" % (len(E), u.nwf))
+    if n_states is not None and n_states != len(E):
+        raise td.TDWError("the uniform solution at kc = %.6f has %d states below its cutoff, not %d"
+                          % (u.kc, len(E), n_states))
+    if first is not None:
+        K, E, U, V = K[:first], E[:first], U[:first], V[:first]
     wf = states_on_gpu(torch, p, K, U, V, device=device)
     if noise > 0:
         g = torch.Generator(device=device)
@@ -145,7 +180,7 @@ def config2(torch, precision=td.FP64, n_states=16384, seed=1, noise=1.0e-6, devi
     pots[5 * N:6 * N] = u.alph_b
     fb = np.array([td.fbeta(e, u.beta) for e in E])
     eF = 0.5
-    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
+    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF, E=E,
                 name="3D uniform %s %dx%dx%d, %d states" % ({td.ASLDA: "ASLDA", td.SLDAE: "SLDAE"}.get(
                     functional, "SLDA"), nx, ny, nz, len(E)))
 
@@ -163,14 +198,15 @@ def select_states(wf, En, kt, cnt, n_states, dim):
     return wfs, En[order], kts, cnts
 
 
-def uniform_start(dim, shape, dx=1.0, n_states=None, scheme=td.AB5AM5, dt_eF=0.005, T=1.0e-9):
+def uniform_start(dim, shape, dx=1.0, n_states=None, scheme=td.AB5AM5, dt_eF=0.005, T=1.0e-9, dt_perp=None,
+                  kc=None):
     """Self-consistent uniform SLDA state at kF = 1 on a 1D/2D/3D lattice
-    (transverse box from shape for dim < 3) and its plane-wave states (all,
-    or the n_states of lowest |E|), host arrays: params, wf [2n, N], pots
-    (12N), fbeta, kt, cnt."""
+    (transverse box from shape for dim < 3, spacing dt_perp, default dx) and
+    its plane-wave states (all, or the n_states of lowest |E|), host arrays:
+    params, wf [2n, N], pots (12N), fbeta, kt, cnt."""
     nx, ny, nz = shape
-    p = _base_params(dim, nx, ny, nz, dx, scheme=scheme, dt_eF=dt_eF)
-    u = uniform_slda(p, T=T)
+    p = _base_params(dim, nx, ny, nz, dx, scheme=scheme, dt_eF=dt_eF, dperp=dt_perp)
+    u = uniform_slda(p, T=T, kc=kc)
     p.mu_a, p.mu_b, p.ec = u.mu_a, u.mu_b, u.ec
     wf, En, kt, cnt = td.uniform_states(p, u)
     if n_states is not None and n_states < En.size:
@@ -252,16 +288,20 @@ def config3(torch, precision=td.SPLIT, n_states=24576, seed=3, device="cuda:0", 
     pots[2 * N:4 * N:2] = u.delta
     pots[4 * N:6 * N] = 1.0
     fb = np.array([td.fbeta(e, u.beta) for e in E])
-    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF,
+    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF, E=E,
                 name="3D trapped SLDA with a vortex line %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
 
 
-def config1(n_states=128, shape=(64, 8, 8), dx=1.0, amp_eF=0.1, width_dx=4.0):
-    """1D uniform SLDA at unitarity: 64-point periodic lattice (8x8
-    transverse box, which the uniform solution needs to reach kF = 1), the
-    128 lowest-|E| plane-wave BdG states, Delta ~ 0.5 eF, and a Gaussian
+def config1(n_states=128, shape=(64, 4, 4), dx=1.0, dperp=2.5, kc=KC_1D_128, amp_eF=0.1, width_dx=4.0):
+    """1D uniform SLDA at unitarity: 64-point periodic lattice with dx = 1
+    (a 4x4 transverse box of spacing 2.5, which carries the kF = 1 density),
+    the uniform solution's own state set -- all 128 plane-wave BdG states
+    with -ec < E < ec at kc = KC_1D_128 (Delta = 0.47 eF) -- and a Gaussian
     V_ext of amplitude 0.1 eF and width 4 dx on from t = 0 (host arrays)."""
-    s = uniform_start(1, shape, dx=dx, n_states=n_states)
+    s = uniform_start(1, shape, dx=dx, dt_perp=dperp, kc=kc)
+    if s["En"].size != n_states:
+        raise td.TDWError("the 1D uniform solution has %d states below its cutoff, not %d"
+                          % (s["En"].size, n_states))
     p = s["params"]
     eF = 0.5
     w = -0.5 / (width_dx * dx) ** 2

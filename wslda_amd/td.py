@@ -51,7 +51,8 @@ class Params(C.Structure):
                 ("p_nmin", C.c_double), ("p_nmax", C.c_double), ("denseps", C.c_double),
                 ("vext", Vext), ("spinsymmetry", C.c_int),
                 ("kF", C.c_double), ("qfalpha", C.c_double), ("qfbeta", C.c_double), ("qfgamma", C.c_double),
-                ("force_a1", C.c_int), ("precision", C.c_int)]
+                ("force_a1", C.c_int), ("precision", C.c_int),
+                ("hkf_mode", C.c_int), ("hkf_mu", C.c_double), ("hkf_T", C.c_double)]
 
 
 class KMode(C.Structure):
