@@ -231,6 +231,19 @@ int tdw_finalize(tdw_ctx *c, tdw_state *s);
 int tdw_fused_supported(tdw_ctx *c);
 /* pca.c:939-975: normalise, derivatives, densities, tau (potentials kept). */
 int tdw_initial_measure(tdw_ctx *c, tdw_state *s);
+/* Self-start of the ABM history (selfstart 1, pca.c:1086-1392): K = 3, 4, 5
+ * steps (AB3AM4, AB4AM5, AB5AM5) of the midpoint rule with exp(-i (H -
+ * <H>) dt) as its Taylor polynomial of order 4, 5, 5 (taylor_expansion_
+ * contribution), the potentials of iteration 0 throughout, no quantum
+ * friction.  Leaves wf advanced by K steps (the caller's iteration counter
+ * is not advanced, as in the reference), f[0] = fkm1 = (H - <H>) psi of the
+ * last self-start step and f[1..K-1] = that of the step before (the
+ * reference's host shift copies slot j to j + 1 for increasing j), then the
+ * derivatives, densities with tau and j, and the potentials of the new
+ * state.  f[0..2] serve as scratch; the two kept entries live in f[3], f[4]
+ * for AB5AM5, in f[3] and scratch0 for AB4AM5, in scratch0 and scratch1
+ * for AB3AM4 (state-sized, may be NULL where not needed).  FP64 only. */
+int tdw_selfstart(tdw_ctx *c, tdw_state *s, tdw_complex *scratch0, tdw_complex *scratch1);
 
 /* td checkpoint files (hpc-engine/tdwslda_checkpoint.c:118-364), byte layout
  * of a single-rank run: header (t0, mu[2], ec, kF, nwf, eF, Effg, beta), then

@@ -105,6 +105,8 @@ def lib():
             "tdwo_energy": (None, [cp, I, P, P, P]),
             "tdwo_step": (I, [cp, C.POINTER(State), I, I]),
             "tdwo_initial_measure": (None, [cp, C.POINTER(State)]),
+            "tdwo_taylor": (None, [cp, I, D, I, P, P, P]),
+            "tdwo_selfstart": (I, [cp, C.POINTER(State)]),
             "tdwo_uniform_solve": (I, [cp, D, D, D, D, D, D, D, D, D, I, I, C.POINTER(Uniform)]),
             "tdwo_uniform_count": (I, [cp, C.POINTER(Uniform)]),
             "tdwo_uniform_wf": (I, [cp, C.POINTER(Uniform), I, I, P, P, P, P]),
@@ -218,6 +220,12 @@ class Problem:
 
     def step(self, it, grad_last=0):
         return lib().tdwo_step(C.byref(self.cfg), C.byref(self.st), it, grad_last)
+
+    def selfstart(self):
+        """Taylor self-start of pca.c:1086-1392 (history filled, state
+        advanced by 3 / 4 / 5 steps, densities and potentials of the new
+        state)."""
+        return lib().tdwo_selfstart(C.byref(self.cfg), C.byref(self.st))
 
     def energy(self, it):
         e = np.zeros(11)

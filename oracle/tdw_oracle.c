@@ -1512,6 +1512,143 @@ int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last)
     return 0;
 }
 
+/* Taylor term of the self-start (pca_kernels.cu:1032-1085): contr =
+ * (-i dt / order) hpsi, upd += contr, over 2 n N complex values. */
+void tdwo_taylor(const tdwo_cfg *c, int order, double dt, int n, const double *hpsi, double *upd, double *contr)
+{
+    const size_t tot = 2 * (size_t)n * tdwo_npts(c);
+    const zc *H = (const zc *)hpsi;
+    zc *U = (zc *)upd, *T = (zc *)contr;
+    const double tc = -1.0 * dt / (double)order;
+    for (size_t i = 0; i < tot; i++) {
+        const zc v = H[i] * (I * tc);
+        T[i] = v;
+        U[i] += v;
+    }
+}
+
+/* Derivatives the self-start takes of an array before H is applied to it
+ * (pca.c:1119-1126, 1179-1186): all of them with gradients, the Laplacian
+ * alone otherwise; lap(alpha psi) into al with a varying mass
+ * (pca.c:1152-1154, 1188-1191). */
+static void ss_derivs(const tdwo_cfg *c, tdwo_state *s, const double *in, int grad, int vm, double *al)
+{
+    const int n = s->n;
+    if (grad) tdwo_derivatives(c, 2 * n, in, s->ddx, s->ddy, s->ddz, s->lap);
+    else tdwo_laplace(c, 2 * n, in, s->lap);
+    if (vm) {
+        tdwo_alpha_wf(c, n, in, s->pots, al);
+        tdwo_laplace(c, 2 * n, al, al);
+    }
+}
+
+/* apply_hamiltonian as the self-start calls it (qfswitch 0): the mean-field
+ * filter acts on V, Delta and A in place inside every call
+ * (pca_kernels.cu:733-764, 786-790), then H - <H> (useqpe NULL: <H> into
+ * qpe; otherwise the given values). */
+static void ss_apply_h(const tdwo_cfg *c, tdwo_state *s, const double *in, double *out, int vm, const double *al,
+                       const double *useqpe, double *qpe)
+{
+    const int n = s->n;
+    if (c->hkf_mode >= 1) {
+        const size_t N = tdwo_npts(c);
+        tdwo_hf_filter_d(c, 2, s->pots, s->pots, c->hkf_mu, c->hkf_T);
+        if (c->hkf_mode >= 2) tdwo_hf_filter_c(c, s->pots + 2 * N, s->pots + 2 * N, c->hkf_mu, c->hkf_T);
+        if (vm) tdwo_hf_filter_d(c, 6, s->pots + 6 * N, s->pots + 6 * N, c->hkf_mu, c->hkf_T);
+    }
+    if (vm) tdwo_apply_h_full(c, n, in, out, s->ddx, s->ddy, s->ddz, s->lap, al, s->pots, s->kt, useqpe, qpe);
+    else tdwo_apply_h(c, n, in, out, s->lap, s->pots, s->kt, useqpe, qpe);
+}
+
+/* Densities and potentials of an array, at iteration 0 (pca.c:1127-1159,
+ * 1223-1251): densities (with gradients, tau), the functional, and with a
+ * varying mass the filter of alpha. */
+static void ss_fields(const tdwo_cfg *c, tdwo_state *s, const double *wf, int grad, int vm)
+{
+    const int n = s->n;
+    tdwo_densities(c, n, wf, s->ddx, s->ddy, s->ddz, s->lap, s->fbeta, s->kt, s->cnt, s->dens, grad);
+    if (grad) tdwo_density_tau(c, s->dens);
+    tdwo_potentials(c, 0, s->dens, s->pots);
+    if (vm && c->hkf_mode >= 1) {
+        const size_t N = tdwo_npts(c);
+        tdwo_hf_filter_d(c, 2, s->pots + 4 * N, s->pots + 4 * N, c->hkf_mu, c->hkf_T);
+    }
+}
+
+/* Self-start (pca.c:1086-1392, selfstart 1): K = 3, 4, 5 steps (AB3AM4,
+ * AB4AM5, AB5AM5), each the midpoint rule with exp(-i (H - <H>) dt) taken as
+ * its Taylor polynomial of order E = 4, 5, 5:
+ *   predictor: densities / potentials of psi (it = 0), F = (H - <H>) psi
+ *     kept as this step's history entry, psi_m = sum_k (-i dt/2 (H - <H>))^k
+ *     psi / k!, normalised;
+ *   corrector: densities / potentials of psi_m, then psi <- sum_k (-i dt (H
+ *     - <H>))^k psi / k!, normalised, with H of the midpoint potentials and
+ *     <H> of its first application.
+ * The gradients are taken only with a varying effective mass (pca.c:1117,
+ * 1213: no quantum friction here).  The history entries go through a host
+ * buffer whose shift copies slot j to slot j + 1 for increasing j
+ * (pca.c:1166), so every slot but the first ends up holding the same entry:
+ * fkm1 = F_{K-1} and fkm2 .. fkmK = F_{K-2}; the restatement keeps that.
+ * The iteration counter is not advanced (pca.c:1144, 1240).  Afterwards:
+ * derivatives, densities with gradients, tau and the potentials of the new
+ * state (pca.c:1336-1356).  f[0..2] are used as scratch (fkm1..fkm3). */
+int tdwo_selfstart(const tdwo_cfg *c, tdwo_state *s)
+{
+    const int n = s->n;
+    const size_t N = tdwo_npts(c);
+    const size_t tot = 2 * (size_t)n * N;
+    const int K = c->scheme == TDWO_AB3AM4 ? 3 : c->scheme == TDWO_AB4AM5 ? 4 : 5;
+    const int E = c->scheme == TDWO_AB3AM4 ? 4 : 5;
+    const int vm = c->functional == TDWO_ASLDA || (c->functional == TDWO_SLDAE && !c->force_a1);
+    const int grad = vm;
+    const double dt = c->dt;
+    double *m1 = s->f[0], *m2 = s->f[1], *m3 = s->f[2];
+    double *al = vm ? (double *)malloc(sizeof(zc) * tot) : NULL;
+    double *qpe0 = (double *)malloc(sizeof(double) * (size_t)n);
+    double *slots = (double *)malloc(sizeof(zc) * tot * (size_t)K);
+    for (int step = 0; step < K; step++) {
+        /* predictor: half a step from psi */
+        ss_derivs(c, s, s->wf, grad, 0, NULL);
+        ss_fields(c, s, s->wf, grad, vm);
+        if (vm) ss_derivs(c, s, s->wf, grad, vm, al);
+        memcpy(m3, s->wf, sizeof(zc) * tot);
+        ss_apply_h(c, s, m3, m1, vm, al, NULL, qpe0);
+        for (int j = 0; j < K - 1; j++)
+            memcpy(slots + 2 * tot * (size_t)(j + 1), slots + 2 * tot * (size_t)j, sizeof(zc) * tot);
+        memcpy(slots, m1, sizeof(zc) * tot);
+        tdwo_taylor(c, 1, 0.5 * dt, n, m1, m3, m2);
+        for (int k = 1; k < E; k++) {
+            ss_derivs(c, s, m2, grad, vm, al);
+            ss_apply_h(c, s, m2, m1, vm, al, qpe0, NULL);
+            tdwo_taylor(c, k + 1, 0.5 * dt, n, m1, m3, m2);
+        }
+        tdwo_normalize(c, n, m3);
+        /* corrector: the whole step from psi with the midpoint's potentials */
+        ss_derivs(c, s, m3, grad, 0, NULL);
+        ss_fields(c, s, m3, grad, vm);
+        ss_derivs(c, s, s->wf, grad, vm, al);
+        ss_apply_h(c, s, s->wf, m1, vm, al, NULL, qpe0);
+        tdwo_taylor(c, 1, dt, n, m1, s->wf, m2);
+        for (int k = 1; k < E; k++) {
+            ss_derivs(c, s, m2, grad, vm, al);
+            ss_apply_h(c, s, m2, m1, vm, al, qpe0, NULL);
+            tdwo_taylor(c, k + 1, dt, n, m1, s->wf, m2);
+        }
+        tdwo_normalize(c, n, s->wf);
+    }
+    for (int j = 0; j < K; j++) memcpy(s->f[j], slots + 2 * tot * (size_t)j, sizeof(zc) * tot);
+    memcpy(s->qpe, qpe0, sizeof(double) * (size_t)n);
+    /* measurement of the new state (pca.c:1336-1356) */
+    tdwo_derivatives(c, 2 * n, s->wf, s->ddx, s->ddy, s->ddz, s->lap);
+    tdwo_densities(c, n, s->wf, s->ddx, s->ddy, s->ddz, s->lap, s->fbeta, s->kt, s->cnt, s->dens, 1);
+    tdwo_density_tau(c, s->dens);
+    tdwo_potentials(c, 0, s->dens, s->pots);
+    free(slots);
+    free(qpe0);
+    free(al);
+    return 0;
+}
+
 void tdwo_initial_measure(const tdwo_cfg *c, tdwo_state *s)
 {
     tdwo_normalize(c, s->n, s->wf);

@@ -217,6 +217,13 @@ int tdwo_step(const tdwo_cfg *c, tdwo_state *s, int it, int grad_last);
 /* Initial measurement of pca.c:939-975: normalise, derivatives, densities,
  * tau; potentials are left untouched. */
 void tdwo_initial_measure(const tdwo_cfg *c, tdwo_state *s);
+/* Taylor term of the self-start (pca_kernels.cu:1032-1085). */
+void tdwo_taylor(const tdwo_cfg *c, int order, double dt, int n, const double *hpsi, double *upd, double *contr);
+/* Self-start of pca.c:1086-1392 (selfstart 1): K steps of the Taylor-series
+ * midpoint rule, history fkm1..fkmK filled as the reference fills it, then
+ * derivatives, densities, tau and potentials of the new state.  f[0..K-1]
+ * must all be allocated (f[0..2] serve as scratch). */
+int tdwo_selfstart(const tdwo_cfg *c, tdwo_state *s);
 
 /* --- uniform solution (pca_uniform.h) ----------------------------------- */
 typedef struct {

@@ -1,0 +1,126 @@
+"""Taylor self-start (pca.c:1086-1392, pca_kernels.cu:1032-1085) on the GPU.
+
+taylor_expansion_contribution is checked against its closed form on
+plane-wave eigenstates (each term is (-i lam dt)^k / k! psi); tdw_selfstart
+against the oracle's restatement (itself pinned to the exact propagator of a
+linear problem in tests/test_oracle_selfstart.py) at 1e-12, and then through
+ABM steps that continue from the self-started history."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests import tdgauss
+from tests.test_gpu_parity import TOL, rel, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no HIP device")
+    return t
+
+
+def test_taylor_terms_closed_form(torch):
+    """H applied without the <H> subtraction (useqpe = 0) to plane-wave
+    eigenstates of the uniform solution returns E psi; chained with
+    taylor_expansion_contribution for orders 1..5 the k-th contribution is
+    (-i E dt)^k / k! psi and the accumulated update psi sum_k of them."""
+    from wslda_amd import problems, td
+    say("case: taylor_expansion_contribution, plane-wave eigenstates, orders 1..5")
+    s = problems.uniform_start(3, (8, 8, 8), n_states=40)
+    p = s["params"]
+    n = s["wf"].shape[0] // 2
+    ctx = td.TDContext(p, n, 0)
+    ctx.set_states(s["fbeta"])
+    L = td.lib()
+    L.tdw_bind(ctx.h)
+    try:
+        ptr = lambda t: C.c_void_p(t.data_ptr())
+        psi = torch.as_tensor(s["wf"]).to("cuda:0")
+        pots = torch.as_tensor(s["pots"]).to("cuda:0")
+        dens = torch.zeros(16 * ctx.N, dtype=torch.float64, device="cuda:0")
+        zero = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+        lap = torch.zeros_like(psi)
+        hpsi = torch.zeros_like(psi)
+        contr = psi.clone()
+        upd = psi.clone()
+        dt = 0.04
+        En = torch.as_tensor(s["En"]).to("cuda:0")
+        expect_upd = psi.clone()
+        for k in range(1, 6):
+            td.check(L.compute_laplace(2 * n, ptr(contr), ptr(lap), 256), "compute_laplace")
+            td.check(L.apply_hamiltonian(0, n, ptr(contr), ptr(hpsi), None, None, None, ptr(lap), None, ptr(dens),
+                                         ptr(pots), C.c_double(0.0), ptr(zero), C.c_double(1.0), 256), "apply_h")
+            td.check(L.taylor_expansion_contribution(k, C.c_double(dt), n, ptr(hpsi), ptr(upd), ptr(contr), 256),
+                     "taylor")
+            ctx.sync()
+            fac = ((-1j * dt * En) ** k / math.factorial(k))
+            want = psi * torch.cat([fac, fac])[:, None]
+            expect_upd = expect_upd + want
+            assert rel(contr.cpu().numpy(), want.cpu().numpy()) < 1e-11, k
+            assert rel(upd.cpu().numpy(), expect_upd.cpu().numpy()) < 1e-12, k
+    finally:
+        L.tdw_bind(None)
+    ctx.close()
+
+
+def run_pair(torch, cfg, wf, fbeta, kt, cnt, pots, fused, steps=3):
+    from wslda_amd import td
+    P = po.Problem(cfg, wf, fbeta, kt=kt, cnt=cnt, pots=pots)
+    n = P.n
+    ctx = td.TDContext(tdgauss.to_params(cfg), n, 0)
+    ctx.set_states(fbeta, kt, cnt)
+    R = td.TDRun(ctx, wf, pots, fused=fused)
+    P.initial_measure()
+    R.initial_measure()
+    P.selfstart()
+    R.selfstart()
+    k = {34: 3, 45: 4, 55: 5}[cfg.scheme]
+    assert rel(R.wf.cpu().numpy(), P.wf) < 1e-11
+    assert rel(R.dens.cpu().numpy(), P.dens) < TOL
+    assert rel(R.pots.cpu().numpy(), P.pots) < TOL
+    hg, ho = R.history(), P.history()
+    for j in range(k):
+        assert rel(hg[j].cpu().numpy(), ho[j]) < 1e-11, j
+    for it in range(steps):
+        grad = it == steps - 1
+        P.step(it, grad_last=grad)
+        R.step(it, grad_last=grad)
+    R.finalize()
+    ctx.sync()
+    assert rel(R.dens.cpu().numpy(), P.dens) < TOL
+    assert rel(R.wf.cpu().numpy(), P.wf) < 1e-11
+    ctx.close()
+
+
+@pytest.mark.parametrize("dim,scheme", [(1, 45), (1, 34), (2, 55), (3, 55)])
+def test_selfstart_td_gauss_vs_oracle(torch, dim, scheme):
+    """td-gauss (SLDA, V_ext switched on at the start so the states move)
+    on 1D / 2D / 3D lattices: the self-started state, history, densities and
+    potentials, and three ABM steps after it, against the oracle."""
+    say("case: selfstart td-gauss dim=%d scheme=%d" % (dim, scheme))
+    s = tdgauss.setup(dim)
+    cfg = s["cfg"]
+    cfg.scheme = scheme
+    cfg.vext.t_start, cfg.vext.t_switch = -1.0, 0.5
+    run_pair(torch, cfg, s["wf"], s["fbeta"], s["kt"], s["cnt"], s["pots"], fused=(dim == 3))
+
+
+def test_selfstart_aslda_and_filter_vs_oracle(torch):
+    """ASLDA (varying mass: gradients every application, lap(alpha psi)) and
+    the mean-field filter at hkf_mode 2, which every apply_hamiltonian call of
+    the self-start applies again in place, on the 1D td-gauss problem."""
+    say("case: selfstart ASLDA 1D with hkf_mode 2")
+    s = tdgauss.setup(1)
+    cfg = s["cfg"]
+    cfg.functional = po.ASLDA
+    cfg.mf_correction = 0
+    cfg.hkf_mode, cfg.hkf_mu, cfg.hkf_T = 2, 3.0, 0.5
+    cfg.vext.t_start, cfg.vext.t_switch = -1.0, 0.5
+    run_pair(torch, cfg, s["wf"], s["fbeta"], s["kt"], s["cnt"], s["pots"], fused=False)

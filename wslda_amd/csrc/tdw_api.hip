@@ -527,6 +527,108 @@ extern "C" int tdw_initial_measure(tdw_ctx *c, tdw_state *s)
     return half_densities(c, s, 1);
 }
 
+// ---------------------------------------------------------------------------
+// self-start: the Taylor-series midpoint steps of pca.c:1086-1392
+// ---------------------------------------------------------------------------
+// derivatives of an array before H acts on it (pca.c:1119-1126, 1179-1186)
+static int ss_derivs(tdw_ctx *c, tdw_state *s, const double2 *in, bool grad)
+{
+    const int n = s->n;
+    if (grad) return derivatives(c, 2 * n, in, D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap));
+    return laplace(c, 2 * n, in, D2(s->lap));
+}
+// densities and potentials at iteration 0; with a varying mass the filter of
+// alpha (pca.c:1127-1159, 1223-1251)
+static int ss_fields(tdw_ctx *c, tdw_state *s, const double2 *wf, bool grad, bool vm)
+{
+    TDW_TRY(densities(c, s->n, wf, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), c->d_fbeta, nullptr,
+                      c->d_kt, c->d_cnt, s->dens, grad));
+    if (grad) TDW_TRY(density_tau(c, s->dens));
+    TDW_TRY(potentials(c, 0, s->dens, s->pots));
+    if (vm && c->p.hkf_mode >= 1) {
+        const long N = (long)c->N;
+        TDW_TRY(grid_filter_d(c, 2, s->pots + 4 * N, s->pots + 4 * N, c->p.hkf_mu, c->p.hkf_T));
+    }
+    return 0;
+}
+// apply_hamiltonian as the self-start calls it (no friction): V, Delta, A
+// filtered in place inside every call (pca_kernels.cu:733-764, 786-790), then
+// H in - <H> in (useqpe NULL: <H> into qpe)
+static int ss_apply_h(tdw_ctx *c, tdw_state *s, const double2 *in, double2 *out, bool vm, const double *useqpe,
+                      double *qpe)
+{
+    TDW_TRY(hkf_fields(c, s->pots, false));
+    if (vm)
+        return apply_h_aslda(c, s->n, in, out, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), nullptr, s->pots,
+                             c->d_kt, useqpe, qpe);
+    return apply_h(c, 0, s->n, in, out, CD2(s->lap), s->pots, c->d_kt, useqpe, qpe);
+}
+static int copy_state(tdw_ctx *c, int n, const tdw_complex *src, tdw_complex *dst)
+{
+    if (src == dst) return 0;
+    return (int)hipMemcpyAsync(dst, src, sizeof(double2) * 2 * (size_t)n * c->N, hipMemcpyDeviceToDevice, c->stream);
+}
+
+extern "C" int tdw_selfstart(tdw_ctx *c, tdw_state *s, tdw_complex *scratch0, tdw_complex *scratch1)
+{
+    if (!c || !s || s->n > c->nwf || !s->wf || !s->lap || !s->dens || !s->pots) return TDW_EINVAL;
+    if (c->p.precision != TDW_FP64) return TDW_EINVAL;
+    const int K = nhist(c), E = c->p.scheme == TDW_AB3AM4 ? 4 : 5;
+    for (int j = 0; j < K; j++)
+        if (!s->f[j]) return TDW_EINVAL;
+    if ((K <= 4 && !scratch0) || (K == 3 && !scratch1)) return TDW_EINVAL;
+    const bool vm = varying_mass(c), grad = vm;
+    if (grad && (!s->ddx || (c->dim >= 2 && !s->ddy) || (c->dim >= 3 && !s->ddz))) return TDW_EINVAL;
+    TDW_TRY(fused_finalize(c, s));
+    const int n = s->n;
+    const double dt = c->p.dt;
+    double2 *wf = D2(s->wf), *m1 = D2(s->f[0]), *m2 = D2(s->f[1]), *m3 = D2(s->f[2]);
+    // the history entries the reference keeps (see below): F_{K-2} in B1,
+    // F_{K-1} in B0 -- spare history slots where the scheme has them
+    tdw_complex *B1 = K == 5 ? s->f[4] : K == 4 ? s->f[3] : scratch1;
+    tdw_complex *B0 = K == 5 ? s->f[3] : scratch0;
+    double *qpe0 = s->qpe ? s->qpe : c->d_qpe;
+    for (int step = 0; step < K; step++) {
+        // predictor: half a step from psi
+        TDW_TRY(ss_derivs(c, s, wf, grad));
+        TDW_TRY(ss_fields(c, s, wf, grad, vm));
+        TDW_TRY(copy_state(c, n, s->wf, s->f[2]));
+        TDW_TRY(ss_apply_h(c, s, m3, m1, vm, nullptr, qpe0));
+        if (step == K - 2) TDW_TRY(copy_state(c, n, s->f[0], B1));
+        if (step == K - 1) TDW_TRY(copy_state(c, n, s->f[0], B0));
+        TDW_TRY(taylor(c, 1, 0.5 * dt, n, m1, m3, m2));
+        for (int k = 1; k < E; k++) {
+            TDW_TRY(ss_derivs(c, s, m2, grad));
+            TDW_TRY(ss_apply_h(c, s, m2, m1, vm, qpe0, qpe0));
+            TDW_TRY(taylor(c, k + 1, 0.5 * dt, n, m1, m3, m2));
+        }
+        TDW_TRY(normalize(c, n, m3, c->d_nrm));
+        // corrector: the whole step from psi with the midpoint's potentials
+        TDW_TRY(ss_derivs(c, s, m3, grad));
+        TDW_TRY(ss_fields(c, s, m3, grad, vm));
+        TDW_TRY(ss_derivs(c, s, wf, grad));
+        TDW_TRY(ss_apply_h(c, s, wf, m1, vm, nullptr, qpe0));
+        TDW_TRY(taylor(c, 1, dt, n, m1, wf, m2));
+        for (int k = 1; k < E; k++) {
+            TDW_TRY(ss_derivs(c, s, m2, grad));
+            TDW_TRY(ss_apply_h(c, s, m2, m1, vm, qpe0, qpe0));
+            TDW_TRY(taylor(c, k + 1, dt, n, m1, wf, m2));
+        }
+        TDW_TRY(normalize(c, n, wf, c->d_nrm));
+    }
+    // history as the reference leaves it: its host buffer shifts slot j to
+    // slot j + 1 for increasing j (pca.c:1166), so fkm1 = F_{K-1} and every
+    // other slot holds F_{K-2}
+    TDW_TRY(copy_state(c, n, B0, s->f[0]));
+    for (int j = 1; j < K; j++) TDW_TRY(copy_state(c, n, B1, s->f[j]));
+    // the new state's measurement (pca.c:1336-1356)
+    TDW_TRY(derivatives(c, 2 * n, wf, D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap)));
+    TDW_TRY(densities(c, n, wf, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), c->d_fbeta, nullptr, c->d_kt,
+                      c->d_cnt, s->dens, 1));
+    TDW_TRY(density_tau(c, s->dens));
+    return potentials(c, 0, s->dens, s->pots);
+}
+
 extern "C" int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 {
     if (!c || !s || s->n > c->nwf) return TDW_EINVAL;

@@ -173,6 +173,7 @@ def _sigs():
         "tdw_finalize": (I, [P, C.POINTER(State)]),
         "tdw_fused_supported": (I, [P]),
         "tdw_initial_measure": (I, [P, C.POINTER(State)]),
+        "tdw_selfstart": (I, [P, C.POINTER(State), P, P]),
         "tdw_adj_forward_step": (I, [P, C.POINTER(State), I, P, P, C.POINTER(TapeStep)]),
         "tdw_adj_forward_step_to": (I, [P, C.POINTER(State), I, P, P, C.POINTER(TapeStep)]),
         "tdw_checkpoint_nwf": (I, [C.c_char_p, C.POINTER(C.c_int)]),
@@ -558,6 +559,20 @@ class TDRun:
 
     def initial_measure(self):
         check(lib().tdw_initial_measure(self.ctx.h, C.byref(self.st)), "tdw_initial_measure")
+
+    def selfstart(self):
+        """Taylor self-start of the history (selfstart 1, pca.c:1086-1392):
+        the state advances by 3 / 4 / 5 steps, the history is filled and the
+        densities and potentials belong to the new state.  AB4AM5 / AB3AM4
+        need one / two state-sized scratch arrays, allocated here for the
+        call."""
+        torch = self.ctx.torch
+        k = {AB3AM4: 3, AB4AM5: 4, AB5AM5: 5}[self.ctx.params.scheme]
+        scr = [torch.empty_like(self.wf) for _ in range(max(0, 5 - k))]
+        ptrs = [t.data_ptr() for t in scr] + [None, None]
+        check(lib().tdw_selfstart(self.ctx.h, C.byref(self.st), ptrs[0], ptrs[1]), "tdw_selfstart")
+        self.ctx.sync()
+        del scr
 
     def step(self, it, grad_last=False):
         fn = lib().tdw_step_fused if self.fused else lib().tdw_step
