@@ -153,6 +153,8 @@ def _sigs():
         "tdw_sync": (I, [P]),
         "tdw_npts": (Z, [P]),
         "tdw_set_params": (I, [P, C.POINTER(Params)]),
+        "tdw_set_user_params": (I, [P, P]),
+        "tdw_get_user_params": (I, [P, P]),
         "tdw_set_states": (I, [P, P, P, P]),
         "tdw_abm_step1": (I, [P, I, P, P, P, P, P, P]),
         "tdw_abm_step4": (I, [P, I, P, P, P]),
