@@ -304,10 +304,11 @@ def summarize(g, args, mode):
 
 
 def density_errors(d, ref, N, currents=True):
-    """Relative error per density field: max-abs difference over the
-    field's own max-abs value (nu as a complex field, j = (j_x, j_y, j_z) as
-    one field).  currents=False leaves j out (a uniform state, whose currents
-    are zero up to noise)."""
+    """Relative error per density field of the gate (rho_a, rho_b, |nu|,
+    tau_a, tau_b, j): max-abs difference over the field's own max-abs value
+    (nu as a complex field; j = j_a + j_b, the total current, as one vector
+    field).  currents=False leaves j out (a uniform state, whose currents are
+    zero up to noise)."""
     import numpy as np
     out = {}
     nu, nur = d[0:2 * N:2] + 1j * d[1:2 * N:2], ref[0:2 * N:2] + 1j * ref[1:2 * N:2]
@@ -316,9 +317,19 @@ def density_errors(d, ref, N, currents=True):
         r = ref[i * N:(i + 1) * N]
         out[k] = float(np.abs(d[i * N:(i + 1) * N] - r).max() / np.abs(r).max())
     if currents:
-        for k, i0 in (("j_a", 4), ("j_b", 9)):
-            r = ref[i0 * N:(i0 + 3) * N]
-            out[k] = float(np.abs(d[i0 * N:(i0 + 3) * N] - r).max() / np.abs(r).max())
+        j, jr = d[4 * N:7 * N] + d[9 * N:12 * N], ref[4 * N:7 * N] + ref[9 * N:12 * N]
+        out["j"] = float(np.abs(j - jr).max() / np.abs(jr).max())
+    return out
+
+
+def spin_current_errors(d, ref, N):
+    """j_a and j_b each against its own maximum (reported beside the gate)."""
+    import numpy as np
+    out = {}
+    for k, i0 in (("j_a", 4), ("j_b", 9)):
+        r = ref[i0 * N:(i0 + 3) * N]
+        out[k] = float(np.abs(d[i0 * N:(i0 + 3) * N] - r).max() / np.abs(r).max())
+        out["|%s|_inf" % k] = float(np.abs(r).max())
     return out
 
 
@@ -452,7 +463,8 @@ def trap_pair(args):
         "split_value": sp["qps"], "fp64_value": fp["qps"], "unit": "qp-steps/s",
         "split_ms_per_step": sp["ms"], "fp64_ms_per_step": fp["ms"], "speedup_vs_fp64": sp["qps"] / fp["qps"],
         "density_rel_err_vs_fp64": err, "max_density_rel_err": max(err.values()),
-        "j_inf_fp64": float(abs(fp["dens"][4 * N:7 * N]).max()),
+        "current_per_spin_rel_err": spin_current_errors(sp["dens"], fp["dens"], N),
+        "j_inf_fp64": float(abs(fp["dens"][4 * N:7 * N] + fp["dens"][9 * N:12 * N]).max()),
         "energy_rel_diff_vs_fp64": abs(Esp - E64) / abs(E64),
         "energy_drift_fp64": drift(fp), "energy_drift_split": drift(sp), "drift_steps": steps,
         "split_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(sp["rep"].items(), key=lambda kv: -kv[1][0])},

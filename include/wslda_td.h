@@ -307,10 +307,10 @@ int tdw_st3d_load(const tdw_params *p, const char *prefix, int iogroups, const i
  *                         the device densities / potentials at step it, then
  *                         the cycle count and the .wtxt rewrite
  *                         (pca.c:1623-1630).  V is written as V - V_ext(it)
- *                         (except at it == 0 for inittype 1..3), delta_ext as
- *                         delta and velocity_ext as zero (the built-in V_ext
- *                         has no pairing or velocity field); writeprec 'f'
- *                         stores FP32 casts. */
+ *                         (except at it == 0 for inittype 1..3), delta_ext
+ *                         and velocity_ext from the problem's callbacks (zero
+ *                         for the built-in problem, which has neither);
+ *                         writeprec 'f' stores FP32 casts. */
 typedef struct {
     const char *outprefix;   /* md.outprefix                                    */
     const char *writevar;    /* writevar keys, separated by spaces or commas   */
@@ -414,6 +414,34 @@ void *tdw_stream(tdw_ctx *c);
  * the counters; returns the number of entries. */
 int tdw_timing(tdw_ctx *c, int enable);
 int tdw_timing_report(tdw_ctx *c, int max, char *names, double *total_ms, int *count);
+
+/* ---------------------------------------------------------------------------
+ * Project builds.  `python -m wslda_amd.build --project DIR --codedim D`
+ * compiles a problem's predefines.h and problem-definition.h (DIR) into the
+ * library, as the reference compiles them into its kernel unit
+ * (pca_kernels.cu:28-42): its v_ext, delta_ext, velocity_ext and
+ * scattering_length callbacks, and with ENABLE_MODIFY_* its modify_potentials,
+ * modify_energies and modify_densities, then drive the engine in place of
+ * the built-in V_ext (params.vext is ignored).  The lattice and functional
+ * are compiled in: tdw_create refuses others with TDW_EINVAL, and an
+ * external velocity needs a varying-mass functional (ASLDA, SLDAE), whose
+ * step carries the current terms it enters (TDW_EINVAL otherwise).  The
+ * problem's user parameters and extra data reach its callbacks as params /
+ * dc_params and extra_data (tdw_set_user_params / memcopy_const_params,
+ * memcopy_extra_data).  The project library also exports the problem's own
+ * host functions (include/wslda_td_project.h).
+ * ------------------------------------------------------------------------- */
+#define TDW_PROJ_V_EXT 0x01u             /* ENABLE_V_EXT               */
+#define TDW_PROJ_DELTA_EXT 0x02u         /* ENABLE_DELTA_EXT           */
+#define TDW_PROJ_VELOCITY_EXT 0x04u      /* ENABLE_VELOCITY_EXT        */
+#define TDW_PROJ_MODIFY_POTENTIALS 0x08u /* ENABLE_MODIFY_POTENTIALS   */
+#define TDW_PROJ_MODIFY_ENERGIES 0x10u   /* ENABLE_MODIFY_ENERGIES     */
+#define TDW_PROJ_MODIFY_DENSITIES 0x20u  /* ENABLE_MODIFY_DENSITIES    */
+/* 1 and the compiled settings (lattice, functional, integration and
+ * regularisation schemes, stabilisation densities, DENSEPSILON, spin
+ * symmetry, SLDA_FORCE_A1) written into *p, the TDW_PROJ_* switches into
+ * *flags (either may be NULL) in a project library; 0 in the plain one. */
+int tdw_project_info(tdw_params *p, unsigned *flags);
 
 /* ---------------------------------------------------------------------------
  * Drop-in layer: the reference's entry points, acting on the bound context.

@@ -28,9 +28,13 @@ def torch():
 
 def test_taylor_terms_closed_form(torch):
     """H applied without the <H> subtraction (useqpe = 0) to plane-wave
-    eigenstates of the uniform solution returns E psi; chained with
+    eigenstates of the uniform solution returns lam psi; chained with
     taylor_expansion_contribution for orders 1..5 the k-th contribution is
-    (-i E dt)^k / k! psi and the accumulated update psi sum_k of them."""
+    (-i lam dt)^k / k! psi and the accumulated update psi sum_k of them.
+    lam is each state's Rayleigh quotient <psi|H psi> / <psi|psi> from the
+    first H application (the solver's E_n differs from it by the solution's
+    convergence tolerance, which (E dt)^k amplifies k-fold relative to the
+    small terms); it equals E_n to 1e-9."""
     from wslda_amd import problems, td
     say("case: taylor_expansion_contribution, plane-wave eigenstates, orders 1..5")
     s = problems.uniform_start(3, (8, 8, 8), n_states=40)
@@ -60,7 +64,17 @@ def test_taylor_terms_closed_form(torch):
             td.check(L.taylor_expansion_contribution(k, C.c_double(dt), n, ptr(hpsi), ptr(upd), ptr(contr), 256),
                      "taylor")
             ctx.sync()
-            fac = ((-1j * dt * En) ** k / math.factorial(k))
+            if k == 1:
+                # per state (u row a and v row n + a together)
+                num = (psi[:n].conj() * hpsi[:n]).sum(dim=1).real + (psi[n:].conj() * hpsi[n:]).sum(dim=1).real
+                den = (psi[:n].abs() ** 2).sum(dim=1) + (psi[n:].abs() ** 2).sum(dim=1)
+                lam = num / den
+                say("  max |lam - E_n| = %.3e" % (lam - En).abs().max().item())
+                assert (lam - En).abs().max().item() < 1e-9
+                # the states are eigenstates: H psi = lam psi to round-off
+                resid = (hpsi - psi * torch.cat([lam, lam])[:, None]).abs().max().item()
+                assert resid < 1e-10 * hpsi.abs().max().item(), resid
+            fac = ((-1j * dt * lam) ** k / math.factorial(k))
             want = psi * torch.cat([fac, fac])[:, None]
             expect_upd = expect_upd + want
             assert rel(contr.cpu().numpy(), want.cpu().numpy()) < 1e-11, k

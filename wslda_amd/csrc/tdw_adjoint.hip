@@ -641,7 +641,10 @@ static int adj_supported(const tdw_ctx *c)
     // the backward step differentiates the SLDA step as written: no
     // spin-symmetric folding and no mean-field filter between the functional
     // and H
-    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64 && !c->p.spinsymmetry && c->p.hkf_mode < 1;
+    // and no problem callbacks of a project build (the gradient is taken
+    // with respect to the built-in V_ext parameters)
+    return c->p.functional == TDW_SLDA && c->p.precision == TDW_FP64 && !c->p.spinsymmetry && c->p.hkf_mode < 1 &&
+           !c->proj;
 }
 
 }  // namespace tdw

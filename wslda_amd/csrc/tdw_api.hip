@@ -201,6 +201,7 @@ extern "C" int tdw_create(const tdw_params *p, int nwf, int device, tdw_ctx **ou
         e = (int)hipMemcpy(c->d_cnt, ones, sizeof(int) * nwf, hipMemcpyHostToDevice);
         free(ones);
     }
+    if (!e) e = project_attach(c);
     if (e) {
         tdw_destroy(c);
         return e;
@@ -223,6 +224,7 @@ extern "C" int tdw_destroy(tdw_ctx *c)
                     c->d_gscr};
     for (void *b : bufs)
         if (b) hipFree(b);
+    project_release(c);
     fused_release(c);
     if (c->timers) {
         for (auto &r : *c->timers) { hipEventDestroy(r.a); hipEventDestroy(r.b); }
@@ -295,8 +297,9 @@ extern "C" int tdw_set_params(tdw_ctx *c, const tdw_params *p)
     if (p->dim != c->p.dim || p->nx != c->p.nx || (c->dim >= 2 && p->ny != c->p.ny) ||
         (c->dim >= 3 && p->nz != c->p.nz))
         return TDW_EINVAL;
+    if (c->proj && p->functional != c->p.functional) return TDW_EINVAL;
     c->p = *p;
-    return 0;
+    return project_consts(c);
 }
 
 extern "C" int tdw_set_states(tdw_ctx *c, const double *h_fbeta, const double *h_kt, const int *h_cnt)
@@ -373,7 +376,9 @@ extern "C" int tdw_apply_h(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw
     if (!c || n > c->nwf || in == out) return TDW_EINVAL;
     // ASLDA's H needs the gradients and lap(alpha psi): tdw_apply_h_full
     if (varying_mass(c)) return TDW_EINVAL;
-    return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
+    const double *hp = h_view(c, it, pots);
+    if (!hp) return TDW_EINVAL;
+    return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), hp, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
 }
 extern "C" int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_complex *out,
                                 const tdw_complex *ddx, const tdw_complex *ddy, const tdw_complex *ddz,
@@ -381,11 +386,13 @@ extern "C" int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in
                                 const double *useqpe, double *qpe)
 {
     if (!c || n > c->nwf || in == out) return TDW_EINVAL;
+    const double *hp = h_view(c, it, pots);
+    if (!hp) return TDW_EINVAL;
     if (!varying_mass(c))
-        return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
+        return apply_h(c, it, n, CD2(in), D2(out), CD2(lap), hp, c->d_kt, useqpe, qpe ? qpe : c->d_qpe);
     if (!ddx || (c->dim >= 2 && !ddy) || (c->dim >= 3 && !ddz) || out == ddx || out == ddy || out == ddz)
         return TDW_EINVAL;
-    return apply_h_aslda(c, n, CD2(in), D2(out), CD2(ddx), CD2(ddy), CD2(ddz), CD2(lap), CD2(alap), pots, c->d_kt,
+    return apply_h_aslda(c, n, CD2(in), D2(out), CD2(ddx), CD2(ddy), CD2(ddz), CD2(lap), CD2(alap), hp, c->d_kt,
                          useqpe, qpe ? qpe : c->d_qpe);
 }
 extern "C" int tdw_alpha_wf(tdw_ctx *c, int n, const tdw_complex *in, const double *pots, tdw_complex *out)
@@ -397,7 +404,7 @@ extern "C" int tdw_set_user_params(tdw_ctx *c, const double *h_params)
 {
     if (!c || !h_params) return TDW_EINVAL;
     for (int k = 0; k < TDW_MAX_USER_PARAMS; k++) c->user_params[k] = h_params[k];
-    return 0;
+    return project_consts(c);
 }
 extern "C" int tdw_get_user_params(const tdw_ctx *c, double *h_params)
 {
@@ -459,24 +466,28 @@ static int step_aslda(tdw_ctx *c, tdw_state *s, int it, int k, double *qpe)
     TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(aslda_densities(c, s));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
     // two-pass lattices fold the predictor's <H> subtraction into the corrector
     const bool fold = aslda_h_fused_supported(c);
+    const double *hp = h_view(c, it + 1, s->pots);
+    if (!hp) return TDW_EINVAL;
     TDW_TRY(apply_h_aslda(c, n, D2(s->wf), D2(s->lap), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
-                          s->pots, c->d_kt, nullptr, qpe, !fold));
+                          hp, c->d_kt, nullptr, qpe, !fold));
     // corrector
     if (fold) TDW_TRY(abm4q(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1]), qpe));
     else TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(aslda_densities(c, s));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
     tdw_complex *tmp = s->f[k - 1];
     for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
+    hp = h_view(c, it + 1, s->pots);
+    if (!hp) return TDW_EINVAL;
     return apply_h_aslda(c, n, D2(s->wf), D2(s->f[0]), D2(s->ddx), D2(s->ddy), D2(s->ddz), D2(s->lap), nullptr,
-                         s->pots, c->d_kt, nullptr, qpe);
+                         hp, c->d_kt, nullptr, qpe);
 }
 
 extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
@@ -500,22 +511,26 @@ extern "C" int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     TDW_TRY(abm1(c, n, D2(s->wf), D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]), D2(s->f[4])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     TDW_TRY(half_densities(c, s, 0, plane));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
-    if (plane) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->lap), s->pots, c->d_kt, nullptr, qpe));
-    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    const double *hp = h_view(c, it + 1, s->pots);
+    if (!hp) return TDW_EINVAL;
+    if (plane) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->lap), hp, c->d_kt, nullptr, qpe));
+    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), hp, c->d_kt, nullptr, qpe));
     // corrector
     TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), CD2(s->f[k - 1])));
     TDW_TRY(normalize(c, n, D2(s->wf), c->d_nrm));
     const bool plane_c = plane && !grad_last;
     TDW_TRY(half_densities(c, s, grad_last, plane_c));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
     tdw_complex *tmp = s->f[k - 1];
     for (int j = k - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
-    if (plane_c) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->f[0]), s->pots, c->d_kt, nullptr, qpe));
-    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, qpe));
+    hp = h_view(c, it + 1, s->pots);
+    if (!hp) return TDW_EINVAL;
+    if (plane_c) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->f[0]), hp, c->d_kt, nullptr, qpe));
+    else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), hp, c->d_kt, nullptr, qpe));
     return 0;
 }
 
@@ -544,7 +559,7 @@ static int ss_fields(tdw_ctx *c, tdw_state *s, const double2 *wf, bool grad, boo
     TDW_TRY(densities(c, s->n, wf, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), c->d_fbeta, nullptr,
                       c->d_kt, c->d_cnt, s->dens, grad));
     if (grad) TDW_TRY(density_tau(c, s->dens));
-    TDW_TRY(potentials(c, 0, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, 0, s->dens, s->pots));
     if (vm && c->p.hkf_mode >= 1) {
         const long N = (long)c->N;
         TDW_TRY(grid_filter_d(c, 2, s->pots + 4 * N, s->pots + 4 * N, c->p.hkf_mu, c->p.hkf_T));
@@ -558,10 +573,12 @@ static int ss_apply_h(tdw_ctx *c, tdw_state *s, const double2 *in, double2 *out,
                       double *qpe)
 {
     TDW_TRY(hkf_fields(c, s->pots, false));
+    const double *hp = h_view(c, 0, s->pots);
+    if (!hp) return TDW_EINVAL;
     if (vm)
-        return apply_h_aslda(c, s->n, in, out, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), nullptr, s->pots,
+        return apply_h_aslda(c, s->n, in, out, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), nullptr, hp,
                              c->d_kt, useqpe, qpe);
-    return apply_h(c, 0, s->n, in, out, CD2(s->lap), s->pots, c->d_kt, useqpe, qpe);
+    return apply_h(c, 0, s->n, in, out, CD2(s->lap), hp, c->d_kt, useqpe, qpe);
 }
 static int copy_state(tdw_ctx *c, int n, const tdw_complex *src, tdw_complex *dst)
 {
@@ -626,7 +643,7 @@ extern "C" int tdw_selfstart(tdw_ctx *c, tdw_state *s, tdw_complex *scratch0, td
     TDW_TRY(densities(c, n, wf, CD2(s->ddx), CD2(s->ddy), CD2(s->ddz), CD2(s->lap), c->d_fbeta, nullptr, c->d_kt,
                       c->d_cnt, s->dens, 1));
     TDW_TRY(density_tau(c, s->dens));
-    return potentials(c, 0, s->dens, s->pots);
+    return step_potentials(c, 0, s->dens, s->pots);
 }
 
 extern "C" int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last)
@@ -688,7 +705,7 @@ extern "C" int memcopy_const(double mu_a, double mu_b, double ec, double t0, dou
     c->p.ec = ec;
     c->p.t0 = t0;
     c->p.dt = dt;
-    return 0;
+    return project_consts(c);
 }
 // the user parameter array (MAX_USER_PARAMS doubles) of the problem's
 // callbacks (wslda_cuda_utils.h:80-90): kept on the bound context and, in a
@@ -702,7 +719,7 @@ extern "C" int memcopy_const_sclgth(double a)
 {
     BOUND();
     c->p.sclgth = a;
-    return 0;
+    return project_consts(c);
 }
 
 extern "C" int create_cufftPlans(int batch_size, int nwfip, size_t *workSize)
@@ -879,7 +896,7 @@ extern "C" int memcopy_extra_data(size_t extra_data_size, void *extra_data)
     BOUND();
     c->extra_data = extra_data;
     c->extra_size = extra_data_size;
-    return 0;
+    return project_consts(c);
 }
 // workspace views of the work area given to set_workspace_for_cufftPlan
 // (tdwslda_memory_management.cu:31-94): [kernels 24 n doubles]
@@ -956,12 +973,16 @@ static int apply_h_ref(tdw_ctx *c, int it, int n, tdw_complex *in, tdw_complex *
     // mean-field filtering, in place, after the friction terms
     // (pca_kernels.cu:733-764, 786-790); alpha is the driver's to filter
     TDW_TRY(hkf_fields(c, pots, false));
+    // Delta_ext and the external velocity act inside this H application only
+    // (pca_kernels.cu:274-275, 315-327)
+    const double *hp = h_view(c, it, pots);
+    if (!hp) return TDW_EINVAL;
     if (varying_mass(c)) {
         if (!dx || (c->dim >= 2 && !dy) || (c->dim >= 3 && !dz) || !alap || in == out) return TDW_EINVAL;
-        TDW_TRY(apply_h_aslda(c, n, CD2(in), D2(out), CD2(dx), CD2(dy), CD2(dz), CD2(lap), CD2(alap), pots, kt,
+        TDW_TRY(apply_h_aslda(c, n, CD2(in), D2(out), CD2(dx), CD2(dy), CD2(dz), CD2(lap), CD2(alap), hp, kt,
                               useqpe, c->d_qpe));
     } else {
-        TDW_TRY(apply_h(c, it, n, CD2(in), D2(out), CD2(lap), pots, kt, useqpe, c->d_qpe));
+        TDW_TRY(apply_h(c, it, n, CD2(in), D2(out), CD2(lap), hp, kt, useqpe, c->d_qpe));
     }
     if (qf) TDW_TRY(qf_remove(c, qf, pots));
     // <H> is left at the head of the work area (pca_kernels.cu:812-822)

@@ -26,6 +26,7 @@ struct FParams {
     double R0, R1;
     int cubic, mf, bdg, aslda, sldae, a1;
     double sclgth;
+    const double *agrid;   // project build: scattering_length() per point (N), or null
 };
 
 __device__ __forceinline__ double p_reg(const FParams &f, double n)

@@ -611,9 +611,11 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     else
         TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
                           s->dens, 0));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
-    TDW_TRY(hyz<SP>(c, n, yp, T, H, s->pots, c->d_sp, c->d_ep, nullptr));
+    const double *hp = h_view(c, it + 1, s->pots);
+    if (!hp) return TDW_EINVAL;
+    TDW_TRY(hyz<SP>(c, n, yp, T, H, hp, c->d_sp, c->d_ep, nullptr));
 
     // ---- corrector ----
     const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
@@ -632,12 +634,14 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
         else
             TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
                               s->dens, 0));
-        TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
-    TDW_TRY(hkf_fields(c, s->pots, true));
+        TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
+        TDW_TRY(hkf_fields(c, s->pots, true));
+        hp = h_view(c, it + 1, s->pots);
+        if (!hp) return TDW_EINVAL;
         tdw_complex *tmp = s->f[h - 1];
         for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
         s->f[0] = tmp;
-        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], s->pots, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
+        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], hp, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
         c->pending = 1;
         return 0;
     }
@@ -648,21 +652,23 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     TDW_TRY(densities(c, n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap,
                       c->d_fbeta, nullptr, c->d_kt, c->d_cnt, s->dens, 1));
     TDW_TRY(density_tau(c, s->dens));
-    TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
+    TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
+    hp = h_view(c, it + 1, s->pots);
+    if (!hp) return TDW_EINVAL;
     tdw_complex *tmp = s->f[h - 1];
     for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
     if constexpr (SP) {
         // exact H psi - <H> psi into ddx (free after the densities), then FP32 history
-        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->ddx, (double2 *)s->lap, s->pots, c->d_kt, nullptr, qpe));
+        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->ddx, (double2 *)s->lap, hp, c->d_kt, nullptr, qpe));
         const long tot = 2L * n * N;
         TDW_TIMED(c, "f_to32");
         hipLaunchKernelGGL(f_to32, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, (const double2 *)s->ddx,
                            (float2 *)s->f[0], tot);
         TDW_HIP(hipGetLastError());
     } else {
-        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, s->pots, c->d_kt, nullptr, qpe));
+        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, hp, c->d_kt, nullptr, qpe));
     }
     return 0;
 }

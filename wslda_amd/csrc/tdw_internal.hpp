@@ -10,6 +10,8 @@
 
 #define TDW_MAXST 8
 
+struct tdw_project;   // a project build's callback table (tdw_hooks.hpp)
+
 struct tdw_timer_rec {
     const char *name;
     hipEvent_t a, b;
@@ -69,6 +71,16 @@ struct tdw_ctx {
     size_t extra_size;
     // memcopy_const_params(): the problem's user parameters
     double user_params[TDW_MAX_USER_PARAMS];
+    // project build: the problem's callbacks and the fields they fill
+    // (tdw_hooks.hip; null without a project)
+    const tdw_project *proj;
+    double *d_uext;                      // 2N V_ext of spins a, b
+    double *d_agrid;                     // N scattering length
+    double *d_potsH;                     // 12N potentials as H reads them
+    double2 *d_dext;                     // N Delta_ext
+    double *d_vel;                       // 6N velocity of spins a, b
+    double *d_epp;                       // 7N per-point energies (modify_energies)
+    double *h_densx;                     // 12N pinned host densities (modify_densities)
     // per-kernel timing
     int timing;
     std::vector<tdw_timer_rec> *timers;  // pending records

@@ -7,6 +7,7 @@
 #include "tdw_functional.hpp"
 #include "tdw_internal.hpp"
 #include "tdw_physics.hpp"
+#include "tdw_hooks.hpp"
 
 namespace tdw {
 
@@ -584,6 +585,15 @@ VextDev make_vext(const tdw_ctx *c, int it)
     V.s = c->p.vext.switch_on ? smooth_step(t, c->p.vext.t_start, c->p.vext.t_stop, c->p.vext.t_switch, 1.0) : 1.0;
     V.dim = c->dim; V.nx = c->nx; V.ny = c->dim >= 2 ? c->ny : 1; V.nz = c->dim >= 3 ? c->nz : 1;
     V.dx = c->p.dx; V.dy = c->p.dy; V.dz = c->p.dz;
+    V.grid = nullptr;
+    V.npts = (long)c->N;
+    if (c->proj) {
+        // the problem's v_ext replaces the built-in family; without
+        // ENABLE_V_EXT the external potential is zero (pca_kernels.cu:32-36)
+        V.v.ngauss = 0;
+        V.v.omega = 0.0;
+        V.grid = vext_grid(c, it);
+    }
     return V;
 }
 
@@ -621,11 +631,12 @@ FParams fparams(const tdw_ctx *c)
     f.sldae = c->p.functional == TDW_SLDAE;
     f.a1 = c->p.force_a1 != 0;
     f.sclgth = c->p.sclgth;
+    f.agrid = nullptr;
     return f;
 }
 
 __global__ void __launch_bounds__(NT)
-k_potentials(FParams f, VextDev V, long N, int dim, double cc, const double *__restrict__ dens,
+k_potentials(FParams f0, VextDev V, long N, int dim, double cc, const double *__restrict__ dens,
              double *__restrict__ pots)
 {
     long i = (long)blockIdx.x * NT + threadIdx.x;
@@ -634,6 +645,8 @@ k_potentials(FParams f, VextDev V, long N, int dim, double cc, const double *__r
     const double2 nu = make_double2(dens[2 * i], dens[2 * i + 1]);
     double ua, ub;
     vext_point(V, i, ua, ub);
+    FParams f = f0;
+    if (f0.agrid) f.sclgth = f0.agrid[i];
     double Va = pots[i], Vb = pots[N + i];
     double2 d;
     if (f.aslda || f.sldae) {
@@ -674,9 +687,12 @@ int potentials(tdw_ctx *c, int it, const double *dens, double *pots, double ccco
 {
     TDW_TIMED(c, "potentials");
     long N = (long)c->N;
-    hipLaunchKernelGGL(k_potentials, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, fparams(c),
-                       make_vext(c, it), N, c->dim, cccoeff, dens, pots);
-    return (int)hipGetLastError();
+    FParams f = fparams(c);
+    f.agrid = sclgth_grid(c, it);
+    hipLaunchKernelGGL(k_potentials, dim3((N + NT - 1) / NT), dim3(NT), 0, c->stream, f, make_vext(c, it), N,
+                       c->dim, cccoeff, dens, pots);
+    TDW_HIP(hipGetLastError());
+    return project_modify_potentials(c, it, dens, pots);
 }
 
 // ---------------------------------------------------------------------------
@@ -993,8 +1009,9 @@ int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const doub
 // energy
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT)
-k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restrict__ dens,
-         const double *__restrict__ pots, double *__restrict__ part)
+k_energy(FParams f0, VextDev V, long N, int dim, double VE, const double *__restrict__ dens,
+         const double *__restrict__ pots, const double2 *__restrict__ dext, const double *__restrict__ vel,
+         double *__restrict__ pp, double *__restrict__ part)
 {
     __shared__ double red[NT / 64];
     const long i = (long)blockIdx.x * NT + threadIdx.x;
@@ -1002,6 +1019,8 @@ k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restr
 #pragma unroll
     for (int k = 0; k < TDW_NITEMS; k++) e[k] = 0.0;
     if (i < N) {
+        FParams f = f0;
+        if (f0.agrid) f.sclgth = f0.agrid[i];
         const double na = dens[2 * N + i], nb = dens[7 * N + i];
         const double tx1 = dens[4 * N + i], tx2 = dens[9 * N + i];
         const double ty1 = dim >= 2 ? dens[5 * N + i] : 0.0, ty2 = dim >= 2 ? dens[10 * N + i] : 0.0;
@@ -1034,6 +1053,22 @@ k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restr
         double ua, ub;
         vext_point(V, i, ua, ub);
         e[TDW_EPOTEXT] = (na * ua + nb * ub) * VE;
+        // problem callbacks: Re(conj(nu) Delta_ext) (-2) and -(j_a.v_a + j_b.v_b)
+        // (pca_kernels.cu:159, 180-185)
+        if (dext) e[TDW_EPAIREXT] = (nu.x * dext[i].x + nu.y * dext[i].y) * (-2.0) * VE;
+        if (vel)
+            e[TDW_EVELEXT] = (tx1 * vel[i] + ty1 * vel[N + i] + tz1 * vel[2 * N + i] + tx2 * vel[3 * N + i] +
+                              ty2 * vel[4 * N + i] + tz2 * vel[5 * N + i]) *
+                             (-1.0) * VE;
+        if (pp) {
+            // modify_energies acts on the per-point contributions; their sums
+            // are formed after it (k_pp_sum)
+#pragma unroll
+            for (int k = 0; k < TDW_EVELEXT + 1; k++) {
+                pp[(long)k * N + i] = e[k];
+                e[k] = 0.0;
+            }
+        }
         e[TDW_NPARTA] = na;
         e[TDW_NPARTB] = nb;
         if (dim >= 2) {
@@ -1051,14 +1086,38 @@ k_energy(FParams f, VextDev V, long N, int dim, double VE, const double *__restr
     }
 }
 
+// block sums of the per-point energy rows [EKIN .. EVELEXT] into the rows of
+// part that k_energy left zero
+__global__ void __launch_bounds__(NT) k_pp_sum(long N, const double *__restrict__ pp, double *__restrict__ part)
+{
+    __shared__ double red[NT / 64];
+    const long i = (long)blockIdx.x * NT + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < TDW_EVELEXT + 1; k++) {
+        double s = block_sum(i < N ? pp[(long)k * N + i] : 0.0, red);
+        if (threadIdx.x == 0) part[(long)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
 int energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e)
 {
     const long N = (long)c->N;
     const int nb = (int)((N + NT - 1) / NT);
     TDW_TRY(ensure_part(c, (size_t)TDW_NITEMS * nb));
     double VE = dvol_energy(c);
-    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(NT), 0, c->stream, fparams(c), make_vext(c, it), N, c->dim, VE,
-                       dens, pots, c->d_part);
+    FParams f = fparams(c);
+    f.agrid = sclgth_grid(c, it);
+    const double2 *dext = dext_field(c, it, pots);
+    const double *vel = vel_field(c, it);
+    const bool modify = c->proj && c->proj->modify_energies;
+    double *pp = modify ? c->d_epp : nullptr;
+    hipLaunchKernelGGL(k_energy, dim3(nb), dim3(NT), 0, c->stream, f, make_vext(c, it), N, c->dim, VE, dens, pots,
+                       dext, vel, pp, c->d_part);
+    if (modify) {
+        TDW_HIP(hipGetLastError());
+        TDW_TRY(c->proj->modify_energies(c->stream, it, (double *)dens, (double *)pots, pp));
+        hipLaunchKernelGGL(k_pp_sum, dim3(nb), dim3(NT), 0, c->stream, N, pp, c->d_part);
+    }
     hipLaunchKernelGGL(k_rowsum, dim3(1), dim3(64), 0, c->stream, c->d_part, TDW_NITEMS, nb, 1.0, c->d_esum);
     TDW_HIP(hipGetLastError());
     double e[TDW_NITEMS];

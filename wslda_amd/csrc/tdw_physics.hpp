@@ -14,6 +14,8 @@ struct VextDev {
     double s;          // switch value at this time
     int dim, nx, ny, nz;
     double dx, dy, dz;
+    const double *grid;   // project build: V_ext of spin a, b (2 N); overrides the above
+    long npts;
 };
 
 __device__ __forceinline__ void decode(int dim, int ny, int nz, long i, int &ix, int &iy, int &iz)
@@ -30,6 +32,11 @@ __device__ __forceinline__ void decode(int dim, int ny, int nz, long i, int &ix,
 
 __device__ __forceinline__ void vext_point(const VextDev &V, long i, double &va, double &vb)
 {
+    if (V.grid) {
+        va = V.grid[i];
+        vb = V.grid[V.npts + i];
+        return;
+    }
     int ix, iy, iz;
     decode(V.dim, V.ny, V.nz, i, ix, iy, iz);
     const double x = V.dx * (ix - V.nx / 2);
@@ -45,8 +52,24 @@ __device__ __forceinline__ void vext_point(const VextDev &V, long i, double &va,
     vb = (V.v.flip_b ? -g : g) + trap;
 }
 
-// V_ext of the context at step it (switch value evaluated on the host)
+// V_ext of the context at step it (switch value evaluated on the host; in a
+// project build the problem's v_ext field of that step, zero without
+// ENABLE_V_EXT)
 VextDev make_vext(const tdw_ctx *c, int it);
+// project build (tdw_hooks.hip): the engine's side of the problem's callbacks
+int project_attach(tdw_ctx *c);
+void project_release(tdw_ctx *c);
+int project_consts(tdw_ctx *c);
+const double *vext_grid(const tdw_ctx *c, int it);
+const double *sclgth_grid(const tdw_ctx *c, int it);
+const double2 *dext_field(tdw_ctx *c, int it, const double *pots);
+const double *vel_field(tdw_ctx *c, int it);
+// the potentials H reads: pots itself, or a copy with Delta + Delta_ext and
+// A_s - v_s when the problem has those callbacks (null on a launch error)
+const double *h_view(tdw_ctx *c, int it, const double *pots);
+int project_modify_potentials(tdw_ctx *c, int it, const double *dens, double *pots);
+// modify_densities (host) before the potentials of a whole step
+int step_potentials(tdw_ctx *c, int it, double *dens, double *pots);
 // functional parameters of the context
 FParams fparams(const tdw_ctx *c);
 

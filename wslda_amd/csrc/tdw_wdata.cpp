@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "tdw_internal.hpp"
+#include "tdw_hooks.hpp"
 
 namespace {
 
@@ -298,8 +299,20 @@ extern "C" int tdw_wdata_write(tdw_wdata *w, int it, const double *d_dens, const
         case S_J_A: src = hd + 4 * N; cnt = 3 * N; break;
         case S_J_B: src = hd + 9 * N; cnt = 3 * N; break;
         case S_NU: src = hd; cnt = 2 * N; break;
-        case S_DELTA:
-        case S_DELTAEXT: src = hp + 2 * N; cnt = 2 * N; break;
+        case S_DELTA: src = hp + 2 * N; cnt = 2 * N; break;
+        case S_DELTAEXT:
+            // delta_ext(delta) of the problem; zero without ENABLE_DELTA_EXT
+            // (wslda_writevars.c:415-425, pca_kernels.cu:1120-1147)
+            tmp.assign(2 * N, 0.0);
+            if (c->proj && (c->proj->flags & TDW_PROJ_DELTA_EXT)) {
+                TDW_TRY(c->proj->delta_ext(c->stream, it, (const double2 *)(d_pots + 2 * N), c->d_dext));
+                TDW_HIP(hipMemcpyAsync(tmp.data(), c->d_dext, sizeof(double) * 2 * N, hipMemcpyDeviceToHost,
+                                       c->stream));
+                TDW_HIP(hipStreamSynchronize(c->stream));
+            }
+            src = tmp.data();
+            cnt = 2 * N;
+            break;
         case S_ALPHA_A: src = hp + 4 * N; break;
         case S_ALPHA_B: src = hp + 5 * N; break;
         case S_A_A: src = hp + 6 * N; cnt = 3 * N; break;
@@ -317,7 +330,14 @@ extern "C" int tdw_wdata_write(tdw_wdata *w, int it, const double *d_dens, const
         }
         case S_VELEXT_A:
         case S_VELEXT_B:
+            // velocity_ext() of the problem (wslda_writevars.c:426-441)
             tmp.assign(3 * N, 0.0);
+            if (c->proj && c->proj->velocity_ext) {
+                TDW_TRY(c->proj->velocity_ext(c->stream, it, v.src == S_VELEXT_B ? 1 : 0, c->d_vel));
+                TDW_HIP(hipMemcpyAsync(tmp.data(), c->d_vel, sizeof(double) * 3 * N, hipMemcpyDeviceToHost,
+                                       c->stream));
+                TDW_HIP(hipStreamSynchronize(c->stream));
+            }
             src = tmp.data();
             cnt = 3 * N;
             break;
@@ -341,25 +361,38 @@ extern "C" int tdw_wdata_destroy(tdw_wdata *w)
 // (pca_kernels.cu:1120-1180; 1D twins ccpca_kernels.cu:1087-1141) on the
 // bound context.
 // ---------------------------------------------------------------------------
+// The reference evaluates delta_ext(delta) (zero without ENABLE_DELTA_EXT)
+// and velocity_ext() of the problem; the built-in problem has neither, so both
+// are zero there.
 extern "C" int get_delta_ext(int datadim, int it, void *deltain, void *data)
 {
     (void)datadim;
-    (void)it;
     tdw_ctx *c = tdw_bound();
     if (!c) return TDW_ENOCTX;
     if (!deltain || !data) return TDW_EINVAL;
+    const size_t N = c->N;
+    if (c->proj && (c->proj->flags & TDW_PROJ_DELTA_EXT)) {
+        TDW_TRY(c->proj->delta_ext(c->stream, it, (const double2 *)deltain, c->d_dext));
+        TDW_HIP(hipStreamSynchronize(c->stream));
+        return (int)hipMemcpy(data, c->d_dext, sizeof(double2) * N, hipMemcpyDeviceToHost);
+    }
     TDW_HIP(hipStreamSynchronize(c->stream));
-    return (int)hipMemcpy(data, deltain, sizeof(double) * 2 * c->N, hipMemcpyDeviceToHost);
+    memset(data, 0, sizeof(double) * 2 * N);
+    return 0;
 }
 
 extern "C" int get_velocity_ext(int datadim, int spin, int it, double *data)
 {
     (void)datadim;
-    (void)spin;
-    (void)it;
     tdw_ctx *c = tdw_bound();
     if (!c) return TDW_ENOCTX;
     if (!data) return TDW_EINVAL;
-    memset(data, 0, sizeof(double) * 3 * c->N);
+    const size_t N = c->N;
+    if (c->proj && c->proj->velocity_ext) {
+        TDW_TRY(c->proj->velocity_ext(c->stream, it, spin ? 1 : 0, c->d_vel));
+        TDW_HIP(hipStreamSynchronize(c->stream));
+        return (int)hipMemcpy(data, c->d_vel, sizeof(double) * 3 * N, hipMemcpyDeviceToHost);
+    }
+    memset(data, 0, sizeof(double) * 3 * N);
     return 0;
 }

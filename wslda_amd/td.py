@@ -24,6 +24,7 @@ TDW_EINVAL, TDW_ENOCTX, TDW_ESIZE, TDW_ENOMEM = -1001, -1002, -1003, -1004
 EKIN, EPOT, EPAIR, ECURRENT, EPOTEXT, EPAIREXT, EVELEXT, NPARTA, NPARTB, LZA, LZB = range(11)
 NITEMS = 11
 MAX_GAUSS = 4
+MAX_USER_PARAMS = 32
 
 
 class TDWError(RuntimeError):
@@ -140,9 +141,7 @@ def gauss_vext(gaussians, omega=0.0, switch=None, flip_b=0):
     return v
 
 
-_lib = None
-
-_SIGS = None
+_libs = {}
 
 
 def _sigs():
@@ -199,6 +198,7 @@ def _sigs():
         "tdw_timing_report": (I, [P, I, P, P, P]),
         "tdw_bind": (I, [P]),
         "tdw_bound": (P, []),
+        "tdw_project_info": (I, [C.POINTER(Params), C.POINTER(C.c_uint)]),
         # drop-in layer (reference names)
         "set_gpu": (I, [I]),
         "gpu_malloc": (I, [Z, C.POINTER(C.c_void_p)]),
@@ -269,20 +269,33 @@ def _sigs():
 EXPORTS = sorted(_sigs().keys())
 
 
-def lib():
-    """Load the HIP library; raises if it was not built."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIBPATH):
-            raise TDWError("libwslda_td.so not built: run `python -m wslda_amd.build` "
-                           "(or __graft_entry__.build())")
-        L = C.CDLL(LIBPATH)
+def lib(path=None):
+    """Load the HIP library (default: the engine's own libwslda_td.so; a
+    project build's library by its path); raises if it was not built."""
+    path = os.path.abspath(path or LIBPATH)
+    L = _libs.get(path)
+    if L is None:
+        if not os.path.exists(path):
+            raise TDWError("%s not built: run `python -m wslda_amd.build` (a project: "
+                           "`python -m wslda_amd.build --project DIR --codedim D`)" % path)
+        L = C.CDLL(path)
         for name, (res, args) in _sigs().items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return L
+
+
+def project_info(path):
+    """(Params, flags) a project library was compiled with: lattice,
+    functional, schemes and stabilisation constants of its predefines.h, and
+    the TDW_PROJ_* switches; None for the plain library."""
+    p = Params()
+    fl = C.c_uint(0)
+    if lib(path).tdw_project_info(C.byref(p), C.byref(fl)) != 1:
+        return None
+    return p, fl.value
 
 
 def check(rc, what=""):
@@ -299,22 +312,23 @@ def _ptr(t):
 class TDContext:
     """One td run's library context (lattice, functional, constants)."""
 
-    def __init__(self, params, nwf, device=0):
+    def __init__(self, params, nwf, device=0, libpath=None):
         import torch
         if not torch.cuda.is_available():
             raise TDWError("no HIP device: the td step runs only on the GPU")
+        self.L = lib(libpath)
         self.torch = torch
         self.params = params
         self.nwf = nwf
         self.device = device
         h = C.c_void_p()
-        check(lib().tdw_create(C.byref(params), nwf, device, C.byref(h)), "tdw_create")
+        check(self.L.tdw_create(C.byref(params), nwf, device, C.byref(h)), "tdw_create")
         self.h = h
-        self.N = int(lib().tdw_npts(h))
+        self.N = int(self.L.tdw_npts(h))
 
     def close(self):
         if getattr(self, "h", None):
-            lib().tdw_destroy(self.h)
+            self.L.tdw_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -324,17 +338,17 @@ class TDContext:
             pass
 
     def sync(self):
-        check(lib().tdw_sync(self.h), "tdw_sync")
+        check(self.L.tdw_sync(self.h), "tdw_sync")
 
     def set_params(self, params):
         self.params = params
-        check(lib().tdw_set_params(self.h, C.byref(params)), "tdw_set_params")
+        check(self.L.tdw_set_params(self.h, C.byref(params)), "tdw_set_params")
 
     def set_states(self, fbeta, kt=None, cnt=None):
         fb = np.ascontiguousarray(fbeta, dtype=np.float64)
         kt_ = None if kt is None else np.ascontiguousarray(kt, dtype=np.float64)
         cnt_ = None if cnt is None else np.ascontiguousarray(cnt, dtype=np.int32)
-        check(lib().tdw_set_states(self.h, fb.ctypes.data_as(C.c_void_p),
+        check(self.L.tdw_set_states(self.h, fb.ctypes.data_as(C.c_void_p),
                                    None if kt_ is None else kt_.ctypes.data_as(C.c_void_p),
                                    None if cnt_ is None else cnt_.ctypes.data_as(C.c_void_p)),
               "tdw_set_states")
@@ -347,75 +361,75 @@ class TDContext:
     def abm_step1(self, y, f):
         self._pre()
         fs = [_ptr(x) for x in f] + [None] * (5 - len(f))
-        check(lib().tdw_abm_step1(self.h, self.nwf_of(y), _ptr(y), *fs), "abm_step1")
+        check(self.L.tdw_abm_step1(self.h, self.nwf_of(y), _ptr(y), *fs), "abm_step1")
 
     def abm_step4(self, hy, y, flast):
         self._pre()
-        check(lib().tdw_abm_step4(self.h, self.nwf_of(y), _ptr(hy), _ptr(y), _ptr(flast)), "abm_step4")
+        check(self.L.tdw_abm_step4(self.h, self.nwf_of(y), _ptr(hy), _ptr(y), _ptr(flast)), "abm_step4")
 
     def normalize_wf(self, wf, norms=None):
         self._pre()
-        check(lib().tdw_normalize(self.h, self.nwf_of(wf), _ptr(wf), _ptr(norms)), "normalize_wf")
+        check(self.L.tdw_normalize(self.h, self.nwf_of(wf), _ptr(wf), _ptr(norms)), "normalize_wf")
 
     def compute_laplace(self, inp, out):
         self._pre()
         m = inp.numel() // self.N
-        check(lib().tdw_laplace(self.h, m, _ptr(inp), _ptr(out)), "compute_laplace")
+        check(self.L.tdw_laplace(self.h, m, _ptr(inp), _ptr(out)), "compute_laplace")
 
     def compute_derivatives(self, inp, ddx, ddy, ddz, lap):
         self._pre()
         m = inp.numel() // self.N
-        check(lib().tdw_derivatives(self.h, m, _ptr(inp), _ptr(ddx), _ptr(ddy), _ptr(ddz), _ptr(lap)),
+        check(self.L.tdw_derivatives(self.h, m, _ptr(inp), _ptr(ddx), _ptr(ddy), _ptr(ddz), _ptr(lap)),
               "compute_derivatives")
 
     def compute_laplace_real(self, f, out):
         self._pre()
-        check(lib().tdw_laplace_real(self.h, _ptr(f), _ptr(out)), "compute_laplace_real_f")
+        check(self.L.tdw_laplace_real(self.h, _ptr(f), _ptr(out)), "compute_laplace_real_f")
 
     def calculate_densities(self, wf, ddx, ddy, ddz, lap, dens, gradients, weights=None):
         self._pre()
-        check(lib().tdw_densities(self.h, self.nwf_of(wf), _ptr(wf), _ptr(ddx), _ptr(ddy), _ptr(ddz),
+        check(self.L.tdw_densities(self.h, self.nwf_of(wf), _ptr(wf), _ptr(ddx), _ptr(ddy), _ptr(ddz),
                                   _ptr(lap), _ptr(weights), _ptr(dens), int(gradients)),
               "calculate_densities")
 
     def density_caculate_tau(self, dens):
         self._pre()
-        check(lib().tdw_density_tau(self.h, _ptr(dens)), "density_caculate_tau")
+        check(self.L.tdw_density_tau(self.h, _ptr(dens)), "density_caculate_tau")
 
     def compute_potentials(self, it, dens, pots):
         self._pre()
-        check(lib().tdw_potentials(self.h, it, _ptr(dens), _ptr(pots)), "compute_potentials")
+        check(self.L.tdw_potentials(self.h, it, _ptr(dens), _ptr(pots)), "compute_potentials")
 
     def apply_hamiltonian(self, it, wf_in, wf_out, lap, pots, useqpe=None, qpe=None):
         self._pre()
-        check(lib().tdw_apply_h(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(lap),
+        check(self.L.tdw_apply_h(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(lap),
                                 _ptr(pots), _ptr(useqpe), _ptr(qpe)), "apply_hamiltonian")
 
     def apply_hamiltonian_full(self, it, wf_in, wf_out, ddx, ddy, ddz, lap, alap, pots, useqpe=None, qpe=None):
         """H application with the gradients and lap(alpha psi) (alap None:
         formed by the library); the form ASLDA needs."""
         self._pre()
-        check(lib().tdw_apply_h_full(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(ddx),
+        check(self.L.tdw_apply_h_full(self.h, it, self.nwf_of(wf_in), _ptr(wf_in), _ptr(wf_out), _ptr(ddx),
                                      _ptr(ddy), _ptr(ddz), _ptr(lap), _ptr(alap), _ptr(pots), _ptr(useqpe),
                                      _ptr(qpe)), "apply_hamiltonian_full")
 
     def multiply_wf_by_alpha(self, wf_in, wf_out, pots):
         self._pre()
-        check(lib().tdw_alpha_wf(self.h, self.nwf_of(wf_in), _ptr(wf_in), _ptr(pots), _ptr(wf_out)),
+        check(self.L.tdw_alpha_wf(self.h, self.nwf_of(wf_in), _ptr(wf_in), _ptr(pots), _ptr(wf_out)),
               "multiply_wf_by_alpha")
 
     def compute_energy(self, it, dens, pots):
         self._pre()
         e = np.zeros(NITEMS)
-        check(lib().tdw_energy(self.h, it, _ptr(dens), _ptr(pots), e.ctypes.data_as(C.c_void_p)),
+        check(self.L.tdw_energy(self.h, it, _ptr(dens), _ptr(pots), e.ctypes.data_as(C.c_void_p)),
               "compute_energy")
         return e
 
     def stream_ptr(self):
-        return int(lib().tdw_stream(self.h))
+        return int(self.L.tdw_stream(self.h))
 
     def timing(self, enable):
-        check(lib().tdw_timing(self.h, int(enable)), "tdw_timing")
+        check(self.L.tdw_timing(self.h, int(enable)), "tdw_timing")
 
     def timing_report(self):
         """{kernel name: (total ms, launches)} since the last report."""
@@ -423,7 +437,7 @@ class TDContext:
         names = C.create_string_buffer(32 * mx)
         ms = np.zeros(mx)
         cnt = np.zeros(mx, dtype=np.int32)
-        k = lib().tdw_timing_report(self.h, mx, names, ms.ctypes.data_as(C.c_void_p),
+        k = self.L.tdw_timing_report(self.h, mx, names, ms.ctypes.data_as(C.c_void_p),
                                     cnt.ctypes.data_as(C.c_void_p))
         if k < 0:
             raise TDWError("tdw_timing_report failed with code %d" % k)
@@ -438,7 +452,7 @@ class TDContext:
         self._pre()
         va = self.torch.empty(self.N, dtype=self.torch.float64, device="cuda:%d" % self.device)
         vb = self.torch.empty_like(va)
-        check(lib().tdw_vext_eval(self.h, it, _ptr(va), _ptr(vb)), "tdw_vext_eval")
+        check(self.L.tdw_vext_eval(self.h, it, _ptr(va), _ptr(vb)), "tdw_vext_eval")
         self.sync()
         return va, vb
 
@@ -560,7 +574,7 @@ class TDRun:
         return [by[self.st.f[k]] for k in range(5)]
 
     def initial_measure(self):
-        check(lib().tdw_initial_measure(self.ctx.h, C.byref(self.st)), "tdw_initial_measure")
+        check(self.ctx.L.tdw_initial_measure(self.ctx.h, C.byref(self.st)), "tdw_initial_measure")
 
     def selfstart(self):
         """Taylor self-start of the history (selfstart 1, pca.c:1086-1392):
@@ -572,17 +586,17 @@ class TDRun:
         k = {AB3AM4: 3, AB4AM5: 4, AB5AM5: 5}[self.ctx.params.scheme]
         scr = [torch.empty_like(self.wf) for _ in range(max(0, 5 - k))]
         ptrs = [t.data_ptr() for t in scr] + [None, None]
-        check(lib().tdw_selfstart(self.ctx.h, C.byref(self.st), ptrs[0], ptrs[1]), "tdw_selfstart")
+        check(self.ctx.L.tdw_selfstart(self.ctx.h, C.byref(self.st), ptrs[0], ptrs[1]), "tdw_selfstart")
         self.ctx.sync()
         del scr
 
     def step(self, it, grad_last=False):
-        fn = lib().tdw_step_fused if self.fused else lib().tdw_step
+        fn = self.ctx.L.tdw_step_fused if self.fused else self.ctx.L.tdw_step
         check(fn(self.ctx.h, C.byref(self.st), it, int(grad_last)), "tdw_step")
 
     def finalize(self):
         """Apply a fused step's pending normalisation / <H> subtraction."""
-        check(lib().tdw_finalize(self.ctx.h, C.byref(self.st)), "tdw_finalize")
+        check(self.ctx.L.tdw_finalize(self.ctx.h, C.byref(self.st)), "tdw_finalize")
 
     def energy(self, it):
         return self.ctx.compute_energy(it, self.dens, self.pots)
@@ -593,7 +607,7 @@ class TDRun:
         sc = CheckpointScalars(**scalars)
         fb = np.ascontiguousarray(fbeta, dtype=np.float64)
         ktv = None if kt is None else np.ascontiguousarray(kt, dtype=np.float64)
-        check(lib().tdw_checkpoint_save(self.ctx.h, path.encode(), C.byref(self.st), float(t0),
+        check(self.ctx.L.tdw_checkpoint_save(self.ctx.h, path.encode(), C.byref(self.st), float(t0),
                                         fb.ctypes.data_as(C.c_void_p),
                                         None if ktv is None else ktv.ctypes.data_as(C.c_void_p), C.byref(sc)),
               "tdw_checkpoint_save")
@@ -606,7 +620,7 @@ class TDRun:
         fb = np.zeros(n)
         kt = np.zeros(2 * n)
         sc = CheckpointScalars()
-        check(lib().tdw_checkpoint_load(self.ctx.h, path.encode(), C.byref(self.st), C.byref(t0),
+        check(self.ctx.L.tdw_checkpoint_load(self.ctx.h, path.encode(), C.byref(self.st), C.byref(t0),
                                         fb.ctypes.data_as(C.c_void_p), kt.ctypes.data_as(C.c_void_p), C.byref(sc)),
               "tdw_checkpoint_load")
         dim = self.ctx.params.dim
@@ -623,6 +637,7 @@ class WData:
     def __init__(self, ctx, outprefix, writevar="default", writeprec="d", spinsymmetry=0, overwrite=1,
                  inittype=0, t0=0.0, dt=1.0, codename=None, consts=None):
         self.ctx = ctx
+        self.L = ctx.L
         wp = WDataParams()
         wp.outprefix = str(outprefix).encode()
         wp.writevar = writevar.encode()
@@ -631,22 +646,22 @@ class WData:
         wp.t0, wp.dt = float(t0), float(dt)
         wp.codename = codename.encode() if codename else None
         h = C.c_void_p()
-        check(lib().tdw_wdata_create(ctx.h, C.byref(wp), C.byref(h)), "tdw_wdata_create")
+        check(ctx.L.tdw_wdata_create(ctx.h, C.byref(wp), C.byref(h)), "tdw_wdata_create")
         self.h = h
         for k, v in (consts or {}).items():
-            check(lib().tdw_wdata_setconst(self.h, k.encode(), float(v), None), "tdw_wdata_setconst")
-        check(lib().tdw_wdata_begin(self.h), "tdw_wdata_begin")
+            check(self.L.tdw_wdata_setconst(self.h, k.encode(), float(v), None), "tdw_wdata_setconst")
+        check(self.L.tdw_wdata_begin(self.h), "tdw_wdata_begin")
 
     def write(self, it, dens, pots):
-        check(lib().tdw_wdata_write(self.h, int(it), _ptr(dens), _ptr(pots)), "tdw_wdata_write")
+        check(self.L.tdw_wdata_write(self.h, int(it), _ptr(dens), _ptr(pots)), "tdw_wdata_write")
 
     @property
     def cycles(self):
-        return lib().tdw_wdata_cycles(self.h)
+        return self.L.tdw_wdata_cycles(self.h)
 
     def close(self):
         if self.h:
-            lib().tdw_wdata_destroy(self.h)
+            self.L.tdw_wdata_destroy(self.h)
             self.h = None
 
     def __del__(self):
