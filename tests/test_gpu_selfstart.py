@@ -27,14 +27,16 @@ def torch():
 
 
 def test_taylor_terms_closed_form(torch):
-    """H applied without the <H> subtraction (useqpe = 0) to plane-wave
-    eigenstates of the uniform solution returns lam psi; chained with
-    taylor_expansion_contribution for orders 1..5 the k-th contribution is
-    (-i lam dt)^k / k! psi and the accumulated update psi sum_k of them.
-    lam is each state's Rayleigh quotient <psi|H psi> / <psi|psi> from the
-    first H application (the solver's E_n differs from it by the solution's
-    convergence tolerance, which (E dt)^k amplifies k-fold relative to the
-    small terms); it equals E_n to 1e-9."""
+    """taylor_expansion_contribution (pca_kernels.cu:1032-1085) on plane-wave
+    eigenstates of the uniform solution, orders 1..5, H applied without the
+    <H> subtraction (useqpe = 0):
+    - each call gives contr = (-i dt / k) H contr_prev and upd += contr
+      exactly (the recursion, to round-off);
+    - the k-th term is (-i lam dt)^k / k! psi with lam the state's Rayleigh
+      quotient (equal to the solver's E_n): to 1e-12 for k = 1, then within
+      1e-12 (K / lam)^(k - 1) with K / lam <= 100 -- round-off of a
+      finite-precision eigenstate lies along high-momentum directions, which
+      H amplifies by the kinetic scale K over lam at each order."""
     from wslda_amd import problems, td
     say("case: taylor_expansion_contribution, plane-wave eigenstates, orders 1..5")
     s = problems.uniform_start(3, (8, 8, 8), n_states=40)
@@ -61,9 +63,15 @@ def test_taylor_terms_closed_form(torch):
             td.check(L.compute_laplace(2 * n, ptr(contr), ptr(lap), 256), "compute_laplace")
             td.check(L.apply_hamiltonian(0, n, ptr(contr), ptr(hpsi), None, None, None, ptr(lap), None, ptr(dens),
                                          ptr(pots), C.c_double(0.0), ptr(zero), C.c_double(1.0), 256), "apply_h")
+            ctx.sync()
+            upd_prev = upd.clone()
             td.check(L.taylor_expansion_contribution(k, C.c_double(dt), n, ptr(hpsi), ptr(upd), ptr(contr), 256),
                      "taylor")
             ctx.sync()
+            # the recursion itself
+            step = hpsi * (-1j * dt / k)
+            assert rel(contr.cpu().numpy(), step.cpu().numpy()) < 1e-15, k
+            assert rel(upd.cpu().numpy(), (upd_prev + contr).cpu().numpy()) < 1e-15, k
             if k == 1:
                 # per state (u row a and v row n + a together)
                 num = (psi[:n].conj() * hpsi[:n]).sum(dim=1).real + (psi[n:].conj() * hpsi[n:]).sum(dim=1).real
@@ -71,14 +79,14 @@ def test_taylor_terms_closed_form(torch):
                 lam = num / den
                 say("  max |lam - E_n| = %.3e" % (lam - En).abs().max().item())
                 assert (lam - En).abs().max().item() < 1e-9
-                # the states are eigenstates: H psi = lam psi to round-off
-                resid = (hpsi - psi * torch.cat([lam, lam])[:, None]).abs().max().item()
-                assert resid < 1e-10 * hpsi.abs().max().item(), resid
             fac = ((-1j * dt * lam) ** k / math.factorial(k))
             want = psi * torch.cat([fac, fac])[:, None]
             expect_upd = expect_upd + want
-            assert rel(contr.cpu().numpy(), want.cpu().numpy()) < 1e-11, k
-            assert rel(upd.cpu().numpy(), expect_upd.cpu().numpy()) < 1e-12, k
+            e_contr = rel(contr.cpu().numpy(), want.cpu().numpy())
+            e_upd = rel(upd.cpu().numpy(), expect_upd.cpu().numpy())
+            say("  order %d: term %.2e, update %.2e" % (k, e_contr, e_upd))
+            assert e_contr < 1e-12 * 100.0 ** (k - 1), k
+            assert e_upd < 1e-12, k
     finally:
         L.tdw_bind(None)
     ctx.close()
