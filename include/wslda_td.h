@@ -83,6 +83,7 @@ typedef struct { float re, im; } tdw_complex32;
 #define TDW_ENOMEM (-1004)
 #define TDW_EEXIST (-1005)  /* output file exists and overwrite is off   */
 #define TDW_EIO (-1006)     /* file could not be opened / written       */
+#define TDW_ESCRATCH (-1007) /* input larger than the no-allocation scratch */
 
 /* External potential of the built-in problem: a sum of Gaussians
  * amp*s(t)*exp(wx*(x-cx)^2 + wy*(y-cy)^2 + wz*(z-cz)^2), a harmonic trap
@@ -160,6 +161,16 @@ int tdw_abm_step1(tdw_ctx *c, int n, tdw_complex *y, tdw_complex *f1, tdw_comple
                   tdw_complex *f3, tdw_complex *f4, tdw_complex *f5);
 int tdw_abm_step4(tdw_ctx *c, int n, const tdw_complex *hy, tdw_complex *y, const tdw_complex *flast);
 int tdw_normalize(tdw_ctx *c, int n, tdw_complex *wf, double *norms /* n, may be NULL */);
+/* tdw_normalize that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The partial sums (and
+ * the norms when norms is NULL) go to a fixed per-device array of
+ * TDW_NOALLOC_SCRATCH_DOUBLES doubles set up by tdw_create; inputs needing
+ * more (n * ceil(N / 4096) doubles, + n without norms) return TDW_ESCRATCH.
+ * All calls on one device share that array: calls that overlap on a device
+ * race.  Same kernels, same results as tdw_normalize. */
+#define TDW_NOALLOC_SCRATCH_DOUBLES (1 << 20)
+int tdw_normalize_noalloc(tdw_ctx *c, int n, tdw_complex *wf, double *norms /* n, may be NULL */);
 int tdw_laplace(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *out);
 int tdw_derivatives(tdw_ctx *c, int m, const tdw_complex *in, tdw_complex *ddx,
                     tdw_complex *ddy, tdw_complex *ddz, tdw_complex *lap);

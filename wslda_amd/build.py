@@ -22,7 +22,7 @@ SRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwslda_td.so")
 ARCH = os.environ.get("TDW_OFFLOAD_ARCH", "gfx950")
-UNITS = ["tdw_api.hip", "tdw_fft.hip", "tdw_physics.hip", "tdw_fused.hip", "tdw_adjoint.hip", "tdw_aslda.hip", "tdw_grid.hip", "tdw_hooks.hip", "tdw_uniform.cpp", "tdw_checkpoint.cpp", "tdw_wdata.cpp", "tdw_stio.cpp"]
+UNITS = ["tdw_api.hip", "tdw_fft.hip", "tdw_physics.hip", "tdw_fused.hip", "tdw_adjoint.hip", "tdw_aslda.hip", "tdw_grid.hip", "tdw_hooks.hip", "tdw_noalloc.hip", "tdw_uniform.cpp", "tdw_checkpoint.cpp", "tdw_wdata.cpp", "tdw_stio.cpp"]
 
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -195,6 +195,7 @@ extern "C" int tdw_create(const tdw_params *p, int nwf, int device, tdw_ctx **ou
     if (!e) e = (int)hipMalloc(&c->d_rtmp, sizeof(double2) * c->N);
     if (!e) e = (int)hipMalloc(&c->d_vext, sizeof(double) * 2 * c->N);
     if (!e) e = (int)hipMemset(c->d_kt, 0, sizeof(double) * 2 * nwf);
+    if (!e) e = noalloc_setup(device);
     if (!e) {
         int *ones = (int *)malloc(sizeof(int) * nwf);
         for (int i = 0; i < nwf; i++) ones[i] = 1;

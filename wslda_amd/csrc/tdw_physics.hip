@@ -158,25 +158,44 @@ static long pick_chunk(const tdw_ctx *c, int *nchunk)
     return chunk;
 }
 
-int norms(tdw_ctx *c, int n, const double2 *wf, double *d_nrm)
+int norm_chunks(const tdw_ctx *c)
+{
+    int nchunk;
+    pick_chunk(c, &nchunk);
+    return nchunk;
+}
+
+// every (chunk, state) slot of part is written before the row sums read it
+int norms_with(tdw_ctx *c, int n, const double2 *wf, double *d_nrm, double *part)
 {
     TDW_TIMED(c, "norms");
     int nchunk;
     long chunk = pick_chunk(c, &nchunk);
-    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
-    hipLaunchKernelGGL(k_norm_part, dim3(nchunk, n), dim3(NT), 0, c->stream, wf, n, (long)c->N, chunk, c->d_part);
-    hipLaunchKernelGGL(k_rowsum, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk,
-                       dvol_norm(c), d_nrm);
+    hipLaunchKernelGGL(k_norm_part, dim3(nchunk, n), dim3(NT), 0, c->stream, wf, n, (long)c->N, chunk, part);
+    hipLaunchKernelGGL(k_rowsum, dim3((n + 255) / 256), dim3(256), 0, c->stream, part, n, nchunk, dvol_norm(c),
+                       d_nrm);
     return (int)hipGetLastError();
 }
 
-int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm)
+int normalize_with(tdw_ctx *c, int n, double2 *wf, double *d_nrm, double *part)
 {
-    TDW_TRY(norms(c, n, wf, d_nrm));
+    TDW_TRY(norms_with(c, n, wf, d_nrm, part));
     long tot = 2L * n * (long)c->N;
     TDW_TIMED(c, "scale");
     hipLaunchKernelGGL(k_scale_states, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, wf, n, (long)c->N, d_nrm);
     return (int)hipGetLastError();
+}
+
+int norms(tdw_ctx *c, int n, const double2 *wf, double *d_nrm)
+{
+    TDW_TRY(ensure_part(c, (size_t)n * norm_chunks(c)));
+    return norms_with(c, n, wf, d_nrm, c->d_part);
+}
+
+int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm)
+{
+    TDW_TRY(ensure_part(c, (size_t)n * norm_chunks(c)));
+    return normalize_with(c, n, wf, d_nrm, c->d_part);
 }
 
 // ---------------------------------------------------------------------------

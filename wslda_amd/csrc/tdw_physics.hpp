@@ -106,6 +106,14 @@ int abm1_to(tdw_ctx *c, int n, const double2 *yin, double2 *y, double2 *cout, do
 int abm4(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl);
 int norms(tdw_ctx *c, int n, const double2 *wf, double *d_nrm);
 int normalize(tdw_ctx *c, int n, double2 *wf, double *d_nrm);
+// the same two with the caller's partial-sum buffer (n * norm_chunks(c)
+// doubles) instead of the context's
+int norm_chunks(const tdw_ctx *c);
+int norms_with(tdw_ctx *c, int n, const double2 *wf, double *d_nrm, double *part);
+int normalize_with(tdw_ctx *c, int n, double2 *wf, double *d_nrm, double *part);
+// the no-allocation scratch of this device (tdw_noalloc.hip), set up once per
+// device by tdw_create with the device current
+int noalloc_setup(int device);
 int laplace(tdw_ctx *c, int m, const double2 *in, double2 *out);
 int derivatives(tdw_ctx *c, int m, const double2 *in, double2 *ddx, double2 *ddy, double2 *ddz, double2 *lap);
 int laplace_real(tdw_ctx *c, const double *f, double *lap);

@@ -21,6 +21,8 @@ SPHERICAL_CUTOFF, CUBIC_CUTOFF = 88, 89
 AB3AM4, AB4AM5, AB5AM5 = 34, 45, 55
 FP64, SPLIT = 0, 1
 TDW_EINVAL, TDW_ENOCTX, TDW_ESIZE, TDW_ENOMEM = -1001, -1002, -1003, -1004
+TDW_ESCRATCH = -1007
+NOALLOC_SCRATCH_DOUBLES = 1 << 20
 EKIN, EPOT, EPAIR, ECURRENT, EPOTEXT, EPAIREXT, EVELEXT, NPARTA, NPARTB, LZA, LZB = range(11)
 NITEMS = 11
 MAX_GAUSS = 4
@@ -158,6 +160,7 @@ def _sigs():
         "tdw_abm_step1": (I, [P, I, P, P, P, P, P, P]),
         "tdw_abm_step4": (I, [P, I, P, P, P]),
         "tdw_normalize": (I, [P, I, P, P]),
+        "tdw_normalize_noalloc": (I, [P, I, P, P]),
         "tdw_laplace": (I, [P, I, P, P]),
         "tdw_derivatives": (I, [P, I, P, P, P, P, P]),
         "tdw_laplace_real": (I, [P, P, P]),
