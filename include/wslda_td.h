@@ -201,6 +201,9 @@ int tdw_apply_h_full(tdw_ctx *c, int it, int n, const tdw_complex *in, tdw_compl
 /* out = alpha_a u, alpha_b v (multiply_wf_by_alpha, pca_kernels.cu:1016);
  * in place allowed. */
 int tdw_alpha_wf(tdw_ctx *c, int n, const tdw_complex *in, const double *pots, tdw_complex *out);
+/* <wf1_a|wf2_a> = dV sum (u1* u2 + v1* v2), real parts into re[a] and
+ * imaginary parts into im[a] (either may be NULL; device arrays). */
+int tdw_overlap(tdw_ctx *c, int n, const tdw_complex *wf1, const tdw_complex *wf2, double *re, double *im);
 int tdw_energy(tdw_ctx *c, int it, const double *dens, const double *pots, double *h_e /* 11 */);
 int tdw_vext_eval(tdw_ctx *c, int it, double *va, double *vb);
 
@@ -555,11 +558,11 @@ int taylor_expansion_contribution(int it, double dt, int n, tdw_complex *wf_hpsi
                                   tdw_complex *wf_update, tdw_complex *wf_contr,
                                   int nthreads);                     /* pca_kernels.cu:1076 */
 int symmetrize_densities_device(double *d_densities);                /* pca_dens.cu:343 */
-/* <wf1_a|wf2_a> = dV sum (u1* u2 + v1* v2) for a < n into overlap_re[a] and
- * overlap_im[a] (either may be NULL; device arrays); workarea is unused.
- * The reference copies the real part into overlap_im too
- * (pca_kernels.cu:903-906); here overlap_im is the imaginary part its
- * documentation names. */
+/* Re <wf1_a|wf2_a> = Re dV sum (u1* u2 + v1* v2) for a < n into
+ * overlap_re[a] and, as the reference does (it copies the first reduced sum
+ * into both outputs, pca_kernels.cu:899-907), into overlap_im[a] too (either
+ * may be NULL; device arrays); workarea is unused.  tdw_overlap gives the
+ * imaginary part. */
 int compute_ovelap(int n, tdw_complex *wf1, tdw_complex *wf2, double *overlap_re,
                    double *overlap_im, double *workarea, int nthreads); /* pca_kernels.cu:877 */
 /* sum of size device doubles into partial_sums[0] (mode 0 only) */

@@ -872,7 +872,9 @@ extern "C" int high_frequency_filter_massive_d(int n, double *in, double *out, d
     return grid_filter_d(c, n, in, out, fd_mu, fd_T);
 }
 // overlaps of state pairs (pca_kernels.cu:877-911); workarea is not needed
-// (the sums run in the context), overlap_im receives the imaginary parts
+// (the sums run in the context).  As in the reference, overlap_im receives
+// the real part too (it copies the first of the two reduced sums into both
+// outputs, :899-907); tdw_overlap gives the imaginary part.
 extern "C" int compute_ovelap(int n, tdw_complex *wf1, tdw_complex *wf2, double *overlap_re, double *overlap_im,
                               double *workarea, int nthreads)
 {
@@ -880,7 +882,17 @@ extern "C" int compute_ovelap(int n, tdw_complex *wf1, tdw_complex *wf2, double 
     (void)workarea;
     (void)nthreads;
     if (n < 0 || n > c->nwf || ((!wf1 || !wf2) && n > 0)) return TDW_EINVAL;
-    return grid_overlap(c, n, CD2(wf1), CD2(wf2), overlap_re, overlap_im);
+    double *re = overlap_re ? overlap_re : overlap_im;
+    if (!re) return 0;
+    TDW_TRY(grid_overlap(c, n, CD2(wf1), CD2(wf2), re, nullptr));
+    if (overlap_re && overlap_im && n > 0)
+        TDW_HIP(hipMemcpyAsync(overlap_im, overlap_re, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+extern "C" int tdw_overlap(tdw_ctx *c, int n, const tdw_complex *wf1, const tdw_complex *wf2, double *re, double *im)
+{
+    if (!c || n < 0 || n > c->nwf || ((!wf1 || !wf2) && n > 0)) return TDW_EINVAL;
+    return grid_overlap(c, n, CD2(wf1), CD2(wf2), re, im);
 }
 // sum of size doubles into partial_sums[0] (wslda_cuda_utils.h:250-278)
 extern "C" int local_reductionR(double *array, int size, double *partial_sums, int threads, int mode)

@@ -170,6 +170,7 @@ def _sigs():
         "tdw_apply_h": (I, [P, I, I, P, P, P, P, P, P]),
         "tdw_apply_h_full": (I, [P, I, I, P, P, P, P, P, P, P, P, P, P]),
         "tdw_alpha_wf": (I, [P, I, P, P, P]),
+        "tdw_overlap": (I, [P, I, P, P, P, P]),
         "tdw_energy": (I, [P, I, P, P, P]),
         "tdw_vext_eval": (I, [P, I, P, P]),
         "tdw_step": (I, [P, C.POINTER(State), I, I]),
