@@ -396,17 +396,18 @@ def run_adjoint(args):
     }
 
 
-def trap_leg(args, prec, nst, steps, timed):
-    """The trapped-vortex workload (32^3) in one precision: initial
-    measurement, three steps (the last a gradient step), then `steps` timed
-    steps with gradient steps every meas_every and at the end.  Every array
-    is released before returning."""
+def trap_leg(args, prec, nst, steps, timed, build=None, name="trap-vortex"):
+    """A 3D workload (default: the 32^3 trapped vortex) in one precision:
+    initial measurement, three steps (the last a gradient step), then
+    `steps` timed steps with gradient steps every meas_every and at the end.
+    Every array is released before returning."""
     import torch
     from wslda_amd import problems, td
-    sys.stderr.write("trap-vortex: %s, %d states, %d steps\n" % ("split" if prec == td.SPLIT else "fp64", nst,
-                                                                 steps))
+    sys.stderr.write("%s: %s, %s states, %d steps\n" % (name, "split" if prec == td.SPLIT else "fp64",
+                                                        "all" if nst < 0 else nst, steps))
     sys.stderr.flush()
-    w = problems.config3(torch, precision=prec, n_states=nst)
+    build = build or problems.config3
+    w = build(torch, precision=prec, first=None if nst < 0 else nst)
     p = w["params"]
     n = w["wf"].shape[0] // 2
     ctx = td.TDContext(p, n, 0)
@@ -444,7 +445,7 @@ def trap_leg(args, prec, nst, steps, timed):
 
 def trap_pair(args):
     """Split precision and FP64 on the whole trapped-vortex workload (32^3,
-    24576 states), one after the other (each alone fills most of the HBM):
+    24586 states), one after the other (each alone fills most of the HBM):
     both throughputs, the speedup, and the split run's density errors
     against the FP64 run after the same steps from the same seeded start --
     the current j included, measured against its own maximum (the vortex
@@ -458,8 +459,9 @@ def trap_pair(args):
     drift = lambda r: abs(r["e1"][:7].sum() - r["e0"][:7].sum()) / abs(r["e0"][:7].sum())
     N = fp["N"]
     return {
-        "workload": "td-SLDA 3D 32^3, harmonic trap omega = 0.1 eF, vortex line imprinted on u, %d Gaussian-envelope "
-                    "plane-wave BdG states (seeded noise), AB5AM5, %d timed steps" % (sp["n"], steps),
+        "workload": "td-SLDA 3D 32^3, harmonic trap omega = 0.1 eF on from t = 0, vortex line imprinted on u, the "
+                    "uniform solution's %d plane-wave BdG states below its cutoff, seeded noise in a Gaussian "
+                    "envelope, AB5AM5, %d timed steps" % (sp["n"], steps),
         "split_value": sp["qps"], "fp64_value": fp["qps"], "unit": "qp-steps/s",
         "split_ms_per_step": sp["ms"], "fp64_ms_per_step": fp["ms"], "speedup_vs_fp64": sp["qps"] / fp["qps"],
         "density_rel_err_vs_fp64": err, "max_density_rel_err": max(err.values()),
@@ -469,6 +471,43 @@ def trap_pair(args):
         "energy_drift_fp64": drift(fp), "energy_drift_split": drift(sp), "drift_steps": steps,
         "split_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(sp["rep"].items(), key=lambda kv: -kv[1][0])},
         "fp64_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(fp["rep"].items(), key=lambda kv: -kv[1][0])},
+    }
+
+
+def tube_pair(args):
+    """The tube Josephson junction (32 x 32 x 96, barrier 0.3 eF, pi phase
+    step) in split precision and FP64 on one GPU's share of its 12062
+    states (--tube-states, default 6031: half of them, a two-GPU split),
+    with the split run's density errors against the FP64 run."""
+    from wslda_amd import problems, td
+    steps = args.trap_steps or args.steps
+    sp = trap_leg(args, td.SPLIT, args.tube_states, steps, True, build=problems.config5, name="tube")
+    fp = trap_leg(args, td.FP64, args.tube_states, steps, True, build=problems.config5, name="tube")
+    err = density_errors(sp["dens"], fp["dens"], fp["N"], currents=True)
+    E64, Esp = fp["e1"][:7].sum(), sp["e1"][:7].sum()
+    N = fp["N"]
+    return {
+        "workload": "td-SLDA 3D tube 32x32x96, Gaussian barrier 0.3 eF of width 3 dx across z, pi phase step "
+                    "imprinted on u, %d of the uniform solution's 12062 states below its cutoff (one GPU's "
+                    "share), AB5AM5, %d timed steps" % (sp["n"], steps),
+        "split_value": sp["qps"], "fp64_value": fp["qps"], "unit": "qp-steps/s",
+        "split_ms_per_step": sp["ms"], "fp64_ms_per_step": fp["ms"], "speedup_vs_fp64": sp["qps"] / fp["qps"],
+        "density_rel_err_vs_fp64": err, "max_density_rel_err": max(err.values()),
+        "current_per_spin_rel_err": spin_current_errors(sp["dens"], fp["dens"], N),
+        "energy_rel_diff_vs_fp64": abs(Esp - E64) / abs(E64),
+        "split_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(sp["rep"].items(), key=lambda kv: -kv[1][0])},
+        "fp64_kernels_ms_per_step": {k: v[0] / steps for k, v in sorted(fp["rep"].items(), key=lambda kv: -kv[1][0])},
+    }
+
+
+def run_tube(args):
+    """--tube: the tube pair as its own JSON line."""
+    t = tube_pair(args)
+    return {
+        "metric": "qp-wavefunction steps/s, split precision, tube Josephson junction",
+        "value": t["split_value"], "unit": "qp-steps/s", "n_gpus": 1, "steps": args.trap_steps or args.steps,
+        "ms_per_step": t["split_ms_per_step"], "higher_is_better": True, "dtype": "f32-operator/f64-state",
+        "data": "synthetic", "config": {"workload": t["workload"] + ", split precision"}, "tube": t,
     }
 
 
@@ -552,10 +591,15 @@ def main():
                     help="segments the forward run tapes directly (each 2K+1 state arrays of HBM)")
     ap.add_argument("--trap", action="store_true",
                     help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
-    ap.add_argument("--trap-states", type=int, default=24576,
-                    help="states of the trapped-vortex pair in the default line (0: leave the pair out)")
+    ap.add_argument("--trap-states", type=int, default=-1,
+                    help="states of the trapped-vortex pair in the default line (-1: the workload's whole "
+                         "state set, 24586 states; 0: leave the pair out; n: its first n states)")
     ap.add_argument("--trap-steps", type=int, default=0, help="timed steps of the trapped-vortex pair "
                                                               "(0: --steps)")
+    ap.add_argument("--tube", action="store_true",
+                    help="the tube Josephson junction (32x32x96) in split precision and FP64 as its own line")
+    ap.add_argument("--tube-states", type=int, default=6031,
+                    help="states of the tube runs (one GPU's share of its 12062; -1: all)")
     ap.add_argument("--aslda", action="store_true",
                     help="instead of the step benchmark: the ASLDA step on the 3D uniform workload (24^3)")
     ap.add_argument("--sldae", action="store_true",
@@ -572,6 +616,9 @@ def main():
         return
     if args.trap:
         print(json.dumps(run_trap(args)))
+        return
+    if args.tube:
+        print(json.dumps(run_tube(args)))
         return
     if args.aslda:
         print(json.dumps(run_aslda(args)))
@@ -627,7 +674,7 @@ def main():
         res["cpu_baseline"] = cpu_baseline(g, args)
     g["ctx"].close()
     del g
-    if args.mode == "both" and args.trap_states > 0:
+    if args.mode == "both" and args.trap_states != 0:
         import torch
         torch.cuda.synchronize()
         torch.cuda.empty_cache()

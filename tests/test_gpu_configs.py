@@ -137,11 +137,12 @@ def test_config2_full_start_is_stationary(torch):
 
 def test_config3_trap_vortex_slice_vs_oracle(torch):
     """The 32^3 trapped-vortex workload (trap 0.1 eF, vortex phase on u,
-    Gaussian envelopes, seeded noise) on 48 states, FP64: 5 fused PECE steps
-    with gradient steps at 1 and 4 against the oracle at 1e-12."""
+    seeded noise in a Gaussian envelope) on the first 48 of its states, FP64:
+    5 fused PECE steps with gradient steps at 1 and 4 against the oracle at
+    1e-12."""
     say("case: config 3 (32^3 trap + vortex, 48 states), FP64 fused, 5 steps vs oracle")
     from wslda_amd import problems, td
-    w = problems.config3(torch, precision=td.FP64, n_states=48)
+    w = problems.config3(torch, precision=td.FP64, first=48)
     p = w["params"]
     n = w["wf"].shape[0] // 2
     cfg, P = oracle_of(w)

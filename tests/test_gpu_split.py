@@ -166,7 +166,7 @@ def test_split_rejects_unfused_paths(torch):
 
 def test_split_vs_fp64_trap_vortex_currents(torch):
     """The 32^3 trapped-vortex workload (trap 0.1 eF, vortex phase on u) on
-    its 1024 lowest-|E| states, 40 steps with gradient steps every 20:
+    its first 1024 states, 40 steps with gradient steps every 20:
     every density field of the split run, the current j = j_a + j_b included
     and measured against its own maximum (the vortex carries real currents),
     within 1e-6 of the FP64 run; total energy within 1e-8.  j_a and j_b
@@ -175,7 +175,7 @@ def test_split_vs_fp64_trap_vortex_currents(torch):
     own-maximum ratio is the larger one."""
     say("case: split vs fp64, 32^3 trap + vortex, 1024 states, 40 steps, j against |j|_inf")
     from wslda_amd import problems, td
-    w = problems.config3(torch, precision=td.FP64, n_states=1024)
+    w = problems.config3(torch, precision=td.FP64, first=1024)
     p64 = w["params"]
     psp = td.Params.from_buffer_copy(p64)
     psp.precision = td.SPLIT

@@ -1,0 +1,67 @@
+"""Why the trapped-vortex workload starts from the uniform state set (oracle,
+CPU).  The round-1 start -- the lowest-|E| plane waves of the whole lattice
+times a Gaussian envelope in the trap -- is far from any stationary state of
+the functional: its energy runs away and the run ends in NaN within t eF ~ 1,
+identically with dt and dt/2 (so the cause is the continuous dynamics of
+that start, not the integrator or the precision).  The uniform solution's
+own state set quenched by the same trap and vortex stays finite, with an
+energy that changes by well under a percent over the same time."""
+import ctypes as C
+
+import numpy as np
+
+from oracle import pyoracle as po
+
+
+def run(w, dtfac, steps, every):
+    from wslda_amd import td  # noqa: F401  (host helpers only)
+    p = w["params"]
+    cfg = po.Cfg()
+    C.memmove(C.addressof(cfg), C.addressof(p), C.sizeof(cfg))
+    cfg.dt *= dtfac
+    P = po.Problem(cfg, w["wf"].numpy().copy(), w["fbeta"], pots=w["pots"].copy())
+    P.initial_measure()
+    out = []
+    for it in range(int(round(steps / dtfac))):
+        g = (it + 1) % int(round(every / dtfac)) == 0
+        P.step(it, grad_last=g)
+        if g:
+            out.append(P.energy(it + 1)[:7].sum())
+    return np.array(out)
+
+
+def enveloped_start(shape, n):
+    import torch
+    from wslda_amd import problems, td
+    nx, ny, nz = shape
+    p = problems._base_params(3, nx, ny, nz, 1.0, vext=td.gauss_vext([], omega=0.05))
+    u = problems.uniform_slda(p)
+    K, E, U, V = problems.plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf)
+    order = np.sort(np.argsort(np.abs(E), kind="stable")[:n])
+    K, E, U, V = K[order], E[order], U[order], V[order]
+    X = torch.arange(nx, dtype=torch.float64) - nx // 2
+    gx, gy, gz = torch.meshgrid(X, X, X, indexing="ij")
+    R = 0.3 * nx
+    env = torch.exp(-(gx ** 2 + gy ** 2 + gz ** 2) / (2 * R * R)).reshape(-1)
+    wf = problems.states_on_gpu(torch, p, K, U, V, device="cpu", envelope=env.to(torch.complex128))
+    p.mu_a, p.mu_b, p.ec = u.mu_a, u.mu_b, u.ec
+    N = nx * ny * nz
+    pots = np.zeros(12 * N)
+    pots[:N], pots[N:2 * N], pots[2 * N:4 * N:2], pots[4 * N:6 * N] = u.V_a, u.V_b, u.delta, 1.0
+    return dict(params=p, wf=wf, fbeta=np.array([td.fbeta(e, u.beta) for e in E]), pots=pots)
+
+
+def test_trap_starts():
+    import torch
+    from wslda_amd import problems, td
+    po.lib().tdwo_set_threads(8)
+    # round-1 start: the energy runs away (to NaN) the same way at dt and dt / 2
+    w = enveloped_start((12, 12, 12), 600)
+    e1, e2 = run(w, 1.0, 100, 50), run(w, 0.5, 100, 50)
+    assert abs(e1[1] - e1[0]) > 2.0 * abs(e1[0])
+    assert abs(e1[1] - e2[1]) < 0.05 * abs(e1[1] - e1[0])
+    # the workload's start: the uniform state set in the same trap, with the vortex
+    w = problems.config3(torch, precision=td.FP64, device="cpu", shape=(12, 12, 12))
+    e = run(w, 1.0, 200, 50)
+    assert np.isfinite(e).all()
+    assert np.abs(e - e[0]).max() < 0.02 * abs(e[0])

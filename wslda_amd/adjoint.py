@@ -9,7 +9,9 @@ the new history entry, the norms, <H> values, densities and potentials of
 every step), then the adjoint steps run backwards through the segment.
 
 The observable is a linear functional of the final densities,
-O = sum_r g(r) . (rho_a, rho_b, Re nu, Im nu)(r), given by the 4*N array g.
+O = sum_r g(r) . (rho_a, rho_b, Re nu, Im nu)(r), given by the 4*N array g,
+or a function of the final state with its own seed (state_observable), such
+as the current through a plane of the lattice (PlaneCurrent).
 """
 import ctypes as C
 import math
@@ -37,7 +39,7 @@ class AdjointProblem:
     recomputes only the others."""
 
     def __init__(self, params, wf0, pots0, fbeta, kt=None, cnt=None, nsteps=100, K=10, it0=0, dens_grad=None,
-                 device=0, taped_segments=1):
+                 device=0, taped_segments=1, state_observable=None):
         import torch
         self.torch = torch
         self.params = params
@@ -57,6 +59,10 @@ class AdjointProblem:
         self.taped = {}
         self._tps = []
         self.taped_segments = taped_segments
+        # state_observable(run) -> (O, dO/dy* as a [2n, N] tensor, paired by
+        # Re sum conj(seed) dy): an observable of the final state itself
+        self.state_observable = state_observable
+        self.fbeta = np.asarray(fbeta, dtype=np.float64)
 
     # -- parameters -------------------------------------------------------
     def set_vext(self, vext):
@@ -160,6 +166,8 @@ class AdjointProblem:
 
     def observable(self, R=None):
         R = R or self.run
+        if self.state_observable is not None:
+            return float(self.state_observable(R)[0])
         d = R.dens
         N = self.N
         g = self.dens_grad
@@ -187,7 +195,11 @@ class AdjointProblem:
             A.lf[j] = lf[j].data_ptr()
             A.w[j] = w[j].data_ptr()
         A.lpots, A.lgrid, A.theta_grad = lpots.data_ptr(), lgrid.data_ptr(), theta.data_ptr()
-        L = td.lib()
+        if self.state_observable is not None:
+            # dL/dy' of the last step's output state
+            ly.copy_(self.state_observable(self.run)[1])
+            torch.cuda.synchronize()
+        L = self.ctx.L
         last = True
         for (k0, y0, hist0, pots0) in reversed(self.ckpts):
             ks = min(K, self.nsteps - k0)
@@ -207,7 +219,7 @@ class AdjointProblem:
                 hist = [(tf[r - j] if r - j >= 0 else hist0[j - r - 1]) for j in range(1, h + 1)]
                 fp = (C.c_void_p * h)(*[t.data_ptr() for t in hist])
                 pots_prev = tgrid[r - 1, 3] if r > 0 else pots0
-                dg = self.dens_grad if (last and r == ks - 1) else None
+                dg = self.dens_grad if (last and r == ks - 1 and self.state_observable is None) else None
                 td.check(L.tdw_adj_backward_step(self.ctx.h, C.byref(A), self.it0 + k0 + r, _p(ty[r]), fp,
                                                  _p(ty[r + 1]), _p(tf[r]), C.byref(tapes[r]), _p(pots_prev),
                                                  _p(dg)), "tdw_adj_backward_step")
@@ -306,3 +318,73 @@ def disk_weights(params, center, radius):
     g[:N] = m
     g[N:2 * N] = m
     return g
+
+
+class PlaneCurrent:
+    """O = sum over the plane z = z_index of (j_a + j_b) . e_axis dA: the
+    particle current through a lattice plane normal to `axis` (0, 1, 2),
+    with the densities' currents j_a = sum_n f_n Im(u_n* d u_n),
+    j_b = -sum_n (1 - f_n) Im(v_n* d v_n) (the spectral first derivative with
+    the Nyquist mode zero, as calculate_densities takes it).  Its seed for
+    the adjoint, with O paired to a state change by Re sum conj(seed) dpsi:
+    u_n: -i f_n (g d u_n + d(g u_n)), v_n: +i (1 - f_n)(g d v_n + d(g v_n)),
+    g the plane's area weight (d is antisymmetric on the periodic lattice)."""
+
+    def __init__(self, ctx, fbeta, axis=2, index=None):
+        import torch
+        self.ctx = ctx
+        self.torch = torch
+        p = ctx.params
+        self.axis = axis
+        shape = (p.nx, p.ny, p.nz)
+        sp = (p.dx, p.dy, p.dz)
+        idx = shape[axis] // 2 if index is None else index
+        g = np.zeros(shape)
+        area = sp[(axis + 1) % 3] * sp[(axis + 2) % 3]
+        sl = [slice(None)] * 3
+        sl[axis] = idx
+        g[tuple(sl)] = area
+        dev = "cuda:%d" % ctx.device
+        self.g = torch.as_tensor(g.ravel()).to(dev)
+        self.f = torch.as_tensor(np.asarray(fbeta, dtype=np.float64)).to(dev)
+
+    def _d(self, a):
+        """Spectral first derivative along the axis of the rows of a."""
+        t = self.torch
+        outs = [t.zeros_like(a) for _ in range(4)]
+        self.ctx.compute_derivatives(a, outs[0], outs[1], outs[2], outs[3])
+        self.ctx.sync()
+        return outs[self.axis]
+
+    def __call__(self, run):
+        t = self.torch
+        y = run.wf
+        n = y.shape[0] // 2
+        dy = self._d(y)
+        gy = y * self.g[None, :]
+        dgy = self._d(gy)
+        f = self.f[:n, None]
+        fm = 1.0 - f
+        ja = (f * (y[:n].conj() * dy[:n]).imag).sum(dim=0)
+        jb = -(fm * (y[n:].conj() * dy[n:]).imag).sum(dim=0)
+        O = float(((ja + jb) * self.g).sum())
+        seed = t.empty_like(y)
+        seed[:n] = -1j * f * (self.g[None, :] * dy[:n] + dgy[:n])
+        seed[n:] = 1j * fm * (self.g[None, :] * dy[n:] + dgy[n:])
+        return O, seed
+
+
+def barrier_vext(theta, axis=2):
+    """A Gaussian barrier theta = (height, width) across the lattice centre,
+    normal to `axis`: height exp(-s^2 / (2 width^2))."""
+    h, w = float(theta[0]), float(theta[1])
+    ww = [0.0, 0.0, 0.0]
+    ww[axis] = -0.5 / (w * w)
+    return td.gauss_vext([(h, (0.0, 0.0, 0.0), tuple(ww))])
+
+
+def barrier_grad(theta, g, axis=2):
+    """Chain rule from the Gaussian-slot gradient g (MAX_GAUSS x 7) to
+    (height, width): the exponent coefficient is -1/(2 w^2)."""
+    w = float(theta[1])
+    return np.array([g[0, 0], g[0, 4 + axis] / (w ** 3)])

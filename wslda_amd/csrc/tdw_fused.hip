@@ -207,6 +207,8 @@ __global__ void f_norm_final(const double *__restrict__ part, int n, int per, do
 // that leaves few idle lanes when each thread owns E/NT points
 constexpr int nt_hyz(int E)
 {
+    // planes above 1024 points (the 32 x 96 tube section): 4 points per lane
+    if (E > 1024 && E % 4 == 0 && E / 4 <= 1024 && (E / 4) % 64 == 0) return E / 4;
     return E % 256 == 0 ? 256 : E % 192 == 0 ? 192 : E <= 64 ? 64 : E <= 128 ? 128 : E <= 192 ? 192 : 256;
 }
 
@@ -234,7 +236,7 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
             C v[R1];
 #pragma unroll
             for (int n1 = 0; n1 < R1; n1++) v[n1] = pa[(n1 * R2 + n2) * IS];
-            rf::small_dft<R1, -1, C>::run(v);
+            rf::dft_inplace<R1, -1, C>(v);
 #pragma unroll
             for (int k1 = 1; k1 < R1; k1++) v[k1] = cmul(v[k1], tw[(n2 * k1) % L]);
 #pragma unroll
@@ -250,14 +252,14 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
             C v[R2];
 #pragma unroll
             for (int n2 = 0; n2 < R2; n2++) v[n2] = pa[(k1 * R2 + n2) * IS];
-            rf::small_dft<R2, -1, C>::run(v);
+            rf::dft_inplace<R2, -1, C>(v);
 #pragma unroll
             for (int k2 = 0; k2 < R2; k2++) {
                 const int f = k1 + R1 * k2;
                 const int m = (f < L / 2) ? f : f - L;
                 v[k2] = v[k2] * (Rl)(-ks * (double)(m * m));
             }
-            rf::small_dft<R2, +1, C>::run(v);
+            rf::dft_inplace<R2, +1, C>(v);
 #pragma unroll
             for (int n2 = 0; n2 < R2; n2++) {
                 C w = tw[(n2 * k1) % L];
@@ -269,13 +271,13 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
             C v[R1];
 #pragma unroll
             for (int n = 0; n < R1; n++) v[n] = pa[n * IS];
-            rf::small_dft<R1, -1, C>::run(v);
+            rf::dft_inplace<R1, -1, C>(v);
 #pragma unroll
             for (int f = 0; f < R1; f++) {
                 const int m = (f < L / 2) ? f : f - L;
                 v[f] = v[f] * (Rl)(-ks * (double)(m * m));
             }
-            rf::small_dft<R1, +1, C>::run(v);
+            rf::dft_inplace<R1, +1, C>(v);
 #pragma unroll
             for (int n = 0; n < R1; n++) pa[n * IS] = v[n];
         }
@@ -289,7 +291,7 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
             C v[R1];
 #pragma unroll
             for (int k1 = 0; k1 < R1; k1++) v[k1] = pa[(k1 * R2 + n2) * IS];
-            rf::small_dft<R1, +1, C>::run(v);
+            rf::dft_inplace<R1, +1, C>(v);
 #pragma unroll
             for (int n1 = 0; n1 < R1; n1++) pa[(n1 * R2 + n2) * IS] = v[n1];
         }
@@ -492,6 +494,7 @@ template <bool SP> struct FusedFns {
         if (ny == 8 && nz == 8) { *h = f_hyz<8, 8, SP>; return true; }
         if (ny == 8 && nz == 12) { *h = f_hyz<8, 12, SP>; return true; }
         if (ny == 12 && nz == 6) { *h = f_hyz<12, 6, SP>; return true; }
+        if (ny == 32 && nz == 96) { *h = f_hyz<32, 96, SP>; return true; }
         return false;
     }
 };

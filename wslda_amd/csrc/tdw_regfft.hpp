@@ -208,8 +208,22 @@ template <int L, int SIGN, typename C> __device__ __forceinline__ void fft(C (&x
     for (int i = 0; i < L; i++) x[i] = y[i];
 }
 
-// L = R1 * R2 with both factors small DFTs (2, 3, 4, 8): the two factors of
-// the in-place transforms in LDS
+// in-place DFT of R values: the small DFTs for 1, 2, 3, 4, 8 points, the
+// recursive register FFT for other lengths (e.g. 12 = 4 * 3)
+template <int R, int SIGN, typename C> __device__ __forceinline__ void dft_inplace(C *v)
+{
+    if constexpr (R == 1 || R == 2 || R == 3 || R == 4 || R == 8) {
+        small_dft<R, SIGN, C>::run(v);
+    } else {
+        C y[R];
+        reg_fft<R, SIGN, C>::template run<0, 1, R>(*reinterpret_cast<C(*)[R]>(v), y);
+#pragma unroll
+        for (int i = 0; i < R; i++) v[i] = y[i];
+    }
+}
+
+// L = R1 * R2: the two factors of the in-place transforms in LDS, R1 a
+// small DFT (2, 3, 4, 8), R2 a small DFT or a register FFT (96 = 8 * 12)
 template <int L> struct fac2 {
     static constexpr int R1 = (L == 8 || L == 4 || L == 3 || L == 2) ? L
                             : (L % 8 == 0) ? 8 : (L % 4 == 0) ? 4 : (L % 3 == 0) ? 3 : 2;

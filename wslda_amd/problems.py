@@ -242,22 +242,40 @@ def config4(n_states=4096, seed=4, shape=(64, 64, 8), dx=1.0):
     return s
 
 
-def config3(torch, precision=td.SPLIT, n_states=24576, seed=3, device="cuda:0", shape=(32, 32, 32), dx=1.0,
-            omega_eF=0.1, envelope_frac=0.3, noise=1.0e-3):
+# Cutoff at which the 32^3 uniform solution's own state set (dx = 1) has
+# 24586 states, the shell count nearest 24576 (kc in [2.815, 2.821]).
+KC_32CUBED_24586 = 2.818
+
+
+def config3(torch, precision=td.SPLIT, n_states=None, seed=3, device="cuda:0", shape=(32, 32, 32), dx=1.0,
+            omega_eF=0.1, envelope_frac=0.3, noise=1.0e-6, first=None, kc=None):
     """3D SLDA in a harmonic trap with an imprinted vortex line: 32^3
-    lattice, trap 0.5 omega^2 r^2 with omega = 0.1 eF, the n_states lowest-|E|
-    plane-wave BdG states of the uniform solution multiplied by a Gaussian
-    envelope exp(-r^2 / (2 R^2)) (R = envelope_frac * L) with seeded
-    random-phase amplitude noise, and the phase exp(i phi) of a vortex line
-    along z imprinted on the u components (so Delta ~ exp(i phi))."""
+    lattice, trap 0.5 omega^2 r^2 with omega = 0.1 eF switched on at t = 0,
+    the uniform solution's own state set (every plane-wave BdG state with
+    -ec < E < ec: 24586 states at kc = KC_32CUBED_24586, the shell count
+    nearest 24576), the phase exp(i phi) of a vortex line along z imprinted
+    on the u components (so Delta ~ exp(i phi)), and seeded random-phase
+    noise whose amplitude follows a Gaussian envelope exp(-r^2 / (2 R^2))
+    (R = envelope_frac * L).  The start is the uniform stationary state
+    quenched by the trap and the vortex, so the run stays a physical
+    evolution for thousands of steps.  n_states, if given, must equal the
+    size of the set; first keeps only the first states (a slice for parity
+    tests)."""
     nx, ny, nz = shape
     eF = 0.5
     vext = td.gauss_vext([], omega=omega_eF * eF)
     p = _base_params(3, nx, ny, nz, dx, precision=precision, vext=vext)
-    u = uniform_slda(p)
-    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, np.inf)
-    order = np.sort(np.argsort(np.abs(E), kind="stable")[:n_states])
-    K, E, U, V = K[order], E[order], U[order], V[order]
+    if kc is None and shape == (32, 32, 32) and dx == 1.0:
+        kc = KC_32CUBED_24586
+    u = uniform_slda(p, kc=kc)
+    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, u.ec)
+    if len(E) != u.nwf:
+        raise td.TDWError("plane-wave set (%d) differs from the solver's state count (%d)" % (len(E), u.nwf))
+    if n_states is not None and n_states != len(E):
+        raise td.TDWError("the uniform solution at kc = %.6f has %d states below its cutoff, not %d"
+                          % (u.kc, len(E), n_states))
+    if first is not None:
+        K, E, U, V = K[:first], E[:first], U[:first], V[:first]
     X = (torch.arange(nx, dtype=torch.float64, device=device) - nx // 2) * dx
     Y = (torch.arange(ny, dtype=torch.float64, device=device) - ny // 2) * dx
     Z = (torch.arange(nz, dtype=torch.float64, device=device) - nz // 2) * dx
@@ -268,8 +286,7 @@ def config3(torch, precision=td.SPLIT, n_states=24576, seed=3, device="cuda:0", 
     # cell so that no lattice point sits on the core
     phi = torch.atan2(gy + 0.5 * dx, gx + 0.5 * dx).reshape(-1)
     phase = torch.polar(torch.ones_like(phi), phi)
-    wf = states_on_gpu(torch, p, K, U, V, device=device, envelope=env.to(torch.complex128),
-                       phase=phase)
+    wf = states_on_gpu(torch, p, K, U, V, device=device, phase=phase)
     if noise > 0:
         g = torch.Generator(device=device)
         g.manual_seed(seed)
@@ -290,6 +307,74 @@ def config3(torch, precision=td.SPLIT, n_states=24576, seed=3, device="cuda:0", 
     fb = np.array([td.fbeta(e, u.beta) for e in E])
     return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF, E=E,
                 name="3D trapped SLDA with a vortex line %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
+
+
+# Cutoff at which the uniform solution's own state set on the 32 x 32 x 96
+# tube (dx = 1) has 12062 states, the shell count nearest 12000
+# (kc in [1.566, 1.569]).
+KC_TUBE_12062 = 1.5675
+
+
+def tube_phase(nz, dz, width):
+    """Phase profile along the tube axis with a step of pi at the barrier
+    (z = 0) and the step back at the box edge z = +-L/2, each a tanh of the
+    given width, so that the profile is periodic: -pi/2 on the left half,
+    +pi/2 on the right half."""
+    L = nz * dz
+    z = (np.arange(nz) - nz // 2) * dz
+    return 0.5 * math.pi * (np.tanh(z / width) - np.tanh((z - L / 2) / width) - np.tanh((z + L / 2) / width))
+
+
+def config5(torch, precision=td.SPLIT, n_states=None, seed=5, device="cuda:0", shape=(32, 32, 96), dx=1.0,
+            barrier_eF=0.3, barrier_width_dx=3.0, phase_width_dx=2.0, noise=1.0e-6, first=None, kc=None):
+    """3D elongated-tube Josephson junction: 32 x 32 x 96 lattice (tube axis
+    z), a Gaussian barrier V_ext = V0 exp(-z^2 / (2 w^2)) across the tube with
+    V0 = 0.3 eF and w = 3 dx, on from t = 0, and a phase step of pi imprinted
+    on the u components across it (tube_phase), on the uniform solution's own
+    state set (12062 states at kc = KC_TUBE_12062, the shell count nearest
+    12000) with seeded random-phase noise.  n_states, if given, must equal
+    the size of the set; first keeps the first states of the set (one GPU's
+    share of the states, or a slice for parity tests)."""
+    nx, ny, nz = shape
+    eF = 0.5
+    w = barrier_width_dx * dx
+    vext = td.gauss_vext([(barrier_eF * eF, (0.0, 0.0, 0.0), (0.0, 0.0, -0.5 / (w * w)))])
+    p = _base_params(3, nx, ny, nz, dx, precision=precision, vext=vext)
+    if kc is None and shape == (32, 32, 96) and dx == 1.0:
+        kc = KC_TUBE_12062
+    u = uniform_slda(p, kc=kc)
+    K, E, U, V = plane_wave_list(p, u.mu_a, u.mu_b, u.V_a, u.V_b, u.delta, u.ec)
+    if len(E) != u.nwf:
+        raise td.TDWError("plane-wave set (%d) differs from the solver's state count (%d)" % (len(E), u.nwf))
+    if n_states is not None and n_states != len(E):
+        raise td.TDWError("the uniform solution at kc = %.6f has %d states below its cutoff, not %d"
+                          % (u.kc, len(E), n_states))
+    if first is not None:
+        K, E, U, V = K[:first], E[:first], U[:first], V[:first]
+    phz = torch.as_tensor(tube_phase(nz, dx, phase_width_dx * dx), device=device)
+    phase = torch.polar(torch.ones(nx * ny * nz, dtype=torch.float64, device=device),
+                        phz.repeat(nx * ny))
+    wf = states_on_gpu(torch, p, K, U, V, device=device, phase=phase)
+    if noise > 0:
+        g = torch.Generator(device=device)
+        g.manual_seed(seed)
+        vol = (nx * dx) * (ny * dx) * (nz * dx)
+        chunk = 2048
+        for a0 in range(0, wf.shape[0], chunk):
+            rows = wf[a0:a0 + chunk]
+            th = torch.rand(rows.shape, generator=g, dtype=torch.float64, device=device) * (2 * PI)
+            rows += torch.polar(torch.full_like(th, noise / math.sqrt(vol)), th)
+            del th
+    p.mu_a, p.mu_b, p.ec = u.mu_a, u.mu_b, u.ec
+    N = nx * ny * nz
+    pots = np.zeros(12 * N)
+    pots[:N] = u.V_a
+    pots[N:2 * N] = u.V_b
+    pots[2 * N:4 * N:2] = u.delta
+    pots[4 * N:6 * N] = 1.0
+    fb = np.array([td.fbeta(e, u.beta) for e in E])
+    return dict(params=p, wf=wf, fbeta=fb, kt=None, cnt=None, pots=pots, uniform=u, eF=eF, E=E,
+                name="3D tube Josephson junction %dx%dx%d, %d states" % (nx, ny, nz, len(E)))
 
 
 def config1(n_states=128, shape=(64, 4, 4), dx=1.0, dperp=2.5, kc=KC_1D_128, amp_eF=0.1, width_dx=4.0):
