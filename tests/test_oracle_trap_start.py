@@ -1,9 +1,9 @@
 """Why the trapped-vortex workload starts from the uniform state set (oracle,
-CPU).  The round-1 start -- the lowest-|E| plane waves of the whole lattice
-times a Gaussian envelope in the trap -- is far from any stationary state of
-the functional: its energy runs away and the run ends in NaN within t eF ~ 1,
-identically with dt and dt/2 (so the cause is the continuous dynamics of
-that start, not the integrator or the precision).  The uniform solution's
+CPU).  An enveloped plane-wave start -- the lowest-|E| plane waves of the
+whole lattice times a Gaussian envelope in the trap -- is far from any
+stationary state of the functional: its energy runs away and the run ends in
+NaN within t eF ~ 1, identically with dt and dt/2 (so the cause is the
+continuous dynamics of that start, not the integrator or the precision).  The uniform solution's
 own state set quenched by the same trap and vortex stays finite, with an
 energy that changes by well under a percent over the same time."""
 import ctypes as C
@@ -55,7 +55,8 @@ def test_trap_starts():
     import torch
     from wslda_amd import problems, td
     po.lib().tdwo_set_threads(8)
-    # round-1 start: the energy runs away (to NaN) the same way at dt and dt / 2
+    # the enveloped plane-wave start: the energy runs away (to NaN) the same
+    # way at dt and dt / 2
     w = enveloped_start((12, 12, 12), 600)
     e1, e2 = run(w, 1.0, 100, 50), run(w, 0.5, 100, 50)
     assert abs(e1[1] - e1[0]) > 2.0 * abs(e1[0])

@@ -207,29 +207,44 @@ __global__ void f_norm_final(const double *__restrict__ part, int n, int per, do
 // that leaves few idle lanes when each thread owns E/NT points
 constexpr int nt_hyz(int E)
 {
-    // planes above 1024 points (the 32 x 96 tube section): 4 points per lane
-    if (E > 1024 && E % 4 == 0 && E / 4 <= 1024 && (E / 4) % 64 == 0) return E / 4;
     return E % 256 == 0 ? 256 : E % 192 == 0 ? 192 : E <= 64 ? 64 : E <= 128 ? 128 : E <= 192 ? 192 : 256;
 }
 
-// In-place second derivative along one axis of P pencils held in LDS, for
+// In-place transforms along one axis of P pencils held in LDS, for
 // L = R1*R2: with n = n1*R2 + n2 and f = k1 + R1*k2,
 //   X[f] = sum_n2 W_R2^{n2 k2} W_L^{n2 k1} sum_n1 x[n1 R2 + n2] W_R1^{n1 k1},
 // stage 1 (per n2): R1-point DFT over n1, twiddle W_L^{n2 k1}, stored at k1*R2 + n2;
-// stage 2 (per k1): R2-point DFT over n2 -> X[k1 + R1 k2], times -ks*m(f)^2,
-//   inverse R2-point DFT and the conjugate twiddle, stored at k1*R2 + n2;
+// stage 2 (per k1): R2-point DFT over n2 -> X[k1 + R1 k2], times mul(m(f), q)
+//   for pencil q, inverse R2-point DFT and the conjugate twiddle, stored at
+//   k1*R2 + n2;
 // stage 3 (per n2): inverse R1-point DFT over k1, stored at n1*R2 + n2.
+// MODE 0 runs all three (with mul = -ks m^2 the second derivative); MODE 1
+// stops after the forward R2-point DFT and leaves X[k1 + R1 k2] at position
+// k1*R2 + k2; MODE 2 starts from that layout and runs the inverse half.
 // Every stage reads and writes the same positions, so one buffer suffices.
 // Consecutive lanes take consecutive pencils: with pencils one element or one
 // padded row apart, a wave's LDS accesses fall in distinct banks.
 
-template <int L, int P, int QD, int QH, int QL, int IS, int NT, typename C>
-__device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
+// frequency index m(f) in (-L/2, L/2] of the DFT output at position p (MODE 1 layout)
+template <int L> __host__ __device__ constexpr int axis_freq(int p)
+{
+    constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
+    const int f = R2 > 1 ? p / R2 + R1 * (p % R2) : p;
+    return f < L / 2 ? f : f - L;
+}
+
+struct KsMul {
+    double ks;
+    __device__ double operator()(int m, int) const { return -ks * (double)(m * m); }
+};
+
+template <int L, int P, int QD, int QH, int QL, int IS, int NT, int MODE, typename C, typename MUL>
+__device__ __forceinline__ void dft_axis(C *A, const C *tw, int tid, MUL mul)
 {
     typedef decltype(C().x) Rl;
     constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
     // stage 1
-    if constexpr (R2 > 1) {
+    if constexpr (R2 > 1 && MODE != 2) {
         for (int b = tid; b < P * R2; b += NT) {
             const int n2 = b / P, q = b - n2 * P;
             C *pa = A + (q / QD) * QH + (q % QD) * QL;
@@ -244,7 +259,7 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
         }
         __syncthreads();
     }
-    // stage 2 (for R2 == 1 this is the whole transform)
+    // stage 2 (for R2 == 1 the whole transform)
     for (int b = tid; b < P * R1; b += NT) {
         const int k1 = b / P, q = b - k1 * P;
         C *pa = A + (q / QD) * QH + (q % QD) * QL;
@@ -252,12 +267,19 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
             C v[R2];
 #pragma unroll
             for (int n2 = 0; n2 < R2; n2++) v[n2] = pa[(k1 * R2 + n2) * IS];
-            rf::dft_inplace<R2, -1, C>(v);
+            if constexpr (MODE != 2) {
+                rf::dft_inplace<R2, -1, C>(v);
+                if constexpr (MODE == 1) {
 #pragma unroll
-            for (int k2 = 0; k2 < R2; k2++) {
-                const int f = k1 + R1 * k2;
-                const int m = (f < L / 2) ? f : f - L;
-                v[k2] = v[k2] * (Rl)(-ks * (double)(m * m));
+                    for (int k2 = 0; k2 < R2; k2++) pa[(k1 * R2 + k2) * IS] = v[k2];
+                    continue;
+                }
+#pragma unroll
+                for (int k2 = 0; k2 < R2; k2++) {
+                    const int f = k1 + R1 * k2;
+                    const int m = (f < L / 2) ? f : f - L;
+                    v[k2] = v[k2] * (Rl)mul(m, q);
+                }
             }
             rf::dft_inplace<R2, +1, C>(v);
 #pragma unroll
@@ -271,20 +293,24 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
             C v[R1];
 #pragma unroll
             for (int n = 0; n < R1; n++) v[n] = pa[n * IS];
-            rf::dft_inplace<R1, -1, C>(v);
+            if constexpr (MODE != 2) {
+                rf::dft_inplace<R1, -1, C>(v);
+                if constexpr (MODE == 0) {
 #pragma unroll
-            for (int f = 0; f < R1; f++) {
-                const int m = (f < L / 2) ? f : f - L;
-                v[f] = v[f] * (Rl)(-ks * (double)(m * m));
+                    for (int f = 0; f < R1; f++) {
+                        const int m = (f < L / 2) ? f : f - L;
+                        v[f] = v[f] * (Rl)mul(m, q);
+                    }
+                }
             }
-            rf::dft_inplace<R1, +1, C>(v);
+            if constexpr (MODE != 1) rf::dft_inplace<R1, +1, C>(v);
 #pragma unroll
             for (int n = 0; n < R1; n++) pa[n * IS] = v[n];
         }
     }
     __syncthreads();
     // stage 3
-    if constexpr (R2 > 1) {
+    if constexpr (R2 > 1 && MODE != 1) {
         for (int b = tid; b < P * R2; b += NT) {
             const int n2 = b / P, q = b - n2 * P;
             C *pa = A + (q / QD) * QH + (q % QD) * QL;
@@ -297,6 +323,13 @@ __device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
         }
         __syncthreads();
     }
+}
+
+// second derivative along one axis, in place (ks = (2 pi / L_phys)^2 / L)
+template <int L, int P, int QD, int QH, int QL, int IS, int NT, typename C>
+__device__ __forceinline__ void lap_axis(C *A, const C *tw, int tid, double ks)
+{
+    dft_axis<L, P, QD, QH, QL, IS, NT, 0>(A, tw, tid, KsMul{ks});
 }
 
 struct HParams {
@@ -391,6 +424,115 @@ f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *
     if (tid == 0) part[blockIdx.x] = ssum;
 }
 
+// multiplier of the y pass of a 2D transform: -(k_y^2 + k_z^2) / (LY LZ),
+// with the z part per position of the z-transformed row in zt
+struct YZMul {
+    double ky;
+    const double *zt;
+    int lz;
+    __device__ double operator()(int m, int q) const { return -(ky * (double)(m * m) + zt[q % lz]); }
+};
+
+// yz pass for planes above 1024 points (the 32 x 96 tube section): D_y + D_z
+// as one 2D transform in place (forward along z; forward, scale and inverse
+// along y; inverse along z), NC components in LDS at a time.  Only the plane
+// values stay in registers (T is read when H psi is formed), so two or three
+// workgroups share a CU where the plane-pair kernel above fits one.
+template <int LY, int LZ, bool SP, int NC, int NT>
+__global__ void __launch_bounds__(NT)
+f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
+          const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
+          const hc_t<SP> *__restrict__ twy_g, const hc_t<SP> *__restrict__ twz_g, double kys, double kzs,
+          HParams hp, double *__restrict__ part)
+{
+    typedef hc_t<SP> C;
+    constexpr int E = LY * LZ;              // points of one plane
+    constexpr int EP = LY * (LZ + 1);       // one component's plane in LDS, row stride LZ + 1
+    constexpr int KM = E / NT;              // owned points per thread
+    static_assert(E % NT == 0 && (NC == 1 || NC == 2), "plane split");
+    __shared__ C twy[LY], twz[LZ];
+    __shared__ double zt[LZ];
+    __shared__ double red[NT / 64];
+    extern __shared__ double2 lds_raw[];
+    C *A = (C *)lds_raw;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
+    for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
+    // k_z^2 / (LY LZ) at each position of a z-transformed row
+    for (int k = tid; k < LZ; k += NT) {
+        const int m = axis_freq<LZ>(k);
+        zt[k] = kzs / LY * (double)(m * m);
+    }
+    const int a = blockIdx.x / NX;
+    const int x = blockIdx.x - a * NX;
+    // workgroup-uniform bases (scalar registers) and 32-bit offsets e
+    const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
+    const C *yu = y + gu, *yv = y + gv, *Tu = T + gu, *Tv = T + gv;
+    C *hu_o = hout + gu, *hv_o = hout + gv;
+    const double *Vab = pots + (long)x * E, *Vbb = pots + N + (long)x * E;
+    const double *Db = pots + 2 * N + 2L * x * E, *aab = pots + 4 * N + (long)x * E, *abb = pots + 5 * N + (long)x * E;
+    C Xu[KM], Xv[KM];
+#pragma unroll
+    for (int k = 0; k < KM; k++) {
+        const int e = tid + k * NT;
+        Xu[k] = yu[e];
+        Xv[k] = yv[e];
+    }
+    const double s = sc[a];
+    double ssum = 0.0;
+#pragma unroll
+    for (int c0 = 0; c0 < 2; c0 += NC) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+            if constexpr (NC == 2) {
+                A[pe] = Xu[k];
+                A[EP + pe] = Xv[k];
+            } else {
+                A[pe] = c0 ? Xv[k] : Xu[k];
+            }
+        }
+        __syncthreads();
+        // z pencils (c, iy): offset c*EP + iy*(LZ+1), stride 1; y pencils
+        // (c, iz): offset c*EP + iz, stride LZ+1
+        dft_axis<LZ, NC * LY, LY, EP, LZ + 1, 1, NT, 1>(A, twz, tid, KsMul{0.0});
+        dft_axis<LY, NC * LZ, LZ, EP, 1, LZ + 1, NT, 0>(A, twy, tid, YZMul{kys / LZ, zt, LZ});
+        dft_axis<LZ, NC * LY, LY, EP, LZ + 1, 1, NT, 2>(A, twz, tid, KsMul{0.0});
+        // H psi on owned points for the components in LDS (pointwise terms
+        // and the sums in FP64)
+#pragma unroll
+        for (int k = 0; k < KM; k++) {
+            const int e = tid + k * NT;
+            const int pe = (e / LZ) * (LZ + 1) + e % LZ;
+            const double2 pu = to64(Xu[k]) * s, pv = to64(Xv[k]) * s;
+            const double2 D = make_double2(Db[2 * e], Db[2 * e + 1]);
+#pragma unroll
+            for (int cc = 0; cc < NC; cc++) {
+                const int cmp = c0 + cc;
+                if (cmp == 0) {
+                    const double2 lu = to64(Tu[e] + A[cc * EP + pe]) * s;
+                    const double aa = hp.bdg ? -0.5 : aab[e] * -0.5;
+                    const double Va = Vab[e] - hp.mu_a;
+                    const double2 hu = pu * Va + cmul(pv, D) + lu * aa;
+                    hu_o[e] = cvt<C>(hu);
+                    ssum += cmulc(pu, hu).x;
+                } else {
+                    const double2 lv = to64(Tv[e] + A[cc * EP + pe]) * s;
+                    const double ab = hp.bdg ? -0.5 : abb[e] * -0.5;
+                    const double Vb = Vbb[e] - hp.mu_b;
+                    const double2 hv = pv * (-Vb) + cmulc(D, pu) - lv * ab;
+                    hv_o[e] = cvt<C>(hv);
+                    ssum += cmulc(pv, hv).x;
+                }
+            }
+        }
+    }
+    ssum = block_sum(ssum, red);
+    if (tid == 0) part[blockIdx.x] = ssum;
+}
+
 __global__ void f_e_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ e,
                           double *__restrict__ e2)
 {
@@ -457,6 +599,14 @@ void fused_release(tdw_ctx *c)
     c->d_sp = c->d_s2 = c->d_ep = nullptr;
 }
 
+// the big-plane yz pass: both components in LDS (32 x 97 planes: 49.7 KB in
+// FP32, 99 KB in FP64); 512 threads in FP32 (114 VGPRs: two workgroups per
+// CU), 1024 in FP64 (the registers allow one workgroup per CU either way)
+template <bool SP> struct HyzBig {
+    static constexpr int NC = 2;
+    static constexpr int NT = SP ? 512 : 1024;
+};
+
 template <bool SP> struct FusedFns {
     typedef hc_t<SP> HC;
     typedef void (*pred_fn)(double2 *, HC *, const HC *, const HC *, const HC *, const HC *, HC *, HC *,
@@ -494,16 +644,28 @@ template <bool SP> struct FusedFns {
         if (ny == 8 && nz == 8) { *h = f_hyz<8, 8, SP>; return true; }
         if (ny == 8 && nz == 12) { *h = f_hyz<8, 12, SP>; return true; }
         if (ny == 12 && nz == 6) { *h = f_hyz<12, 6, SP>; return true; }
-        if (ny == 32 && nz == 96) { *h = f_hyz<32, 96, SP>; return true; }
+        if (ny == 32 && nz == 96) { *h = f_hyz_big<32, 96, SP, HyzBig<SP>::NC, HyzBig<SP>::NT>; return true; }
         return false;
     }
 };
 
 static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
 
+static bool hyz_big(const tdw_ctx *c) { return c->ny * c->nz > 1024; }
+
+// threads of the yz pass
+static int hyz_nt(const tdw_ctx *c)
+{
+    if (!hyz_big(c)) return nt_hyz(c->ny * c->nz);
+    return split_mode(c) ? HyzBig<true>::NT : HyzBig<false>::NT;
+}
+
+// dynamic LDS of the yz pass: the components it holds at once
 static size_t hyz_lds(const tdw_ctx *c)
 {
-    return (split_mode(c) ? sizeof(float2) : sizeof(double2)) * 2 * (size_t)c->ny * (c->nz + 1);
+    const size_t plane = (split_mode(c) ? sizeof(float2) : sizeof(double2)) * (size_t)c->ny * (c->nz + 1);
+    if (!hyz_big(c)) return 2 * plane;
+    return (split_mode(c) ? HyzBig<true>::NC : HyzBig<false>::NC) * plane;
 }
 
 bool fused_supported(const tdw_ctx *c)
@@ -564,7 +726,7 @@ static int hyz(tdw_ctx *c, int n, const hc_t<SP> *y, const hc_t<SP> *T, hc_t<SP>
     // the norm partials are consumed already: reuse the partial buffer
     {
         TDW_TIMED(c, "f_hyz");
-        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(nt_hyz(c->ny * c->nz)), lds, c->stream, y, T, out, pots,
+        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(hyz_nt(c)), lds, c->stream, y, T, out, pots,
                            scale, n,
                            (long)c->N, c->nx, twy, twz, kys, kzs, hp, c->d_fpart);
         TDW_HIP(hipGetLastError());

@@ -146,6 +146,31 @@ template <int SIGN, typename C> struct small_dft<8, SIGN, C> {
     }
 };
 
+// 12 points as a prime-factor (Good-Thomas) transform, 12 = 4 * 3 with no
+// twiddles: input n = (3 n1 + 4 n2) mod 12, output k = (9 k1 + 4 k2) mod 12,
+// so W_12^{nk} = W_4^{n1 k1} W_3^{n2 k2}; four 3-point DFTs then three
+// 4-point DFTs, every index fixed at compile time (the values stay in the
+// registers of v).
+template <int SIGN, typename C> struct small_dft<12, SIGN, C> {
+    static __device__ __forceinline__ void run(C *v)
+    {
+        C a[4][3];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; n1++)
+#pragma unroll
+            for (int n2 = 0; n2 < 3; n2++) a[n1][n2] = v[(3 * n1 + 4 * n2) % 12];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; n1++) small_dft<3, SIGN, C>::run(a[n1]);
+#pragma unroll
+        for (int k2 = 0; k2 < 3; k2++) {
+            C b[4] = {a[0][k2], a[1][k2], a[2][k2], a[3][k2]};
+            small_dft<4, SIGN, C>::run(b);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) v[(9 * k1 + 4 * k2) % 12] = b[k1];
+        }
+    }
+};
+
 // DFT of length L of x[OFF + STRIDE*m], m < L, written in natural order to
 // y[0..L).  Recursion: L = R*M, sub-transforms of stride STRIDE*R.
 template <int L, int SIGN, typename C> struct reg_fft {
@@ -208,11 +233,11 @@ template <int L, int SIGN, typename C> __device__ __forceinline__ void fft(C (&x
     for (int i = 0; i < L; i++) x[i] = y[i];
 }
 
-// in-place DFT of R values: the small DFTs for 1, 2, 3, 4, 8 points, the
-// recursive register FFT for other lengths (e.g. 12 = 4 * 3)
+// in-place DFT of R values: the small DFTs for 1, 2, 3, 4, 8 and 12 points,
+// the recursive register FFT for other lengths (e.g. 6 = 2 * 3)
 template <int R, int SIGN, typename C> __device__ __forceinline__ void dft_inplace(C *v)
 {
-    if constexpr (R == 1 || R == 2 || R == 3 || R == 4 || R == 8) {
+    if constexpr (R == 1 || R == 2 || R == 3 || R == 4 || R == 8 || R == 12) {
         small_dft<R, SIGN, C>::run(v);
     } else {
         C y[R];
