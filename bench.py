@@ -270,12 +270,14 @@ def cpu_baseline(g, args):
                        % ("x".join(str(x) for x in (cfg.nx, cfg.ny, cfg.nz)), ns, n, k, threads, dt))
 
 
-# HBM bytes per launch of the dominant kernel, from the FETCH_SIZE and
-# WRITE_SIZE counter runs of profiles/fused_step_24cubed_16384states.md
-# (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000 by the copy-kernel calibration),
-# keyed by (kernel, lattice, states, scheme, precision)
+# HBM bytes per launch of the dominant kernel, keyed by (kernel, lattice,
+# states, scheme, precision): FETCH_SIZE x 2 + WRITE_SIZE (KiB x 1024) of one
+# launch, from the counter runs of `bench.py --steps 4 --warmup 1 --no-cpu
+# --meas-every 1000 --copy-calib-mib 4096 --trap-states 0` (the 4096 MiB copy
+# reads FETCH_SIZE = half its bytes and WRITE_SIZE = all of them)
 MEASURED_TRAFFIC = {
-    ("f_pred", (24, 24, 24), 16384, 55, "fp64"): 72486234112,
+    ("f_pred", (24, 24, 24), 16308, 55, "fp64"): (21137743 * 2 + 28183519) * 1024,
+    ("f_pred", (24, 24, 24), 16308, 55, "split"): (12331069 * 2 + 17614946) * 1024,
 }
 
 
@@ -333,7 +335,7 @@ def spin_current_errors(d, ref, N):
     return out
 
 
-def run_adjoint(args):
+def run_adjoint(args, calibrate=True):
     """2D SLDA 64x64 with 4096 states, two Gaussian beams, particle number in
     a disk of radius 10 dx after args.adj_steps steps, checkpoints every
     args.adj_k steps: time of the plain forward run, of the checkpointed
@@ -380,7 +382,8 @@ def run_adjoint(args):
     err = max(abs(v["adjoint"] - v["fd"]) for v in fd.values()) / gmax if fd else None
     n = prob.n
     prob.close()
-    copy_calibration(args)
+    if calibrate:
+        copy_calibration(args)
     return {
         "metric": "adjoint gradient cost / forward cost, and gradient error vs central finite differences",
         "value": (t_fwd_ck + t_bwd) / t_fwd, "unit": "x forward", "higher_is_better": False,
@@ -511,6 +514,76 @@ def run_tube(args):
     }
 
 
+def run_tube_adjoint(args):
+    """--tube-adjoint: the tube's barrier-current gradient on one GPU's
+    memory share (--tadj-states, default 360 states), --adj-steps steps
+    (default 1000) with checkpoints every --adj-k (default 40): the forward
+    run in split precision (checkpoints with FP32 history), the last segment
+    taped and the backward sweep in FP64.  Times: the plain split forward,
+    the fused FP64 forward, the checkpointed forward and the backward; the
+    peak device memory."""
+    import numpy as np
+    import torch
+    from wslda_amd import adjoint, problems, td
+    nst = args.tadj_states
+    w = problems.config5(torch, precision=td.FP64, first=nst)
+    theta = np.array([0.3 * w["eF"], 3.0])
+
+    def problem(fp, taped):
+        p = td.Params.from_buffer_copy(w["params"])
+        prob = adjoint.AdjointProblem(p, w["wf"], w["pots"], w["fbeta"], nsteps=args.adj_steps, K=args.adj_k,
+                                      taped_segments=taped, forward_precision=fp)
+        prob.state_observable = adjoint.PlaneCurrent(prob.ctx, w["fbeta"], axis=2)
+        prob.set_vext(adjoint.barrier_vext(theta))
+        return prob
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.time()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.time() - t0, r
+
+    # the fused FP64 forward, for reference
+    prob = problem(td.FP64, 1)
+    prob.forward(keep_checkpoints=False)
+    t_f64, _ = timed(lambda: prob.forward(keep_checkpoints=False))
+    O64 = prob.observable()
+    prob.close()
+    del prob
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    prob = problem(td.SPLIT, args.adj_taped)
+    prob.forward(keep_checkpoints=False)     # warm-up
+    t_fwd, _ = timed(lambda: prob.forward(keep_checkpoints=False))
+    O_split = prob.observable()
+    t_ck, _ = timed(lambda: prob.forward(keep_checkpoints=True))
+    O = prob.observable()
+    ckpt_bytes = sum(t.numel() * t.element_size() for c in prob.ckpts for t in [c[1]] + list(c[2]) + [c[3]])
+    t_bwd, gslot = timed(prob.backward)
+    grad = adjoint.barrier_grad(theta, gslot)
+    peak = torch.cuda.max_memory_allocated()
+    n = prob.n
+    prob.close()
+    S = 2 * 16 * 32 * 32 * 96
+    return {
+        "metric": "adjoint gradient cost / split forward cost, tube Josephson junction",
+        "value": (t_ck + t_bwd) / t_fwd, "unit": "x split forward", "higher_is_better": False,
+        "n_gpus": 1, "dtype": "f32-operator/f64-state forward, f64 backward", "data": "synthetic",
+        "config": {"workload": "td-SLDA 3D tube 32x32x96, barrier 0.3 eF width 3 dx, pi phase step, %d of the "
+                               "uniform solution's 12062 states (one GPU's memory share), AB5AM5, %d steps, "
+                               "checkpoints every %d, last %d segment(s) taped; observable: the current "
+                               "through the barrier plane at the final time" % (n, args.adj_steps, args.adj_k,
+                                                                               args.adj_taped)},
+        "split_forward_s": t_fwd, "fp64_fused_forward_s": t_f64, "forward_with_checkpoints_s": t_ck,
+        "backward_s": t_bwd, "over_fp64_forward": (t_ck + t_bwd) / t_f64,
+        "observable": O, "observable_split_forward": O_split, "observable_fp64_forward": O64,
+        "grad_height_width": [float(x) for x in grad],
+        "checkpoint_GB": ckpt_bytes / 1e9, "checkpoint_state_arrays": ckpt_bytes / (S * n),
+        "peak_device_GB": peak / 1e9,
+    }
+
+
 def run_trap(args):
     """--trap: the trapped-vortex split / FP64 pair as its own JSON line."""
     t = trap_pair(args)
@@ -600,6 +673,15 @@ def main():
                     help="the tube Josephson junction (32x32x96) in split precision and FP64 as its own line")
     ap.add_argument("--tube-states", type=int, default=6031,
                     help="states of the tube runs (one GPU's share of its 12062; -1: all)")
+    ap.add_argument("--line-tube-steps", type=int, default=50,
+                    help="timed steps of the tube pair in the default line (0: leave it out)")
+    ap.add_argument("--line-adjoint", type=int, default=1,
+                    help="the 2D adjoint workload (cost and FD check) in the default line (0: leave it out)")
+    ap.add_argument("--tube-adjoint", action="store_true",
+                    help="instead of the step benchmark: the tube's barrier-current gradient (split forward, "
+                         "FP64 backward) on one GPU's memory share; --adj-steps, --adj-k and --adj-taped apply "
+                         "(defaults here 1000, 40, 1)")
+    ap.add_argument("--tadj-states", type=int, default=360, help="states of --tube-adjoint")
     ap.add_argument("--aslda", action="store_true",
                     help="instead of the step benchmark: the ASLDA step on the 3D uniform workload (24^3)")
     ap.add_argument("--sldae", action="store_true",
@@ -619,6 +701,15 @@ def main():
         return
     if args.tube:
         print(json.dumps(run_tube(args)))
+        return
+    if args.tube_adjoint:
+        if "--adj-steps" not in sys.argv:
+            args.adj_steps = 1000
+        if "--adj-k" not in sys.argv:
+            args.adj_k = 40
+        if "--adj-taped" not in sys.argv:
+            args.adj_taped = 1
+        print(json.dumps(run_tube_adjoint(args)))
         return
     if args.aslda:
         print(json.dumps(run_aslda(args)))
@@ -679,6 +770,21 @@ def main():
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         res["trap_vortex"] = trap_pair(args)
+    if args.mode == "both" and args.line_tube_steps > 0:
+        import copy
+        import torch
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        a2 = copy.copy(args)
+        a2.trap_steps = args.line_tube_steps
+        res["tube"] = tube_pair(a2)
+    if args.mode == "both" and args.line_adjoint:
+        import torch
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        adj = run_adjoint(args, calibrate=False)
+        res["adjoint"] = {k: adj[k] for k in ("config", "value", "unit", "forward_s", "forward_with_checkpoints_s",
+                                              "backward_s", "fd_check", "max_rel_err_vs_fd")}
     nb = copy_calibration(args)
     if nb:
         res["copy_calib_bytes"] = nb

@@ -124,3 +124,35 @@ def test_tube_barrier_gradient_vs_finite_differences(torch):
     assert abs(O) > 0.0
     assert err <= 1e-5
     prob.close()
+
+
+def test_tube_fused_and_split_forward_gradient(torch):
+    """The same barrier gradient with the untaped segments of the forward
+    run on the fused step: FP64 (against the modular forward, 1e-8) and
+    split precision with FP32-history checkpoints (against the modular FP64
+    gradient, 1e-5; the backward sweep is FP64 in all three)."""
+    say("case: tube 32x32x96, 64 states, 24 steps, K = 6: modular / fused FP64 / split forward, adjoint gradient")
+    from wslda_amd import adjoint, problems, td
+    w = problems.config5(torch, precision=td.FP64, first=64)
+    theta = np.array([0.3 * w["eF"], 3.0])
+    res = {}
+    for name, fp in (("modular", None), ("fused", td.FP64), ("split", td.SPLIT)):
+        p = td.Params.from_buffer_copy(w["params"])
+        prob = adjoint.AdjointProblem(p, w["wf"], w["pots"], w["fbeta"], nsteps=24, K=6, taped_segments=1,
+                                      forward_precision=fp)
+        prob.state_observable = adjoint.PlaneCurrent(prob.ctx, w["fbeta"], axis=2)
+        prob.set_vext(adjoint.barrier_vext(theta))
+        prob.forward()
+        if fp == td.SPLIT:
+            # the checkpoints keep the FP32 history
+            assert all(t.dtype == torch.complex64 for t in prob.ckpts[1][2])
+        O = prob.observable()
+        res[name] = (O, adjoint.barrier_grad(theta, prob.backward()))
+        prob.close()
+        say("  %-8s O = %.10e  grad %s" % (name, O, np.array2string(res[name][1], precision=9)))
+    O0, g0 = res["modular"]
+    for name, tol in (("fused", 1e-8), ("split", 1e-5)):
+        O, g = res[name]
+        eo, eg = abs(O - O0) / abs(O0), np.abs(g - g0).max() / np.abs(g0).max()
+        say("  %s vs modular: O %.2e, gradient %.2e" % (name, eo, eg))
+        assert eo <= tol and eg <= tol, (name, eo, eg)

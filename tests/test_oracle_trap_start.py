@@ -1,11 +1,12 @@
 """Why the trapped-vortex workload starts from the uniform state set (oracle,
-CPU).  An enveloped plane-wave start -- the lowest-|E| plane waves of the
-whole lattice times a Gaussian envelope in the trap -- is far from any
-stationary state of the functional: its energy runs away and the run ends in
-NaN within t eF ~ 1, identically with dt and dt/2 (so the cause is the
-continuous dynamics of that start, not the integrator or the precision).  The uniform solution's
-own state set quenched by the same trap and vortex stays finite, with an
-energy that changes by well under a percent over the same time."""
+CPU, 12^3 lattice, 300 states each).  An enveloped plane-wave start -- the
+lowest-|E| plane waves of the whole lattice times a Gaussian envelope in the
+trap -- is far from any stationary state of the functional: its energy grows
+without bound (by more than 30 % between steps 50 and 100; larger such starts
+reach NaN within t eF ~ 1), identically with dt and dt/2, so the cause is the
+continuous dynamics of that start, not the integrator or the precision.  The
+uniform solution's states quenched by the same trap and vortex keep their
+energy within 1 % over the same time."""
 import ctypes as C
 
 import numpy as np
@@ -55,14 +56,15 @@ def test_trap_starts():
     import torch
     from wslda_amd import problems, td
     po.lib().tdwo_set_threads(8)
-    # the enveloped plane-wave start: the energy runs away (to NaN) the same
-    # way at dt and dt / 2
-    w = enveloped_start((12, 12, 12), 600)
+    # the enveloped plane-wave start: the energy runs away the same way at
+    # dt and dt / 2
+    w = enveloped_start((12, 12, 12), 300)
     e1, e2 = run(w, 1.0, 100, 50), run(w, 0.5, 100, 50)
-    assert abs(e1[1] - e1[0]) > 2.0 * abs(e1[0])
+    assert abs(e1[1] - e1[0]) > 0.3 * abs(e1[0])
     assert abs(e1[1] - e2[1]) < 0.05 * abs(e1[1] - e1[0])
-    # the workload's start: the uniform state set in the same trap, with the vortex
-    w = problems.config3(torch, precision=td.FP64, device="cpu", shape=(12, 12, 12))
-    e = run(w, 1.0, 200, 50)
+    # the workload's start: the uniform state set (its first 300 states) in
+    # the same trap, with the vortex
+    w = problems.config3(torch, precision=td.FP64, device="cpu", shape=(12, 12, 12), first=300)
+    e = run(w, 1.0, 100, 50)
     assert np.isfinite(e).all()
-    assert np.abs(e - e[0]).max() < 0.02 * abs(e[0])
+    assert np.abs(e - e[0]).max() < 0.01 * abs(e[0])

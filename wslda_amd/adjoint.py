@@ -12,6 +12,12 @@ The observable is a linear functional of the final densities,
 O = sum_r g(r) . (rho_a, rho_b, Re nu, Im nu)(r), given by the 4*N array g,
 or a function of the final state with its own seed (state_observable), such
 as the current through a plane of the lattice (PlaneCurrent).
+
+On 3D lattices the segments that are not taped may run the fused step
+instead (forward_precision = td.FP64 or td.SPLIT).  In split precision the
+checkpoints keep the FP32 history (3.5 state arrays instead of 6); every
+taped segment, the recompute and the backward sweep stay FP64 on the modular
+step, so the gradient is the FP64 map's, linearised along the split run.
 """
 import ctypes as C
 import math
@@ -39,13 +45,25 @@ class AdjointProblem:
     recomputes only the others."""
 
     def __init__(self, params, wf0, pots0, fbeta, kt=None, cnt=None, nsteps=100, K=10, it0=0, dens_grad=None,
-                 device=0, taped_segments=1, state_observable=None):
+                 device=0, taped_segments=1, state_observable=None, forward_precision=None):
         import torch
         self.torch = torch
         self.params = params
         self.n = wf0.shape[0] // 2
         self.ctx = td.TDContext(params, self.n, device)
         self.ctx.set_states(fbeta, kt, cnt)
+        # forward_precision (td.FP64 / td.SPLIT): the segments that are not
+        # taped run the fused step on a context of their own
+        self.ctx_fwd = None
+        if forward_precision is not None:
+            pf = td.Params.from_buffer_copy(params)
+            pf.precision = forward_precision
+            self.ctx_fwd = td.TDContext(pf, self.n, device)
+            self.ctx_fwd.set_states(fbeta, kt, cnt)
+            if self.ctx_fwd.L.tdw_fused_supported(self.ctx_fwd.h) != 1:
+                self.ctx_fwd.close()
+                raise td.TDWError("the fused step does not run this lattice / functional: "
+                                  "no fused forward pass for the adjoint")
         self.dev = "cuda:%d" % device
         self.wf0 = torch.as_tensor(wf0).to(self.dev).to(torch.complex128).clone()
         self.pots0 = torch.as_tensor(np.ascontiguousarray(pots0, dtype=np.float64)).to(self.dev)
@@ -68,14 +86,27 @@ class AdjointProblem:
     def set_vext(self, vext):
         self.params.vext = vext
         self.ctx.set_params(self.params)
+        if self.ctx_fwd is not None:
+            pf = self.ctx_fwd.params
+            pf.vext = vext
+            self.ctx_fwd.set_params(pf)
 
     # -- forward ----------------------------------------------------------
     def _new_run(self, wf, pots, hist=None):
+        """An FP64 modular run from (y, potentials, history); FP32 history
+        entries (a split checkpoint) are widened."""
         R = td.TDRun(self.ctx, wf.clone(), pots.clone().cpu().numpy())
         if hist is not None:
             for j in range(self.h):
                 R.f[j].copy_(hist[j])
         return R
+
+    def _fwd_run(self):
+        """The run of the segments that are not taped: the fused step on
+        ctx_fwd when a forward precision was given, else the modular step."""
+        if self.ctx_fwd is None:
+            return self._new_run(self.wf0, self.pots0)
+        return td.TDRun(self.ctx_fwd, self.wf0.clone(), self.pots0.clone().cpu().numpy(), fused=True)
 
     def _tape(self, i=0):
         """Tape buffer i: one K-step segment (allocated on first use)."""
@@ -142,25 +173,37 @@ class AdjointProblem:
         """Runs nsteps steps; returns the run (final y, densities, potentials).
         With checkpoints, the last taped_segments segments run with the taped
         step, so the backward sweep does not recompute them."""
-        R = self._new_run(self.wf0, self.pots0)
-        R.initial_measure()
+        # release the previous run and checkpoints before allocating anew
+        self.run = None
         self.ckpts = []
         self.taped = {}
+        R = self._fwd_run()
+        R.initial_measure()
         K = self.K
         nseg = (self.nsteps + K - 1) // K
         first_taped = max(0, nseg - self.taped_segments)
         for k in range(self.nsteps):
             seg, r = divmod(k, K)
             if keep_checkpoints and r == 0:
+                if R.fused:
+                    R.finalize()
                 self.ckpts.append((k, R.wf.clone(), [t.clone() for t in R.history()[:self.h]], R.pots.clone()))
             if keep_checkpoints and seg >= first_taped:
+                if R.ctx is not self.ctx:
+                    # the taped segments run the FP64 modular step from the
+                    # checkpoint just taken
+                    _, y0, hist0, pots0 = self.ckpts[-1]
+                    del R
+                    R = self._new_run(y0, pots0, hist0)
                 self._taped_step(R, k, r, seg - first_taped)
                 if r == K - 1 or k == self.nsteps - 1:
                     self.taped[seg] = seg - first_taped
                     self._restore(R)
             else:
                 R.step(self.it0 + k, grad_last=False)
-        self.ctx.sync()
+        if R.fused:
+            R.finalize()
+        R.ctx.sync()
         self.run = R
         return R
 
@@ -204,6 +247,8 @@ class AdjointProblem:
         for (k0, y0, hist0, pots0) in reversed(self.ckpts):
             ks = min(K, self.nsteps - k0)
             seg = k0 // K
+            # a split checkpoint's FP32 history, widened for this segment
+            hist0 = [t if t.dtype == torch.complex128 else t.to(torch.complex128) for t in hist0]
             if seg in self.taped:
                 ti = self.taped.pop(seg)
             else:
@@ -233,6 +278,8 @@ class AdjointProblem:
         self.run = None
         self._tps = []
         self.ctx.close()
+        if self.ctx_fwd is not None:
+            self.ctx_fwd.close()
 
 
 def beams_vext(theta, switch=None):
