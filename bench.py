@@ -69,16 +69,16 @@ def design_units(name, scheme, split=False):
     h = nhist(scheme)
     if split:
         return {
-            "f_pred": 1 + 0.5 * h + 1 + 0.5 + 0.5 + 0.5,   # y, f_1..f_h; c, y_p (FP32), T, f_1 fix-up
+            "f_pred": 1 + 0.5 * h + 1 + 0.5 + 0.5,         # y, f_1..f_h; c, y_p (FP32), f_1 fix-up
             "f_corr": 1 + 0.5 + 0.5 + 1 + 0.5 + 0.5,       # c, H y_p, y_p (FP32); y, T, y (FP32)
-            "f_hyz": 0.5 + 0.5 + 0.5,                       # y (FP32), T; H y
+            "f_hyz": (1.0 + 1.5) / 2,                       # y_p (FP32); H | y (FP32), T; f_new
             "densities": 0.5,                               # y (FP32)
             "f_fixup": 2,
         }.get(name, 0)
     return {
-        "f_pred": (h + 1) + 4,   # read y, f_1..f_h; write y_p, c, T (= D_x y_p), f_1 (pending <H> fix-up)
-        "f_corr": 5,             # read y_p, H y_p, c; write y, T
-        "f_hyz": 3,              # read y, T; write H y
+        "f_pred": (h + 1) + 3,   # read y, f_1..f_h; write y_p, c, f_1 (pending <H> fix-up)
+        "f_corr": 5,             # read y_p, H y_p, c; write y, T (= D_x y)
+        "f_hyz": (2 + 3) / 2,    # per launch, the two of a step: y_p; H | y, T; f_new
         "densities": 1,          # read y
         "f_fixup": 2,
     }.get(name, 0)

@@ -30,6 +30,7 @@
 #include "tdw_physics.hpp"
 #include "tdw_regfft.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace tdw {
@@ -89,11 +90,16 @@ __device__ __forceinline__ void put_norm_part(double nrm, bool wave, long P, lon
 // predictor x-pass
 //   plain: y <- y_p (FP64), fold <- c = y_k - i dt sum_j a_j f_j (FP64)
 //   split: y <- c (FP64),   fold <- y_p (FP32 copy)
+// No D_x of the prediction is stored: the corrector applies the x part of
+// the kinetic term itself, and the x part of <psi_p|H psi_p> comes from here
+// by Parseval, sum_x conj(y) (-1/2 D_x) y = 1/2 kscale sum_m m^2 |Y_m|^2
+// (u rows; the v rows carry the opposite sign), unscaled by the state's
+// normalisation (xpart, laid out as the norm partials).
 // ---------------------------------------------------------------------------
 template <int LX, int SCH, bool SP>
 __global__ void __launch_bounds__(FT)
 f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP> *f3, const hc_t<SP> *f4,
-       const hc_t<SP> *f5, hc_t<SP> *fold, hc_t<SP> *__restrict__ T, const double *__restrict__ sk,
+       const hc_t<SP> *f5, hc_t<SP> *fold, double *__restrict__ xpart, const double *__restrict__ sk,
        const double *__restrict__ ek, int pending, int n, long N, int S, double dt, double kscale, int wave,
        double *__restrict__ part)
 {
@@ -101,7 +107,7 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
     const long P = (long)blockIdx.x * FT + threadIdx.x;
     const long Ptot = 2L * n * S;
     const bool act = P < Ptot;
-    double nrm = 0.0;
+    double nrm = 0.0, xk = 0.0;
     if (act) {
         const long a2 = P / S, j = P - a2 * S;
         const int a = (int)(a2 < n ? a2 : a2 - n);
@@ -134,11 +140,16 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
             r[i] = cvt<HC>(yp);
             nrm += cnorm(yp);
         }
-        dxx_inplace<LX>(r, kscale);
+        rf::fft<LX, -1, HC>(r);
 #pragma unroll
-        for (int i = 0; i < LX; i++) T[base + (long)i * S] = r[i];
+        for (int m = 0; m < LX; m++) {
+            const int mm = (m < LX / 2) ? m : m - LX;
+            xk += (double)(mm * mm) * cnorm(to64(r[m]));
+        }
+        xk *= (a2 < n ? 0.5 : -0.5) * kscale;
     }
     put_norm_part(nrm, wave != 0, P, Ptot, part);
+    put_norm_part(xk, wave != 0, P, Ptot, xpart);
 }
 
 // ---------------------------------------------------------------------------
@@ -146,6 +157,11 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
 //   plain: y holds y_p, cin c;  split: y holds c, cin the FP32 y_p, and the
 //   FP32 copy of the new y goes to y32 (H's buffer: each element of hin is
 //   read by the thread that then writes it)
+// hin holds H psi_p without the x part of the kinetic term (the yz pass of
+// the prediction reads no D_x); that part and the <H> subtraction act on the
+// lane's pencil of y_p at once: (-/+ 1/2 D_x - <H>) y_p (u / v rows) is one
+// multiplier, sigma kscale m^2 / 2 - <H> / LX, between a forward and an
+// inverse transform.  Then (H - <H>) psi_p = hin + s (that pencil).
 // ---------------------------------------------------------------------------
 template <int LX, bool SP>
 __global__ void __launch_bounds__(FT)
@@ -154,6 +170,7 @@ f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict_
        int S, double dta0, double kscale, int wave, double *__restrict__ part, float2 *y32)
 {
     typedef hc_t<SP> HC;
+    typedef decltype(HC().x) R;
     const long P = (long)blockIdx.x * FT + threadIdx.x;
     const long Ptot = 2L * n * S;
     const bool act = P < Ptot;
@@ -162,20 +179,44 @@ f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict_
         const long a2 = P / S, j = P - a2 * S;
         const int a = (int)(a2 < n ? a2 : a2 - n);
         const double s = sp_[a], e = ep_[a];
+        const double sig = a2 < n ? 0.5 : -0.5;
         const long base = a2 * N + j;
         HC r[LX];
 #pragma unroll
         for (int i = 0; i < LX; i++) {
-            const long idx = base + (long)i * S;
-            const double2 yp = SP ? to64(cin[idx]) : y[idx];
+            if constexpr (SP) r[i] = cin[base + (long)i * S];   // the FP32 copy of y_p
+            else r[i] = y[base + (long)i * S];                  // y_p
+        }
+        rf::fft<LX, -1, HC>(r);
+        __builtin_amdgcn_sched_barrier(0);
+        const double ka = sig * kscale, kb = -e / LX;
+#pragma unroll
+        for (int m = 0; m < LX; m++) {
+            const int mm = (m < LX / 2) ? m : m - LX;
+            r[m] = r[m] * (R)fma(ka, (double)(mm * mm), kb);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        rf::fft<LX, +1, HC>(r);
+        // keep the loop's loads out of the transforms, and form the loop's
+        // addresses anew rather than holding the preload's across them
+        // (registers)
+        __builtin_amdgcn_sched_barrier(0);
+        long lb = base;
+        int ls = S;
+        asm volatile("" : "+v"(lb), "+s"(ls));
+#pragma unroll
+        for (int i = 0; i < LX; i++) {
+            const long idx = lb + (long)i * ls;
             const double2 cc = SP ? y[idx] : to64(cin[idx]);
-            const double2 psi = yp * s;
-            const double2 fp = to64(hin[idx]) - psi * e;        // (H - <H>) psi_p
+            const double2 fp = to64(hin[idx]) + to64(r[i]) * s;   // (H - <H>) psi_p
             const double2 yn = imul(fp, -dta0) + cc;
             y[idx] = yn;
             r[i] = cvt<HC>(yn);
             if constexpr (SP) y32[idx] = to32(yn);
             nrm += cnorm(yn);
+            // bound how far the scheduler lifts the next points' loads
+            // (registers: r[] is live across the loop)
+            if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
         }
         dxx_inplace<LX>(r, kscale);
 #pragma unroll
@@ -337,7 +378,10 @@ struct HParams {
     int bdg;
 };
 
-template <int LY, int LZ, bool SP, int NT = nt_hyz(LY * LZ)>
+// TX: add T (D_x of the plane, from the x-pass) to the transverse terms;
+// without it the output lacks the x part of the kinetic term (the
+// prediction's yz pass: the corrector adds that part itself)
+template <int LY, int LZ, bool SP, bool TX, int NT = nt_hyz(LY * LZ)>
 __global__ void __launch_bounds__(NT)
 f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
       const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
@@ -366,8 +410,13 @@ f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *
         if (e < E) {
             Xu[k] = y[gu + e];
             Xv[k] = y[gv + e];
-            Ou[k] = T[gu + e];
-            Ov[k] = T[gv + e];
+            if constexpr (TX) {
+                Ou[k] = T[gu + e];
+                Ov[k] = T[gv + e];
+            } else {
+                Ou[k] = cvt<C>(make_double2(0.0, 0.0));
+                Ov[k] = cvt<C>(make_double2(0.0, 0.0));
+            }
         }
     }
     // z axis: pencils (c, iy), offset c*EP + iy*(LZ+1), stride 1; y axis:
@@ -438,7 +487,7 @@ struct YZMul {
 // along y; inverse along z), NC components in LDS at a time.  Only the plane
 // values stay in registers (T is read when H psi is formed), so two or three
 // workgroups share a CU where the plane-pair kernel above fits one.
-template <int LY, int LZ, bool SP, int NC, int NT>
+template <int LY, int LZ, bool SP, bool TX, int NC, int NT>
 __global__ void __launch_bounds__(NT)
 f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
           const double *__restrict__ pots, const double *__restrict__ sc, int n, long N, int NX,
@@ -512,14 +561,14 @@ f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<S
             for (int cc = 0; cc < NC; cc++) {
                 const int cmp = c0 + cc;
                 if (cmp == 0) {
-                    const double2 lu = to64(Tu[e] + A[cc * EP + pe]) * s;
+                    const double2 lu = to64(TX ? Tu[e] + A[cc * EP + pe] : A[cc * EP + pe]) * s;
                     const double aa = hp.bdg ? -0.5 : aab[e] * -0.5;
                     const double Va = Vab[e] - hp.mu_a;
                     const double2 hu = pu * Va + cmul(pv, D) + lu * aa;
                     hu_o[e] = cvt<C>(hu);
                     ssum += cmulc(pu, hu).x;
                 } else {
-                    const double2 lv = to64(Tv[e] + A[cc * EP + pe]) * s;
+                    const double2 lv = to64(TX ? Tv[e] + A[cc * EP + pe] : A[cc * EP + pe]) * s;
                     const double ab = hp.bdg ? -0.5 : abb[e] * -0.5;
                     const double Vb = Vbb[e] - hp.mu_b;
                     const double2 hv = pv * (-Vb) + cmulc(D, pu) - lv * ab;
@@ -533,13 +582,24 @@ f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<S
     if (tid == 0) part[blockIdx.x] = ssum;
 }
 
+// <H> per state from the yz pass's partials (per per state); with xpart,
+// plus the x part of the kinetic term from the predictor (xper entries per
+// component, unscaled: times s2 = s^2 of the state)
 __global__ void f_e_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ e,
-                          double *__restrict__ e2)
+                          double *__restrict__ e2, const double *__restrict__ xpart, int xper,
+                          const double *__restrict__ s2)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     double s = 0.0;
     for (int k = 0; k < per; k++) s += part[(long)a * per + k];
+    if (xpart) {
+        double x = 0.0;
+        const double *pu = xpart + (long)a * xper, *pv = xpart + ((long)n + a) * xper;
+        for (int k = 0; k < xper; k++) x += pu[k];
+        for (int k = 0; k < xper; k++) x += pv[k];
+        s += x * s2[a];
+    }
     e[a] = s * dv;
     if (e2) e2[a] = s * dv;
 }
@@ -609,7 +669,7 @@ template <bool SP> struct HyzBig {
 
 template <bool SP> struct FusedFns {
     typedef hc_t<SP> HC;
-    typedef void (*pred_fn)(double2 *, HC *, const HC *, const HC *, const HC *, const HC *, HC *, HC *,
+    typedef void (*pred_fn)(double2 *, HC *, const HC *, const HC *, const HC *, const HC *, HC *, double *,
                             const double *, const double *, int, int, long, int, double, double, int, double *);
     typedef void (*corr_fn)(double2 *, const HC *, const HC *, HC *, const double *, const double *, int, long, int,
                             double, double, int, double *, float2 *);
@@ -635,18 +695,22 @@ template <bool SP> struct FusedFns {
         }
     }
 
-    static bool hyz(int ny, int nz, hyz_fn *h)
+    template <bool TX> static bool hyz_t(int ny, int nz, hyz_fn *h)
     {
-        if (ny == 12 && nz == 12) { *h = f_hyz<12, 12, SP>; return true; }
-        if (ny == 24 && nz == 24) { *h = f_hyz<24, 24, SP>; return true; }
-        if (ny == 32 && nz == 32) { *h = f_hyz<32, 32, SP>; return true; }
-        if (ny == 16 && nz == 16) { *h = f_hyz<16, 16, SP>; return true; }
-        if (ny == 8 && nz == 8) { *h = f_hyz<8, 8, SP>; return true; }
-        if (ny == 8 && nz == 12) { *h = f_hyz<8, 12, SP>; return true; }
-        if (ny == 12 && nz == 6) { *h = f_hyz<12, 6, SP>; return true; }
-        if (ny == 32 && nz == 96) { *h = f_hyz_big<32, 96, SP, HyzBig<SP>::NC, HyzBig<SP>::NT>; return true; }
+        if (ny == 12 && nz == 12) { *h = f_hyz<12, 12, SP, TX>; return true; }
+        if (ny == 24 && nz == 24) { *h = f_hyz<24, 24, SP, TX>; return true; }
+        if (ny == 32 && nz == 32) { *h = f_hyz<32, 32, SP, TX>; return true; }
+        if (ny == 16 && nz == 16) { *h = f_hyz<16, 16, SP, TX>; return true; }
+        if (ny == 8 && nz == 8) { *h = f_hyz<8, 8, SP, TX>; return true; }
+        if (ny == 8 && nz == 12) { *h = f_hyz<8, 12, SP, TX>; return true; }
+        if (ny == 12 && nz == 6) { *h = f_hyz<12, 6, SP, TX>; return true; }
+        if (ny == 32 && nz == 96) { *h = f_hyz_big<32, 96, SP, TX, HyzBig<SP>::NC, HyzBig<SP>::NT>; return true; }
         return false;
     }
+
+    // tx: the pass adds T (the corrected state's); without it, the
+    // prediction's pass
+    static bool hyz(int ny, int nz, bool tx, hyz_fn *h) { return tx ? hyz_t<true>(ny, nz, h) : hyz_t<false>(ny, nz, h); }
 };
 
 static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
@@ -678,7 +742,7 @@ bool fused_supported(const tdw_ctx *c)
     FusedFns<false>::corr_fn cf;
     FusedFns<false>::hyz_fn h;
     if (!FusedFns<false>::xpass(c->nx, c->p.scheme, &p, &cf)) return false;
-    if (!FusedFns<false>::hyz(c->ny, c->nz, &h)) return false;
+    if (!FusedFns<false>::hyz(c->ny, c->nz, true, &h)) return false;
     return hyz_lds(c) <= (size_t)c->max_lds;
 }
 
@@ -710,15 +774,19 @@ static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave
     return (int)hipGetLastError();
 }
 
+// The yz pass and <H>.  With xpart (the prediction's pass) T is not read:
+// the x part of the kinetic term enters <H> from the predictor's partials
+// (xper per component, times s2) and H psi from the corrector.
 template <bool SP>
 static int hyz(tdw_ctx *c, int n, const hc_t<SP> *y, const hc_t<SP> *T, hc_t<SP> *out, const double *pots,
-               const double *scale, double *e_out, double *e_out2)
+               const double *scale, double *e_out, double *e_out2, const double *xpart = nullptr, int xper = 0,
+               const double *s2 = nullptr)
 {
     HParams hp{c->p.mu_a, c->p.mu_b, c->p.functional == TDW_BDG};
     const double kys = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
     const double kzs = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
     typename FusedFns<SP>::hyz_fn h;
-    FusedFns<SP>::hyz(c->ny, c->nz, &h);
+    FusedFns<SP>::hyz(c->ny, c->nz, xpart == nullptr, &h);
     const size_t lds = hyz_lds(c);
     const hc_t<SP> *twy, *twz;
     if constexpr (SP) { twy = c->d_tw32[1]; twz = c->d_tw32[2]; }
@@ -733,7 +801,7 @@ static int hyz(tdw_ctx *c, int n, const hc_t<SP> *y, const hc_t<SP> *T, hc_t<SP>
     }
     TDW_TIMED(c, "f_e");
     hipLaunchKernelGGL(f_e_final, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_fpart, n, c->nx,
-                       dvol_norm(c), e_out, e_out2);
+                       dvol_norm(c), e_out, e_out2, xpart, xper, s2);
     return (int)hipGetLastError();
 }
 
@@ -745,15 +813,19 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     const int S = c->S;
     const int wave = (S % 64) == 0;
     TDW_TRY(ensure_state_vecs(c));
-    TDW_TRY(ensure_f(c, wave ? (size_t)2 * n * (S / 64) : (size_t)2 * n * S));
-    TDW_TRY(ensure_f(c, (size_t)n * c->nx));
+    // partial sums: the norm partials (then the yz pass's <H> partials) at
+    // the start of d_fpart, the predictor's x-kinetic partials behind both
+    const int per = wave ? S / 64 : S;
+    const size_t xoff = std::max((size_t)2 * n * per, (size_t)n * c->nx);
+    TDW_TRY(ensure_f(c, xoff + (size_t)2 * n * per));
+    double *xpart = c->d_fpart + xoff;
     typename FusedFns<SP>::pred_fn fp;
     typename FusedFns<SP>::corr_fn fc;
     FusedFns<SP>::xpass(c->nx, c->p.scheme, &fp, &fc);
     const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
     const long nblk = (2L * n * S + FT - 1) / FT;
-    // T (D_x of the x-pass output) lives in ddx, H of the prediction in lap;
-    // in the split form both hold FP32 values
+    // T (D_x of the corrected state) lives in ddx, H of the prediction in
+    // lap; in the split form both hold FP32 values
     double2 *y = (double2 *)s->wf;
     HC *T = (HC *)s->ddx, *H = (HC *)s->lap;
     HC **f = (HC **)s->f;
@@ -762,8 +834,8 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     // ---- predictor ----
     {
         TDW_TIMED(c, "f_pred");
-        hipLaunchKernelGGL(fp, dim3(nblk), dim3(FT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], f[h - 1], T,
-                           c->d_scale, c->d_qpe, c->pending, n, N, S, c->p.dt, kxs, wave, c->d_fpart);
+        hipLaunchKernelGGL(fp, dim3(nblk), dim3(FT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], f[h - 1],
+                           xpart, c->d_scale, c->d_qpe, c->pending, n, N, S, c->p.dt, kxs, wave, c->d_fpart);
         TDW_HIP(hipGetLastError());
     }
     c->pending = 0;
@@ -780,7 +852,7 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     TDW_TRY(hkf_fields(c, s->pots, true));
     const double *hp = h_view(c, it + 1, s->pots);
     if (!hp) return TDW_EINVAL;
-    TDW_TRY(hyz<SP>(c, n, yp, T, H, hp, c->d_sp, c->d_ep, nullptr));
+    TDW_TRY(hyz<SP>(c, n, yp, T, H, hp, c->d_sp, c->d_ep, nullptr, xpart, per, c->d_s2));
 
     // ---- corrector ----
     const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;

@@ -146,30 +146,45 @@ template <int SIGN, typename C> struct small_dft<8, SIGN, C> {
     }
 };
 
-// 12 points as a prime-factor (Good-Thomas) transform, 12 = 4 * 3 with no
-// twiddles: input n = (3 n1 + 4 n2) mod 12, output k = (9 k1 + 4 k2) mod 12,
-// so W_12^{nk} = W_4^{n1 k1} W_3^{n2 k2}; four 3-point DFTs then three
-// 4-point DFTs, every index fixed at compile time (the values stay in the
-// registers of v).
-template <int SIGN, typename C> struct small_dft<12, SIGN, C> {
+// Prime-factor (Good-Thomas) transform of N = N1 * N2 points, gcd(N1, N2) =
+// 1, with no twiddles: input n = (N2 n1 + N1 n2) mod N, output
+// k = (N2 i2 k1 + N1 i1 k2) mod N with i2 = N2^-1 mod N1, i1 = N1^-1 mod N2,
+// so W_N^{nk} = W_N1^{n1 k1} W_N2^{n2 k2}; N1 DFTs of N2 points, then N2 of
+// N1 points, every index fixed at compile time, so the values stay in the
+// registers of v (12 = 4 * 3, 24 = 8 * 3).
+constexpr int mod_inverse(int a, int m)
+{
+    for (int x = 1; x < m; x++)
+        if ((a * x) % m == 1) return x;
+    return 0;
+}
+
+template <int N1, int N2, int SIGN, typename C> struct pfa_dft {
+    static constexpr int N = N1 * N2;
+    static constexpr int I2 = mod_inverse(N2 % N1, N1), I1 = mod_inverse(N1 % N2, N2);
     static __device__ __forceinline__ void run(C *v)
     {
-        C a[4][3];
+        C a[N1][N2];
 #pragma unroll
-        for (int n1 = 0; n1 < 4; n1++)
+        for (int n1 = 0; n1 < N1; n1++)
 #pragma unroll
-            for (int n2 = 0; n2 < 3; n2++) a[n1][n2] = v[(3 * n1 + 4 * n2) % 12];
+            for (int n2 = 0; n2 < N2; n2++) a[n1][n2] = v[(N2 * n1 + N1 * n2) % N];
 #pragma unroll
-        for (int n1 = 0; n1 < 4; n1++) small_dft<3, SIGN, C>::run(a[n1]);
+        for (int n1 = 0; n1 < N1; n1++) small_dft<N2, SIGN, C>::run(a[n1]);
 #pragma unroll
-        for (int k2 = 0; k2 < 3; k2++) {
-            C b[4] = {a[0][k2], a[1][k2], a[2][k2], a[3][k2]};
-            small_dft<4, SIGN, C>::run(b);
+        for (int k2 = 0; k2 < N2; k2++) {
+            C b[N1];
 #pragma unroll
-            for (int k1 = 0; k1 < 4; k1++) v[(9 * k1 + 4 * k2) % 12] = b[k1];
+            for (int n1 = 0; n1 < N1; n1++) b[n1] = a[n1][k2];
+            small_dft<N1, SIGN, C>::run(b);
+#pragma unroll
+            for (int k1 = 0; k1 < N1; k1++) v[(N2 * I2 * k1 + N1 * I1 * k2) % N] = b[k1];
         }
     }
 };
+
+template <int SIGN, typename C> struct small_dft<12, SIGN, C> : pfa_dft<4, 3, SIGN, C> {};
+template <int SIGN, typename C> struct small_dft<24, SIGN, C> : pfa_dft<8, 3, SIGN, C> {};
 
 // DFT of length L of x[OFF + STRIDE*m], m < L, written in natural order to
 // y[0..L).  Recursion: L = R*M, sub-transforms of stride STRIDE*R.
@@ -227,10 +242,15 @@ template <int L, int SIGN, typename C> struct reg_fft {
 // in-place transform of a register array
 template <int L, int SIGN, typename C> __device__ __forceinline__ void fft(C (&x)[L])
 {
-    C y[L];
-    reg_fft<L, SIGN, C>::template run<0, 1, L>(x, y);
+    if constexpr (L == 12 || L == 24) {
+        // in place, no twiddles (fewer values live at once)
+        small_dft<L, SIGN, C>::run(x);
+    } else {
+        C y[L];
+        reg_fft<L, SIGN, C>::template run<0, 1, L>(x, y);
 #pragma unroll
-    for (int i = 0; i < L; i++) x[i] = y[i];
+        for (int i = 0; i < L; i++) x[i] = y[i];
+    }
 }
 
 // in-place DFT of R values: the small DFTs for 1, 2, 3, 4, 8 and 12 points,
