@@ -276,8 +276,8 @@ def cpu_baseline(g, args):
 # --meas-every 1000 --copy-calib-mib 4096 --trap-states 0` (the 4096 MiB copy
 # reads FETCH_SIZE = half its bytes and WRITE_SIZE = all of them)
 MEASURED_TRAFFIC = {
-    ("f_pred", (24, 24, 24), 16308, 55, "fp64"): (21137743 * 2 + 28183519) * 1024,
-    ("f_pred", (24, 24, 24), 16308, 55, "split"): (12331069 * 2 + 17614946) * 1024,
+    ("f_pred", (24, 24, 24), 16308, 55, "fp64"): (21137708 * 2 + 21139931) * 1024,
+    ("f_pred", (24, 24, 24), 16308, 55, "split"): (12331066 * 2 + 14094705) * 1024,
 }
 
 

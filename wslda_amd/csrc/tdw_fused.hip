@@ -3,16 +3,21 @@
 // The reference step (pca.c:1401-1570) touches the state arrays ~57 times per
 // step.  Here one step is six passes:
 //   f_pred  (x-pencils, one lane per pencil, FFT in registers): ABM predictor
-//           (tdwslda_ode_integrator.h:26-135) + D_x of the prediction + norm
-//           partials; also applies the pending normalisation of y and the
-//           pending <H> subtraction of the newest history entry.
+//           (tdwslda_ode_integrator.h:26-135) + norm partials + the x part
+//           of <psi_p|H psi_p> (Parseval); also applies the pending
+//           normalisation of y and the pending <H> subtraction of the newest
+//           history entry.
 //   dens    densities of the prediction, normalisation folded into the
 //           occupation weights (f*s^2).
-//   f_hyz   (yz-planes, Stockham in LDS): D_y + D_z + T, then H psi
-//           (pca_kernels.cu:233-391) and the <psi|H psi> partials.
-//   f_corr  (x-pencils): AM corrector y = c - i dt a0 (H psi - <H> psi)
-//           (:138-220) + D_x + norm partials.
-//   dens, f_hyz again: H of the corrected state into the newest history slot.
+//   f_hyz   (yz-planes, two-factor FFT in LDS): D_y + D_z, then H psi
+//           without its x kinetic part (pca_kernels.cu:233-391) and the
+//           <psi|H psi> partials.
+//   f_corr  (x-pencils): the x kinetic part and the <H> subtraction applied
+//           to the prediction's pencil in Fourier space, AM corrector
+//           y = c - i dt a0 (H psi - <H> psi) (:138-220), D_x of the new y
+//           (T) + norm partials.
+//   dens, f_hyz again (now adding T): H of the corrected state into the
+//   newest history slot.
 // Normalisation (pca_kernels.cu:955-976) and the <H> subtraction
 // (pca_kernels.cu:806-832) are carried as per-state scalars (s = 1/sqrt(norm),
 // e = <H>) into the next pass instead of extra passes over the state: the
@@ -482,11 +487,12 @@ struct YZMul {
     __device__ double operator()(int m, int q) const { return -(ky * (double)(m * m) + zt[q % lz]); }
 };
 
-// yz pass for planes above 1024 points (the 32 x 96 tube section): D_y + D_z
-// as one 2D transform in place (forward along z; forward, scale and inverse
-// along y; inverse along z), NC components in LDS at a time.  Only the plane
-// values stay in registers (T is read when H psi is formed), so two or three
-// workgroups share a CU where the plane-pair kernel above fits one.
+// yz pass with D_y + D_z as one 2D transform in place (forward along z;
+// forward, scale and inverse along y; inverse along z), NC components in LDS
+// at a time.  Only the plane values stay in registers (T is read when H psi
+// is formed), so more workgroups share a CU than with the per-axis form
+// above: planes above 1024 points (the 32 x 96 tube section, where that form
+// fits one workgroup per CU) and the FP64 24^2 and 32^2 planes.
 template <int LY, int LZ, bool SP, bool TX, int NC, int NT>
 __global__ void __launch_bounds__(NT)
 f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
@@ -698,8 +704,18 @@ template <bool SP> struct FusedFns {
     template <bool TX> static bool hyz_t(int ny, int nz, hyz_fn *h)
     {
         if (ny == 12 && nz == 12) { *h = f_hyz<12, 12, SP, TX>; return true; }
-        if (ny == 24 && nz == 24) { *h = f_hyz<24, 24, SP, TX>; return true; }
-        if (ny == 32 && nz == 32) { *h = f_hyz<32, 32, SP, TX>; return true; }
+        // FP64 24^2 and 32^2 planes: the in-place 2D transform (90 / 106
+        // VGPRs against 110 / 141 for the per-axis form: one more wave per SIMD)
+        if (ny == 24 && nz == 24) {
+            if constexpr (SP) *h = f_hyz<24, 24, SP, TX>;
+            else *h = f_hyz_big<24, 24, SP, TX, 2, nt_hyz(24 * 24)>;
+            return true;
+        }
+        if (ny == 32 && nz == 32) {
+            if constexpr (SP) *h = f_hyz<32, 32, SP, TX>;
+            else *h = f_hyz_big<32, 32, SP, TX, 2, nt_hyz(32 * 32)>;
+            return true;
+        }
         if (ny == 16 && nz == 16) { *h = f_hyz<16, 16, SP, TX>; return true; }
         if (ny == 8 && nz == 8) { *h = f_hyz<8, 8, SP, TX>; return true; }
         if (ny == 8 && nz == 12) { *h = f_hyz<8, 12, SP, TX>; return true; }
@@ -715,6 +731,8 @@ template <bool SP> struct FusedFns {
 
 static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
 
+// planes above 1024 points (the tube section) run f_hyz_big with its own
+// thread counts; FP64 24^2 and 32^2 planes run it with nt_hyz's
 static bool hyz_big(const tdw_ctx *c) { return c->ny * c->nz > 1024; }
 
 // threads of the yz pass
