@@ -71,6 +71,13 @@ def test_td_gauss_2d_first_rows():
     run_and_compare(2, "ss0-2d", 11)
 
 
+def test_td_gauss_3d_first_row():
+    """The 3D run (12^3 lattice, 1781 states) through its first measurement
+    (286 AB5AM5 steps, the gradient step at its end): row 0 and row 1 at
+    all printed digits (about two minutes of CPU)."""
+    run_and_compare(3, "ss0-3d", 1)
+
+
 @pytest.mark.slow
 def test_td_gauss_3d_first_rows():
     run_and_compare(3, "ss0-3d", 2)

@@ -672,6 +672,7 @@ static int adj_forward(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_out, dou
     // the densities of a non-gradient step do not read the Laplacian, and 2D
     // lattices with a plane kernel form it inside the H application
     const bool plane = h2d_supported(c);
+    const bool fused = !plane && fused_supported(c);
     if (y_out) {
         TDW_TRY(abm1_to(c, n, CD2(s->wf), D2(y_out), cbuf, D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]),
                         D2(s->f[4])));
@@ -681,18 +682,22 @@ static int adj_forward(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_out, dou
                         D2(s->f[4])));
     }
     const double2 *flast = cbuf ? cbuf : CD2(s->f[k - 1]);
+    // 3D lattices of the fused step: H through its x and yz passes, with
+    // ddy as the D_x scratch (the Laplacian array is then not formed)
+    double2 *hT = (fused && s->ddy) ? D2(s->ddy) : nullptr;
     TDW_TRY(normalize(c, n, D2(s->wf), t->scal));
-    if (!plane) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    if (!plane && !hT) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
     TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
                       s->dens, 0));
     TDW_TRY(copy_d(c, s->dens, t->dens_p, 12 * N));
     TDW_TRY(potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(copy_d(c, s->pots, t->pots_p, 12 * N));
     if (plane) TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->lap), s->pots, c->d_kt, nullptr, t->scal + n));
+    else if (hT) TDW_TRY(fused_apply_h(c, n, D2(s->wf), D2(s->lap), hT, s->pots, t->scal + n));
     else TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->lap), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + n));
     TDW_TRY(abm4(c, n, CD2(s->lap), D2(s->wf), flast));
     TDW_TRY(normalize(c, n, D2(s->wf), t->scal + 2 * n));
-    if (!plane) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
+    if (!plane && !hT) TDW_TRY(laplace(c, 2 * n, D2(s->wf), D2(s->lap)));
     TDW_TRY(densities(c, n, D2(s->wf), nullptr, nullptr, nullptr, D2(s->lap), c->d_fbeta, nullptr, c->d_kt, c->d_cnt,
                       s->dens, 0));
     TDW_TRY(copy_d(c, s->dens, t->dens_c, 12 * N));
@@ -703,6 +708,8 @@ static int adj_forward(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_out, dou
     s->f[0] = tmp;
     if (plane)
         TDW_TRY(apply_h_plane(c, it + 1, n, D2(s->wf), D2(s->f[0]), s->pots, c->d_kt, nullptr, t->scal + 3 * n));
+    else if (hT)
+        TDW_TRY(fused_apply_h(c, n, D2(s->wf), D2(s->f[0]), hT, s->pots, t->scal + 3 * n));
     else
         TDW_TRY(apply_h(c, it + 1, n, D2(s->wf), D2(s->f[0]), CD2(s->lap), s->pots, c->d_kt, nullptr, t->scal + 3 * n));
     TDW_HIP(hipMemcpyAsync(qpe, t->scal + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
@@ -728,13 +735,15 @@ extern "C" int tdw_adj_forward_step_to(tdw_ctx *c, tdw_state *s, int it, tdw_com
     if (!c || !s || !t || !y_next || !f_next || !s->ddx || s->n > c->nwf || t->n != s->n) return TDW_EINVAL;
     if (!adj_supported(c)) return TDW_EINVAL;
     const int h = abm_coef(c->p.scheme).h;
-    // every buffer the step writes must differ from those it reads
-    const tdw_complex *bufs[4] = {y_next, f_next, s->ddx, s->lap};
-    for (int i = 0; i < 4; i++) {
+    // every buffer the step writes must differ from those it reads (ddy:
+    // the D_x scratch of 3D lattices with the fused passes)
+    const tdw_complex *bufs[5] = {y_next, f_next, s->ddx, s->lap, s->ddy};
+    const int nb = s->ddy ? 5 : 4;
+    for (int i = 0; i < nb; i++) {
         if (bufs[i] == s->wf) return TDW_EINVAL;
         for (int j = 0; j < h; j++)
             if (bufs[i] == s->f[j]) return TDW_EINVAL;
-        for (int j = i + 1; j < 4; j++)
+        for (int j = i + 1; j < nb; j++)
             if (bufs[i] == bufs[j]) return TDW_EINVAL;
     }
     return adj_forward(c, s, it, y_next, D2(s->ddx), f_next, t);
@@ -765,11 +774,15 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
 
     // history refresh f_new = H(U') y' - e' y'  (mu = dL/df_new)
     const bool plane = h2d_supported(c);
+    const bool fused = !plane && fused_supported(c);
     if (plane) TDW_TRY(apply_h_only_plane(c, n, MU, W1, t->pots_c, c->d_kt, CD2(y1), dv, sv1));
     else {
         TDW_TRY(sdot(c, n, MU, CD2(y1), dv, sv1));
-        TDW_TRY(laplace(c, 2 * n, MU, W2));
-        TDW_TRY(apply_h_only(c, n, MU, W1, W2, t->pots_c, c->d_kt));
+        if (fused) TDW_TRY(fused_apply_h(c, n, MU, W1, W2, t->pots_c, nullptr));
+        else {
+            TDW_TRY(laplace(c, 2 * n, MU, W2));
+            TDW_TRY(apply_h_only(c, n, MU, W1, W2, t->pots_c, c->d_kt));
+        }
     }
     TDW_TRY(cross_upd(c, n, LY, LY, W1, CD2(fnew), CD2(y1), MU, CD2(y1), sv1, e_n, e_n, A->lpots, lUc));
     // U' = Phi(D', U_p, V_ext)
@@ -804,6 +817,7 @@ extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const
     }
     // F_p = H(U_p) yh_p - e_p yh_p
     if (plane) TDW_TRY(apply_h_only_plane(c, n, W0, W1, t->pots_p, c->d_kt));
+    else if (fused) TDW_TRY(fused_apply_h(c, n, W0, W1, W2, t->pots_p, nullptr));
     else {
         TDW_TRY(laplace(c, 2 * n, W0, W2));
         TDW_TRY(apply_h_only(c, n, W0, W1, W2, t->pots_p, c->d_kt));

@@ -453,8 +453,9 @@ f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *
             }
         }
     }
-    // H psi on owned points (pointwise terms and the sums in FP64)
-    const double s = sc[a];
+    // H psi on owned points (pointwise terms and the sums in FP64); sc null:
+    // the input is the state itself (scale 1)
+    const double s = sc ? sc[a] : 1.0;
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; k++) {
@@ -533,7 +534,7 @@ f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<S
         Xu[k] = yu[e];
         Xv[k] = yv[e];
     }
-    const double s = sc[a];
+    const double s = sc ? sc[a] : 1.0;
     double ssum = 0.0;
 #pragma unroll
     for (int c0 = 0; c0 < 2; c0 += NC) {
@@ -586,6 +587,34 @@ f_hyz_big(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<S
     }
     ssum = block_sum(ssum, red);
     if (tid == 0) part[blockIdx.x] = ssum;
+}
+
+// T = D_x in (x-pencils, one lane per pencil): the x-pass of an H
+// application to arbitrary vectors (the adjoint's)
+template <int LX>
+__global__ void __launch_bounds__(FT)
+f_dx(const double2 *__restrict__ in, double2 *__restrict__ T, int n, long N, int S, double kscale)
+{
+    const long P = (long)blockIdx.x * FT + threadIdx.x;
+    if (P >= 2L * n * S) return;
+    const long a2 = P / S, j = P - a2 * S;
+    const long base = a2 * N + j;
+    double2 r[LX];
+#pragma unroll
+    for (int i = 0; i < LX; i++) r[i] = in[base + (long)i * S];
+    dxx_inplace<LX>(r, kscale);
+#pragma unroll
+    for (int i = 0; i < LX; i++) T[base + (long)i * S] = r[i];
+}
+
+// out -= e_a in (the <H> subtraction, pca_kernels.cu:806-832)
+__global__ void __launch_bounds__(FT)
+f_sub(const double2 *__restrict__ in, double2 *__restrict__ out, const double *__restrict__ e, int n, long N)
+{
+    const long i = (long)blockIdx.x * FT + threadIdx.x;
+    if (i >= 2L * n * N) return;
+    const long a = (i / N) % n;
+    out[i] = out[i] - in[i] * e[a];
 }
 
 // <H> per state from the yz pass's partials (per per state); with xpart,
@@ -924,6 +953,41 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
         TDW_HIP(hipGetLastError());
     } else {
         TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, hp, c->d_kt, nullptr, qpe));
+    }
+    return 0;
+}
+
+// H applied to arbitrary FP64 vectors on the fused passes: T = D_x in
+// (f_dx), then the yz pass adding T (scale 1).  With qpe: <H> per state into
+// qpe and out -= <H> in (apply_h); without: H in only (apply_h_only).
+int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe)
+{
+    if (!fused_supported(c) || c->p.precision != TDW_FP64) return TDW_EINVAL;
+    TDW_TRY(ensure_state_vecs(c));
+    TDW_TRY(ensure_f(c, (size_t)n * c->nx));
+    const int S = c->S;
+    const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
+    void (*dx)(const double2 *, double2 *, int, long, int, double) = nullptr;
+    switch (c->nx) {
+    case 8: dx = f_dx<8>; break;
+    case 12: dx = f_dx<12>; break;
+    case 16: dx = f_dx<16>; break;
+    case 24: dx = f_dx<24>; break;
+    case 32: dx = f_dx<32>; break;
+    default: return TDW_EINVAL;
+    }
+    {
+        TDW_TIMED(c, "f_dx");
+        hipLaunchKernelGGL(dx, dim3((2L * n * S + FT - 1) / FT), dim3(FT), 0, c->stream, in, T, n, (long)c->N, S,
+                           kxs);
+        TDW_HIP(hipGetLastError());
+    }
+    TDW_TRY(hyz<false>(c, n, in, T, out, pots, nullptr, qpe ? qpe : c->d_ep, nullptr));
+    if (qpe) {
+        const long tot = 2L * n * (long)c->N;
+        TDW_TIMED(c, "f_sub");
+        hipLaunchKernelGGL(f_sub, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, in, out, qpe, n, (long)c->N);
+        TDW_HIP(hipGetLastError());
     }
     return 0;
 }

@@ -9,4 +9,9 @@ int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
 // history entry, leaving the arrays exactly as the reference step does.
 int fused_finalize(tdw_ctx *c, tdw_state *s);
 void fused_release(tdw_ctx *c);
+// H applied to arbitrary FP64 vectors (2n rows) on the fused x and yz
+// passes, T a state-sized scratch; with qpe, <H> per state into qpe and
+// out -= <H> in (as apply_h), without it H in only (as apply_h_only).  3D
+// lattices that fused_supported covers, FP64 contexts.
+int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe);
 }  // namespace tdw
