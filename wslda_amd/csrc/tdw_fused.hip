@@ -91,22 +91,61 @@ __device__ __forceinline__ void put_norm_part(double nrm, bool wave, long P, lon
     }
 }
 
+// the x part of the kinetic term, sigma (-1/2 D_x) (sigma = +1 on u rows,
+// -1 on v rows), applied in place to a pencil held in registers:
+// F^-1 (sigma kscale m^2 / 2 + add) F, with add = -<H> / LX when the <H>
+// subtraction rides along
+template <int LX, typename C>
+__device__ __forceinline__ void xkin_inplace(C (&r)[LX], double sig_kscale_half, double add)
+{
+    typedef decltype(r[0].x) R;
+    rf::fft<LX, -1, C>(r);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int m = 0; m < LX; m++) {
+        const int mm = (m < LX / 2) ? m : m - LX;
+        r[m] = r[m] * (R)fma(sig_kscale_half, (double)(mm * mm), add);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    rf::fft<LX, +1, C>(r);
+}
+
+// sum_x conj(y) sigma (-1/2 D_x) y of a pencil by Parseval, sigma kscale / 2
+// sum_m m^2 |Y_m|^2 (r is transformed in place)
+template <int LX, typename C>
+__device__ __forceinline__ double xkin_energy(C (&r)[LX], double sig_kscale_half)
+{
+    rf::fft<LX, -1, C>(r);
+    double x = 0.0;
+#pragma unroll
+    for (int m = 0; m < LX; m++) {
+        const int mm = (m < LX / 2) ? m : m - LX;
+        x += (double)(mm * mm) * cnorm(to64(r[m]));
+    }
+    return x * sig_kscale_half;
+}
+
 // ---------------------------------------------------------------------------
 // predictor x-pass
 //   plain: y <- y_p (FP64), fold <- c = y_k - i dt sum_j a_j f_j (FP64)
 //   split: y <- c (FP64),   fold <- y_p (FP32 copy)
+// With pending (the previous fused step's state): y_k = s y, and the newest
+// history entry f1 holds H psi_k without the x part of its kinetic term and
+// without the <H> subtraction; both are added here,
+//   F1 = f1 + s sigma (-1/2 D_x) y - e y_k,
+// the D_x from the lane's pencil of y (rounded to FP32 in the split form, as
+// the operator is), and F1 is stored back.
 // No D_x of the prediction is stored: the corrector applies the x part of
 // the kinetic term itself, and the x part of <psi_p|H psi_p> comes from here
-// by Parseval, sum_x conj(y) (-1/2 D_x) y = 1/2 kscale sum_m m^2 |Y_m|^2
-// (u rows; the v rows carry the opposite sign), unscaled by the state's
-// normalisation (xpart, laid out as the norm partials).
+// by Parseval, unscaled by the state's normalisation (xpart, laid out as the
+// norm partials).
 // ---------------------------------------------------------------------------
 template <int LX, int SCH, bool SP>
 __global__ void __launch_bounds__(FT)
 f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP> *f3, const hc_t<SP> *f4,
-       const hc_t<SP> *f5, hc_t<SP> *fold, double *__restrict__ xpart, const double *__restrict__ sk,
-       const double *__restrict__ ek, int pending, int n, long N, int S, double dt, double kscale, int wave,
-       double *__restrict__ part)
+       const hc_t<SP> *f5, hc_t<SP> *fold, double *__restrict__ xpart,
+       const double *__restrict__ sk, const double *__restrict__ ek, int pending, int n, long N, int S, double dt,
+       double kscale, int wave, double *__restrict__ part)
 {
     typedef hc_t<SP> HC;
     const long P = (long)blockIdx.x * FT + threadIdx.x;
@@ -118,15 +157,27 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
         const int a = (int)(a2 < n ? a2 : a2 - n);
         const double s = pending ? sk[a] : 1.0;
         const double e = pending ? ek[a] : 0.0;
+        const double skh = (a2 < n ? 0.5 : -0.5) * kscale;
         const long base = a2 * N + j;
         HC r[LX];
+        if (pending) {
+            // the pencil of y (read again in the loop below, from cache)
+#pragma unroll
+            for (int i = 0; i < LX; i++) r[i] = cvt<HC>(y[base + (long)i * S]);
+            xkin_inplace<LX>(r, skh, 0.0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the loop's addresses are formed here, not held across the transforms
+        long lb = base;
+        int ls = S;
+        asm volatile("" : "+v"(lb), "+s"(ls));
 #pragma unroll
         for (int i = 0; i < LX; i++) {
-            const long idx = base + (long)i * S;
+            const long idx = lb + (long)i * ls;
             const double2 yk = y[idx] * s;
             double2 F1 = to64(f1[idx]);
             if (pending) {
-                F1 = F1 - yk * e;
+                F1 = F1 + to64(r[i]) * s - yk * e;
                 f1[idx] = cvt<HC>(F1);
             }
             const double2 F2 = to64(f2[idx]), F3 = to64(f3[idx]);
@@ -144,14 +195,9 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
             }
             r[i] = cvt<HC>(yp);
             nrm += cnorm(yp);
+            if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
         }
-        rf::fft<LX, -1, HC>(r);
-#pragma unroll
-        for (int m = 0; m < LX; m++) {
-            const int mm = (m < LX / 2) ? m : m - LX;
-            xk += (double)(mm * mm) * cnorm(to64(r[m]));
-        }
-        xk *= (a2 < n ? 0.5 : -0.5) * kscale;
+        xk = xkin_energy<LX>(r, skh);
     }
     put_norm_part(nrm, wave != 0, P, Ptot, part);
     put_norm_part(xk, wave != 0, P, Ptot, xpart);
@@ -166,25 +212,27 @@ f_pred(double2 *__restrict__ y, hc_t<SP> *f1, const hc_t<SP> *f2, const hc_t<SP>
 // the prediction reads no D_x); that part and the <H> subtraction act on the
 // lane's pencil of y_p at once: (-/+ 1/2 D_x - <H>) y_p (u / v rows) is one
 // multiplier, sigma kscale m^2 / 2 - <H> / LX, between a forward and an
-// inverse transform.  Then (H - <H>) psi_p = hin + s (that pencil).
+// inverse transform.  Then (H - <H>) psi_p = hin + s (that pencil).  The x
+// part of <psi|H psi> of the new y goes to xpart by Parseval (the history
+// refresh's yz pass reads no D_x either; the next predictor completes the
+// entry).
 // ---------------------------------------------------------------------------
 template <int LX, bool SP>
 __global__ void __launch_bounds__(FT)
 f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict__ cin,
-       hc_t<SP> *__restrict__ T, const double *__restrict__ sp_, const double *__restrict__ ep_, int n, long N,
+       double *__restrict__ xpart, const double *__restrict__ sp_, const double *__restrict__ ep_, int n, long N,
        int S, double dta0, double kscale, int wave, double *__restrict__ part, float2 *y32)
 {
     typedef hc_t<SP> HC;
-    typedef decltype(HC().x) R;
     const long P = (long)blockIdx.x * FT + threadIdx.x;
     const long Ptot = 2L * n * S;
     const bool act = P < Ptot;
-    double nrm = 0.0;
+    double nrm = 0.0, xk = 0.0;
     if (act) {
         const long a2 = P / S, j = P - a2 * S;
         const int a = (int)(a2 < n ? a2 : a2 - n);
         const double s = sp_[a], e = ep_[a];
-        const double sig = a2 < n ? 0.5 : -0.5;
+        const double skh = (a2 < n ? 0.5 : -0.5) * kscale;
         const long base = a2 * N + j;
         HC r[LX];
 #pragma unroll
@@ -192,16 +240,7 @@ f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict_
             if constexpr (SP) r[i] = cin[base + (long)i * S];   // the FP32 copy of y_p
             else r[i] = y[base + (long)i * S];                  // y_p
         }
-        rf::fft<LX, -1, HC>(r);
-        __builtin_amdgcn_sched_barrier(0);
-        const double ka = sig * kscale, kb = -e / LX;
-#pragma unroll
-        for (int m = 0; m < LX; m++) {
-            const int mm = (m < LX / 2) ? m : m - LX;
-            r[m] = r[m] * (R)fma(ka, (double)(mm * mm), kb);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-        rf::fft<LX, +1, HC>(r);
+        xkin_inplace<LX>(r, skh, -e / LX);
         // keep the loop's loads out of the transforms, and form the loop's
         // addresses anew rather than holding the preload's across them
         // (registers)
@@ -223,11 +262,10 @@ f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict_
             // (registers: r[] is live across the loop)
             if (i % 4 == 3) __builtin_amdgcn_sched_barrier(0);
         }
-        dxx_inplace<LX>(r, kscale);
-#pragma unroll
-        for (int i = 0; i < LX; i++) T[base + (long)i * S] = r[i];
+        xk = xkin_energy<LX>(r, skh);
     }
     put_norm_part(nrm, wave != 0, P, Ptot, part);
+    put_norm_part(xk, wave != 0, P, Ptot, xpart);
 }
 
 // s = 1/sqrt(norm * dV), s2 = s^2 per state; part holds `per` entries per component
@@ -653,6 +691,34 @@ f_fixup(double2 *__restrict__ y, hc_t<SP> *__restrict__ f0, const double *__rest
     if (do_f) f0[i] = cvt<hc_t<SP>>(to64(f0[i]) - v * e[a]);
 }
 
+// The pending state of a fused step made explicit (tdw_finalize): y *= s,
+// and the newest history entry completed, f0 += s sigma (-1/2 D_x) y - e y_k
+// (x-pencils, as the predictor does it)
+template <int LX, bool SP>
+__global__ void __launch_bounds__(FT)
+f_fixup_x(double2 *__restrict__ y, hc_t<SP> *__restrict__ f0, const double *__restrict__ sk,
+          const double *__restrict__ ek, int n, long N, int S, double kscale)
+{
+    typedef hc_t<SP> HC;
+    const long P = (long)blockIdx.x * FT + threadIdx.x;
+    if (P >= 2L * n * S) return;
+    const long a2 = P / S, j = P - a2 * S;
+    const int a = (int)(a2 < n ? a2 : a2 - n);
+    const double s = sk[a], e = ek[a];
+    const long base = a2 * N + j;
+    HC r[LX];
+#pragma unroll
+    for (int i = 0; i < LX; i++) r[i] = cvt<HC>(y[base + (long)i * S]);
+    xkin_inplace<LX>(r, (a2 < n ? 0.5 : -0.5) * kscale, 0.0);
+#pragma unroll
+    for (int i = 0; i < LX; i++) {
+        const long idx = base + (long)i * S;
+        const double2 v = y[idx] * s;
+        y[idx] = v;
+        f0[idx] = cvt<HC>(to64(f0[idx]) + to64(r[i]) * s - v * e);
+    }
+}
+
 // FP64 -> FP32 copy (the measurement step's exact H psi into the split history)
 __global__ void __launch_bounds__(FT) f_to32(const double2 *__restrict__ src, float2 *__restrict__ dst, long tot)
 {
@@ -706,8 +772,9 @@ template <bool SP> struct FusedFns {
     typedef hc_t<SP> HC;
     typedef void (*pred_fn)(double2 *, HC *, const HC *, const HC *, const HC *, const HC *, HC *, double *,
                             const double *, const double *, int, int, long, int, double, double, int, double *);
-    typedef void (*corr_fn)(double2 *, const HC *, const HC *, HC *, const double *, const double *, int, long, int,
-                            double, double, int, double *, float2 *);
+    typedef void (*corr_fn)(double2 *, const HC *, const HC *, double *, const double *, const double *, int, long,
+                            int, double, double, int, double *, float2 *);
+    typedef void (*fix_fn)(double2 *, HC *, const double *, const double *, int, long, int, double);
     typedef void (*hyz_fn)(const HC *, const HC *, HC *, const double *, const double *, int, long, int,
                            const HC *, const HC *, double, double, HParams, double *);
 
@@ -726,6 +793,18 @@ template <bool SP> struct FusedFns {
         case 16: *p = pick_pred<16>(sch); *cf = f_corr<16, SP>; return true;
         case 24: *p = pick_pred<24>(sch); *cf = f_corr<24, SP>; return true;
         case 32: *p = pick_pred<32>(sch); *cf = f_corr<32, SP>; return true;
+        default: return false;
+        }
+    }
+
+    static bool fix(int nx, fix_fn *f)
+    {
+        switch (nx) {
+        case 8: *f = f_fixup_x<8, SP>; return true;
+        case 12: *f = f_fixup_x<12, SP>; return true;
+        case 16: *f = f_fixup_x<16, SP>; return true;
+        case 24: *f = f_fixup_x<24, SP>; return true;
+        case 32: *f = f_fixup_x<32, SP>; return true;
         default: return false;
         }
     }
@@ -804,12 +883,24 @@ template <bool SP> static int fixup(tdw_ctx *c, double2 *y, void *f0, int n, int
     return (int)hipGetLastError();
 }
 
+template <bool SP> static int fixup_x(tdw_ctx *c, tdw_state *s)
+{
+    typename FusedFns<SP>::fix_fn f;
+    if (!FusedFns<SP>::fix(c->nx, &f)) return TDW_EINVAL;
+    const int n = s->n, S = c->S;
+    const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
+    TDW_TIMED(c, "f_fixup");
+    hipLaunchKernelGGL(f, dim3((2L * n * S + FT - 1) / FT), dim3(FT), 0, c->stream, (double2 *)s->wf,
+                       (hc_t<SP> *)s->f[0], c->d_scale, c->d_qpe, n, (long)c->N, S, kxs);
+    return (int)hipGetLastError();
+}
+
 int fused_finalize(tdw_ctx *c, tdw_state *s)
 {
     if (!c->pending) return 0;
     c->pending = 0;
-    if (split_mode(c)) return fixup<true>(c, (double2 *)s->wf, s->f[0], s->n, 1);
-    return fixup<false>(c, (double2 *)s->wf, s->f[0], s->n, 1);
+    if (split_mode(c)) return fixup_x<true>(c, s);
+    return fixup_x<false>(c, s);
 }
 
 static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave)
@@ -871,8 +962,9 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     FusedFns<SP>::xpass(c->nx, c->p.scheme, &fp, &fc);
     const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
     const long nblk = (2L * n * S + FT - 1) / FT;
-    // T (D_x of the corrected state) lives in ddx, H of the prediction in
-    // lap; in the split form both hold FP32 values
+    // H of the prediction lives in lap (in the split form FP32 values, and
+    // then the FP32 copy of the corrected state); ddx is no longer written
+    // by the fused passes (T is only the yz kernels' argument)
     double2 *y = (double2 *)s->wf;
     HC *T = (HC *)s->ddx, *H = (HC *)s->lap;
     HC **f = (HC **)s->f;
@@ -905,8 +997,8 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
     {
         TDW_TIMED(c, "f_corr");
-        hipLaunchKernelGGL(fc, dim3(nblk), dim3(FT), 0, c->stream, y, H, f[h - 1], T, c->d_sp, c->d_ep, n, N, S,
-                           c->p.dt * a0, kxs, wave, c->d_fpart, SP ? (float2 *)H : nullptr);
+        hipLaunchKernelGGL(fc, dim3(nblk), dim3(FT), 0, c->stream, y, H, f[h - 1], xpart, c->d_sp, c->d_ep, n, N,
+                           S, c->p.dt * a0, kxs, wave, c->d_fpart, SP ? (float2 *)H : nullptr);
         TDW_HIP(hipGetLastError());
     }
     TDW_TRY(xpass_norm(c, n, c->d_scale, c->d_s2, wave));
@@ -925,7 +1017,10 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
         tdw_complex *tmp = s->f[h - 1];
         for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
         s->f[0] = tmp;
-        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], hp, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr));
+        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], hp, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr,
+                        xpart, per, c->d_s2));
+        // pending: y unscaled, and f[0] without the x kinetic part and the
+        // <H> subtraction (the next predictor or tdw_finalize adds both)
         c->pending = 1;
         return 0;
     }
