@@ -69,16 +69,18 @@ def design_units(name, scheme, split=False):
     h = nhist(scheme)
     if split:
         return {
-            "f_pred": 1 + 0.5 * h + 1 + 0.5 + 0.5,         # y, f_1..f_h; c, y_p (FP32), f_1 fix-up
-            "f_corr": 1 + 0.5 + 0.5 + 1 + 0.5 + 0.5,       # c, H y_p, y_p (FP32); y, T, y (FP32)
-            "f_hyz": (1.0 + 1.5) / 2,                       # y_p (FP32); H | y (FP32), T; f_new
+            # y (twice: its pencil for the x kinetic part of f_1, then the
+            # points), f_1..f_h; c, y_p (FP32), f_1 completed
+            "f_pred": 2 + 0.5 * h + 1 + 0.5 + 0.5,
+            "f_corr": 1 + 0.5 + 0.5 + 1 + 0.5,             # c, H y_p, y_p (FP32); y, y (FP32)
+            "f_hyz": 1.0,                                   # y (FP32); H
             "densities": 0.5,                               # y (FP32)
             "f_fixup": 2,
         }.get(name, 0)
     return {
-        "f_pred": (h + 1) + 3,   # read y, f_1..f_h; write y_p, c, f_1 (pending <H> fix-up)
-        "f_corr": 5,             # read y_p, H y_p, c; write y, T (= D_x y)
-        "f_hyz": (2 + 3) / 2,    # per launch, the two of a step: y_p; H | y, T; f_new
+        "f_pred": (h + 2) + 3,   # read y (twice, as above), f_1..f_h; write y_p, c, f_1 (completed)
+        "f_corr": 4,             # read y_p, H y_p, c; write y
+        "f_hyz": 2,              # read y; write H y
         "densities": 1,          # read y
         "f_fixup": 2,
     }.get(name, 0)
@@ -276,8 +278,8 @@ def cpu_baseline(g, args):
 # --meas-every 1000 --copy-calib-mib 4096 --trap-states 0` (the 4096 MiB copy
 # reads FETCH_SIZE = half its bytes and WRITE_SIZE = all of them)
 MEASURED_TRAFFIC = {
-    ("f_pred", (24, 24, 24), 16308, 55, "fp64"): (21137708 * 2 + 21139931) * 1024,
-    ("f_pred", (24, 24, 24), 16308, 55, "split"): (12331066 * 2 + 14094705) * 1024,
+    ("f_pred", (24, 24, 24), 16308, 55, "fp64"): (24629179 * 2 + 21140835) * 1024,
+    ("f_pred", (24, 24, 24), 16308, 55, "split"): (15591706 * 2 + 14094742) * 1024,
 }
 
 
