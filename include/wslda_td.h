@@ -225,21 +225,22 @@ typedef struct {
 int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
 /* Same step with the fused kernels (six passes over the state arrays instead
  * of the reference's ~57 touches); same results to rounding.  Between fused
- * steps the stored wf is unnormalised and f[0] still holds H psi: the
- * normalisation and <H> subtraction are applied by the next fused step, or by
- * tdw_finalize(), which every other entry point calls first.  Densities,
- * potentials and qpe are exact after every step.  Lattices the fused kernels
- * are not built for, and ASLDA, run the modular step (tdw_fused_supported()
- * says which).
+ * steps the stored wf is unnormalised and f[0] holds H psi without the x
+ * part of its kinetic term and before the <H> subtraction: the next fused
+ * step, or tdw_finalize(), which every other entry point calls first,
+ * completes both.  Densities, potentials and qpe are exact after every step.
+ * Lattices the fused kernels are not built for, and ASLDA, run the modular
+ * step (tdw_fused_supported() says which).
  *
  * Split precision (params.precision == TDW_SPLIT): the x / yz FFTs and the
  * kinetic term run in FP32 complex, and f[0..4] hold FP32 complex values
  * (8 bytes each; a buffer of 2*n*N*8 bytes is enough).  The state wf, every
- * ABM update of it (the AM partial corrector is kept as the FP32 increment
- * c - y_p and added in FP64), the densities, norms and <H> stay FP64;
- * ddx and lap are used as FP32 scratch between steps.  Gradient steps
- * (grad_last) compute derivatives, tau, j and H psi in FP64.  tdw_step and
- * lattices without fused kernels return TDW_EINVAL in this mode. */
+ * ABM update of it (the AM partial corrector c is kept in FP64 in wf between
+ * the predictor and the corrector, the prediction as an FP32 copy), the
+ * densities, norms and <H> stay FP64; lap is used as FP32 scratch within a
+ * step.  Gradient steps (grad_last) compute derivatives, tau, j and H psi in
+ * FP64.  tdw_step and lattices without fused kernels return TDW_EINVAL in
+ * this mode. */
 int tdw_step_fused(tdw_ctx *c, tdw_state *s, int it, int grad_last);
 int tdw_finalize(tdw_ctx *c, tdw_state *s);
 int tdw_fused_supported(tdw_ctx *c);
