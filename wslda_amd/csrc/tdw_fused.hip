@@ -1,32 +1,34 @@
 // tdw_fused.hip -- fused PECE step for 3D lattices (kernel families a + b).
 //
 // The reference step (pca.c:1401-1570) touches the state arrays ~57 times per
-// step.  Here one step is six passes:
-//   f_pred  (x-pencils, one lane per pencil, FFT in registers): ABM predictor
-//           (tdwslda_ode_integrator.h:26-135) + norm partials + the x part
-//           of <psi_p|H psi_p> (Parseval); also applies the pending
-//           normalisation of y and the pending <H> subtraction of the newest
-//           history entry.
+// step.  Here one step is six passes, none of which stores a derivative:
+//   f_pred  (x-pencils, one lane per pencil, FFT in registers): completes the
+//           newest history entry (its x kinetic part, from the lane's pencil
+//           of y in Fourier space, and the pending <H> subtraction), applies
+//           the pending normalisation of y, ABM predictor
+//           (tdwslda_ode_integrator.h:26-135), norm partials and the x part
+//           of <psi_p|H psi_p> (Parseval).
 //   dens    densities of the prediction, normalisation folded into the
 //           occupation weights (f*s^2).
-//   f_hyz   (yz-planes, two-factor FFT in LDS): D_y + D_z, then H psi
-//           without its x kinetic part (pca_kernels.cu:233-391) and the
-//           <psi|H psi> partials.
+//   f_hyz   (yz-planes, FFTs in LDS): D_y + D_z, then H psi without its x
+//           kinetic part (pca_kernels.cu:233-391) and the <psi|H psi>
+//           partials.
 //   f_corr  (x-pencils): the x kinetic part and the <H> subtraction applied
 //           to the prediction's pencil in Fourier space, AM corrector
-//           y = c - i dt a0 (H psi - <H> psi) (:138-220), D_x of the new y
-//           (T) + norm partials.
-//   dens, f_hyz again (now adding T): H of the corrected state into the
-//   newest history slot.
+//           y = c - i dt a0 (H psi - <H> psi) (:138-220), norm partials and
+//           the x part of <psi|H psi> of the new y.
+//   dens, f_hyz again: H of the corrected state, without its x kinetic part,
+//   into the newest history slot.
 // Normalisation (pca_kernels.cu:955-976) and the <H> subtraction
 // (pca_kernels.cu:806-832) are carried as per-state scalars (s = 1/sqrt(norm),
 // e = <H>) into the next pass instead of extra passes over the state: the
-// stored y stays unnormalised and the newest history entry keeps H psi until
-// the next predictor (or tdw_finalize) applies them.
+// stored y stays unnormalised and the newest history entry lacks its x
+// kinetic part and the subtraction until the next predictor (or
+// tdw_finalize) adds them.
 //
-// Split precision: the history, T and H are FP32; y and every ABM update of
-// it stay FP64.  The predictor writes the FP64 partial corrector c into y
-// and an FP32 copy of the prediction into the history slot it consumed; the
+// Split precision: the history and H are FP32; y and every ABM update of it
+// stay FP64.  The predictor writes the FP64 partial corrector c into y and
+// an FP32 copy of the prediction into the history slot it consumed; the
 // densities and the yz pass read that copy.  The corrector reads c and the
 // copy, writes the FP64 y and an FP32 copy of it (into H's buffer, which it
 // has consumed) for the second density and yz passes.
@@ -421,9 +423,10 @@ struct HParams {
     int bdg;
 };
 
-// TX: add T (D_x of the plane, from the x-pass) to the transverse terms;
-// without it the output lacks the x part of the kinetic term (the
-// prediction's yz pass: the corrector adds that part itself)
+// TX: add T (D_x of the plane, from an x-pass) to the transverse terms (H
+// of arbitrary vectors, fused_apply_h); without it the output lacks the x
+// part of the kinetic term (the fused step's yz passes: its x-passes add
+// that part themselves)
 template <int LY, int LZ, bool SP, bool TX, int NT = nt_hyz(LY * LZ)>
 __global__ void __launch_bounds__(NT)
 f_hyz(const hc_t<SP> *__restrict__ y, const hc_t<SP> *__restrict__ T, hc_t<SP> *__restrict__ hout,
@@ -832,8 +835,7 @@ template <bool SP> struct FusedFns {
         return false;
     }
 
-    // tx: the pass adds T (the corrected state's); without it, the
-    // prediction's pass
+    // tx: the pass adds T (fused_apply_h); without it, the fused step's passes
     static bool hyz(int ny, int nz, bool tx, hyz_fn *h) { return tx ? hyz_t<true>(ny, nz, h) : hyz_t<false>(ny, nz, h); }
 };
 
