@@ -360,6 +360,7 @@ def run_adjoint(args, calibrate=True):
     torch.cuda.synchronize()
     t_fwd = time.time() - t0
     O = prob.observable()
+    torch.cuda.reset_peak_memory_stats()
     t0 = time.time()
     prob.forward(keep_checkpoints=True)
     torch.cuda.synchronize()
@@ -368,6 +369,7 @@ def run_adjoint(args, calibrate=True):
     gslot = prob.backward()
     torch.cuda.synchronize()
     t_bwd = time.time() - t0
+    peak = torch.cuda.max_memory_allocated()
     grad = adjoint.beams_grad(theta, gslot)
     fd = {}
     h = 1e-6
@@ -392,10 +394,10 @@ def run_adjoint(args, calibrate=True):
         "n_gpus": 1, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "td-SLDA 2D 64x64 lattice (8-point transverse box), %d qp states, AB5AM5, two "
                                "Gaussian beams (seeded), N in a disk r < 10 dx after %d steps, checkpoints every %d, "
-                               "last %d segments taped in the forward run" % (n, args.adj_steps, args.adj_k,
-                                                                             args.adj_taped)},
+                               "last %d segments taped in the forward run (y per step)" % (n, args.adj_steps, args.adj_k,
+                                                                                           args.adj_taped)},
         "forward_s": t_fwd, "forward_with_checkpoints_s": t_fwd_ck, "backward_s": t_bwd,
-        "backward_over_forward": t_bwd / t_fwd, "observable": O,
+        "backward_over_forward": t_bwd / t_fwd, "observable": O, "peak_device_GB": peak / 1e9,
         "grad_theta": [float(x) for x in grad], "fd_check": fd, "max_rel_err_vs_fd": err,
         "qp_steps_per_s_forward": n * args.adj_steps / t_fwd,
     }
@@ -662,8 +664,9 @@ def main():
     ap.add_argument("--adj-k", type=int, default=50)
     ap.add_argument("--adj-states", type=int, default=4096)
     ap.add_argument("--adj-fd", type=int, default=1, help="central-difference check of three parameters (0: off)")
-    ap.add_argument("--adj-taped", type=int, default=4,
-                    help="segments the forward run tapes directly (each 2K+1 state arrays of HBM)")
+    ap.add_argument("--adj-taped", type=int, default=9,
+                    help="segments the forward run tapes directly (each K+1 state arrays of HBM: y per step; "
+                         "the backward sweep rebuilds the history entries)")
     ap.add_argument("--trap", action="store_true",
                     help="instead of the step benchmark: split precision on the trapped vortex (32^3)")
     ap.add_argument("--trap-states", type=int, default=-1,
@@ -786,7 +789,8 @@ def main():
         torch.cuda.empty_cache()
         adj = run_adjoint(args, calibrate=False)
         res["adjoint"] = {k: adj[k] for k in ("config", "value", "unit", "forward_s", "forward_with_checkpoints_s",
-                                              "backward_s", "fd_check", "max_rel_err_vs_fd")}
+                                              "backward_s", "fd_check", "max_rel_err_vs_fd",
+                                              "peak_device_GB")}
     nb = copy_calibration(args)
     if nb:
         res["copy_calib_bytes"] = nb

@@ -392,6 +392,14 @@ int tdw_adj_forward_step(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_copy, 
  * s->wf, s->f[0..h-1] and from each other. */
 int tdw_adj_forward_step_to(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_next, tdw_complex *f_next,
                             tdw_tape_step *t);
+/* The history entry taped step `it` wrote, rebuilt from y after that step
+ * (y1) and the step's tape t: H y1 - <H> y1 with t->pots_c, the H
+ * application the step used and the <H> it recorded (t->scal[3n..4n)), so a
+ * tape may keep y alone and rebuild the entries during the backward sweep.
+ * scratch: a state-sized array (the Laplacian or D_x of y1).  fnew and
+ * scratch must differ from y1 and from each other. */
+int tdw_adj_history_entry(tdw_ctx *c, int it, const tdw_tape_step *t, const tdw_complex *y1, tdw_complex *fnew,
+                          tdw_complex *scratch);
 int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *a, int it, const tdw_complex *y0,
                           const tdw_complex *const *f, const tdw_complex *y1, const tdw_complex *fnew,
                           const tdw_tape_step *t, const double *pots_prev, const double *dens_grad);

@@ -31,7 +31,7 @@ CASES = [
 ]
 
 
-def setup_problem(dim, shape, nst, scheme, steps, K, taped=1):
+def setup_problem(dim, shape, nst, scheme, steps, K, taped=1, tape_history=False):
     from wslda_amd import adjoint, problems
     s = problems.uniform_start(dim, shape, n_states=nst, scheme=scheme)
     p = s["params"]
@@ -43,7 +43,7 @@ def setup_problem(dim, shape, nst, scheme, steps, K, taped=1):
         theta[[2, 6]] = 0.0
     g = adjoint.disk_weights(p, (0.0, 0.0, 0.0), 0.3 * L)
     prob = adjoint.AdjointProblem(p, s["wf"], s["pots"], s["fbeta"], s["kt"], s["cnt"], nsteps=steps, K=K,
-                                  dens_grad=g, taped_segments=taped)
+                                  dens_grad=g, taped_segments=taped, tape_history=tape_history)
     return prob, theta
 
 
@@ -129,3 +129,34 @@ def test_autograd_function_matches_adjoint(torch):
     fd = (adjoint.vext_observable(torch.tensor(tp), prob).item() - adjoint.vext_observable(torch.tensor(tm), prob).item()) / (2 * h)
     assert abs(fd - g_auto[3]) <= GRAD_TOL * np.abs(g_auto).max()
     prob.close()
+
+
+@pytest.mark.parametrize("dim,shape,nst,scheme,steps,K,taped", CASES)
+def test_light_tape_matches_full_tape(torch, dim, shape, nst, scheme, steps, K, taped):
+    """A tape of y alone, whose history entries the backward sweep rebuilds
+    (tdw_adj_history_entry, one H application per step, across segment
+    boundaries too), gives the gradient of the tape that keeps the entries:
+    the rebuild runs the H application the taped step ran, on the same
+    inputs.  One more segment is taped than in the FD cases, so that a taped
+    segment starts from the tape of the one before it, and a second run tapes
+    every segment (no recompute at all)."""
+    for tp in (taped + 1, 99):
+        say("case: light vs full tape, dim=%d shape=%s scheme=%d steps=%d K=%d taped=%d" % (
+            dim, shape, scheme, steps, K, tp))
+        out = {}
+        for full in (True, False):
+            prob, theta = setup_problem(dim, shape, nst, scheme, steps, K, tp, tape_history=full)
+            prob.set_vext(gauss_slots(theta, dim))
+            prob.forward()
+            g = prob.backward()
+            out[full] = (g, prob.lambda_y0.clone(), len(prob.ckpts))
+            prob.close()
+        g1, g0 = out[True][0], out[False][0]
+        l1, l0 = out[True][1], out[False][1]
+        dg = np.abs(g1 - g0).max() / np.abs(g1).max()
+        dl = float((l1 - l0).abs().max() / l1.abs().max())
+        say("  max rel diff: gradient %.2e, dL/dy0 %.2e; checkpoints kept %d (full) / %d (light)" % (
+            dg, dl, out[True][2], out[False][2]))
+        assert dg <= 1e-13 and dl <= 1e-13
+        # the light tape keeps no checkpoint for a taped segment after another
+        assert out[False][2] < out[True][2]

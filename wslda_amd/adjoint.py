@@ -5,8 +5,17 @@ The forward run is the modular step (pca.c:1401-1570, non-gradient form) from
 a start state with zero ABM history.  Every K steps it keeps a checkpoint (y,
 history, potentials).  The backward sweep takes the checkpoints in reverse:
 each K-step segment is recomputed with the taped step (which also records y,
-the new history entry, the norms, <H> values, densities and potentials of
-every step), then the adjoint steps run backwards through the segment.
+the norms, <H> values, densities and potentials of every step), then the
+adjoint steps run backwards through the segment.  The last taped_segments
+segments are taped by the forward run itself and not recomputed.
+
+The tape keeps one state array per step (y); the history entries the
+backward steps read are rebuilt from it (tdw_adj_history_entry: H y - <H> y
+with the step's potentials and <H>, the H application the step used), one per step,
+in a window of h + 1 entries.  That costs one H application per backward
+step and halves the tape, so twice as many segments fit in memory untouched
+by the recompute.  tape_history=True keeps the entries on the tape instead
+(two state arrays per step).
 
 The observable is a linear functional of the final densities,
 O = sum_r g(r) . (rho_a, rho_b, Re nu, Im nu)(r), given by the 4*N array g,
@@ -41,11 +50,11 @@ class AdjointProblem:
     weights; nsteps: steps; K: checkpoint interval; it0: step index of the
     start; dens_grad: 4*N observable weights (see module doc);
     taped_segments: how many of the last K-step segments the forward run
-    tapes directly (each holds 2K+1 state arrays), so the backward sweep
-    recomputes only the others."""
+    tapes directly (each holds K+1 state arrays, 2K+1 with tape_history),
+    so the backward sweep recomputes only the others."""
 
     def __init__(self, params, wf0, pots0, fbeta, kt=None, cnt=None, nsteps=100, K=10, it0=0, dens_grad=None,
-                 device=0, taped_segments=1, state_observable=None, forward_precision=None):
+                 device=0, taped_segments=1, state_observable=None, forward_precision=None, tape_history=False):
         import torch
         self.torch = torch
         self.params = params
@@ -77,6 +86,10 @@ class AdjointProblem:
         self.taped = {}
         self._tps = []
         self.taped_segments = taped_segments
+        self.tape_history = tape_history
+        # the history buffer a taped step writes while the run's own h
+        # entries are all live (AB5AM5)
+        self._fspare = None
         # state_observable(run) -> (O, dO/dy* as a [2n, N] tensor, paired by
         # Re sum conj(seed) dy): an observable of the final state itself
         self.state_observable = state_observable
@@ -115,7 +128,7 @@ class AdjointProblem:
             n, N, K = self.n, self.N, self.K
             z = lambda: torch.zeros((2 * n, N), dtype=torch.complex128, device=self.dev)
             ty = [z() for _ in range(K + 1)]
-            tf = [z() for _ in range(K)]
+            tf = [z() for _ in range(K)] if self.tape_history else []
             tscal = torch.zeros((K, 4 * n), dtype=torch.float64, device=self.dev)
             tgrid = torch.zeros((K, 4, 12 * N), dtype=torch.float64, device=self.dev)
             tapes = []
@@ -129,20 +142,38 @@ class AdjointProblem:
             self._tps.append((ty, tf, tscal, tgrid, tapes))
         return self._tps[i]
 
+    def _z(self):
+        return self.torch.zeros((2 * self.n, self.N), dtype=self.torch.complex128, device=self.dev)
+
+    def _free_entry(self, R):
+        """A history buffer of the run that the next step does not read (the
+        entry its rotation dropped last), for the new entry."""
+        if self._fspare is None:
+            self._fspare = self._z()
+        live = {R.st.f[j] for j in range(self.h)}
+        for t in list(R.f) + [self._fspare]:
+            if t.data_ptr() not in live:
+                return t
+        raise td.TDWError("no free history buffer")
+
     def _taped_step(self, R, k, r, i=0):
-        """Step k (row r of tape buffer i) with y and the new history entry
-        written straight into the tape (tdw_adj_forward_step_to).  At r == 0
-        the run's y is first copied into the tape's row 0."""
+        """Step k (row r of tape buffer i) with y written straight into the
+        tape (tdw_adj_forward_step_to), and the new history entry into the
+        tape (tape_history) or into a free history buffer of the run.  At
+        r == 0 the run's y is first copied into the tape's row 0."""
         ty, tf, _, _, tapes = self._tape(i)
         if r == 0 and R.st.wf != ty[0].data_ptr():
             ty[0].copy_(self._tensor_at(R, R.st.wf))
             R.st.wf = ty[0].data_ptr()
+        fnext = tf[r] if self.tape_history else self._free_entry(R)
         td.check(td.lib().tdw_adj_forward_step_to(self.ctx.h, C.byref(R.st), self.it0 + k, _p(ty[r + 1]),
-                                                  _p(tf[r]), C.byref(tapes[r])), "tdw_adj_forward_step_to")
+                                                  _p(fnext), C.byref(tapes[r])), "tdw_adj_forward_step_to")
 
     def _tensor_at(self, R, ptr):
-        """The tensor (the run's own or a tape row) whose data is at ptr."""
-        for t in [R.wf] + list(R.f):
+        """The tensor (the run's own, the spare history buffer or a tape row)
+        whose data is at ptr."""
+        spare = [self._fspare] if self._fspare is not None else []
+        for t in [R.wf] + list(R.f) + spare:
             if t.data_ptr() == ptr:
                 return t
         for (ty, tf, _, _, _) in self._tps:
@@ -184,7 +215,10 @@ class AdjointProblem:
         first_taped = max(0, nseg - self.taped_segments)
         for k in range(self.nsteps):
             seg, r = divmod(k, K)
-            if keep_checkpoints and r == 0:
+            # a taped segment that follows another one needs no checkpoint
+            # on the light tape: the backward sweep rebuilds its start
+            # history from the segment before
+            if keep_checkpoints and r == 0 and (self.tape_history or seg <= first_taped):
                 if R.fused:
                     R.finalize()
                 self.ckpts.append((k, R.wf.clone(), [t.clone() for t in R.history()[:self.h]], R.pots.clone()))
@@ -243,12 +277,32 @@ class AdjointProblem:
             ly.copy_(self.state_observable(self.run)[1])
             torch.cuda.synchronize()
         L = self.ctx.L
+        nseg = (self.nsteps + K - 1) // K
+        ck = {k0 // K: (k0, y0, hist0, pots0) for (k0, y0, hist0, pots0) in self.ckpts}
+        # light tape: the history entries rebuilt so far, by the global step
+        # that wrote them (at most h + 1 live), and their free buffers
+        win = {}
+        pool = [] if self.tape_history else [z() for _ in range(h + 1)]
+        torch.cuda.synchronize()
+
+        def rebuilt(kg, yrow, tape):
+            if kg not in win:
+                buf = pool.pop() if pool else z()
+                # w[2] is free between backward steps: the rebuild's scratch
+                td.check(L.tdw_adj_history_entry(self.ctx.h, self.it0 + kg, C.byref(tape), _p(yrow), _p(buf),
+                                                 _p(w[2])), "tdw_adj_history_entry")
+                win[kg] = buf
+            return win[kg]
+
         last = True
-        for (k0, y0, hist0, pots0) in reversed(self.ckpts):
+        for seg in reversed(range(nseg)):
+            k0 = seg * K
             ks = min(K, self.nsteps - k0)
-            seg = k0 // K
-            # a split checkpoint's FP32 history, widened for this segment
-            hist0 = [t if t.dtype == torch.complex128 else t.to(torch.complex128) for t in hist0]
+            hist0 = pots0 = None
+            if seg in ck:
+                _, y0, hist0, pots0 = ck[seg]
+                # a split checkpoint's FP32 history, widened for this segment
+                hist0 = [t if t.dtype == torch.complex128 else t.to(torch.complex128) for t in hist0]
             if seg in self.taped:
                 ti = self.taped.pop(seg)
             else:
@@ -259,14 +313,30 @@ class AdjointProblem:
                     self._taped_step(R, k0 + r, r, ti)
                 del R
             ty, tf, tscal, tgrid, tapes = self._tape(ti)
+            # without a checkpoint (a taped segment after another on the light
+            # tape) the start comes from the segment before, still on the tape
+            prev = None if seg in ck else self._tape(self.taped[seg - 1])
+
+            def entry(kg):
+                q = kg - k0
+                if q >= 0:
+                    return tf[q] if self.tape_history else rebuilt(kg, ty[q + 1], tapes[q])
+                if hist0 is not None:
+                    return hist0[-q - 1]
+                return rebuilt(kg, prev[0][K + q + 1], prev[4][K + q])
+
             # adjoint sweep through the segment
             for r in reversed(range(ks)):
-                hist = [(tf[r - j] if r - j >= 0 else hist0[j - r - 1]) for j in range(1, h + 1)]
+                kg = k0 + r
+                for key in [key for key in win if key > kg]:
+                    pool.append(win.pop(key))
+                fnew = entry(kg)
+                hist = [entry(kg - j) for j in range(1, h + 1)]
                 fp = (C.c_void_p * h)(*[t.data_ptr() for t in hist])
-                pots_prev = tgrid[r - 1, 3] if r > 0 else pots0
+                pots_prev = tgrid[r - 1, 3] if r > 0 else (pots0 if pots0 is not None else prev[3][K - 1, 3])
                 dg = self.dens_grad if (last and r == ks - 1 and self.state_observable is None) else None
-                td.check(L.tdw_adj_backward_step(self.ctx.h, C.byref(A), self.it0 + k0 + r, _p(ty[r]), fp,
-                                                 _p(ty[r + 1]), _p(tf[r]), C.byref(tapes[r]), _p(pots_prev),
+                td.check(L.tdw_adj_backward_step(self.ctx.h, C.byref(A), self.it0 + kg, _p(ty[r]), fp,
+                                                 _p(ty[r + 1]), _p(fnew), C.byref(tapes[r]), _p(pots_prev),
                                                  _p(dg)), "tdw_adj_backward_step")
             last = False
         self.ctx.sync()

@@ -749,6 +749,26 @@ extern "C" int tdw_adj_forward_step_to(tdw_ctx *c, tdw_state *s, int it, tdw_com
     return adj_forward(c, s, it, y_next, D2(s->ddx), f_next, t);
 }
 
+// The history entry taped step `it` left, rebuilt from y after the step and
+// the step's tape: H(pots_c) y1 with the H application that step used, minus
+// the <H> it recorded (scal[3n..4n)).  2D plane lattices subtract inside the
+// plane pass, the others with the step's own subtraction kernel.
+extern "C" int tdw_adj_history_entry(tdw_ctx *c, int it, const tdw_tape_step *t, const tdw_complex *y1,
+                                     tdw_complex *fnew, tdw_complex *scratch)
+{
+    if (!c || !t || !y1 || !fnew || !scratch || t->n <= 0 || t->n > c->nwf) return TDW_EINVAL;
+    if (fnew == y1 || scratch == y1 || scratch == fnew) return TDW_EINVAL;
+    if (!adj_supported(c)) return TDW_EINVAL;
+    const int n = t->n;
+    const double *e_c = t->scal + 3 * n;
+    const bool plane = h2d_supported(c);
+    const bool fused = !plane && fused_supported(c);
+    if (plane) return apply_h_plane(c, it + 1, n, CD2(y1), D2(fnew), t->pots_c, c->d_kt, e_c, nullptr);
+    if (fused) return fused_apply_h(c, n, CD2(y1), D2(fnew), D2(scratch), t->pots_c, nullptr, e_c);
+    TDW_TRY(laplace(c, 2 * n, CD2(y1), D2(scratch)));
+    return apply_h(c, it + 1, n, CD2(y1), D2(fnew), CD2(scratch), t->pots_c, c->d_kt, e_c, nullptr);
+}
+
 extern "C" int tdw_adj_backward_step(tdw_ctx *c, tdw_adj_state *A, int it, const tdw_complex *y0,
                                      const tdw_complex *const *f, const tdw_complex *y1, const tdw_complex *fnew,
                                      const tdw_tape_step *t, const double *pots_prev, const double *dens_grad)

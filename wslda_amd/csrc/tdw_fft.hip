@@ -372,14 +372,15 @@ k_lap2d(const double2 *in, double2 *out, long N, const double2 *__restrict__ twx
 //   (H psi)_u =  (V_a - mu_a + alpha_a kz^2/2) u + Delta v  - alpha_a/2 lap u
 //   (H psi)_v = -(V_b - mu_b + alpha_b kz^2/2) v + Delta* u + alpha_b/2 lap v
 // part[p] = Re sum_i conj(psi_p) (H psi)_p; with dotw, also
-// part2[p] = Re sum_i conj(psi_p) dotw_p.  out must not alias in (the other
-// component is read from in).
+// part2[p] = Re sum_i conj(psi_p) dotw_p.  With esub, out = (H - esub_a) psi
+// (the <H> subtraction with a known <H>, in the same pass).  out must not
+// alias in (the other component is read from in).
 template <int NX, int NY>
 __global__ void __launch_bounds__(NTHR)
 k_h2d(const double2 *__restrict__ in, double2 *__restrict__ out, long N, int n, const double2 *__restrict__ twx_g,
       const double2 *__restrict__ twy_g, double ksx, double ksy, const double *__restrict__ pots,
       const double *__restrict__ kt, int bdg, double mu_a, double mu_b, double *__restrict__ part,
-      const double2 *__restrict__ dotw, double *__restrict__ part2)
+      const double2 *__restrict__ dotw, double *__restrict__ part2, const double *__restrict__ esub)
 {
     constexpr int NYP = NY + 1, E = NX * NY, KM = (E + NTHR - 1) / NTHR;
     extern __shared__ double2 lds2[];
@@ -412,6 +413,7 @@ k_h2d(const double2 *__restrict__ in, double2 *__restrict__ out, long N, int n, 
     __syncthreads();
     lap_plane<NX, NY>(A, twx, twy, tid, ksx, ksy);
     const double k2t = kt ? kt[a] * kt[a] : 0.0;
+    const double es = esub ? esub[a] : 0.0;
     double s = 0.0, s2 = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; k++) {
@@ -430,8 +432,9 @@ k_h2d(const double2 *__restrict__ in, double2 *__restrict__ out, long N, int n, 
             const double Vb = pots[N + e] - mu_b + (-1.0 * ab * k2t);
             r = x * (-Vb) + cmulc(D, y) - lp * ab;
         }
-        out[own + e] = r;
         s += cmulc(x, r).x;
+        if (esub) r = r - x * es;
+        out[own + e] = r;
         if (dotw) s2 += cmulc(x, dotw[own + e]).x;
     }
     s = block_sum(s, red);
@@ -479,7 +482,8 @@ int lap2d(tdw_ctx *c, int m, const double2 *in, double2 *out, int *done)
 }
 
 typedef void (*h2d_fn)(const double2 *, double2 *, long, int, const double2 *, const double2 *, double, double,
-                       const double *, const double *, int, double, double, double *, const double2 *, double *);
+                       const double *, const double *, int, double, double, double *, const double2 *, double *,
+                       const double *);
 
 static bool h2d_for(const tdw_ctx *c, h2d_fn *f, size_t *lds)
 {
@@ -507,7 +511,7 @@ bool h2d_supported(const tdw_ctx *c)
 }
 
 int h2d(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, const double *kt, double *part,
-        const double2 *dotw, double *part2)
+        const double2 *dotw, double *part2, const double *esub)
 {
     h2d_fn f;
     size_t lds;
@@ -516,7 +520,8 @@ int h2d(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, 
     const double N = (double)c->N;
     const double ksx = pow(2.0 * M_PI / c->Lphys[0], 2) / N, ksy = pow(2.0 * M_PI / c->Lphys[1], 2) / N;
     hipLaunchKernelGGL(f, dim3(2 * n), dim3(NTHR), lds, c->stream, in, out, (long)c->N, n, c->d_tw[0], c->d_tw[1],
-                       ksx, ksy, pots, kt, c->p.functional == TDW_BDG, c->p.mu_a, c->p.mu_b, part, dotw, part2);
+                       ksx, ksy, pots, kt, c->p.functional == TDW_BDG, c->p.mu_a, c->p.mu_b, part, dotw, part2,
+                       esub);
     TDW_HIP(hipGetLastError());
     return 0;
 }

@@ -1056,8 +1056,10 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
 
 // H applied to arbitrary FP64 vectors on the fused passes: T = D_x in
 // (f_dx), then the yz pass adding T (scale 1).  With qpe: <H> per state into
-// qpe and out -= <H> in (apply_h); without: H in only (apply_h_only).
-int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe)
+// qpe and out -= <H> in (apply_h); without: H in only (apply_h_only).  With
+// use_e: out -= use_e in, a known <H> (qpe is then not written).
+int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe,
+                  const double *use_e)
 {
     if (!fused_supported(c) || c->p.precision != TDW_FP64) return TDW_EINVAL;
     TDW_TRY(ensure_state_vecs(c));
@@ -1079,11 +1081,12 @@ int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
                            kxs);
         TDW_HIP(hipGetLastError());
     }
-    TDW_TRY(hyz<false>(c, n, in, T, out, pots, nullptr, qpe ? qpe : c->d_ep, nullptr));
-    if (qpe) {
+    TDW_TRY(hyz<false>(c, n, in, T, out, pots, nullptr, (qpe && !use_e) ? qpe : c->d_ep, nullptr));
+    const double *e = use_e ? use_e : qpe;
+    if (e) {
         const long tot = 2L * n * (long)c->N;
         TDW_TIMED(c, "f_sub");
-        hipLaunchKernelGGL(f_sub, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, in, out, qpe, n, (long)c->N);
+        hipLaunchKernelGGL(f_sub, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, in, out, e, n, (long)c->N);
         TDW_HIP(hipGetLastError());
     }
     return 0;

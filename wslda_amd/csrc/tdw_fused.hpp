@@ -13,5 +13,6 @@ void fused_release(tdw_ctx *c);
 // passes, T a state-sized scratch; with qpe, <H> per state into qpe and
 // out -= <H> in (as apply_h), without it H in only (as apply_h_only).  3D
 // lattices that fused_supported covers, FP64 contexts.
-int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe);
+int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe,
+                  const double *use_e = nullptr);
 }  // namespace tdw

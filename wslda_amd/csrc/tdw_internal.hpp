@@ -127,11 +127,12 @@ int launch_outer(tdw_ctx *c, int m, const double2 *in, double2 *T, double2 *ddx,
 int lap2d(tdw_ctx *c, int m, const double2 *in, double2 *out, int *done);
 // 2D H application with the Laplacian kept in LDS (one pass per plane);
 // part: 2n per-plane sums Re<psi_p, (H psi)_p>; with dotw (state-sized),
-// part2: 2n per-plane sums Re<psi_p, dotw_p>.  h2d_supported says whether
-// the lattice has the kernel; in and out must differ.
+// part2: 2n per-plane sums Re<psi_p, dotw_p>; with esub (n per-state <H>),
+// out = H psi - esub psi.  h2d_supported says whether the lattice has the
+// kernel; in and out must differ.
 bool h2d_supported(const tdw_ctx *c);
 int h2d(tdw_ctx *c, int n, const double2 *in, double2 *out, const double *pots, const double *kt, double *part,
-        const double2 *dotw = nullptr, double *part2 = nullptr);
+        const double2 *dotw = nullptr, double *part2 = nullptr, const double *esub = nullptr);
 // Inner (y[,z]) pass: out = T + D_y(in) + D_z(in) [ddy, ddz].  T may be NULL.
 int launch_inner(tdw_ctx *c, int m, const double2 *in, const double2 *T, double2 *out,
                  double2 *ddy, double2 *ddz, int mode);

@@ -797,18 +797,19 @@ int apply_h_plane(tdw_ctx *c, int it, int n, const double2 *in, double2 *out, co
 {
     (void)it;
     TDW_TRY(ensure_part(c, (size_t)2 * n));
+    if (useqpe) {
+        // a known <H>: subtracted inside the plane pass
+        TDW_TRY(h2d(c, n, in, out, pots, kt, c->d_part, nullptr, nullptr, useqpe));
+        if (d_qpe && d_qpe != useqpe)
+            TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        return 0;
+    }
     TDW_TRY(h2d(c, n, in, out, pots, kt, c->d_part));
     TDW_TIMED(c, "apply_h");
-    const double *q = useqpe;
-    if (!q) {
-        hipLaunchKernelGGL(k_plane_pairs, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, dvol_norm(c),
-                           d_qpe);
-        q = d_qpe;
-    } else if (d_qpe && d_qpe != useqpe) {
-        TDW_HIP(hipMemcpyAsync(d_qpe, useqpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
-    }
+    hipLaunchKernelGGL(k_plane_pairs, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, dvol_norm(c),
+                       d_qpe);
     const long N = (long)c->N, tot = 2L * n * N;
-    hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, q);
+    hipLaunchKernelGGL(k_sub_qpe, dim3((tot + NT - 1) / NT), dim3(NT), 0, c->stream, in, out, n, N, d_qpe);
     return (int)hipGetLastError();
 }
 

@@ -181,6 +181,7 @@ def _sigs():
         "tdw_selfstart": (I, [P, C.POINTER(State), P, P]),
         "tdw_adj_forward_step": (I, [P, C.POINTER(State), I, P, P, C.POINTER(TapeStep)]),
         "tdw_adj_forward_step_to": (I, [P, C.POINTER(State), I, P, P, C.POINTER(TapeStep)]),
+        "tdw_adj_history_entry": (I, [P, I, C.POINTER(TapeStep), P, P, P]),
         "tdw_checkpoint_nwf": (I, [C.c_char_p, C.POINTER(C.c_int)]),
         "tdw_checkpoint_save": (I, [P, C.c_char_p, C.POINTER(State), D, P, P, C.POINTER(CheckpointScalars)]),
         "tdw_checkpoint_load": (I, [P, C.c_char_p, C.POINTER(State), C.POINTER(C.c_double), P, P,
