@@ -89,16 +89,19 @@ def design_units(name, scheme, split=False):
 def modular_units(name):
     """The ASLDA step's kernels (3D, FP64): each kernel's own compulsory
     reads and writes of state-sized arrays, in S per state and launch (its
-    algorithmic bytes at its boundaries).  On lattices with the two-pass
-    kernels a step (AB5AM5) is abm_step1 8, the corrector with the
-    predictor's <H> subtraction folded in 4, and twice: norms 1 + scale 2,
-    the density x and yz passes 1 + 1 (gradients kept in LDS), the H x and
-    yz passes 2 + 3; plus the corrector H's <H> subtraction 3 -- 35 S
-    against the 13 S compulsory for a whole step.  The modular kernels of
-    other lattices: derivatives 3 + 5, densities 5, alpha_wf 2,
-    lap(alpha psi) 2 + 3, apply_h 7 + 3, abm_step4 3."""
-    return {"abm_step1": 8, "abm_step4": 3, "abm_step4q": 4, "norms": 1, "scale": 2, "dens_gx": 1, "dens_gyz": 1,
-            "ah_x": 2,
+    algorithmic bytes at its boundaries).  The folded step of tdw_step_fused
+    (AB5AM5) is a_pred 9 (y, f1..f5 read; f1 completed, y_p and the partial
+    corrector written), a_corr 4, and twice the density x and yz passes
+    1 + 1 (gradients kept in LDS) and the H x and yz passes 2 + 3 -- 27 S
+    against the 13 S compulsory for a whole step.  The modular step
+    (tdw_step) on the same lattices: abm_step1 8, the corrector with the
+    predictor's <H> subtraction folded in 4, twice norms 1 + scale 2, the
+    same density and H passes, and the history entry's <H> subtraction 3 --
+    35 S.  The modular kernels of other lattices: derivatives 3 + 5,
+    densities 5, alpha_wf 2, lap(alpha psi) 2 + 3, apply_h 7 + 3,
+    abm_step4 3."""
+    return {"a_pred": 9, "a_corr": 4, "abm_step1": 8, "abm_step4": 3, "abm_step4q": 4, "norms": 1, "scale": 2,
+            "dens_gx": 1, "dens_gyz": 1, "ah_x": 2,
             "ah_yz": 3, "sub_qpe": 3, "fft_outer_deriv": 3, "fft_inner_deriv": 5, "densities": 5, "alpha_wf": 2,
             "fft_outer": 2, "fft_inner": 3, "apply_h": 10}.get(name, 0)
 
@@ -601,9 +604,9 @@ def run_trap(args):
 
 def run_aslda(args):
     """ASLDA (density-dependent effective mass, current corrections) on the
-    3D uniform workload: the modular step with gradients on both half-steps,
-    FP64, its dominant kernel against the HBM roofline, and the oracle's
-    ASLDA step on a sample of the same states."""
+    3D uniform workload: the folded step (tdw_step_fused) with gradients on
+    both half-steps, FP64, its dominant kernel against the HBM roofline, and
+    the oracle's ASLDA step on a sample of the same states."""
     from wslda_amd import td
     if args.reserve_gb > 0:
         reserve_release(args.reserve_gb)
@@ -618,7 +621,7 @@ def run_aslda(args):
         "value": head["value"], "unit": "qp-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "td-%s 3D uniform, 24x24x24 lattice, %d qp states, AB5AM5, plane-wave BdG "
-                               "+ seeded 1e-6 noise, modular step" % ("SLDAE (a kF = -1.1)" if args.sldae
+                               "+ seeded 1e-6 noise, folded step" % ("SLDAE (a kF = -1.1)" if args.sldae
                                                                       else "ASLDA", g["n"])},
         "roofline": roof, "kernels_ms_per_step": head["kernels_ms_per_step"],
         "energy_drift": head["energy_drift"],

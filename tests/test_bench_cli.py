@@ -34,3 +34,13 @@ def test_design_units_add_up_to_the_measured_step():
 def test_algorithmic_units_add_up_to_the_compulsory_step():
     assert step_sum(bench.algorithmic_units, 55, False) == bench.step_units(55, False) == 13
     assert step_sum(bench.algorithmic_units, 55, True) == bench.step_units(55, True) == 9
+
+
+def test_aslda_units_add_up():
+    """The ASLDA step at AB5AM5: the folded step (one predictor and one
+    corrector pass, two density and two H pass pairs) moves 27 S per
+    state-step, the modular one 35 S."""
+    u = bench.modular_units
+    pair = 2 * (u("dens_gx") + u("dens_gyz") + u("ah_x") + u("ah_yz"))
+    assert u("a_pred") + u("a_corr") + pair == 27
+    assert u("abm_step1") + u("abm_step4q") + 2 * (u("norms") + u("scale")) + u("sub_qpe") + pair == 35

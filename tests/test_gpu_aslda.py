@@ -191,25 +191,48 @@ def test_aslda_td_gauss_1d_golden_on_gpu(torch):
     ctx.close()
 
 
-def test_aslda_uses_modular_step(torch):
-    """tdw_step_fused on an ASLDA context runs the modular step (the fused
-    passes apply the constant-mass H) with the same result; split precision
-    is refused."""
+@pytest.mark.parametrize("scheme", [po.AB3AM4, po.AB4AM5, po.AB5AM5])
+def test_aslda_folded_step_matches_modular(torch, scheme):
+    """tdw_step_fused on a 3D ASLDA context runs the folded step (the
+    normalisations and <H> subtractions carried as per-state scalars into
+    the next pass, the stored y unnormalised in between): after
+    tdw_finalize the state, every history entry, the densities and the
+    potentials agree with the modular step (tdw_step) to 1e-12, across a
+    measurement step in the middle; split precision is refused.  V_ext is on
+    from the start, so that the states move and the history entries
+    (H - <H>) psi are far from zero."""
     from wslda_amd import td
-    say("case: ASLDA fused entry point falls back to the modular step")
-    s = tdgauss.setup(3, functional=po.ASLDA)
+    say("case: ASLDA folded step vs modular, scheme %d" % scheme)
+    s = tdgauss.setup(3, scheme=scheme, functional=po.ASLDA)
+    s["cfg"].vext.t_start, s["cfg"].vext.t_switch = -1.0, 0.5
     ctx, R = gpu_run(torch, s)
     ctx2, R2 = gpu_run(torch, s)
     R2.fused = True
     assert td.lib().tdw_fused_supported(ctx2.h) == 0
     R.initial_measure()
     R2.initial_measure()
-    for k in range(2):
-        R.step(k)
-        R2.step(k)
+    h = {po.AB3AM4: 3, po.AB4AM5: 4, po.AB5AM5: 5}[scheme]
+    for k in range(8):
+        g = k == 3
+        R.step(k, grad_last=g)
+        R2.step(k, grad_last=g)
+        if g:
+            # the measurement step leaves nothing pending
+            ctx.sync()
+            ctx2.sync()
+            assert rel(R2.wf.cpu().numpy(), R.wf.cpu().numpy()) <= TOL
+    R2.finalize()
     ctx.sync()
     ctx2.sync()
-    assert rel(R2.wf.cpu().numpy(), R.wf.cpu().numpy()) == 0.0
+    errs = {"wf": rel(R2.wf.cpu().numpy(), R.wf.cpu().numpy()),
+            "dens": rel(R2.dens.cpu().numpy(), R.dens.cpu().numpy()),
+            "pots": rel(R2.pots.cpu().numpy(), R.pots.cpu().numpy())}
+    for j, (a, b) in enumerate(zip(R2.history()[:h], R.history()[:h])):
+        errs["f%d" % j] = rel(a.cpu().numpy(), b.cpu().numpy())
+    fmax = max(float(t.abs().max()) for t in R.history()[:h])
+    say("  max rel diff: " + ", ".join("%s %.1e" % kv for kv in errs.items()) + "; max |f| %.2e" % fmax)
+    assert fmax > 1e-4
+    assert max(errs.values()) <= TOL, errs
     ctx.close()
     ctx2.close()
     ctx3 = td.TDContext(tdgauss.to_params(s["cfg"], precision=1), s["wf"].shape[0] // 2, 0)

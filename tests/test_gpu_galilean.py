@@ -75,12 +75,13 @@ def test_boost_closed_forms_gpu(torch, functional, sclgth):
     assert not bad, err
 
 
-@pytest.mark.parametrize("fused", [False])
+@pytest.mark.parametrize("fused", [False, True])
 def test_boosted_aslda_gpu_steps_keep_flow(torch, fused):
     """Three ASLDA PECE steps of the boosted state from its exact history
-    on the GPU: rho, tau, j, |nu| and the energy stay put to 1e-11."""
+    on the GPU (the modular step, and the folded step of tdw_step_fused):
+    rho, tau, j, |nu| and the energy stay put to 1e-11."""
     from wslda_amd import td
-    say("case: boosted ASLDA 12^3, 3 PECE steps from the exact history")
+    say("case: boosted ASLDA 12^3, 3 PECE steps from the exact history, fused=%d" % fused)
     cfg, u, wf, fb = solution(po.ASLDA)
     N = NX ** 3
     n = wf.shape[0] // 2

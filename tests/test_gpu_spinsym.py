@@ -180,7 +180,7 @@ def gpu_run(torch, s, fused=False, precision=0):
 
 @pytest.mark.parametrize("dim,functional,fused,nsteps", [(1, po.SLDA, False, 60), (2, po.SLDA, False, 12),
                                                         (3, po.SLDA, False, 4), (3, po.SLDA, True, 4),
-                                                        (3, po.ASLDA, False, 4)])
+                                                        (3, po.ASLDA, False, 4), (3, po.ASLDA, True, 4)])
 def test_pece_steps(torch, dim, functional, fused, nsteps):
     say("case: spin-symmetric PECE steps dim=%d functional=%d fused=%d nsteps=%d" % (dim, functional, fused, nsteps))
     s = ss_setup(dim, functional, NX=16 if functional == po.ASLDA else 12)

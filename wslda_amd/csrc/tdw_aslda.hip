@@ -17,6 +17,7 @@
 // an axis has its Nyquist mode set to zero, as the reference's derivatives
 // do (pca_derivative.cu:169-171).
 #include "tdw_fft.hpp"
+#include "tdw_fused.hpp"
 #include "tdw_physics.hpp"
 #include "tdw_regfft.hpp"
 
@@ -112,7 +113,7 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
 template <int LX, int PX>
 __global__ void __launch_bounds__(AT)
 ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__restrict__ pots, int n, long N, int S,
-     const double2 *__restrict__ twx_g, double k1s, double k2s)
+     const double2 *__restrict__ twx_g, double k1s, double k2s, const double *__restrict__ sc)
 {
     constexpr int E = LX * PX;
     constexpr int KM = (E + AT - 1) / AT;
@@ -129,6 +130,7 @@ ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__res
     const double *alpha = pots + (vcomp ? 5 : 4) * N;
     const double *Ax = pots + (vcomp ? 9 : 6) * N;
     const double sg = vcomp ? 0.25 : -0.25;
+    const double ys = sc ? sc[vcomp ? a2 - n : a2] : 1.0;
     double2 X[KM], acc[KM];
     double al[KM];
 #pragma unroll
@@ -137,7 +139,7 @@ ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__res
         if (e < E) {
             const int i = e / PX, q = e - i * PX;
             const long off = (long)i * S + q;
-            X[k] = y[base + off];
+            X[k] = y[base + off] * ys;
             al[k] = alpha[fb + off];
             A[e] = X[k];
         }
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(nt_ayz(LY *LZ), (LY * LZ <= 576) ? 3 : 2)
 ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__restrict__ out,
       const double *__restrict__ pots, const double *__restrict__ hx, int n, long N, int NX,
       const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y, double k1z,
-      double k2z, double mu_a, double mu_b, double *__restrict__ part)
+      double k2z, double mu_a, double mu_b, double *__restrict__ part, const double *__restrict__ sc)
 {
     constexpr int E = LY * LZ;
     constexpr int EP = LY * (LZ + 1);   // padded plane in LDS: row stride LZ + 1
@@ -203,14 +205,15 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
     const int x = blockIdx.x - a * NX;
     const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
     const long g0 = (long)x * E;
+    const double ys = sc ? sc[a] : 1.0;
     double2 Xu[KM], Xv[KM], Ou[KM], Ov[KM];
     double al[KM], bl[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * NT;
         if (e < E) {
-            Xu[k] = y[gu + e];
-            Xv[k] = y[gv + e];
+            Xu[k] = y[gu + e] * ys;
+            Xv[k] = y[gv + e] * ys;
             Ou[k] = T[gu + e];
             Ov[k] = T[gv + e];
             al[k] = pots[4 * N + g0 + e];
@@ -516,9 +519,11 @@ __global__ void ah_qpe(const double *__restrict__ part, int n, int NX, double dv
     qpe[a] = dv * s;
 }
 
-typedef void (*ahx_fn)(const double2 *, double2 *, const double *, int, long, int, const double2 *, double, double);
+typedef void (*ahx_fn)(const double2 *, double2 *, const double *, int, long, int, const double2 *, double, double,
+                       const double *);
 typedef void (*ahyz_fn)(const double2 *, const double2 *, double2 *, const double *, const double *, int, long, int,
-                        const double2 *, const double2 *, double, double, double, double, double, double, double *);
+                        const double2 *, const double2 *, double, double, double, double, double, double, double *,
+                        const double *);
 
 // x pass for nx points and PX pencils per workgroup (S must be a multiple
 // of PX)
@@ -648,9 +653,10 @@ bool aslda_h_fused_supported(const tdw_ctx *c)
 }
 
 // out = H in with <H> into qpe (no subtraction); T: 2*n*N complex scratch;
-// hx: the fields of aslda_fields; part: n*NX doubles
+// hx: the fields of aslda_fields; part: n*NX doubles; sc (n per-state
+// factors, or null): H acts on sc_a in_a
 int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots,
-                  const double *hx, double *part, double *d_qpe)
+                  const double *hx, double *part, double *d_qpe, const double *sc)
 {
     ahx_fn fx;
     ahyz_fn fy;
@@ -662,7 +668,8 @@ int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
         TDW_TIMED(c, "ah_x");
         const long nblk = 2L * n * (S / px);
         const double k1 = 2.0 * M_PI / c->Lphys[0] / c->nx, k2 = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
-        hipLaunchKernelGGL(fx, dim3((unsigned)nblk), dim3(AT), 0, c->stream, in, T, pots, n, N, S, c->d_tw[0], k1, k2);
+        hipLaunchKernelGGL(fx, dim3((unsigned)nblk), dim3(AT), 0, c->stream, in, T, pots, n, N, S, c->d_tw[0], k1, k2,
+                           sc);
     }
     {
         TDW_TIMED(c, "ah_yz");
@@ -670,10 +677,202 @@ int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
         const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
         const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
         hipLaunchKernelGGL(fy, dim3((unsigned)((long)n * c->nx)), dim3(nt), lds, c->stream, in, T, out, pots, hx, n, N,
-                           c->nx, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, c->p.mu_a, c->p.mu_b, part);
+                           c->nx, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, c->p.mu_a, c->p.mu_b, part, sc);
         hipLaunchKernelGGL(ah_qpe, dim3((n + 255) / 256), dim3(256), 0, c->stream, part, n, c->nx, dvol_norm(c), d_qpe);
     }
     return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The ASLDA step with its streaming work folded into two passes
+// (pca.c:1443-1465 and :1510-1535 in the current-correction form).  The
+// modular form spends 21 S per state-step on the predictor (8), two norm +
+// scale pairs (3 + 3), the corrector (4) and the history entry's <H>
+// subtraction (3).  Here the normalisation and the subtraction are carried as
+// per-state scalars into the next pass over the state, as the fused SLDA step
+// carries them:
+//   a_pred  completes the newest history entry (F1 = f1 - e y_k, y_k = s y,
+//           written back), ABM predictor y_p = y_k - i dt sum b_j F_j and the
+//           partial corrector c = y_k - i dt sum a_j F_j (into the oldest
+//           slot, which it has read), norm partials of y_p;
+//   dens    densities of s_p y_p (weights f s_p^2, gradients in the passes);
+//   H       the two passes on s_p y_p, <H>_p;
+//   a_corr  y_c = -i dt a0 (H - <H>_p s_p y_p) + c, norm partials of y_c;
+//   dens, H again on s_c y_c into the newest slot, without the subtraction.
+// 9 + 2 + 5 + 4 + 2 + 5 = 27 S per state-step (AB5AM5).  The stored y stays
+// unnormalised and f[0] unsubtracted (pending 2) until the next step's a_pred
+// or fused_finalize (y *= s, f0 -= e y) applies them.
+// ---------------------------------------------------------------------------
+static constexpr int PT = 256;
+
+// grid (nchunk, n): block b of state a runs points [b chunk, (b+1) chunk) of
+// both components; part[a * nchunk + b] = sum |y_p|^2 over them.  sk, ek
+// (pending): y_k = sk_a y and f1 -= ek_a y_k.  The partial corrector goes to
+// the oldest history slot (f3 / f4 / f5), read first by the same thread.
+template <int SCH>
+__global__ void __launch_bounds__(PT)
+k_apred(double2 *y, double2 *f1, const double2 *f2, double2 *f3, double2 *f4, double2 *f5,
+        const double *__restrict__ sk, const double *__restrict__ ek, int n, long N, long chunk, double dt,
+        double *__restrict__ part)
+{
+    __shared__ double red[PT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    const double s = sk ? sk[a] : 1.0, e = sk ? ek[a] : 0.0;
+    double2 *fold = SCH == 34 ? f3 : SCH == 45 ? f4 : f5;
+    double nrm = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += PT) {
+        double q = 0.0;
+#pragma unroll
+        for (int comp = 0; comp < 2; comp++) {
+            const long idx = (comp ? (long)n + a : (long)a) * N + i;
+            const double2 yk = y[idx] * s;
+            double2 F1 = f1[idx];
+            if (sk) {
+                F1 = F1 - yk * e;
+                f1[idx] = F1;
+            }
+            const double2 F2 = f2[idx], F3 = f3[idx];
+            const double2 F4 = SCH >= 45 ? f4[idx] : make_double2(0.0, 0.0);
+            const double2 F5 = SCH >= 55 ? f5[idx] : make_double2(0.0, 0.0);
+            double2 sp, sc;
+            abm_sums<SCH>(F1, F2, F3, F4, F5, sp, sc);
+            const double2 yp = yk + imul(sp, -dt);
+            y[idx] = yp;
+            fold[idx] = yk + imul(sc, -dt);
+            q += cnorm(yp);
+        }
+        nrm += q;
+    }
+    nrm = block_sum(nrm, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = nrm;
+}
+
+// y = -i dta0 (hy - ep_a yh) + cin with yh = sp_a y (the normalised
+// prediction), norm partials of the new y as k_apred's
+__global__ void __launch_bounds__(PT)
+k_acorr(double2 *__restrict__ y, const double2 *__restrict__ hy, const double2 *__restrict__ cin,
+        const double *__restrict__ sp, const double *__restrict__ ep, int n, long N, long chunk, double dta0,
+        double *__restrict__ part)
+{
+    __shared__ double red[PT / 64];
+    const int a = blockIdx.y;
+    const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
+    const double s = sp[a], e = ep[a];
+    double nrm = 0.0;
+    for (long i = i0 + threadIdx.x; i < i1; i += PT) {
+        double q = 0.0;
+#pragma unroll
+        for (int comp = 0; comp < 2; comp++) {
+            const long idx = (comp ? (long)n + a : (long)a) * N + i;
+            const double2 yh = y[idx] * s;
+            const double2 yc = imul(hy[idx] - yh * e, -dta0) + cin[idx];
+            y[idx] = yc;
+            q += cnorm(yc);
+        }
+        nrm += q;
+    }
+    nrm = block_sum(nrm, red);
+    if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = nrm;
+}
+
+// s = 1/sqrt(dv sum_k part[a nchunk + k]), s2 = s^2 (the normalisation of
+// pca_kernels.cu:955-976 as a per-state factor)
+__global__ void k_ascale(const double *__restrict__ part, int n, int nchunk, double dv, double *__restrict__ s,
+                         double *__restrict__ s2)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double t = 0.0;
+    for (int k = 0; k < nchunk; k++) t += part[(long)a * nchunk + k];
+    const double sc = 1.0 / sqrt(t * dv);
+    s[a] = sc;
+    s2[a] = sc * sc;
+}
+
+bool aslda_step_supported(const tdw_ctx *c)
+{
+    return varying_mass(c) && c->p.precision == TDW_FP64 && aslda_h_fused_supported(c) &&
+           dens_grad_fused_supported(c);
+}
+
+static int ascale(tdw_ctx *c, int n, int nchunk, double *s, double *s2)
+{
+    TDW_TIMED(c, "a_norm");
+    hipLaunchKernelGGL(k_ascale, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_part, n, nchunk, dvol_norm(c), s,
+                       s2);
+    return (int)hipGetLastError();
+}
+
+// densities, tau, potentials and H of the state s_a y (H into out, <H> into e)
+static int aslda_half(tdw_ctx *c, tdw_state *st, int it, const double2 *y, const double *s, const double *s2,
+                      double2 *out, double *e)
+{
+    const int n = st->n;
+    TDW_TRY(dens_grad_fused(c, n, y, c->d_fbeta, s2, st->dens));
+    TDW_TRY(density_tau(c, st->dens));
+    TDW_TRY(step_potentials(c, it, st->dens, st->pots));
+    TDW_TRY(hkf_fields(c, st->pots, true));
+    const double *hp = h_view(c, it, st->pots);
+    if (!hp) return TDW_EINVAL;
+    TDW_TRY(aslda_fields(c, hp));
+    TDW_TRY(ensure_part(c, (size_t)n * c->nx + n));
+    return aslda_h_fused(c, n, y, out, (double2 *)st->ddx, hp, c->d_hx, c->d_part, e, s);
+}
+
+int aslda_fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    if (!s->ddx || c->pending == 1) return TDW_EINVAL;
+    const int n = s->n;
+    const long N = (long)c->N;
+    const int h = c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5;
+    TDW_TRY(fused_state_vecs(c));
+    const long chunk = N < 4096 ? N : 4096;
+    const int nchunk = (int)((N + chunk - 1) / chunk);
+    TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+    double2 *y = (double2 *)s->wf;
+    double2 **f = (double2 **)s->f;
+    double2 *H = (double2 *)s->lap;
+    double *qpe = s->qpe ? s->qpe : c->d_qpe;
+    const bool pend = c->pending == 2;
+    const dim3 g(nchunk, n);
+    // ---- predictor ----
+    {
+        TDW_TIMED(c, "a_pred");
+        const double *sk = pend ? c->d_scale : nullptr, *ek = pend ? c->d_qpe : nullptr;
+        const double dt = c->p.dt;
+        if (h == 3)
+            hipLaunchKernelGGL(k_apred<34>, g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
+                               chunk, dt, c->d_part);
+        else if (h == 4)
+            hipLaunchKernelGGL(k_apred<45>, g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
+                               chunk, dt, c->d_part);
+        else
+            hipLaunchKernelGGL(k_apred<55>, g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
+                               chunk, dt, c->d_part);
+        TDW_HIP(hipGetLastError());
+    }
+    c->pending = 0;
+    TDW_TRY(ascale(c, n, nchunk, c->d_sp, c->d_s2));
+    TDW_TRY(aslda_half(c, s, it + 1, y, c->d_sp, c->d_s2, H, c->d_ep));
+    // ---- corrector ----
+    {
+        TDW_TIMED(c, "a_corr");
+        const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
+        TDW_TRY(ensure_part(c, (size_t)n * nchunk));
+        hipLaunchKernelGGL(k_acorr, g, dim3(PT), 0, c->stream, y, H, f[h - 1], c->d_sp, c->d_ep, n, N, chunk,
+                           c->p.dt * a0, c->d_part);
+        TDW_HIP(hipGetLastError());
+    }
+    TDW_TRY(ascale(c, n, nchunk, c->d_scale, c->d_s2));
+    tdw_complex *tmp = s->f[h - 1];
+    for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
+    s->f[0] = tmp;
+    TDW_TRY(aslda_half(c, s, it + 1, y, c->d_scale, c->d_s2, (double2 *)s->f[0], c->d_qpe));
+    if (qpe != c->d_qpe) TDW_HIP(hipMemcpyAsync(qpe, c->d_qpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+    c->pending = 2;
+    // a measurement step leaves the arrays as the reference's step does
+    return grad_last ? fused_finalize(c, s) : 0;
 }
 
 }  // namespace tdw

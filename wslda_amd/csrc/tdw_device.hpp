@@ -49,6 +49,24 @@ template <typename C> __device__ __forceinline__ C cvt(double2 a);
 template <> __device__ __forceinline__ double2 cvt<double2>(double2 a) { return a; }
 template <> __device__ __forceinline__ float2 cvt<float2>(double2 a) { return to32(a); }
 
+// ABM coefficient sums of the predictor (sp) and of the corrector's history
+// part (sc) over the newest entries a, b, d, e, g (tdwslda_ode_integrator.h:26-97)
+template <int SCH>
+__device__ __forceinline__ void abm_sums(double2 a, double2 b, double2 d, double2 e, double2 g, double2 &sp,
+                                         double2 &sc)
+{
+    if constexpr (SCH == 34) {
+        sp = a * (23. / 12.) - b * (16. / 12.) + d * (5. / 12.);
+        sc = a * (19. / 24.) - b * (5. / 24.) + d * (1. / 24.);
+    } else if constexpr (SCH == 45) {
+        sp = a * (55. / 24.) - b * (59. / 24.) + d * (37. / 24.) - e * (9. / 24.);
+        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
+    } else {
+        sp = a * (1901. / 720.) - b * (2774. / 720.) + d * (2616. / 720.) - e * (1274. / 720.) + g * (251. / 720.);
+        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
+    }
+}
+
 // Lattice momentum of index i on an axis of n points and length L, standard
 // FFT order (hpc-engine/pca_derivative.cu:150-160).
 __device__ __forceinline__ double kval(int i, int n, double L)

@@ -49,25 +49,6 @@ static constexpr int FT = 256;
 // the state y stays FP64 and every update of it is done in FP64.
 template <bool SP> using hc_t = typename std::conditional<SP, float2, double2>::type;
 
-// ---------------------------------------------------------------------------
-// ABM coefficient sums (tdwslda_ode_integrator.h:26-97)
-// ---------------------------------------------------------------------------
-template <int SCH>
-__device__ __forceinline__ void abm_sums(double2 a, double2 b, double2 d, double2 e, double2 g, double2 &sp,
-                                         double2 &sc)
-{
-    if constexpr (SCH == 34) {
-        sp = a * (23. / 12.) - b * (16. / 12.) + d * (5. / 12.);
-        sc = a * (19. / 24.) - b * (5. / 24.) + d * (1. / 24.);
-    } else if constexpr (SCH == 45) {
-        sp = a * (55. / 24.) - b * (59. / 24.) + d * (37. / 24.) - e * (9. / 24.);
-        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
-    } else {
-        sp = a * (1901. / 720.) - b * (2774. / 720.) + d * (2616. / 720.) - e * (1274. / 720.) + g * (251. / 720.);
-        sc = a * (646. / 720.) - b * (264. / 720.) + d * (106. / 720.) - e * (19. / 720.);
-    }
-}
-
 template <int LX, typename C>
 __device__ __forceinline__ void dxx_inplace(C (&r)[LX], double kscale)
 {
@@ -744,7 +725,7 @@ static int ensure_f(tdw_ctx *c, size_t nd)
     return 0;
 }
 
-static int ensure_state_vecs(tdw_ctx *c)
+int fused_state_vecs(tdw_ctx *c)
 {
     if (c->d_sp) return 0;
     TDW_HIP(hipMalloc(&c->d_sp, sizeof(double) * c->nwf));
@@ -899,8 +880,11 @@ template <bool SP> static int fixup_x(tdw_ctx *c, tdw_state *s)
 
 int fused_finalize(tdw_ctx *c, tdw_state *s)
 {
-    if (!c->pending) return 0;
+    const int p = c->pending;
+    if (!p) return 0;
     c->pending = 0;
+    // the ASLDA step's pending state lacks only the scale and the subtraction
+    if (p == 2) return fixup<false>(c, (double2 *)s->wf, s->f[0], s->n, 1);
     if (split_mode(c)) return fixup_x<true>(c, s);
     return fixup_x<false>(c, s);
 }
@@ -952,7 +936,7 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     const long N = (long)c->N;
     const int S = c->S;
     const int wave = (S % 64) == 0;
-    TDW_TRY(ensure_state_vecs(c));
+    TDW_TRY(fused_state_vecs(c));
     // partial sums: the norm partials (then the yz pass's <H> partials) at
     // the start of d_fpart, the predictor's x-kinetic partials behind both
     const int per = wave ? S / 64 : S;
@@ -1062,7 +1046,7 @@ int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
                   const double *use_e)
 {
     if (!fused_supported(c) || c->p.precision != TDW_FP64) return TDW_EINVAL;
-    TDW_TRY(ensure_state_vecs(c));
+    TDW_TRY(fused_state_vecs(c));
     TDW_TRY(ensure_f(c, (size_t)n * c->nx));
     const int S = c->S;
     const double kxs = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
@@ -1094,6 +1078,7 @@ int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
 
 int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
 {
+    if (aslda_step_supported(c)) return aslda_fused_step(c, s, it, grad_last);
     if (!fused_supported(c)) {
         // the split form exists only as the fused step: its history arrays
         // hold FP32 values, which the FP64 step cannot continue from

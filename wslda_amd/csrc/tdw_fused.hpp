@@ -9,6 +9,8 @@ int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
 // history entry, leaving the arrays exactly as the reference step does.
 int fused_finalize(tdw_ctx *c, tdw_state *s);
 void fused_release(tdw_ctx *c);
+// the per-state scalars of the fused steps (d_sp, d_s2, d_ep), allocated on first use
+int fused_state_vecs(tdw_ctx *c);
 // H applied to arbitrary FP64 vectors (2n rows) on the fused x and yz
 // passes, T a state-sized scratch; with qpe, <H> per state into qpe and
 // out -= <H> in (as apply_h), without it H in only (as apply_h_only).  3D

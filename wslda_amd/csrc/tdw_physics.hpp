@@ -177,10 +177,19 @@ int apply_h_aslda(tdw_ctx *c, int n, const double2 *in, double2 *out, const doub
 int abm4q(tdw_ctx *c, int n, const double2 *hy, double2 *y, const double2 *fl, const double *qpe);
 // ASLDA H on 3D lattices in an x-pencil and a yz-plane pass (tdw_aslda.hip):
 // out = H in without the subtraction, <H> into d_qpe; T state-sized scratch,
-// hx the grid fields of aslda_fields, part n*NX doubles
+// hx the grid fields of aslda_fields, part n*NX doubles; with sc (n
+// per-state factors) H acts on sc_a in_a
 bool aslda_h_fused_supported(const tdw_ctx *c);
+// hx = [lap alpha_a, lap alpha_b, div A_a, div A_b] of the potentials into c->d_hx
+int aslda_fields(tdw_ctx *c, const double *pots);
 int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots,
-                  const double *hx, double *part, double *d_qpe);
+                  const double *hx, double *part, double *d_qpe, const double *sc = nullptr);
+// The ASLDA step on 3D lattices of the two-pass kernels (FP64) with the ABM
+// updates, normalisations and <H> subtractions carried as per-state scalars:
+// it leaves y unnormalised and f[0] without its subtraction (c->pending = 2)
+// until the next such step or fused_finalize applies them.
+bool aslda_step_supported(const tdw_ctx *c);
+int aslda_fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
 // Densities with gradients (rho, nu, tau before its lap(rho) correction, j)
 // on 3D lattices in an x and a yz pass that keep the gradients in LDS
 // (tdw_aslda.hip); the same dens block as densities(..., grad = 1)
