@@ -229,8 +229,12 @@ int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
  * part of its kinetic term and before the <H> subtraction: the next fused
  * step, or tdw_finalize(), which every other entry point calls first,
  * completes both.  Densities, potentials and qpe are exact after every step.
- * Lattices the fused kernels are not built for, and ASLDA, run the modular
- * step (tdw_fused_supported() says which).
+ * ASLDA and SLDAE (varying effective mass) on 3D lattices of the two-pass
+ * kernels run the folded step: the modular step's density and H passes with
+ * the normalisations and the <H> subtraction carried as per-state scalars,
+ * so between steps wf is unnormalised and f[0] lacks only the subtraction
+ * (same completion rules).  Other lattices run the modular step
+ * (tdw_fused_supported() says whether the constant-mass fused passes run).
  *
  * Split precision (params.precision == TDW_SPLIT): the x / yz FFTs and the
  * kinetic term run in FP32 complex, and f[0..4] hold FP32 complex values
