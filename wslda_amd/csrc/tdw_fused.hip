@@ -826,19 +826,19 @@ static bool split_mode(const tdw_ctx *c) { return c->p.precision == TDW_SPLIT; }
 // thread counts; FP64 24^2 and 32^2 planes run it with nt_hyz's
 static bool hyz_big(const tdw_ctx *c) { return c->ny * c->nz > 1024; }
 
-// threads of the yz pass
-static int hyz_nt(const tdw_ctx *c)
+// threads of the yz pass in the precision SP of its arrays
+template <bool SP> static int hyz_nt(const tdw_ctx *c)
 {
     if (!hyz_big(c)) return nt_hyz(c->ny * c->nz);
-    return split_mode(c) ? HyzBig<true>::NT : HyzBig<false>::NT;
+    return HyzBig<SP>::NT;
 }
 
 // dynamic LDS of the yz pass: the components it holds at once
-static size_t hyz_lds(const tdw_ctx *c)
+template <bool SP> static size_t hyz_lds(const tdw_ctx *c)
 {
-    const size_t plane = (split_mode(c) ? sizeof(float2) : sizeof(double2)) * (size_t)c->ny * (c->nz + 1);
+    const size_t plane = (SP ? sizeof(float2) : sizeof(double2)) * (size_t)c->ny * (c->nz + 1);
     if (!hyz_big(c)) return 2 * plane;
-    return (split_mode(c) ? HyzBig<true>::NC : HyzBig<false>::NC) * plane;
+    return HyzBig<SP>::NC * plane;
 }
 
 bool fused_supported(const tdw_ctx *c)
@@ -852,7 +852,8 @@ bool fused_supported(const tdw_ctx *c)
     FusedFns<false>::hyz_fn h;
     if (!FusedFns<false>::xpass(c->nx, c->p.scheme, &p, &cf)) return false;
     if (!FusedFns<false>::hyz(c->ny, c->nz, true, &h)) return false;
-    return hyz_lds(c) <= (size_t)c->max_lds;
+    // the FP64 yz pass runs in both forms (H of FP64 vectors, gradient steps)
+    return hyz_lds<false>(c) <= (size_t)c->max_lds && hyz_lds<true>(c) <= (size_t)c->max_lds;
 }
 
 static int nhist(const tdw_ctx *c) { return c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5; }
@@ -911,14 +912,14 @@ static int hyz(tdw_ctx *c, int n, const hc_t<SP> *y, const hc_t<SP> *T, hc_t<SP>
     const double kzs = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
     typename FusedFns<SP>::hyz_fn h;
     FusedFns<SP>::hyz(c->ny, c->nz, xpart == nullptr, &h);
-    const size_t lds = hyz_lds(c);
+    const size_t lds = hyz_lds<SP>(c);
     const hc_t<SP> *twy, *twz;
     if constexpr (SP) { twy = c->d_tw32[1]; twz = c->d_tw32[2]; }
     else { twy = c->d_tw[1]; twz = c->d_tw[2]; }
     // the norm partials are consumed already: reuse the partial buffer
     {
         TDW_TIMED(c, "f_hyz");
-        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(hyz_nt(c)), lds, c->stream, y, T, out, pots,
+        hipLaunchKernelGGL(h, dim3((long)n * c->nx), dim3(hyz_nt<SP>(c)), lds, c->stream, y, T, out, pots,
                            scale, n,
                            (long)c->N, c->nx, twy, twz, kys, kzs, hp, c->d_fpart);
         TDW_HIP(hipGetLastError());
@@ -1011,11 +1012,21 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
         return 0;
     }
     // measurement step: normalise in place, then the reference's derivative /
-    // density / tau / potential / H sequence, in FP64 in both forms
+    // density / tau / potential / H sequence, in FP64 in both forms.  Where
+    // the lattice has the in-pass gradient density kernels, the densities
+    // with tau and j come from them and H from the fused x and yz passes
+    // (no derivative array is stored); elsewhere the four derivative arrays
+    // and the modular density and H kernels.
     TDW_TRY(fixup<SP>(c, y, nullptr, n, 0));
-    TDW_TRY(derivatives(c, 2 * n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap));
-    TDW_TRY(densities(c, n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap,
-                      c->d_fbeta, nullptr, c->d_kt, c->d_cnt, s->dens, 1));
+    const bool gpass = dens_grad_fused_supported(c);
+    if (gpass) {
+        TDW_TRY(dens_grad_fused(c, n, y, c->d_fbeta, nullptr, s->dens));
+    } else {
+        TDW_TRY(derivatives(c, 2 * n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz,
+                            (double2 *)s->lap));
+        TDW_TRY(densities(c, n, y, (double2 *)s->ddx, (double2 *)s->ddy, (double2 *)s->ddz, (double2 *)s->lap,
+                          c->d_fbeta, nullptr, c->d_kt, c->d_cnt, s->dens, 1));
+    }
     TDW_TRY(density_tau(c, s->dens));
     TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
@@ -1026,12 +1037,17 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     s->f[0] = tmp;
     if constexpr (SP) {
         // exact H psi - <H> psi into ddx (free after the densities), then FP32 history
-        TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->ddx, (double2 *)s->lap, hp, c->d_kt, nullptr, qpe));
+        if (gpass)
+            TDW_TRY(fused_apply_h(c, n, y, (double2 *)s->ddx, (double2 *)s->ddy, hp, qpe));
+        else
+            TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->ddx, (double2 *)s->lap, hp, c->d_kt, nullptr, qpe));
         const long tot = 2L * n * N;
         TDW_TIMED(c, "f_to32");
         hipLaunchKernelGGL(f_to32, dim3((tot + FT - 1) / FT), dim3(FT), 0, c->stream, (const double2 *)s->ddx,
                            (float2 *)s->f[0], tot);
         TDW_HIP(hipGetLastError());
+    } else if (gpass) {
+        TDW_TRY(fused_apply_h(c, n, y, (double2 *)s->f[0], (double2 *)s->ddx, hp, qpe));
     } else {
         TDW_TRY(apply_h(c, it + 1, n, y, (double2 *)s->f[0], (double2 *)s->lap, hp, c->d_kt, nullptr, qpe));
     }
@@ -1045,7 +1061,8 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
 int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots, double *qpe,
                   const double *use_e)
 {
-    if (!fused_supported(c) || c->p.precision != TDW_FP64) return TDW_EINVAL;
+    // FP64 vectors and FP64 passes, in either form of the context
+    if (!fused_supported(c)) return TDW_EINVAL;
     TDW_TRY(fused_state_vecs(c));
     TDW_TRY(ensure_f(c, (size_t)n * c->nx));
     const int S = c->S;
