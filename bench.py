@@ -354,7 +354,8 @@ def run_adjoint(args, calibrate=True):
     theta = s["theta"]
     g = adjoint.disk_weights(p, s["center"], s["radius"])
     prob = adjoint.AdjointProblem(p, s["wf"], s["pots"], s["fbeta"], s["kt"], s["cnt"], nsteps=args.adj_steps,
-                                  K=args.adj_k, dens_grad=g, taped_segments=args.adj_taped)
+                                  K=args.adj_k, dens_grad=g, taped_segments=args.adj_taped,
+                                  tape_history=args.adj_tape_history == 1)
     prob.set_vext(adjoint.beams_vext(theta))
     prob.forward(keep_checkpoints=False)     # warm-up
     torch.cuda.synchronize()
@@ -533,13 +534,16 @@ def run_tube_adjoint(args):
     import torch
     from wslda_amd import adjoint, problems, td
     nst = args.tadj_states
+    # one taped segment fits either way at this size, so the tape keeps the
+    # history entries rather than rebuilding them in the sweep
+    full_tape = args.adj_tape_history != 0
     w = problems.config5(torch, precision=td.FP64, first=nst)
     theta = np.array([0.3 * w["eF"], 3.0])
 
     def problem(fp, taped):
         p = td.Params.from_buffer_copy(w["params"])
         prob = adjoint.AdjointProblem(p, w["wf"], w["pots"], w["fbeta"], nsteps=args.adj_steps, K=args.adj_k,
-                                      taped_segments=taped, forward_precision=fp)
+                                      taped_segments=taped, forward_precision=fp, tape_history=full_tape)
         prob.state_observable = adjoint.PlaneCurrent(prob.ctx, w["fbeta"], axis=2)
         prob.set_vext(adjoint.barrier_vext(theta))
         return prob
@@ -579,9 +583,10 @@ def run_tube_adjoint(args):
         "n_gpus": 1, "dtype": "f32-operator/f64-state forward, f64 backward", "data": "synthetic",
         "config": {"workload": "td-SLDA 3D tube 32x32x96, barrier 0.3 eF width 3 dx, pi phase step, %d of the "
                                "uniform solution's 12062 states (one GPU's memory share), AB5AM5, %d steps, "
-                               "checkpoints every %d, last %d segment(s) taped; observable: the current "
-                               "through the barrier plane at the final time" % (n, args.adj_steps, args.adj_k,
-                                                                               args.adj_taped)},
+                               "checkpoints every %d, last %d segment(s) taped (%s); observable: the current "
+                               "through the barrier plane at the final time" % (
+                                   n, args.adj_steps, args.adj_k, args.adj_taped,
+                                   "y and history entries" if full_tape else "y per step")},
         "split_forward_s": t_fwd, "fp64_fused_forward_s": t_f64, "forward_with_checkpoints_s": t_ck,
         "backward_s": t_bwd, "over_fp64_forward": (t_ck + t_bwd) / t_f64,
         "observable": O, "observable_split_forward": O_split, "observable_fp64_forward": O64,
@@ -690,6 +695,9 @@ def main():
                          "FP64 backward) on one GPU's memory share; --adj-steps, --adj-k and --adj-taped apply "
                          "(defaults here 1000, 40, 1)")
     ap.add_argument("--tadj-states", type=int, default=360, help="states of --tube-adjoint")
+    ap.add_argument("--adj-tape-history", type=int, default=-1,
+                    help="1: the adjoint tape keeps the history entries (2 state arrays per step); 0: y alone, "
+                         "the entries rebuilt in the backward sweep (-1: the workload's default)")
     ap.add_argument("--aslda", action="store_true",
                     help="instead of the step benchmark: the ASLDA step on the 3D uniform workload (24^3)")
     ap.add_argument("--sldae", action="store_true",

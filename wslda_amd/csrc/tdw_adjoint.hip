@@ -673,6 +673,13 @@ static int adj_forward(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_out, dou
     // lattices with a plane kernel form it inside the H application
     const bool plane = h2d_supported(c);
     const bool fused = !plane && fused_supported(c);
+    if (fused && s->ddx && s->ddy) {
+        // 3D lattices of the fused step: its passes (finalised at the end of
+        // the step), with the tape read from its own per-state scalars
+        TDW_TRY(fused_adj_step(c, s, it, y_out, cbuf, f_next, t));
+        TDW_HIP(hipMemcpyAsync(qpe, t->scal + 3 * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+        return 0;
+    }
     if (y_out) {
         TDW_TRY(abm1_to(c, n, CD2(s->wf), D2(y_out), cbuf, D2(s->f[0]), D2(s->f[1]), D2(s->f[2]), D2(s->f[3]),
                         D2(s->f[4])));

@@ -251,9 +251,10 @@ f_corr(double2 *__restrict__ y, const hc_t<SP> *hin, const hc_t<SP> *__restrict_
     put_norm_part(xk, wave != 0, P, Ptot, xpart);
 }
 
-// s = 1/sqrt(norm * dV), s2 = s^2 per state; part holds `per` entries per component
+// s = 1/sqrt(norm * dV), s2 = s^2 per state (and norm * dV itself into nrm
+// when given); part holds `per` entries per component
 __global__ void f_norm_final(const double *__restrict__ part, int n, int per, double dv, double *__restrict__ s,
-                             double *__restrict__ s2)
+                             double *__restrict__ s2, double *__restrict__ nrm_out)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
@@ -265,6 +266,7 @@ __global__ void f_norm_final(const double *__restrict__ part, int n, int per, do
     const double sc = 1.0 / sqrt(nrm);
     s[a] = sc;
     s2[a] = sc * sc;
+    if (nrm_out) nrm_out[a] = nrm;
 }
 
 // ---------------------------------------------------------------------------
@@ -890,12 +892,12 @@ int fused_finalize(tdw_ctx *c, tdw_state *s)
     return fixup_x<false>(c, s);
 }
 
-static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave)
+static int xpass_norm(tdw_ctx *c, int n, double *s_out, double *s2_out, int wave, double *nrm_out = nullptr)
 {
     const int per = wave ? c->S / 64 : c->S;
     TDW_TIMED(c, "f_norm");
     hipLaunchKernelGGL(f_norm_final, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->d_fpart, n, per,
-                       dvol_norm(c), s_out, s2_out);
+                       dvol_norm(c), s_out, s2_out, nrm_out);
     return (int)hipGetLastError();
 }
 
@@ -930,9 +932,22 @@ static int hyz(tdw_ctx *c, int n, const hc_t<SP> *y, const hc_t<SP> *T, hc_t<SP>
     return (int)hipGetLastError();
 }
 
-template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+// tape (FP64 only, no gradient step): also record what the adjoint's
+// backward step reads (norms, <H>, densities and potentials of both halves,
+// in the tape's slots); cbuf: the partial corrector's buffer instead of the
+// oldest history entry (which then stays intact); f_next: the new history
+// entry's buffer instead of the dropped oldest one.
+static int tape_copy(tdw_ctx *c, const double *src, double *dst, long nd)
+{
+    return (int)hipMemcpyAsync(dst, src, sizeof(double) * nd, hipMemcpyDeviceToDevice, c->stream);
+}
+
+template <bool SP>
+static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int grad_last, tdw_tape_step *tape = nullptr,
+                        double2 *cbuf = nullptr, tdw_complex *f_next = nullptr)
 {
     typedef hc_t<SP> HC;
+    if ((tape || cbuf || f_next) && (SP || grad_last)) return TDW_EINVAL;
     const int n = s->n, h = nhist(c);
     const long N = (long)c->N;
     const int S = c->S;
@@ -955,17 +970,18 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     double2 *y = (double2 *)s->wf;
     HC *T = (HC *)s->ddx, *H = (HC *)s->lap;
     HC **f = (HC **)s->f;
+    HC *fold = cbuf ? (HC *)cbuf : f[h - 1];
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
 
     // ---- predictor ----
     {
         TDW_TIMED(c, "f_pred");
-        hipLaunchKernelGGL(fp, dim3(nblk), dim3(FT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], f[h - 1],
+        hipLaunchKernelGGL(fp, dim3(nblk), dim3(FT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], fold,
                            xpart, c->d_scale, c->d_qpe, c->pending, n, N, S, c->p.dt, kxs, wave, c->d_fpart);
         TDW_HIP(hipGetLastError());
     }
     c->pending = 0;
-    TDW_TRY(xpass_norm(c, n, c->d_sp, c->d_s2, wave));
+    TDW_TRY(xpass_norm(c, n, c->d_sp, c->d_s2, wave, tape ? tape->scal : nullptr));
     // the prediction: y itself (plain) or its FP32 copy in the consumed
     // history slot (split)
     const HC *yp = SP ? (const HC *)f[h - 1] : (const HC *)y;
@@ -974,21 +990,23 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
     else
         TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
                           s->dens, 0));
+    if (tape) TDW_TRY(tape_copy(c, s->dens, tape->dens_p, 12 * N));
     TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
     TDW_TRY(hkf_fields(c, s->pots, true));
+    if (tape) TDW_TRY(tape_copy(c, s->pots, tape->pots_p, 12 * N));
     const double *hp = h_view(c, it + 1, s->pots);
     if (!hp) return TDW_EINVAL;
-    TDW_TRY(hyz<SP>(c, n, yp, T, H, hp, c->d_sp, c->d_ep, nullptr, xpart, per, c->d_s2));
+    TDW_TRY(hyz<SP>(c, n, yp, T, H, hp, c->d_sp, c->d_ep, tape ? tape->scal + n : nullptr, xpart, per, c->d_s2));
 
     // ---- corrector ----
     const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
     {
         TDW_TIMED(c, "f_corr");
-        hipLaunchKernelGGL(fc, dim3(nblk), dim3(FT), 0, c->stream, y, H, f[h - 1], xpart, c->d_sp, c->d_ep, n, N,
+        hipLaunchKernelGGL(fc, dim3(nblk), dim3(FT), 0, c->stream, y, H, fold, xpart, c->d_sp, c->d_ep, n, N,
                            S, c->p.dt * a0, kxs, wave, c->d_fpart, SP ? (float2 *)H : nullptr);
         TDW_HIP(hipGetLastError());
     }
-    TDW_TRY(xpass_norm(c, n, c->d_scale, c->d_s2, wave));
+    TDW_TRY(xpass_norm(c, n, c->d_scale, c->d_s2, wave, tape ? tape->scal + 2 * n : nullptr));
     if (!grad_last) {
         // the corrected state: y (plain) or its FP32 copy in H's buffer (split)
         const HC *yc = SP ? (const HC *)H : (const HC *)y;
@@ -997,15 +1015,18 @@ template <bool SP> static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int
         else
             TDW_TRY(densities(c, n, y, nullptr, nullptr, nullptr, nullptr, c->d_fbeta, c->d_s2, c->d_kt, c->d_cnt,
                               s->dens, 0));
+        if (tape) TDW_TRY(tape_copy(c, s->dens, tape->dens_c, 12 * N));
         TDW_TRY(step_potentials(c, it + 1, s->dens, s->pots));
         TDW_TRY(hkf_fields(c, s->pots, true));
+        if (tape) TDW_TRY(tape_copy(c, s->pots, tape->pots_c, 12 * N));
         hp = h_view(c, it + 1, s->pots);
         if (!hp) return TDW_EINVAL;
-        tdw_complex *tmp = s->f[h - 1];
+        tdw_complex *tmp = f_next ? f_next : s->f[h - 1];
         for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
         s->f[0] = tmp;
-        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], hp, c->d_scale, c->d_qpe, qpe != c->d_qpe ? qpe : nullptr,
-                        xpart, per, c->d_s2));
+        double *e2 = tape ? tape->scal + 3 * n : (qpe != c->d_qpe ? qpe : nullptr);
+        TDW_TRY(hyz<SP>(c, n, yc, T, (HC *)s->f[0], hp, c->d_scale, c->d_qpe, e2, xpart, per, c->d_s2));
+        if (tape && qpe != c->d_qpe) TDW_TRY(tape_copy(c, c->d_qpe, qpe, n));
         // pending: y unscaled, and f[0] without the x kinetic part and the
         // <H> subtraction (the next predictor or tdw_finalize adds both)
         c->pending = 1;
@@ -1091,6 +1112,24 @@ int fused_apply_h(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T
         TDW_HIP(hipGetLastError());
     }
     return 0;
+}
+
+// The adjoint's taped step on the fused FP64 passes: y goes to y_out (NULL:
+// in place) and the step is finalised at its end, so y_out holds the
+// normalised state and the new history entry is complete, as the modular
+// taped step leaves them.
+int fused_adj_step(tdw_ctx *c, tdw_state *s, int it, tdw_complex *y_out, double2 *cbuf, tdw_complex *f_next,
+                   tdw_tape_step *t)
+{
+    if (!fused_supported(c) || split_mode(c)) return TDW_EINVAL;
+    TDW_TRY(fused_finalize(c, s));
+    if (y_out && y_out != s->wf) {
+        TDW_HIP(hipMemcpyAsync(y_out, s->wf, sizeof(double2) * 2 * (size_t)s->n * c->N, hipMemcpyDeviceToDevice,
+                               c->stream));
+        s->wf = y_out;
+    }
+    TDW_TRY(fused_step_t<false>(c, s, it, 0, t, cbuf, f_next));
+    return fused_finalize(c, s);
 }
 
 int fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
