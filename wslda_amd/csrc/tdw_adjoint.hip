@@ -244,13 +244,17 @@ k_pot_vjp(FParams f, VextDev V, long N, const double *__restrict__ dens, const d
     }
 }
 
-__global__ void k_gsum(const double *__restrict__ gpart, int nblk, double *__restrict__ out)
+// out[p] += sum over the nblk workgroup partials of parameter p: one
+// workgroup per parameter, a fixed strided order per thread and a fixed tree
+// (deterministic for a given nblk)
+__global__ void __launch_bounds__(AT) k_gsum(const double *__restrict__ gpart, int nblk, double *__restrict__ out)
 {
-    const int p = threadIdx.x;
-    if (p >= NTHETA) return;
+    __shared__ double red[AT / 64];
+    const int p = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += gpart[(long)b * NTHETA + p];
-    out[p] += s;
+    for (int b = threadIdx.x; b < nblk; b += AT) s += gpart[(long)b * NTHETA + p];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[p] += s;
 }
 
 // ---------------------------------------------------------------------------
@@ -568,7 +572,7 @@ static int pot_vjp(tdw_ctx *c, int it, const double *dens, const double *pots_pr
     TDW_TRY(ensure_part(c, (size_t)nblk * NTHETA));
     hipLaunchKernelGGL(k_pot_vjp, dim3(nblk), dim3(PT), 0, c->stream, fparams(c), make_vext(c, it), N, dens,
                        pots_prev, lU, dens_add, lD, lUprev, c->d_part);
-    hipLaunchKernelGGL(k_gsum, dim3(1), dim3(64), 0, c->stream, c->d_part, nblk, theta_grad);
+    hipLaunchKernelGGL(k_gsum, dim3(NTHETA), dim3(AT), 0, c->stream, c->d_part, nblk, theta_grad);
     return (int)hipGetLastError();
 }
 
