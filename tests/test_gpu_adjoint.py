@@ -160,3 +160,34 @@ def test_light_tape_matches_full_tape(torch, dim, shape, nst, scheme, steps, K, 
         assert dg <= 1e-13 and dl <= 1e-13
         # the light tape keeps no checkpoint for a taped segment after another
         assert out[False][2] < out[True][2]
+
+
+def test_fused_taped_step_matches_modular_step(torch):
+    """On 3D lattices of the fused step the taped forward step runs the fused
+    FP64 passes (finalised at each step's end).  From the same start, its
+    tape agrees with the modular step (tdw_step) step by step to 1e-12: y
+    after the step, the new history entry, the densities and potentials of
+    the corrected state and its <H>."""
+    from wslda_amd import td
+    say("case: fused taped step vs modular step, 3D 8^3")
+    steps, K = 6, 6
+    prob, theta = setup_problem(3, (8, 8, 8), 48, 45, steps, K, 99, tape_history=True)
+    prob.set_vext(gauss_slots(theta, 3))
+    prob.forward()
+    ty, tf, tscal, tgrid, _ = prob._tape(prob.taped[0])
+    R = td.TDRun(prob.ctx, prob.wf0.clone(), prob.pots0.clone().cpu().numpy())
+    R.initial_measure()
+    n = prob.n
+    worst = 0.0
+    for k in range(steps):
+        R.step(prob.it0 + k, grad_last=False)
+        prob.ctx.sync()
+        errs = [float((ty[k + 1] - R.wf).abs().max() / R.wf.abs().max()),
+                float((tf[k] - R.history()[0]).abs().max() / R.history()[0].abs().max()),
+                float((tgrid[k, 2] - R.dens).abs().max() / R.dens.abs().max()),
+                float((tgrid[k, 3] - R.pots).abs().max() / R.pots.abs().max()),
+                float((tscal[k, 3 * n:] - R.qpe).abs().max() / R.qpe.abs().max())]
+        worst = max(worst, max(errs))
+    say("  max rel diff over %d steps (y, f_new, dens, pots, <H>): %.2e" % (steps, worst))
+    assert worst <= 1e-12
+    prob.close()
