@@ -117,20 +117,24 @@ def test_aslda_building_blocks(torch, dim):
     ctx.close()
 
 
-@pytest.mark.parametrize("dim,nsteps", [(1, 286), (2, 60), (3, 6)])
-def test_aslda_pece_steps(torch, dim, nsteps):
+@pytest.mark.parametrize("dim,nsteps,fused", [(1, 286, False), (2, 60, False), (3, 6, False), (3, 6, True)])
+def test_aslda_pece_steps(torch, dim, nsteps, fused):
     """ASLDA PECE steps (gradients on both half-steps) from the uniform state
-    with V_ext switching on."""
-    say("case: ASLDA PECE steps dim=%d nsteps=%d" % (dim, nsteps))
+    with V_ext switching on; in 3D also through tdw_step_fused (the folded
+    step), finalised before the comparison."""
+    say("case: ASLDA PECE steps dim=%d nsteps=%d fused=%d" % (dim, nsteps, fused))
     s = tdgauss.setup(dim, functional=po.ASLDA)
     it0 = int(round(6.0 / s["eF"] / s["cfg"].dt))
     P = po.Problem(s["cfg"], s["wf"], s["fbeta"], kt=s["kt"], cnt=s["cnt"], pots=s["pots"])
     P.initial_measure()
     ctx, R = gpu_run(torch, s)
+    R.fused = fused
     R.initial_measure()
     for k in range(nsteps):
         P.step(it0 + k, grad_last=0)
         R.step(it0 + k, grad_last=False)
+    if fused:
+        R.finalize()
     ctx.sync()
     N = ctx.N
     d = R.dens.cpu().numpy()

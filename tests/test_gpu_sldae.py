@@ -60,19 +60,23 @@ def test_sldae_uniform_solution(torch):
     assert u.nwf == s["u"].nwf
 
 
-@pytest.mark.parametrize("dim,nsteps", [(1, 286), (2, 40), (3, 6)])
-def test_sldae_pece_steps(torch, dim, nsteps):
-    """PECE steps with V_ext switching on: densities, alpha, energies."""
-    say("case: SLDAE PECE steps dim=%d nsteps=%d" % (dim, nsteps))
+@pytest.mark.parametrize("dim,nsteps,fused", [(1, 286, False), (2, 40, False), (3, 6, False), (3, 6, True)])
+def test_sldae_pece_steps(torch, dim, nsteps, fused):
+    """PECE steps with V_ext switching on: densities, alpha, energies; in 3D
+    also through tdw_step_fused (the folded varying-mass step), finalised
+    before the comparison."""
+    say("case: SLDAE PECE steps dim=%d nsteps=%d fused=%d" % (dim, nsteps, fused))
     s = tdgauss.setup(dim, functional=po.SLDAE)
     it0 = int(round(6.0 / s["eF"] / s["cfg"].dt))
     P = po.Problem(s["cfg"], s["wf"], s["fbeta"], kt=s["kt"], cnt=s["cnt"], pots=s["pots"])
     P.initial_measure()
-    ctx, R = gpu_run(torch, s)
+    ctx, R = gpu_run(torch, s, fused=fused)
     R.initial_measure()
     for k in range(nsteps):
         P.step(it0 + k, grad_last=0)
         R.step(it0 + k, grad_last=False)
+    if fused:
+        R.finalize()
     ctx.sync()
     N = ctx.N
     d = R.dens.cpu().numpy()
