@@ -86,7 +86,7 @@ def design_units(name, scheme, split=False):
     }.get(name, 0)
 
 
-def modular_units(name):
+def modular_units(name, split=False):
     """The ASLDA step's kernels (3D, FP64): each kernel's own compulsory
     reads and writes of state-sized arrays, in S per state and launch (its
     algorithmic bytes at its boundaries).  The folded step of tdw_step_fused
@@ -100,6 +100,13 @@ def modular_units(name):
     35 S.  The modular kernels of other lattices: derivatives 3 + 5,
     densities 5, alpha_wf 2, lap(alpha psi) 2 + 3, apply_h 7 + 3,
     abm_step4 3."""
+    if split:
+        # the split form: FP32 history, prediction copy, H and T (0.5 S
+        # each), FP64 y and partial corrector: a_pred 1 + 2.5 read, 0.5 +
+        # 1 + 0.5 written; a_corr 1 + 0.5 + 0.5 read, 1 + 0.5 written; the
+        # density and H passes on FP32 arrays -- 16 S per state-step
+        return {"a_pred": 5.5, "a_corr": 3.5, "dens_gx": 0.5, "dens_gyz": 0.5, "ah_x": 1,
+                "ah_yz": 1.5}.get(name, 0)
     return {"a_pred": 9, "a_corr": 4, "abm_step1": 8, "abm_step4": 3, "abm_step4q": 4, "norms": 1, "scale": 2,
             "dens_gx": 1, "dens_gyz": 1, "ah_x": 2,
             "ah_yz": 3, "sub_qpe": 3, "fft_outer_deriv": 3, "fft_inner_deriv": 5, "densities": 5, "alpha_wf": 2,
@@ -205,7 +212,7 @@ def run_gpu(args, prec):
     davg = dtot / dcnt / 1000.0
     S = 2 * N * 16
     if args.aslda:
-        dbytes = modular_units(dname) * S * n
+        dbytes = modular_units(dname, prec == td.SPLIT) * S * n
         moved = dbytes
     else:
         dbytes = algorithmic_units(dname, p.scheme, prec == td.SPLIT) * S * n
@@ -633,6 +640,21 @@ def run_aslda(args):
         "step_state_traffic_S": sum(modular_units(k) * v[1] for k, v in g["rep"].items()) / args.steps,
         "step_compulsory_S": step_units(g["scheme"]),
     }
+    if args.mode == "both":
+        # the same workload in split precision, from the same start
+        gs = run_gpu(args, td.SPLIT)
+        sp = summarize(gs, args, "split")
+        err = density_errors(gs["R"].dens.cpu().numpy(), g["R"].dens.cpu().numpy(), g["N"], currents=False)
+        e64, es = sum(g["e1"][:7]), sum(gs["e1"][:7])
+        res["split"] = {"value": sp["value"], "ms_per_step": sp["ms_per_step"],
+                        "speedup_vs_fp64": sp["value"] / res["value"], "dtype": "f32-operator/f64-state",
+                        "density_rel_err_vs_fp64": err, "max_density_rel_err": max(err.values()),
+                        "energy_rel_diff_vs_fp64": abs(es - e64) / abs(e64), "energy_drift": sp["energy_drift"],
+                        "roofline": sp["roofline"], "kernels_ms_per_step": sp["kernels_ms_per_step"],
+                        "step_state_traffic_S": sum(modular_units(k, True) * v[1]
+                                                    for k, v in gs["rep"].items()) / args.steps}
+        gs["ctx"].close()
+        del gs
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(g, args)
     g["ctx"].close()

@@ -39,8 +39,11 @@ def test_algorithmic_units_add_up_to_the_compulsory_step():
 def test_aslda_units_add_up():
     """The ASLDA step at AB5AM5: the folded step (one predictor and one
     corrector pass, two density and two H pass pairs) moves 27 S per
-    state-step, the modular one 35 S."""
+    state-step, its split form 16 S, the modular one 35 S."""
     u = bench.modular_units
     pair = 2 * (u("dens_gx") + u("dens_gyz") + u("ah_x") + u("ah_yz"))
     assert u("a_pred") + u("a_corr") + pair == 27
     assert u("abm_step1") + u("abm_step4q") + 2 * (u("norms") + u("scale")) + u("sub_qpe") + pair == 35
+    # split precision: FP32 history, copies, H and T
+    us = lambda k: bench.modular_units(k, True)
+    assert us("a_pred") + us("a_corr") + 2 * (us("dens_gx") + us("dens_gyz") + us("ah_x") + us("ah_yz")) == 16

@@ -21,9 +21,15 @@
 #include "tdw_physics.hpp"
 #include "tdw_regfft.hpp"
 
+#include <type_traits>
+
 namespace tdw {
 
 static constexpr int AT = 256;
+
+// storage of the passes' inputs and outputs: FP64, or FP32 in the
+// split-precision step (the transforms in FP32, the sums in FP64)
+template <bool SP> using ac_t = typename std::conditional<SP, float2, double2>::type;
 
 // ---------------------------------------------------------------------------
 // In-place derivatives along one axis of P pencils held in LDS (L = R1*R2,
@@ -35,19 +41,20 @@ static constexpr int AT = 256;
 // receives the first derivative (factor i k1s m, Nyquist mode zero).  Pencil q starts at
 // (q / QD) * QH + (q % QD) * QL, element stride IS.
 // ---------------------------------------------------------------------------
-template <int L, int P, int QD, int QH, int QL, int IS, int NT, bool GRAD>
-__device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *tw, int tid, double k1s, double k2s)
+template <int L, int P, int QD, int QH, int QL, int IS, int NT, bool GRAD, typename C = double2>
+__device__ __forceinline__ void axis_d12(C *A, C *B, const C *tw, int tid, double k1s, double k2s)
 {
+    typedef decltype(C().x) R;
     constexpr int R1 = rf::fac2<L>::R1, R2 = rf::fac2<L>::R2;
     static_assert(R2 > 1, "axis_d12 needs L = R1 * R2 with R2 > 1");
     // stage 1: R1-point DFTs over n1, twiddle W_L^{n2 k1}
     for (int b = tid; b < P * R2; b += NT) {
         const int n2 = b / P, q = b - n2 * P;
-        double2 *pa = A + (q / QD) * QH + (q % QD) * QL;
-        double2 v[R1];
+        C *pa = A + (q / QD) * QH + (q % QD) * QL;
+        C v[R1];
 #pragma unroll
         for (int n1 = 0; n1 < R1; n1++) v[n1] = pa[(n1 * R2 + n2) * IS];
-        rf::small_dft<R1, -1, double2>::run(v);
+        rf::small_dft<R1, -1, C>::run(v);
 #pragma unroll
         for (int k1 = 1; k1 < R1; k1++) v[k1] = cmul(v[k1], tw[(n2 * k1) % L]);
 #pragma unroll
@@ -58,23 +65,23 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
     for (int b = tid; b < P * R1; b += NT) {
         const int k1 = b / P, q = b - k1 * P;
         const int off = (q / QD) * QH + (q % QD) * QL;
-        double2 *pa = A + off;
-        double2 v[R2], g[R2];
+        C *pa = A + off;
+        C v[R2], g[R2];
 #pragma unroll
         for (int n2 = 0; n2 < R2; n2++) v[n2] = pa[(k1 * R2 + n2) * IS];
-        rf::small_dft<R2, -1, double2>::run(v);
+        rf::small_dft<R2, -1, C>::run(v);
 #pragma unroll
         for (int k2 = 0; k2 < R2; k2++) {
             const int f = k1 + R1 * k2;
             const int m = (f < L / 2) ? f : f - L;
-            if constexpr (GRAD) g[k2] = (f == L / 2) ? make_double2(0.0, 0.0) : imul(v[k2], k1s * (double)m);
-            v[k2] = v[k2] * (-k2s * (double)(m * m));
+            if constexpr (GRAD) g[k2] = (f == L / 2) ? cvt<C>(make_double2(0.0, 0.0)) : imul(v[k2], (R)(k1s * (double)m));
+            v[k2] = v[k2] * (R)(-k2s * (double)(m * m));
         }
-        rf::small_dft<R2, +1, double2>::run(v);
-        if constexpr (GRAD) rf::small_dft<R2, +1, double2>::run(g);
+        rf::small_dft<R2, +1, C>::run(v);
+        if constexpr (GRAD) rf::small_dft<R2, +1, C>::run(g);
 #pragma unroll
         for (int n2 = 0; n2 < R2; n2++) {
-            double2 wc = tw[(n2 * k1) % L];
+            C wc = tw[(n2 * k1) % L];
             wc.y = -wc.y;
             pa[(k1 * R2 + n2) * IS] = n2 == 0 ? v[n2] : cmul(v[n2], wc);
             if constexpr (GRAD) B[off + (k1 * R2 + n2) * IS] = n2 == 0 ? g[n2] : cmul(g[n2], wc);
@@ -85,18 +92,18 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
     for (int b = tid; b < P * R2; b += NT) {
         const int n2 = b / P, q = b - n2 * P;
         const int off = (q / QD) * QH + (q % QD) * QL;
-        double2 v[R1];
-        double2 *pa = A + off;
+        C v[R1];
+        C *pa = A + off;
 #pragma unroll
         for (int k1 = 0; k1 < R1; k1++) v[k1] = pa[(k1 * R2 + n2) * IS];
-        rf::small_dft<R1, +1, double2>::run(v);
+        rf::small_dft<R1, +1, C>::run(v);
 #pragma unroll
         for (int n1 = 0; n1 < R1; n1++) pa[(n1 * R2 + n2) * IS] = v[n1];
         if constexpr (GRAD) {
-            double2 *pb = B + off;
+            C *pb = B + off;
 #pragma unroll
             for (int k1 = 0; k1 < R1; k1++) v[k1] = pb[(k1 * R2 + n2) * IS];
-            rf::small_dft<R1, +1, double2>::run(v);
+            rf::small_dft<R1, +1, C>::run(v);
 #pragma unroll
             for (int n1 = 0; n1 < R1; n1++) pb[(n1 * R2 + n2) * IS] = v[n1];
         }
@@ -110,15 +117,17 @@ __device__ __forceinline__ void axis_d12(double2 *A, double2 *B, const double2 *
 // q, point i), so the loads and stores of consecutive lanes are consecutive
 // in memory.  k1s = 2 pi / (Lx * LX), k2s = (2 pi / Lx)^2 / LX.
 // ---------------------------------------------------------------------------
-template <int LX, int PX>
+template <int LX, int PX, bool SP>
 __global__ void __launch_bounds__(AT)
-ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__restrict__ pots, int n, long N, int S,
-     const double2 *__restrict__ twx_g, double k1s, double k2s, const double *__restrict__ sc)
+ah_x(const ac_t<SP> *__restrict__ y, ac_t<SP> *__restrict__ T, const double *__restrict__ pots, int n, long N, int S,
+     const ac_t<SP> *__restrict__ twx_g, double k1s, double k2s, const double *__restrict__ sc)
 {
+    typedef ac_t<SP> C;
+    typedef decltype(C().x) R;
     constexpr int E = LX * PX;
     constexpr int KM = (E + AT - 1) / AT;
-    __shared__ double2 twx[LX];
-    __shared__ double2 A[E], B[E];
+    __shared__ C twx[LX];
+    __shared__ C A[E], B[E];
     const int tid = threadIdx.x;
     for (int k = tid; k < LX; k += AT) twx[k] = twx_g[k];
     const int nb = S / PX;
@@ -131,7 +140,7 @@ ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__res
     const double *Ax = pots + (vcomp ? 9 : 6) * N;
     const double sg = vcomp ? 0.25 : -0.25;
     const double ys = sc ? sc[vcomp ? a2 - n : a2] : 1.0;
-    double2 X[KM], acc[KM];
+    C X[KM], acc[KM];
     double al[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
@@ -139,35 +148,35 @@ ah_x(const double2 *__restrict__ y, double2 *__restrict__ T, const double *__res
         if (e < E) {
             const int i = e / PX, q = e - i * PX;
             const long off = (long)i * S + q;
-            X[k] = y[base + off] * ys;
+            X[k] = y[base + off] * (R)ys;
             al[k] = alpha[fb + off];
             A[e] = X[k];
         }
     }
     __syncthreads();
-    axis_d12<LX, PX, PX, 0, 1, PX, AT, true>(A, B, twx, tid, k1s, k2s);
+    axis_d12<LX, PX, PX, 0, 1, PX, AT, true, C>(A, B, twx, tid, k1s, k2s);
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * AT;
         if (e < E) {
             const int i = e / PX, q = e - i * PX;
-            acc[k] = A[e] * (sg * al[k]) + imul(B[e], -Ax[fb + (long)i * S + q]);
+            acc[k] = A[e] * (R)(sg * al[k]) + imul(B[e], (R)(-Ax[fb + (long)i * S + q]));
         }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * AT;
-        if (e < E) A[e] = X[k] * al[k];
+        if (e < E) A[e] = X[k] * (R)al[k];
     }
     __syncthreads();
-    axis_d12<LX, PX, PX, 0, 1, PX, AT, false>(A, nullptr, twx, tid, k1s, k2s);
+    axis_d12<LX, PX, PX, 0, 1, PX, AT, false, C>(A, nullptr, twx, tid, k1s, k2s);
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * AT;
         if (e < E) {
             const int i = e / PX, q = e - i * PX;
-            T[base + (long)i * S + q] = acc[k] + A[e] * sg;
+            T[base + (long)i * S + q] = acc[k] + A[e] * (R)sg;
         }
     }
 }
@@ -183,21 +192,23 @@ constexpr int nt_ayz(int E)
 // part[block] = Re <psi|H psi> of the
 // plane.  hx = [lap alpha_a, lap alpha_b, div A_a, div A_b].
 // ---------------------------------------------------------------------------
-template <int LY, int LZ>
+template <int LY, int LZ, bool SP>
 __global__ void __launch_bounds__(nt_ayz(LY *LZ), (LY * LZ <= 576) ? 3 : 2)
-ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__restrict__ out,
+ah_yz(const ac_t<SP> *__restrict__ y, const ac_t<SP> *__restrict__ T, ac_t<SP> *__restrict__ out,
       const double *__restrict__ pots, const double *__restrict__ hx, int n, long N, int NX,
-      const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y, double k1z,
+      const ac_t<SP> *__restrict__ twy_g, const ac_t<SP> *__restrict__ twz_g, double k1y, double k2y, double k1z,
       double k2z, double mu_a, double mu_b, double *__restrict__ part, const double *__restrict__ sc)
 {
+    typedef ac_t<SP> C;
+    typedef decltype(C().x) R;
     constexpr int E = LY * LZ;
     constexpr int EP = LY * (LZ + 1);   // padded plane in LDS: row stride LZ + 1
     constexpr int NT = nt_ayz(E);
     constexpr int KM = (E + NT - 1) / NT;
-    __shared__ double2 twy[LY], twz[LZ];
+    __shared__ C twy[LY], twz[LZ];
     __shared__ double red[NT / 64];
     extern __shared__ double2 lds_raw[];
-    double2 *A = lds_raw, *B = lds_raw + 2 * EP;
+    C *A = (C *)lds_raw, *B = (C *)lds_raw + 2 * EP;
     const int tid = threadIdx.x;
     for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
     for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
@@ -206,14 +217,14 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
     const long gu = (long)a * N + (long)x * E, gv = ((long)n + a) * N + (long)x * E;
     const long g0 = (long)x * E;
     const double ys = sc ? sc[a] : 1.0;
-    double2 Xu[KM], Xv[KM], Ou[KM], Ov[KM];
+    C Xu[KM], Xv[KM], Ou[KM], Ov[KM];
     double al[KM], bl[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) {
         const int e = tid + k * NT;
         if (e < E) {
-            Xu[k] = y[gu + e] * ys;
-            Xv[k] = y[gv + e] * ys;
+            Xu[k] = y[gu + e] * (R)ys;
+            Xv[k] = y[gv + e] * (R)ys;
             Ou[k] = T[gu + e];
             Ov[k] = T[gv + e];
             al[k] = pots[4 * N + g0 + e];
@@ -237,16 +248,16 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
             }
         }
         __syncthreads();
-        if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, true>(A, B, twz, tid, k1z, k2z);
-        else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, true>(A, B, twy, tid, k1y, k2y);
+        if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, true, C>(A, B, twz, tid, k1z, k2z);
+        else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, true, C>(A, B, twy, tid, k1y, k2y);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
                 const int pe = (e / LZ) * (LZ + 1) + e % LZ;
                 const double Aa = pots[(6 + d) * N + g0 + e], Ab = pots[(9 + d) * N + g0 + e];
-                Ou[k] = Ou[k] + A[pe] * (-0.25 * al[k]) + imul(B[pe], -Aa);
-                Ov[k] = Ov[k] + A[EP + pe] * (0.25 * bl[k]) + imul(B[EP + pe], -Ab);
+                Ou[k] = Ou[k] + A[pe] * (R)(-0.25 * al[k]) + imul(B[pe], (R)(-Aa));
+                Ov[k] = Ov[k] + A[EP + pe] * (R)(0.25 * bl[k]) + imul(B[EP + pe], (R)(-Ab));
             }
         }
         // alpha psi: d^2 into A
@@ -256,23 +267,24 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
             const int e = tid + k * NT;
             if (e < E) {
                 const int pe = (e / LZ) * (LZ + 1) + e % LZ;
-                A[pe] = Xu[k] * al[k];
-                A[EP + pe] = Xv[k] * bl[k];
+                A[pe] = Xu[k] * (R)al[k];
+                A[EP + pe] = Xv[k] * (R)bl[k];
             }
         }
         __syncthreads();
-        if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, false>(A, nullptr, twz, tid, k1z, k2z);
-        else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, false>(A, nullptr, twy, tid, k1y, k2y);
+        if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, false, C>(A, nullptr, twz, tid, k1z, k2z);
+        else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, false, C>(A, nullptr, twy, tid, k1y, k2y);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * NT;
             if (e < E) {
                 const int pe = (e / LZ) * (LZ + 1) + e % LZ;
-                Ou[k] = Ou[k] + A[pe] * -0.25;
-                Ov[k] = Ov[k] + A[EP + pe] * 0.25;
+                Ou[k] = Ou[k] + A[pe] * (R)-0.25;
+                Ov[k] = Ov[k] + A[EP + pe] * (R)0.25;
             }
         }
     }
+    // the pointwise terms, the partner component and <psi|H psi> in FP64
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; k++) {
@@ -282,17 +294,17 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
             const double la = 0.25 * hx[gi], lb = 0.25 * hx[N + gi];
             const double ca = -0.5 * hx[2 * N + gi], cb = -0.5 * hx[3 * N + gi];
             const double2 D = make_double2(pots[2 * N + 2 * gi], pots[2 * N + 2 * gi + 1]);
-            const double2 tu = Xu[k], tv = Xv[k];
+            const double2 tu = to64(Xu[k]), tv = to64(Xv[k]);
             double2 u = cmul(tu, make_double2(pots[gi] - mu_a, ca));
             double2 v = make_double2(0.0, 0.0) - cmul(tv, make_double2(pots[N + gi] - mu_b, -1.0 * cb));
             u = u + cmul(tv, D);
             v = v + cmulc(D, tu);
             u = u + tu * la;
             v = v - tv * lb;
-            u = u + Ou[k];
-            v = v + Ov[k];
-            out[gu + e] = u;
-            out[gv + e] = v;
+            u = u + to64(Ou[k]);
+            v = v + to64(Ov[k]);
+            out[gu + e] = cvt<C>(u);
+            out[gv + e] = cvt<C>(v);
             ssum += cmulc(tu, u).x + cmulc(tv, v).x;
         }
     }
@@ -311,15 +323,16 @@ ah_yz(const double2 *__restrict__ y, const double2 *__restrict__ T, double2 *__r
 // x pass: both components of PX consecutive x pencils in LDS (u at A[0..E),
 // v at A[E..2E), pencil q, point i at q + i*PX).  Fields per chunk:
 // rho_a, rho_b, Re nu, Im nu, tau_a (x), tau_b (x), j_a,x, j_b,x.
-template <int LX, int PX>
+template <int LX, int PX, bool SP>
 __global__ void __launch_bounds__(AT, (LX * PX <= 512) ? 3 : 2)
-dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
-     int S, int G, const double2 *__restrict__ twx_g, double k1s, double k2s, double *__restrict__ part)
+dg_x(const ac_t<SP> *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
+     int S, int G, const ac_t<SP> *__restrict__ twx_g, double k1s, double k2s, double *__restrict__ part)
 {
+    typedef ac_t<SP> C;
     constexpr int E = LX * PX;
     constexpr int KM = (E + AT - 1) / AT;
-    __shared__ double2 twx[LX];
-    __shared__ double2 A[2 * E], B[2 * E];
+    __shared__ C twx[LX];
+    __shared__ C A[2 * E], B[2 * E];
     const int tid = threadIdx.x;
     for (int k = tid; k < LX; k += AT) twx[k] = twx_g[k];
     const int nb = S / PX;
@@ -330,8 +343,8 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
     double na[KM], nbv[KM], nux[KM], nuy[KM], ta[KM], tb[KM], ja[KM], jbx[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) na[k] = nbv[k] = nux[k] = nuy[k] = ta[k] = tb[k] = ja[k] = jbx[k] = 0.0;
-    double2 Xu[KM], Xv[KM];
-    auto load = [&](int a, double2 *U, double2 *V) {
+    C Xu[KM], Xv[KM];
+    auto load = [&](int a, C *U, C *V) {
         const long bu = (long)a * N + fb0, bv = ((long)n + a) * N + fb0;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
@@ -358,21 +371,21 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
             }
         }
         __syncthreads();
-        axis_d12<LX, 2 * PX, PX, E, 1, PX, AT, true>(A, B, twx, tid, k1s, k2s);
+        axis_d12<LX, 2 * PX, PX, E, 1, PX, AT, true, C>(A, B, twx, tid, k1s, k2s);
 #pragma unroll
         for (int k = 0; k < KM; k++) {
             const int e = tid + k * AT;
             if (e < E) {
-                const double2 u = Xu[k], v = Xv[k];
+                const double2 u = to64(Xu[k]), v = to64(Xv[k]);
                 na[k] += cnorm(u) * fb;
                 nbv[k] += cnorm(v) * fbm;
                 const double2 uv = cmul(u, make_double2(v.x, -v.y)) * ((fbm - fb) / 2.0);
                 nux[k] += uv.x;
                 nuy[k] += uv.y;
-                ta[k] += cmulc(u, A[e]).x * fb;
-                tb[k] += cmulc(v, A[E + e]).x * fbm;
-                ja[k] += cmulc(u, B[e]).y * fb;
-                jbx[k] -= cmulc(v, B[E + e]).y * fbm;
+                ta[k] += cmulc(u, to64(A[e])).x * fb;
+                tb[k] += cmulc(v, to64(A[E + e])).x * fbm;
+                ja[k] += cmulc(u, to64(B[e])).y * fb;
+                jbx[k] -= cmulc(v, to64(B[E + e])).y * fbm;
             }
         }
     }
@@ -398,19 +411,20 @@ dg_x(const double2 *__restrict__ y, const double *__restrict__ fbeta, const doub
 // yz pass: one workgroup per (chunk, x plane), both components of the plane
 // in LDS.  Fields per chunk: tau_a (y+z), tau_b (y+z), j_a,y, j_a,z, j_b,y,
 // j_b,z.
-template <int LY, int LZ>
+template <int LY, int LZ, bool SP>
 __global__ void __launch_bounds__(nt_ayz(LY *LZ), (LY * LZ <= 576) ? 3 : 2)
-dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
-      int NX, int G, const double2 *__restrict__ twy_g, const double2 *__restrict__ twz_g, double k1y, double k2y,
+dg_yz(const ac_t<SP> *__restrict__ y, const double *__restrict__ fbeta, const double *__restrict__ wgt, int n, long N,
+      int NX, int G, const ac_t<SP> *__restrict__ twy_g, const ac_t<SP> *__restrict__ twz_g, double k1y, double k2y,
       double k1z, double k2z, double *__restrict__ part)
 {
+    typedef ac_t<SP> C;
     constexpr int E = LY * LZ;
     constexpr int EP = LY * (LZ + 1);   // padded plane in LDS: row stride LZ + 1
     constexpr int NT = nt_ayz(E);
     constexpr int KM = (E + NT - 1) / NT;
-    __shared__ double2 twy[LY], twz[LZ];
+    __shared__ C twy[LY], twz[LZ];
     extern __shared__ double2 lds_raw[];
-    double2 *A = lds_raw, *B = lds_raw + 2 * EP;
+    C *A = (C *)lds_raw, *B = (C *)lds_raw + 2 * EP;
     const int tid = threadIdx.x;
     for (int k = tid; k < LY; k += NT) twy[k] = twy_g[k];
     for (int k = tid; k < LZ; k += NT) twz[k] = twz_g[k];
@@ -421,8 +435,8 @@ dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const dou
     double ta[KM], tb[KM], jay[KM], jaz[KM], jby[KM], jbz[KM];
 #pragma unroll
     for (int k = 0; k < KM; k++) ta[k] = tb[k] = jay[k] = jaz[k] = jby[k] = jbz[k] = 0.0;
-    double2 Xu[KM], Xv[KM];
-    auto load = [&](int a, double2 *U, double2 *V) {
+    C Xu[KM], Xv[KM];
+    auto load = [&](int a, C *U, C *V) {
         const long gu = (long)a * N + g0, gv = ((long)n + a) * N + g0;
 #pragma unroll
         for (int k = 0; k < KM; k++) {
@@ -451,16 +465,17 @@ dg_yz(const double2 *__restrict__ y, const double *__restrict__ fbeta, const dou
                 }
             }
             __syncthreads();
-            if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, true>(A, B, twz, tid, k1z, k2z);
-            else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, true>(A, B, twy, tid, k1y, k2y);
+            if (ax == 0) axis_d12<LZ, 2 * LY, LY, EP, LZ + 1, 1, NT, true, C>(A, B, twz, tid, k1z, k2z);
+            else axis_d12<LY, 2 * LZ, LZ, EP, 1, LZ + 1, NT, true, C>(A, B, twy, tid, k1y, k2y);
 #pragma unroll
             for (int k = 0; k < KM; k++) {
                 const int e = tid + k * NT;
                 if (e < E) {
                     const int pe = (e / LZ) * (LZ + 1) + e % LZ;
-                    ta[k] += cmulc(Xu[k], A[pe]).x * fb;
-                    tb[k] += cmulc(Xv[k], A[EP + pe]).x * fbm;
-                    const double gu_ = cmulc(Xu[k], B[pe]).y * fb, gv_ = cmulc(Xv[k], B[EP + pe]).y * fbm;
+                    const double2 xu = to64(Xu[k]), xv = to64(Xv[k]);
+                    ta[k] += cmulc(xu, to64(A[pe])).x * fb;
+                    tb[k] += cmulc(xv, to64(A[EP + pe])).x * fbm;
+                    const double gu_ = cmulc(xu, to64(B[pe])).y * fb, gv_ = cmulc(xv, to64(B[EP + pe])).y * fbm;
                     if (ax == 0) { jaz[k] += gu_; jbz[k] -= gv_; }
                     else { jay[k] += gu_; jby[k] -= gv_; }
                 }
@@ -519,103 +534,110 @@ __global__ void ah_qpe(const double *__restrict__ part, int n, int NX, double dv
     qpe[a] = dv * s;
 }
 
-typedef void (*ahx_fn)(const double2 *, double2 *, const double *, int, long, int, const double2 *, double, double,
-                       const double *);
-typedef void (*ahyz_fn)(const double2 *, const double2 *, double2 *, const double *, const double *, int, long, int,
-                        const double2 *, const double2 *, double, double, double, double, double, double, double *,
-                        const double *);
+template <bool SP> struct AFns {
+    typedef ac_t<SP> C;
+    typedef void (*ahx_fn)(const C *, C *, const double *, int, long, int, const C *, double, double, const double *);
+    typedef void (*ahyz_fn)(const C *, const C *, C *, const double *, const double *, int, long, int, const C *,
+                            const C *, double, double, double, double, double, double, double *, const double *);
+    typedef void (*dgx_fn)(const C *, const double *, const double *, int, long, int, int, const C *, double, double,
+                           double *);
+    typedef void (*dgyz_fn)(const C *, const double *, const double *, int, long, int, int, const C *, const C *,
+                            double, double, double, double, double *);
 
-// x pass for nx points and PX pencils per workgroup (S must be a multiple
-// of PX)
-static bool ahx_for(int nx, int S, ahx_fn *f, int *px)
-{
-    if (S % 32 == 0) {
-        *px = 32;
-        switch (nx) {
-        case 12: *f = ah_x<12, 32>; return true;
-        case 16: *f = ah_x<16, 32>; return true;
-        case 24: *f = ah_x<24, 32>; return true;
-        case 32: *f = ah_x<32, 32>; return true;
-        default: return false;
+    // x pass for nx points and PX pencils per workgroup (S a multiple of PX)
+    static bool ahx(int nx, int S, ahx_fn *f, int *px)
+    {
+        if (S % 32 == 0) {
+            *px = 32;
+            switch (nx) {
+            case 12: *f = ah_x<12, 32, SP>; return true;
+            case 16: *f = ah_x<16, 32, SP>; return true;
+            case 24: *f = ah_x<24, 32, SP>; return true;
+            case 32: *f = ah_x<32, 32, SP>; return true;
+            default: return false;
+            }
         }
-    }
-    if (S % 16 == 0) {
-        *px = 16;
-        switch (nx) {
-        case 12: *f = ah_x<12, 16>; return true;
-        case 16: *f = ah_x<16, 16>; return true;
-        case 24: *f = ah_x<24, 16>; return true;
-        case 32: *f = ah_x<32, 16>; return true;
-        default: return false;
+        if (S % 16 == 0) {
+            *px = 16;
+            switch (nx) {
+            case 12: *f = ah_x<12, 16, SP>; return true;
+            case 16: *f = ah_x<16, 16, SP>; return true;
+            case 24: *f = ah_x<24, 16, SP>; return true;
+            case 32: *f = ah_x<32, 16, SP>; return true;
+            default: return false;
+            }
         }
+        return false;
     }
-    return false;
-}
 
-static bool ahyz_for(int ny, int nz, ahyz_fn *f, int *nt)
-{
-    if (ny == 12 && nz == 12) { *f = ah_yz<12, 12>; *nt = nt_ayz(144); return true; }
-    if (ny == 16 && nz == 16) { *f = ah_yz<16, 16>; *nt = nt_ayz(256); return true; }
-    if (ny == 24 && nz == 24) { *f = ah_yz<24, 24>; *nt = nt_ayz(576); return true; }
-    if (ny == 32 && nz == 32) { *f = ah_yz<32, 32>; *nt = nt_ayz(1024); return true; }
-    return false;
-}
+    static bool ahyz(int ny, int nz, ahyz_fn *f, int *nt)
+    {
+        if (ny == 12 && nz == 12) { *f = ah_yz<12, 12, SP>; *nt = nt_ayz(144); return true; }
+        if (ny == 16 && nz == 16) { *f = ah_yz<16, 16, SP>; *nt = nt_ayz(256); return true; }
+        if (ny == 24 && nz == 24) { *f = ah_yz<24, 24, SP>; *nt = nt_ayz(576); return true; }
+        if (ny == 32 && nz == 32) { *f = ah_yz<32, 32, SP>; *nt = nt_ayz(1024); return true; }
+        return false;
+    }
 
-typedef void (*dgx_fn)(const double2 *, const double *, const double *, int, long, int, int, const double2 *, double,
-                       double, double *);
-typedef void (*dgyz_fn)(const double2 *, const double *, const double *, int, long, int, int, const double2 *,
-                        const double2 *, double, double, double, double, double *);
-
-static bool dgx_for(int nx, int S, dgx_fn *f, int *px)
-{
-    if (S % 32 == 0) {
-        *px = 32;
-        switch (nx) {
-        case 12: *f = dg_x<12, 32>; return true;
-        case 16: *f = dg_x<16, 32>; return true;
-        case 24: *f = dg_x<24, 16>; *px = 16; return true;   // fewer registers, three waves per SIMD
-        case 32: *f = dg_x<32, 16>; *px = 16; return true;   // 2 x 2 x 32 x 32 points would exceed 64 KB
-        default: return false;
+    static bool dgx(int nx, int S, dgx_fn *f, int *px)
+    {
+        if (S % 32 == 0) {
+            *px = 32;
+            switch (nx) {
+            case 12: *f = dg_x<12, 32, SP>; return true;
+            case 16: *f = dg_x<16, 32, SP>; return true;
+            case 24: *f = dg_x<24, 16, SP>; *px = 16; return true;   // fewer registers, three waves per SIMD
+            case 32: *f = dg_x<32, 16, SP>; *px = 16; return true;   // 2 x 2 x 32 x 32 points would exceed 64 KB
+            default: return false;
+            }
         }
-    }
-    if (S % 16 == 0) {
-        *px = 16;
-        switch (nx) {
-        case 12: *f = dg_x<12, 16>; return true;
-        case 16: *f = dg_x<16, 16>; return true;
-        case 24: *f = dg_x<24, 16>; return true;
-        case 32: *f = dg_x<32, 16>; return true;
-        default: return false;
+        if (S % 16 == 0) {
+            *px = 16;
+            switch (nx) {
+            case 12: *f = dg_x<12, 16, SP>; return true;
+            case 16: *f = dg_x<16, 16, SP>; return true;
+            case 24: *f = dg_x<24, 16, SP>; return true;
+            case 32: *f = dg_x<32, 16, SP>; return true;
+            default: return false;
+            }
         }
+        return false;
     }
-    return false;
-}
 
-static bool dgyz_for(int ny, int nz, dgyz_fn *f, int *nt)
-{
-    if (ny == 12 && nz == 12) { *f = dg_yz<12, 12>; *nt = nt_ayz(144); return true; }
-    if (ny == 16 && nz == 16) { *f = dg_yz<16, 16>; *nt = nt_ayz(256); return true; }
-    if (ny == 24 && nz == 24) { *f = dg_yz<24, 24>; *nt = nt_ayz(576); return true; }
-    if (ny == 32 && nz == 32) { *f = dg_yz<32, 32>; *nt = nt_ayz(1024); return true; }
-    return false;
-}
+    static bool dgyz(int ny, int nz, dgyz_fn *f, int *nt)
+    {
+        if (ny == 12 && nz == 12) { *f = dg_yz<12, 12, SP>; *nt = nt_ayz(144); return true; }
+        if (ny == 16 && nz == 16) { *f = dg_yz<16, 16, SP>; *nt = nt_ayz(256); return true; }
+        if (ny == 24 && nz == 24) { *f = dg_yz<24, 24, SP>; *nt = nt_ayz(576); return true; }
+        if (ny == 32 && nz == 32) { *f = dg_yz<32, 32, SP>; *nt = nt_ayz(1024); return true; }
+        return false;
+    }
+
+    static const C *tw(const tdw_ctx *c, int ax)
+    {
+        if constexpr (SP) return c->d_tw32[ax];
+        else return c->d_tw[ax];
+    }
+};
 
 bool dens_grad_fused_supported(const tdw_ctx *c)
 {
     if (c->dim != 3) return false;
-    dgx_fn fx;
-    dgyz_fn fy;
+    AFns<false>::dgx_fn fx;
+    AFns<false>::dgyz_fn fy;
     int nt, px;
-    if (!dgx_for(c->nx, c->S, &fx, &px) || !dgyz_for(c->ny, c->nz, &fy, &nt)) return false;
+    if (!AFns<false>::dgx(c->nx, c->S, &fx, &px) || !AFns<false>::dgyz(c->ny, c->nz, &fy, &nt)) return false;
     return sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1) <= (size_t)c->max_lds;
 }
 
-int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, const double *wgt, double *dens)
+template <bool SP>
+static int dens_grad_t(tdw_ctx *c, int n, const ac_t<SP> *wf, const double *fbeta, const double *wgt, double *dens)
 {
-    dgx_fn fx;
-    dgyz_fn fy;
+    typedef AFns<SP> F;
+    typename F::dgx_fn fx;
+    typename F::dgyz_fn fy;
     int nt, px;
-    if (!dgx_for(c->nx, c->S, &fx, &px) || !dgyz_for(c->ny, c->nz, &fy, &nt)) return TDW_ESIZE;
+    if (!F::dgx(c->nx, c->S, &fx, &px) || !F::dgyz(c->ny, c->nz, &fy, &nt)) return TDW_ESIZE;
     const long N = (long)c->N;
     const int S = c->S;
     // states per chunk: keep >= ~4096 workgroups in the x pass
@@ -628,59 +650,72 @@ int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, c
     {
         TDW_TIMED(c, "dens_gx");
         hipLaunchKernelGGL(fx, dim3((unsigned)((long)nchunk * (S / px))), dim3(AT), 0, c->stream, wf, fbeta, wgt, n, N,
-                           S, G, c->d_tw[0], k1, k2, px_part);
+                           S, G, F::tw(c, 0), k1, k2, px_part);
     }
     TDW_TIMED(c, "dens_gyz");
-    const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1);
+    const size_t lds = sizeof(typename F::C) * 4 * (size_t)c->ny * (c->nz + 1);
     const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
     const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
     hipLaunchKernelGGL(fy, dim3((unsigned)((long)nchunk * c->nx)), dim3(nt), lds, c->stream, wf, fbeta, wgt, n, N,
-                       c->nx, G, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, pyz_part);
+                       c->nx, G, F::tw(c, 1), F::tw(c, 2), k1y, k2y, k1z, k2z, pyz_part);
     hipLaunchKernelGGL(dg_final, dim3((unsigned)((12 * N + AT - 1) / AT)), dim3(AT), 0, c->stream, px_part, pyz_part,
                        nchunk, N, dens_transverse(c), dens);
     TDW_TRY((int)hipGetLastError());
     return spin_fold(c, dens);
 }
 
+int dens_grad_fused(tdw_ctx *c, int n, const double2 *wf, const double *fbeta, const double *wgt, double *dens)
+{
+    return dens_grad_t<false>(c, n, wf, fbeta, wgt, dens);
+}
+
 bool aslda_h_fused_supported(const tdw_ctx *c)
 {
     if (c->dim != 3) return false;
-    ahx_fn fx;
-    ahyz_fn fy;
+    AFns<false>::ahx_fn fx;
+    AFns<false>::ahyz_fn fy;
     int nt, px;
-    if (!ahx_for(c->nx, c->S, &fx, &px) || !ahyz_for(c->ny, c->nz, &fy, &nt)) return false;
+    if (!AFns<false>::ahx(c->nx, c->S, &fx, &px) || !AFns<false>::ahyz(c->ny, c->nz, &fy, &nt)) return false;
     return sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1) <= (size_t)c->max_lds;
 }
 
-// out = H in with <H> into qpe (no subtraction); T: 2*n*N complex scratch;
-// hx: the fields of aslda_fields; part: n*NX doubles; sc (n per-state
-// factors, or null): H acts on sc_a in_a
-int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots,
-                  const double *hx, double *part, double *d_qpe, const double *sc)
+// out = H in with <H> into qpe (no subtraction); T: 2*n*N scratch of the
+// passes' precision; hx: the fields of aslda_fields; part: n*NX doubles; sc
+// (n per-state factors, or null): H acts on sc_a in_a
+template <bool SP>
+static int aslda_h_t(tdw_ctx *c, int n, const ac_t<SP> *in, ac_t<SP> *out, ac_t<SP> *T, const double *pots,
+                     const double *hx, double *part, double *d_qpe, const double *sc)
 {
-    ahx_fn fx;
-    ahyz_fn fy;
+    typedef AFns<SP> F;
+    typename F::ahx_fn fx;
+    typename F::ahyz_fn fy;
     int nt, px;
-    if (!ahx_for(c->nx, c->S, &fx, &px) || !ahyz_for(c->ny, c->nz, &fy, &nt)) return TDW_ESIZE;
+    if (!F::ahx(c->nx, c->S, &fx, &px) || !F::ahyz(c->ny, c->nz, &fy, &nt)) return TDW_ESIZE;
     const long N = (long)c->N;
     const int S = c->S;
     {
         TDW_TIMED(c, "ah_x");
         const long nblk = 2L * n * (S / px);
         const double k1 = 2.0 * M_PI / c->Lphys[0] / c->nx, k2 = pow(2.0 * M_PI / c->Lphys[0], 2) / c->nx;
-        hipLaunchKernelGGL(fx, dim3((unsigned)nblk), dim3(AT), 0, c->stream, in, T, pots, n, N, S, c->d_tw[0], k1, k2,
+        hipLaunchKernelGGL(fx, dim3((unsigned)nblk), dim3(AT), 0, c->stream, in, T, pots, n, N, S, F::tw(c, 0), k1, k2,
                            sc);
     }
     {
         TDW_TIMED(c, "ah_yz");
-        const size_t lds = sizeof(double2) * 4 * (size_t)c->ny * (c->nz + 1);
+        const size_t lds = sizeof(typename F::C) * 4 * (size_t)c->ny * (c->nz + 1);
         const double k1y = 2.0 * M_PI / c->Lphys[1] / c->ny, k2y = pow(2.0 * M_PI / c->Lphys[1], 2) / c->ny;
         const double k1z = 2.0 * M_PI / c->Lphys[2] / c->nz, k2z = pow(2.0 * M_PI / c->Lphys[2], 2) / c->nz;
         hipLaunchKernelGGL(fy, dim3((unsigned)((long)n * c->nx)), dim3(nt), lds, c->stream, in, T, out, pots, hx, n, N,
-                           c->nx, c->d_tw[1], c->d_tw[2], k1y, k2y, k1z, k2z, c->p.mu_a, c->p.mu_b, part, sc);
+                           c->nx, F::tw(c, 1), F::tw(c, 2), k1y, k2y, k1z, k2z, c->p.mu_a, c->p.mu_b, part, sc);
         hipLaunchKernelGGL(ah_qpe, dim3((n + 255) / 256), dim3(256), 0, c->stream, part, n, c->nx, dvol_norm(c), d_qpe);
     }
     return (int)hipGetLastError();
+}
+
+int aslda_h_fused(tdw_ctx *c, int n, const double2 *in, double2 *out, double2 *T, const double *pots,
+                  const double *hx, double *part, double *d_qpe, const double *sc)
+{
+    return aslda_h_t<false>(c, n, in, out, T, pots, hx, part, d_qpe, sc);
 }
 
 // ---------------------------------------------------------------------------
@@ -707,19 +742,22 @@ static constexpr int PT = 256;
 
 // grid (nchunk, n): block b of state a runs points [b chunk, (b+1) chunk) of
 // both components; part[a * nchunk + b] = sum |y_p|^2 over them.  sk, ek
-// (pending): y_k = sk_a y and f1 -= ek_a y_k.  The partial corrector goes to
-// the oldest history slot (f3 / f4 / f5), read first by the same thread.
-template <int SCH>
+// (pending): y_k = sk_a y and f1 -= ek_a y_k.  The other output goes to the
+// oldest history slot (f3 / f4 / f5), read first by the same thread:
+//   plain: y <- y_p, slot <- c = y_k - i dt sum a_j F_j (FP64)
+//   split: y <- c (FP64), slot <- the FP32 copy of y_p (the history is FP32)
+template <int SCH, bool SP>
 __global__ void __launch_bounds__(PT)
-k_apred(double2 *y, double2 *f1, const double2 *f2, double2 *f3, double2 *f4, double2 *f5,
+k_apred(double2 *y, ac_t<SP> *f1, const ac_t<SP> *f2, ac_t<SP> *f3, ac_t<SP> *f4, ac_t<SP> *f5,
         const double *__restrict__ sk, const double *__restrict__ ek, int n, long N, long chunk, double dt,
         double *__restrict__ part)
 {
+    typedef ac_t<SP> HC;
     __shared__ double red[PT / 64];
     const int a = blockIdx.y;
     const long i0 = (long)blockIdx.x * chunk, i1 = min(N, i0 + chunk);
     const double s = sk ? sk[a] : 1.0, e = sk ? ek[a] : 0.0;
-    double2 *fold = SCH == 34 ? f3 : SCH == 45 ? f4 : f5;
+    HC *fold = SCH == 34 ? f3 : SCH == 45 ? f4 : f5;
     double nrm = 0.0;
     for (long i = i0 + threadIdx.x; i < i1; i += PT) {
         double q = 0.0;
@@ -727,19 +765,24 @@ k_apred(double2 *y, double2 *f1, const double2 *f2, double2 *f3, double2 *f4, do
         for (int comp = 0; comp < 2; comp++) {
             const long idx = (comp ? (long)n + a : (long)a) * N + i;
             const double2 yk = y[idx] * s;
-            double2 F1 = f1[idx];
+            double2 F1 = to64(f1[idx]);
             if (sk) {
                 F1 = F1 - yk * e;
-                f1[idx] = F1;
+                f1[idx] = cvt<HC>(F1);
             }
-            const double2 F2 = f2[idx], F3 = f3[idx];
-            const double2 F4 = SCH >= 45 ? f4[idx] : make_double2(0.0, 0.0);
-            const double2 F5 = SCH >= 55 ? f5[idx] : make_double2(0.0, 0.0);
+            const double2 F2 = to64(f2[idx]), F3 = to64(f3[idx]);
+            const double2 F4 = SCH >= 45 ? to64(f4[idx]) : make_double2(0.0, 0.0);
+            const double2 F5 = SCH >= 55 ? to64(f5[idx]) : make_double2(0.0, 0.0);
             double2 sp, sc;
             abm_sums<SCH>(F1, F2, F3, F4, F5, sp, sc);
             const double2 yp = yk + imul(sp, -dt);
-            y[idx] = yp;
-            fold[idx] = yk + imul(sc, -dt);
+            if constexpr (SP) {
+                y[idx] = yk + imul(sc, -dt);
+                fold[idx] = to32(yp);
+            } else {
+                y[idx] = yp;
+                fold[idx] = yk + imul(sc, -dt);
+            }
             q += cnorm(yp);
         }
         nrm += q;
@@ -748,12 +791,16 @@ k_apred(double2 *y, double2 *f1, const double2 *f2, double2 *f3, double2 *f4, do
     if (threadIdx.x == 0) part[(long)a * gridDim.x + blockIdx.x] = nrm;
 }
 
-// y = -i dta0 (hy - ep_a yh) + cin with yh = sp_a y (the normalised
-// prediction), norm partials of the new y as k_apred's
+// y_c = -i dta0 (hy - ep_a yh) + c with yh = sp_a y_p (the normalised
+// prediction), norm partials of y_c as k_apred's.
+//   plain: y holds y_p, cin c;  split: y holds c, cin the FP32 y_p, and the
+//   FP32 copy of y_c goes to y32 (hy's buffer: each element of hy is read by
+//   the thread that then writes it)
+template <bool SP>
 __global__ void __launch_bounds__(PT)
-k_acorr(double2 *__restrict__ y, const double2 *__restrict__ hy, const double2 *__restrict__ cin,
+k_acorr(double2 *__restrict__ y, const ac_t<SP> *hy, const ac_t<SP> *__restrict__ cin,
         const double *__restrict__ sp, const double *__restrict__ ep, int n, long N, long chunk, double dta0,
-        double *__restrict__ part)
+        double *__restrict__ part, float2 *y32)
 {
     __shared__ double red[PT / 64];
     const int a = blockIdx.y;
@@ -765,9 +812,11 @@ k_acorr(double2 *__restrict__ y, const double2 *__restrict__ hy, const double2 *
 #pragma unroll
         for (int comp = 0; comp < 2; comp++) {
             const long idx = (comp ? (long)n + a : (long)a) * N + i;
-            const double2 yh = y[idx] * s;
-            const double2 yc = imul(hy[idx] - yh * e, -dta0) + cin[idx];
+            const double2 yh = (SP ? to64(cin[idx]) : y[idx]) * s;
+            const double2 cc = SP ? y[idx] : to64(cin[idx]);
+            const double2 yc = imul(to64(hy[idx]) - yh * e, -dta0) + cc;
             y[idx] = yc;
+            if constexpr (SP) y32[idx] = to32(yc);
             q += cnorm(yc);
         }
         nrm += q;
@@ -792,8 +841,8 @@ __global__ void k_ascale(const double *__restrict__ part, int n, int nchunk, dou
 
 bool aslda_step_supported(const tdw_ctx *c)
 {
-    return varying_mass(c) && c->p.precision == TDW_FP64 && aslda_h_fused_supported(c) &&
-           dens_grad_fused_supported(c);
+    return varying_mass(c) && (c->p.precision == TDW_FP64 || c->p.precision == TDW_SPLIT) &&
+           aslda_h_fused_supported(c) && dens_grad_fused_supported(c);
 }
 
 static int ascale(tdw_ctx *c, int n, int nchunk, double *s, double *s2)
@@ -804,12 +853,14 @@ static int ascale(tdw_ctx *c, int n, int nchunk, double *s, double *s2)
     return (int)hipGetLastError();
 }
 
-// densities, tau, potentials and H of the state s_a y (H into out, <H> into e)
-static int aslda_half(tdw_ctx *c, tdw_state *st, int it, const double2 *y, const double *s, const double *s2,
-                      double2 *out, double *e)
+// densities, tau, potentials and H of the state s_a y (H into out, <H> into
+// e; T the x pass's scratch)
+template <bool SP>
+static int aslda_half(tdw_ctx *c, tdw_state *st, int it, const ac_t<SP> *y, const double *s, const double *s2,
+                      ac_t<SP> *out, double *e, ac_t<SP> *T)
 {
     const int n = st->n;
-    TDW_TRY(dens_grad_fused(c, n, y, c->d_fbeta, s2, st->dens));
+    TDW_TRY(dens_grad_t<SP>(c, n, y, c->d_fbeta, s2, st->dens));
     TDW_TRY(density_tau(c, st->dens));
     TDW_TRY(step_potentials(c, it, st->dens, st->pots));
     TDW_TRY(hkf_fields(c, st->pots, true));
@@ -817,11 +868,27 @@ static int aslda_half(tdw_ctx *c, tdw_state *st, int it, const double2 *y, const
     if (!hp) return TDW_EINVAL;
     TDW_TRY(aslda_fields(c, hp));
     TDW_TRY(ensure_part(c, (size_t)n * c->nx + n));
-    return aslda_h_fused(c, n, y, out, (double2 *)st->ddx, hp, c->d_hx, c->d_part, e, s);
+    return aslda_h_t<SP>(c, n, y, out, T, hp, c->d_hx, c->d_part, e, s);
 }
 
-int aslda_fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+// FP64 -> FP32 copy (the measurement step's FP64 H psi into the split history)
+__global__ void __launch_bounds__(PT) k_ato32(const double2 *__restrict__ src, float2 *__restrict__ dst, long tot)
 {
+    const long i = (long)blockIdx.x * PT + threadIdx.x;
+    if (i < tot) dst[i] = to32(src[i]);
+}
+
+template <int SCH, bool SP>
+static void launch_apred(tdw_ctx *c, dim3 g, double2 *y, ac_t<SP> **f, const double *sk, const double *ek, int n,
+                         long N, long chunk)
+{
+    hipLaunchKernelGGL((k_apred<SCH, SP>), g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
+                       chunk, c->p.dt, c->d_part);
+}
+
+template <bool SP> static int aslda_step_t(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    typedef ac_t<SP> HC;
     if (!s->ddx || c->pending == 1) return TDW_EINVAL;
     const int n = s->n;
     const long N = (long)c->N;
@@ -831,8 +898,10 @@ int aslda_fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     const int nchunk = (int)((N + chunk - 1) / chunk);
     TDW_TRY(ensure_part(c, (size_t)n * nchunk));
     double2 *y = (double2 *)s->wf;
-    double2 **f = (double2 **)s->f;
-    double2 *H = (double2 *)s->lap;
+    HC **f = (HC **)s->f;
+    // H of the prediction in lap (FP32 in the split form, then the FP32
+    // copy of the corrected state); ddx is the x pass's scratch
+    HC *H = (HC *)s->lap, *T = (HC *)s->ddx;
     double *qpe = s->qpe ? s->qpe : c->d_qpe;
     const bool pend = c->pending == 2;
     const dim3 g(nchunk, n);
@@ -840,39 +909,56 @@ int aslda_fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
     {
         TDW_TIMED(c, "a_pred");
         const double *sk = pend ? c->d_scale : nullptr, *ek = pend ? c->d_qpe : nullptr;
-        const double dt = c->p.dt;
-        if (h == 3)
-            hipLaunchKernelGGL(k_apred<34>, g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
-                               chunk, dt, c->d_part);
-        else if (h == 4)
-            hipLaunchKernelGGL(k_apred<45>, g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
-                               chunk, dt, c->d_part);
-        else
-            hipLaunchKernelGGL(k_apred<55>, g, dim3(PT), 0, c->stream, y, f[0], f[1], f[2], f[3], f[4], sk, ek, n, N,
-                               chunk, dt, c->d_part);
+        if (h == 3) launch_apred<34, SP>(c, g, y, f, sk, ek, n, N, chunk);
+        else if (h == 4) launch_apred<45, SP>(c, g, y, f, sk, ek, n, N, chunk);
+        else launch_apred<55, SP>(c, g, y, f, sk, ek, n, N, chunk);
         TDW_HIP(hipGetLastError());
     }
     c->pending = 0;
     TDW_TRY(ascale(c, n, nchunk, c->d_sp, c->d_s2));
-    TDW_TRY(aslda_half(c, s, it + 1, y, c->d_sp, c->d_s2, H, c->d_ep));
+    // the prediction: y itself (plain) or its FP32 copy in the oldest slot (split)
+    const HC *yp = SP ? (const HC *)f[h - 1] : (const HC *)y;
+    TDW_TRY(aslda_half<SP>(c, s, it + 1, yp, c->d_sp, c->d_s2, H, c->d_ep, T));
     // ---- corrector ----
     {
         TDW_TIMED(c, "a_corr");
         const double a0 = (c->p.scheme == TDW_AB3AM4) ? 9. / 24. : 251. / 720.;
         TDW_TRY(ensure_part(c, (size_t)n * nchunk));
-        hipLaunchKernelGGL(k_acorr, g, dim3(PT), 0, c->stream, y, H, f[h - 1], c->d_sp, c->d_ep, n, N, chunk,
-                           c->p.dt * a0, c->d_part);
+        hipLaunchKernelGGL(k_acorr<SP>, g, dim3(PT), 0, c->stream, y, H, f[h - 1], c->d_sp, c->d_ep, n, N, chunk,
+                           c->p.dt * a0, c->d_part, SP ? (float2 *)H : nullptr);
         TDW_HIP(hipGetLastError());
     }
     TDW_TRY(ascale(c, n, nchunk, c->d_scale, c->d_s2));
+    // the corrected state: y (plain) or its FP32 copy in H's buffer (split)
+    const HC *yc = SP ? (const HC *)H : (const HC *)y;
     tdw_complex *tmp = s->f[h - 1];
     for (int j = h - 1; j > 0; j--) s->f[j] = s->f[j - 1];
     s->f[0] = tmp;
-    TDW_TRY(aslda_half(c, s, it + 1, y, c->d_scale, c->d_s2, (double2 *)s->f[0], c->d_qpe));
+    if (SP && grad_last) {
+        // measurement step of the split form: the densities (tau and j with
+        // them) and H psi of the corrected state in FP64, from y itself (ddy
+        // holds H psi, ddz the x pass's scratch), H psi then rounded into the
+        // FP32 history
+        if (!s->ddy || !s->ddz) return TDW_EINVAL;
+        double2 *H64 = (double2 *)s->ddy, *T64 = (double2 *)s->ddz;
+        TDW_TRY(aslda_half<false>(c, s, it + 1, y, c->d_scale, c->d_s2, H64, c->d_qpe, T64));
+        const long tot = 2L * n * N;
+        TDW_TIMED(c, "a_to32");
+        hipLaunchKernelGGL(k_ato32, dim3((tot + PT - 1) / PT), dim3(PT), 0, c->stream, H64, (float2 *)s->f[0], tot);
+        TDW_HIP(hipGetLastError());
+    } else {
+        TDW_TRY(aslda_half<SP>(c, s, it + 1, yc, c->d_scale, c->d_s2, (HC *)s->f[0], c->d_qpe, T));
+    }
     if (qpe != c->d_qpe) TDW_HIP(hipMemcpyAsync(qpe, c->d_qpe, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
     c->pending = 2;
     // a measurement step leaves the arrays as the reference's step does
     return grad_last ? fused_finalize(c, s) : 0;
+}
+
+int aslda_fused_step(tdw_ctx *c, tdw_state *s, int it, int grad_last)
+{
+    if (c->p.precision == TDW_SPLIT) return aslda_step_t<true>(c, s, it, grad_last);
+    return aslda_step_t<false>(c, s, it, grad_last);
 }
 
 }  // namespace tdw

@@ -887,7 +887,10 @@ int fused_finalize(tdw_ctx *c, tdw_state *s)
     if (!p) return 0;
     c->pending = 0;
     // the ASLDA step's pending state lacks only the scale and the subtraction
-    if (p == 2) return fixup<false>(c, (double2 *)s->wf, s->f[0], s->n, 1);
+    if (p == 2) {
+        if (split_mode(c)) return fixup<true>(c, (double2 *)s->wf, s->f[0], s->n, 1);
+        return fixup<false>(c, (double2 *)s->wf, s->f[0], s->n, 1);
+    }
     if (split_mode(c)) return fixup_x<true>(c, s);
     return fixup_x<false>(c, s);
 }
