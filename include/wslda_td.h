@@ -233,8 +233,10 @@ int tdw_step(tdw_ctx *c, tdw_state *s, int it, int grad_last);
  * kernels run the folded step: the modular step's density and H passes with
  * the normalisations and the <H> subtraction carried as per-state scalars,
  * so between steps wf is unnormalised and f[0] lacks only the subtraction
- * (same completion rules).  Other lattices run the modular step
- * (tdw_fused_supported() says whether the constant-mass fused passes run).
+ * (same completion rules); it has a split form too (below; its
+ * measurement steps also use ddy and ddz as FP64 scratch).  Other lattices
+ * run the modular step (tdw_fused_supported() says whether the
+ * constant-mass fused passes run).
  *
  * Split precision (params.precision == TDW_SPLIT): the x / yz FFTs and the
  * kinetic term run in FP32 complex, and f[0..4] hold FP32 complex values
