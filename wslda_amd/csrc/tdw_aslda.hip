@@ -890,6 +890,9 @@ template <bool SP> static int aslda_step_t(tdw_ctx *c, tdw_state *s, int it, int
 {
     typedef ac_t<SP> HC;
     if (!s->ddx || c->pending == 1) return TDW_EINVAL;
+    // the split form's measurement step uses ddy and ddz as FP64 scratch:
+    // refused before anything is written
+    if (SP && grad_last && (!s->ddy || !s->ddz)) return TDW_EINVAL;
     const int n = s->n;
     const long N = (long)c->N;
     const int h = c->p.scheme == TDW_AB3AM4 ? 3 : c->p.scheme == TDW_AB4AM5 ? 4 : 5;
@@ -939,7 +942,6 @@ template <bool SP> static int aslda_step_t(tdw_ctx *c, tdw_state *s, int it, int
         // them) and H psi of the corrected state in FP64, from y itself (ddy
         // holds H psi, ddz the x pass's scratch), H psi then rounded into the
         // FP32 history
-        if (!s->ddy || !s->ddz) return TDW_EINVAL;
         double2 *H64 = (double2 *)s->ddy, *T64 = (double2 *)s->ddz;
         TDW_TRY(aslda_half<false>(c, s, it + 1, y, c->d_scale, c->d_s2, H64, c->d_qpe, T64));
         const long tot = 2L * n * N;
