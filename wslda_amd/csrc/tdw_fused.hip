@@ -951,6 +951,9 @@ static int fused_step_t(tdw_ctx *c, tdw_state *s, int it, int grad_last, tdw_tap
 {
     typedef hc_t<SP> HC;
     if ((tape || cbuf || f_next) && (SP || grad_last)) return TDW_EINVAL;
+    // a gradient step's FP64 H psi and scratch live in ddx (and ddy):
+    // refused before anything is written
+    if (grad_last && (!s->ddx || !s->ddy || !s->ddz || !s->lap)) return TDW_EINVAL;
     const int n = s->n, h = nhist(c);
     const long N = (long)c->N;
     const int S = c->S;
