@@ -14,7 +14,8 @@
  *     (hpc-engine/pca_kernels.h:44-95, pca_derivative.h:12-22,
  *     pca_dens.h:5-19, wslda_cuda_utils.h:47-157).  They act on the context
  *     bound with tdw_bind(); a project build binds it from its predefines.h
- *     (see INTEGRATION.md).  The 1D and 2D codes of the reference declare
+ *     (python -m wslda_amd.build --project DIR).  The 1D and 2D codes of the
+ *     reference declare
  *     apply_hamiltonian / calculate_densities with extra arguments under the
  *     same names (ccpca_kernels.h:74-77, cpca_kernels.h:74-77,
  *     ccpca_dens.h:5-11); those are exported here as *_1d / *_2d and mapped
