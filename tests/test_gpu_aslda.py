@@ -248,9 +248,9 @@ def test_aslda_folded_step_matches_modular(torch, scheme):
     ctx3.close()
 
 
-@pytest.mark.parametrize("scheme", [po.AB4AM5, po.AB5AM5])
-def test_aslda_split_step_vs_fp64(torch, scheme):
-    """The folded ASLDA step in split precision (FP32 transforms, H and
+@pytest.mark.parametrize("functional,scheme", [(po.ASLDA, po.AB4AM5), (po.ASLDA, po.AB5AM5), (po.SLDAE, po.AB4AM5)])
+def test_aslda_split_step_vs_fp64(torch, functional, scheme):
+    """The folded ASLDA (and SLDAE) step in split precision (FP32 transforms, H and
     history; FP64 state, ABM updates, density sums, norms and <H>; the
     measurement step's densities and H in FP64) against the FP64 folded
     step over 20 steps with V_ext moving the states and a measurement step
@@ -259,8 +259,8 @@ def test_aslda_split_step_vs_fp64(torch, scheme):
     (and held to 2e-6): it is only the small flow 20 steps of V_ext drive,
     and the 1e-6 gate on j runs on the boosted flow."""
     from wslda_amd import td
-    say("case: ASLDA split vs FP64 folded step, scheme %d" % scheme)
-    s = tdgauss.setup(3, scheme=scheme, functional=po.ASLDA)
+    say("case: split vs FP64 folded step, functional %d, scheme %d" % (functional, scheme))
+    s = tdgauss.setup(3, scheme=scheme, functional=functional)
     s["cfg"].vext.t_start, s["cfg"].vext.t_switch = -1.0, 0.5
     runs = []
     for prec in (0, 1):
